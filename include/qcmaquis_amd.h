@@ -1,0 +1,212 @@
+/*
+ * qcmaquis_amd.h -- C ABI of the MI355X-native symmetry-blocked DMRG contraction path.
+ *
+ * The path: Davidson H*psi (Boundary x MPOTensor x MPS site problem) and the left / right
+ * boundary-growth steps, on one GPU.  Everything that crosses this ABI is plain C: pointers,
+ * sizes and small POD descriptors.  FP64 is the value type at the ABI; device-internal
+ * precisions are selected with a flag.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree dmrg/framework/dmrg/):
+ *   contraction::Engine<...>::{overlap_mpo_left_step, overlap_mpo_right_step,
+ *                              contraction_schedule}     mp_tensors/contractions/engine.hpp:47-140
+ *   create_contraction_schedule                         contractions/create_schedule/site_hamil_schedule.hpp:40-93
+ *   super_hamil_mv                                      solver/super_hamil_mv.hpp:100-155
+ *   solve<double> (Jacobi-Davidson)                     solver/solver.h:64-68, solver/solver.cpp:13-28
+ *   common::overlap_mpo_left_step / _right_step         contractions/boundary_ops/move_boundary.hpp:147-404
+ *   Boundary / BoundaryIndex                            mp_tensors/boundary.h:76-406
+ *
+ * Error convention: every entry point returns MQ_OK (0) or a negative MQ_ERR_* code and never
+ * exits the process or throws across the ABI (the reference calls exit() on CUDA/cuBLAS errors,
+ * tasks.cpp:231-235).  mq_last_error() gives the message of the last failure on a context.
+ *
+ * Threading: a context is bound to one device and one host thread at a time.
+ */
+#ifndef QCMAQUIS_AMD_H
+#define QCMAQUIS_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MQ_OK              0
+#define MQ_ERR_INVALID    -1   /* bad argument / inconsistent descriptor (reference: std::runtime_error) */
+#define MQ_ERR_HIP        -2   /* HIP runtime failure (reference: exit(EXIT_FAILURE)) */
+#define MQ_ERR_NOMEM      -3   /* device allocation failed (reference: std::out_of_range, accelerator.cpp:129) */
+#define MQ_ERR_UNSUPPORTED -4  /* feature not available in this build */
+
+/* symmetry groups (reference: block_matrix/symmetry/2u1.h, su2u1.h) */
+#define MQ_SYM_TWOU1 1         /* charge = (N_up, N_down) */
+#define MQ_SYM_SU2U1 2         /* charge = (N, 2S) */
+
+/* device arithmetic of the sector GEMMs */
+#define MQ_PREC_FP64   0
+#define MQ_PREC_FP32   1
+#define MQ_PREC_BF16X3 2
+
+/* MPS pairing (reference: MPSTensor::make_left_paired / make_right_paired, mpstensor.hpp:211-237) */
+#define MQ_RIGHT_PAIRED 0      /* blocks per left charge:  ls x (sum_sigma d_sigma * rs(lc+sigma)) */
+#define MQ_LEFT_PAIRED  1      /* blocks per right charge: (sum_sigma d_sigma * ls(rc-sigma)) x rs */
+
+#define MQ_NO_PARTNER 0xFFFFFFFFu
+
+typedef struct mq_ctx_s*        mq_ctx_t;
+typedef struct mq_mpo_s*        mq_mpo_t;
+typedef struct mq_boundary_s*   mq_boundary_t;
+typedef struct mq_mps_s*        mq_mps_t;
+typedef struct mq_hpsi_plan_s*  mq_hpsi_plan_t;
+
+/* Sector index.  Charges must be sorted strictly descending in lexicographic order, which is the
+ * invariant of the reference's Index (indexing_stable.hpp:129-176). */
+typedef struct {
+    int32_t         n;          /* number of sectors */
+    const int32_t*  charges;    /* n x 2 */
+    const uint32_t* sizes;      /* n */
+} mq_index_desc;
+
+/* MPO tensor (reference: MPOTensor, mpotensor.h:45-135; term_descriptor / Hermitian / BondProperty,
+ * mpotensor_detail.h:48-239; SparseOperator, sparse_operator.h:164-233). */
+typedef struct {
+    /* operator table: op o has blocks [op_block_ptr[o], op_block_ptr[o+1]) */
+    int32_t         n_ops;
+    const int32_t*  op_spin;          /* n_ops: 2K of the operator (SiteOperator::spin().get()), 0 if abelian */
+    const int64_t*  op_block_ptr;     /* n_ops + 1 */
+    const int32_t*  block_charges;    /* n_blocks x 4: left charge (2 ints), right charge (2 ints) */
+    const int64_t*  block_entry_ptr;  /* n_blocks + 1 (global block numbering) */
+    const uint32_t* entry_rc;         /* n_entries x 2: row (ss1), col (ss2) */
+    const int32_t*  entry_spins;      /* n_entries x 2: row_spin, col_spin; NULL for abelian */
+    const double*   entry_coeff;      /* n_entries */
+    /* tensor: terms of (b1,b2) are stored consecutively, sorted by (b1, b2) */
+    uint32_t        row_dim, col_dim;
+    int64_t         n_terms;
+    const uint32_t* term_b;           /* n_terms x 2: (b1, b2) */
+    const uint32_t* term_op;          /* n_terms */
+    const double*   term_scale;       /* n_terms */
+    /* bond properties; any may be NULL (spins -> 0, no Hermitian partners) */
+    const int32_t*  left_spin;        /* row_dim */
+    const int32_t*  right_spin;       /* col_dim */
+    const uint32_t* left_herm;        /* row_dim, MQ_NO_PARTNER = none, b = self-adjoint */
+    const int32_t*  left_phase;       /* row_dim */
+    const uint32_t* right_herm;       /* col_dim */
+    const int32_t*  right_phase;      /* col_dim */
+} mq_mpo_desc;
+
+/* Boundary index (reference: BoundaryIndex, boundary.h:76-234).  Cohort ci holds the blocks of the
+ * charge pair (bra[cohort_lr[2ci]], ket[cohort_lr[2ci+1]]); block b of cohort ci starts at element
+ * offsets[ci*aux_dim+b] of the cohort data (-1 = absent); blocks are ls x rs column-major. */
+typedef struct {
+    mq_index_desc   bra, ket;
+    uint32_t        aux_dim;
+    int32_t         n_cohorts;
+    const uint32_t* cohort_lr;        /* n_cohorts x 2 */
+    const int64_t*  offsets;          /* n_cohorts x aux_dim */
+    const double*   conj_scales;      /* n_cohorts x aux_dim, NULL -> 1 */
+    const uint8_t*  transposes;       /* n_cohorts x aux_dim, NULL -> 0 */
+    const uint8_t*  tr;               /* n_cohorts, NULL -> 0 */
+} mq_bindex_desc;
+
+typedef struct {
+    int32_t  n_cohorts;
+    uint32_t aux_dim;
+    int32_t  n_bra, n_ket;
+    uint64_t total_elements;          /* sum over cohorts with tr == 0 of n_blocks * ls * rs */
+} mq_boundary_info_t;
+
+/* Statistics of an H*psi plan.  Flop model = ScheduleNew::total_flops (solver/tasks.cpp:389-398,
+ * 778-791); bytes = compulsory FP64 traffic of one H*psi (boundary cohorts referenced + psi + psi'). */
+typedef struct {
+    uint64_t psi_elements;
+    uint64_t n_t_tasks, n_cohorts, n_lines, n_terms;
+    uint64_t flops_total;             /* reference model */
+    uint64_t flops_gemm;              /* T-GEMM + L*S GEMM part of the reference model */
+    uint64_t flops_gemm_exec;         /* flops of the GEMMs as executed by this plan */
+    uint64_t bytes_compulsory;        /* FP64 bytes */
+    uint64_t t_elements, s_elements, l_panel_elements, r_tr_elements;
+    double   build_seconds;
+} mq_hpsi_stats_t;
+
+typedef struct {
+    int32_t  iterations;
+    double   residual_norm;
+    double   seconds;
+    double   hpsi_seconds;
+} mq_davidson_stats_t;
+
+/* ---- context ---------------------------------------------------------------------------- */
+int         mq_ctx_create(int device, int symmetry, mq_ctx_t* out);
+int         mq_ctx_destroy(mq_ctx_t ctx);
+const char* mq_last_error(mq_ctx_t ctx);
+int         mq_ctx_sync(mq_ctx_t ctx);
+void*       mq_ctx_stream(mq_ctx_t ctx);      /* hipStream_t of the context */
+int         mq_version(void);
+
+/* ---- MPO ---------------------------------------------------------------------------------- */
+int mq_mpo_create(mq_ctx_t ctx, const mq_mpo_desc* desc, mq_mpo_t* out);
+int mq_mpo_destroy(mq_mpo_t mpo);
+
+/* ---- device-resident boundary store (reference: Boundary + storage::gpu, storage.h:547-671) -- */
+/* initial boundary: one 1x1 cohort per sector, aux_dim blocks, all ones (boundary.h:257-283) */
+int mq_boundary_create_initial(mq_ctx_t ctx, const mq_index_desc* idx, uint32_t aux_dim, mq_boundary_t* out);
+/* data: data of the cohorts with tr == 0 concatenated in cohort order (n_blocks*ls*rs elements each),
+ * host memory.  Cohorts with tr != 0 alias their partner's storage and carry no data. */
+int mq_boundary_upload(mq_ctx_t ctx, const mq_bindex_desc* desc, const double* data, mq_boundary_t* out);
+int mq_boundary_info(mq_boundary_t b, mq_boundary_info_t* info);
+/* export the index: arrays sized from mq_boundary_info (any pointer may be NULL) */
+int mq_boundary_export_index(mq_boundary_t b, int32_t* bra_charges, uint32_t* bra_sizes,
+                             int32_t* ket_charges, uint32_t* ket_sizes, uint32_t* cohort_lr,
+                             int64_t* offsets, double* conj_scales, uint8_t* transposes, uint8_t* tr);
+int mq_boundary_download(mq_boundary_t b, double* data);
+int mq_boundary_destroy(mq_boundary_t b);
+
+/* ---- MPS site tensors in HBM ------------------------------------------------------------ */
+/* data: blocks of the given pairing concatenated in block order, column-major, host memory */
+int mq_mps_upload(mq_ctx_t ctx, const mq_index_desc* left, const mq_index_desc* phys,
+                  const mq_index_desc* right, int pairing, const double* data, mq_mps_t* out);
+int mq_mps_elements(mq_mps_t m, uint64_t* n);
+int mq_mps_download(mq_mps_t m, int pairing, double* data);
+int mq_mps_destroy(mq_mps_t m);
+
+/* ---- boundary growth (reference: move_boundary.hpp:147-404) -------------------------------- */
+/* new L' = bra^dagger . (L x W x ket) ; symmetric: skip Hermitian partners and complete the index
+ * by complement_transpose (move_boundary.hpp:191-208) */
+int mq_lstep(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_mps_t bra, mq_mps_t ket,
+             int symmetric, int precision, mq_boundary_t* out);
+int mq_rstep(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t right, mq_mps_t bra, mq_mps_t ket,
+             int symmetric, int precision, mq_boundary_t* out);
+
+/* ---- H*psi (reference: create_contraction_schedule + super_hamil_mv) ----------------------- */
+int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_boundary_t right,
+                        const mq_index_desc* left_i, const mq_index_desc* phys_i,
+                        const mq_index_desc* right_i, mq_hpsi_plan_t* out);
+int mq_hpsi_plan_stats(mq_hpsi_plan_t plan, mq_hpsi_stats_t* stats);
+/* psi: right-paired blocks of left_i concatenated, each ls x stripe column-major, no padding */
+int mq_hpsi_plan_psi_elements(mq_hpsi_plan_t plan, uint64_t* n);
+/* device pointers (on the context's device), stream-ordered on the context stream */
+int mq_hpsi(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out_dev, int precision);
+/* host pointers, synchronous */
+int mq_hpsi_host(mq_hpsi_plan_t plan, const double* psi_in, double* psi_out, int precision);
+int mq_hpsi_plan_destroy(mq_hpsi_plan_t plan);
+
+/* ---- Jacobi-Davidson (reference: solve_ietl_jcd, ietl_jacobi_davidson.h:35-76,
+ *      jacobi_davidson::calculate_eigenvalue, ietl/jacobi.h:386-474) ------------------------ */
+int mq_davidson(mq_hpsi_plan_t plan, const double* psi0, int n_ortho, const double* const* ortho,
+                double tol, int max_iter, int precision, double* eval, double* psi_out,
+                mq_davidson_stats_t* stats);
+
+/* ---- host-only utilities (no device needed) ------------------------------------------------ */
+/* Wigner 6j / 9j symbols, arguments as 2j (reference: gsl_sf_coupling_6j / _9j, extern "C" in
+ * block_matrix/symmetry/gsl_coupling.h:30-34; selection rules of wignerj/coupling.c) */
+double mq_wigner_6j(int two_ja, int two_jb, int two_jc, int two_jd, int two_je, int two_jf);
+double mq_wigner_9j(int two_ja, int two_jb, int two_jc, int two_jd, int two_je, int two_jf,
+                    int two_jg, int two_jh, int two_ji);
+/* position of charge (q0, q1) in an index (reference Index::position, indexing_stable.hpp:129-140):
+ * n if absent, MQ_ERR_INVALID if the index is not sorted strictly descending */
+int mq_index_position(const mq_index_desc* idx, int32_t q0, int32_t q1);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QCMAQUIS_AMD_H */

@@ -1,0 +1,155 @@
+"""ctypes mirror of include/qcmaquis_amd.h and the loader of the in-tree library.
+
+The product path runs only through the HIP library: if ``libqcmaquis_amd.so`` is missing the
+import of :func:`lib` raises; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libqcmaquis_amd.so")
+
+MQ_OK = 0
+MQ_ERR_INVALID = -1
+MQ_ERR_HIP = -2
+MQ_ERR_NOMEM = -3
+MQ_ERR_UNSUPPORTED = -4
+MQ_SYM_TWOU1 = 1
+MQ_SYM_SU2U1 = 2
+MQ_PREC_FP64 = 0
+MQ_PREC_FP32 = 1
+MQ_PREC_BF16X3 = 2
+MQ_RIGHT_PAIRED = 0
+MQ_LEFT_PAIRED = 1
+MQ_NO_PARTNER = 0xFFFFFFFF
+
+P = C.POINTER
+
+
+class IndexDesc(C.Structure):
+    _fields_ = [("n", C.c_int32), ("charges", P(C.c_int32)), ("sizes", P(C.c_uint32))]
+
+
+class MPODesc(C.Structure):
+    _fields_ = [
+        ("n_ops", C.c_int32),
+        ("op_spin", P(C.c_int32)),
+        ("op_block_ptr", P(C.c_int64)),
+        ("block_charges", P(C.c_int32)),
+        ("block_entry_ptr", P(C.c_int64)),
+        ("entry_rc", P(C.c_uint32)),
+        ("entry_spins", P(C.c_int32)),
+        ("entry_coeff", P(C.c_double)),
+        ("row_dim", C.c_uint32),
+        ("col_dim", C.c_uint32),
+        ("n_terms", C.c_int64),
+        ("term_b", P(C.c_uint32)),
+        ("term_op", P(C.c_uint32)),
+        ("term_scale", P(C.c_double)),
+        ("left_spin", P(C.c_int32)),
+        ("right_spin", P(C.c_int32)),
+        ("left_herm", P(C.c_uint32)),
+        ("left_phase", P(C.c_int32)),
+        ("right_herm", P(C.c_uint32)),
+        ("right_phase", P(C.c_int32)),
+    ]
+
+
+class BIndexDesc(C.Structure):
+    _fields_ = [
+        ("bra", IndexDesc),
+        ("ket", IndexDesc),
+        ("aux_dim", C.c_uint32),
+        ("n_cohorts", C.c_int32),
+        ("cohort_lr", P(C.c_uint32)),
+        ("offsets", P(C.c_int64)),
+        ("conj_scales", P(C.c_double)),
+        ("transposes", P(C.c_uint8)),
+        ("tr", P(C.c_uint8)),
+    ]
+
+
+class BoundaryInfo(C.Structure):
+    _fields_ = [("n_cohorts", C.c_int32), ("aux_dim", C.c_uint32), ("n_bra", C.c_int32), ("n_ket", C.c_int32),
+                ("total_elements", C.c_uint64)]
+
+
+class HpsiStats(C.Structure):
+    _fields_ = [
+        ("psi_elements", C.c_uint64),
+        ("n_t_tasks", C.c_uint64),
+        ("n_cohorts", C.c_uint64),
+        ("n_lines", C.c_uint64),
+        ("n_terms", C.c_uint64),
+        ("flops_total", C.c_uint64),
+        ("flops_gemm", C.c_uint64),
+        ("flops_gemm_exec", C.c_uint64),
+        ("bytes_compulsory", C.c_uint64),
+        ("t_elements", C.c_uint64),
+        ("s_elements", C.c_uint64),
+        ("l_panel_elements", C.c_uint64),
+        ("r_tr_elements", C.c_uint64),
+        ("build_seconds", C.c_double),
+    ]
+
+
+class DavidsonStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("residual_norm", C.c_double), ("seconds", C.c_double),
+                ("hpsi_seconds", C.c_double)]
+
+
+vp = C.c_void_p
+i32 = C.c_int
+dp = P(C.c_double)
+
+# (name, restype, argtypes) of every exported entry point
+SIGNATURES = [
+    ("mq_version", i32, []),
+    ("mq_ctx_create", i32, [i32, i32, P(vp)]),
+    ("mq_ctx_destroy", i32, [vp]),
+    ("mq_last_error", C.c_char_p, [vp]),
+    ("mq_ctx_sync", i32, [vp]),
+    ("mq_ctx_stream", vp, [vp]),
+    ("mq_mpo_create", i32, [vp, P(MPODesc), P(vp)]),
+    ("mq_mpo_destroy", i32, [vp]),
+    ("mq_boundary_create_initial", i32, [vp, P(IndexDesc), C.c_uint32, P(vp)]),
+    ("mq_boundary_upload", i32, [vp, P(BIndexDesc), dp, P(vp)]),
+    ("mq_boundary_info", i32, [vp, P(BoundaryInfo)]),
+    ("mq_boundary_export_index", i32, [vp, P(C.c_int32), P(C.c_uint32), P(C.c_int32), P(C.c_uint32),
+                                       P(C.c_uint32), P(C.c_int64), dp, P(C.c_uint8), P(C.c_uint8)]),
+    ("mq_boundary_download", i32, [vp, dp]),
+    ("mq_boundary_destroy", i32, [vp]),
+    ("mq_mps_upload", i32, [vp, P(IndexDesc), P(IndexDesc), P(IndexDesc), i32, dp, P(vp)]),
+    ("mq_mps_elements", i32, [vp, P(C.c_uint64)]),
+    ("mq_mps_download", i32, [vp, i32, dp]),
+    ("mq_mps_destroy", i32, [vp]),
+    ("mq_lstep", i32, [vp, vp, vp, vp, vp, i32, i32, P(vp)]),
+    ("mq_rstep", i32, [vp, vp, vp, vp, vp, i32, i32, P(vp)]),
+    ("mq_hpsi_plan_create", i32, [vp, vp, vp, vp, P(IndexDesc), P(IndexDesc), P(IndexDesc), P(vp)]),
+    ("mq_hpsi_plan_stats", i32, [vp, P(HpsiStats)]),
+    ("mq_hpsi_plan_psi_elements", i32, [vp, P(C.c_uint64)]),
+    ("mq_hpsi", i32, [vp, vp, vp, i32]),
+    ("mq_hpsi_host", i32, [vp, dp, dp, i32]),
+    ("mq_hpsi_plan_destroy", i32, [vp]),
+    ("mq_davidson", i32, [vp, dp, i32, P(dp), C.c_double, i32, i32, dp, dp, P(DavidsonStats)]),
+    ("mq_wigner_6j", C.c_double, [i32] * 6),
+    ("mq_wigner_9j", C.c_double, [i32] * 9),
+    ("mq_index_position", i32, [P(IndexDesc), C.c_int32, C.c_int32]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libqcmaquis_amd.so is not built; run `python -m qcmaquis_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib

@@ -1,0 +1,759 @@
+// C ABI of the contraction path: context, HBM block store, plan upload and execution, growth
+// steps and the Jacobi-Davidson eigensolver.  No C++ type crosses the ABI; errors are returned as
+// codes with the message kept on the context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "core.hpp"
+#include "coupling.hpp"
+#include "descriptors.hpp"
+#include "device.hpp"
+#include "plans.hpp"
+#include "qcmaquis_amd.h"
+
+using namespace mq;
+
+struct mq_ctx_s {
+    int device = 0;
+    Sym sym = Sym::TwoU1;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double* dwork = nullptr;  // DOT_BLOCKS partials + 64 scalars
+};
+
+struct mq_mpo_s {
+    mq_ctx_t ctx;
+    MPO mpo;
+};
+
+struct mq_boundary_s {
+    mq_ctx_t ctx;
+    BoundaryIndex idx;
+    double* data = nullptr;
+    uint64_t n = 0;
+};
+
+struct mq_mps_s {
+    mq_ctx_t ctx;
+    MPSLayout lay;
+    double* rp = nullptr;
+    double* lp = nullptr;
+    bool rp_ok = false, lp_ok = false;
+    ReshapeSeg* d_segs = nullptr;
+    int32_t n_segs = 0;
+};
+
+namespace {
+
+void hipcheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(MQ_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+T* dev_alloc(uint64_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(MQ_ERR_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+    }
+    return static_cast<T*>(p);
+}
+
+template <class T>
+T* dev_upload(std::vector<T> const& v, hipStream_t s) {
+    T* p = dev_alloc<T>(v.size());
+    if (!v.empty()) hipcheck(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    return p;
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int d) {
+        (void)hipGetDevice(&prev);
+        if (prev != d) hipcheck(hipSetDevice(d), "hipSetDevice");
+    }
+    ~DeviceGuard() {
+        int cur;
+        if (hipGetDevice(&cur) == hipSuccess && prev >= 0 && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+template <class F>
+int guarded(mq_ctx_t ctx, F&& f) {
+    if (!ctx) return MQ_ERR_INVALID;
+    try {
+        DeviceGuard g(ctx->device);
+        f();
+        return MQ_OK;
+    } catch (Error const& e) {
+        ctx->err = e.what();
+        return e.code;
+    } catch (std::bad_alloc const&) {
+        ctx->err = "host allocation failed";
+        return MQ_ERR_NOMEM;
+    } catch (std::exception const& e) {
+        ctx->err = e.what();
+        return MQ_ERR_INVALID;
+    }
+}
+
+std::vector<GemmTile> make_tiles(std::vector<GemmDesc> const& probs) {
+    std::vector<GemmTile> tiles;
+    std::vector<int32_t> order(probs.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = int32_t(i);
+    // longest reductions first for load balance
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return probs[a].K > probs[b].K; });
+    for (int32_t p : order) {
+        auto const& g = probs[p];
+        int32_t tm = (g.M + GEMM_BM - 1) / GEMM_BM, tn = (g.N + GEMM_BN - 1) / GEMM_BN;
+        for (int32_t b = 0; b < g.batch; ++b)
+            for (int32_t j = 0; j < tn; ++j)
+                for (int32_t i = 0; i < tm; ++i) tiles.push_back(GemmTile{p, b, i, j});
+    }
+    return tiles;
+}
+
+// device copy of a GEMM stage
+struct GemmStage {
+    GemmDesc* probs = nullptr;
+    GemmTile* tiles = nullptr;
+    int64_t n_tiles = 0;
+    void upload(std::vector<GemmDesc> const& v, hipStream_t s) {
+        auto t = make_tiles(v);
+        probs = dev_upload(v, s);
+        tiles = dev_upload(t, s);
+        n_tiles = int64_t(t.size());
+    }
+    void run(Bases const& b, hipStream_t s, int precision) const {
+        if (precision != MQ_PREC_FP64)
+            throw Error(MQ_ERR_UNSUPPORTED, "only MQ_PREC_FP64 sector GEMMs are available in this build");
+        hipcheck(launch_gemm_f64(probs, tiles, n_tiles, b, s), "gemm launch");
+    }
+    void release() {
+        dev_free(probs);
+        dev_free(tiles);
+        probs = nullptr;
+        tiles = nullptr;
+    }
+};
+
+Index index_from(const mq_index_desc* d) {
+    require(d != nullptr, "null index descriptor");
+    return Index(*d);
+}
+
+void ensure_rp(mq_mps_t m) {
+    if (m->rp_ok) return;
+    require(m->lp_ok, "mps: no valid data");
+    hipcheck(launch_reshape(m->d_segs, m->n_segs, m->lp, m->rp, 0, m->ctx->stream), "reshape");
+    m->rp_ok = true;
+}
+void ensure_lp(mq_mps_t m) {
+    if (m->lp_ok) return;
+    require(m->rp_ok, "mps: no valid data");
+    hipcheck(launch_reshape(m->d_segs, m->n_segs, m->rp, m->lp, 1, m->ctx->stream), "reshape");
+    m->lp_ok = true;
+}
+
+}  // namespace
+
+struct mq_hpsi_plan_s {
+    mq_ctx_t ctx;
+    HpsiPlanHost host;
+    const double* rdata = nullptr;
+    GemmStage tg, cg;
+    SLine* lines = nullptr;
+    STerm* terms = nullptr;
+    int64_t n_lines = 0;
+    double *T = nullptr, *S = nullptr, *Lpanel = nullptr;
+    double build_seconds = 0;
+};
+
+// ================================================================================================
+extern "C" {
+
+int mq_version(void) { return 1; }
+
+double mq_wigner_6j(int a, int b, int c, int d, int e, int f) { return wigner_6j(a, b, c, d, e, f); }
+double mq_wigner_9j(int a, int b, int c, int d, int e, int f, int g, int h, int i) {
+    return wigner_9j(a, b, c, d, e, f, g, h, i);
+}
+int mq_index_position(const mq_index_desc* idx, int32_t q0, int32_t q1) {
+    if (!idx) return MQ_ERR_INVALID;
+    try {
+        Index I(*idx);
+        return int(I.position(Charge(q0, q1)));
+    } catch (...) {
+        return MQ_ERR_INVALID;
+    }
+}
+
+int mq_ctx_create(int device, int symmetry, mq_ctx_t* out) {
+    if (!out) return MQ_ERR_INVALID;
+    *out = nullptr;
+    if (symmetry != MQ_SYM_TWOU1 && symmetry != MQ_SYM_SU2U1) return MQ_ERR_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MQ_ERR_HIP;
+    auto* c = new mq_ctx_s();
+    c->device = device;
+    c->sym = Sym(symmetry);
+    int rc = guarded(c, [&] {
+        hipcheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream create");
+        c->dwork = dev_alloc<double>(DOT_BLOCKS + 64);
+    });
+    if (rc != MQ_OK) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return MQ_OK;
+}
+
+int mq_ctx_destroy(mq_ctx_t ctx) {
+    if (!ctx) return MQ_OK;
+    int rc = guarded(ctx, [&] {
+        (void)hipStreamSynchronize(ctx->stream);
+        dev_free(ctx->dwork);
+        (void)hipStreamDestroy(ctx->stream);
+    });
+    delete ctx;
+    return rc;
+}
+
+const char* mq_last_error(mq_ctx_t ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mq_ctx_sync(mq_ctx_t ctx) {
+    return guarded(ctx, [&] { hipcheck(hipStreamSynchronize(ctx->stream), "stream synchronize"); });
+}
+
+void* mq_ctx_stream(mq_ctx_t ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+// ---- MPO -------------------------------------------------------------------------------------
+int mq_mpo_create(mq_ctx_t ctx, const mq_mpo_desc* desc, mq_mpo_t* out) {
+    return guarded(ctx, [&] {
+        require(desc && out, "mq_mpo_create: null argument");
+        *out = new mq_mpo_s{ctx, MPO(*desc)};
+    });
+}
+int mq_mpo_destroy(mq_mpo_t m) {
+    delete m;
+    return MQ_OK;
+}
+
+// ---- boundaries ------------------------------------------------------------------------------
+int mq_boundary_create_initial(mq_ctx_t ctx, const mq_index_desc* idx, uint32_t aux_dim, mq_boundary_t* out) {
+    return guarded(ctx, [&] {
+        require(out && aux_dim >= 1, "mq_boundary_create_initial: bad argument");
+        auto* b = new mq_boundary_s{ctx, initial_boundary_index(index_from(idx), aux_dim)};
+        std::unique_ptr<mq_boundary_s> guard(b);
+        b->n = b->idx.layout();
+        b->data = dev_alloc<double>(b->n);
+        hipcheck(launch_fill(b->data, b->n, 1.0, ctx->stream), "fill");
+        hipcheck(hipStreamSynchronize(ctx->stream), "sync");
+        *out = guard.release();
+    });
+}
+
+int mq_boundary_upload(mq_ctx_t ctx, const mq_bindex_desc* d, const double* data, mq_boundary_t* out) {
+    return guarded(ctx, [&] {
+        require(d && out, "mq_boundary_upload: bad argument");
+        auto* b = new mq_boundary_s{ctx, boundary_index_from(*d)};
+        std::unique_ptr<mq_boundary_s> guard(b);
+        b->n = b->idx.layout();
+        b->data = dev_alloc<double>(b->n);
+        if (b->n) {
+            require(data != nullptr, "mq_boundary_upload: null data");
+            hipcheck(hipMemcpyAsync(b->data, data, b->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload");
+        }
+        hipcheck(hipStreamSynchronize(ctx->stream), "sync");
+        *out = guard.release();
+    });
+}
+
+int mq_boundary_info(mq_boundary_t b, mq_boundary_info_t* info) {
+    if (!b || !info) return MQ_ERR_INVALID;
+    info->n_cohorts = int32_t(b->idx.n_cohorts());
+    info->aux_dim = b->idx.aux_dim;
+    info->n_bra = int32_t(b->idx.bra.size());
+    info->n_ket = int32_t(b->idx.ket.size());
+    info->total_elements = b->n;
+    return MQ_OK;
+}
+
+int mq_boundary_export_index(mq_boundary_t b, int32_t* bra_charges, uint32_t* bra_sizes, int32_t* ket_charges,
+                             uint32_t* ket_sizes, uint32_t* cohort_lr, int64_t* offsets, double* conj_scales,
+                             uint8_t* transposes, uint8_t* tr) {
+    if (!b) return MQ_ERR_INVALID;
+    BoundaryIndex const& B = b->idx;
+    for (size_t i = 0; i < B.bra.size(); ++i) {
+        if (bra_charges) { bra_charges[2 * i] = B.bra.charges[i].q[0]; bra_charges[2 * i + 1] = B.bra.charges[i].q[1]; }
+        if (bra_sizes) bra_sizes[i] = B.bra.sizes[i];
+    }
+    for (size_t i = 0; i < B.ket.size(); ++i) {
+        if (ket_charges) { ket_charges[2 * i] = B.ket.charges[i].q[0]; ket_charges[2 * i + 1] = B.ket.charges[i].q[1]; }
+        if (ket_sizes) ket_sizes[i] = B.ket.sizes[i];
+    }
+    for (uint32_t c = 0; c < B.n_cohorts(); ++c) {
+        if (cohort_lr) { cohort_lr[2 * c] = B.clr[c].first; cohort_lr[2 * c + 1] = B.clr[c].second; }
+        for (uint32_t k = 0; k < B.aux_dim; ++k) {
+            if (offsets) offsets[uint64_t(c) * B.aux_dim + k] = B.offsets[c][k];
+            if (conj_scales) conj_scales[uint64_t(c) * B.aux_dim + k] = B.conj_scales[c][k];
+            if (transposes) transposes[uint64_t(c) * B.aux_dim + k] = B.transposes[c][k];
+        }
+        if (tr) tr[c] = B.tr[c];
+    }
+    return MQ_OK;
+}
+
+int mq_boundary_download(mq_boundary_t b, double* data) {
+    if (!b) return MQ_ERR_INVALID;
+    return guarded(b->ctx, [&] {
+        require(data || !b->n, "mq_boundary_download: null data");
+        hipcheck(hipStreamSynchronize(b->ctx->stream), "sync");
+        if (b->n) hipcheck(hipMemcpy(data, b->data, b->n * sizeof(double), hipMemcpyDeviceToHost), "download");
+    });
+}
+
+int mq_boundary_destroy(mq_boundary_t b) {
+    if (!b) return MQ_OK;
+    int rc = guarded(b->ctx, [&] {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        dev_free(b->data);
+    });
+    delete b;
+    return rc;
+}
+
+// ---- MPS ---------------------------------------------------------------------------------------
+int mq_mps_upload(mq_ctx_t ctx, const mq_index_desc* left, const mq_index_desc* phys, const mq_index_desc* right,
+                  int pairing, const double* data, mq_mps_t* out) {
+    return guarded(ctx, [&] {
+        require(out && (pairing == MQ_RIGHT_PAIRED || pairing == MQ_LEFT_PAIRED), "mq_mps_upload: bad argument");
+        auto* m = new mq_mps_s{ctx, MPSLayout(index_from(left), index_from(phys), index_from(right))};
+        std::unique_ptr<mq_mps_s> guard(m);
+        MPSLayout const& L = m->lay;
+        require(L.elements() == L.lp_off.back(), "mq_mps_upload: inconsistent MPS sectors");
+        std::vector<ReshapeSeg> segs = reshape_segments(L);
+        m->n_segs = int32_t(segs.size());
+        m->d_segs = dev_upload(segs, ctx->stream);
+        uint64_t n = L.elements();
+        m->rp = dev_alloc<double>(n);
+        m->lp = dev_alloc<double>(n);
+        if (n) {
+            require(data != nullptr, "mq_mps_upload: null data");
+            double* dst = pairing == MQ_RIGHT_PAIRED ? m->rp : m->lp;
+            hipcheck(hipMemcpyAsync(dst, data, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload");
+        }
+        (pairing == MQ_RIGHT_PAIRED ? m->rp_ok : m->lp_ok) = true;
+        hipcheck(hipStreamSynchronize(ctx->stream), "sync");
+        *out = guard.release();
+    });
+}
+
+int mq_mps_elements(mq_mps_t m, uint64_t* n) {
+    if (!m || !n) return MQ_ERR_INVALID;
+    *n = m->lay.elements();
+    return MQ_OK;
+}
+
+int mq_mps_download(mq_mps_t m, int pairing, double* data) {
+    if (!m) return MQ_ERR_INVALID;
+    return guarded(m->ctx, [&] {
+        require(pairing == MQ_RIGHT_PAIRED || pairing == MQ_LEFT_PAIRED, "mq_mps_download: bad pairing");
+        if (pairing == MQ_RIGHT_PAIRED) ensure_rp(m); else ensure_lp(m);
+        hipcheck(hipStreamSynchronize(m->ctx->stream), "sync");
+        uint64_t n = m->lay.elements();
+        if (n) hipcheck(hipMemcpy(data, pairing == MQ_RIGHT_PAIRED ? m->rp : m->lp, n * sizeof(double),
+                                  hipMemcpyDeviceToHost), "download");
+    });
+}
+
+int mq_mps_destroy(mq_mps_t m) {
+    if (!m) return MQ_OK;
+    int rc = guarded(m->ctx, [&] {
+        (void)hipStreamSynchronize(m->ctx->stream);
+        dev_free(m->rp);
+        dev_free(m->lp);
+        dev_free(m->d_segs);
+    });
+    delete m;
+    return rc;
+}
+
+// ---- growth steps ------------------------------------------------------------------------------
+static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double const* bra, uint64_t bra_n,
+                       double const* ket, uint64_t ket_n, uint64_t bnd_n, int precision, mq_boundary_t* out) {
+    uint64_t sizes[BASE_COUNT] = {0};
+    sizes[BASE_T] = P.t_elems;
+    sizes[BASE_S] = P.s_elems;
+    sizes[BASE_BOUNDARY] = bnd_n;
+    sizes[BASE_BRA] = bra_n;
+    sizes[BASE_KET] = ket_n;
+    sizes[BASE_NEW] = P.out_elems;
+    check_gemms(P.t_gemms, sizes);
+    check_gemms(P.c_gemms, sizes);
+    check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
+
+    auto* nb = new mq_boundary_s{ctx, std::move(P.out)};
+    std::unique_ptr<mq_boundary_s> guard(nb);
+    nb->n = P.out_elems;
+    nb->data = dev_alloc<double>(nb->n);
+    double* T = dev_alloc<double>(P.t_elems);
+    double* S = dev_alloc<double>(P.s_elems);
+    GemmStage tg, cg;
+    SLine* lines = nullptr;
+    STerm* terms = nullptr;
+    try {
+        tg.upload(P.t_gemms, ctx->stream);
+        cg.upload(P.c_gemms, ctx->stream);
+        lines = dev_upload(P.lines, ctx->stream);
+        terms = dev_upload(P.terms, ctx->stream);
+        Bases b{};
+        b.p[BASE_T] = T;
+        b.p[BASE_S] = S;
+        b.p[BASE_BOUNDARY] = const_cast<double*>(bnd);
+        b.p[BASE_BRA] = const_cast<double*>(bra);
+        b.p[BASE_KET] = const_cast<double*>(ket);
+        b.p[BASE_NEW] = nb->data;
+        hipcheck(launch_fill(S, P.s_elems, 0.0, ctx->stream), "fill");
+        tg.run(b, ctx->stream, precision);
+        hipcheck(launch_sassembly(lines, int64_t(P.lines.size()), terms, T, S, ctx->stream), "sassembly");
+        cg.run(b, ctx->stream, precision);
+        hipcheck(hipStreamSynchronize(ctx->stream), "growth step");
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        tg.release(); cg.release(); dev_free(lines); dev_free(terms); dev_free(T); dev_free(S);
+        throw;
+    }
+    tg.release(); cg.release(); dev_free(lines); dev_free(terms); dev_free(T); dev_free(S);
+    *out = guard.release();
+}
+
+int mq_lstep(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_mps_t bra, mq_mps_t ket, int symmetric, int precision,
+             mq_boundary_t* out) {
+    return guarded(ctx, [&] {
+        require(mpo && left && bra && ket && out, "mq_lstep: null argument");
+        GrowPlanHost P = build_lstep_plan(ctx->sym, mpo->mpo, left->idx, bra->lay, ket->lay, symmetric != 0);
+        ensure_lp(bra);
+        ensure_rp(ket);
+        run_growth(ctx, P, left->data, bra->lp, bra->lay.elements(), ket->rp, ket->lay.elements(), left->n,
+                   precision, out);
+    });
+}
+
+int mq_rstep(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t right, mq_mps_t bra, mq_mps_t ket, int symmetric, int precision,
+             mq_boundary_t* out) {
+    return guarded(ctx, [&] {
+        require(mpo && right && bra && ket && out, "mq_rstep: null argument");
+        GrowPlanHost P = build_rstep_plan(ctx->sym, mpo->mpo, right->idx, bra->lay, ket->lay, symmetric != 0);
+        ensure_rp(bra);
+        ensure_rp(ket);
+        run_growth(ctx, P, right->data, bra->rp, bra->lay.elements(), ket->rp, ket->lay.elements(), right->n,
+                   precision, out);
+    });
+}
+
+// ---- H*psi ------------------------------------------------------------------------------------
+int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_boundary_t right,
+                        const mq_index_desc* left_i, const mq_index_desc* phys_i, const mq_index_desc* right_i,
+                        mq_hpsi_plan_t* out) {
+    return guarded(ctx, [&] {
+        require(mpo && left && right && out, "mq_hpsi_plan_create: null argument");
+        auto t0 = std::chrono::steady_clock::now();
+        auto* p = new mq_hpsi_plan_s{ctx, build_hpsi_plan(ctx->sym, mpo->mpo, left->idx, right->idx, index_from(left_i),
+                                                          index_from(phys_i), index_from(right_i))};
+        std::unique_ptr<mq_hpsi_plan_s> guard(p);
+        HpsiPlanHost& H = p->host;
+        uint64_t psi_n = H.mps.elements();
+        uint64_t sizes[BASE_COUNT] = {0};
+        sizes[BASE_PSI_IN] = psi_n;
+        sizes[BASE_PSI_OUT] = psi_n;
+        sizes[BASE_T] = H.t_elems;
+        sizes[BASE_S] = H.s_elems;
+        sizes[BASE_LPANEL] = H.lpanel_elems;
+        sizes[BASE_BOUNDARY] = right->n;
+        check_gemms(H.t_gemms, sizes);
+        check_gemms(H.c_gemms, sizes);
+        check_lines(H.lines, H.terms, H.t_elems, H.s_elems);
+        for (auto const& j : H.lpanel_jobs)
+            require(j.src >= 0 && j.dst >= 0 && uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= left->n &&
+                        uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= H.lpanel_elems,
+                    "L panel job outside its buffers");
+        p->rdata = right->data;
+        p->T = dev_alloc<double>(H.t_elems);
+        p->S = dev_alloc<double>(H.s_elems);
+        p->Lpanel = dev_alloc<double>(H.lpanel_elems);
+        p->tg.upload(H.t_gemms, ctx->stream);
+        p->cg.upload(H.c_gemms, ctx->stream);
+        p->lines = dev_upload(H.lines, ctx->stream);
+        p->terms = dev_upload(H.terms, ctx->stream);
+        p->n_lines = int64_t(H.lines.size());
+        CopyJob* jobs = dev_upload(H.lpanel_jobs, ctx->stream);
+        hipError_t e = launch_copy_jobs(jobs, int32_t(H.lpanel_jobs.size()), left->data, p->Lpanel, ctx->stream);
+        hipError_t e2 = hipStreamSynchronize(ctx->stream);
+        dev_free(jobs);
+        hipcheck(e, "L panel copy");
+        hipcheck(e2, "L panel copy");
+        // the host tables are on the device now; keep only what the statistics need
+        H.terms.clear(); H.terms.shrink_to_fit();
+        H.lines.clear(); H.lines.shrink_to_fit();
+        p->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out = guard.release();
+    });
+}
+
+int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
+    if (!p || !s) return MQ_ERR_INVALID;
+    HpsiPlanHost const& H = p->host;
+    s->psi_elements = H.mps.elements();
+    s->n_t_tasks = H.n_t_tasks;
+    s->n_cohorts = H.n_cohorts;
+    s->n_lines = uint64_t(p->n_lines);
+    s->n_terms = H.flops.n_terms;
+    s->flops_total = H.flops.flops_total;
+    s->flops_gemm = H.flops.flops_gemm;
+    s->flops_gemm_exec = H.flops.flops_gemm_exec;
+    s->bytes_compulsory = H.bytes_compulsory;
+    s->t_elements = H.t_elems;
+    s->s_elements = H.s_elems;
+    s->l_panel_elements = H.lpanel_elems;
+    s->r_tr_elements = 0;
+    s->build_seconds = p->build_seconds;
+    return MQ_OK;
+}
+
+int mq_hpsi_plan_psi_elements(mq_hpsi_plan_t p, uint64_t* n) {
+    if (!p || !n) return MQ_ERR_INVALID;
+    *n = p->host.mps.elements();
+    return MQ_OK;
+}
+
+static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
+    hipStream_t s = p->ctx->stream;
+    Bases b{};
+    b.p[BASE_PSI_IN] = const_cast<double*>(in);
+    b.p[BASE_PSI_OUT] = out;
+    b.p[BASE_T] = p->T;
+    b.p[BASE_S] = p->S;
+    b.p[BASE_LPANEL] = p->Lpanel;
+    b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
+    hipcheck(launch_fill(p->S, p->host.s_elems, 0.0, s), "fill");
+    p->tg.run(b, s, precision);
+    hipcheck(launch_sassembly(p->lines, p->n_lines, p->terms, p->T, p->S, s), "sassembly");
+    p->cg.run(b, s, precision);
+}
+
+int mq_hpsi(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
+    if (!p) return MQ_ERR_INVALID;
+    return guarded(p->ctx, [&] {
+        require(in && out && in != out, "mq_hpsi: input and output must be distinct device buffers");
+        hpsi_enqueue(p, in, out, precision);
+    });
+}
+
+int mq_hpsi_host(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
+    if (!p) return MQ_ERR_INVALID;
+    return guarded(p->ctx, [&] {
+        require(in && out, "mq_hpsi_host: null argument");
+        uint64_t n = p->host.mps.elements();
+        double* din = dev_alloc<double>(n);
+        double* dout = dev_alloc<double>(n);
+        try {
+            hipcheck(hipMemcpyAsync(din, in, n * sizeof(double), hipMemcpyHostToDevice, p->ctx->stream), "upload");
+            hpsi_enqueue(p, din, dout, precision);
+            hipcheck(hipMemcpyAsync(out, dout, n * sizeof(double), hipMemcpyDeviceToHost, p->ctx->stream), "download");
+            hipcheck(hipStreamSynchronize(p->ctx->stream), "hpsi");
+        } catch (...) {
+            (void)hipStreamSynchronize(p->ctx->stream);
+            dev_free(din);
+            dev_free(dout);
+            throw;
+        }
+        dev_free(din);
+        dev_free(dout);
+    });
+}
+
+int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
+    if (!p) return MQ_OK;
+    int rc = guarded(p->ctx, [&] {
+        (void)hipStreamSynchronize(p->ctx->stream);
+        p->tg.release();
+        p->cg.release();
+        dev_free(p->lines);
+        dev_free(p->terms);
+        dev_free(p->T);
+        dev_free(p->S);
+        dev_free(p->Lpanel);
+    });
+    delete p;
+    return rc;
+}
+
+}  // extern "C"
+
+// ---- Jacobi-Davidson ----------------------------------------------------------------------------
+namespace {
+
+// lowest eigenpair of a small symmetric matrix by cyclic Jacobi rotations (n <= 64)
+void small_sym_lowest(std::vector<double> a, int n, double& theta, std::vector<double>& vec) {
+    std::vector<double> v(size_t(n) * n, 0.0);
+    for (int i = 0; i < n; ++i) v[size_t(i) * n + i] = 1.0;
+    auto A = [&](int i, int j) -> double& { return a[size_t(i) * n + j]; };
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) off += A(i, j) * A(i, j);
+        if (off < 1e-300) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                if (std::fabs(A(p, q)) < 1e-300) continue;
+                double tau = (A(q, q) - A(p, p)) / (2.0 * A(p, q));
+                double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < n; ++k) {
+                    double akp = A(k, p), akq = A(k, q);
+                    A(k, p) = c * akp - s * akq;
+                    A(k, q) = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double apk = A(p, k), aqk = A(q, k);
+                    A(p, k) = c * apk - s * aqk;
+                    A(q, k) = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    double vkp = v[size_t(k) * n + p], vkq = v[size_t(k) * n + q];
+                    v[size_t(k) * n + p] = c * vkp - s * vkq;
+                    v[size_t(k) * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    int best = 0;
+    for (int i = 1; i < n; ++i)
+        if (A(i, i) < A(best, best)) best = i;
+    theta = A(best, best);
+    vec.resize(n);
+    for (int k = 0; k < n; ++k) vec[k] = v[size_t(k) * n + best];
+}
+
+}  // namespace
+
+extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, const double* const* ortho, double tol,
+                           int max_iter, int precision, double* eval, double* psi_out, mq_davidson_stats_t* stats) {
+    if (!p) return MQ_ERR_INVALID;
+    return guarded(p->ctx, [&] {
+        require(psi0 && eval && psi_out && max_iter >= 1 && n_ortho >= 0, "mq_davidson: bad argument");
+        require(n_ortho == 0 || ortho, "mq_davidson: null ortho vectors");
+        auto t_start = std::chrono::steady_clock::now();
+        hipStream_t s = p->ctx->stream;
+        const uint64_t n = p->host.mps.elements();
+        require(n > 0, "mq_davidson: empty site problem");
+        // the reference truncates the ortho list to at most n-1 vectors (ietl_jacobi_davidson.h:41-42)
+        int no = std::min<int64_t>(n_ortho, int64_t(n) - 1);
+        std::vector<double*> bufs;
+        auto alloc = [&]() { bufs.push_back(dev_alloc<double>(n)); return bufs.back(); };
+        double* sc = p->ctx->dwork + DOT_BLOCKS;  // device scalars
+        double* work = p->ctx->dwork;
+        double hpsi_s = 0;
+        try {
+            std::vector<double*> V(max_iter), VA(max_iter), O(no);
+            for (int i = 0; i < max_iter; ++i) { V[i] = alloc(); VA[i] = alloc(); }
+            double* u = alloc();
+            double* r = alloc();
+            for (int k = 0; k < no; ++k) {
+                O[k] = alloc();
+                hipcheck(hipMemcpyAsync(O[k], ortho[k], n * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+            }
+            hipcheck(hipMemcpyAsync(V[0], psi0, n * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+            auto dot_host = [&](const double* x, const double* y) {
+                double h;
+                hipcheck(launch_dot(x, y, n, work, sc, s), "dot");
+                hipcheck(hipMemcpyAsync(&h, sc, sizeof(double), hipMemcpyDeviceToHost, s), "dot");
+                hipcheck(hipStreamSynchronize(s), "dot");
+                return h;
+            };
+            // y -= (x.y)/(x.x) x   with coefficients kept on the device
+            auto subtract_along = [&](double* y, const double* x, bool normalized) {
+                hipcheck(launch_dot(x, y, n, work, sc + 1, s), "dot");
+                if (!normalized) hipcheck(launch_dot(x, x, n, work, sc + 2, s), "dot");
+                hipcheck(launch_axpy_dev(y, x, n, sc + 1, normalized ? nullptr : sc + 2, -1.0, s), "axpy");
+            };
+            // Gram-Schmidt of the ortho vectors among themselves (ietl_jacobi_davidson.h:43-47)
+            for (int a = 1; a < no; ++a)
+                for (int b0 = 0; b0 < a; ++b0) subtract_along(O[a], O[b0], false);
+            auto project = [&](double* t) {
+                for (int k = 0; k < no; ++k) subtract_along(t, O[k], false);
+            };
+            project(V[0]);
+            std::vector<double> M(size_t(max_iter) * max_iter, 0.0);
+            double theta = 0, rnorm = 0;
+            int it = 0;
+            while (true) {
+                double* t = V[it];
+                // modified Gram-Schmidt with one refinement when the norm collapses (jacobi.h:409-415)
+                double tau = std::sqrt(dot_host(t, t));
+                for (int i = 0; i < it; ++i) subtract_along(t, V[i], true);
+                if (std::sqrt(dot_host(t, t)) < 0.25 * tau)
+                    for (int i = 0; i < it; ++i) subtract_along(t, V[i], true);
+                project(t);
+                hipcheck(launch_dot(t, t, n, work, sc, s), "dot");
+                hipcheck(launch_scale_rsqrt(t, n, sc, s), "scale");
+                auto h0 = std::chrono::steady_clock::now();
+                hpsi_enqueue(p, t, VA[it], precision);
+                hipcheck(hipStreamSynchronize(s), "hpsi");
+                hpsi_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
+                for (int i = 0; i <= it; ++i) M[size_t(i) * max_iter + it] = dot_host(V[i], VA[it]);
+                int dim = it + 1;
+                std::vector<double> Ms(size_t(dim) * dim), svec;
+                for (int i = 0; i < dim; ++i)
+                    for (int j = i; j < dim; ++j)
+                        Ms[size_t(i) * dim + j] = Ms[size_t(j) * dim + i] = M[size_t(i) * max_iter + j];
+                small_sym_lowest(Ms, dim, theta, svec);
+                // u = V s, uA = VA s
+                hipcheck(launch_axpby(u, V[0], n, 0.0, svec[0], s), "axpby");
+                hipcheck(launch_axpby(r, VA[0], n, 0.0, svec[0], s), "axpby");
+                for (int j = 1; j < dim; ++j) {
+                    hipcheck(launch_axpby(u, V[j], n, 1.0, svec[j], s), "axpby");
+                    hipcheck(launch_axpby(r, VA[j], n, 1.0, svec[j], s), "axpby");
+                }
+                project(r);
+                hipcheck(launch_axpby(r, u, n, 1.0, -theta, s), "axpby");  // r = uA - theta u
+                ++it;
+                rnorm = std::sqrt(dot_host(r, r));
+                bool conv = rnorm <= tol * std::fabs(theta) || rnorm < tol;  // iteration.h:57
+                if (conv || it >= max_iter) break;
+                // correction t = -r + (r.u)/(u.u) u  (jcd_gmres_solver with no GMRES steps, jacobi.h:152-156)
+                double dru = dot_host(r, u), duu = dot_host(u, u);
+                hipcheck(launch_axpby(V[it], r, n, 0.0, -1.0, s), "axpby");
+                hipcheck(launch_axpby(V[it], u, n, 1.0, dru / duu, s), "axpby");
+            }
+            hipcheck(hipMemcpyAsync(psi_out, u, n * sizeof(double), hipMemcpyDeviceToHost, s), "download");
+            hipcheck(hipStreamSynchronize(s), "davidson");
+            *eval = theta;
+            if (stats) {
+                stats->iterations = it;
+                stats->residual_norm = rnorm;
+                stats->hpsi_seconds = hpsi_s;
+                stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            for (auto b : bufs) dev_free(b);
+            throw;
+        }
+        for (auto b : bufs) dev_free(b);
+    });
+}

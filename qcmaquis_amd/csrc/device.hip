@@ -1,0 +1,311 @@
+// gfx950 kernels of the contraction path.
+//
+//  * gemm_f64_kernel: grouped, batched FP64 GEMM over a host-built tile table.  64x64 output tile
+//    per 256-thread workgroup (four wave64s, 32x32 each as 2x2 v_mfma_f64_16x16x4f64), operands
+//    staged through LDS in K-slices of 16 with the next slice prefetched into registers.  Each
+//    output tile is owned by exactly one workgroup, so there are no atomics.
+//  * sassembly_kernel: the MPO-term sum.  One workgroup per output tile ("line"); the line's
+//    (alpha, T offset) terms are staged in LDS and every thread accumulates its tile elements in
+//    the term order of the reference (create_s / create_s_r, solver/tasks.cpp:425-494).
+//  * copy / reshape / fill kernels for panel set-up and MPS pairing changes.
+//  * FP64 Davidson vector kernels with device-resident coefficients.
+#include "device.hpp"
+
+namespace mq {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int BK = 16;
+constexpr int LDS_PAD = 1;
+
+__device__ inline double ld_or_zero(const double* p, bool ok) { return ok ? *p : 0.0; }
+
+__global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmDesc* __restrict__ probs,
+                                                       const GemmTile* __restrict__ tiles, Bases bases) {
+    __shared__ double As[BK][GEMM_BM + LDS_PAD];
+    __shared__ double Bs[BK][GEMM_BN + LDS_PAD];
+
+    const GemmTile tile = tiles[blockIdx.x];
+    const GemmDesc g = probs[tile.prob];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int m0 = tile.tm * GEMM_BM, n0 = tile.tn * GEMM_BN;
+    const bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B;
+
+    const double* A = bases.p[g.base_a] + g.a + int64_t(tile.batch) * g.sa;
+    const double* B = bases.p[g.base_b] + g.b + int64_t(tile.batch) * g.sb;
+    double* C = bases.p[g.base_c] + g.c + int64_t(tile.batch) * g.sc;
+
+    // per-thread load coordinates: 4 elements of the A tile and 4 of the B tile
+    // A tile (64 x 16): non-transposed -> m fastest, transposed -> k fastest
+    int a_m[4], a_k[4], b_k[4], b_n[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (!ta) { a_m[i] = tid & 63; a_k[i] = (tid >> 6) + 4 * i; }
+        else     { a_k[i] = tid & 15; a_m[i] = (tid >> 4) + 16 * i; }
+        if (!tb) { b_k[i] = tid & 15; b_n[i] = (tid >> 4) + 16 * i; }
+        else     { b_n[i] = tid & 63; b_k[i] = (tid >> 6) + 4 * i; }
+    }
+    double ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int m = m0 + a_m[i], k = k0 + a_k[i];
+            bool ok = (m < g.M) && (k < g.K);
+            const double* pa = ta ? A + int64_t(m) * g.lda + k : A + m + int64_t(k) * g.lda;
+            ra[i] = ld_or_zero(pa, ok);
+            int n = n0 + b_n[i], kb = k0 + b_k[i];
+            bool okb = (n < g.N) && (kb < g.K);
+            const double* pb = tb ? B + n + int64_t(kb) * g.ldb : B + kb + int64_t(n) * g.ldb;
+            rb[i] = ld_or_zero(pb, okb);
+        }
+    };
+
+    double4_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int fr = lane & 15, fk = lane >> 4;
+
+    if (g.K > 0) load(0);
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            As[a_k[i]][a_m[i]] = ra[i];
+            Bs[b_k[i]][b_n[i]] = rb[i];
+        }
+        __syncthreads();
+        if (k0 + BK < g.K) load(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            double af[2], bf[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) af[i] = As[kk + fk][wm + 16 * i + fr];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bf[j] = Bs[kk + fk][wn + 16 * j + fr];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    }
+
+    // D layout of v_mfma_f64_16x16x4f64: lane holds rows 4*(lane/16) .. +3 of column lane%16
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            int n = n0 + wn + 16 * j + fr;
+            if (n >= g.N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int m = m0 + wm + 16 * i + 4 * fk + r;
+                if (m < g.M) C[m + int64_t(n) * g.ldc] = acc[i][j][r];
+            }
+        }
+}
+
+constexpr int SA_THREADS = 256;
+constexpr int SA_CHUNK = 256;
+
+__global__ __launch_bounds__(SA_THREADS) void sassembly_kernel(const SLine* __restrict__ lines,
+                                                                const STerm* __restrict__ terms,
+                                                                const double* __restrict__ T, double* __restrict__ S) {
+    __shared__ double s_alpha[SA_CHUNK];
+    __shared__ int64_t s_off[SA_CHUNK];
+    const SLine ln = lines[blockIdx.x];
+    const int n_el = ln.rows * ln.cols;
+    const int tid = threadIdx.x;
+    // up to 4 accumulators per thread in registers; larger tiles loop over passes
+    for (int base = 0; base < n_el; base += 4 * SA_THREADS) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int t0 = ln.t0; t0 < ln.t1; t0 += SA_CHUNK) {
+            int nt = min(SA_CHUNK, ln.t1 - t0);
+            __syncthreads();
+            if (tid < nt) {
+                s_alpha[tid] = terms[t0 + tid].alpha;
+                s_off[tid] = terms[t0 + tid].t_off;
+            }
+            __syncthreads();
+            for (int k = 0; k < nt; ++k) {
+                const double a = s_alpha[k];
+                const double* src = T + s_off[k];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    int idx = base + e * SA_THREADS + tid;
+                    if (idx < n_el) acc[e] += a * src[idx];
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int idx = base + e * SA_THREADS + tid;
+            if (idx < n_el) {
+                int r = idx % ln.rows, c = idx / ln.rows;
+                S[ln.s_off + r + int64_t(c) * ln.lds] = acc[e];
+            }
+        }
+    }
+}
+
+__global__ void fill_kernel(double* p, uint64_t n, double v) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+// grid: x = job, y = chunk of the job's elements
+__global__ void copy_jobs_kernel(const CopyJob* __restrict__ jobs, const double* __restrict__ src,
+                                 double* __restrict__ dst) {
+    const CopyJob j = jobs[blockIdx.x];
+    const int64_t per = int64_t(j.rows) * j.cols;
+    const int64_t total = per * j.nblocks;
+    for (int64_t i = int64_t(blockIdx.y) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.y) * blockDim.x) {
+        int64_t b = i / per, e = i % per;
+        int r = int(e % j.rows), c = int(e / j.rows);
+        // destination block rows x cols; transposed source block is cols x rows
+        double v = j.trans ? src[j.src + b * per + c + int64_t(r) * j.cols] : src[j.src + b * per + e];
+        dst[j.dst + b * per + e] = v;
+    }
+}
+
+// element (ss, rr, ll) of a segment: rp[ll + (ss*rs + rr)*ls]  <->  lp[ss*ls + ll + rr*lp_ld]
+__global__ void reshape_kernel(const ReshapeSeg* __restrict__ segs, const double* __restrict__ src,
+                               double* __restrict__ dst, int to_left) {
+    const ReshapeSeg s = segs[blockIdx.x];
+    const int64_t total = int64_t(s.ls) * s.rs * s.d;
+    for (int64_t i = int64_t(blockIdx.y) * blockDim.x + threadIdx.x; i < total; i += int64_t(gridDim.y) * blockDim.x) {
+        int ll = int(i % s.ls);
+        int64_t q = i / s.ls;  // = ss*rs + rr
+        int rr = int(q % s.rs), ss = int(q / s.rs);
+        int64_t rp = s.rp + ll + q * s.ls;
+        int64_t lp = s.lp + int64_t(ss) * s.ls + ll + int64_t(rr) * s.lp_ld;
+        if (to_left) dst[lp] = src[rp];
+        else dst[rp] = src[lp];
+    }
+}
+
+constexpr int DOT_THREADS = 256;
+
+__device__ inline double block_sum(double v) {
+    __shared__ double red[DOT_THREADS / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < DOT_THREADS / 64; ++i) r += red[i];
+    return r;
+}
+
+__global__ __launch_bounds__(DOT_THREADS) void dot_partial_kernel(const double* __restrict__ x,
+                                                                  const double* __restrict__ y, uint64_t n,
+                                                                  double* __restrict__ work) {
+    double s = 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * DOT_THREADS + threadIdx.x; i < n; i += uint64_t(gridDim.x) * DOT_THREADS)
+        s += x[i] * y[i];
+    double r = block_sum(s);
+    if (threadIdx.x == 0) work[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(DOT_THREADS) void dot_final_kernel(const double* __restrict__ work, int nparts,
+                                                                double* __restrict__ out) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += DOT_THREADS) s += work[i];
+    double r = block_sum(s);
+    if (threadIdx.x == 0) out[0] = r;
+}
+
+__global__ void axpy_dev_kernel(double* __restrict__ y, const double* __restrict__ x, uint64_t n,
+                                const double* __restrict__ num, const double* __restrict__ den, double sign) {
+    const double coef = sign * num[0] / (den ? den[0] : 1.0);
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        y[i] += coef * x[i];
+}
+
+__global__ void axpby_kernel(double* __restrict__ y, const double* __restrict__ x, uint64_t n, double a, double b) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        y[i] = a * y[i] + b * x[i];
+}
+
+__global__ void scale_rsqrt_kernel(double* __restrict__ x, uint64_t n, const double* __restrict__ nrm2) {
+    const double s = 1.0 / sqrt(nrm2[0]);
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        x[i] *= s;
+}
+
+inline unsigned grid_for(uint64_t n, int threads, unsigned cap = 8192) {
+    uint64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    return unsigned(g > cap ? cap : g);
+}
+
+}  // namespace
+
+hipError_t launch_gemm_f64(const GemmDesc* probs, const GemmTile* tiles, int64_t n_tiles, Bases bases,
+                           hipStream_t stream) {
+    if (n_tiles <= 0) return hipSuccess;
+    for (int64_t off = 0; off < n_tiles; off += 0x7fffffff) {
+        int64_t cnt = n_tiles - off < 0x7fffffff ? n_tiles - off : 0x7fffffff;
+        hipLaunchKernelGGL(gemm_f64_kernel, dim3(unsigned(cnt)), dim3(256), 0, stream, probs, tiles + off, bases);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_sassembly(const SLine* lines, int64_t n_lines, const STerm* terms, const double* T, double* S,
+                            hipStream_t stream) {
+    if (n_lines <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sassembly_kernel, dim3(unsigned(n_lines)), dim3(SA_THREADS), 0, stream, lines, terms, T, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double* p, uint64_t n, double v, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_jobs(const CopyJob* jobs, int32_t n_jobs, const double* src, double* dst, hipStream_t stream) {
+    if (n_jobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_jobs_kernel, dim3(unsigned(n_jobs), 16), dim3(256), 0, stream, jobs, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_reshape(const ReshapeSeg* segs, int32_t n_segs, const double* src, double* dst, int to_left,
+                          hipStream_t stream) {
+    if (n_segs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(reshape_kernel, dim3(unsigned(n_segs), 8), dim3(256), 0, stream, segs, src, dst, to_left);
+    return hipGetLastError();
+}
+
+hipError_t launch_dot(const double* x, const double* y, uint64_t n, double* work, double* out, hipStream_t stream) {
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(DOT_BLOCKS), dim3(DOT_THREADS), 0, stream, x, y, n, work);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(DOT_THREADS), 0, stream, work, DOT_BLOCKS, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy_dev(double* y, const double* x, uint64_t n, const double* num, const double* den, double sign,
+                           hipStream_t stream) {
+    hipLaunchKernelGGL(axpy_dev_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, y, x, n, num, den, sign);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpby(double* y, const double* x, uint64_t n, double a, double b, hipStream_t stream) {
+    hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, y, x, n, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_rsqrt(double* x, uint64_t n, const double* nrm2, hipStream_t stream) {
+    hipLaunchKernelGGL(scale_rsqrt_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, x, n, nrm2);
+    return hipGetLastError();
+}
+
+}  // namespace mq

@@ -1,0 +1,51 @@
+// Launchers of the gfx950 kernels (device.hip).  All launches are stream-ordered; none allocates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "descriptors.hpp"
+#include "plans.hpp"
+
+namespace mq {
+
+// operand base pointers of a grouped GEMM launch, indexed by BASE_*
+struct Bases {
+    double* p[BASE_COUNT];
+};
+
+// Tile of the grouped GEMM: problem, batch entry and output tile coordinates.
+struct GemmTile {
+    int32_t prob, batch, tm, tn;
+};
+
+constexpr int GEMM_BM = 64;
+constexpr int GEMM_BN = 64;
+
+hipError_t launch_gemm_f64(const GemmDesc* probs, const GemmTile* tiles, int64_t n_tiles, Bases bases,
+                           hipStream_t stream);
+hipError_t launch_sassembly(const SLine* lines, int64_t n_lines, const STerm* terms, const double* T, double* S,
+                            hipStream_t stream);
+hipError_t launch_fill(double* p, uint64_t n, double v, hipStream_t stream);
+hipError_t launch_copy_jobs(const CopyJob* jobs, int32_t n_jobs, const double* src, double* dst, hipStream_t stream);
+
+// right-paired <-> left-paired MPS reshape segments (one per (left sector, sigma) with a right
+// sector); rp / lp are the element offsets of the (0, col) / (row, 0) corners
+using ReshapeSeg = ReshapeSegHost;
+hipError_t launch_reshape(const ReshapeSeg* segs, int32_t n_segs, const double* src, double* dst, int to_left,
+                          hipStream_t stream);
+
+// ---- Davidson vector kernels (FP64) ---------------------------------------------------------
+// out[0] = x . y  (deterministic two-level reduction; work holds DOT_BLOCKS partials)
+constexpr int DOT_BLOCKS = 512;
+hipError_t launch_dot(const double* x, const double* y, uint64_t n, double* work, double* out, hipStream_t stream);
+// y += coef * x with coef = sign * num[0] / (den ? den[0] : 1)  (coefficients stay on the device)
+hipError_t launch_axpy_dev(double* y, const double* x, uint64_t n, const double* num, const double* den, double sign,
+                           hipStream_t stream);
+// y = a*y + b*x (host scalars)
+hipError_t launch_axpby(double* y, const double* x, uint64_t n, double a, double b, hipStream_t stream);
+// x *= 1/sqrt(nrm2[0])
+hipError_t launch_scale_rsqrt(double* x, uint64_t n, const double* nrm2, hipStream_t stream);
+
+}  // namespace mq

@@ -1,0 +1,789 @@
+#include "plans.hpp"
+
+#include <cmath>
+#include <memory>
+#include <unordered_map>
+#include <unordered_set>
+
+#include "coupling.hpp"
+
+namespace mq {
+
+namespace {
+
+// Output-tile accumulator of one cohort: units (sigma', ss2) with their term lines, in the
+// reference's insertion order (Cohort / SUnit, solver/tasks.h:50-190, tasks.cpp:72-196).
+struct CohortAcc {
+    struct Unit {
+        uint32_t offset = 0, ms = 0;
+        std::vector<double> alpha;
+        std::vector<uint32_t> ti, col;
+        std::vector<uint32_t> line_count, line_b;
+        uint32_t pending = 0;
+    };
+    std::vector<uint32_t> sfold;
+    std::vector<Unit> units;
+    uint64_t stripe = 0;
+    std::vector<uint32_t> bcount;  // terms per MPO index b (growth steps)
+    uint64_t n_terms = 0;
+
+    CohortAcc(Index const& phys, uint32_t mpodim) : sfold(phys.size()), bcount(mpodim, 0) {
+        uint32_t acc = 0;
+        for (size_t s = 0; s < phys.size(); ++s) {
+            sfold[s] = acc;
+            acc += phys.sizes[s];
+        }
+        units.resize(acc);
+    }
+    void add_unit(size_t s, uint32_t d, uint32_t ms, uint64_t offset) {
+        for (uint32_t i = 0; i < d; ++i) {
+            units[sfold[s] + i].offset = uint32_t(i * ms + offset);
+            units[sfold[s] + i].ms = ms;
+            stripe += ms;
+        }
+    }
+    void push(size_t s, uint32_t ss2, double alpha, uint32_t ti, uint32_t col) {
+        Unit& u = units[sfold[s] + ss2];
+        u.alpha.push_back(alpha);
+        u.ti.push_back(ti);
+        u.col.push_back(col);
+        ++u.pending;
+        ++n_terms;
+    }
+    void add_line(uint32_t b) {
+        for (auto& u : units)
+            if (u.pending) {
+                u.line_count.push_back(u.pending);
+                u.line_b.push_back(b);
+                if (b < bcount.size()) bcount[b] += u.pending;
+                u.pending = 0;
+            }
+    }
+};
+
+// T work list of one MPS block with the reference's (mps_offset, ci) -> ti lookup
+// (MPSBlock::get_ti, tasks.cpp:766-775).
+struct TTask {
+    uint64_t psi_col;  // column offset in the ket block
+    uint32_t ci, ci_eff;
+    uint32_t lb_ket;
+    bool used = false;
+    uint64_t t_off = 0;
+    uint64_t size = 0;   // elements of the T intermediate
+    uint32_t rows = 0;   // its leading dimension
+};
+struct TList {
+    std::vector<TTask> tasks;
+    std::unordered_map<uint64_t, uint32_t> lookup;
+    void add(uint64_t col, uint32_t ci, uint32_t ci_eff, uint32_t lb_ket) {
+        uint64_t key = (col << 24) ^ ci;
+        require(col < (uint64_t(1) << 40) && ci < (1u << 24), "T list: key overflow");
+        if (!lookup.count(key)) lookup[key] = uint32_t(tasks.size());
+        tasks.push_back(TTask{col, ci, ci_eff, lb_ket});
+    }
+    uint32_t get(uint64_t col, uint32_t ci) const {
+        auto it = lookup.find((col << 24) ^ ci);
+        return it == lookup.end() ? UINT32_MAX : it->second;
+    }
+};
+
+inline int casenr(int32_t rspin, int32_t cspin) {
+    if (rspin == 2 && cspin == 2) return 3;
+    if (rspin == 2) return 1;
+    if (cspin == 2) return 2;
+    return 0;
+}
+
+// iterate the sparse entries of one W block (reference: detail::op_iterate, op_iterate.hpp:37-69)
+inline void op_iterate(OpBlock const& blk, const double cpl[4], CohortAcc& coh, size_t s, uint32_t m2_size,
+                       TList& tl, uint64_t mps_offset, uint32_t ci, uint32_t boundary_col) {
+    for (auto const& e : blk.entries) {
+        double alpha = e.coeff * cpl[casenr(e.row_spin, e.col_spin)];
+        uint32_t ti = tl.get(mps_offset + uint64_t(e.row) * m2_size, ci);
+        if (ti != UINT32_MAX) {
+            tl.tasks[ti].used = true;
+            coh.push(s, e.col, alpha, ti, boundary_col);
+        }
+    }
+}
+
+inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// a term reads a contiguous rows x cols tile at element offset `off` of its T intermediate;
+// refuse anything that would leave the intermediate
+void check_tile(TTask const& t, uint64_t off, uint32_t rows, uint32_t cols) {
+    require(t.used && t.rows == rows && off + uint64_t(rows) * cols <= t.size,
+            "S-assembly term reads outside its T intermediate (boundary / MPS index mismatch)");
+}
+
+void check_cohort_shape(BoundaryIndex const& B, uint32_t ci, uint32_t rows, uint32_t cols, const char* what) {
+    if (B.lsz[ci] != rows || B.rsz[ci] != cols)
+        throw Error(MQ_ERR_INVALID, std::string(what) + ": boundary cohort shape does not match the MPS sectors");
+}
+
+GemmDesc gemm(int32_t M, int32_t N, int32_t K, int base_a, int64_t a, int32_t lda, int base_b, int64_t b,
+              int32_t ldb, int base_c, int64_t c, int32_t ldc, int32_t flags = 0, int32_t batch = 1,
+              int64_t sa = 0, int64_t sb = 0, int64_t sc = 0) {
+    GemmDesc g{};
+    g.a = a; g.b = b; g.c = c;
+    g.sa = sa; g.sb = sb; g.sc = sc;
+    g.M = M; g.N = N; g.K = K; g.batch = batch;
+    g.lda = std::max(lda, 1); g.ldb = std::max(ldb, 1); g.ldc = std::max(ldc, 1);
+    g.flags = flags;
+    g.base_a = base_a; g.base_b = base_b; g.base_c = base_c;
+    return g;
+}
+
+// right-boundary T-GEMMs shared by H*psi and the right step:
+// T(ls_ket x nb*rs_bra) = psi_ket(:, col : col+rs_ket) * R[ci_eff], per-block transposed when the
+// cohort is stored as its Hermitian partner (reference: MPSBlock::create_T, tasks.cpp:650-752)
+void right_t_gemm(BoundaryIndex const& R, TTask const& t, uint32_t ls_ket, uint32_t rs_ket, uint32_t rs_bra,
+                  int64_t psi_block_off, int psi_base, std::vector<GemmDesc>& out) {
+    uint32_t nb = R.nblk[t.ci_eff];
+    int64_t a = psi_block_off + int64_t(t.psi_col) * ls_ket;
+    if (t.ci == t.ci_eff) {
+        check_cohort_shape(R, t.ci_eff, rs_ket, rs_bra, "T-GEMM");
+        out.push_back(gemm(ls_ket, int32_t(nb * rs_bra), rs_ket, psi_base, a, ls_ket, BASE_BOUNDARY,
+                           int64_t(R.data_off[t.ci_eff]), rs_ket, BASE_T, int64_t(t.t_off), ls_ket));
+    } else {
+        check_cohort_shape(R, t.ci_eff, rs_bra, rs_ket, "T-GEMM");
+        out.push_back(gemm(ls_ket, rs_bra, rs_ket, psi_base, a, ls_ket, BASE_BOUNDARY, int64_t(R.data_off[t.ci_eff]),
+                           rs_bra, BASE_T, int64_t(t.t_off), ls_ket, GEMM_TRANS_B, int32_t(nb), 0,
+                           int64_t(rs_bra) * rs_ket, int64_t(ls_ket) * rs_bra));
+    }
+}
+
+void validate_boundary_offsets(BoundaryIndex const& B) {
+    for (uint32_t ci = 0; ci < B.n_cohorts(); ++ci) {
+        uint64_t bs = B.block_size(ci);
+        for (uint32_t b = 0; b < B.aux_dim; ++b) {
+            int64_t o = B.offsets[ci][b];
+            if (o < 0) continue;
+            uint32_t owner = B.tr[ci] ? B.partner(ci) : ci;
+            require(owner < B.n_cohorts(), "boundary: transposed cohort without partner");
+            uint64_t obs = B.block_size(owner);
+            require(obs > 0 && uint64_t(o) % obs == 0 && uint64_t(o) < B.cohort_size(owner),
+                    "boundary: block offset outside its cohort");
+            (void)bs;
+        }
+    }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------------------------
+// H*psi  (reference: create_contraction_schedule -> rshtm_t_tasks + shtm_tasks,
+//         site_hamil_schedule.hpp:40-93, right_tasks.hpp:39-78, site_hamil_tasks.hpp:41-137)
+// ----------------------------------------------------------------------------------------------
+HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, BoundaryIndex const& R,
+                             Index const& left_i, Index const& phys_i, Index const& right_i) {
+    HpsiPlanHost P;
+    P.mps = MPSLayout(left_i, phys_i, right_i);
+    require(L.aux_dim == mpo.row_dim, "hpsi: left boundary aux dim != MPO row dim");
+    require(R.aux_dim == mpo.col_dim, "hpsi: right boundary aux dim != MPO col dim");
+    validate_boundary_offsets(L);
+    validate_boundary_offsets(R);
+    ProductBasis rpb = ProductBasis::right_paired(phys_i, right_i);
+    const bool su2 = sym == Sym::SU2U1;
+
+    struct CohRec {
+        uint32_t lb_in, lb_out, ci, ci_eff, nb;
+        CohortAcc acc;
+    };
+    std::vector<CohRec> cohorts;
+    std::vector<TList> tlists(left_i.size());
+    std::unordered_set<uint32_t> l_used, r_used;
+
+    for (uint32_t lb_in = 0; lb_in < left_i.size(); ++lb_in) {
+        Charge lc_in = left_i.charges[lb_in];
+        uint32_t ls_in = left_i.sizes[lb_in];
+        TList& tl = tlists[lb_in];
+        // T work list (rshtm_t_tasks)
+        for (size_t s = 0; s < phys_i.size(); ++s) {
+            Charge rc_ket = lc_in + phys_i.charges[s];
+            size_t rb_ket = right_i.position(rc_ket);
+            if (rb_ket == right_i.size()) continue;
+            uint32_t rs_ket = right_i.sizes[rb_ket];
+            uint64_t ket_offset = rpb(phys_i.charges[s], rc_ket);
+            for (size_t rb_bra = 0; rb_bra < right_i.size(); ++rb_bra) {
+                uint32_t ci = R.cohort_index(rc_ket, right_i.charges[rb_bra]);
+                if (ci == R.n_cohorts()) continue;
+                uint32_t ci_eff = R.tr[ci] ? R.cohort_index(right_i.charges[rb_bra], rc_ket) : ci;
+                require(ci_eff < R.n_cohorts(), "hpsi: transposed right cohort without partner");
+                for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) tl.add(ket_offset + ss * rs_ket, ci, ci_eff, lb_in);
+            }
+        }
+        // cohorts (shtm_tasks)
+        for (uint32_t lb_out = 0; lb_out < left_i.size(); ++lb_out) {
+            Charge lc_out = left_i.charges[lb_out];
+            uint32_t ls_out = left_i.sizes[lb_out];
+            uint32_t ci = L.cohort_index(lc_out, lc_in);
+            if (ci == L.n_cohorts()) continue;
+            uint32_t ci_eff = L.tr[ci] ? L.cohort_index(lc_in, lc_out) : ci;
+            require(ci_eff < L.n_cohorts(), "hpsi: transposed left cohort without partner");
+            CohRec rec{lb_in, lb_out, ci, ci_eff, L.nblk[ci_eff], CohortAcc(phys_i, 0)};
+            CohortAcc& coh = rec.acc;
+            for (size_t s = 0; s < phys_i.size(); ++s) {
+                Charge phys_out = phys_i.charges[s];
+                Charge rc_out = lc_out + phys_out;
+                size_t rb_out = right_i.position(rc_out);
+                if (rb_out == right_i.size()) continue;
+                uint32_t rs_out = right_i.sizes[rb_out];
+                uint64_t out_offset = rpb(phys_out, rc_out);
+                coh.add_unit(s, phys_i.sizes[s], rs_out, out_offset);
+                std::unique_ptr<Wigner9jCache> w9j;
+                if (su2) w9j.reset(new Wigner9jCache(spin_of(lc_in), spin_of(lc_out), spin_of(rc_out)));
+
+                for (uint32_t b1 = 0; b1 < mpo.row_dim; ++b1) {
+                    if (!L.has_block(ci, b1)) continue;
+                    uint32_t left_idx = uint32_t(L.offset(ci, b1) / (int64_t(ls_in) * ls_out));
+                    int A = mpo.left_spin[b1];
+                    if (su2 && !triangle(spin_of(lc_in), A, spin_of(lc_out))) continue;
+                    for (auto const& cell : mpo.rows[b1]) {
+                        uint32_t b2 = cell.other;
+                        for (uint32_t t = cell.t0; t < cell.t1; ++t) {
+                            SiteOp const& W = mpo.ops[mpo.terms[t].op];
+                            int K = W.spin, Ap = mpo.right_spin[b2];
+                            for (auto const& blk : W.blocks) {
+                                if (phys_out != blk.rc) continue;
+                                Charge phys_in = blk.lc;
+                                Charge rc_in = lc_in + phys_in;
+                                uint32_t ci_right = R.cohort_index(rc_in, rc_out);
+                                if (!R.has_block(ci_right, b2)) continue;
+                                size_t rb_in = right_i.position(rc_in);
+                                if (rb_in == right_i.size()) continue;
+                                uint32_t rs_in = right_i.sizes[rb_in];
+                                uint64_t in_offset = rpb(phys_in, rc_in);
+                                int64_t right_offset = R.offset(ci_right, b2);
+                                double cpl[4];
+                                double scale = R.conjugate_scale(ci_right, b2) * mpo.terms[t].scale *
+                                               L.conjugate_scale(ci, b1);
+                                if (su2)
+                                    w9j->set_scale(A, K, Ap, spin_of(rc_in), scale, cpl);
+                                else
+                                    cpl[0] = cpl[1] = cpl[2] = cpl[3] = scale;
+                                op_iterate(blk, cpl, coh, s, rs_in, tl, in_offset, ci_right,
+                                           uint32_t(right_offset / rs_in));
+                            }
+                        }
+                    }
+                    coh.add_line(left_idx);
+                }
+            }
+            if (coh.n_terms) cohorts.push_back(std::move(rec));
+        }
+    }
+
+    // --- T buffer layout (only tasks referenced by some alpha are executed) ---
+    for (uint32_t lb_in = 0; lb_in < left_i.size(); ++lb_in) {
+        uint32_t ls_in = left_i.sizes[lb_in];
+        for (auto& t : tlists[lb_in].tasks) {
+            uint64_t csz = R.cohort_size(t.ci_eff);
+            // the reference's flop model counts every T task (tasks.cpp:778-791)
+            P.flops.flops_total += 2ull * ls_in * csz;
+            P.flops.flops_gemm += 2ull * ls_in * csz;
+            ++P.n_t_tasks;
+            if (!t.used) continue;
+            auto lr = R.clr[t.ci];
+            uint32_t rs_ket = R.bra.sizes[lr.first], rs_bra = R.ket.sizes[lr.second];
+            size_t rb = right_i.position(R.bra.charges[lr.first]);
+            require(rb < right_i.size() && right_i.sizes[rb] == rs_ket,
+                    "hpsi: right boundary index does not match the MPS right index");
+            t.rows = ls_in;
+            t.size = uint64_t(ls_in) * R.nblk[t.ci_eff] * rs_bra;
+            t.t_off = P.t_elems;
+            P.t_elems += round_up(t.size, 32);
+            r_used.insert(t.ci_eff);
+            right_t_gemm(R, t, ls_in, rs_ket, rs_bra, int64_t(P.mps.rp_off[lb_in]), BASE_PSI_IN, P.t_gemms);
+            P.flops.flops_gemm_exec += 2ull * ls_in * csz;
+        }
+    }
+
+    // --- S panels / L panels per output block: cohorts of one lb_out stacked along K ---
+    std::vector<uint64_t> ktot(left_i.size(), 0);
+    std::vector<uint64_t> koff(cohorts.size());
+    for (size_t c = 0; c < cohorts.size(); ++c) {
+        koff[c] = ktot[cohorts[c].lb_out];
+        ktot[cohorts[c].lb_out] += uint64_t(cohorts[c].nb) * left_i.sizes[cohorts[c].lb_in];
+    }
+    std::vector<uint64_t> s_off(left_i.size()), l_off(left_i.size());
+    for (size_t lb = 0; lb < left_i.size(); ++lb) {
+        s_off[lb] = P.s_elems;
+        P.s_elems += round_up(ktot[lb] * P.mps.rp_cols[lb], 32);
+        l_off[lb] = P.lpanel_elems;
+        P.lpanel_elems += round_up(uint64_t(left_i.sizes[lb]) * ktot[lb], 32);
+    }
+    for (size_t c = 0; c < cohorts.size(); ++c) {
+        auto& rec = cohorts[c];
+        uint32_t ls_in = left_i.sizes[rec.lb_in], ls_out = left_i.sizes[rec.lb_out];
+        uint64_t ldS = ktot[rec.lb_out];
+        require(rec.acc.stripe == P.mps.rp_cols[rec.lb_out], "hpsi: cohort stripe != psi block width");
+        // L panel copy job
+        CopyJob job{};
+        job.dst = int64_t(l_off[rec.lb_out] + uint64_t(ls_out) * koff[c]);
+        job.src = int64_t(L.data_off[rec.ci_eff]);
+        if (rec.ci == rec.ci_eff) {
+            check_cohort_shape(L, rec.ci_eff, ls_out, ls_in, "hpsi L panel");
+            job.rows = int32_t(ls_out);
+            job.cols = int32_t(uint64_t(rec.nb) * ls_in);
+            job.nblocks = 1;
+            job.trans = 0;
+        } else {
+            check_cohort_shape(L, rec.ci_eff, ls_in, ls_out, "hpsi L panel");
+            job.rows = int32_t(ls_out);
+            job.cols = int32_t(ls_in);
+            job.nblocks = int32_t(rec.nb);
+            job.trans = 1;
+        }
+        if (uint64_t(rec.nb) * ls_in * ls_out) P.lpanel_jobs.push_back(job);
+        l_used.insert(rec.ci_eff);
+        // S lines
+        for (auto& u : rec.acc.units) {
+            uint32_t seek = 0;
+            for (size_t l = 0; l < u.line_count.size(); ++l) {
+                SLine line{};
+                line.s_off = int64_t(s_off[rec.lb_out] + koff[c] + uint64_t(u.line_b[l]) * ls_in + uint64_t(u.offset) * ldS);
+                line.lds = int32_t(ldS);
+                line.rows = int32_t(ls_in);
+                line.cols = int32_t(u.ms);
+                line.t0 = int32_t(P.terms.size());
+                for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
+                    TTask const& t = tlists[rec.lb_in].tasks[u.ti[k]];
+                    check_tile(t, uint64_t(u.col[k]) * ls_in, ls_in, u.ms);
+                    P.terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * ls_in)});
+                }
+                line.t1 = int32_t(P.terms.size());
+                seek += u.line_count[l];
+                P.lines.push_back(line);
+            }
+            P.flops.flops_total += 2ull * ls_in * u.ms * u.alpha.size();
+        }
+        uint64_t g = 2ull * ls_out * ls_in * rec.nb * rec.acc.stripe;
+        P.flops.flops_total += g;
+        P.flops.flops_gemm += g;
+        P.flops.flops_gemm_exec += g;
+        P.flops.n_terms += rec.acc.n_terms;
+    }
+    P.n_cohorts = cohorts.size();
+    for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
+        uint32_t ls = left_i.sizes[lb];
+        P.c_gemms.push_back(gemm(int32_t(ls), int32_t(P.mps.rp_cols[lb]), int32_t(ktot[lb]), BASE_LPANEL,
+                                 int64_t(l_off[lb]), int32_t(ls), BASE_S, int64_t(s_off[lb]), int32_t(ktot[lb]),
+                                 BASE_PSI_OUT, int64_t(P.mps.rp_off[lb]), int32_t(ls)));
+    }
+    uint64_t bytes = 0;
+    for (auto ci : l_used) bytes += L.cohort_size(ci);
+    for (auto ci : r_used) bytes += R.cohort_size(ci);
+    bytes += 2 * P.mps.elements();
+    P.bytes_compulsory = 8 * bytes;
+    return P;
+}
+
+// ----------------------------------------------------------------------------------------------
+// left growth (reference: overlap_mpo_left_step, move_boundary.hpp:147-279, with
+//               lshtm_t_tasks / lshtm_tasks, left_tasks.hpp:38-172)
+// ----------------------------------------------------------------------------------------------
+GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, MPSLayout const& bra,
+                              MPSLayout const& ket, bool symmetric) {
+    GrowPlanHost P;
+    require(bra.phys == ket.phys, "lstep: bra and ket physical indices differ");
+    require(L.aux_dim == mpo.row_dim, "lstep: boundary aux dim != MPO row dim");
+    validate_boundary_offsets(L);
+    Index const& phys = ket.phys;
+    Index const& left_i = ket.left;
+    ProductBasis ket_rpb = ProductBasis::right_paired(phys, ket.right);
+    ProductBasis bra_lpb = ProductBasis::left_paired(phys, bra.left);
+    const bool su2 = sym == Sym::SU2U1;
+
+    struct CohRec {
+        uint32_t rb_bra, rb_ket;
+        CohortAcc acc;
+        std::vector<int64_t> offs;
+        uint32_t nrows;
+    };
+    std::vector<CohRec> cohorts;
+    std::vector<TList> tlists(ket.right.size());
+
+    for (uint32_t rb_ket = 0; rb_ket < ket.right.size(); ++rb_ket) {
+        Charge rc_ket = ket.right.charges[rb_ket];
+        uint32_t rs_ket = ket.right.sizes[rb_ket];
+        TList& tl = tlists[rb_ket];
+        for (size_t s = 0; s < phys.size(); ++s) {
+            Charge lc_ket = rc_ket - phys.charges[s];
+            size_t lb_ket = left_i.position(lc_ket);
+            if (lb_ket == left_i.size()) continue;
+            uint64_t ket_offset = ket_rpb(phys.charges[s], rc_ket);
+            for (size_t lb_bra = 0; lb_bra < left_i.size(); ++lb_bra) {
+                uint32_t ci = L.cohort_index(left_i.charges[lb_bra], lc_ket);
+                if (ci == L.n_cohorts()) continue;
+                uint32_t ci_eff = L.tr[ci] ? L.cohort_index(lc_ket, left_i.charges[lb_bra]) : ci;
+                require(ci_eff < L.n_cohorts(), "lstep: transposed cohort without partner");
+                for (uint32_t ss = 0; ss < phys.sizes[s]; ++ss)
+                    tl.add(ket_offset + ss * rs_ket, ci, ci_eff, uint32_t(lb_ket));
+            }
+        }
+        for (uint32_t rb_bra = 0; rb_bra < bra.right.size(); ++rb_bra) {
+            Charge rc_bra = bra.right.charges[rb_bra];
+            if (std::abs(particle_number(sym, rc_bra) - particle_number(sym, rc_ket)) > 2) continue;
+            CohRec rec{rb_bra, rb_ket, CohortAcc(phys, mpo.col_dim), {}, 0};
+            CohortAcc& coh = rec.acc;
+            for (size_t s = 0; s < phys.size(); ++s) {
+                Charge phys_out = phys.charges[s];
+                Charge lc_bra = rc_bra - phys_out;
+                size_t lb_bra = bra.left.position(lc_bra);
+                if (lb_bra == bra.left.size()) continue;
+                uint32_t ls_bra = bra.left.sizes[lb_bra];
+                uint64_t bra_offset = bra_lpb(phys_out, lc_bra);
+                coh.add_unit(s, phys.sizes[s], ls_bra, bra_offset);
+                for (uint32_t b2 = 0; b2 < mpo.col_dim; ++b2) {
+                    if (symmetric && mpo.right_herm.skip(b2, rc_bra, rc_ket)) continue;
+                    int Ap = mpo.right_spin[b2];
+                    if (su2 && !triangle(spin_of(rc_ket), Ap, spin_of(rc_bra))) continue;
+                    for (auto const& cell : mpo.cols[b2]) {
+                        uint32_t b1 = cell.other;
+                        for (uint32_t t = cell.t0; t < cell.t1; ++t) {
+                            SiteOp const& W = mpo.ops[mpo.terms[t].op];
+                            int K = W.spin, A = mpo.left_spin[b1];
+                            for (auto const& blk : W.blocks) {
+                                if (phys_out != blk.rc) continue;
+                                Charge phys_in = blk.lc;
+                                Charge lc_ket = rc_ket - phys_in;
+                                size_t lb_ket = left_i.position(lc_ket);
+                                if (lb_ket == left_i.size()) continue;
+                                uint32_t ci = L.cohort_index(lc_bra, lc_ket);
+                                if (!L.has_block(ci, b1)) continue;
+                                uint32_t ls_ket = left_i.sizes[lb_ket];
+                                uint64_t ket_offset = ket_rpb(phys_in, rc_ket);
+                                int64_t left_offset = L.offset(ci, b1);
+                                double cpl[4];
+                                double scale = L.conjugate_scale(ci, b1) * mpo.terms[t].scale;
+                                if (su2)
+                                    set_coupling(spin_of(lc_ket), std::abs(spin_of(rc_ket - lc_ket)), spin_of(rc_ket), A,
+                                                 K, Ap, spin_of(lc_bra), std::abs(spin_of(rc_bra - lc_bra)),
+                                                 spin_of(rc_bra), scale, cpl);
+                                else
+                                    cpl[0] = cpl[1] = cpl[2] = cpl[3] = scale;
+                                op_iterate(blk, cpl, coh, s, rs_ket, tl, ket_offset, ci, uint32_t(left_offset / ls_ket));
+                            }
+                        }
+                    }
+                    coh.add_line(b2);
+                }
+            }
+            if (!coh.n_terms) continue;
+            // finalize: compact the MPO indices with terms (Cohort::compute_mpo_offsets, tasks.cpp:507-526)
+            uint64_t bs = uint64_t(bra.right.sizes[rb_bra]) * rs_ket;
+            rec.offs.assign(mpo.col_dim, -1);
+            std::vector<uint32_t> pos(mpo.col_dim, 0);
+            uint32_t cnt = 0;
+            for (uint32_t b = 0; b < mpo.col_dim; ++b)
+                if (coh.bcount[b]) { pos[b] = cnt; rec.offs[b] = int64_t(bs * cnt); ++cnt; }
+            rec.nrows = cnt;
+            for (auto& u : coh.units)
+                for (auto& b : u.line_b) b = pos[b];
+            cohorts.push_back(std::move(rec));
+        }
+    }
+
+    // new boundary index (move_boundary.hpp:202-208)
+    P.out = BoundaryIndex(bra.right, ket.right, mpo.col_dim);
+    std::vector<uint32_t> new_ci(cohorts.size());
+    for (size_t c = 0; c < cohorts.size(); ++c)
+        new_ci[c] = P.out.add_cohort(cohorts[c].rb_bra, cohorts[c].rb_ket, cohorts[c].offs);
+    if (symmetric) P.out.complement_transpose(sym, mpo.right_herm, true);
+    P.out_elems = P.out.layout();
+
+    // T buffer + T-GEMMs: T^b (ls_bra x rs_ket) = L_b (or L_b^T) * psi_ket slice (create_T_left, tasks.cpp:545-648)
+    for (uint32_t rb_ket = 0; rb_ket < ket.right.size(); ++rb_ket) {
+        uint32_t rs_ket = ket.right.sizes[rb_ket];
+        for (auto& t : tlists[rb_ket].tasks) {
+            auto lr = L.clr[t.ci];
+            uint32_t bls = L.bra.sizes[lr.first], brs = L.ket.sizes[lr.second];
+            uint32_t nb = L.nblk[t.ci_eff];
+            uint64_t flops = 2ull * bls * brs * rs_ket * nb;
+            P.flops.flops_total += flops;
+            P.flops.flops_gemm += flops;
+            if (!t.used) continue;
+            require(brs == left_i.sizes[t.lb_ket], "lstep: boundary ket sector size != MPS left sector size");
+            t.rows = bls;
+            t.size = uint64_t(bls) * rs_ket * nb;
+            t.t_off = P.t_elems;
+            P.t_elems += round_up(t.size, 32);
+            int64_t b = int64_t(ket.rp_off[t.lb_ket] + t.psi_col * brs);
+            if (t.ci == t.ci_eff) {
+                check_cohort_shape(L, t.ci_eff, bls, brs, "lstep T");
+                P.t_gemms.push_back(gemm(bls, rs_ket, brs, BASE_BOUNDARY, int64_t(L.data_off[t.ci_eff]), bls, BASE_KET, b,
+                                         brs, BASE_T, int64_t(t.t_off), bls, 0, int32_t(nb), int64_t(bls) * brs, 0,
+                                         int64_t(bls) * rs_ket));
+            } else {
+                check_cohort_shape(L, t.ci_eff, brs, bls, "lstep T");
+                P.t_gemms.push_back(gemm(bls, rs_ket, brs, BASE_BOUNDARY, int64_t(L.data_off[t.ci_eff]), brs, BASE_KET, b,
+                                         brs, BASE_T, int64_t(t.t_off), bls, GEMM_TRANS_A, int32_t(nb),
+                                         int64_t(bls) * brs, 0, int64_t(bls) * rs_ket));
+            }
+            P.flops.flops_gemm_exec += flops;
+        }
+    }
+
+    // S buffers and final GEMMs: L'(rs_bra x nS*rs_ket) = A_bra^T (stripe x rs_bra)^T * S (prop_l, tasks.cpp:198-241)
+    for (size_t c = 0; c < cohorts.size(); ++c) {
+        auto& rec = cohorts[c];
+        uint32_t rs_ket = ket.right.sizes[rec.rb_ket], rs_bra = bra.right.sizes[rec.rb_bra];
+        uint64_t stripe = rec.acc.stripe;
+        require(stripe == bra.lp_rows[rec.rb_bra], "lstep: cohort stripe != bra left-paired rows");
+        uint64_t sbase = P.s_elems;
+        P.s_elems += round_up(stripe * rec.nrows * rs_ket, 32);
+        for (auto& u : rec.acc.units) {
+            uint32_t seek = 0;
+            for (size_t l = 0; l < u.line_count.size(); ++l) {
+                SLine line{};
+                line.s_off = int64_t(sbase + u.offset + uint64_t(u.line_b[l]) * rs_ket * stripe);
+                line.lds = int32_t(stripe);
+                line.rows = int32_t(u.ms);
+                line.cols = int32_t(rs_ket);
+                line.t0 = int32_t(P.terms.size());
+                for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
+                    TTask const& t = tlists[rec.rb_ket].tasks[u.ti[k]];
+                    check_tile(t, uint64_t(u.col[k]) * rs_ket, u.ms, rs_ket);
+                    P.terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * rs_ket)});
+                }
+                line.t1 = int32_t(P.terms.size());
+                seek += u.line_count[l];
+                P.lines.push_back(line);
+            }
+            P.flops.flops_total += 2ull * rs_ket * u.ms * u.alpha.size();
+        }
+        uint32_t ci_new = new_ci[c];
+        P.c_gemms.push_back(gemm(int32_t(rs_bra), int32_t(rec.nrows * rs_ket), int32_t(stripe), BASE_BRA,
+                                 int64_t(bra.lp_off[rec.rb_bra]), int32_t(stripe), BASE_S, int64_t(sbase), int32_t(stripe),
+                                 BASE_NEW, int64_t(P.out.data_off[ci_new]), int32_t(rs_bra), GEMM_TRANS_A));
+        uint64_t g = 2ull * rs_bra * rec.nrows * rs_ket * stripe;
+        P.flops.flops_total += g;
+        P.flops.flops_gemm += g;
+        P.flops.flops_gemm_exec += g;
+        P.flops.n_terms += rec.acc.n_terms;
+    }
+    return P;
+}
+
+// ----------------------------------------------------------------------------------------------
+// right growth (reference: overlap_mpo_right_step, move_boundary.hpp:281-404, with
+//                rshtm_t_tasks / rshtm_tasks, right_tasks.hpp:39-167)
+// ----------------------------------------------------------------------------------------------
+GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, MPSLayout const& bra,
+                              MPSLayout const& ket, bool symmetric) {
+    GrowPlanHost P;
+    require(bra.phys == ket.phys, "rstep: bra and ket physical indices differ");
+    require(R.aux_dim == mpo.col_dim, "rstep: boundary aux dim != MPO col dim");
+    validate_boundary_offsets(R);
+    Index const& phys = ket.phys;
+    ProductBasis ket_rpb = ProductBasis::right_paired(phys, ket.right);
+    ProductBasis bra_rpb = ProductBasis::right_paired(phys, bra.right);
+    const bool su2 = sym == Sym::SU2U1;
+
+    struct CohRec {
+        uint32_t lb_ket, lb_bra;
+        CohortAcc acc;
+        std::vector<int64_t> offs;
+        uint32_t nrows;
+    };
+    std::vector<CohRec> cohorts;
+    std::vector<TList> tlists(ket.left.size());
+
+    for (uint32_t lb_ket = 0; lb_ket < ket.left.size(); ++lb_ket) {
+        Charge lc_ket = ket.left.charges[lb_ket];
+        uint32_t ls_ket = ket.left.sizes[lb_ket];
+        TList& tl = tlists[lb_ket];
+        for (size_t s = 0; s < phys.size(); ++s) {
+            Charge rc_ket = lc_ket + phys.charges[s];
+            size_t rb_ket = ket.right.position(rc_ket);
+            if (rb_ket == ket.right.size()) continue;
+            uint32_t rs_ket = ket.right.sizes[rb_ket];
+            uint64_t ket_offset = ket_rpb(phys.charges[s], rc_ket);
+            for (size_t rb_bra = 0; rb_bra < bra.right.size(); ++rb_bra) {
+                uint32_t ci = R.cohort_index(rc_ket, bra.right.charges[rb_bra]);
+                if (ci == R.n_cohorts()) continue;
+                uint32_t ci_eff = R.tr[ci] ? R.cohort_index(bra.right.charges[rb_bra], rc_ket) : ci;
+                require(ci_eff < R.n_cohorts(), "rstep: transposed cohort without partner");
+                for (uint32_t ss = 0; ss < phys.sizes[s]; ++ss) tl.add(ket_offset + ss * rs_ket, ci, ci_eff, lb_ket);
+            }
+        }
+        for (uint32_t lb_bra = 0; lb_bra < bra.left.size(); ++lb_bra) {
+            Charge lc_bra = bra.left.charges[lb_bra];
+            if (std::abs(particle_number(sym, lc_ket) - particle_number(sym, lc_bra)) > 2) continue;
+            CohRec rec{lb_ket, lb_bra, CohortAcc(phys, mpo.row_dim), {}, 0};
+            CohortAcc& coh = rec.acc;
+            for (size_t s = 0; s < phys.size(); ++s) {
+                Charge phys_out = phys.charges[s];
+                Charge rc_bra = lc_bra + phys_out;
+                size_t rb_bra = bra.right.position(rc_bra);
+                if (rb_bra == bra.right.size()) continue;
+                uint32_t rs_bra = bra.right.sizes[rb_bra];
+                uint64_t bra_offset = bra_rpb(phys_out, rc_bra);
+                coh.add_unit(s, phys.sizes[s], rs_bra, bra_offset);
+                std::unique_ptr<Wigner9jCache> w9j;
+                if (su2) w9j.reset(new Wigner9jCache(spin_of(lc_ket), spin_of(lc_bra), spin_of(rc_bra)));
+                for (uint32_t b1 = 0; b1 < mpo.row_dim; ++b1) {
+                    if (symmetric && mpo.left_herm.skip(b1, lc_ket, lc_bra)) continue;
+                    int A = mpo.left_spin[b1];
+                    if (su2 && !triangle(spin_of(lc_ket), A, spin_of(lc_bra))) continue;
+                    for (auto const& cell : mpo.rows[b1]) {
+                        uint32_t b2 = cell.other;
+                        for (uint32_t t = cell.t0; t < cell.t1; ++t) {
+                            SiteOp const& W = mpo.ops[mpo.terms[t].op];
+                            int K = W.spin, Ap = mpo.right_spin[b2];
+                            for (auto const& blk : W.blocks) {
+                                if (phys_out != blk.rc) continue;
+                                Charge phys_in = blk.lc;
+                                Charge rc_ket = lc_ket + phys_in;
+                                uint32_t ci = R.cohort_index(rc_ket, rc_bra);
+                                if (!R.has_block(ci, b2)) continue;
+                                size_t rb_ket = ket.right.position(rc_ket);
+                                if (rb_ket == ket.right.size()) continue;
+                                uint32_t rs_ket = ket.right.sizes[rb_ket];
+                                uint64_t ket_offset = ket_rpb(phys_in, rc_ket);
+                                int64_t right_offset = R.offset(ci, b2);
+                                double cpl[4];
+                                double scale = R.conjugate_scale(ci, b2) * mpo.terms[t].scale;
+                                if (su2)
+                                    w9j->set_scale(A, K, Ap, spin_of(rc_ket), scale, cpl);
+                                else
+                                    cpl[0] = cpl[1] = cpl[2] = cpl[3] = scale;
+                                op_iterate(blk, cpl, coh, s, rs_ket, tl, ket_offset, ci, uint32_t(right_offset / rs_ket));
+                            }
+                        }
+                    }
+                    coh.add_line(b1);
+                }
+            }
+            if (!coh.n_terms) continue;
+            uint64_t bs = uint64_t(ls_ket) * bra.left.sizes[lb_bra];
+            rec.offs.assign(mpo.row_dim, -1);
+            std::vector<uint32_t> pos(mpo.row_dim, 0);
+            uint32_t cnt = 0;
+            for (uint32_t b = 0; b < mpo.row_dim; ++b)
+                if (coh.bcount[b]) { pos[b] = cnt; rec.offs[b] = int64_t(bs * cnt); ++cnt; }
+            rec.nrows = cnt;
+            for (auto& u : coh.units)
+                for (auto& b : u.line_b) b = pos[b];
+            cohorts.push_back(std::move(rec));
+        }
+    }
+
+    // new boundary index (move_boundary.hpp:332-338): complement_transpose is always applied
+    P.out = BoundaryIndex(ket.left, bra.left, mpo.row_dim);
+    std::vector<uint32_t> new_ci(cohorts.size());
+    for (size_t c = 0; c < cohorts.size(); ++c)
+        new_ci[c] = P.out.add_cohort(cohorts[c].lb_ket, cohorts[c].lb_bra, cohorts[c].offs);
+    P.out.complement_transpose(sym, mpo.left_herm, false);
+    P.out_elems = P.out.layout();
+
+    for (uint32_t lb_ket = 0; lb_ket < ket.left.size(); ++lb_ket) {
+        uint32_t ls_ket = ket.left.sizes[lb_ket];
+        for (auto& t : tlists[lb_ket].tasks) {
+            uint64_t flops = 2ull * ls_ket * R.cohort_size(t.ci_eff);
+            P.flops.flops_total += flops;
+            P.flops.flops_gemm += flops;
+            if (!t.used) continue;
+            auto lr = R.clr[t.ci];
+            uint32_t rs_ket = R.bra.sizes[lr.first], rs_bra = R.ket.sizes[lr.second];
+            t.rows = ls_ket;
+            t.size = uint64_t(ls_ket) * R.nblk[t.ci_eff] * rs_bra;
+            t.t_off = P.t_elems;
+            P.t_elems += round_up(t.size, 32);
+            require(ket.right.position(R.bra.charges[lr.first]) < ket.right.size() &&
+                        ket.right.sizes[ket.right.position(R.bra.charges[lr.first])] == rs_ket,
+                    "rstep: right boundary index does not match the ket right index");
+            right_t_gemm(R, t, ls_ket, rs_ket, rs_bra, int64_t(ket.rp_off[lb_ket]), BASE_KET, P.t_gemms);
+            P.flops.flops_gemm_exec += flops;
+        }
+    }
+
+    // S (nS*ls_ket x stripe) and R'_b = S_b * B_bra^T per MPO row (prop_r, tasks.cpp:243-284)
+    for (size_t c = 0; c < cohorts.size(); ++c) {
+        auto& rec = cohorts[c];
+        uint32_t ls_ket = ket.left.sizes[rec.lb_ket], ls_bra = bra.left.sizes[rec.lb_bra];
+        uint64_t stripe = rec.acc.stripe;
+        require(stripe == bra.rp_cols[rec.lb_bra], "rstep: cohort stripe != bra right-paired columns");
+        uint64_t lds = uint64_t(rec.nrows) * ls_ket;
+        uint64_t sbase = P.s_elems;
+        P.s_elems += round_up(lds * stripe, 32);
+        for (auto& u : rec.acc.units) {
+            uint32_t seek = 0;
+            for (size_t l = 0; l < u.line_count.size(); ++l) {
+                SLine line{};
+                line.s_off = int64_t(sbase + uint64_t(u.line_b[l]) * ls_ket + uint64_t(u.offset) * lds);
+                line.lds = int32_t(lds);
+                line.rows = int32_t(ls_ket);
+                line.cols = int32_t(u.ms);
+                line.t0 = int32_t(P.terms.size());
+                for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
+                    TTask const& t = tlists[rec.lb_ket].tasks[u.ti[k]];
+                    check_tile(t, uint64_t(u.col[k]) * ls_ket, ls_ket, u.ms);
+                    P.terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * ls_ket)});
+                }
+                line.t1 = int32_t(P.terms.size());
+                seek += u.line_count[l];
+                P.lines.push_back(line);
+            }
+            P.flops.flops_total += 2ull * ls_ket * u.ms * u.alpha.size();
+        }
+        uint32_t ci_new = new_ci[c];
+        P.c_gemms.push_back(gemm(int32_t(ls_ket), int32_t(ls_bra), int32_t(stripe), BASE_S, int64_t(sbase), int32_t(lds),
+                                 BASE_BRA, int64_t(bra.rp_off[rec.lb_bra]), int32_t(ls_bra), BASE_NEW,
+                                 int64_t(P.out.data_off[ci_new]), int32_t(ls_ket), GEMM_TRANS_B, int32_t(rec.nrows),
+                                 int64_t(ls_ket), 0, int64_t(ls_ket) * ls_bra));
+        uint64_t g = 2ull * rec.nrows * ls_ket * stripe * ls_bra;
+        P.flops.flops_total += g;
+        P.flops.flops_gemm += g;
+        P.flops.flops_gemm_exec += g;
+        P.flops.n_terms += rec.acc.n_terms;
+    }
+    return P;
+}
+
+// ---- host-side bounds validation of every device table (a table entry that would leave its
+//      buffer is rejected before anything is launched) -----------------------------------------
+void check_gemms(std::vector<GemmDesc> const& v, const uint64_t* sizes) {
+    auto span = [](int64_t off, int64_t stride, int32_t batch, int64_t rows, int64_t cols, int64_t ld) {
+        // [lo, hi] accessed element offsets of a rows x cols column-major operand
+        int64_t last = off + int64_t(batch - 1) * stride + (rows - 1) + (cols - 1) * ld;
+        return std::make_pair(off, last);
+    };
+    for (auto const& g : v) {
+        require(g.M >= 0 && g.N >= 0 && g.K >= 0 && g.batch >= 1, "gemm table: negative dimension");
+        require(g.sa >= 0 && g.sb >= 0 && g.sc >= 0, "gemm table: negative stride");
+        for (int b : {g.base_a, g.base_b, g.base_c}) require(b >= 0 && b < BASE_COUNT, "gemm table: bad base");
+        if (g.M == 0 || g.N == 0) continue;
+        bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B;
+        if (g.K > 0) {
+            int64_t ar = ta ? g.K : g.M, ac = ta ? g.M : g.K;
+            require(g.lda >= ar, "gemm table: lda too small");
+            auto sa = span(g.a, g.sa, g.batch, ar, ac, g.lda);
+            require(sa.first >= 0 && uint64_t(sa.second) < sizes[g.base_a], "gemm table: A outside its buffer");
+            int64_t br = tb ? g.N : g.K, bc = tb ? g.K : g.N;
+            require(g.ldb >= br, "gemm table: ldb too small");
+            auto sb = span(g.b, g.sb, g.batch, br, bc, g.ldb);
+            require(sb.first >= 0 && uint64_t(sb.second) < sizes[g.base_b], "gemm table: B outside its buffer");
+        }
+        require(g.ldc >= g.M, "gemm table: ldc too small");
+        auto sc = span(g.c, g.sc, g.batch, g.M, g.N, g.ldc);
+        require(sc.first >= 0 && uint64_t(sc.second) < sizes[g.base_c], "gemm table: C outside its buffer");
+    }
+}
+
+void check_lines(std::vector<SLine> const& lines, std::vector<STerm> const& terms, uint64_t t_size, uint64_t s_size) {
+    for (auto const& l : lines) {
+        require(l.rows >= 0 && l.cols >= 0 && l.lds >= l.rows && l.t0 >= 0 && l.t1 >= l.t0 &&
+                    uint64_t(l.t1) <= terms.size(), "S table: malformed line");
+        if (!l.rows || !l.cols) continue;
+        int64_t last = l.s_off + (l.rows - 1) + int64_t(l.cols - 1) * l.lds;
+        require(l.s_off >= 0 && uint64_t(last) < s_size, "S table: line outside the S buffer");
+        for (int t = l.t0; t < l.t1; ++t)
+            require(terms[t].t_off >= 0 && uint64_t(terms[t].t_off) + uint64_t(l.rows) * l.cols <= t_size,
+                    "S table: term outside the T buffer");
+    }
+}
+
+
+}  // namespace mq

@@ -1,0 +1,213 @@
+"""Python binding of the contraction engine (mirrors contraction::Engine / solve_site_problem).
+
+Reference entry points (dmrg/framework/dmrg/):
+  Engine::overlap_mpo_left_step / overlap_mpo_right_step   mp_tensors/contractions/engine.hpp:53-76
+  Engine::contraction_schedule + super_hamil_mv             engine.hpp:117-126, solver/super_hamil_mv.hpp
+  solve_site_problem (Jacobi-Davidson)                      optimize/solver_interface.hpp:30-77
+
+Every call goes through libqcmaquis_amd.so; errors raise :class:`EngineError` with the library's
+message (the reference throws std::runtime_error or exits).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .model import BoundarySpec, Index
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One device, one stream (mq_ctx_create)."""
+
+    def __init__(self, device=0, symmetry=abi.MQ_SYM_TWOU1):
+        self.lib = abi.lib()
+        h = C.c_void_p()
+        rc = self.lib.mq_ctx_create(int(device), int(symmetry), C.byref(h))
+        if rc != 0:
+            raise EngineError(rc, "mq_ctx_create failed (is a GPU visible?)")
+        self.h = h
+        self.symmetry = symmetry
+
+    def check(self, rc):
+        if rc != 0:
+            raise EngineError(rc, self.lib.mq_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(self.lib.mq_ctx_sync(self.h))
+
+    @property
+    def stream(self):
+        return self.lib.mq_ctx_stream(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.mq_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+
+    def close(self):
+        if self.h and self.ctx.h:
+            getattr(self.ctx.lib, self._destroy)(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceMPO(_Handle):
+    _destroy = "mq_mpo_destroy"
+
+    def __init__(self, ctx, spec):
+        d = spec.desc()
+        h = C.c_void_p()
+        ctx.check(ctx.lib.mq_mpo_create(ctx.h, C.byref(d), C.byref(h)))
+        super().__init__(ctx, h)
+        self.spec = spec
+
+
+class DeviceBoundary(_Handle):
+    _destroy = "mq_boundary_destroy"
+
+    @classmethod
+    def upload(cls, ctx, spec):
+        d = spec.desc()
+        h = C.c_void_p()
+        data = np.ascontiguousarray(spec.data, np.float64)
+        ctx.check(ctx.lib.mq_boundary_upload(ctx.h, C.byref(d), _dp(data), C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def initial(cls, ctx, index, aux_dim=1):
+        h = C.c_void_p()
+        d = index.desc()
+        ctx.check(ctx.lib.mq_boundary_create_initial(ctx.h, C.byref(d), int(aux_dim), C.byref(h)))
+        return cls(ctx, h)
+
+    def info(self):
+        i = abi.BoundaryInfo()
+        self.ctx.check(self.ctx.lib.mq_boundary_info(self.h, C.byref(i)))
+        return i
+
+    def download(self):
+        """BoundarySpec with the index and data of this boundary."""
+        i = self.info()
+        nb, nk, nc, aux = i.n_bra, i.n_ket, i.n_cohorts, i.aux_dim
+        bch = np.zeros(2 * nb, np.int32); bsz = np.zeros(nb, np.uint32)
+        kch = np.zeros(2 * nk, np.int32); ksz = np.zeros(nk, np.uint32)
+        lr = np.zeros(2 * nc, np.uint32); off = np.zeros(nc * aux, np.int64)
+        cs = np.zeros(nc * aux, np.float64); tp = np.zeros(nc * aux, np.uint8); tr = np.zeros(nc, np.uint8)
+        P = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+        self.ctx.check(self.ctx.lib.mq_boundary_export_index(
+            self.h, P(bch, C.c_int32), P(bsz, C.c_uint32), P(kch, C.c_int32), P(ksz, C.c_uint32), P(lr, C.c_uint32),
+            P(off, C.c_int64), P(cs, C.c_double), P(tp, C.c_uint8), P(tr, C.c_uint8)))
+        data = np.zeros(i.total_elements)
+        self.ctx.check(self.ctx.lib.mq_boundary_download(self.h, _dp(data)))
+        bra = Index(bch.reshape(-1, 2).tolist(), bsz.tolist())
+        ket = Index(kch.reshape(-1, 2).tolist(), ksz.tolist())
+        cohorts = [dict(lr=(int(lr[2 * c]), int(lr[2 * c + 1])), offsets=off[c * aux:(c + 1) * aux].tolist(),
+                        conj_scales=cs[c * aux:(c + 1) * aux].tolist(), transposes=tp[c * aux:(c + 1) * aux].tolist(),
+                        tr=int(tr[c])) for c in range(nc)]
+        return BoundarySpec(bra, ket, aux, cohorts, data)
+
+
+class DeviceMPS(_Handle):
+    _destroy = "mq_mps_destroy"
+
+    def __init__(self, ctx, layout, data, pairing=abi.MQ_RIGHT_PAIRED):
+        data = np.ascontiguousarray(data, np.float64)
+        if data.size != layout.n:
+            raise ValueError("MPS data size does not match its layout")
+        h = C.c_void_p()
+        l, p, r = layout.left.desc(), layout.phys.desc(), layout.right.desc()
+        ctx.check(ctx.lib.mq_mps_upload(ctx.h, C.byref(l), C.byref(p), C.byref(r), int(pairing), _dp(data), C.byref(h)))
+        super().__init__(ctx, h)
+        self.layout = layout
+
+    def download(self, pairing=abi.MQ_RIGHT_PAIRED):
+        out = np.zeros(self.layout.n)
+        self.ctx.check(self.ctx.lib.mq_mps_download(self.h, int(pairing), _dp(out)))
+        return out
+
+
+def lstep(ctx, mpo, left, bra, ket, symmetric=True, precision=abi.MQ_PREC_FP64):
+    """overlap_mpo_left_step (move_boundary.hpp:147-279)"""
+    h = C.c_void_p()
+    ctx.check(ctx.lib.mq_lstep(ctx.h, mpo.h, left.h, bra.h, ket.h, int(bool(symmetric)), int(precision), C.byref(h)))
+    return DeviceBoundary(ctx, h)
+
+
+def rstep(ctx, mpo, right, bra, ket, symmetric=True, precision=abi.MQ_PREC_FP64):
+    """overlap_mpo_right_step (move_boundary.hpp:281-404)"""
+    h = C.c_void_p()
+    ctx.check(ctx.lib.mq_rstep(ctx.h, mpo.h, right.h, bra.h, ket.h, int(bool(symmetric)), int(precision), C.byref(h)))
+    return DeviceBoundary(ctx, h)
+
+
+class HpsiPlan(_Handle):
+    """create_contraction_schedule + the device work lists of one site (mq_hpsi_plan_create)."""
+
+    _destroy = "mq_hpsi_plan_destroy"
+
+    def __init__(self, ctx, mpo, left, right, layout):
+        h = C.c_void_p()
+        l, p, r = layout.left.desc(), layout.phys.desc(), layout.right.desc()
+        ctx.check(ctx.lib.mq_hpsi_plan_create(ctx.h, mpo.h, left.h, right.h, C.byref(l), C.byref(p), C.byref(r),
+                                              C.byref(h)))
+        super().__init__(ctx, h)
+        self.layout = layout
+        self._refs = (mpo, left, right)
+
+    def stats(self):
+        s = abi.HpsiStats()
+        self.ctx.check(self.ctx.lib.mq_hpsi_plan_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in abi.HpsiStats._fields_}
+
+    def apply(self, psi, precision=abi.MQ_PREC_FP64):
+        """H*psi for a host vector (right-paired blocks concatenated)."""
+        psi = np.ascontiguousarray(psi, np.float64)
+        if psi.size != self.layout.n:
+            raise ValueError("psi size does not match the plan")
+        out = np.zeros_like(psi)
+        self.ctx.check(self.ctx.lib.mq_hpsi_host(self.h, _dp(psi), _dp(out), int(precision)))
+        return out
+
+    def apply_device(self, in_ptr, out_ptr, precision=abi.MQ_PREC_FP64):
+        """Stream-ordered H*psi on device pointers (context stream)."""
+        self.ctx.check(self.ctx.lib.mq_hpsi(self.h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), int(precision)))
+
+    def davidson(self, psi0, ortho=(), tol=1e-8, max_iter=8, precision=abi.MQ_PREC_FP64):
+        """Jacobi-Davidson lowest eigenpair (solve<double>, solver/solver.cpp:13-28)."""
+        psi0 = np.ascontiguousarray(psi0, np.float64)
+        out = np.zeros_like(psi0)
+        ev = C.c_double()
+        st = abi.DavidsonStats()
+        orth = [np.ascontiguousarray(o, np.float64) for o in ortho]
+        arr = (C.POINTER(C.c_double) * max(1, len(orth)))(*[_dp(o) for o in orth])
+        self.ctx.check(self.ctx.lib.mq_davidson(self.h, _dp(psi0), len(orth), arr, float(tol), int(max_iter),
+                                                int(precision), C.byref(ev), _dp(out), C.byref(st)))
+        return ev.value, out, {f: getattr(st, f) for f, _ in abi.DavidsonStats._fields_}

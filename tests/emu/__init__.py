@@ -1,0 +1,95 @@
+"""ctypes binding of the host-side table emulator (plan_emulator.cpp), built on first use with g++
+from the product's host sources."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+CSRC = os.path.join(ROOT, "qcmaquis_amd", "csrc")
+LIB = os.path.join(HERE, "libplanemu.so")
+_lib = None
+
+
+def build():
+    srcs = [os.path.join(HERE, "plan_emulator.cpp")] + [os.path.join(CSRC, f) for f in
+                                                         ("plans.cpp", "coupling.cpp", "descriptors.cpp")]
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+                    "-o", LIB + ".tmp"] + srcs, check=True)
+    os.replace(LIB + ".tmp", LIB)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        from qcmaquis_amd import abi
+        L = C.CDLL(LIB)
+        P = C.POINTER
+        dp = P(C.c_double)
+        idx = P(abi.IndexDesc)
+        L.emu_hpsi.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, P(abi.BIndexDesc), dp, idx, idx, idx, dp, dp]
+        for f in (L.emu_lstep, L.emu_rstep):
+            f.restype = C.c_void_p
+            f.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, idx, idx, idx, dp, idx, idx, idx, dp, C.c_int]
+        L.emu_boundary_info.argtypes = [C.c_void_p, P(C.c_int), P(C.c_uint), P(C.c_uint64)]
+        L.emu_boundary_export.argtypes = [C.c_void_p, P(C.c_uint32), P(C.c_int64), dp, P(C.c_uint8), P(C.c_uint8), dp]
+        L.emu_boundary_free.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def hpsi(sym, mpo, L, R, layout, psi):
+    psi = np.ascontiguousarray(psi, np.float64)
+    out = np.zeros_like(psi)
+    md, Ld, Rd = mpo.desc(), L.desc(), R.desc()
+    li, pi, ri = layout.left.desc(), layout.phys.desc(), layout.right.desc()
+    Ldat, Rdat = np.ascontiguousarray(L.data), np.ascontiguousarray(R.data)
+    rc = lib().emu_hpsi(sym, C.byref(md), C.byref(Ld), _dp(Ldat), C.byref(Rd), _dp(Rdat), C.byref(li), C.byref(pi),
+                        C.byref(ri), _dp(psi), _dp(out))
+    if rc != 0:
+        raise RuntimeError("emu_hpsi failed")
+    return out
+
+
+def _grow(fn, sym, mpo, B, bra_layout, bra_rp, ket_layout, ket_rp, symmetric, bra_idx, ket_idx):
+    from qcmaquis_amd.model import BoundarySpec
+    md, Bd = mpo.desc(), B.desc()
+    d = [x.desc() for x in (bra_layout.left, bra_layout.phys, bra_layout.right, ket_layout.left, ket_layout.phys,
+                            ket_layout.right)]
+    Bdat = np.ascontiguousarray(B.data)
+    bra_rp = np.ascontiguousarray(bra_rp, np.float64)
+    ket_rp = np.ascontiguousarray(ket_rp, np.float64)
+    h = fn(sym, C.byref(md), C.byref(Bd), _dp(Bdat), C.byref(d[0]), C.byref(d[1]), C.byref(d[2]), _dp(bra_rp),
+           C.byref(d[3]), C.byref(d[4]), C.byref(d[5]), _dp(ket_rp), int(bool(symmetric)))
+    if not h:
+        raise RuntimeError("emulated growth step failed")
+    L = lib()
+    nc, aux, ne = C.c_int(), C.c_uint(), C.c_uint64()
+    L.emu_boundary_info(h, C.byref(nc), C.byref(aux), C.byref(ne))
+    n, a = nc.value, aux.value
+    lr = np.zeros(2 * n, np.uint32); off = np.zeros(n * a, np.int64); cs = np.zeros(n * a)
+    tp = np.zeros(n * a, np.uint8); tr = np.zeros(n, np.uint8); data = np.zeros(ne.value)
+    P = lambda x, t: x.ctypes.data_as(C.POINTER(t))
+    L.emu_boundary_export(h, P(lr, C.c_uint32), P(off, C.c_int64), _dp(cs), P(tp, C.c_uint8), P(tr, C.c_uint8), _dp(data))
+    L.emu_boundary_free(h)
+    cohorts = [dict(lr=(int(lr[2 * c]), int(lr[2 * c + 1])), offsets=off[c * a:(c + 1) * a].tolist(),
+                    conj_scales=cs[c * a:(c + 1) * a].tolist(), transposes=tp[c * a:(c + 1) * a].tolist(),
+                    tr=int(tr[c])) for c in range(n)]
+    return BoundarySpec(bra_idx, ket_idx, a, cohorts, data)
+
+
+def lstep(sym, mpo, L, bra_layout, bra_rp, ket_layout, ket_rp, symmetric=True):
+    return _grow(lib().emu_lstep, sym, mpo, L, bra_layout, bra_rp, ket_layout, ket_rp, symmetric, bra_layout.right,
+                 ket_layout.right)
+
+
+def rstep(sym, mpo, R, bra_layout, bra_rp, ket_layout, ket_rp, symmetric=True):
+    return _grow(lib().emu_rstep, sym, mpo, R, bra_layout, bra_rp, ket_layout, ket_rp, symmetric, ket_layout.left,
+                 bra_layout.left)

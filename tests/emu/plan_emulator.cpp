@@ -1,0 +1,208 @@
+// Host-side executor of the product's device tables -- test code.
+//
+// Builds the product's plans (qcmaquis_amd/csrc/plans.cpp) and runs their GEMM problems, S lines
+// and copy / reshape jobs on the CPU with the same index arithmetic as the gfx950 kernels, so the
+// plan builder and its bounds checks are tested without a GPU.  Not linked into the product.
+#include <cstdio>
+#include <memory>
+#include <vector>
+
+#include "descriptors.hpp"
+#include "plans.hpp"
+
+using namespace mq;
+
+namespace {
+
+void run_gemms(std::vector<GemmDesc> const& G, double* const* base) {
+    for (auto const& g : G) {
+        bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B;
+        for (int b = 0; b < g.batch; ++b) {
+            const double* A = base[g.base_a] + g.a + int64_t(b) * g.sa;
+            const double* B = base[g.base_b] + g.b + int64_t(b) * g.sb;
+            double* Cm = base[g.base_c] + g.c + int64_t(b) * g.sc;
+            for (int n = 0; n < g.N; ++n)
+                for (int m = 0; m < g.M; ++m) {
+                    double s = 0.0;
+                    for (int k = 0; k < g.K; ++k) {
+                        double a = ta ? A[int64_t(m) * g.lda + k] : A[m + int64_t(k) * g.lda];
+                        double v = tb ? B[n + int64_t(k) * g.ldb] : B[k + int64_t(n) * g.ldb];
+                        s += a * v;
+                    }
+                    Cm[m + int64_t(n) * g.ldc] = s;
+                }
+        }
+    }
+}
+
+void run_lines(std::vector<SLine> const& lines, std::vector<STerm> const& terms, const double* T, double* S) {
+    for (auto const& ln : lines) {
+        int n_el = ln.rows * ln.cols;
+        for (int idx = 0; idx < n_el; ++idx) {
+            double acc = 0.0;
+            for (int t = ln.t0; t < ln.t1; ++t) acc += terms[t].alpha * T[terms[t].t_off + idx];
+            S[ln.s_off + idx % ln.rows + int64_t(idx / ln.rows) * ln.lds] = acc;
+        }
+    }
+}
+
+void run_copy(std::vector<CopyJob> const& jobs, const double* src, double* dst) {
+    for (auto const& j : jobs) {
+        int64_t per = int64_t(j.rows) * j.cols;
+        for (int64_t i = 0; i < per * j.nblocks; ++i) {
+            int64_t b = i / per, e = i % per;
+            int r = int(e % j.rows), c = int(e / j.rows);
+            dst[j.dst + b * per + e] = j.trans ? src[j.src + b * per + c + int64_t(r) * j.cols] : src[j.src + b * per + e];
+        }
+    }
+}
+
+std::vector<double> to_left(MPSLayout const& L, const double* rp) {
+    std::vector<double> lp(L.elements(), 0.0);
+    for (auto const& s : reshape_segments(L)) {
+        int64_t total = int64_t(s.ls) * s.rs * s.d;
+        for (int64_t i = 0; i < total; ++i) {
+            int ll = int(i % s.ls);
+            int64_t q = i / s.ls;
+            int rr = int(q % s.rs), ss = int(q / s.rs);
+            lp[s.lp + int64_t(ss) * s.ls + ll + int64_t(rr) * s.lp_ld] = rp[s.rp + ll + q * s.ls];
+        }
+    }
+    return lp;
+}
+
+struct EmuBoundary {
+    BoundaryIndex idx;
+    std::vector<double> data;
+};
+
+}  // namespace
+
+extern "C" {
+
+int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const double* Ldata, const mq_bindex_desc* Rd,
+             const double* Rdata, const mq_index_desc* li, const mq_index_desc* pi, const mq_index_desc* ri,
+             const double* psi, double* out) {
+    try {
+        MPO mpo(*md);
+        BoundaryIndex L = boundary_index_from(*Ld), R = boundary_index_from(*Rd);
+        uint64_t Ln = L.layout(), Rn = R.layout();
+        HpsiPlanHost P = build_hpsi_plan(Sym(sym), mpo, L, R, Index(*li), Index(*pi), Index(*ri));
+        uint64_t n = P.mps.elements();
+        uint64_t sizes[BASE_COUNT] = {0};
+        sizes[BASE_PSI_IN] = sizes[BASE_PSI_OUT] = n;
+        sizes[BASE_T] = P.t_elems;
+        sizes[BASE_S] = P.s_elems;
+        sizes[BASE_LPANEL] = P.lpanel_elems;
+        sizes[BASE_BOUNDARY] = Rn;
+        check_gemms(P.t_gemms, sizes);
+        check_gemms(P.c_gemms, sizes);
+        check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
+        for (auto const& j : P.lpanel_jobs)
+            require(uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= Ln &&
+                    uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= P.lpanel_elems, "copy job out of range");
+        std::vector<double> T(P.t_elems + 1), S(P.s_elems + 1, 0.0), Lp(P.lpanel_elems + 1);
+        run_copy(P.lpanel_jobs, Ldata, Lp.data());
+        double* base[BASE_COUNT] = {nullptr};
+        base[BASE_PSI_IN] = const_cast<double*>(psi);
+        base[BASE_PSI_OUT] = out;
+        base[BASE_T] = T.data();
+        base[BASE_S] = S.data();
+        base[BASE_LPANEL] = Lp.data();
+        base[BASE_BOUNDARY] = const_cast<double*>(Rdata);
+        run_gemms(P.t_gemms, base);
+        run_lines(P.lines, P.terms, T.data(), S.data());
+        run_gemms(P.c_gemms, base);
+        return 0;
+    } catch (std::exception const& e) {
+        fprintf(stderr, "emu_hpsi: %s\n", e.what());
+        return -1;
+    }
+}
+
+static EmuBoundary* grow(bool left, int sym, const mq_mpo_desc* md, const mq_bindex_desc* Bd, const double* Bdata,
+                         const mq_index_desc* bl, const mq_index_desc* bp, const mq_index_desc* br, const double* bra_rp,
+                         const mq_index_desc* kl, const mq_index_desc* kp, const mq_index_desc* kr, const double* ket_rp,
+                         int symmetric) {
+    MPO mpo(*md);
+    BoundaryIndex B = boundary_index_from(*Bd);
+    uint64_t Bn = B.layout();
+    MPSLayout bra{Index(*bl), Index(*bp), Index(*br)}, ket{Index(*kl), Index(*kp), Index(*kr)};
+    GrowPlanHost P = left ? build_lstep_plan(Sym(sym), mpo, B, bra, ket, symmetric != 0)
+                          : build_rstep_plan(Sym(sym), mpo, B, bra, ket, symmetric != 0);
+    std::vector<double> bra_data = left ? to_left(bra, bra_rp) : std::vector<double>(bra_rp, bra_rp + bra.elements());
+    uint64_t sizes[BASE_COUNT] = {0};
+    sizes[BASE_T] = P.t_elems;
+    sizes[BASE_S] = P.s_elems;
+    sizes[BASE_BOUNDARY] = Bn;
+    sizes[BASE_BRA] = bra.elements();
+    sizes[BASE_KET] = ket.elements();
+    sizes[BASE_NEW] = P.out_elems;
+    check_gemms(P.t_gemms, sizes);
+    check_gemms(P.c_gemms, sizes);
+    check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
+    auto* res = new EmuBoundary{std::move(P.out), std::vector<double>(P.out_elems, 0.0)};
+    std::vector<double> T(P.t_elems + 1), S(P.s_elems + 1, 0.0);
+    double* base[BASE_COUNT] = {nullptr};
+    base[BASE_T] = T.data();
+    base[BASE_S] = S.data();
+    base[BASE_BOUNDARY] = const_cast<double*>(Bdata);
+    base[BASE_BRA] = bra_data.data();
+    base[BASE_KET] = const_cast<double*>(ket_rp);
+    base[BASE_NEW] = res->data.data();
+    run_gemms(P.t_gemms, base);
+    run_lines(P.lines, P.terms, T.data(), S.data());
+    run_gemms(P.c_gemms, base);
+    return res;
+}
+
+void* emu_lstep(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const double* Ldata, const mq_index_desc* bl,
+                const mq_index_desc* bp, const mq_index_desc* br, const double* bra_rp, const mq_index_desc* kl,
+                const mq_index_desc* kp, const mq_index_desc* kr, const double* ket_rp, int symmetric) {
+    try {
+        return grow(true, sym, md, Ld, Ldata, bl, bp, br, bra_rp, kl, kp, kr, ket_rp, symmetric);
+    } catch (std::exception const& e) {
+        fprintf(stderr, "emu_lstep: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+void* emu_rstep(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Rd, const double* Rdata, const mq_index_desc* bl,
+                const mq_index_desc* bp, const mq_index_desc* br, const double* bra_rp, const mq_index_desc* kl,
+                const mq_index_desc* kp, const mq_index_desc* kr, const double* ket_rp, int symmetric) {
+    try {
+        return grow(false, sym, md, Rd, Rdata, bl, bp, br, bra_rp, kl, kp, kr, ket_rp, symmetric);
+    } catch (std::exception const& e) {
+        fprintf(stderr, "emu_rstep: %s\n", e.what());
+        return nullptr;
+    }
+}
+
+int emu_boundary_info(void* h, int* n_cohorts, unsigned* aux, uint64_t* n_elems) {
+    auto* b = static_cast<EmuBoundary*>(h);
+    *n_cohorts = int(b->idx.n_cohorts());
+    *aux = b->idx.aux_dim;
+    *n_elems = b->data.size();
+    return 0;
+}
+
+int emu_boundary_export(void* h, uint32_t* lr, int64_t* offsets, double* cs, uint8_t* tp, uint8_t* tr, double* data) {
+    auto* b = static_cast<EmuBoundary*>(h);
+    BoundaryIndex const& B = b->idx;
+    for (uint32_t c = 0; c < B.n_cohorts(); ++c) {
+        lr[2 * c] = B.clr[c].first;
+        lr[2 * c + 1] = B.clr[c].second;
+        for (uint32_t k = 0; k < B.aux_dim; ++k) {
+            offsets[uint64_t(c) * B.aux_dim + k] = B.offsets[c][k];
+            cs[uint64_t(c) * B.aux_dim + k] = B.conj_scales[c][k];
+            tp[uint64_t(c) * B.aux_dim + k] = B.transposes[c][k];
+        }
+        tr[c] = B.tr[c];
+    }
+    std::copy(b->data.begin(), b->data.end(), data);
+    return 0;
+}
+
+void emu_boundary_free(void* h) { delete static_cast<EmuBoundary*>(h); }
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+"""Pin the CPU oracle (restated reference schedule + kernels) against the dense einsum definitions
+of the three contractions on small random TwoU1 chains, including Hermitian-compressed boundaries
+(complement_transpose) and both BLAS back-ends of the oracle."""
+import numpy as np
+import pytest
+
+from oracle import dense
+from qcmaquis_amd import abi
+from tests.chains import make_chain, oracle_left, oracle_right, rel_err
+
+SEEDS = [1, 2, 3, 4]
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_lstep_vs_dense(oracle_lib, seed, symmetric):
+    ch = make_chain(seed, n_sites=4, max_m=3, D=6)
+    L = oracle_left(oracle_lib, ch, 3, symmetric=symmetric)
+    for k in range(3):
+        new = L[k + 1]
+        ref = dense.lstep(ch.mpos[k], L[k], ch.layouts[k], ch.data[k], ch.layouts[k], ch.data[k])
+        n = 0
+        for b, rs, cs, blk in dense.computed_blocks(new):
+            assert rel_err(blk, ref[b, rs, cs]) < 1e-12
+            n += 1
+        assert n > 0
+        if not symmetric:
+            # every non-zero dense block must be present
+            assert np.allclose(dense.dense_boundary(new), ref, atol=1e-12 * max(1, np.abs(ref).max()))
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_rstep_vs_dense(oracle_lib, seed, symmetric):
+    ch = make_chain(seed, n_sites=4, max_m=3, D=6)
+    if not symmetric:
+        # the reference's right step always completes the index with complement_transpose
+        # (move_boundary.hpp:337), which needs skipped Hermitian partners: an MPO without
+        # registered partners is the only consistent input for symmetric=false
+        for m in ch.mpos:
+            m.left_herm = None
+            m.right_herm = None
+    R = oracle_right(oracle_lib, ch, 1, symmetric=symmetric)
+    for k in range(3, 0, -1):
+        new = R[k]
+        ref = dense.rstep(ch.mpos[k], R[k + 1], ch.layouts[k], ch.data[k], ch.layouts[k], ch.data[k])
+        n = 0
+        for b, rs, cs, blk in dense.computed_blocks(new):
+            assert rel_err(blk, ref[b, rs, cs]) < 1e-12
+            n += 1
+        assert n > 0
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+@pytest.mark.parametrize("site", [1, 2])
+@pytest.mark.parametrize("blas", [False, True])
+def test_hpsi_vs_dense(oracle_lib, seed, site, blas):
+    ch = make_chain(seed, n_sites=4, max_m=4, D=6)
+    L = oracle_left(oracle_lib, ch, site)[site]
+    R = oracle_right(oracle_lib, ch, site + 1)[site + 1]
+    if blas:
+        assert oracle_lib.use_blas(1) is not None
+    else:
+        oracle_lib.lib().oracle_set_blas(None, 0)
+    try:
+        lay = ch.layouts[site]
+        psi = np.random.default_rng(seed + 100).normal(size=lay.n)
+        out = oracle_lib.hpsi(ch.sym, ch.mpos[site], L, R, lay, psi)
+        ref, full = dense.hpsi(ch.mpos[site], L, R, lay, psi)
+        assert rel_err(out, ref) < 1e-12
+        # the dense result is confined to symmetry-allowed blocks (charge conservation of the chain)
+        assert np.isclose(np.linalg.norm(full), np.linalg.norm(ref), rtol=1e-12)
+    finally:
+        oracle_lib.lib().oracle_set_blas(None, 0)
+
+
+def test_hermitian_compression_consistent(oracle_lib):
+    """complement_transpose produces transposed cohorts whose offsets alias the partner cohort"""
+    ch = make_chain(7, n_sites=4, max_m=3, D=8)
+    L = oracle_left(oracle_lib, ch, 2, symmetric=True)[2]
+    trs = [c for c in L.cohorts if c["tr"]]
+    assert trs, "expected transposed cohorts"
+    for c in trs:
+        l, r = c["lr"]
+        j = L.cohort_index(L.ket.charges[r], L.bra.charges[l])
+        assert j < len(L.cohorts) and not L.cohorts[j]["tr"]
