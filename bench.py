@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark of the Davidson H*psi step (Boundary x MPOTensor x MPS) on one MI355X.
+
+One "step" = one H*psi of the site problem with psi, psi', boundaries and work lists resident in
+HBM (one Davidson iteration's operator application, reference super_hamil_mv).  The value is the
+reference's own flop model (ScheduleNew::total_flops, solver/tasks.cpp:389-398, 778-791) per
+second.  Also reported: the roofline of the dominant kernel (the grouped FP64 MFMA GEMM) from HIP
+events recorded on the launch stream, and the CPU oracle (restated reference CPU path, OpenMP +
+OpenBLAS) timed on the host cores on the same site problem.
+
+  python bench.py [--steps K] [--warmup W] [--workload synthetic] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"]
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def synthetic_workload(ctx, seed=7):
+    """Two-site TwoU1 site problem in the middle of a synthetic chain: MPS sector structure of an
+    8-site two-site chain (16 states / 9 sectors per site), sparse MPO rows (<= 6 cells per row,
+    bond dimension 256), boundaries grown on the device with the engine's growth steps."""
+    from qcmaquis_amd import abi
+    from qcmaquis_amd.engine import DeviceBoundary, DeviceMPO, DeviceMPS, lstep, rstep
+    from qcmaquis_amd.synthetic import TWOU1_SITE, Chain, fuse_index
+    rng = np.random.default_rng(seed)
+    two = fuse_index(TWOU1_SITE, TWOU1_SITE)
+    n, mid = 8, 4
+    ch = Chain(rng, abi.MQ_SYM_TWOU1, n, (8, 8), 40, 256, phys=two, density=0.02, row_cells=6, min_m=8)
+    mpos = [DeviceMPO(ctx, m) for m in ch.mpos]
+    mps = [DeviceMPS(ctx, l, d) for l, d in zip(ch.layouts, ch.data)]
+    L = DeviceBoundary.initial(ctx, ch.bonds[0], ch.mpos[0].row_dim)
+    for k in range(mid):
+        L = lstep(ctx, mpos[k], L, mps[k], mps[k])
+    R = DeviceBoundary.initial(ctx, ch.bonds[n], ch.mpos[-1].col_dim)
+    for k in range(n - 1, mid, -1):
+        R = rstep(ctx, mpos[k], R, mps[k], mps[k])
+    lay = ch.layouts[mid]
+    desc = {"workload": "synthetic two-site TwoU1 site problem (random sparse MPO, D=256, 8-site chain, "
+                        "sector dims 8-40)",
+            "symmetry": "TwoU1", "M_left": lay.left.total(), "M_right": lay.right.total(),
+            "sectors_left": len(lay.left), "sectors_right": len(lay.right),
+            "D_left": ch.mpos[mid].row_dim, "D_right": ch.mpos[mid].col_dim, "psi_elements": lay.n}
+    return dict(mpo=mpos[mid], mpo_spec=ch.mpos[mid], L=L, R=R, layout=lay, sym=abi.MQ_SYM_TWOU1, desc=desc)
+
+
+def cpu_baseline(w, flops, budget_s=30.0):
+    """the oracle (restated reference CPU path) on the same site problem, all host cores"""
+    import oracle
+    from qcmaquis_amd.model import MPSLayout  # noqa: F401
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+    blas = oracle.use_blas(1)
+    Ls, Rs = w["L"].download(), w["R"].download()
+    psi = np.random.default_rng(1).normal(size=w["layout"].n)
+    t0 = time.perf_counter()
+    oracle.hpsi(w["sym"], w["mpo_spec"], Ls, Rs, w["layout"], psi, nthreads=cores)
+    dt = time.perf_counter() - t0
+    reps = 1
+    while dt * (reps + 1) < budget_s and reps < 5:
+        t1 = time.perf_counter()
+        oracle.hpsi(w["sym"], w["mpo_spec"], Ls, Rs, w["layout"], psi, nthreads=cores)
+        dt += time.perf_counter() - t1
+        reps += 1
+    per = dt / reps
+    return {"value": flops / per / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+            "sample": f"{reps} full H*psi of the benchmark site problem ({per:.3f} s each, schedule build excluded; "
+                      f"OpenMP dynamic over MPS blocks, dgemm from {'OpenBLAS' if blas else 'a triple loop'})",
+            "seconds_per_hpsi": per}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="synthetic")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the site problem on a single MI355X "
+              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from qcmaquis_amd import abi
+    from qcmaquis_amd.engine import Context, HpsiPlan
+
+    torch.cuda.init()
+    ctx = Context(0, abi.MQ_SYM_TWOU1)
+    t0 = time.perf_counter()
+    w = synthetic_workload(ctx)
+    log("workload built in %.1f s: %s" % (time.perf_counter() - t0, w["desc"]))
+    plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
+    st = plan.stats()
+    log("plan:", {k: st[k] for k in ("n_t_tasks", "n_cohorts", "n_lines", "n_terms", "flops_total", "flops_gemm",
+                                      "flops_gemm_exec", "bytes_compulsory", "build_seconds")})
+    n = w["layout"].n
+    psi = torch.randn(n, dtype=torch.float64, device="cuda")
+    out = torch.empty_like(psi)
+    torch.cuda.synchronize()
+    ctx.sync()
+    for _ in range(args.warmup):
+        plan.apply_device(psi.data_ptr(), out.data_ptr())
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.apply_device(psi.data_ptr(), out.data_ptr())
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall / args.steps * 1e3
+    log("timed %d H*psi: %.3f ms each" % (args.steps, ms))
+
+    # per-stage HIP-event timing on the launch stream (separate pass, synchronous per step)
+    import ctypes as C
+    stage = np.zeros(4)
+    sms = (C.c_float * 4)()
+    nrep = max(3, min(args.steps, 10))
+    for _ in range(nrep):
+        ctx.check(ctx.lib.mq_hpsi_timed(plan.h, C.c_void_p(psi.data_ptr()), C.c_void_p(out.data_ptr()), 0, sms))
+        stage += np.array(list(sms))
+    stage /= nrep
+    log("stage ms (fill, T-GEMM, S-assembly, L*S GEMM):", np.round(stage, 4).tolist())
+    gemm_ms = stage[1] + stage[3]
+    achieved = st["flops_gemm_exec"] / (gemm_ms * 1e-3) / 1e12
+    value = st["flops_total"] / (ms * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": dict(w["desc"], step="one H*psi (Davidson iteration) of the site problem, inputs resident in HBM",
+                       flops_per_step_reference_model=st["flops_total"], gemm_flops_executed=st["flops_gemm_exec"],
+                       stage_ms={"s_clear": round(stage[0], 4), "t_gemm": round(stage[1], 4),
+                                 "s_assembly": round(stage[2], 4), "ls_gemm": round(stage[3], 4)}),
+        "roofline": {"bound": "mfma", "kernel": "gemm_f64_kernel (T-GEMM + L*S GEMM launches)",
+                     "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": None},
+    }
+    if not args.no_cpu:
+        try:
+            line["cpu_baseline"] = cpu_baseline(w, st["flops_total"])
+            log("cpu baseline:", line["cpu_baseline"])
+        except Exception as e:  # the GPU measurement stands on its own
+            log("cpu baseline failed:", repr(e))
+            line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    plan.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -185,6 +185,11 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t plan, mq_hpsi_stats_t* stats);
 int mq_hpsi_plan_psi_elements(mq_hpsi_plan_t plan, uint64_t* n);
 /* device pointers (on the context's device), stream-ordered on the context stream */
 int mq_hpsi(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out_dev, int precision);
+/* as mq_hpsi, synchronous, with the duration of each stage measured by HIP events recorded on
+ * the context stream: stage_ms[0] S clear, [1] T-GEMMs, [2] S-assembly, [3] output GEMMs
+ * (reference timing: cudaEvent around the GPU H*psi, super_hamil_mv.hpp:61-84) */
+int mq_hpsi_timed(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out_dev, int precision,
+                  float* stage_ms);
 /* host pointers, synchronous */
 int mq_hpsi_host(mq_hpsi_plan_t plan, const double* psi_in, double* psi_out, int precision);
 int mq_hpsi_plan_destroy(mq_hpsi_plan_t plan);
@@ -194,6 +199,23 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t plan);
 int mq_davidson(mq_hpsi_plan_t plan, const double* psi0, int n_ortho, const double* const* ortho,
                 double tol, int max_iter, int precision, double* eval, double* psi_out,
                 mq_davidson_stats_t* stats);
+
+/* ---- quantum-chemistry Hamiltonian MPO (host only; input producer of the path) -------------
+ * reference: models/chem/2u1/model.hpp (create_terms), generate_mpo/tagged_mpo_maker_optim.hpp,
+ * mp_tensors/ts_ops.hpp:96-185 (two-site MPO).  h1: L x L row-major h_pq; eri: L^4 row-major
+ * (pq|rs) (chemists' notation, all 8 permutational symmetries expected); terms with
+ * |coefficient| <= cutoff are dropped (integral_cutoff, DmrgParameters.h:76). */
+typedef struct mq_qc_s* mq_qc_t;
+int mq_qc_build(int symmetry, int L, const double* h1, const double* eri, double ecore, double cutoff,
+                mq_qc_t* out);
+/* L, the L+1 MPO bond dimensions, the core energy (any pointer may be NULL) */
+int mq_qc_info(mq_qc_t qc, int* L, uint32_t* bond_dims, double* ecore);
+/* descriptor of the site MPO (twosite = 0) or of the two-site MPO of sites (site, site+1); the
+ * arrays it points to are owned by the handle and stay valid until mq_qc_destroy */
+int mq_qc_site_mpo(mq_qc_t qc, int site, int twosite, mq_mpo_desc* out);
+/* physical index of one site (twosite = 0) or of two fused sites */
+int mq_qc_phys(mq_qc_t qc, int twosite, mq_index_desc* out);
+int mq_qc_destroy(mq_qc_t qc);
 
 /* ---- host-only utilities (no device needed) ------------------------------------------------ */
 /* Wigner 6j / 9j symbols, arguments as 2j (reference: gsl_sf_coupling_6j / _9j, extern "C" in

@@ -129,9 +129,15 @@ SIGNATURES = [
     ("mq_hpsi_plan_stats", i32, [vp, P(HpsiStats)]),
     ("mq_hpsi_plan_psi_elements", i32, [vp, P(C.c_uint64)]),
     ("mq_hpsi", i32, [vp, vp, vp, i32]),
+    ("mq_hpsi_timed", i32, [vp, vp, vp, i32, P(C.c_float)]),
     ("mq_hpsi_host", i32, [vp, dp, dp, i32]),
     ("mq_hpsi_plan_destroy", i32, [vp]),
     ("mq_davidson", i32, [vp, dp, i32, P(dp), C.c_double, i32, i32, dp, dp, P(DavidsonStats)]),
+    ("mq_qc_build", i32, [i32, i32, dp, dp, C.c_double, C.c_double, P(vp)]),
+    ("mq_qc_info", i32, [vp, P(i32), P(C.c_uint32), dp]),
+    ("mq_qc_site_mpo", i32, [vp, i32, i32, P(MPODesc)]),
+    ("mq_qc_phys", i32, [vp, i32, P(IndexDesc)]),
+    ("mq_qc_destroy", i32, [vp]),
     ("mq_wigner_6j", C.c_double, [i32] * 6),
     ("mq_wigner_9j", C.c_double, [i32] * 9),
     ("mq_index_position", i32, [P(IndexDesc), C.c_int32, C.c_int32]),

@@ -564,6 +564,42 @@ int mq_hpsi(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
     });
 }
 
+int mq_hpsi_timed(mq_hpsi_plan_t p, const double* in, double* out, int precision, float* stage_ms) {
+    if (!p) return MQ_ERR_INVALID;
+    return guarded(p->ctx, [&] {
+        require(in && out && in != out && stage_ms, "mq_hpsi_timed: bad argument");
+        hipStream_t s = p->ctx->stream;
+        hipEvent_t ev[5] = {};
+        try {
+            for (auto& e : ev) hipcheck(hipEventCreate(&e), "event create");
+            Bases b{};
+            b.p[BASE_PSI_IN] = const_cast<double*>(in);
+            b.p[BASE_PSI_OUT] = out;
+            b.p[BASE_T] = p->T;
+            b.p[BASE_S] = p->S;
+            b.p[BASE_LPANEL] = p->Lpanel;
+            b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
+            hipcheck(hipEventRecord(ev[0], s), "event");
+            hipcheck(launch_fill(p->S, p->host.s_elems, 0.0, s), "fill");
+            hipcheck(hipEventRecord(ev[1], s), "event");
+            p->tg.run(b, s, precision);
+            hipcheck(hipEventRecord(ev[2], s), "event");
+            hipcheck(launch_sassembly(p->lines, p->n_lines, p->terms, p->T, p->S, s), "sassembly");
+            hipcheck(hipEventRecord(ev[3], s), "event");
+            p->cg.run(b, s, precision);
+            hipcheck(hipEventRecord(ev[4], s), "event");
+            hipcheck(hipEventSynchronize(ev[4]), "event sync");
+            for (int i = 0; i < 4; ++i) hipcheck(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]), "elapsed");
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            for (auto e : ev)
+                if (e) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto e : ev) (void)hipEventDestroy(e);
+    });
+}
+
 int mq_hpsi_host(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
     if (!p) return MQ_ERR_INVALID;
     return guarded(p->ctx, [&] {

@@ -90,25 +90,28 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmDesc* __restric
             for (int i = 0; i < 2; ++i) af[i] = As[kk + fk][wm + 16 * i + fr];
 #pragma unroll
             for (int j = 0; j < 2; ++j) bf[j] = Bs[kk + fk][wn + 16 * j + fr];
+            // operands swapped: the instruction accumulates the transposed tile C^T = B^T A^T
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
         }
     }
 
-    // D layout of v_mfma_f64_16x16x4f64: lane holds rows 4*(lane/16) .. +3 of column lane%16
+    // Output layout of v_mfma_f64_16x16x4f64 (measured on gfx950): register r of lane l holds
+    // D[l / 16 + 4 r][l % 16].  With the swapped operands D = C^T, so lane l holds
+    // C[l % 16][l / 16 + 4 r]: consecutive lanes store consecutive rows of a column.
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            int n = n0 + wn + 16 * j + fr;
-            if (n >= g.N) continue;
+            int m = m0 + wm + 16 * i + fr;
+            if (m >= g.M) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int m = m0 + wm + 16 * i + 4 * fk + r;
-                if (m < g.M) C[m + int64_t(n) * g.ldc] = acc[i][j][r];
+                int n = n0 + wn + 16 * j + fk + 4 * r;
+                if (n < g.N) C[m + int64_t(n) * g.ldc] = acc[i][j][r];
             }
         }
 }

@@ -113,8 +113,9 @@ def _bond(rng, sym, D, edge):
     return ch, sp, herm, ph
 
 
-def random_mpo_chain(rng, sym, phys_list, D, density=0.35):
-    """list of MPOSpec, one per site, with bond dimension D in the bulk"""
+def random_mpo_chain(rng, sym, phys_list, D, density=0.35, row_cells=None):
+    """list of MPOSpec, one per site, with bond dimension D in the bulk; row_cells caps the number
+    of occupied cells per MPO row (quantum-chemistry MPOs have O(1-10) per row)"""
     n = len(phys_list)
     bonds = [_bond(rng, sym, D, edge=(k == 0 or k == n)) for k in range(n + 1)]
     mpos = []
@@ -155,12 +156,15 @@ def random_mpo_chain(rng, sym, phys_list, D, density=0.35):
                     break
         rng.shuffle(cells)
         used_l, used_r = set(), set()
+        per_row = {}
         chosen = []
         for b1, b2, op in cells:
-            if rng.random() < density or b1 not in used_l or b2 not in used_r:
+            full = row_cells is not None and per_row.get(b1, 0) >= row_cells
+            if (rng.random() < density and not full) or b1 not in used_l or b2 not in used_r:
                 chosen.append((b1, b2, op))
                 used_l.add(b1)
                 used_r.add(b2)
+                per_row[b1] = per_row.get(b1, 0) + 1
         for b1, b2, op in chosen:
             ops.append(op)
             terms.setdefault((b1, b2), []).append((len(ops) - 1, float(rng.uniform(0.5, 1.5))))
@@ -176,11 +180,11 @@ def random_mps_data(rng, layout):
 class Chain:
     """MPO chain + MPS site layouts/data with consistent indices."""
 
-    def __init__(self, rng, sym, n_sites, total, max_m, D, phys=None, density=0.35):
+    def __init__(self, rng, sym, n_sites, total, max_m, D, phys=None, density=0.35, row_cells=None, min_m=1):
         self.sym = sym
         phys = phys or (SU2U1_SITE if sym == abi.MQ_SYM_SU2U1 else TWOU1_SITE)
         self.phys = [phys] * n_sites
-        self.bonds = random_mps_indices(rng, sym, self.phys, total, max_m)
-        self.mpos = random_mpo_chain(rng, sym, self.phys, D, density)
+        self.bonds = random_mps_indices(rng, sym, self.phys, total, max_m, min_m)
+        self.mpos = random_mpo_chain(rng, sym, self.phys, D, density, row_cells)
         self.layouts = [MPSLayout(self.bonds[k], self.phys[k], self.bonds[k + 1]) for k in range(n_sites)]
         self.data = [random_mps_data(rng, l) for l in self.layouts]
