@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -286,14 +287,16 @@ struct QCBuilder {
     }
 
     void add_string(int n, const uint16_t* pos, const uint32_t* tag, double c, int& rightmost_left, int& leftmost_right) {
+        // identity paths must reach every term's first site and leave from every term's last site
+        // (TaggedMPOMaker::add_term, tagged_mpo_maker_optim.hpp:204-205)
+        rightmost_left = std::max(rightmost_left, int(pos[0]));
+        leftmost_right = std::min(leftmost_right, int(pos[n - 1]));
         if (n == 1) {
             Mat4 const& m = table.ops[tag[0]];
             for (int i = 0; i < 16; ++i) site_terms[pos[0]][i] += c * m[i];
             has_site_term[pos[0]] = 1;
             return;
         }
-        rightmost_left = std::max(rightmost_left, int(pos[0]));
-        leftmost_right = std::min(leftmost_right, int(pos[n - 1]));
         Key TL = key_left(), TR = key_right();
         auto fill_for = [&](int first_pending) { return odd(n - first_pending, tag + first_pending) ? tag_fill : tag_id; };
         if (n == 2) {
@@ -309,7 +312,9 @@ struct QCBuilder {
             detach(pos[1], k1, k2, tag[1], c);
             insert_fill(pos[1] + 1, pos[2], k2, fill_for(2));
             attach(pos[2], k2, TR, tag[2]);
-        } else {
+        } else if (pos[1] + pos[2] < L - 1) {
+            // gap between the 2nd and 3rd operator left of the centre: carry the applied pair
+            // (the reference's add_4term keying)
             Key k1 = key_ops(1, pos, tag);
             attach(pos[0], TL, k1, tag[0]);
             insert_fill(pos[0] + 1, pos[1], k1, fill_for(1));
@@ -318,6 +323,19 @@ struct QCBuilder {
             insert_fill(pos[1] + 1, pos[2], k2, fill_for(2));
             Key k3 = key_ops(1, pos + 3, tag + 3);
             detach(pos[2], k2, k3, tag[2], c);
+            insert_fill(pos[2] + 1, pos[3], k3, fill_for(3));
+            attach(pos[3], k3, TR, tag[3]);
+        } else {
+            // gap right of the centre: carry the pending pair instead, so that no bond holds more
+            // than ~min(p, L-p)^2 pair states
+            Key k1 = key_ops(1, pos, tag);
+            attach(pos[0], TL, k1, tag[0]);
+            insert_fill(pos[0] + 1, pos[1], k1, fill_for(1));
+            Key k2 = key_ops(2, pos + 2, tag + 2);
+            detach(pos[1], k1, k2, tag[1], c);
+            insert_fill(pos[1] + 1, pos[2], k2, fill_for(2));
+            Key k3 = key_ops(1, pos + 3, tag + 3);
+            attach(pos[2], k2, k3, tag[2]);
             insert_fill(pos[2] + 1, pos[3], k3, fill_for(3));
             attach(pos[3], k3, TR, tag[3]);
         }
@@ -368,6 +386,15 @@ struct QCBuilder {
                     else if (site[i] == q) m = mul(m, cre[i] ? cd[spin[i]] : cc[spin[i]]);
                 }
                 if (is_zero(m)) return;
+                // canonical sign (first non-zero entry positive); the sign moves to the coefficient
+                for (double v : m)
+                    if (v != 0.0) {
+                        if (v < 0) {
+                            for (auto& x : m) x = -x;
+                            coeff = -coeff;
+                        }
+                        break;
+                    }
                 k.pos[k.n] = uint16_t(q);
                 k.tag[k.n] = table.tag(m);
                 ++k.n;
@@ -609,8 +636,10 @@ int mq_qc_build(int symmetry, int L, const double* h1, const double* eri, double
         *out = q;
         return MQ_OK;
     } catch (mq::Error const& e) {
+        fprintf(stderr, "mq_qc_build: %s\n", e.what());
         return e.code;
-    } catch (...) {
+    } catch (std::exception const& e) {
+        fprintf(stderr, "mq_qc_build: %s\n", e.what());
         return MQ_ERR_INVALID;
     }
 }

@@ -1,0 +1,283 @@
+"""Two-site DMRG sweeps over the HIP contraction engine (restating optimize/ts_optimize.hpp).
+
+Per two-site step (reference ts_optimize::sweep, ts_optimize.hpp:114-362):
+  1. psi2 = TwoSiteTensor(mps[i], mps[i+1])                        (twositetensor.hpp:28-120)
+  2. Jacobi-Davidson on the device with the H*psi plan of (L_i, W_i W_i+1, R_i+2)
+                                                                   (solver_interface.hpp:30-77)
+  3. SVD split with the reference truncation rule (estimate_truncation / svd_truncate,
+     block_matrix_algorithms.h:135-244; twositetensor.hpp:153-200)
+  4. boundary growth on the device (overlap_mpo_left/right_step)
+The SVD split runs on the host (numpy/LAPACK) in this version; boundaries and the contraction live
+in HBM.  MPS site tensors are kept as right-paired vectors with their MPSLayout.
+"""
+import time
+
+import numpy as np
+
+from . import abi
+from .engine import Context, DeviceBoundary, DeviceMPO, DeviceMPS, HpsiPlan, lstep, rstep
+from .model import Index, MPSLayout, qadd, qsub
+
+
+# ---- block helpers ---------------------------------------------------------------------------
+def rp_block(lay, data, lb):
+    ls = lay.left.sizes[lb]
+    return data[lay.rp_off[lb]:lay.rp_off[lb + 1]].reshape(lay.rp_cols[lb], ls).T
+
+
+def twosite_offsets(phys):
+    """(s1, s2) -> (fused charge, offset inside the fused sector); ProductBasis(phys, phys) order"""
+    off, acc = {}, {}
+    for s1, d1 in zip(phys.charges, phys.sizes):
+        for s2, d2 in zip(phys.charges, phys.sizes):
+            c = qadd(s1, s2)
+            off[(s1, s2)] = (c, acc.get(c, 0))
+            acc[c] = acc.get(c, 0) + d1 * d2
+    return off
+
+
+def merge_two(la, a, lb, b, phys2, off12):
+    """psi2(l, s1 s2, r) = sum_m A(l, s1, m) B(m, s2, r) as a right-paired vector of
+    MPSLayout(la.left, phys2, lb.right)"""
+    lay2 = MPSLayout(la.left, phys2, lb.right)
+    out = np.zeros(lay2.n)
+    for l_i, lc in enumerate(la.left.charges):
+        ls = la.left.sizes[l_i]
+        blk2 = out[lay2.rp_off[l_i]:lay2.rp_off[l_i + 1]].reshape(lay2.rp_cols[l_i], ls).T
+        A = rp_block(la, a, l_i)
+        for s1 in la.phys.charges:
+            m = qadd(lc, s1)
+            mb = la.right.position(m)
+            if mb >= len(la.right):
+                continue
+            ms = la.right.sizes[mb]
+            Aslice = A[:, la.rpb[(s1, m)]: la.rpb[(s1, m)] + ms]
+            B = rp_block(lb, b, mb)
+            for s2 in lb.phys.charges:
+                r = qadd(m, s2)
+                rb = lb.right.position(r)
+                if rb >= len(lb.right):
+                    continue
+                rs = lb.right.sizes[rb]
+                c, o = off12[(s1, s2)]
+                col = lay2.rpb[(c, r)] + o * rs
+                blk2[:, col:col + rs] += Aslice @ B[:, lb.rpb[(s2, r)]: lb.rpb[(s2, r)] + rs]
+    return lay2, out
+
+
+def split_matrices(lay2, psi2, phys, off12):
+    """X_m[(s1, l), (s2, r)] for every middle charge m, rows in ProductBasis(phys, left) order and
+    columns in ProductBasis(phys, right, -s2 + r) order"""
+    left, right = lay2.left, lay2.right
+    mids = {}
+    for lc in left.charges:
+        for s1 in phys.charges:
+            mids.setdefault(qadd(lc, s1), None)
+    mats = {}
+    for m in mids:
+        rows, rows_n = [], 0
+        for s1 in phys.charges:
+            lb = left.position(qsub(m, s1))
+            if lb < len(left):
+                rows.append((s1, lb, rows_n))
+                rows_n += left.sizes[lb]
+        cols, cols_n = [], 0
+        for s2 in phys.charges:
+            rb = right.position(qadd(m, s2))
+            if rb < len(right):
+                cols.append((s2, rb, cols_n))
+                cols_n += right.sizes[rb]
+        if not rows or not cols:
+            continue
+        X = np.zeros((rows_n, cols_n))
+        for s1, lb, r0 in rows:
+            ls = left.sizes[lb]
+            blk = rp_block(lay2, psi2, lb)
+            for s2, rb, c0 in cols:
+                rs = right.sizes[rb]
+                c, o = off12[(s1, s2)]
+                col = lay2.rpb[(c, right.charges[rb])] + o * rs
+                X[r0:r0 + ls, c0:c0 + rs] = blk[:, col:col + rs]
+        mats[m] = (X, rows, cols)
+    return mats
+
+
+def estimate_truncation(svals, Mmax, cutoff):
+    """keeps per block, truncated weight (estimate_truncation, block_matrix_algorithms.h:135-181):
+    all values >= max(cutoff * s_max, s_[Mmax]) are kept"""
+    allv = np.sort(np.concatenate([s for s in svals.values()]))[::-1]
+    cut = cutoff * allv[0]
+    if allv.size > Mmax:
+        cut = max(cut, allv[Mmax])
+    keeps = {m: int(np.sum(s >= cut)) for m, s in svals.items()}
+    tw = float(np.sum(allv[allv < cut] ** 2) / np.sum(allv ** 2))
+    return keeps, tw
+
+
+def split_two(lay2, psi2, phys, off12, Mmax, cutoff, to_right):
+    """SVD split of a two-site tensor; to_right: A = U left-normalised, B = S V^T (split_mps_l2r),
+    else A = U S, B = V^T right-normalised (split_mps_r2l)"""
+    mats = split_matrices(lay2, psi2, phys, off12)
+    svd = {m: np.linalg.svd(X, full_matrices=False) for m, (X, _, _) in mats.items()}
+    keeps, tw = estimate_truncation({m: s[1] for m, s in svd.items()}, Mmax, cutoff)
+    mid = Index([m for m in keeps if keeps[m] > 0], [keeps[m] for m in keeps if keeps[m] > 0])
+    la = MPSLayout(lay2.left, phys, mid)
+    lb = MPSLayout(mid, phys, lay2.right)
+    a_lp = np.zeros(la.n)
+    b_rp = np.zeros(lb.n)
+    for m, (U, S, Vt) in svd.items():
+        k = keeps[m]
+        if k == 0:
+            continue
+        Uk, Sk, Vk = U[:, :k], S[:k], Vt[:k, :]
+        if to_right:
+            Am, Bm = Uk, Sk[:, None] * Vk
+        else:
+            Am, Bm = Uk * Sk[None, :], Vk
+        _, rows, cols = mats[m]
+        mb = mid.position(m)
+        # A left-paired block of right sector m: rows (s1, l) in the same order as X
+        a_blk = a_lp[la.lp_off[mb]:la.lp_off[mb + 1]].reshape(k, la.lp_rows[mb]).T
+        for s1, lbi, r0 in rows:
+            ls = lay2.left.sizes[lbi]
+            o = la.lpb[(s1, lay2.left.charges[lbi])]
+            a_blk[o:o + ls, :] = Am[r0:r0 + ls, :]
+        b_blk = b_rp[lb.rp_off[mb]:lb.rp_off[mb + 1]].reshape(lb.rp_cols[mb], k).T
+        for s2, rbi, c0 in cols:
+            rs = lay2.right.sizes[rbi]
+            o = lb.rpb[(s2, lay2.right.charges[rbi])]
+            b_blk[:, o:o + rs] = Bm[:, c0:c0 + rs]
+    return la, la.to_right(a_lp), lb, b_rp, tw
+
+
+# ---- MPS initialisation ------------------------------------------------------------------------
+def allowed_bonds(phys_list, total, m_cap, full_right=False):
+    """bond indices with per-sector sizes min(#states from the left, #states to the right, m_cap)
+    (reference allowed_sectors, mp_tensors/mps_sectors.h); full_right: sizes = #states to the
+    right, so right-normalised tensors span the whole right Hilbert space"""
+    from math import comb
+    n = len(phys_list)
+    bonds = []
+    for k in range(n + 1):
+        d = {}
+        for nu in range(0, min(k, total[0]) + 1):
+            for nd in range(0, min(k, total[1]) + 1):
+                left = comb(k, nu) * comb(k, nd)
+                ru, rd = total[0] - nu, total[1] - nd
+                if ru > n - k or rd > n - k:
+                    continue
+                right = comb(n - k, ru) * comb(n - k, rd)
+                s = (right if k > 0 else 1) if full_right else min(left, right, m_cap)
+                if s > 0:
+                    d[(nu, nd)] = s
+        bonds.append(Index.from_dict(d))
+    return bonds
+
+
+def right_normalize(layouts, data):
+    """right-canonical form by QR sweeps from the right (MPS::canonize)"""
+    n = len(layouts)
+    for k in range(n - 1, 0, -1):
+        lay, x = layouts[k], data[k]
+        new_sizes, Rts = [], []
+        blocks = []
+        for lb in range(len(lay.left)):
+            X = rp_block(lay, x, lb)
+            Q, R = np.linalg.qr(X.T)
+            blocks.append(Q.T)
+            Rts.append(R.T)
+            new_sizes.append(Q.shape[1])
+        mid = Index(lay.left.charges, new_sizes)
+        nl = MPSLayout(mid, lay.phys, lay.right)
+        nx = np.zeros(nl.n)
+        for lb in range(len(mid)):
+            nx[nl.rp_off[lb]:nl.rp_off[lb + 1]] = blocks[lb].T.reshape(-1)
+        layouts[k], data[k] = nl, nx
+        # absorb R^T into the left neighbour's right bond
+        pl, px = layouts[k - 1], data[k - 1]
+        lp = pl.to_left(px)
+        npl = MPSLayout(pl.left, pl.phys, mid)
+        nlp = np.zeros(npl.n)
+        for rb in range(len(pl.right)):
+            rows = pl.lp_rows[rb]
+            blk = lp[pl.lp_off[rb]:pl.lp_off[rb + 1]].reshape(pl.right.sizes[rb], rows).T
+            mb = mid.position(pl.right.charges[rb])
+            out = blk @ Rts[mb]
+            nlp[npl.lp_off[mb]:npl.lp_off[mb + 1]] = out.T.reshape(-1)
+        layouts[k - 1], data[k - 1] = npl, npl.to_right(nlp)
+    data[0] = data[0] / np.linalg.norm(data[0])
+    return layouts, data
+
+
+class TwoSiteDMRG:
+    """Two-site DMRG for a TwoU1 quantum-chemistry Hamiltonian (QCHamiltonian)."""
+
+    def __init__(self, ham, n_up, n_dn, M, ctx=None, init_m=8, seed=42, tol=1e-8, max_iter=8, cutoff=1e-16,
+                 verbose=False):
+        self.ham, self.M, self.tol, self.max_iter, self.cutoff = ham, M, tol, max_iter, cutoff
+        self.verbose = verbose
+        self.ctx = ctx or Context(0, abi.MQ_SYM_TWOU1)
+        self.L = ham.L
+        self.phys = ham.phys()
+        self.phys2 = ham.phys(True)
+        self.off12 = twosite_offsets(self.phys)
+        rng = np.random.default_rng(seed)
+        bonds = allowed_bonds([self.phys] * self.L, (n_up, n_dn), init_m)
+        self.lay = [MPSLayout(bonds[k], self.phys, bonds[k + 1]) for k in range(self.L)]
+        self.data = [rng.normal(size=l.n) for l in self.lay]
+        right_normalize(self.lay, self.data)
+        self.mpo1 = [DeviceMPO(self.ctx, ham.mpo(k)) for k in range(self.L)]
+        self.mpo2 = {}
+        # boundaries: left[k] of bond k, right[k] of bond k
+        self.left = {0: DeviceBoundary.initial(self.ctx, self.lay[0].left, self.mpo1[0].spec.row_dim)}
+        self.right = {self.L: DeviceBoundary.initial(self.ctx, self.lay[-1].right, self.mpo1[-1].spec.col_dim)}
+        for k in range(self.L - 1, 1, -1):
+            m = DeviceMPS(self.ctx, self.lay[k], self.data[k])
+            self.right[k] = rstep(self.ctx, self.mpo1[k], self.right[k + 1], m, m)
+        self.energies, self.trunc = [], []
+        self.timing = {"davidson": 0.0, "hpsi": 0.0, "plan": 0.0, "svd": 0.0, "grow": 0.0, "merge": 0.0}
+        self.hpsi_flops = 0
+
+    def ts_mpo(self, i):
+        if i not in self.mpo2:
+            self.mpo2[i] = DeviceMPO(self.ctx, self.ham.mpo(i, True))
+        return self.mpo2[i]
+
+    def step(self, i, to_right):
+        t0 = time.perf_counter()
+        lay2, psi2 = merge_two(self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1], self.phys2, self.off12)
+        t1 = time.perf_counter()
+        plan = HpsiPlan(self.ctx, self.ts_mpo(i), self.left[i], self.right[i + 2], lay2)
+        t2 = time.perf_counter()
+        e, psi, st = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter)
+        t3 = time.perf_counter()
+        self.hpsi_flops += plan.stats()["flops_total"] * st["iterations"]
+        self.timing["hpsi"] += st["hpsi_seconds"]
+        plan.close()
+        la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right)
+        t4 = time.perf_counter()
+        self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1] = la, a, lb, b
+        if to_right:
+            m = DeviceMPS(self.ctx, la, a)
+            self.left[i + 1] = lstep(self.ctx, self.mpo1[i], self.left[i], m, m)
+        else:
+            m = DeviceMPS(self.ctx, lb, b)
+            self.right[i + 1] = rstep(self.ctx, self.mpo1[i + 1], self.right[i + 2], m, m)
+        t5 = time.perf_counter()
+        for k, v in (("merge", t1 - t0), ("plan", t2 - t1), ("davidson", t3 - t2), ("svd", t4 - t3), ("grow", t5 - t4)):
+            self.timing[k] += v
+        self.energies.append(e + self.ham.ecore)
+        self.trunc.append(tw)
+        if self.verbose:
+            print(f"  site {i} {'->' if to_right else '<-'} E={e + self.ham.ecore:.12f} M={sum(lb.left.sizes)} "
+                  f"tw={tw:.2e} it={st['iterations']}", flush=True)
+        return e + self.ham.ecore
+
+    def sweep(self):
+        """one full sweep (2L-2 two-site steps, ts_optimize.hpp:128-131)"""
+        t0 = time.perf_counter()
+        for i in range(0, self.L - 1):
+            self.step(i, True)
+        for i in range(self.L - 2, -1, -1):
+            self.step(i, False)
+        return self.energies[-1], time.perf_counter() - t0

@@ -1,0 +1,212 @@
+"""Quantum-chemistry inputs: seeded integrals, FCIDUMP files and the Hamiltonian MPO chain.
+
+Integral generator (the benchmark configurations):
+  h_pp = linear ramp -10 -> +2 Eh; h_pq (p != q) = N(0, 0.1) exp(-|p-q|/4), symmetrised;
+  (pq|rs) = sum_k L^k_pq L^k_rs with L^k_pq = N(0, 0.3) exp(-|p-q|/4) symmetrised in pq, so all
+  8 permutational symmetries hold and the ERI matrix is positive semidefinite.
+FCIDUMP: the reference's parser skips a 4-line header and reads "value i j k l" lines, 1-based,
+with the core energy on "0 0 0 0" (models/chem/parse_integrals.h:36-156).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .model import Index, MPOSpec, SiteOp
+
+
+def make_integrals(L, seed, rank=16):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    p = np.arange(L)
+    decay = np.exp(-np.abs(p[:, None] - p[None, :]) / 4.0)
+    off = rng.normal(0.0, 0.1, (L, L)) * decay
+    h = 0.5 * (off + off.T)
+    np.fill_diagonal(h, np.linspace(-10.0, 2.0, L))
+    Lk = rng.normal(0.0, 0.3, (rank, L, L)) * decay[None]
+    Lk = 0.5 * (Lk + Lk.transpose(0, 2, 1))
+    eri = np.einsum("kpq,krs->pqrs", Lk, Lk)
+    return h, eri, 0.0
+
+
+def write_fcidump(path, h, eri, ecore=0.0, nelec=None, ms2=0, tol=1e-14):
+    L = h.shape[0]
+    nelec = L if nelec is None else nelec
+    with open(path, "w") as f:
+        f.write(f" &FCI NORB={L},NELEC={nelec},MS2={ms2},\n")
+        f.write("  ORBSYM=" + ",".join(["1"] * L) + ",\n")
+        f.write("  ISYM=1,\n")
+        f.write(" &END\n")
+        for i in range(L):
+            for j in range(i + 1):
+                for k in range(L):
+                    for l in range(k + 1):
+                        if (i * (i + 1) // 2 + j) < (k * (k + 1) // 2 + l):
+                            continue
+                        v = eri[i, j, k, l]
+                        if abs(v) > tol:
+                            f.write(f"{v: .16e} {i + 1} {j + 1} {k + 1} {l + 1}\n")
+        for i in range(L):
+            for j in range(i + 1):
+                if abs(h[i, j]) > tol:
+                    f.write(f"{h[i, j]: .16e} {i + 1} {j + 1} 0 0\n")
+        f.write(f"{ecore: .16e} 0 0 0 0\n")
+
+
+def read_fcidump(path):
+    """(h, eri, ecore); the first 4 lines are the header the reference parser skips"""
+    lines = open(path).read().splitlines()
+    head = " ".join(lines[:4])
+    L = int(head.split("NORB=")[1].split(",")[0])
+    h = np.zeros((L, L))
+    eri = np.zeros((L, L, L, L))
+    ecore = 0.0
+    for ln in lines[4:]:
+        parts = ln.split()
+        if len(parts) != 5:
+            continue
+        v = float(parts[0])
+        i, j, k, l = (int(x) for x in parts[1:])
+        if i == j == k == l == 0:
+            ecore = v
+        elif k == 0 and l == 0:
+            h[i - 1, j - 1] = h[j - 1, i - 1] = v
+        else:
+            i, j, k, l = i - 1, j - 1, k - 1, l - 1
+            for a, b, c, d in ((i, j, k, l), (j, i, k, l), (i, j, l, k), (j, i, l, k),
+                               (k, l, i, j), (l, k, i, j), (k, l, j, i), (l, k, j, i)):
+                eri[a, b, c, d] = v
+    return h, eri, ecore
+
+
+class RawMPO:
+    """An MPO tensor whose descriptor arrays live in the QC builder (mq_qc_site_mpo)."""
+
+    def __init__(self, owner, desc):
+        self._owner = owner
+        self._desc = desc
+        self.row_dim, self.col_dim = desc.row_dim, desc.col_dim
+
+    def desc(self):
+        return self._desc
+
+    def to_spec(self):
+        """MPOSpec copy (for the dense test checker)"""
+        d = self._desc
+        ops = []
+        for o in range(d.n_ops):
+            blocks = []
+            for bl in range(d.op_block_ptr[o], d.op_block_ptr[o + 1]):
+                lc = (d.block_charges[4 * bl], d.block_charges[4 * bl + 1])
+                rc = (d.block_charges[4 * bl + 2], d.block_charges[4 * bl + 3])
+                ents = [(d.entry_rc[2 * e], d.entry_rc[2 * e + 1], d.entry_coeff[e])
+                        for e in range(d.block_entry_ptr[bl], d.block_entry_ptr[bl + 1])]
+                blocks.append((lc, rc, ents))
+            ops.append(SiteOp(blocks, d.op_spin[o] if d.op_spin else 0))
+        terms = {}
+        for t in range(d.n_terms):
+            terms.setdefault((d.term_b[2 * t], d.term_b[2 * t + 1]), []).append((d.term_op[t], d.term_scale[t]))
+        arr = lambda p, n: None if not p else [p[i] for i in range(n)]
+        return MPOSpec(d.row_dim, d.col_dim, ops, terms, arr(d.left_spin, d.row_dim), arr(d.right_spin, d.col_dim),
+                       arr(d.left_herm, d.row_dim), arr(d.left_phase, d.row_dim), arr(d.right_herm, d.col_dim),
+                       arr(d.right_phase, d.col_dim))
+
+
+class QCHamiltonian:
+    """TwoU1 quantum-chemistry MPO chain (mq_qc_build)."""
+
+    def __init__(self, h, eri, ecore=0.0, cutoff=1e-20):
+        self.lib = abi.lib()
+        self.L = h.shape[0]
+        self.h = np.ascontiguousarray(h, np.float64)
+        self.eri = np.ascontiguousarray(eri, np.float64)
+        hd = C.c_void_p()
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        rc = self.lib.mq_qc_build(abi.MQ_SYM_TWOU1, self.L, dp(self.h), dp(self.eri), float(ecore), float(cutoff),
+                                  C.byref(hd))
+        if rc != 0:
+            raise RuntimeError(f"mq_qc_build failed ({rc})")
+        self.h_qc = hd
+        self.ecore = float(ecore)
+        dims = (C.c_uint32 * (self.L + 1))()
+        self.lib.mq_qc_info(hd, None, dims, None)
+        self.bond_dims = list(dims)
+        self._cache = {}
+
+    def phys(self, twosite=False):
+        d = abi.IndexDesc()
+        self.lib.mq_qc_phys(self.h_qc, int(twosite), C.byref(d))
+        return Index([(d.charges[2 * i], d.charges[2 * i + 1]) for i in range(d.n)], [d.sizes[i] for i in range(d.n)])
+
+    def mpo(self, site, twosite=False):
+        key = (site, twosite)
+        if key not in self._cache:
+            d = abi.MPODesc()
+            rc = self.lib.mq_qc_site_mpo(self.h_qc, int(site), int(twosite), C.byref(d))
+            if rc != 0:
+                raise RuntimeError("mq_qc_site_mpo failed")
+            self._cache[key] = RawMPO(self, d)
+        return self._cache[key]
+
+    def __del__(self):
+        try:
+            self._cache.clear()
+            self.lib.mq_qc_destroy(self.h_qc)
+        except Exception:
+            pass
+
+
+# ---- independent determinant FCI (spin-summed E_pq form) for small L -------------------------
+def _strings(L, n):
+    from itertools import combinations
+    return [sum(1 << i for i in c) for c in combinations(range(L), n)]
+
+
+def _excitation_tables(L, strings):
+    """E_pq on one spin's strings: dense (ns x ns) per (p, q) with fermionic signs"""
+    pos = {s: i for i, s in enumerate(strings)}
+    ns = len(strings)
+    E = np.zeros((L, L, ns, ns))
+    for i, s in enumerate(strings):
+        for q in range(L):
+            if not (s >> q) & 1:
+                continue
+            t = s & ~(1 << q)
+            sign_q = (-1) ** bin(s & ((1 << q) - 1)).count("1")
+            for p in range(L):
+                if (t >> p) & 1:
+                    continue
+                u = t | (1 << p)
+                sign_p = (-1) ** bin(t & ((1 << p) - 1)).count("1")
+                E[p, q, pos[u], i] += sign_p * sign_q
+    return E
+
+
+def fci_energy(h, eri, n_up, n_dn, ecore=0.0, k=1):
+    """Lowest eigenvalue(s) of H in the (n_up, n_dn) sector by a determinant CI built from the
+    spin-summed generators: H = sum h_pq E_pq + 1/2 sum (pq|rs) (E_pq E_rs - delta_qr E_ps)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    L = h.shape[0]
+    sa, sb = _strings(L, n_up), _strings(L, n_dn)
+    Ea, Eb = _excitation_tables(L, sa), _excitation_tables(L, sb)
+    Ia, Ib = sp.identity(len(sa), format="csr"), sp.identity(len(sb), format="csr")
+    E = [[sp.kron(sp.csr_matrix(Ea[p, q]), Ib) + sp.kron(Ia, sp.csr_matrix(Eb[p, q])) for q in range(L)] for p in range(L)]
+    dim = len(sa) * len(sb)
+    H = sp.csr_matrix((dim, dim))
+    hp = h - 0.5 * np.einsum("pqqs->ps", eri)
+    for p in range(L):
+        for q in range(L):
+            H = H + hp[p, q] * E[p][q]
+    for p in range(L):
+        for q in range(L):
+            G = sp.csr_matrix((dim, dim))
+            for r in range(L):
+                for s in range(L):
+                    if eri[p, q, r, s] != 0.0:
+                        G = G + eri[p, q, r, s] * E[r][s]
+            H = H + 0.5 * (E[p][q] @ G)
+    if dim <= 400:
+        w = np.linalg.eigvalsh(H.toarray())[:k]
+    else:
+        w = np.sort(sla.eigsh(H, k=k, which="SA", tol=1e-12)[0])
+    return w + ecore
