@@ -58,6 +58,39 @@ def synthetic_workload(ctx, seed=7):
     return dict(mpo=mpos[mid], mpo_spec=ch.mpos[mid], L=L, R=R, layout=lay, sym=abi.MQ_SYM_TWOU1, desc=desc)
 
 
+def qc_workload(ctx, L=24, M=512, seed=1002, warm_sweeps=1):
+    """Configuration 2: 24 orbitals, 24 electrons (S_z = 0), TwoU1, M = 512, seeded integrals
+    (PCG64(1002), L-rank 16).  Two-site DMRG from a random MPS: `warm_sweeps` full sweeps at
+    M = 512, then the left-to-right half sweep up to the middle two-site problem, whose H*psi is
+    the benchmark step."""
+    from qcmaquis_amd import abi
+    from qcmaquis_amd.dmrg import TwoSiteDMRG, merge_two
+    from qcmaquis_amd.qc import QCHamiltonian, make_integrals
+    h, eri, ec = make_integrals(L, seed, 16)
+    t0 = time.perf_counter()
+    ham = QCHamiltonian(h, eri, ec)
+    log("MPO built in %.1f s, bond dims max %d" % (time.perf_counter() - t0, max(ham.bond_dims)))
+    d = TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8)
+    sweeps = []
+    for s in range(warm_sweeps):
+        e, dt = d.sweep()
+        sweeps.append(dt)
+        log("sweep %d: E = %.10f, %.1f s, timing %s" % (s, e, dt, {k: round(v, 2) for k, v in d.timing.items()}))
+    mid = L // 2 - 1
+    for i in range(0, mid):
+        d.step(i, True)
+    lay2, psi2 = merge_two(d.lay[mid], d.data[mid], d.lay[mid + 1], d.data[mid + 1], d.phys2, d.off12)
+    mpo = d.ts_mpo(mid)
+    desc = {"workload": f"configuration 2: {L} orbitals, {L} electrons, TwoU1, M={M}, seeded FCIDUMP "
+                        f"(PCG64({seed}), rank 16); two-site H*psi at sites ({mid},{mid + 1}) after {warm_sweeps} sweep(s)",
+            "symmetry": "TwoU1", "L": L, "M": M, "M_left": lay2.left.total(), "M_right": lay2.right.total(),
+            "sectors_left": len(lay2.left), "sectors_right": len(lay2.right), "D_left": mpo.spec.row_dim,
+            "D_right": mpo.spec.col_dim, "psi_elements": lay2.n, "energy": d.energies[-1],
+            "sweep_seconds": sweeps, "sweep_timing_s": {k: round(v, 3) for k, v in d.timing.items()}}
+    return dict(mpo=mpo, mpo_spec=ham.mpo(mid, True), L=d.left[mid], R=d.right[mid + 2], layout=lay2,
+                sym=abi.MQ_SYM_TWOU1, desc=desc, keep=(d, ham))
+
+
 def cpu_baseline(w, flops, budget_s=30.0):
     """the oracle (restated reference CPU path) on the same site problem, all host cores"""
     import oracle
@@ -87,7 +120,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="synthetic")
+    ap.add_argument("--workload", default="qc", choices=["qc", "synthetic"])
+    ap.add_argument("--warm-sweeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -102,7 +136,7 @@ def main():
     torch.cuda.init()
     ctx = Context(0, abi.MQ_SYM_TWOU1)
     t0 = time.perf_counter()
-    w = synthetic_workload(ctx)
+    w = qc_workload(ctx, warm_sweeps=args.warm_sweeps) if args.workload == "qc" else synthetic_workload(ctx)
     log("workload built in %.1f s: %s" % (time.perf_counter() - t0, w["desc"]))
     plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
     st = plan.stats()
@@ -151,7 +185,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic",
+        "data": "synthetic (seeded integrals, random initial MPS)" if args.workload == "qc" else "synthetic",
         "config": dict(w["desc"], step="one H*psi (Davidson iteration) of the site problem, inputs resident in HBM",
                        flops_per_step_reference_model=st["flops_total"], gemm_flops_executed=st["flops_gemm_exec"],
                        stage_ms={"s_clear": round(stage[0], 4), "t_gemm": round(stage[1], 4),

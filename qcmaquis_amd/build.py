@@ -36,14 +36,15 @@ def build_library(verbose=False):
     for src in sources:
         obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
         # host-only translation units: plain C++ against the HIP runtime headers
-        cmd = [HIPCC] + COMMON + ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-c", "-o", obj]
+        cmd = [HIPCC] + COMMON + ["-x", "c++", "-fopenmp", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include", "-c",
+                                  "-o", obj]
         if src.endswith(".hip"):
             cmd = [HIPCC, "--offload-arch=" + ARCH] + COMMON + DEVICE_FLAGS + ["-x", "hip", "-c", "-o", obj]
         jobs.append((cmd + [os.path.join(CSRC, src)], obj))
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(lambda j: _run(j[0]), jobs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + [o for _, o in jobs])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-fopenmp", "-o", tmp] + [o for _, o in jobs])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -185,36 +185,66 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
     validate_boundary_offsets(R);
     ProductBasis rpb = ProductBasis::right_paired(phys_i, right_i);
     const bool su2 = sym == Sym::SU2U1;
+    const size_t NL = left_i.size(), NR = right_i.size(), NP = phys_i.size();
+
+    // lookup tables built once per plan
+    std::vector<uint32_t> pos_lp(NL * NP);
+    std::vector<uint64_t> pb_off(NP * NR, 0);
+    for (size_t lb = 0; lb < NL; ++lb)
+        for (size_t s = 0; s < NP; ++s)
+            pos_lp[lb * NP + s] = uint32_t(right_i.position(left_i.charges[lb] + phys_i.charges[s]));
+    for (size_t s = 0; s < NP; ++s)
+        for (size_t rb = 0; rb < NR; ++rb) pb_off[s * NR + rb] = rpb(phys_i.charges[s], right_i.charges[rb]);
+    std::vector<uint32_t> r_ci(NR * NR);
+    for (size_t a = 0; a < NR; ++a)
+        for (size_t b = 0; b < NR; ++b) r_ci[a * NR + b] = R.cohort_index(right_i.charges[a], right_i.charges[b]);
+    // operator blocks grouped by the physical sector of their output charge
+    struct BlkRef { uint32_t blk; uint32_t s_in; };
+    std::vector<std::vector<BlkRef>> blk_by_out(mpo.ops.size() * NP);
+    for (size_t o = 0; o < mpo.ops.size(); ++o)
+        for (size_t bi = 0; bi < mpo.ops[o].blocks.size(); ++bi) {
+            OpBlock const& blk = mpo.ops[o].blocks[bi];
+            size_t so = phys_i.position(blk.rc), si = phys_i.position(blk.lc);
+            if (so == NP || si == NP) continue;  // a block outside the physical index never matches
+            blk_by_out[o * NP + so].push_back(BlkRef{uint32_t(bi), uint32_t(si)});
+        }
+    // present MPO rows of every left cohort
+    std::vector<std::vector<uint32_t>> l_rows(L.n_cohorts());
+    for (uint32_t ci = 0; ci < L.n_cohorts(); ++ci)
+        for (uint32_t b1 = 0; b1 < mpo.row_dim; ++b1)
+            if (L.has_block(ci, b1)) l_rows[ci].push_back(b1);
 
     struct CohRec {
         uint32_t lb_in, lb_out, ci, ci_eff, nb;
         CohortAcc acc;
     };
-    std::vector<CohRec> cohorts;
-    std::vector<TList> tlists(left_i.size());
+    std::vector<std::vector<CohRec>> coh_by_in(NL);
+    std::vector<TList> tlists(NL);
     std::unordered_set<uint32_t> l_used, r_used;
+    std::string err;
 
-    for (uint32_t lb_in = 0; lb_in < left_i.size(); ++lb_in) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint32_t lb_in = 0; lb_in < NL; ++lb_in) {
+        try {
         Charge lc_in = left_i.charges[lb_in];
         uint32_t ls_in = left_i.sizes[lb_in];
         TList& tl = tlists[lb_in];
         // T work list (rshtm_t_tasks)
-        for (size_t s = 0; s < phys_i.size(); ++s) {
-            Charge rc_ket = lc_in + phys_i.charges[s];
-            size_t rb_ket = right_i.position(rc_ket);
-            if (rb_ket == right_i.size()) continue;
+        for (size_t s = 0; s < NP; ++s) {
+            size_t rb_ket = pos_lp[lb_in * NP + s];
+            if (rb_ket == NR) continue;
             uint32_t rs_ket = right_i.sizes[rb_ket];
-            uint64_t ket_offset = rpb(phys_i.charges[s], rc_ket);
-            for (size_t rb_bra = 0; rb_bra < right_i.size(); ++rb_bra) {
-                uint32_t ci = R.cohort_index(rc_ket, right_i.charges[rb_bra]);
+            uint64_t ket_offset = pb_off[s * NR + rb_ket];
+            for (size_t rb_bra = 0; rb_bra < NR; ++rb_bra) {
+                uint32_t ci = r_ci[rb_ket * NR + rb_bra];
                 if (ci == R.n_cohorts()) continue;
-                uint32_t ci_eff = R.tr[ci] ? R.cohort_index(right_i.charges[rb_bra], rc_ket) : ci;
+                uint32_t ci_eff = R.tr[ci] ? r_ci[rb_bra * NR + rb_ket] : ci;
                 require(ci_eff < R.n_cohorts(), "hpsi: transposed right cohort without partner");
                 for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) tl.add(ket_offset + ss * rs_ket, ci, ci_eff, lb_in);
             }
         }
         // cohorts (shtm_tasks)
-        for (uint32_t lb_out = 0; lb_out < left_i.size(); ++lb_out) {
+        for (uint32_t lb_out = 0; lb_out < NL; ++lb_out) {
             Charge lc_out = left_i.charges[lb_out];
             uint32_t ls_out = left_i.sizes[lb_out];
             uint32_t ci = L.cohort_index(lc_out, lc_in);
@@ -223,37 +253,34 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
             require(ci_eff < L.n_cohorts(), "hpsi: transposed left cohort without partner");
             CohRec rec{lb_in, lb_out, ci, ci_eff, L.nblk[ci_eff], CohortAcc(phys_i, 0)};
             CohortAcc& coh = rec.acc;
-            for (size_t s = 0; s < phys_i.size(); ++s) {
-                Charge phys_out = phys_i.charges[s];
-                Charge rc_out = lc_out + phys_out;
-                size_t rb_out = right_i.position(rc_out);
-                if (rb_out == right_i.size()) continue;
+            for (size_t s = 0; s < NP; ++s) {
+                size_t rb_out = pos_lp[lb_out * NP + s];
+                if (rb_out == NR) continue;
+                Charge rc_out = right_i.charges[rb_out];
                 uint32_t rs_out = right_i.sizes[rb_out];
-                uint64_t out_offset = rpb(phys_out, rc_out);
-                coh.add_unit(s, phys_i.sizes[s], rs_out, out_offset);
+                coh.add_unit(s, phys_i.sizes[s], rs_out, pb_off[s * NR + rb_out]);
                 std::unique_ptr<Wigner9jCache> w9j;
                 if (su2) w9j.reset(new Wigner9jCache(spin_of(lc_in), spin_of(lc_out), spin_of(rc_out)));
 
-                for (uint32_t b1 = 0; b1 < mpo.row_dim; ++b1) {
-                    if (!L.has_block(ci, b1)) continue;
+                for (uint32_t b1 : l_rows[ci]) {
                     uint32_t left_idx = uint32_t(L.offset(ci, b1) / (int64_t(ls_in) * ls_out));
                     int A = mpo.left_spin[b1];
                     if (su2 && !triangle(spin_of(lc_in), A, spin_of(lc_out))) continue;
                     for (auto const& cell : mpo.rows[b1]) {
                         uint32_t b2 = cell.other;
                         for (uint32_t t = cell.t0; t < cell.t1; ++t) {
-                            SiteOp const& W = mpo.ops[mpo.terms[t].op];
+                            uint32_t op = mpo.terms[t].op;
+                            SiteOp const& W = mpo.ops[op];
                             int K = W.spin, Ap = mpo.right_spin[b2];
-                            for (auto const& blk : W.blocks) {
-                                if (phys_out != blk.rc) continue;
-                                Charge phys_in = blk.lc;
-                                Charge rc_in = lc_in + phys_in;
-                                uint32_t ci_right = R.cohort_index(rc_in, rc_out);
+                            for (BlkRef const& br : blk_by_out[size_t(op) * NP + s]) {
+                                OpBlock const& blk = W.blocks[br.blk];
+                                size_t rb_in = pos_lp[lb_in * NP + br.s_in];
+                                if (rb_in == NR) continue;
+                                uint32_t ci_right = r_ci[rb_in * NR + rb_out];
                                 if (!R.has_block(ci_right, b2)) continue;
-                                size_t rb_in = right_i.position(rc_in);
-                                if (rb_in == right_i.size()) continue;
+                                Charge rc_in = right_i.charges[rb_in];
                                 uint32_t rs_in = right_i.sizes[rb_in];
-                                uint64_t in_offset = rpb(phys_in, rc_in);
+                                uint64_t in_offset = pb_off[size_t(br.s_in) * NR + rb_in];
                                 int64_t right_offset = R.offset(ci_right, b2);
                                 double cpl[4];
                                 double scale = R.conjugate_scale(ci_right, b2) * mpo.terms[t].scale *
@@ -270,9 +297,17 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                     coh.add_line(left_idx);
                 }
             }
-            if (coh.n_terms) cohorts.push_back(std::move(rec));
+            if (coh.n_terms) coh_by_in[lb_in].push_back(std::move(rec));
+        }
+        } catch (std::exception const& e) {
+#pragma omp critical
+            err = e.what();
         }
     }
+    if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
+    std::vector<CohRec> cohorts;
+    for (auto& v : coh_by_in)
+        for (auto& c : v) cohorts.push_back(std::move(c));
 
     // --- T buffer layout (only tasks referenced by some alpha are executed) ---
     for (uint32_t lb_in = 0; lb_in < left_i.size(); ++lb_in) {
@@ -395,46 +430,75 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
     ProductBasis bra_lpb = ProductBasis::left_paired(phys, bra.left);
     const bool su2 = sym == Sym::SU2U1;
 
+    const size_t NP = phys.size(), NKL = left_i.size(), NKR = ket.right.size(), NBL = bra.left.size(),
+                 NBR = bra.right.size();
+    // lookup tables built once per step
+    std::vector<uint32_t> kl_of(NKR * NP), bl_of(NBR * NP);  // left sector of (right sector - sigma)
+    std::vector<uint64_t> kpb(NP * NKR), bpb(NP * NBL);
+    for (size_t r = 0; r < NKR; ++r)
+        for (size_t x = 0; x < NP; ++x) {
+            kl_of[r * NP + x] = uint32_t(left_i.position(ket.right.charges[r] - phys.charges[x]));
+            kpb[x * NKR + r] = ket_rpb(phys.charges[x], ket.right.charges[r]);
+        }
+    for (size_t r = 0; r < NBR; ++r)
+        for (size_t x = 0; x < NP; ++x) bl_of[r * NP + x] = uint32_t(bra.left.position(bra.right.charges[r] - phys.charges[x]));
+    for (size_t l = 0; l < NBL; ++l)
+        for (size_t x = 0; x < NP; ++x) bpb[x * NBL + l] = bra_lpb(phys.charges[x], bra.left.charges[l]);
+    std::vector<uint32_t> l_ci_kk(NKL * NKL), l_ci_bk(NBL * NKL);
+    for (size_t x = 0; x < NKL; ++x)
+        for (size_t y = 0; y < NKL; ++y) l_ci_kk[x * NKL + y] = L.cohort_index(left_i.charges[x], left_i.charges[y]);
+    for (size_t x = 0; x < NBL; ++x)
+        for (size_t y = 0; y < NKL; ++y) l_ci_bk[x * NKL + y] = L.cohort_index(bra.left.charges[x], left_i.charges[y]);
+    struct BlkRef { uint32_t blk; uint32_t s_in; };
+    std::vector<std::vector<BlkRef>> blk_by_out(mpo.ops.size() * NP);
+    for (size_t o = 0; o < mpo.ops.size(); ++o)
+        for (size_t bi = 0; bi < mpo.ops[o].blocks.size(); ++bi) {
+            OpBlock const& blk = mpo.ops[o].blocks[bi];
+            size_t so = phys.position(blk.rc), si = phys.position(blk.lc);
+            if (so == NP || si == NP) continue;
+            blk_by_out[o * NP + so].push_back(BlkRef{uint32_t(bi), uint32_t(si)});
+        }
+
     struct CohRec {
         uint32_t rb_bra, rb_ket;
         CohortAcc acc;
         std::vector<int64_t> offs;
         uint32_t nrows;
     };
-    std::vector<CohRec> cohorts;
-    std::vector<TList> tlists(ket.right.size());
+    std::vector<std::vector<CohRec>> coh_by(NKR);
+    std::vector<TList> tlists(NKR);
+    std::string err;
 
-    for (uint32_t rb_ket = 0; rb_ket < ket.right.size(); ++rb_ket) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint32_t rb_ket = 0; rb_ket < NKR; ++rb_ket) {
+        try {
         Charge rc_ket = ket.right.charges[rb_ket];
         uint32_t rs_ket = ket.right.sizes[rb_ket];
         TList& tl = tlists[rb_ket];
-        for (size_t s = 0; s < phys.size(); ++s) {
-            Charge lc_ket = rc_ket - phys.charges[s];
-            size_t lb_ket = left_i.position(lc_ket);
-            if (lb_ket == left_i.size()) continue;
-            uint64_t ket_offset = ket_rpb(phys.charges[s], rc_ket);
-            for (size_t lb_bra = 0; lb_bra < left_i.size(); ++lb_bra) {
-                uint32_t ci = L.cohort_index(left_i.charges[lb_bra], lc_ket);
+        for (size_t s = 0; s < NP; ++s) {
+            size_t lb_ket = kl_of[rb_ket * NP + s];
+            if (lb_ket == NKL) continue;
+            uint64_t ket_offset = kpb[s * NKR + rb_ket];
+            for (size_t lb_bra = 0; lb_bra < NKL; ++lb_bra) {
+                uint32_t ci = l_ci_kk[lb_bra * NKL + lb_ket];
                 if (ci == L.n_cohorts()) continue;
-                uint32_t ci_eff = L.tr[ci] ? L.cohort_index(lc_ket, left_i.charges[lb_bra]) : ci;
+                uint32_t ci_eff = L.tr[ci] ? l_ci_kk[lb_ket * NKL + lb_bra] : ci;
                 require(ci_eff < L.n_cohorts(), "lstep: transposed cohort without partner");
                 for (uint32_t ss = 0; ss < phys.sizes[s]; ++ss)
                     tl.add(ket_offset + ss * rs_ket, ci, ci_eff, uint32_t(lb_ket));
             }
         }
-        for (uint32_t rb_bra = 0; rb_bra < bra.right.size(); ++rb_bra) {
+        for (uint32_t rb_bra = 0; rb_bra < NBR; ++rb_bra) {
             Charge rc_bra = bra.right.charges[rb_bra];
             if (std::abs(particle_number(sym, rc_bra) - particle_number(sym, rc_ket)) > 2) continue;
             CohRec rec{rb_bra, rb_ket, CohortAcc(phys, mpo.col_dim), {}, 0};
             CohortAcc& coh = rec.acc;
-            for (size_t s = 0; s < phys.size(); ++s) {
-                Charge phys_out = phys.charges[s];
-                Charge lc_bra = rc_bra - phys_out;
-                size_t lb_bra = bra.left.position(lc_bra);
-                if (lb_bra == bra.left.size()) continue;
+            for (size_t s = 0; s < NP; ++s) {
+                size_t lb_bra = bl_of[rb_bra * NP + s];
+                if (lb_bra == NBL) continue;
+                Charge lc_bra = bra.left.charges[lb_bra];
                 uint32_t ls_bra = bra.left.sizes[lb_bra];
-                uint64_t bra_offset = bra_lpb(phys_out, lc_bra);
-                coh.add_unit(s, phys.sizes[s], ls_bra, bra_offset);
+                coh.add_unit(s, phys.sizes[s], ls_bra, bpb[s * NBL + lb_bra]);
                 for (uint32_t b2 = 0; b2 < mpo.col_dim; ++b2) {
                     if (symmetric && mpo.right_herm.skip(b2, rc_bra, rc_ket)) continue;
                     int Ap = mpo.right_spin[b2];
@@ -442,18 +506,18 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
                     for (auto const& cell : mpo.cols[b2]) {
                         uint32_t b1 = cell.other;
                         for (uint32_t t = cell.t0; t < cell.t1; ++t) {
-                            SiteOp const& W = mpo.ops[mpo.terms[t].op];
+                            uint32_t op = mpo.terms[t].op;
+                            SiteOp const& W = mpo.ops[op];
                             int K = W.spin, A = mpo.left_spin[b1];
-                            for (auto const& blk : W.blocks) {
-                                if (phys_out != blk.rc) continue;
-                                Charge phys_in = blk.lc;
-                                Charge lc_ket = rc_ket - phys_in;
-                                size_t lb_ket = left_i.position(lc_ket);
-                                if (lb_ket == left_i.size()) continue;
-                                uint32_t ci = L.cohort_index(lc_bra, lc_ket);
+                            for (BlkRef const& br : blk_by_out[size_t(op) * NP + s]) {
+                                OpBlock const& blk = W.blocks[br.blk];
+                                size_t lb_ket = kl_of[rb_ket * NP + br.s_in];
+                                if (lb_ket == NKL) continue;
+                                uint32_t ci = l_ci_bk[lb_bra * NKL + lb_ket];
                                 if (!L.has_block(ci, b1)) continue;
+                                Charge lc_ket = left_i.charges[lb_ket];
                                 uint32_t ls_ket = left_i.sizes[lb_ket];
-                                uint64_t ket_offset = ket_rpb(phys_in, rc_ket);
+                                uint64_t ket_offset = kpb[size_t(br.s_in) * NKR + rb_ket];
                                 int64_t left_offset = L.offset(ci, b1);
                                 double cpl[4];
                                 double scale = L.conjugate_scale(ci, b1) * mpo.terms[t].scale;
@@ -481,9 +545,17 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
             rec.nrows = cnt;
             for (auto& u : coh.units)
                 for (auto& b : u.line_b) b = pos[b];
-            cohorts.push_back(std::move(rec));
+            coh_by[rb_ket].push_back(std::move(rec));
+        }
+        } catch (std::exception const& e) {
+#pragma omp critical
+            err = e.what();
         }
     }
+    if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
+    std::vector<CohRec> cohorts;
+    for (auto& v : coh_by)
+        for (auto& c : v) cohorts.push_back(std::move(c));
 
     // new boundary index (move_boundary.hpp:202-208)
     P.out = BoundaryIndex(bra.right, ket.right, mpo.col_dim);

@@ -16,7 +16,7 @@ _lib = None
 def build():
     srcs = [os.path.join(HERE, "plan_emulator.cpp")] + [os.path.join(CSRC, f) for f in
                                                          ("plans.cpp", "coupling.cpp", "descriptors.cpp")]
-    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
                     "-o", LIB + ".tmp"] + srcs, check=True)
     os.replace(LIB + ".tmp", LIB)
 
