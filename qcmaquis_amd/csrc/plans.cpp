@@ -653,46 +653,80 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
     ProductBasis bra_rpb = ProductBasis::right_paired(phys, bra.right);
     const bool su2 = sym == Sym::SU2U1;
 
+    const size_t NP = phys.size(), NKL = ket.left.size(), NKR = ket.right.size(), NBL = bra.left.size(),
+                 NBR = bra.right.size();
+    std::vector<uint32_t> kr_of(NKL * NP), br_of(NBL * NP);  // right sector of (left sector + sigma)
+    std::vector<uint64_t> kpb(NKL * NP), bpb(NBL * NP);
+    for (size_t l = 0; l < NKL; ++l)
+        for (size_t x = 0; x < NP; ++x) {
+            Charge rc = ket.left.charges[l] + phys.charges[x];
+            size_t r = ket.right.position(rc);
+            kr_of[l * NP + x] = uint32_t(r);
+            kpb[l * NP + x] = r < NKR ? ket_rpb(phys.charges[x], rc) : 0;
+        }
+    for (size_t l = 0; l < NBL; ++l)
+        for (size_t x = 0; x < NP; ++x) {
+            Charge rc = bra.left.charges[l] + phys.charges[x];
+            size_t r = bra.right.position(rc);
+            br_of[l * NP + x] = uint32_t(r);
+            bpb[l * NP + x] = r < NBR ? bra_rpb(phys.charges[x], rc) : 0;
+        }
+    std::vector<uint32_t> r_ci_kb(NKR * NBR), r_ci_bk(NBR * NKR);
+    for (size_t x = 0; x < NKR; ++x)
+        for (size_t y = 0; y < NBR; ++y) {
+            r_ci_kb[x * NBR + y] = R.cohort_index(ket.right.charges[x], bra.right.charges[y]);
+            r_ci_bk[y * NKR + x] = R.cohort_index(bra.right.charges[y], ket.right.charges[x]);
+        }
+    struct BlkRef { uint32_t blk; uint32_t s_in; };
+    std::vector<std::vector<BlkRef>> blk_by_out(mpo.ops.size() * NP);
+    for (size_t o = 0; o < mpo.ops.size(); ++o)
+        for (size_t bi = 0; bi < mpo.ops[o].blocks.size(); ++bi) {
+            OpBlock const& blk = mpo.ops[o].blocks[bi];
+            size_t so = phys.position(blk.rc), si = phys.position(blk.lc);
+            if (so == NP || si == NP) continue;
+            blk_by_out[o * NP + so].push_back(BlkRef{uint32_t(bi), uint32_t(si)});
+        }
+
     struct CohRec {
         uint32_t lb_ket, lb_bra;
         CohortAcc acc;
         std::vector<int64_t> offs;
         uint32_t nrows;
     };
-    std::vector<CohRec> cohorts;
-    std::vector<TList> tlists(ket.left.size());
+    std::vector<std::vector<CohRec>> coh_by(NKL);
+    std::vector<TList> tlists(NKL);
+    std::string err;
 
-    for (uint32_t lb_ket = 0; lb_ket < ket.left.size(); ++lb_ket) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (uint32_t lb_ket = 0; lb_ket < NKL; ++lb_ket) {
+        try {
         Charge lc_ket = ket.left.charges[lb_ket];
         uint32_t ls_ket = ket.left.sizes[lb_ket];
         TList& tl = tlists[lb_ket];
-        for (size_t s = 0; s < phys.size(); ++s) {
-            Charge rc_ket = lc_ket + phys.charges[s];
-            size_t rb_ket = ket.right.position(rc_ket);
-            if (rb_ket == ket.right.size()) continue;
+        for (size_t s = 0; s < NP; ++s) {
+            size_t rb_ket = kr_of[lb_ket * NP + s];
+            if (rb_ket == NKR) continue;
             uint32_t rs_ket = ket.right.sizes[rb_ket];
-            uint64_t ket_offset = ket_rpb(phys.charges[s], rc_ket);
-            for (size_t rb_bra = 0; rb_bra < bra.right.size(); ++rb_bra) {
-                uint32_t ci = R.cohort_index(rc_ket, bra.right.charges[rb_bra]);
+            uint64_t ket_offset = kpb[lb_ket * NP + s];
+            for (size_t rb_bra = 0; rb_bra < NBR; ++rb_bra) {
+                uint32_t ci = r_ci_kb[rb_ket * NBR + rb_bra];
                 if (ci == R.n_cohorts()) continue;
-                uint32_t ci_eff = R.tr[ci] ? R.cohort_index(bra.right.charges[rb_bra], rc_ket) : ci;
+                uint32_t ci_eff = R.tr[ci] ? r_ci_bk[rb_bra * NKR + rb_ket] : ci;
                 require(ci_eff < R.n_cohorts(), "rstep: transposed cohort without partner");
                 for (uint32_t ss = 0; ss < phys.sizes[s]; ++ss) tl.add(ket_offset + ss * rs_ket, ci, ci_eff, lb_ket);
             }
         }
-        for (uint32_t lb_bra = 0; lb_bra < bra.left.size(); ++lb_bra) {
+        for (uint32_t lb_bra = 0; lb_bra < NBL; ++lb_bra) {
             Charge lc_bra = bra.left.charges[lb_bra];
             if (std::abs(particle_number(sym, lc_ket) - particle_number(sym, lc_bra)) > 2) continue;
             CohRec rec{lb_ket, lb_bra, CohortAcc(phys, mpo.row_dim), {}, 0};
             CohortAcc& coh = rec.acc;
-            for (size_t s = 0; s < phys.size(); ++s) {
-                Charge phys_out = phys.charges[s];
-                Charge rc_bra = lc_bra + phys_out;
-                size_t rb_bra = bra.right.position(rc_bra);
-                if (rb_bra == bra.right.size()) continue;
+            for (size_t s = 0; s < NP; ++s) {
+                size_t rb_bra = br_of[lb_bra * NP + s];
+                if (rb_bra == NBR) continue;
+                Charge rc_bra = bra.right.charges[rb_bra];
                 uint32_t rs_bra = bra.right.sizes[rb_bra];
-                uint64_t bra_offset = bra_rpb(phys_out, rc_bra);
-                coh.add_unit(s, phys.sizes[s], rs_bra, bra_offset);
+                coh.add_unit(s, phys.sizes[s], rs_bra, bpb[lb_bra * NP + s]);
                 std::unique_ptr<Wigner9jCache> w9j;
                 if (su2) w9j.reset(new Wigner9jCache(spin_of(lc_ket), spin_of(lc_bra), spin_of(rc_bra)));
                 for (uint32_t b1 = 0; b1 < mpo.row_dim; ++b1) {
@@ -702,18 +736,18 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
                     for (auto const& cell : mpo.rows[b1]) {
                         uint32_t b2 = cell.other;
                         for (uint32_t t = cell.t0; t < cell.t1; ++t) {
-                            SiteOp const& W = mpo.ops[mpo.terms[t].op];
+                            uint32_t op = mpo.terms[t].op;
+                            SiteOp const& W = mpo.ops[op];
                             int K = W.spin, Ap = mpo.right_spin[b2];
-                            for (auto const& blk : W.blocks) {
-                                if (phys_out != blk.rc) continue;
-                                Charge phys_in = blk.lc;
-                                Charge rc_ket = lc_ket + phys_in;
-                                uint32_t ci = R.cohort_index(rc_ket, rc_bra);
+                            for (BlkRef const& br : blk_by_out[size_t(op) * NP + s]) {
+                                OpBlock const& blk = W.blocks[br.blk];
+                                size_t rb_ket = kr_of[lb_ket * NP + br.s_in];
+                                if (rb_ket == NKR) continue;
+                                uint32_t ci = r_ci_kb[rb_ket * NBR + rb_bra];
                                 if (!R.has_block(ci, b2)) continue;
-                                size_t rb_ket = ket.right.position(rc_ket);
-                                if (rb_ket == ket.right.size()) continue;
+                                Charge rc_ket = ket.right.charges[rb_ket];
                                 uint32_t rs_ket = ket.right.sizes[rb_ket];
-                                uint64_t ket_offset = ket_rpb(phys_in, rc_ket);
+                                uint64_t ket_offset = kpb[lb_ket * NP + br.s_in];
                                 int64_t right_offset = R.offset(ci, b2);
                                 double cpl[4];
                                 double scale = R.conjugate_scale(ci, b2) * mpo.terms[t].scale;
@@ -738,9 +772,17 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
             rec.nrows = cnt;
             for (auto& u : coh.units)
                 for (auto& b : u.line_b) b = pos[b];
-            cohorts.push_back(std::move(rec));
+            coh_by[lb_ket].push_back(std::move(rec));
+        }
+        } catch (std::exception const& e) {
+#pragma omp critical
+            err = e.what();
         }
     }
+    if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
+    std::vector<CohRec> cohorts;
+    for (auto& v : coh_by)
+        for (auto& c : v) cohorts.push_back(std::move(c));
 
     // new boundary index (move_boundary.hpp:332-338): complement_transpose is always applied
     P.out = BoundaryIndex(ket.left, bra.left, mpo.row_dim);
