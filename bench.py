@@ -91,6 +91,31 @@ def qc_workload(ctx, L=24, M=512, seed=1002, warm_sweeps=1):
                 sym=abi.MQ_SYM_TWOU1, desc=desc, keep=(d, ham))
 
 
+class ProfilerRegion:
+    """roctxProfilerResume/Pause around the timed region: under `rocprofv3 --selected-regions` only
+    the kernels of the timed H*psi calls are traced; without a profiler the calls do nothing."""
+
+    def __init__(self):
+        import ctypes as C
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = C.CDLL(os.path.join("/opt/rocm/lib", name))
+                break
+            except OSError:
+                continue
+
+    def __enter__(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+        return self
+
+    def __exit__(self, *exc):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+        return False
+
+
 def cpu_baseline(w, flops, budget_s=30.0):
     """the oracle (restated reference CPU path) on the same site problem, all host cores"""
     import oracle
@@ -123,6 +148,8 @@ def main():
     ap.add_argument("--workload", default="qc", choices=["qc", "synthetic"])
     ap.add_argument("--warm-sweeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy inside the profiler region (HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the site problem on a single MI355X "
@@ -134,6 +161,15 @@ def main():
     from qcmaquis_amd.engine import Context, HpsiPlan
 
     torch.cuda.init()
+    if args.copy_calibration:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        with ProfilerRegion():
+            b.copy_(a)
+            torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}), flush=True)
+        return
     ctx = Context(0, abi.MQ_SYM_TWOU1)
     t0 = time.perf_counter()
     w = qc_workload(ctx, warm_sweeps=args.warm_sweeps) if args.workload == "qc" else synthetic_workload(ctx)
@@ -141,7 +177,8 @@ def main():
     plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
     st = plan.stats()
     log("plan:", {k: st[k] for k in ("n_t_tasks", "n_cohorts", "n_lines", "n_terms", "flops_total", "flops_gemm",
-                                      "flops_gemm_exec", "bytes_compulsory", "build_seconds")})
+                                      "flops_gemm_exec", "bytes_compulsory", "t_elements", "s_elements",
+                                      "s_line_elements", "gathered_elements", "build_seconds")})
     n = w["layout"].n
     psi = torch.randn(n, dtype=torch.float64, device="cuda")
     out = torch.empty_like(psi)
@@ -151,28 +188,40 @@ def main():
         plan.apply_device(psi.data_ptr(), out.data_ptr())
     ctx.sync()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.apply_device(psi.data_ptr(), out.data_ptr())
-    ctx.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = wall / args.steps * 1e3
-    log("timed %d H*psi: %.3f ms each" % (args.steps, ms))
-
-    # per-stage HIP-event timing on the launch stream (separate pass, synchronous per step)
     import ctypes as C
     stage = np.zeros(4)
     sms = (C.c_float * 4)()
     nrep = max(3, min(args.steps, 10))
-    for _ in range(nrep):
-        ctx.check(ctx.lib.mq_hpsi_timed(plan.h, C.c_void_p(psi.data_ptr()), C.c_void_p(out.data_ptr()), 0, sms))
-        stage += np.array(list(sms))
+    with ProfilerRegion():
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.apply_device(psi.data_ptr(), out.data_ptr())
+        ctx.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        # per-stage HIP-event timing on the launch stream (separate pass, synchronous per step)
+        for _ in range(nrep):
+            ctx.check(ctx.lib.mq_hpsi_timed(plan.h, C.c_void_p(psi.data_ptr()), C.c_void_p(out.data_ptr()), 0, sms))
+            stage += np.array(list(sms))
     stage /= nrep
+    ms = wall / args.steps * 1e3
+    log("timed %d H*psi: %.3f ms each" % (args.steps, ms))
     log("stage ms (fill, T-GEMM, S-assembly, L*S GEMM):", np.round(stage, 4).tolist())
-    gemm_ms = stage[1] + stage[3]
-    achieved = st["flops_gemm_exec"] / (gemm_ms * 1e-3) / 1e12
     value = st["flops_total"] / (ms * 1e-3) / 1e12
+    gemm_ms = stage[1] + stage[3]
+    gemm_tf = st["flops_gemm_exec"] / (gemm_ms * 1e-3) / 1e12
+    # S-assembly: every term reads its T tile, every line writes its S tile (FP64)
+    s_bytes = 8 * (st["gathered_elements"] + st["s_line_elements"])
+    s_gbs = s_bytes / (stage[2] * 1e-3) / 1e9
+    rl_gemm = {"bound": "mfma", "kernel": "gemm_f64_kernel (T-GEMM + L*S GEMM launches)",
+               "achieved": round(gemm_tf, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+               "frac": round(gemm_tf / PEAK_FP64_TFLOPS, 5), "traffic": None,
+               "work_per_launch": "%d FP64 flops" % st["flops_gemm_exec"]}
+    rl_sasm = {"bound": "hbm", "kernel": "sassembly_kernel", "achieved": round(s_gbs, 2), "peak": PEAK_HBM_GBS,
+               "unit": "GB/s", "frac": round(s_gbs / PEAK_HBM_GBS, 5), "traffic": None,
+               "work_per_launch": "%d bytes (%d terms: T tile reads %d el, S writes %d el)" % (
+                   s_bytes, st["n_terms"], st["gathered_elements"], st["s_line_elements"])}
+    dominant, other = (rl_sasm, rl_gemm) if stage[2] >= gemm_ms else (rl_gemm, rl_sasm)
     line = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -190,9 +239,8 @@ def main():
                        flops_per_step_reference_model=st["flops_total"], gemm_flops_executed=st["flops_gemm_exec"],
                        stage_ms={"s_clear": round(stage[0], 4), "t_gemm": round(stage[1], 4),
                                  "s_assembly": round(stage[2], 4), "ls_gemm": round(stage[3], 4)}),
-        "roofline": {"bound": "mfma", "kernel": "gemm_f64_kernel (T-GEMM + L*S GEMM launches)",
-                     "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": None},
+        "roofline": dominant,
+        "roofline_other": other,
     }
     if not args.no_cpu:
         try:

@@ -125,6 +125,8 @@ typedef struct {
     uint64_t bytes_compulsory;        /* FP64 bytes */
     uint64_t t_elements, s_elements, l_panel_elements, r_tr_elements;
     double   build_seconds;
+    uint64_t s_line_elements;         /* sum over S lines of rows * cols (S elements written) */
+    uint64_t gathered_elements;       /* sum over S terms of rows * cols (T elements read) */
 } mq_hpsi_stats_t;
 
 typedef struct {

@@ -90,6 +90,8 @@ class HpsiStats(C.Structure):
         ("l_panel_elements", C.c_uint64),
         ("r_tr_elements", C.c_uint64),
         ("build_seconds", C.c_double),
+        ("s_line_elements", C.c_uint64),
+        ("gathered_elements", C.c_uint64),
     ]
 
 

@@ -178,6 +178,7 @@ struct mq_hpsi_plan_s {
     SLine* lines = nullptr;
     STerm* terms = nullptr;
     int64_t n_lines = 0;
+    uint64_t s_line_elements = 0, gathered_elements = 0;
     double *T = nullptr, *S = nullptr, *Lpanel = nullptr;
     double build_seconds = 0;
 };
@@ -508,6 +509,11 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
         hipcheck(e, "L panel copy");
         hipcheck(e2, "L panel copy");
         // the host tables are on the device now; keep only what the statistics need
+        for (SLine const& l : H.lines) {
+            uint64_t e = uint64_t(l.rows) * uint64_t(l.cols);
+            p->s_line_elements += e;
+            p->gathered_elements += e * uint64_t(l.t1 - l.t0);
+        }
         H.terms.clear(); H.terms.shrink_to_fit();
         H.lines.clear(); H.lines.shrink_to_fit();
         p->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -532,6 +538,8 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
     s->l_panel_elements = H.lpanel_elems;
     s->r_tr_elements = 0;
     s->build_seconds = p->build_seconds;
+    s->s_line_elements = p->s_line_elements;
+    s->gathered_elements = p->gathered_elements;
     return MQ_OK;
 }
 
