@@ -58,7 +58,7 @@ def synthetic_workload(ctx, seed=7):
     return dict(mpo=mpos[mid], mpo_spec=ch.mpos[mid], L=L, R=R, layout=lay, sym=abi.MQ_SYM_TWOU1, desc=desc)
 
 
-def qc_workload(ctx, L=24, M=512, seed=1002, warm_sweeps=1):
+def qc_workload(ctx, L=24, M=512, seed=1002, warm_sweeps=1, dump_bonds=None):
     """Configuration 2: 24 orbitals, 24 electrons (S_z = 0), TwoU1, M = 512, seeded integrals
     (PCG64(1002), L-rank 16).  Two-site DMRG from a random MPS: `warm_sweeps` full sweeps at
     M = 512, then the left-to-right half sweep up to the middle two-site problem, whose H*psi is
@@ -79,6 +79,10 @@ def qc_workload(ctx, L=24, M=512, seed=1002, warm_sweeps=1):
     mid = L // 2 - 1
     for i in range(0, mid):
         d.step(i, True)
+    if dump_bonds:
+        bonds = [d.lay[k].left for k in range(L)] + [d.lay[-1].right]
+        json.dump([{"charges": [list(map(int, q)) for q in b.charges], "sizes": [int(x) for x in b.sizes]}
+                   for b in bonds], open(dump_bonds, "w"))
     lay2, psi2 = merge_two(d.lay[mid], d.data[mid], d.lay[mid + 1], d.data[mid + 1], d.phys2, d.off12)
     mpo = d.ts_mpo(mid)
     desc = {"workload": f"configuration 2: {L} orbitals, {L} electrons, TwoU1, M={M}, seeded FCIDUMP "
@@ -148,6 +152,7 @@ def main():
     ap.add_argument("--workload", default="qc", choices=["qc", "synthetic"])
     ap.add_argument("--warm-sweeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dump-bonds", default=None, help="write the MPS bond indices of the workload (JSON)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy inside the profiler region (HBM counter scale)")
     args = ap.parse_args()
@@ -172,7 +177,7 @@ def main():
         return
     ctx = Context(0, abi.MQ_SYM_TWOU1)
     t0 = time.perf_counter()
-    w = qc_workload(ctx, warm_sweeps=args.warm_sweeps) if args.workload == "qc" else synthetic_workload(ctx)
+    w = qc_workload(ctx, warm_sweeps=args.warm_sweeps, dump_bonds=args.dump_bonds) if args.workload == "qc" else synthetic_workload(ctx)
     log("workload built in %.1f s: %s" % (time.perf_counter() - t0, w["desc"]))
     plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
     st = plan.stats()

@@ -1,5 +1,6 @@
 #include "plans.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <memory>
 #include <unordered_map>
@@ -399,6 +400,12 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
         P.flops.flops_gemm_exec += g;
         P.flops.n_terms += rec.acc.n_terms;
     }
+    // Execution order of the lines: by the T position of their first term, so that lines reading
+    // the T tasks of one input block run together and those tasks stay cache-resident (each line
+    // owns a disjoint S tile, so the order does not change any result).
+    std::stable_sort(P.lines.begin(), P.lines.end(), [&](SLine const& a, SLine const& b) {
+        return P.terms[a.t0].t_off < P.terms[b.t0].t_off;
+    });
     P.n_cohorts = cohorts.size();
     for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
         uint32_t ls = left_i.sizes[lb];

@@ -1,0 +1,78 @@
+"""Summaries of rocprofv3 CSV output for the contraction kernels.
+
+    python -m qcmaquis_amd.profiling KERNEL_TRACE_OR_COUNTER_CSV [--last N] [--scale COUNTER=F ...]
+
+Groups the dispatches of a ``*_kernel_trace.csv`` or ``*_counter_collection.csv`` by kernel name,
+keeps the last N dispatches of each kernel (the timed region of a benchmark is the end of its
+run: a workload set-up that launches the same kernels comes first) and prints a markdown table of
+their count, mean / min / max duration and, for counter files, the mean counter value (KiB for
+FETCH_SIZE / WRITE_SIZE) multiplied by the given calibration factor.
+"""
+import argparse
+import collections
+import csv
+import re
+import sys
+
+
+def short_name(name):
+    m = re.search(r"([A-Za-z_0-9]+)\(", name)
+    base = m.group(1) if m else name
+    return base if base != "kernel" else name[:60]
+
+
+def load(path):
+    rows = list(csv.DictReader(open(path)))
+    out = collections.OrderedDict()
+    for r in rows:
+        out.setdefault(r["Kernel_Name"], []).append(r)
+    return out
+
+
+def summarize(path, last=0, scale=None, match=None):
+    scale = scale or {}
+    table = []
+    for name, rows in load(path).items():
+        if match and not re.search(match, name):
+            continue
+        sel = rows[-last:] if last else rows
+        dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6 for r in sel]
+        ent = {"kernel": short_name(name), "dispatches": len(sel), "of": len(rows),
+               "mean_ms": sum(dur) / len(dur), "min_ms": min(dur), "max_ms": max(dur)}
+        if "Counter_Name" in sel[0]:
+            cn = sel[0]["Counter_Name"]
+            v = [float(r["Counter_Value"]) for r in sel]
+            f = scale.get(cn, 1.0)
+            ent["counter"] = cn
+            ent["mean_kib"] = sum(v) / len(v)
+            ent["scaled_gb"] = ent["mean_kib"] * 1024 * f / 1e9
+        table.append(ent)
+    return table
+
+
+def markdown(table):
+    has_c = any("counter" in t for t in table)
+    head = "| kernel | dispatches (of) | mean ms | min ms | max ms |" + (" counter | mean KiB | scaled GB |" if has_c else "")
+    lines = [head, "|" + "---|" * (head.count("|") - 1)]
+    for t in table:
+        s = "| %s | %d (%d) | %.4f | %.4f | %.4f |" % (t["kernel"], t["dispatches"], t["of"], t["mean_ms"], t["min_ms"],
+                                                    t["max_ms"])
+        if has_c:
+            s += " %s | %.1f | %.4f |" % (t.get("counter", ""), t.get("mean_kib", 0), t.get("scaled_gb", 0))
+        lines.append(s)
+    return "\n".join(lines)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("csv")
+    ap.add_argument("--last", type=int, default=0, help="keep the last N dispatches of each kernel (0 = all)")
+    ap.add_argument("--match", default=None, help="regex on the kernel name")
+    ap.add_argument("--scale", nargs="*", default=[], help="COUNTER=factor calibration, e.g. FETCH_SIZE=2.0")
+    a = ap.parse_args(argv)
+    scale = {k: float(v) for k, v in (s.split("=") for s in a.scale)}
+    print(markdown(summarize(a.csv, a.last, scale, a.match)))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
