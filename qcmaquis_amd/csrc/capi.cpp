@@ -110,43 +110,39 @@ int guarded(mq_ctx_t ctx, F&& f) {
     }
 }
 
-std::vector<GemmTile> make_tiles(std::vector<GemmDesc> const& probs) {
-    std::vector<GemmTile> tiles;
-    std::vector<int32_t> order(probs.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = int32_t(i);
-    // longest reductions first for load balance
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return probs[a].K > probs[b].K; });
-    for (int32_t p : order) {
-        auto const& g = probs[p];
-        int32_t tm = (g.M + GEMM_BM - 1) / GEMM_BM, tn = (g.N + GEMM_BN - 1) / GEMM_BN;
-        for (int32_t b = 0; b < g.batch; ++b)
-            for (int32_t j = 0; j < tn; ++j)
-                for (int32_t i = 0; i < tm; ++i) tiles.push_back(GemmTile{p, b, i, j});
-    }
-    return tiles;
-}
-
-// device copy of a GEMM stage
+// device copy of a grouped GEMM stage and its schedule (tiles per shape, split-K reduce jobs and
+// the workspace of the partial tiles)
 struct GemmStage {
     GemmDesc* probs = nullptr;
-    GemmTile* tiles = nullptr;
-    int64_t n_tiles = 0;
+    GemmTile* tiles[GEMM_N_SHAPES] = {nullptr, nullptr, nullptr};
+    int64_t n_tiles[GEMM_N_SHAPES] = {0, 0, 0};
+    GemmReduce* reduces = nullptr;
+    int64_t n_reduces = 0;
+    double* ws = nullptr;
     void upload(std::vector<GemmDesc> const& v, hipStream_t s) {
-        auto t = make_tiles(v);
+        GemmSchedule sch = build_gemm_schedule(v);
         probs = dev_upload(v, s);
-        tiles = dev_upload(t, s);
-        n_tiles = int64_t(t.size());
+        for (int sh = 0; sh < GEMM_N_SHAPES; ++sh) {
+            tiles[sh] = dev_upload(sch.tiles[sh], s);
+            n_tiles[sh] = int64_t(sch.tiles[sh].size());
+        }
+        reduces = dev_upload(sch.reduces, s);
+        n_reduces = int64_t(sch.reduces.size());
+        ws = dev_alloc<double>(sch.ws_elems);
     }
     void run(Bases const& b, hipStream_t s, int precision) const {
         if (precision != MQ_PREC_FP64)
             throw Error(MQ_ERR_UNSUPPORTED, "only MQ_PREC_FP64 sector GEMMs are available in this build");
-        hipcheck(launch_gemm_f64(probs, tiles, n_tiles, b, s), "gemm launch");
+        hipcheck(launch_gemm_f64(probs, tiles, n_tiles, reduces, n_reduces, ws, b, s), "gemm launch");
     }
     void release() {
         dev_free(probs);
-        dev_free(tiles);
+        for (auto& t : tiles) dev_free(t), t = nullptr;
+        dev_free(reduces);
+        dev_free(ws);
         probs = nullptr;
-        tiles = nullptr;
+        reduces = nullptr;
+        ws = nullptr;
     }
 };
 

@@ -1,9 +1,10 @@
 // gfx950 kernels of the contraction path.
 //
-//  * gemm_f64_kernel: grouped, batched FP64 GEMM over a host-built tile table.  64x64 output tile
-//    per 256-thread workgroup (four wave64s, 32x32 each as 2x2 v_mfma_f64_16x16x4f64), operands
-//    staged through LDS in K-slices of 16 with the next slice prefetched into registers.  Each
-//    output tile is owned by exactly one workgroup, so there are no atomics.
+//  * gemm_f64_kernel: grouped, batched FP64 GEMM over a host-built tile table.  32x128, 64x64 or
+//    128x64 output tiles per 256-thread workgroup (four wave64s in a 2x2 grid, each a block of
+//    v_mfma_f64_16x16x4f64 tiles), operands staged through LDS in K-slices of 16 with the next
+//    slice prefetched into registers.  Each output tile is owned by one workgroup or, when K is
+//    split, by a fixed set of partial tiles summed in order by gemm_reduce_kernel: no atomics.
 //  * sassembly_kernel: the MPO-term sum.  One workgroup per output tile ("line"); the line's
 //    (alpha, T offset) terms are staged in LDS and every thread accumulates its tile elements in
 //    the term order of the reference (create_s / create_s_r, solver/tasks.cpp:425-494).
@@ -22,79 +23,91 @@ constexpr int LDS_PAD = 1;
 
 __device__ inline double ld_or_zero(const double* p, bool ok) { return ok ? *p : 0.0; }
 
+// One BM x BN output tile (BM = 32 FM, BN = 32 FN) per 256-thread workgroup; the four waves form a
+// 2 x 2 grid of 16 FM x 16 FN wave tiles, each FM x FN v_mfma_f64_16x16x4f64 blocks.  The tile
+// reduces over [k0, k1) and writes C, or its partial tile into the workspace (split K).
+template <int FM, int FN>
 __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmDesc* __restrict__ probs,
-                                                       const GemmTile* __restrict__ tiles, Bases bases) {
-    __shared__ double As[BK][GEMM_BM + LDS_PAD];
-    __shared__ double Bs[BK][GEMM_BN + LDS_PAD];
+                                                       const GemmTile* __restrict__ tiles, Bases bases,
+                                                       double* __restrict__ ws) {
+    constexpr int BM = 32 * FM, BN = 32 * FN;
+    constexpr int NA = 2 * FM, NB = 2 * FN;  // operand elements staged per thread and K slice
+    __shared__ double As[BK][BM + LDS_PAD];
+    __shared__ double Bs[BK][BN + LDS_PAD];
 
     const GemmTile tile = tiles[blockIdx.x];
     const GemmDesc g = probs[tile.prob];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int m0 = tile.tm * GEMM_BM, n0 = tile.tn * GEMM_BN;
+    const int m0 = tile.tm * BM, n0 = tile.tn * BN;
     const bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B;
 
     const double* A = bases.p[g.base_a] + g.a + int64_t(tile.batch) * g.sa;
     const double* B = bases.p[g.base_b] + g.b + int64_t(tile.batch) * g.sb;
-    double* C = bases.p[g.base_c] + g.c + int64_t(tile.batch) * g.sc;
 
-    // per-thread load coordinates: 4 elements of the A tile and 4 of the B tile
-    // A tile (64 x 16): non-transposed -> m fastest, transposed -> k fastest
-    int a_m[4], a_k[4], b_k[4], b_n[4];
+    // per-thread load coordinates inside the BM x 16 A slice and the 16 x BN B slice, with the
+    // contiguous dimension of the operand across consecutive threads
+    int a_m[NA], a_k[NA], b_k[NB], b_n[NB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        if (!ta) { a_m[i] = tid & 63; a_k[i] = (tid >> 6) + 4 * i; }
+    for (int i = 0; i < NA; ++i) {
+        if (!ta) { a_m[i] = tid % BM; a_k[i] = tid / BM + (256 / BM) * i; }
         else     { a_k[i] = tid & 15; a_m[i] = (tid >> 4) + 16 * i; }
-        if (!tb) { b_k[i] = tid & 15; b_n[i] = (tid >> 4) + 16 * i; }
-        else     { b_n[i] = tid & 63; b_k[i] = (tid >> 6) + 4 * i; }
     }
-    double ra[4], rb[4];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        if (!tb) { b_k[i] = tid & 15; b_n[i] = (tid >> 4) + 16 * i; }
+        else     { b_n[i] = tid % BN; b_k[i] = tid / BN + (256 / BN) * i; }
+    }
+    const int k_end = tile.k1;
+    double ra[NA], rb[NB];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < NA; ++i) {
             int m = m0 + a_m[i], k = k0 + a_k[i];
-            bool ok = (m < g.M) && (k < g.K);
+            bool ok = (m < g.M) && (k < k_end);
             const double* pa = ta ? A + int64_t(m) * g.lda + k : A + m + int64_t(k) * g.lda;
             ra[i] = ld_or_zero(pa, ok);
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
             int n = n0 + b_n[i], kb = k0 + b_k[i];
-            bool okb = (n < g.N) && (kb < g.K);
+            bool okb = (n < g.N) && (kb < k_end);
             const double* pb = tb ? B + n + int64_t(kb) * g.ldb : B + kb + int64_t(n) * g.ldb;
             rb[i] = ld_or_zero(pb, okb);
         }
     };
 
-    double4_t acc[2][2];
+    double4_t acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
 
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int wm = (wave & 1) * 16 * FM, wn = (wave >> 1) * 16 * FN;
     const int fr = lane & 15, fk = lane >> 4;
 
-    if (g.K > 0) load(0);
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
+    if (tile.k0 < k_end) load(tile.k0);
+    for (int k0 = tile.k0; k0 < k_end; k0 += BK) {
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            As[a_k[i]][a_m[i]] = ra[i];
-            Bs[b_k[i]][b_n[i]] = rb[i];
-        }
+        for (int i = 0; i < NA; ++i) As[a_k[i]][a_m[i]] = ra[i];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) Bs[b_k[i]][b_n[i]] = rb[i];
         __syncthreads();
-        if (k0 + BK < g.K) load(k0 + BK);
+        if (k0 + BK < k_end) load(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
-            double af[2], bf[2];
+            double af[FM], bf[FN];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) af[i] = As[kk + fk][wm + 16 * i + fr];
+            for (int i = 0; i < FM; ++i) af[i] = As[kk + fk][wm + 16 * i + fr];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) bf[j] = Bs[kk + fk][wn + 16 * j + fr];
+            for (int j = 0; j < FN; ++j) bf[j] = Bs[kk + fk][wn + 16 * j + fr];
             // operands swapped: the instruction accumulates the transposed tile C^T = B^T A^T
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < FN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
         }
     }
@@ -102,18 +115,42 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmDesc* __restric
     // Output layout of v_mfma_f64_16x16x4f64 (measured on gfx950): register r of lane l holds
     // D[l / 16 + 4 r][l % 16].  With the swapped operands D = C^T, so lane l holds
     // C[l % 16][l / 16 + 4 r]: consecutive lanes store consecutive rows of a column.
+    double* C = bases.p[g.base_c] + g.c + int64_t(tile.batch) * g.sc;
+    const bool split = tile.ws >= 0;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            int m = m0 + wm + 16 * i + fr;
+        for (int j = 0; j < FN; ++j) {
+            int mm = wm + 16 * i + fr, m = m0 + mm;
             if (m >= g.M) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int n = n0 + wn + 16 * j + fk + 4 * r;
-                if (n < g.N) C[m + int64_t(n) * g.ldc] = acc[i][j][r];
+                int nn = wn + 16 * j + fk + 4 * r, n = n0 + nn;
+                if (n >= g.N) continue;
+                if (split) ws[tile.ws + mm + int64_t(nn) * BM] = acc[i][j][r];
+                else C[m + int64_t(n) * g.ldc] = acc[i][j][r];
             }
         }
+}
+
+// C tile = sum of its split partials, added in split order
+__global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __restrict__ probs,
+                                                          const GemmReduce* __restrict__ jobs, Bases bases,
+                                                          const double* __restrict__ ws) {
+    const GemmReduce job = jobs[blockIdx.x];
+    const GemmDesc g = probs[job.prob];
+    const int bm = gemm_bm(job.shape), bn = gemm_bn(job.shape);
+    const int m0 = job.tm * bm, n0 = job.tn * bn;
+    double* C = bases.p[g.base_c] + g.c + int64_t(job.batch) * g.sc;
+    const int64_t tile_elems = int64_t(bm) * bn;
+    for (int idx = threadIdx.x; idx < bm * bn; idx += 256) {
+        const int mm = idx % bm, nn = idx / bm;
+        const int m = m0 + mm, n = n0 + nn;
+        if (m >= g.M || n >= g.N) continue;
+        double sum = 0.0;
+        for (int s = 0; s < job.nsplit; ++s) sum += ws[job.ws + s * tile_elems + idx];
+        C[m + int64_t(n) * g.ldc] = sum;
+    }
 }
 
 constexpr int SA_THREADS = 256;
@@ -253,12 +290,26 @@ inline unsigned grid_for(uint64_t n, int threads, unsigned cap = 8192) {
 
 }  // namespace
 
-hipError_t launch_gemm_f64(const GemmDesc* probs, const GemmTile* tiles, int64_t n_tiles, Bases bases,
-                           hipStream_t stream) {
-    if (n_tiles <= 0) return hipSuccess;
-    for (int64_t off = 0; off < n_tiles; off += 0x7fffffff) {
-        int64_t cnt = n_tiles - off < 0x7fffffff ? n_tiles - off : 0x7fffffff;
-        hipLaunchKernelGGL(gemm_f64_kernel, dim3(unsigned(cnt)), dim3(256), 0, stream, probs, tiles + off, bases);
+hipError_t launch_gemm_f64(const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
+                           const GemmReduce* reduces, int64_t n_reduces, double* ws, Bases bases, hipStream_t stream) {
+    for (int sh = 0; sh < GEMM_N_SHAPES; ++sh) {
+        for (int64_t off = 0; off < n_tiles[sh]; off += 0x7fffffff) {
+            int64_t cnt = n_tiles[sh] - off < 0x7fffffff ? n_tiles[sh] - off : 0x7fffffff;
+            if (sh == GEMM_32x128)
+                hipLaunchKernelGGL((gemm_f64_kernel<1, 4>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                                   tiles[sh] + off, bases, ws);
+            else if (sh == GEMM_64x64)
+                hipLaunchKernelGGL((gemm_f64_kernel<2, 2>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                                   tiles[sh] + off, bases, ws);
+            else
+                hipLaunchKernelGGL((gemm_f64_kernel<4, 2>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                                   tiles[sh] + off, bases, ws);
+        }
+    }
+    for (int64_t off = 0; off < n_reduces; off += 0x7fffffff) {
+        int64_t cnt = n_reduces - off < 0x7fffffff ? n_reduces - off : 0x7fffffff;
+        hipLaunchKernelGGL(gemm_reduce_kernel, dim3(unsigned(cnt)), dim3(256), 0, stream, probs, reduces + off, bases,
+                           ws);
     }
     return hipGetLastError();
 }

@@ -15,16 +15,10 @@ struct Bases {
     double* p[BASE_COUNT];
 };
 
-// Tile of the grouped GEMM: problem, batch entry and output tile coordinates.
-struct GemmTile {
-    int32_t prob, batch, tm, tn;
-};
-
-constexpr int GEMM_BM = 64;
-constexpr int GEMM_BN = 64;
-
-hipError_t launch_gemm_f64(const GemmDesc* probs, const GemmTile* tiles, int64_t n_tiles, Bases bases,
-                           hipStream_t stream);
+// grouped GEMM stage: tiles of each shape (GemmShape) then the split-K reduce jobs; ws is the
+// workspace of the split partials (GemmSchedule::ws_elems doubles)
+hipError_t launch_gemm_f64(const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
+                           const GemmReduce* reduces, int64_t n_reduces, double* ws, Bases bases, hipStream_t stream);
 hipError_t launch_sassembly(const SLine* lines, int64_t n_lines, const STerm* terms, const double* T, double* S,
                             hipStream_t stream);
 hipError_t launch_fill(double* p, uint64_t n, double v, hipStream_t stream);

@@ -863,6 +863,62 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
     return P;
 }
 
+// ---- grouped GEMM schedule ---------------------------------------------------------------------
+GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t target_items, int32_t min_chunk) {
+    require(min_chunk >= 16 && min_chunk % 16 == 0, "gemm schedule: chunk must be a multiple of 16");
+    GemmSchedule S;
+    auto shape_of = [](GemmDesc const& g) { return g.M <= 32 ? GEMM_32x128 : g.M <= 64 ? GEMM_64x64 : GEMM_128x64; };
+    auto tiles_of = [&](GemmDesc const& g) {
+        int sh = shape_of(g);
+        return int64_t((g.M + gemm_bm(sh) - 1) / gemm_bm(sh)) * ((g.N + gemm_bn(sh) - 1) / gemm_bn(sh)) * g.batch;
+    };
+    int64_t n_tiles = 0;
+    double work = 0;
+    for (auto const& g : probs) {
+        if (g.M <= 0 || g.N <= 0) continue;
+        n_tiles += tiles_of(g);
+        work += double(tiles_of(g)) * g.K;
+    }
+    // K range per workgroup when the stage alone cannot fill the device
+    int64_t chunk = INT64_MAX;
+    if (n_tiles < target_items && target_items > 0) {
+        int64_t c = int64_t(work / double(target_items));
+        chunk = std::max<int64_t>(min_chunk, (c + 15) / 16 * 16);
+    }
+    for (int32_t p = 0; p < int32_t(probs.size()); ++p) {
+        GemmDesc const& g = probs[p];
+        if (g.M <= 0 || g.N <= 0) continue;
+        int sh = shape_of(g);
+        int bm = gemm_bm(sh), bn = gemm_bn(sh);
+        int32_t tm = (g.M + bm - 1) / bm, tn = (g.N + bn - 1) / bn;
+        int64_t splits = 1, step = g.K;
+        if (g.K > chunk) {
+            splits = (g.K + chunk - 1) / chunk;
+            step = ((g.K + splits - 1) / splits + 15) / 16 * 16;
+            splits = (g.K + step - 1) / step;
+        }
+        for (int32_t b = 0; b < g.batch; ++b)
+            for (int32_t j = 0; j < tn; ++j)
+                for (int32_t i = 0; i < tm; ++i) {
+                    if (splits == 1) {
+                        S.tiles[sh].push_back(GemmTile{p, b, i, j, 0, g.K, -1});
+                        continue;
+                    }
+                    int64_t base = int64_t(S.ws_elems);
+                    for (int64_t s = 0; s < splits; ++s)
+                        S.tiles[sh].push_back(GemmTile{p, b, i, j, int32_t(s * step),
+                                                       int32_t(std::min<int64_t>(g.K, (s + 1) * step)),
+                                                       base + s * int64_t(bm) * bn});
+                    S.reduces.push_back(GemmReduce{p, b, i, j, int32_t(splits), sh, base});
+                    S.ws_elems += uint64_t(splits) * bm * bn;
+                }
+    }
+    // longest reductions first for load balance
+    for (auto& v : S.tiles)
+        std::stable_sort(v.begin(), v.end(), [](GemmTile const& a, GemmTile const& b) { return a.k1 - a.k0 > b.k1 - b.k0; });
+    return S;
+}
+
 // ---- host-side bounds validation of every device table (a table entry that would leave its
 //      buffer is rejected before anything is launched) -----------------------------------------
 void check_gemms(std::vector<GemmDesc> const& v, const uint64_t* sizes) {

@@ -38,6 +38,38 @@ struct GemmDesc {
 };
 static_assert(sizeof(GemmDesc) == 96, "GemmDesc layout");
 
+// Execution schedule of a grouped GEMM stage.  Output tiles come in three shapes (BM x BN):
+// 32x128 for M <= 32, 64x64 for M <= 64, 128x64 above; so a problem with a short M edge is not
+// padded to 64 rows and the B panel of a tall problem is read once.  When the stage has too few
+// tiles to fill the device, long reductions are split along K: each split tile writes its
+// partial BM x BN tile (column-major, ld BM) into a workspace and a reduce job adds the
+// partials in split order into C, so the result does not depend on scheduling.
+enum GemmShape { GEMM_32x128 = 0, GEMM_64x64 = 1, GEMM_128x64 = 2, GEMM_N_SHAPES = 3 };
+constexpr int gemm_bm(int shape) { return shape == GEMM_32x128 ? 32 : shape == GEMM_64x64 ? 64 : 128; }
+constexpr int gemm_bn(int shape) { return shape == GEMM_32x128 ? 128 : 64; }
+struct GemmTile {
+    int32_t prob, batch, tm, tn;
+    int32_t k0, k1;   // reduction range of this tile
+    int64_t ws;       // < 0: the tile writes C; else element offset of its partial in the workspace
+};
+static_assert(sizeof(GemmTile) == 32, "GemmTile layout");
+struct GemmReduce {
+    int32_t prob, batch, tm, tn;
+    int32_t nsplit, shape;
+    int64_t ws;       // first partial; partial s at ws + s * BM * BN
+};
+static_assert(sizeof(GemmReduce) == 32, "GemmReduce layout");
+struct GemmSchedule {
+    std::vector<GemmTile> tiles[GEMM_N_SHAPES];
+    std::vector<GemmReduce> reduces;
+    uint64_t ws_elems = 0;
+    uint64_t n_tiles() const { return tiles[0].size() + tiles[1].size() + tiles[2].size(); }
+};
+// target_items: number of workgroups a stage should provide before K is split;
+// min_chunk: shortest K range of a split tile (a multiple of the K step 16)
+GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t target_items = 4096,
+                                 int32_t min_chunk = 256);
+
 // S-assembly output tile (rows x cols at s_off with leading dimension lds) and its term range.
 struct SLine {
     int64_t s_off;

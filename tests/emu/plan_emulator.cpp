@@ -14,24 +14,58 @@ using namespace mq;
 
 namespace {
 
+// executes the stage as the device does: the tiles of the GEMM schedule, each over its K range,
+// writing C or a workspace partial, then the split-K reduce jobs.  The schedule is built with a
+// K chunk of 16 so that every reduction longer than one K slice goes through the split path.
 void run_gemms(std::vector<GemmDesc> const& G, double* const* base) {
-    for (auto const& g : G) {
-        bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B;
-        for (int b = 0; b < g.batch; ++b) {
-            const double* A = base[g.base_a] + g.a + int64_t(b) * g.sa;
-            const double* B = base[g.base_b] + g.b + int64_t(b) * g.sb;
-            double* Cm = base[g.base_c] + g.c + int64_t(b) * g.sc;
-            for (int n = 0; n < g.N; ++n)
-                for (int m = 0; m < g.M; ++m) {
+    GemmSchedule sch = build_gemm_schedule(G, int64_t(1) << 40, 16);
+    std::vector<double> ws(sch.ws_elems + 1, 0.0);
+    std::vector<char> seen(sch.ws_elems + 1, 0);
+    for (int sh = 0; sh < GEMM_N_SHAPES; ++sh) {
+        const int bm = gemm_bm(sh), bn = gemm_bn(sh);
+        for (GemmTile const& t : sch.tiles[sh]) {
+            GemmDesc const& g = G[t.prob];
+            bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B;
+            const double* A = base[g.base_a] + g.a + int64_t(t.batch) * g.sa;
+            const double* B = base[g.base_b] + g.b + int64_t(t.batch) * g.sb;
+            double* Cm = base[g.base_c] + g.c + int64_t(t.batch) * g.sc;
+            for (int nn = 0; nn < bn; ++nn)
+                for (int mm = 0; mm < bm; ++mm) {
+                    int m = t.tm * bm + mm, n = t.tn * bn + nn;
+                    if (m >= g.M || n >= g.N) continue;
                     double s = 0.0;
-                    for (int k = 0; k < g.K; ++k) {
+                    for (int k = t.k0; k < t.k1; ++k) {
                         double a = ta ? A[int64_t(m) * g.lda + k] : A[m + int64_t(k) * g.lda];
                         double v = tb ? B[n + int64_t(k) * g.ldb] : B[k + int64_t(n) * g.ldb];
                         s += a * v;
                     }
-                    Cm[m + int64_t(n) * g.ldc] = s;
+                    if (t.ws < 0) {
+                        Cm[m + int64_t(n) * g.ldc] = s;
+                    } else {
+                        uint64_t w = uint64_t(t.ws) + mm + uint64_t(nn) * bm;
+                        require(w < sch.ws_elems && !seen[w], "emulated gemm: workspace slot out of range or reused");
+                        seen[w] = 1;
+                        ws[w] = s;
+                    }
                 }
         }
+    }
+    for (GemmReduce const& r : sch.reduces) {
+        GemmDesc const& g = G[r.prob];
+        const int bm = gemm_bm(r.shape), bn = gemm_bn(r.shape);
+        double* Cm = base[g.base_c] + g.c + int64_t(r.batch) * g.sc;
+        for (int nn = 0; nn < bn; ++nn)
+            for (int mm = 0; mm < bm; ++mm) {
+                int m = r.tm * bm + mm, n = r.tn * bn + nn;
+                if (m >= g.M || n >= g.N) continue;
+                double s = 0.0;
+                for (int q = 0; q < r.nsplit; ++q) {
+                    uint64_t w = uint64_t(r.ws) + uint64_t(q) * bm * bn + mm + uint64_t(nn) * bm;
+                    require(w < sch.ws_elems && seen[w], "emulated gemm: reduce reads an unwritten partial");
+                    s += ws[w];
+                }
+                Cm[m + int64_t(n) * g.ldc] = s;
+            }
     }
 }
 
