@@ -126,7 +126,9 @@ typedef struct {
     uint64_t t_elements, s_elements, l_panel_elements, r_tr_elements;
     double   build_seconds;
     uint64_t s_line_elements;         /* sum over S lines of rows * cols (S elements written) */
-    uint64_t gathered_elements;       /* sum over S terms of rows * cols (T elements read) */
+    uint64_t gathered_elements;       /* sum over S terms of rows * cols (T elements of the term sum) */
+    uint64_t n_sblocks;               /* 16-line blocks of the matrix-core term sum */
+    uint64_t union_elements;          /* sum over blocks of union tiles * tile elements (T elements read) */
 } mq_hpsi_stats_t;
 
 typedef struct {

@@ -92,6 +92,8 @@ class HpsiStats(C.Structure):
         ("build_seconds", C.c_double),
         ("s_line_elements", C.c_uint64),
         ("gathered_elements", C.c_uint64),
+        ("n_sblocks", C.c_uint64),
+        ("union_elements", C.c_uint64),
     ]
 
 

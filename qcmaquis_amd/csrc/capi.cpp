@@ -146,6 +146,34 @@ struct GemmStage {
     }
 };
 
+// device copy of the block-form MPO-term sum
+struct SAsmStage {
+    SBlock* blocks = nullptr;
+    SWork* work = nullptr;
+    int64_t* utile = nullptr;
+    double* ualpha = nullptr;
+    int64_t* lsoff = nullptr;
+    int32_t* llds = nullptr;
+    int64_t n_work = 0;
+    void upload(SBlockTable const& B, hipStream_t s) {
+        blocks = dev_upload(B.blocks, s);
+        work = dev_upload(B.work, s);
+        utile = dev_upload(B.utile, s);
+        ualpha = dev_upload(B.ualpha, s);
+        lsoff = dev_upload(B.lsoff, s);
+        llds = dev_upload(B.llds, s);
+        n_work = int64_t(B.work.size());
+    }
+    void run(const double* T, double* S, hipStream_t s) const {
+        hipcheck(launch_sblocks(blocks, work, n_work, utile, ualpha, lsoff, llds, T, S, s), "S-assembly");
+    }
+    void release() {
+        dev_free(blocks); dev_free(work); dev_free(utile); dev_free(ualpha); dev_free(lsoff); dev_free(llds);
+        blocks = nullptr; work = nullptr; utile = nullptr; ualpha = nullptr; lsoff = nullptr; llds = nullptr;
+        n_work = 0;
+    }
+};
+
 Index index_from(const mq_index_desc* d) {
     require(d != nullptr, "null index descriptor");
     return Index(*d);
@@ -171,10 +199,9 @@ struct mq_hpsi_plan_s {
     HpsiPlanHost host;
     const double* rdata = nullptr;
     GemmStage tg, cg;
-    SLine* lines = nullptr;
-    STerm* terms = nullptr;
-    int64_t n_lines = 0;
-    uint64_t s_line_elements = 0, gathered_elements = 0;
+    SAsmStage sa;
+    int64_t n_lines = 0, n_sblocks = 0;
+    uint64_t s_line_elements = 0, gathered_elements = 0, union_elements = 0;
     double *T = nullptr, *S = nullptr, *Lpanel = nullptr;
     double build_seconds = 0;
 };
@@ -403,6 +430,8 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
     check_gemms(P.t_gemms, sizes);
     check_gemms(P.c_gemms, sizes);
     check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
+    SBlockTable SB = build_sblocks(P.lines, P.terms);
+    check_sblocks(SB, P.t_elems, P.s_elems);
 
     auto* nb = new mq_boundary_s{ctx, std::move(P.out)};
     std::unique_ptr<mq_boundary_s> guard(nb);
@@ -411,13 +440,11 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
     double* T = dev_alloc<double>(P.t_elems);
     double* S = dev_alloc<double>(P.s_elems);
     GemmStage tg, cg;
-    SLine* lines = nullptr;
-    STerm* terms = nullptr;
+    SAsmStage sa;
     try {
         tg.upload(P.t_gemms, ctx->stream);
         cg.upload(P.c_gemms, ctx->stream);
-        lines = dev_upload(P.lines, ctx->stream);
-        terms = dev_upload(P.terms, ctx->stream);
+        sa.upload(SB, ctx->stream);
         Bases b{};
         b.p[BASE_T] = T;
         b.p[BASE_S] = S;
@@ -427,15 +454,15 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
         b.p[BASE_NEW] = nb->data;
         hipcheck(launch_fill(S, P.s_elems, 0.0, ctx->stream), "fill");
         tg.run(b, ctx->stream, precision);
-        hipcheck(launch_sassembly(lines, int64_t(P.lines.size()), terms, T, S, ctx->stream), "sassembly");
+        sa.run(T, S, ctx->stream);
         cg.run(b, ctx->stream, precision);
         hipcheck(hipStreamSynchronize(ctx->stream), "growth step");
     } catch (...) {
         (void)hipStreamSynchronize(ctx->stream);
-        tg.release(); cg.release(); dev_free(lines); dev_free(terms); dev_free(T); dev_free(S);
+        tg.release(); cg.release(); sa.release(); dev_free(T); dev_free(S);
         throw;
     }
-    tg.release(); cg.release(); dev_free(lines); dev_free(terms); dev_free(T); dev_free(S);
+    tg.release(); cg.release(); sa.release(); dev_free(T); dev_free(S);
     *out = guard.release();
 }
 
@@ -485,6 +512,8 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
         check_gemms(H.t_gemms, sizes);
         check_gemms(H.c_gemms, sizes);
         check_lines(H.lines, H.terms, H.t_elems, H.s_elems);
+        SBlockTable SB = build_sblocks(H.lines, H.terms);
+        check_sblocks(SB, H.t_elems, H.s_elems);
         for (auto const& j : H.lpanel_jobs)
             require(j.src >= 0 && j.dst >= 0 && uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= left->n &&
                         uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= H.lpanel_elems,
@@ -495,9 +524,10 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
         p->Lpanel = dev_alloc<double>(H.lpanel_elems);
         p->tg.upload(H.t_gemms, ctx->stream);
         p->cg.upload(H.c_gemms, ctx->stream);
-        p->lines = dev_upload(H.lines, ctx->stream);
-        p->terms = dev_upload(H.terms, ctx->stream);
+        p->sa.upload(SB, ctx->stream);
         p->n_lines = int64_t(H.lines.size());
+        p->n_sblocks = int64_t(SB.blocks.size());
+        p->union_elements = SB.gathered;
         CopyJob* jobs = dev_upload(H.lpanel_jobs, ctx->stream);
         hipError_t e = launch_copy_jobs(jobs, int32_t(H.lpanel_jobs.size()), left->data, p->Lpanel, ctx->stream);
         hipError_t e2 = hipStreamSynchronize(ctx->stream);
@@ -536,6 +566,8 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
     s->build_seconds = p->build_seconds;
     s->s_line_elements = p->s_line_elements;
     s->gathered_elements = p->gathered_elements;
+    s->n_sblocks = uint64_t(p->n_sblocks);
+    s->union_elements = p->union_elements;
     return MQ_OK;
 }
 
@@ -556,7 +588,7 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
     b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
     hipcheck(launch_fill(p->S, p->host.s_elems, 0.0, s), "fill");
     p->tg.run(b, s, precision);
-    hipcheck(launch_sassembly(p->lines, p->n_lines, p->terms, p->T, p->S, s), "sassembly");
+    p->sa.run(p->T, p->S, s);
     p->cg.run(b, s, precision);
 }
 
@@ -588,7 +620,7 @@ int mq_hpsi_timed(mq_hpsi_plan_t p, const double* in, double* out, int precision
             hipcheck(hipEventRecord(ev[1], s), "event");
             p->tg.run(b, s, precision);
             hipcheck(hipEventRecord(ev[2], s), "event");
-            hipcheck(launch_sassembly(p->lines, p->n_lines, p->terms, p->T, p->S, s), "sassembly");
+            p->sa.run(p->T, p->S, s);
             hipcheck(hipEventRecord(ev[3], s), "event");
             p->cg.run(b, s, precision);
             hipcheck(hipEventRecord(ev[4], s), "event");
@@ -633,8 +665,7 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
         (void)hipStreamSynchronize(p->ctx->stream);
         p->tg.release();
         p->cg.release();
-        dev_free(p->lines);
-        dev_free(p->terms);
+        p->sa.release();
         dev_free(p->T);
         dev_free(p->S);
         dev_free(p->Lpanel);

@@ -5,9 +5,9 @@
 //    v_mfma_f64_16x16x4f64 tiles), operands staged through LDS in K-slices of 16 with the next
 //    slice prefetched into registers.  Each output tile is owned by one workgroup or, when K is
 //    split, by a fixed set of partial tiles summed in order by gemm_reduce_kernel: no atomics.
-//  * sassembly_kernel: the MPO-term sum.  One workgroup per output tile ("line"); the line's
-//    (alpha, T offset) terms are staged in LDS and every thread accumulates its tile elements in
-//    the term order of the reference (create_s / create_s_r, solver/tasks.cpp:425-494).
+//  * sblock_kernel: the MPO-term sum (reference create_s / create_s_r, solver/tasks.cpp:425-494)
+//    as a block-sparse product on the matrix cores: 16 output tiles ("lines") of equal shape at a
+//    time, Y = alpha * X over the union of the T tiles they read.
 //  * copy / reshape / fill kernels for panel set-up and MPS pairing changes.
 //  * FP64 Davidson vector kernels with device-resident coefficients.
 #include "device.hpp"
@@ -153,45 +153,55 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __rest
     }
 }
 
-constexpr int SA_THREADS = 256;
-constexpr int SA_CHUNK = 256;
+// Block-form MPO-term sum on the matrix cores.  A work item is SBLOCK_ELEMS consecutive tile
+// elements of one block of 16 lines; each of the four waves owns 16 * SB_NC of them and computes
+// Y(16 lines x 16 elements) += alpha(16 x 4 tiles) * X(4 tiles x 16 elements) per
+// v_mfma_f64_16x16x4f64 over the block's union of T tiles.  Lane l loads alpha of line l % 16
+// and union tile l / 16, and element l % 16 of that tile: 16 consecutive doubles per tile.
+constexpr int SB_NC = SBLOCK_ELEMS / 64;
+static_assert(SBLOCK_LINES == 16 && SB_NC * 64 == SBLOCK_ELEMS, "S block geometry");
 
-__global__ __launch_bounds__(SA_THREADS) void sassembly_kernel(const SLine* __restrict__ lines,
-                                                                const STerm* __restrict__ terms,
-                                                                const double* __restrict__ T, double* __restrict__ S) {
-    __shared__ double s_alpha[SA_CHUNK];
-    __shared__ int64_t s_off[SA_CHUNK];
-    const SLine ln = lines[blockIdx.x];
-    const int n_el = ln.rows * ln.cols;
-    const int tid = threadIdx.x;
-    // up to 4 accumulators per thread in registers; larger tiles loop over passes
-    for (int base = 0; base < n_el; base += 4 * SA_THREADS) {
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int t0 = ln.t0; t0 < ln.t1; t0 += SA_CHUNK) {
-            int nt = min(SA_CHUNK, ln.t1 - t0);
-            __syncthreads();
-            if (tid < nt) {
-                s_alpha[tid] = terms[t0 + tid].alpha;
-                s_off[tid] = terms[t0 + tid].t_off;
-            }
-            __syncthreads();
-            for (int k = 0; k < nt; ++k) {
-                const double a = s_alpha[k];
-                const double* src = T + s_off[k];
+__global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ blocks, const SWork* __restrict__ work,
+                                                     const int64_t* __restrict__ utile,
+                                                     const double* __restrict__ ualpha,
+                                                     const int64_t* __restrict__ lsoff,
+                                                     const int32_t* __restrict__ llds, const double* __restrict__ T,
+                                                     double* __restrict__ S) {
+    const SWork w = work[blockIdx.x];
+    const SBlock b = blocks[w.blk];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n0 = w.n0 + wave * 16 * SB_NC;
+    if (n0 >= b.E) return;  // no barriers below: waves past the tile end leave early
+    const int li = lane & 15, lk = lane >> 4;
+    double4_t acc[SB_NC];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    int idx = base + e * SA_THREADS + tid;
-                    if (idx < n_el) acc[e] += a * src[idx];
-                }
-            }
+    for (int j = 0; j < SB_NC; ++j) acc[j] = double4_t{0.0, 0.0, 0.0, 0.0};
+    const int64_t* ut = utile + b.u_off;
+    const double* ua = ualpha + b.u_off * SBLOCK_LINES;
+    for (int ks = 0; ks < b.nu; ks += 4) {
+        const int k = ks + lk;
+        const bool kv = k < b.nu;
+        const double a = kv ? ua[k * SBLOCK_LINES + li] : 0.0;  // A[line li][tile k]
+        const int64_t to = kv ? ut[k] : 0;
+        double x[SB_NC];
+#pragma unroll
+        for (int j = 0; j < SB_NC; ++j) {
+            const int e = n0 + 16 * j + li;
+            x[j] = (kv && e < b.E) ? T[to + e] : 0.0;  // B[tile k][element e]
         }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int idx = base + e * SA_THREADS + tid;
-            if (idx < n_el) {
-                int r = idx % ln.rows, c = idx / ln.rows;
-                S[ln.s_off + r + int64_t(c) * ln.lds] = acc[e];
-            }
+        for (int j = 0; j < SB_NC; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x[j], acc[j], 0, 0, 0);
+    }
+    // D[i][e]: lane l, register r -> line i = l / 16 + 4 r, element e = l % 16
+#pragma unroll
+    for (int j = 0; j < SB_NC; ++j) {
+        const int e = n0 + 16 * j + li;
+        if (e >= b.E) continue;
+        const int r_ = e % b.rows, c_ = e / b.rows;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = lk + 4 * r;
+            if (i < b.nlines) S[lsoff[b.line0 + i] + r_ + int64_t(c_) * llds[b.line0 + i]] = acc[j][r];
         }
     }
 }
@@ -314,10 +324,14 @@ hipError_t launch_gemm_f64(const GemmDesc* probs, const GemmTile* const* tiles, 
     return hipGetLastError();
 }
 
-hipError_t launch_sassembly(const SLine* lines, int64_t n_lines, const STerm* terms, const double* T, double* S,
-                            hipStream_t stream) {
-    if (n_lines <= 0) return hipSuccess;
-    hipLaunchKernelGGL(sassembly_kernel, dim3(unsigned(n_lines)), dim3(SA_THREADS), 0, stream, lines, terms, T, S);
+hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_work, const int64_t* utile,
+                          const double* ualpha, const int64_t* lsoff, const int32_t* llds, const double* T, double* S,
+                          hipStream_t stream) {
+    for (int64_t off = 0; off < n_work; off += 0x7fffffff) {
+        int64_t cnt = n_work - off < 0x7fffffff ? n_work - off : 0x7fffffff;
+        hipLaunchKernelGGL(sblock_kernel, dim3(unsigned(cnt)), dim3(256), 0, stream, blocks, work + off, utile, ualpha,
+                           lsoff, llds, T, S);
+    }
     return hipGetLastError();
 }
 

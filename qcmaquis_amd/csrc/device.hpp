@@ -19,8 +19,10 @@ struct Bases {
 // workspace of the split partials (GemmSchedule::ws_elems doubles)
 hipError_t launch_gemm_f64(const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
                            const GemmReduce* reduces, int64_t n_reduces, double* ws, Bases bases, hipStream_t stream);
-hipError_t launch_sassembly(const SLine* lines, int64_t n_lines, const STerm* terms, const double* T, double* S,
-                            hipStream_t stream);
+// block-form MPO-term sum (SBlockTable uploaded): writes every S tile of the table's lines
+hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_work, const int64_t* utile,
+                          const double* ualpha, const int64_t* lsoff, const int32_t* llds, const double* T, double* S,
+                          hipStream_t stream);
 hipError_t launch_fill(double* p, uint64_t n, double v, hipStream_t stream);
 hipError_t launch_copy_jobs(const CopyJob* jobs, int32_t n_jobs, const double* src, double* dst, hipStream_t stream);
 

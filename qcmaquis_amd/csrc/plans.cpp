@@ -863,6 +863,100 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
     return P;
 }
 
+// ---- block form of the MPO-term sum ------------------------------------------------------------
+SBlockTable build_sblocks(std::vector<SLine> const& lines, std::vector<STerm> const& terms) {
+    SBlockTable B;
+    // lines by tile shape; inside a shape by their first T tile (lines reading the same T region
+    // share most of their tiles), longer lines first
+    std::vector<uint32_t> order;
+    std::vector<int64_t> tmin(lines.size(), 0);
+    for (uint32_t i = 0; i < lines.size(); ++i) {
+        SLine const& l = lines[i];
+        if (l.rows <= 0 || l.cols <= 0 || l.t1 <= l.t0) continue;
+        int64_t mn = INT64_MAX;
+        for (int32_t k = l.t0; k < l.t1; ++k) mn = std::min(mn, terms[k].t_off);
+        tmin[i] = mn;
+        order.push_back(i);
+    }
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        SLine const& x = lines[a];
+        SLine const& y = lines[b];
+        if (x.rows != y.rows) return x.rows < y.rows;
+        if (x.cols != y.cols) return x.cols < y.cols;
+        if (tmin[a] != tmin[b]) return tmin[a] < tmin[b];
+        return x.t1 - x.t0 > y.t1 - y.t0;
+    });
+    std::vector<int64_t> u;
+    for (size_t s0 = 0; s0 < order.size();) {
+        SLine const& first = lines[order[s0]];
+        size_t s1 = s0;
+        while (s1 < order.size() && s1 - s0 < size_t(SBLOCK_LINES) && lines[order[s1]].rows == first.rows &&
+               lines[order[s1]].cols == first.cols)
+            ++s1;
+        u.clear();
+        for (size_t q = s0; q < s1; ++q) {
+            SLine const& l = lines[order[q]];
+            for (int32_t k = l.t0; k < l.t1; ++k) u.push_back(terms[k].t_off);
+        }
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        SBlock blk{};
+        blk.u_off = int64_t(B.utile.size());
+        blk.nu = int32_t(u.size());
+        blk.rows = first.rows;
+        blk.E = first.rows * first.cols;
+        blk.nlines = int32_t(s1 - s0);
+        blk.line0 = int32_t(B.lsoff.size());
+        size_t a0 = B.ualpha.size();
+        B.ualpha.resize(a0 + size_t(SBLOCK_LINES) * u.size(), 0.0);
+        for (size_t q = s0; q < s1; ++q) {
+            SLine const& l = lines[order[q]];
+            B.lsoff.push_back(l.s_off);
+            B.llds.push_back(l.lds);
+            for (int32_t k = l.t0; k < l.t1; ++k) {
+                size_t pos = size_t(std::lower_bound(u.begin(), u.end(), terms[k].t_off) - u.begin());
+                B.ualpha[a0 + pos * SBLOCK_LINES + (q - s0)] += terms[k].alpha;
+            }
+        }
+        for (size_t q = s1 - s0; q < size_t(SBLOCK_LINES); ++q) {
+            B.lsoff.push_back(0);
+            B.llds.push_back(1);
+        }
+        B.utile.insert(B.utile.end(), u.begin(), u.end());
+        int32_t bi = int32_t(B.blocks.size());
+        B.blocks.push_back(blk);
+        for (int32_t e = 0; e < blk.E; e += SBLOCK_ELEMS) B.work.push_back(SWork{bi, e});
+        B.gathered += uint64_t(u.size()) * uint64_t(blk.E);
+        s0 = s1;
+    }
+    B.mac = B.gathered * SBLOCK_LINES;
+    return B;
+}
+
+void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size) {
+    require(B.lsoff.size() == B.blocks.size() * SBLOCK_LINES && B.llds.size() == B.lsoff.size() &&
+                B.ualpha.size() == B.utile.size() * SBLOCK_LINES, "S blocks: malformed tables");
+    for (SBlock const& b : B.blocks) {
+        require(b.nu >= 0 && b.E > 0 && b.rows > 0 && b.E % b.rows == 0 && b.nlines > 0 && b.nlines <= SBLOCK_LINES &&
+                    b.u_off >= 0 && uint64_t(b.u_off) + uint64_t(b.nu) <= B.utile.size() && b.line0 >= 0 &&
+                    uint64_t(b.line0) + SBLOCK_LINES <= B.lsoff.size(), "S blocks: malformed block");
+        for (int32_t k = 0; k < b.nu; ++k) {
+            int64_t o = B.utile[size_t(b.u_off) + k];
+            require(o >= 0 && uint64_t(o) + uint64_t(b.E) <= t_size, "S blocks: tile outside the T buffer");
+        }
+        int32_t cols = b.E / b.rows;
+        for (int32_t i = 0; i < b.nlines; ++i) {
+            int64_t so = B.lsoff[size_t(b.line0) + i];
+            int32_t ld = B.llds[size_t(b.line0) + i];
+            require(so >= 0 && ld >= b.rows && uint64_t(so + (b.rows - 1) + int64_t(cols - 1) * ld) < s_size,
+                    "S blocks: line outside the S buffer");
+        }
+    }
+    for (SWork const& w : B.work)
+        require(w.blk >= 0 && size_t(w.blk) < B.blocks.size() && w.n0 >= 0 && w.n0 < B.blocks[size_t(w.blk)].E,
+                "S blocks: malformed work item");
+}
+
 // ---- grouped GEMM schedule ---------------------------------------------------------------------
 GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t target_items, int32_t min_chunk) {
     require(min_chunk >= 16 && min_chunk % 16 == 0, "gemm schedule: chunk must be a multiple of 16");

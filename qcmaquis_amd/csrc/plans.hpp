@@ -38,6 +38,35 @@ struct GemmDesc {
 };
 static_assert(sizeof(GemmDesc) == 96, "GemmDesc layout");
 
+// Block form of the MPO-term sum for the matrix cores.  Lines of equal tile shape are grouped in
+// blocks of SBLOCK_LINES; a block stores the sorted union of the T tiles its lines read and a dense
+// SBLOCK_LINES x nu coefficient matrix (tile-major: alpha of line i for union tile k at
+// ualpha[16 (u_off + k) + i], zero where line i does not read tile k, summed where it reads it
+// twice), so the block's S tiles are Y = alpha * X with X the nu x E matrix of the tiles' elements.
+constexpr int SBLOCK_LINES = 16;
+struct SBlock {
+    int64_t u_off;            // first union tile (utile / ualpha index)
+    int32_t nu, E, rows;      // union size, tile elements (rows x cols), tile rows
+    int32_t nlines, line0;    // lines in the block, first entry of lsoff / llds
+    int32_t pad;
+};
+static_assert(sizeof(SBlock) == 32, "SBlock layout");
+struct SWork {
+    int32_t blk, n0;          // block and first tile element of one workgroup
+};
+struct SBlockTable {
+    std::vector<SBlock> blocks;
+    std::vector<SWork> work;
+    std::vector<int64_t> utile;   // T element offset of each union tile
+    std::vector<double> ualpha;
+    std::vector<int64_t> lsoff;   // S offset of each line (SBLOCK_LINES entries per block)
+    std::vector<int32_t> llds;    // its leading dimension
+    uint64_t gathered = 0;        // sum over blocks of nu * E (T elements read)
+    uint64_t mac = 0;             // SBLOCK_LINES * gathered (matrix-core multiply-adds)
+};
+// work items cover SBLOCK_ELEMS consecutive tile elements of one block
+constexpr int SBLOCK_ELEMS = 128;
+
 // Execution schedule of a grouped GEMM stage.  Output tiles come in three shapes (BM x BN):
 // 32x128 for M <= 32, 64x64 for M <= 64, 128x64 above; so a problem with a short M edge is not
 // padded to 64 rows and the B panel of a tall problem is read once.  When the stage has too few
@@ -81,6 +110,10 @@ struct STerm {
     double alpha;
     int64_t t_off;  // element offset of a contiguous rows x cols tile in the T buffer
 };
+
+// block table of a list of S lines (lines without terms are left out)
+SBlockTable build_sblocks(std::vector<SLine> const& lines, std::vector<STerm> const& terms);
+void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size);
 
 struct FlopStats {
     uint64_t flops_total = 0, flops_gemm = 0, flops_gemm_exec = 0, n_terms = 0;

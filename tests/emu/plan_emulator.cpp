@@ -3,6 +3,7 @@
 // Builds the product's plans (qcmaquis_amd/csrc/plans.cpp) and runs their GEMM problems, S lines
 // and copy / reshape jobs on the CPU with the same index arithmetic as the gfx950 kernels, so the
 // plan builder and its bounds checks are tested without a GPU.  Not linked into the product.
+#include <algorithm>
 #include <cstdio>
 #include <memory>
 #include <vector>
@@ -69,15 +70,40 @@ void run_gemms(std::vector<GemmDesc> const& G, double* const* base) {
     }
 }
 
-void run_lines(std::vector<SLine> const& lines, std::vector<STerm> const& terms, const double* T, double* S) {
-    for (auto const& ln : lines) {
-        int n_el = ln.rows * ln.cols;
-        for (int idx = 0; idx < n_el; ++idx) {
-            double acc = 0.0;
-            for (int t = ln.t0; t < ln.t1; ++t) acc += terms[t].alpha * T[terms[t].t_off + idx];
-            S[ln.s_off + idx % ln.rows + int64_t(idx / ln.rows) * ln.lds] = acc;
-        }
+// the MPO-term sum as the device runs it: the block table built from the lines, each work item
+// covering SBLOCK_ELEMS tile elements of one block (four waves of 16 * SBLOCK_ELEMS / 64); every
+// (line, element) of the table must be written exactly once
+void run_lines(std::vector<SLine> const& lines, std::vector<STerm> const& terms, const double* T, double* S,
+               uint64_t t_size, uint64_t s_size) {
+    SBlockTable B = build_sblocks(lines, terms);
+    check_sblocks(B, t_size, s_size);
+    std::vector<std::vector<char>> done(B.blocks.size());
+    for (size_t i = 0; i < B.blocks.size(); ++i) done[i].assign(size_t(B.blocks[i].E) * SBLOCK_LINES, 0);
+    const int per_wave = 16 * (SBLOCK_ELEMS / 64);
+    for (SWork const& w : B.work) {
+        SBlock const& b = B.blocks[size_t(w.blk)];
+        for (int wave = 0; wave < 4; ++wave)
+            for (int e = w.n0 + wave * per_wave; e < std::min(b.E, w.n0 + (wave + 1) * per_wave); ++e)
+                for (int i = 0; i < b.nlines; ++i) {
+                    double acc = 0.0;
+                    for (int k = 0; k < b.nu; ++k)
+                        acc += B.ualpha[size_t(b.u_off + k) * SBLOCK_LINES + i] * T[B.utile[size_t(b.u_off + k)] + e];
+                    char& d = done[size_t(w.blk)][size_t(e) * SBLOCK_LINES + i];
+                    require(!d, "emulated S blocks: element written twice");
+                    d = 1;
+                    S[B.lsoff[size_t(b.line0 + i)] + e % b.rows + int64_t(e / b.rows) * B.llds[size_t(b.line0 + i)]] = acc;
+                }
     }
+    for (size_t i = 0; i < B.blocks.size(); ++i)
+        for (int e = 0; e < B.blocks[i].E; ++e)
+            for (int l = 0; l < B.blocks[i].nlines; ++l)
+                require(done[i][size_t(e) * SBLOCK_LINES + l], "emulated S blocks: element not written");
+    uint64_t covered = 0;
+    for (auto const& b : B.blocks) covered += uint64_t(b.E) * b.nlines;
+    uint64_t expected = 0;
+    for (auto const& l : lines)
+        if (l.t1 > l.t0) expected += uint64_t(l.rows) * l.cols;
+    require(covered == expected, "emulated S blocks: lines missing from the block table");
 }
 
 void run_copy(std::vector<CopyJob> const& jobs, const double* src, double* dst) {
@@ -145,7 +171,7 @@ int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const dou
         base[BASE_LPANEL] = Lp.data();
         base[BASE_BOUNDARY] = const_cast<double*>(Rdata);
         run_gemms(P.t_gemms, base);
-        run_lines(P.lines, P.terms, T.data(), S.data());
+        run_lines(P.lines, P.terms, T.data(), S.data(), P.t_elems, P.s_elems);
         run_gemms(P.c_gemms, base);
         return 0;
     } catch (std::exception const& e) {
@@ -185,7 +211,7 @@ static EmuBoundary* grow(bool left, int sym, const mq_mpo_desc* md, const mq_bin
     base[BASE_KET] = const_cast<double*>(ket_rp);
     base[BASE_NEW] = res->data.data();
     run_gemms(P.t_gemms, base);
-    run_lines(P.lines, P.terms, T.data(), S.data());
+    run_lines(P.lines, P.terms, T.data(), S.data(), P.t_elems, P.s_elems);
     run_gemms(P.c_gemms, base);
     return res;
 }
