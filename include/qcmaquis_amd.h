@@ -190,7 +190,8 @@ int mq_hpsi_plan_psi_elements(mq_hpsi_plan_t plan, uint64_t* n);
 /* device pointers (on the context's device), stream-ordered on the context stream */
 int mq_hpsi(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out_dev, int precision);
 /* as mq_hpsi, synchronous, with the duration of each stage measured by HIP events recorded on
- * the context stream: stage_ms[0] S clear, [1] T-GEMMs, [2] S-assembly, [3] output GEMMs
+ * the context stream: stage_ms[0] set-up (empty: S needs no clear), [1] T-GEMMs, [2] S-assembly
+ * (MPO-term sum), [3] output GEMMs
  * (reference timing: cudaEvent around the GPU H*psi, super_hamil_mv.hpp:61-84) */
 int mq_hpsi_timed(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out_dev, int precision,
                   float* stage_ms);

@@ -452,7 +452,6 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
         b.p[BASE_BRA] = const_cast<double*>(bra);
         b.p[BASE_KET] = const_cast<double*>(ket);
         b.p[BASE_NEW] = nb->data;
-        hipcheck(launch_fill(S, P.s_elems, 0.0, ctx->stream), "fill");
         tg.run(b, ctx->stream, precision);
         sa.run(T, S, ctx->stream);
         cg.run(b, ctx->stream, precision);
@@ -586,7 +585,6 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
     b.p[BASE_S] = p->S;
     b.p[BASE_LPANEL] = p->Lpanel;
     b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
-    hipcheck(launch_fill(p->S, p->host.s_elems, 0.0, s), "fill");
     p->tg.run(b, s, precision);
     p->sa.run(p->T, p->S, s);
     p->cg.run(b, s, precision);
@@ -616,8 +614,7 @@ int mq_hpsi_timed(mq_hpsi_plan_t p, const double* in, double* out, int precision
             b.p[BASE_LPANEL] = p->Lpanel;
             b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
             hipcheck(hipEventRecord(ev[0], s), "event");
-            hipcheck(launch_fill(p->S, p->host.s_elems, 0.0, s), "fill");
-            hipcheck(hipEventRecord(ev[1], s), "event");
+                    hipcheck(hipEventRecord(ev[1], s), "event");
             p->tg.run(b, s, precision);
             hipcheck(hipEventRecord(ev[2], s), "event");
             p->sa.run(p->T, p->S, s);

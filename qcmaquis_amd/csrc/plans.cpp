@@ -108,6 +108,25 @@ inline void op_iterate(OpBlock const& blk, const double cpl[4], CohortAcc& coh, 
     }
 }
 
+// MPO rows of one unit that received no terms: emitted as lines without terms, so the term sum
+// writes their S tiles as zeros and S needs no separate clear
+template <class MakeLine>
+void add_empty_lines(CohortAcc::Unit const& u, uint32_t nrows, int32_t term_pos, std::vector<SLine>& out,
+                     MakeLine make_line) {
+    if (!u.ms) return;
+    std::vector<char> has(nrows, 0);
+    for (uint32_t b : u.line_b) {
+        require(b < nrows, "S lines: MPO row outside its cohort");
+        has[b] = 1;
+    }
+    for (uint32_t b = 0; b < nrows; ++b)
+        if (!has[b]) {
+            SLine line = make_line(b);
+            line.t0 = line.t1 = term_pos;
+            out.push_back(line);
+        }
+}
+
 inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 // a term reads a contiguous rows x cols tile at element offset `off` of its T intermediate;
@@ -392,6 +411,14 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                 seek += u.line_count[l];
                 P.lines.push_back(line);
             }
+            add_empty_lines(u, rec.nb, int32_t(P.terms.size()), P.lines, [&](uint32_t b) {
+                SLine line{};
+                line.s_off = int64_t(s_off[rec.lb_out] + koff[c] + uint64_t(b) * ls_in + uint64_t(u.offset) * ldS);
+                line.lds = int32_t(ldS);
+                line.rows = int32_t(ls_in);
+                line.cols = int32_t(u.ms);
+                return line;
+            });
             P.flops.flops_total += 2ull * ls_in * u.ms * u.alpha.size();
         }
         uint64_t g = 2ull * ls_out * ls_in * rec.nb * rec.acc.stripe;
@@ -403,9 +430,9 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
     // Execution order of the lines: by the T position of their first term, so that lines reading
     // the T tasks of one input block run together and those tasks stay cache-resident (each line
     // owns a disjoint S tile, so the order does not change any result).
-    std::stable_sort(P.lines.begin(), P.lines.end(), [&](SLine const& a, SLine const& b) {
-        return P.terms[a.t0].t_off < P.terms[b.t0].t_off;
-    });
+    auto first_t = [&](SLine const& l) { return l.t1 > l.t0 ? P.terms[l.t0].t_off : INT64_MAX; };
+    std::stable_sort(P.lines.begin(), P.lines.end(),
+                     [&](SLine const& a, SLine const& b) { return first_t(a) < first_t(b); });
     P.n_cohorts = cohorts.size();
     for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
         uint32_t ls = left_i.sizes[lb];
@@ -630,6 +657,14 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
                 seek += u.line_count[l];
                 P.lines.push_back(line);
             }
+            add_empty_lines(u, rec.nrows, int32_t(P.terms.size()), P.lines, [&](uint32_t b) {
+                SLine line{};
+                line.s_off = int64_t(sbase + u.offset + uint64_t(b) * rs_ket * stripe);
+                line.lds = int32_t(stripe);
+                line.rows = int32_t(u.ms);
+                line.cols = int32_t(rs_ket);
+                return line;
+            });
             P.flops.flops_total += 2ull * rs_ket * u.ms * u.alpha.size();
         }
         uint32_t ci_new = new_ci[c];
@@ -847,6 +882,14 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
                 seek += u.line_count[l];
                 P.lines.push_back(line);
             }
+            add_empty_lines(u, rec.nrows, int32_t(P.terms.size()), P.lines, [&](uint32_t b) {
+                SLine line{};
+                line.s_off = int64_t(sbase + uint64_t(b) * ls_ket + uint64_t(u.offset) * lds);
+                line.lds = int32_t(lds);
+                line.rows = int32_t(ls_ket);
+                line.cols = int32_t(u.ms);
+                return line;
+            });
             P.flops.flops_total += 2ull * ls_ket * u.ms * u.alpha.size();
         }
         uint32_t ci_new = new_ci[c];
@@ -872,8 +915,8 @@ SBlockTable build_sblocks(std::vector<SLine> const& lines, std::vector<STerm> co
     std::vector<int64_t> tmin(lines.size(), 0);
     for (uint32_t i = 0; i < lines.size(); ++i) {
         SLine const& l = lines[i];
-        if (l.rows <= 0 || l.cols <= 0 || l.t1 <= l.t0) continue;
-        int64_t mn = INT64_MAX;
+        if (l.rows <= 0 || l.cols <= 0) continue;
+        int64_t mn = INT64_MAX;  // lines without terms (zero tiles) sort last
         for (int32_t k = l.t0; k < l.t1; ++k) mn = std::min(mn, terms[k].t_off);
         tmin[i] = mn;
         order.push_back(i);

@@ -111,7 +111,7 @@ struct STerm {
     int64_t t_off;  // element offset of a contiguous rows x cols tile in the T buffer
 };
 
-// block table of a list of S lines (lines without terms are left out)
+// block table of a list of S lines (lines without terms become zero tiles)
 SBlockTable build_sblocks(std::vector<SLine> const& lines, std::vector<STerm> const& terms);
 void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size);
 

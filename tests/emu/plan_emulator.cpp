@@ -4,6 +4,7 @@
 // and copy / reshape jobs on the CPU with the same index arithmetic as the gfx950 kernels, so the
 // plan builder and its bounds checks are tested without a GPU.  Not linked into the product.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <memory>
 #include <vector>
@@ -101,8 +102,7 @@ void run_lines(std::vector<SLine> const& lines, std::vector<STerm> const& terms,
     uint64_t covered = 0;
     for (auto const& b : B.blocks) covered += uint64_t(b.E) * b.nlines;
     uint64_t expected = 0;
-    for (auto const& l : lines)
-        if (l.t1 > l.t0) expected += uint64_t(l.rows) * l.cols;
+    for (auto const& l : lines) expected += uint64_t(l.rows) * l.cols;
     require(covered == expected, "emulated S blocks: lines missing from the block table");
 }
 
@@ -161,7 +161,7 @@ int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const dou
         for (auto const& j : P.lpanel_jobs)
             require(uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= Ln &&
                     uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= P.lpanel_elems, "copy job out of range");
-        std::vector<double> T(P.t_elems + 1), S(P.s_elems + 1, 0.0), Lp(P.lpanel_elems + 1);
+        std::vector<double> T(P.t_elems + 1), S(P.s_elems + 1, std::nan("")), Lp(P.lpanel_elems + 1);
         run_copy(P.lpanel_jobs, Ldata, Lp.data());
         double* base[BASE_COUNT] = {nullptr};
         base[BASE_PSI_IN] = const_cast<double*>(psi);
@@ -202,7 +202,7 @@ static EmuBoundary* grow(bool left, int sym, const mq_mpo_desc* md, const mq_bin
     check_gemms(P.c_gemms, sizes);
     check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
     auto* res = new EmuBoundary{std::move(P.out), std::vector<double>(P.out_elems, 0.0)};
-    std::vector<double> T(P.t_elems + 1), S(P.s_elems + 1, 0.0);
+    std::vector<double> T(P.t_elems + 1), S(P.s_elems + 1, std::nan(""));
     double* base[BASE_COUNT] = {nullptr};
     base[BASE_T] = T.data();
     base[BASE_S] = S.data();
