@@ -183,7 +183,8 @@ def main():
     st = plan.stats()
     log("plan:", {k: st[k] for k in ("n_t_tasks", "n_cohorts", "n_lines", "n_terms", "flops_total", "flops_gemm",
                                       "flops_gemm_exec", "bytes_compulsory", "t_elements", "s_elements",
-                                      "s_line_elements", "gathered_elements", "build_seconds")})
+                                      "s_line_elements", "gathered_elements", "n_sblocks", "union_elements",
+                                      "build_seconds")})
     n = w["layout"].n
     psi = torch.randn(n, dtype=torch.float64, device="cuda")
     out = torch.empty_like(psi)
@@ -209,23 +210,50 @@ def main():
             ctx.check(ctx.lib.mq_hpsi_timed(plan.h, C.c_void_p(psi.data_ptr()), C.c_void_p(out.data_ptr()), 0, sms))
             stage += np.array(list(sms))
     stage /= nrep
+    # mixed precision (FP32 operands, intermediates and matrix cores; psi' in FP64): same plan,
+    # same timed protocol, error against the FP64 result of the same psi
+    ref = out.clone()
+    out32 = torch.empty_like(psi)
+    for _ in range(args.warmup):
+        plan.apply_device(psi.data_ptr(), out32.data_ptr(), abi.MQ_PREC_FP32)
+    ctx.sync()
+    torch.cuda.synchronize()
+    stage32 = np.zeros(4)
+    with ProfilerRegion():
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.apply_device(psi.data_ptr(), out32.data_ptr(), abi.MQ_PREC_FP32)
+        ctx.sync()
+        torch.cuda.synchronize()
+        wall32 = time.perf_counter() - t0
+        for _ in range(nrep):
+            ctx.check(ctx.lib.mq_hpsi_timed(plan.h, C.c_void_p(psi.data_ptr()), C.c_void_p(out32.data_ptr()),
+                                            abi.MQ_PREC_FP32, sms))
+            stage32 += np.array(list(sms))
+    stage32 /= nrep
+    ms32 = wall32 / args.steps * 1e3
+    err32 = float((out32 - ref).norm() / ref.norm())
+    log("FP32 H*psi: %.3f ms, stages %s, relative error vs FP64 %.2e" % (ms32, np.round(stage32, 4).tolist(), err32))
     ms = wall / args.steps * 1e3
     log("timed %d H*psi: %.3f ms each" % (args.steps, ms))
-    log("stage ms (fill, T-GEMM, S-assembly, L*S GEMM):", np.round(stage, 4).tolist())
+    log("stage ms (set-up, T-GEMM, S-assembly, L*S GEMM):", np.round(stage, 4).tolist())
     value = st["flops_total"] / (ms * 1e-3) / 1e12
     gemm_ms = stage[1] + stage[3]
     gemm_tf = st["flops_gemm_exec"] / (gemm_ms * 1e-3) / 1e12
-    # S-assembly: every term reads its T tile, every line writes its S tile (FP64)
-    s_bytes = 8 * (st["gathered_elements"] + st["s_line_elements"])
+    # S-assembly as executed: every 16-line block reads the union of its T tiles once and writes
+    # its S tiles (FP64)
+    s_bytes = 8 * (st["union_elements"] + st["s_line_elements"])
     s_gbs = s_bytes / (stage[2] * 1e-3) / 1e9
     rl_gemm = {"bound": "mfma", "kernel": "gemm_f64_kernel (T-GEMM + L*S GEMM launches)",
                "achieved": round(gemm_tf, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                "frac": round(gemm_tf / PEAK_FP64_TFLOPS, 5), "traffic": None,
                "work_per_launch": "%d FP64 flops" % st["flops_gemm_exec"]}
-    rl_sasm = {"bound": "hbm", "kernel": "sassembly_kernel", "achieved": round(s_gbs, 2), "peak": PEAK_HBM_GBS,
+    rl_sasm = {"bound": "hbm", "kernel": "sblock_kernel<double>", "achieved": round(s_gbs, 2), "peak": PEAK_HBM_GBS,
                "unit": "GB/s", "frac": round(s_gbs / PEAK_HBM_GBS, 5), "traffic": None,
-               "work_per_launch": "%d bytes (%d terms: T tile reads %d el, S writes %d el)" % (
-                   s_bytes, st["n_terms"], st["gathered_elements"], st["s_line_elements"])}
+               "work_per_launch": "%d bytes (%d blocks of 16 lines: union T tile reads %d el, S writes %d el; "
+                                  "%d terms, %d el in the per-term formulation)" % (
+                   s_bytes, st["n_sblocks"], st["union_elements"], st["s_line_elements"], st["n_terms"],
+                   st["gathered_elements"])}
     dominant, other = (rl_sasm, rl_gemm) if stage[2] >= gemm_ms else (rl_gemm, rl_sasm)
     line = {
         "metric": METRIC,
@@ -242,8 +270,12 @@ def main():
         "data": "synthetic (seeded integrals, random initial MPS)" if args.workload == "qc" else "synthetic",
         "config": dict(w["desc"], step="one H*psi (Davidson iteration) of the site problem, inputs resident in HBM",
                        flops_per_step_reference_model=st["flops_total"], gemm_flops_executed=st["flops_gemm_exec"],
-                       stage_ms={"s_clear": round(stage[0], 4), "t_gemm": round(stage[1], 4),
-                                 "s_assembly": round(stage[2], 4), "ls_gemm": round(stage[3], 4)}),
+                       stage_ms={"setup": round(stage[0], 4), "t_gemm": round(stage[1], 4),
+                                 "s_assembly": round(stage[2], 4), "ls_gemm": round(stage[3], 4)},
+                       mixed_fp32={"ms_per_step": round(ms32, 4),
+                                   "tflops": round(st["flops_total"] / (ms32 * 1e-3) / 1e12, 4),
+                                   "speedup_vs_fp64": round(ms / ms32, 3), "rel_error_vs_fp64": err32,
+                                   "stage_ms": [round(x, 4) for x in stage32]}),
         "roofline": dominant,
         "roofline_other": other,
     }

@@ -136,6 +136,7 @@ typedef struct {
     double   residual_norm;
     double   seconds;
     double   hpsi_seconds;
+    int32_t  refine_iterations;       /* FP64 iterations after an FP32 run (precision MQ_PREC_FP32) */
 } mq_davidson_stats_t;
 
 /* ---- context ---------------------------------------------------------------------------- */
@@ -145,6 +146,10 @@ const char* mq_last_error(mq_ctx_t ctx);
 int         mq_ctx_sync(mq_ctx_t ctx);
 void*       mq_ctx_stream(mq_ctx_t ctx);      /* hipStream_t of the context */
 int         mq_version(void);
+/* sizes in bytes of the ABI structs, in this order: mq_index_desc, mq_mpo_desc, mq_bindex_desc,
+ * mq_boundary_info_t, mq_hpsi_stats_t, mq_davidson_stats_t; returns how many were written (a
+ * binding checks its own struct layouts against these before the first call) */
+int         mq_abi_struct_sizes(uint32_t* sizes, int n);
 
 /* ---- MPO ---------------------------------------------------------------------------------- */
 int mq_mpo_create(mq_ctx_t ctx, const mq_mpo_desc* desc, mq_mpo_t* out);
@@ -189,8 +194,10 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t plan, mq_hpsi_stats_t* stats);
 int mq_hpsi_plan_psi_elements(mq_hpsi_plan_t plan, uint64_t* n);
 /* device pointers (on the context's device), stream-ordered on the context stream */
 int mq_hpsi(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out_dev, int precision);
+/* precision: MQ_PREC_FP64, or MQ_PREC_FP32 (psi, L and R rounded to FP32, T-GEMMs, term sum and
+ * output GEMMs on the FP32 matrix cores with FP32 intermediates, psi' returned in FP64) */
 /* as mq_hpsi, synchronous, with the duration of each stage measured by HIP events recorded on
- * the context stream: stage_ms[0] set-up (empty: S needs no clear), [1] T-GEMMs, [2] S-assembly
+ * the context stream: stage_ms[0] set-up (FP32: rounding psi), [1] T-GEMMs, [2] S-assembly
  * (MPO-term sum), [3] output GEMMs
  * (reference timing: cudaEvent around the GPU H*psi, super_hamil_mv.hpp:61-84) */
 int mq_hpsi_timed(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out_dev, int precision,

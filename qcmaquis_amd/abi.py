@@ -99,7 +99,7 @@ class HpsiStats(C.Structure):
 
 class DavidsonStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("residual_norm", C.c_double), ("seconds", C.c_double),
-                ("hpsi_seconds", C.c_double)]
+                ("hpsi_seconds", C.c_double), ("refine_iterations", C.c_int32)]
 
 
 vp = C.c_void_p
@@ -109,6 +109,7 @@ dp = P(C.c_double)
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("mq_version", i32, []),
+    ("mq_abi_struct_sizes", i32, [P(C.c_uint32), i32]),
     ("mq_ctx_create", i32, [i32, i32, P(vp)]),
     ("mq_ctx_destroy", i32, [vp]),
     ("mq_last_error", C.c_char_p, [vp]),
@@ -161,5 +162,12 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        # the library writes these structs: a layout mismatch would overwrite Python memory
+        mine = [C.sizeof(t) for t in (IndexDesc, MPODesc, BIndexDesc, BoundaryInfo, HpsiStats, DavidsonStats)]
+        theirs = (C.c_uint32 * len(mine))()
+        got = L.mq_abi_struct_sizes(theirs, len(mine))
+        if got != len(mine) or list(theirs) != mine:
+            raise ImportError("libqcmaquis_amd.so struct layouts %s do not match the Python binding %s; rebuild"
+                              % (list(theirs)[:got], mine))
         _lib = L
     return _lib

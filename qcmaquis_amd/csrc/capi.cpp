@@ -111,7 +111,7 @@ int guarded(mq_ctx_t ctx, F&& f) {
 }
 
 // device copy of a grouped GEMM stage and its schedule (tiles per shape, split-K reduce jobs and
-// the workspace of the partial tiles)
+// the workspace of the partial tiles, sized for FP64 and shared by the FP32 launches)
 struct GemmStage {
     GemmDesc* probs = nullptr;
     GemmTile* tiles[GEMM_N_SHAPES] = {nullptr, nullptr, nullptr};
@@ -130,10 +130,8 @@ struct GemmStage {
         n_reduces = int64_t(sch.reduces.size());
         ws = dev_alloc<double>(sch.ws_elems);
     }
-    void run(Bases const& b, hipStream_t s, int precision) const {
-        if (precision != MQ_PREC_FP64)
-            throw Error(MQ_ERR_UNSUPPORTED, "only MQ_PREC_FP64 sector GEMMs are available in this build");
-        hipcheck(launch_gemm_f64(probs, tiles, n_tiles, reduces, n_reduces, ws, b, s), "gemm launch");
+    void run(Bases const& b, hipStream_t s, GemmPrec prec) const {
+        hipcheck(launch_gemm(prec, probs, tiles, n_tiles, reduces, n_reduces, ws, b, s), "gemm launch");
     }
     void release() {
         dev_free(probs);
@@ -146,15 +144,17 @@ struct GemmStage {
     }
 };
 
-// device copy of the block-form MPO-term sum
+// device copy of the block-form MPO-term sum; the FP32 coefficient table is made on first use
 struct SAsmStage {
     SBlock* blocks = nullptr;
     SWork* work = nullptr;
     int64_t* utile = nullptr;
     double* ualpha = nullptr;
+    float* ualpha32 = nullptr;
     int64_t* lsoff = nullptr;
     int32_t* llds = nullptr;
     int64_t n_work = 0;
+    uint64_t n_alpha = 0;
     void upload(SBlockTable const& B, hipStream_t s) {
         blocks = dev_upload(B.blocks, s);
         work = dev_upload(B.work, s);
@@ -163,13 +163,23 @@ struct SAsmStage {
         lsoff = dev_upload(B.lsoff, s);
         llds = dev_upload(B.llds, s);
         n_work = int64_t(B.work.size());
+        n_alpha = B.ualpha.size();
     }
     void run(const double* T, double* S, hipStream_t s) const {
-        hipcheck(launch_sblocks(blocks, work, n_work, utile, ualpha, lsoff, llds, T, S, s), "S-assembly");
+        hipcheck(launch_sblocks<double>(blocks, work, n_work, utile, ualpha, lsoff, llds, T, S, s), "S-assembly");
+    }
+    void run32(const float* T, float* S, hipStream_t s) {
+        if (!ualpha32) {
+            ualpha32 = dev_alloc<float>(n_alpha);
+            hipcheck(launch_convert(ualpha, ualpha32, n_alpha, s), "convert");
+        }
+        hipcheck(launch_sblocks<float>(blocks, work, n_work, utile, ualpha32, lsoff, llds, T, S, s), "S-assembly");
     }
     void release() {
-        dev_free(blocks); dev_free(work); dev_free(utile); dev_free(ualpha); dev_free(lsoff); dev_free(llds);
-        blocks = nullptr; work = nullptr; utile = nullptr; ualpha = nullptr; lsoff = nullptr; llds = nullptr;
+        dev_free(blocks); dev_free(work); dev_free(utile); dev_free(ualpha); dev_free(ualpha32); dev_free(lsoff);
+        dev_free(llds);
+        blocks = nullptr; work = nullptr; utile = nullptr; ualpha = nullptr; ualpha32 = nullptr; lsoff = nullptr;
+        llds = nullptr;
         n_work = 0;
     }
 };
@@ -203,6 +213,9 @@ struct mq_hpsi_plan_s {
     int64_t n_lines = 0, n_sblocks = 0;
     uint64_t s_line_elements = 0, gathered_elements = 0, union_elements = 0;
     double *T = nullptr, *S = nullptr, *Lpanel = nullptr;
+    // FP32 operands of the mixed-precision H*psi, made on first use
+    float *T32 = nullptr, *S32 = nullptr, *Lpanel32 = nullptr, *R32 = nullptr, *psi32 = nullptr;
+    uint64_t r_elems = 0;
     double build_seconds = 0;
 };
 
@@ -210,6 +223,15 @@ struct mq_hpsi_plan_s {
 extern "C" {
 
 int mq_version(void) { return 1; }
+
+int mq_abi_struct_sizes(uint32_t* sizes, int n) {
+    const uint32_t v[] = {uint32_t(sizeof(mq_index_desc)), uint32_t(sizeof(mq_mpo_desc)),
+                          uint32_t(sizeof(mq_bindex_desc)), uint32_t(sizeof(mq_boundary_info_t)),
+                          uint32_t(sizeof(mq_hpsi_stats_t)), uint32_t(sizeof(mq_davidson_stats_t))};
+    int k = 0;
+    for (; sizes && k < n && k < int(sizeof(v) / sizeof(v[0])); ++k) sizes[k] = v[k];
+    return k;
+}
 
 double mq_wigner_6j(int a, int b, int c, int d, int e, int f) { return wigner_6j(a, b, c, d, e, f); }
 double mq_wigner_9j(int a, int b, int c, int d, int e, int f, int g, int h, int i) {
@@ -452,9 +474,9 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
         b.p[BASE_BRA] = const_cast<double*>(bra);
         b.p[BASE_KET] = const_cast<double*>(ket);
         b.p[BASE_NEW] = nb->data;
-        tg.run(b, ctx->stream, precision);
+        tg.run(b, ctx->stream, GEMM_F64);
         sa.run(T, S, ctx->stream);
-        cg.run(b, ctx->stream, precision);
+        cg.run(b, ctx->stream, GEMM_F64);
         hipcheck(hipStreamSynchronize(ctx->stream), "growth step");
     } catch (...) {
         (void)hipStreamSynchronize(ctx->stream);
@@ -518,6 +540,7 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
                         uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= H.lpanel_elems,
                     "L panel job outside its buffers");
         p->rdata = right->data;
+        p->r_elems = right->n;
         p->T = dev_alloc<double>(H.t_elems);
         p->S = dev_alloc<double>(H.s_elems);
         p->Lpanel = dev_alloc<double>(H.lpanel_elems);
@@ -576,18 +599,63 @@ int mq_hpsi_plan_psi_elements(mq_hpsi_plan_t p, uint64_t* n) {
     return MQ_OK;
 }
 
-static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
+// FP32 copies of the plan's L panels and right boundary plus FP32 T / S / psi buffers
+static void ensure_fp32(mq_hpsi_plan_t p) {
+    if (p->T32) return;
     hipStream_t s = p->ctx->stream;
+    HpsiPlanHost const& H = p->host;
+    p->S32 = dev_alloc<float>(H.s_elems);
+    p->Lpanel32 = dev_alloc<float>(H.lpanel_elems);
+    p->R32 = dev_alloc<float>(p->r_elems);
+    p->psi32 = dev_alloc<float>(H.mps.elements());
+    hipcheck(launch_convert(p->Lpanel, p->Lpanel32, H.lpanel_elems, s), "convert");
+    hipcheck(launch_convert(p->rdata, p->R32, p->r_elems, s), "convert");
+    p->T32 = dev_alloc<float>(H.t_elems);
+}
+
+// H*psi on the context stream; ev (optional, 5 events) brackets the stages set-up, T-GEMMs,
+// term sum and output GEMMs.  FP32: psi, L panels and R are rounded to FP32, the three stages
+// run on the FP32 matrix cores with FP32 T and S, and psi' is written in FP64.
+static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int precision, hipEvent_t* ev = nullptr) {
+    hipStream_t s = p->ctx->stream;
+    auto mark = [&](int i) {
+        if (ev) hipcheck(hipEventRecord(ev[i], s), "event");
+    };
     Bases b{};
-    b.p[BASE_PSI_IN] = const_cast<double*>(in);
     b.p[BASE_PSI_OUT] = out;
-    b.p[BASE_T] = p->T;
-    b.p[BASE_S] = p->S;
-    b.p[BASE_LPANEL] = p->Lpanel;
-    b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
-    p->tg.run(b, s, precision);
-    p->sa.run(p->T, p->S, s);
-    p->cg.run(b, s, precision);
+    if (precision == MQ_PREC_FP64) {
+        b.p[BASE_PSI_IN] = const_cast<double*>(in);
+        b.p[BASE_T] = p->T;
+        b.p[BASE_S] = p->S;
+        b.p[BASE_LPANEL] = p->Lpanel;
+        b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
+        mark(0);
+        mark(1);
+        p->tg.run(b, s, GEMM_F64);
+        mark(2);
+        p->sa.run(p->T, p->S, s);
+        mark(3);
+        p->cg.run(b, s, GEMM_F64);
+        mark(4);
+    } else if (precision == MQ_PREC_FP32) {
+        ensure_fp32(p);
+        b.p[BASE_PSI_IN] = p->psi32;
+        b.p[BASE_T] = p->T32;
+        b.p[BASE_S] = p->S32;
+        b.p[BASE_LPANEL] = p->Lpanel32;
+        b.p[BASE_BOUNDARY] = p->R32;
+        mark(0);
+        hipcheck(launch_convert(in, p->psi32, p->host.mps.elements(), s), "convert");
+        mark(1);
+        p->tg.run(b, s, GEMM_F32);
+        mark(2);
+        p->sa.run32(p->T32, p->S32, s);
+        mark(3);
+        p->cg.run(b, s, GEMM_F32_OUT_F64);
+        mark(4);
+    } else {
+        throw Error(MQ_ERR_UNSUPPORTED, "H*psi precision: MQ_PREC_FP64 and MQ_PREC_FP32 are available");
+    }
 }
 
 int mq_hpsi(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
@@ -606,21 +674,7 @@ int mq_hpsi_timed(mq_hpsi_plan_t p, const double* in, double* out, int precision
         hipEvent_t ev[5] = {};
         try {
             for (auto& e : ev) hipcheck(hipEventCreate(&e), "event create");
-            Bases b{};
-            b.p[BASE_PSI_IN] = const_cast<double*>(in);
-            b.p[BASE_PSI_OUT] = out;
-            b.p[BASE_T] = p->T;
-            b.p[BASE_S] = p->S;
-            b.p[BASE_LPANEL] = p->Lpanel;
-            b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
-            hipcheck(hipEventRecord(ev[0], s), "event");
-                    hipcheck(hipEventRecord(ev[1], s), "event");
-            p->tg.run(b, s, precision);
-            hipcheck(hipEventRecord(ev[2], s), "event");
-            p->sa.run(p->T, p->S, s);
-            hipcheck(hipEventRecord(ev[3], s), "event");
-            p->cg.run(b, s, precision);
-            hipcheck(hipEventRecord(ev[4], s), "event");
+            hpsi_enqueue(p, in, out, precision, ev);
             hipcheck(hipEventSynchronize(ev[4]), "event sync");
             for (int i = 0; i < 4; ++i) hipcheck(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]), "elapsed");
         } catch (...) {
@@ -666,6 +720,11 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
         dev_free(p->T);
         dev_free(p->S);
         dev_free(p->Lpanel);
+        dev_free(p->T32);
+        dev_free(p->S32);
+        dev_free(p->Lpanel32);
+        dev_free(p->R32);
+        dev_free(p->psi32);
     });
     delete p;
     return rc;
@@ -765,10 +824,14 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
             auto project = [&](double* t) {
                 for (int k = 0; k < no; ++k) subtract_along(t, O[k], false);
             };
-            project(V[0]);
             std::vector<double> M(size_t(max_iter) * max_iter, 0.0);
             double theta = 0, rnorm = 0;
             int it = 0;
+            // one Jacobi-Davidson run from V[0] with H*psi in precision `prec`; leaves the Ritz
+            // vector in u
+            auto jd = [&](int prec, double tol_run) {
+            project(V[0]);
+            it = 0;
             while (true) {
                 double* t = V[it];
                 // modified Gram-Schmidt with one refinement when the norm collapses (jacobi.h:409-415)
@@ -780,7 +843,7 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
                 hipcheck(launch_dot(t, t, n, work, sc, s), "dot");
                 hipcheck(launch_scale_rsqrt(t, n, sc, s), "scale");
                 auto h0 = std::chrono::steady_clock::now();
-                hpsi_enqueue(p, t, VA[it], precision);
+                hpsi_enqueue(p, t, VA[it], prec);
                 hipcheck(hipStreamSynchronize(s), "hpsi");
                 hpsi_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
                 for (int i = 0; i <= it; ++i) M[size_t(i) * max_iter + it] = dot_host(V[i], VA[it]);
@@ -801,18 +864,34 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
                 hipcheck(launch_axpby(r, u, n, 1.0, -theta, s), "axpby");  // r = uA - theta u
                 ++it;
                 rnorm = std::sqrt(dot_host(r, r));
-                bool conv = rnorm <= tol * std::fabs(theta) || rnorm < tol;  // iteration.h:57
+                bool conv = rnorm <= tol_run * std::fabs(theta) || rnorm < tol_run;  // iteration.h:57
                 if (conv || it >= max_iter) break;
                 // correction t = -r + (r.u)/(u.u) u  (jcd_gmres_solver with no GMRES steps, jacobi.h:152-156)
                 double dru = dot_host(r, u), duu = dot_host(u, u);
                 hipcheck(launch_axpby(V[it], r, n, 0.0, -1.0, s), "axpby");
                 hipcheck(launch_axpby(V[it], u, n, 1.0, dru / duu, s), "axpby");
             }
+            };
+            int refine = 0;
+            if (precision == MQ_PREC_FP64) {
+                jd(MQ_PREC_FP64, tol);
+            } else {
+                // mixed precision: converge with FP32 H*psi to the FP32 floor, then refine the
+                // Ritz vector with FP64 H*psi until the requested tolerance
+                require(precision == MQ_PREC_FP32, "mq_davidson: precision must be MQ_PREC_FP64 or MQ_PREC_FP32");
+                jd(MQ_PREC_FP32, std::max(tol, 1e-6));
+                int it32 = it;
+                hipcheck(hipMemcpyAsync(V[0], u, n * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
+                jd(MQ_PREC_FP64, tol);
+                refine = it;
+                it += it32;
+            }
             hipcheck(hipMemcpyAsync(psi_out, u, n * sizeof(double), hipMemcpyDeviceToHost, s), "download");
             hipcheck(hipStreamSynchronize(s), "davidson");
             *eval = theta;
             if (stats) {
                 stats->iterations = it;
+                stats->refine_iterations = refine;
                 stats->residual_norm = rnorm;
                 stats->hpsi_seconds = hpsi_s;
                 stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

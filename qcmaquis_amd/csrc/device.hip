@@ -15,25 +15,47 @@
 namespace mq {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
 namespace {
 
 constexpr int BK = 16;
 constexpr int LDS_PAD = 1;
 
-__device__ inline double ld_or_zero(const double* p, bool ok) { return ok ? *p : 0.0; }
+// The 16x16x4 matrix-core step of each compute precision.  Operands (measured on gfx950): lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator register r of lane l holds
+// D[row(l, r)][l % 16] with row = l / 16 + 4 r for FP64 and 4 (l / 16) + r for FP32.
+template <class T> struct Mma;
+template <> struct Mma<double> {
+    typedef double4_t acc_t;
+    static __device__ inline acc_t step(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mma<float> {
+    typedef float4_t acc_t;
+    static __device__ inline acc_t step(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <class T> __device__ inline T ld_or_zero(const T* p, bool ok) { return ok ? *p : T(0); }
 
 // One BM x BN output tile (BM = 32 FM, BN = 32 FN) per 256-thread workgroup; the four waves form a
-// 2 x 2 grid of 16 FM x 16 FN wave tiles, each FM x FN v_mfma_f64_16x16x4f64 blocks.  The tile
-// reduces over [k0, k1) and writes C, or its partial tile into the workspace (split K).
-template <int FM, int FN>
-__global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmDesc* __restrict__ probs,
-                                                       const GemmTile* __restrict__ tiles, Bases bases,
-                                                       double* __restrict__ ws) {
+// 2 x 2 grid of 16 FM x 16 FN wave tiles, each FM x FN matrix-core tiles.  Operands of type TI,
+// computed in TI, output converted to TO.  The tile reduces over [k0, k1) and writes C, or its
+// partial tile (type TI) into the workspace (split K).
+template <int FM, int FN, class TI, class TO>
+__global__ __launch_bounds__(256) void gemm_kernel(const GemmDesc* __restrict__ probs,
+                                                   const GemmTile* __restrict__ tiles, Bases bases,
+                                                   TI* __restrict__ ws) {
     constexpr int BM = 32 * FM, BN = 32 * FN;
     constexpr int NA = 2 * FM, NB = 2 * FN;  // operand elements staged per thread and K slice
-    __shared__ double As[BK][BM + LDS_PAD];
-    __shared__ double Bs[BK][BN + LDS_PAD];
+    __shared__ TI As[BK][BM + LDS_PAD];
+    __shared__ TI Bs[BK][BN + LDS_PAD];
+    typedef typename Mma<TI>::acc_t acc_t;
 
     const GemmTile tile = tiles[blockIdx.x];
     const GemmDesc g = probs[tile.prob];
@@ -43,8 +65,8 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmDesc* __restric
     const int m0 = tile.tm * BM, n0 = tile.tn * BN;
     const bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B;
 
-    const double* A = bases.p[g.base_a] + g.a + int64_t(tile.batch) * g.sa;
-    const double* B = bases.p[g.base_b] + g.b + int64_t(tile.batch) * g.sb;
+    const TI* A = static_cast<const TI*>(bases.p[g.base_a]) + g.a + int64_t(tile.batch) * g.sa;
+    const TI* B = static_cast<const TI*>(bases.p[g.base_b]) + g.b + int64_t(tile.batch) * g.sb;
 
     // per-thread load coordinates inside the BM x 16 A slice and the 16 x BN B slice, with the
     // contiguous dimension of the operand across consecutive threads
@@ -60,29 +82,29 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmDesc* __restric
         else     { b_n[i] = tid % BN; b_k[i] = tid / BN + (256 / BN) * i; }
     }
     const int k_end = tile.k1;
-    double ra[NA], rb[NB];
+    TI ra[NA], rb[NB];
     auto load = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
             int m = m0 + a_m[i], k = k0 + a_k[i];
             bool ok = (m < g.M) && (k < k_end);
-            const double* pa = ta ? A + int64_t(m) * g.lda + k : A + m + int64_t(k) * g.lda;
+            const TI* pa = ta ? A + int64_t(m) * g.lda + k : A + m + int64_t(k) * g.lda;
             ra[i] = ld_or_zero(pa, ok);
         }
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
             int n = n0 + b_n[i], kb = k0 + b_k[i];
             bool okb = (n < g.N) && (kb < k_end);
-            const double* pb = tb ? B + n + int64_t(kb) * g.ldb : B + kb + int64_t(n) * g.ldb;
+            const TI* pb = tb ? B + n + int64_t(kb) * g.ldb : B + kb + int64_t(n) * g.ldb;
             rb[i] = ld_or_zero(pb, okb);
         }
     };
 
-    double4_t acc[FM][FN];
+    acc_t acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     const int wm = (wave & 1) * 16 * FM, wn = (wave >> 1) * 16 * FN;
     const int fr = lane & 15, fk = lane >> 4;
@@ -98,7 +120,7 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmDesc* __restric
         if (k0 + BK < k_end) load(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
-            double af[FM], bf[FN];
+            TI af[FM], bf[FN];
 #pragma unroll
             for (int i = 0; i < FM; ++i) af[i] = As[kk + fk][wm + 16 * i + fr];
 #pragma unroll
@@ -107,15 +129,13 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmDesc* __restric
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < FN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < FN; ++j) acc[i][j] = Mma<TI>::step(bf[j], af[i], acc[i][j]);
         }
     }
 
-    // Output layout of v_mfma_f64_16x16x4f64 (measured on gfx950): register r of lane l holds
-    // D[l / 16 + 4 r][l % 16].  With the swapped operands D = C^T, so lane l holds
-    // C[l % 16][l / 16 + 4 r]: consecutive lanes store consecutive rows of a column.
-    double* C = bases.p[g.base_c] + g.c + int64_t(tile.batch) * g.sc;
+    // D = C^T: register r of lane l holds C[l % 16][row(l, r)] of its 16 x 16 tile, so
+    // consecutive lanes store consecutive rows of a column
+    TO* C = static_cast<TO*>(bases.p[g.base_c]) + g.c + int64_t(tile.batch) * g.sc;
     const bool split = tile.ws >= 0;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -125,74 +145,76 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(const GemmDesc* __restric
             if (m >= g.M) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int nn = wn + 16 * j + fk + 4 * r, n = n0 + nn;
+                int nn = wn + 16 * j + Mma<TI>::row(lane, r), n = n0 + nn;
                 if (n >= g.N) continue;
                 if (split) ws[tile.ws + mm + int64_t(nn) * BM] = acc[i][j][r];
-                else C[m + int64_t(n) * g.ldc] = acc[i][j][r];
+                else C[m + int64_t(n) * g.ldc] = TO(acc[i][j][r]);
             }
         }
 }
 
 // C tile = sum of its split partials, added in split order
+template <class TW, class TO>
 __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __restrict__ probs,
                                                           const GemmReduce* __restrict__ jobs, Bases bases,
-                                                          const double* __restrict__ ws) {
+                                                          const TW* __restrict__ ws) {
     const GemmReduce job = jobs[blockIdx.x];
     const GemmDesc g = probs[job.prob];
     const int bm = gemm_bm(job.shape), bn = gemm_bn(job.shape);
     const int m0 = job.tm * bm, n0 = job.tn * bn;
-    double* C = bases.p[g.base_c] + g.c + int64_t(job.batch) * g.sc;
+    TO* C = static_cast<TO*>(bases.p[g.base_c]) + g.c + int64_t(job.batch) * g.sc;
     const int64_t tile_elems = int64_t(bm) * bn;
     for (int idx = threadIdx.x; idx < bm * bn; idx += 256) {
         const int mm = idx % bm, nn = idx / bm;
         const int m = m0 + mm, n = n0 + nn;
         if (m >= g.M || n >= g.N) continue;
-        double sum = 0.0;
+        TW sum = 0;
         for (int s = 0; s < job.nsplit; ++s) sum += ws[job.ws + s * tile_elems + idx];
-        C[m + int64_t(n) * g.ldc] = sum;
+        C[m + int64_t(n) * g.ldc] = TO(sum);
     }
 }
 
 // Block-form MPO-term sum on the matrix cores.  A work item is SBLOCK_ELEMS consecutive tile
 // elements of one block of 16 lines; each of the four waves owns 16 * SB_NC of them and computes
-// Y(16 lines x 16 elements) += alpha(16 x 4 tiles) * X(4 tiles x 16 elements) per
-// v_mfma_f64_16x16x4f64 over the block's union of T tiles.  Lane l loads alpha of line l % 16
-// and union tile l / 16, and element l % 16 of that tile: 16 consecutive doubles per tile.
+// Y(16 lines x 16 elements) += alpha(16 x 4 tiles) * X(4 tiles x 16 elements) per matrix-core step
+// over the block's union of T tiles.  Lane l loads alpha of line l % 16 and union tile l / 16, and
+// element l % 16 of that tile: 16 consecutive values per tile.
 constexpr int SB_NC = SBLOCK_ELEMS / 64;
 static_assert(SBLOCK_LINES == 16 && SB_NC * 64 == SBLOCK_ELEMS, "S block geometry");
 
+template <class T>
 __global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ blocks, const SWork* __restrict__ work,
-                                                     const int64_t* __restrict__ utile,
-                                                     const double* __restrict__ ualpha,
+                                                     const int64_t* __restrict__ utile, const T* __restrict__ ualpha,
                                                      const int64_t* __restrict__ lsoff,
-                                                     const int32_t* __restrict__ llds, const double* __restrict__ T,
-                                                     double* __restrict__ S) {
+                                                     const int32_t* __restrict__ llds, const T* __restrict__ X,
+                                                     T* __restrict__ S) {
+    typedef typename Mma<T>::acc_t acc_t;
     const SWork w = work[blockIdx.x];
     const SBlock b = blocks[w.blk];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n0 = w.n0 + wave * 16 * SB_NC;
     if (n0 >= b.E) return;  // no barriers below: waves past the tile end leave early
     const int li = lane & 15, lk = lane >> 4;
-    double4_t acc[SB_NC];
+    acc_t acc[SB_NC];
 #pragma unroll
-    for (int j = 0; j < SB_NC; ++j) acc[j] = double4_t{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < SB_NC; ++j) acc[j] = acc_t{0, 0, 0, 0};
     const int64_t* ut = utile + b.u_off;
-    const double* ua = ualpha + b.u_off * SBLOCK_LINES;
+    const T* ua = ualpha + b.u_off * SBLOCK_LINES;
     for (int ks = 0; ks < b.nu; ks += 4) {
         const int k = ks + lk;
         const bool kv = k < b.nu;
-        const double a = kv ? ua[k * SBLOCK_LINES + li] : 0.0;  // A[line li][tile k]
+        const T a = kv ? ua[k * SBLOCK_LINES + li] : T(0);  // A[line li][tile k]
         const int64_t to = kv ? ut[k] : 0;
-        double x[SB_NC];
+        T x[SB_NC];
 #pragma unroll
         for (int j = 0; j < SB_NC; ++j) {
             const int e = n0 + 16 * j + li;
-            x[j] = (kv && e < b.E) ? T[to + e] : 0.0;  // B[tile k][element e]
+            x[j] = (kv && e < b.E) ? X[to + e] : T(0);  // B[tile k][element e]
         }
 #pragma unroll
-        for (int j = 0; j < SB_NC; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x[j], acc[j], 0, 0, 0);
+        for (int j = 0; j < SB_NC; ++j) acc[j] = Mma<T>::step(a, x[j], acc[j]);
     }
-    // D[i][e]: lane l, register r -> line i = l / 16 + 4 r, element e = l % 16
+    // register r of lane l holds Y[line row(l, r)][element l % 16]
 #pragma unroll
     for (int j = 0; j < SB_NC; ++j) {
         const int e = n0 + 16 * j + li;
@@ -200,10 +222,16 @@ __global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ 
         const int r_ = e % b.rows, c_ = e / b.rows;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int i = lk + 4 * r;
+            const int i = Mma<T>::row(lane, r);
             if (i < b.nlines) S[lsoff[b.line0 + i] + r_ + int64_t(c_) * llds[b.line0 + i]] = acc[j][r];
         }
     }
+}
+
+template <class TI, class TO>
+__global__ void convert_kernel(const TI* __restrict__ src, TO* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = TO(src[i]);
 }
 
 __global__ void fill_kernel(double* p, uint64_t n, double v) {
@@ -300,38 +328,67 @@ inline unsigned grid_for(uint64_t n, int threads, unsigned cap = 8192) {
 
 }  // namespace
 
-hipError_t launch_gemm_f64(const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
-                           const GemmReduce* reduces, int64_t n_reduces, double* ws, Bases bases, hipStream_t stream) {
+namespace {
+
+template <class TI, class TO>
+void launch_gemm_t(const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
+                   const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases, hipStream_t stream) {
+    TI* w = static_cast<TI*>(ws);
     for (int sh = 0; sh < GEMM_N_SHAPES; ++sh) {
         for (int64_t off = 0; off < n_tiles[sh]; off += 0x7fffffff) {
             int64_t cnt = n_tiles[sh] - off < 0x7fffffff ? n_tiles[sh] - off : 0x7fffffff;
             if (sh == GEMM_32x128)
-                hipLaunchKernelGGL((gemm_f64_kernel<1, 4>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
-                                   tiles[sh] + off, bases, ws);
+                hipLaunchKernelGGL((gemm_kernel<1, 4, TI, TO>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                                   tiles[sh] + off, bases, w);
             else if (sh == GEMM_64x64)
-                hipLaunchKernelGGL((gemm_f64_kernel<2, 2>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
-                                   tiles[sh] + off, bases, ws);
+                hipLaunchKernelGGL((gemm_kernel<2, 2, TI, TO>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                                   tiles[sh] + off, bases, w);
             else
-                hipLaunchKernelGGL((gemm_f64_kernel<4, 2>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
-                                   tiles[sh] + off, bases, ws);
+                hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                                   tiles[sh] + off, bases, w);
         }
     }
     for (int64_t off = 0; off < n_reduces; off += 0x7fffffff) {
         int64_t cnt = n_reduces - off < 0x7fffffff ? n_reduces - off : 0x7fffffff;
-        hipLaunchKernelGGL(gemm_reduce_kernel, dim3(unsigned(cnt)), dim3(256), 0, stream, probs, reduces + off, bases,
-                           ws);
+        hipLaunchKernelGGL((gemm_reduce_kernel<TI, TO>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                           reduces + off, bases, w);
+    }
+}
+
+}  // namespace
+
+hipError_t launch_gemm(GemmPrec prec, const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
+                       const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases, hipStream_t stream) {
+    switch (prec) {
+        case GEMM_F64: launch_gemm_t<double, double>(probs, tiles, n_tiles, reduces, n_reduces, ws, bases, stream); break;
+        case GEMM_F32: launch_gemm_t<float, float>(probs, tiles, n_tiles, reduces, n_reduces, ws, bases, stream); break;
+        case GEMM_F32_OUT_F64:
+            launch_gemm_t<float, double>(probs, tiles, n_tiles, reduces, n_reduces, ws, bases, stream);
+            break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+template <class T>
 hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_work, const int64_t* utile,
-                          const double* ualpha, const int64_t* lsoff, const int32_t* llds, const double* T, double* S,
+                          const T* ualpha, const int64_t* lsoff, const int32_t* llds, const T* X, T* S,
                           hipStream_t stream) {
     for (int64_t off = 0; off < n_work; off += 0x7fffffff) {
         int64_t cnt = n_work - off < 0x7fffffff ? n_work - off : 0x7fffffff;
-        hipLaunchKernelGGL(sblock_kernel, dim3(unsigned(cnt)), dim3(256), 0, stream, blocks, work + off, utile, ualpha,
-                           lsoff, llds, T, S);
+        hipLaunchKernelGGL(sblock_kernel<T>, dim3(unsigned(cnt)), dim3(256), 0, stream, blocks, work + off, utile,
+                           ualpha, lsoff, llds, X, S);
     }
+    return hipGetLastError();
+}
+template hipError_t launch_sblocks<double>(const SBlock*, const SWork*, int64_t, const int64_t*, const double*,
+                                           const int64_t*, const int32_t*, const double*, double*, hipStream_t);
+template hipError_t launch_sblocks<float>(const SBlock*, const SWork*, int64_t, const int64_t*, const float*,
+                                          const int64_t*, const int32_t*, const float*, float*, hipStream_t);
+
+hipError_t launch_convert(const double* src, float* dst, uint64_t n, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, stream, src, dst, n);
     return hipGetLastError();
 }
 

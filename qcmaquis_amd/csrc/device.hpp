@@ -10,19 +10,28 @@
 
 namespace mq {
 
-// operand base pointers of a grouped GEMM launch, indexed by BASE_*
+// operand base pointers of a grouped GEMM launch, indexed by BASE_* (element type given by the
+// launch's precision)
 struct Bases {
-    double* p[BASE_COUNT];
+    void* p[BASE_COUNT];
 };
 
+// arithmetic of a grouped GEMM launch: operands, compute and workspace in FP64 or FP32; the
+// output in the same type, or FP64 from FP32 operands
+enum GemmPrec { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_F32_OUT_F64 = 2 };
+
 // grouped GEMM stage: tiles of each shape (GemmShape) then the split-K reduce jobs; ws is the
-// workspace of the split partials (GemmSchedule::ws_elems doubles)
-hipError_t launch_gemm_f64(const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
-                           const GemmReduce* reduces, int64_t n_reduces, double* ws, Bases bases, hipStream_t stream);
-// block-form MPO-term sum (SBlockTable uploaded): writes every S tile of the table's lines
+// workspace of the split partials (GemmSchedule::ws_elems elements of the compute type)
+hipError_t launch_gemm(GemmPrec prec, const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
+                       const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases, hipStream_t stream);
+// block-form MPO-term sum (SBlockTable uploaded; alpha, T and S of type T): writes every S tile
+// of the table's lines
+template <class T>
 hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_work, const int64_t* utile,
-                          const double* ualpha, const int64_t* lsoff, const int32_t* llds, const double* T, double* S,
+                          const T* ualpha, const int64_t* lsoff, const int32_t* llds, const T* X, T* S,
                           hipStream_t stream);
+// dst = float(src)
+hipError_t launch_convert(const double* src, float* dst, uint64_t n, hipStream_t stream);
 hipError_t launch_fill(double* p, uint64_t n, double v, hipStream_t stream);
 hipError_t launch_copy_jobs(const CopyJob* jobs, int32_t n_jobs, const double* src, double* dst, hipStream_t stream);
 

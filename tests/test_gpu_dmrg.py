@@ -48,6 +48,11 @@ def test_davidson_vs_dense(ctx2u1, oracle_lib):
     ev1, vec1, _ = plan.davidson(psi2, ortho=[vec], tol=1e-10, max_iter=min(n, 60))
     assert abs(ev1 - w[1]) < 1e-7 * max(1.0, abs(w[1]))
     assert abs(vec1 @ vec) < 1e-8
+    # mixed precision: FP32 H*psi to the FP32 floor, then FP64 refinement to the same tolerance
+    ev2, vec2, st2 = plan.davidson(psi2, tol=1e-10, max_iter=min(n, 60), precision=abi.MQ_PREC_FP32)
+    print(f"mixed davidson {ev2:.12f} iters {st2['iterations']} refine {st2['refine_iterations']}", flush=True)
+    assert abs(ev2 - w[0]) < 1e-8 * max(1.0, abs(w[0]))
+    assert st2["refine_iterations"] >= 1
 
 
 @pytest.mark.parametrize("M,tol", [(400, 1e-9), (16, None)])

@@ -81,6 +81,35 @@ def test_twou1_large_sectors(ctx2u1, oracle_lib):
     _hpsi_all_sites(ctx2u1, oracle_lib, ch, mpos, L_dev, L_or, R_dev, R_or)
 
 
+@pytest.mark.parametrize("sym", [abi.MQ_SYM_TWOU1, abi.MQ_SYM_SU2U1])
+def test_fp32_hpsi(ctx2u1, ctxsu2, oracle_lib, sym):
+    """mixed precision: FP32 operands and intermediates on the FP32 matrix cores, psi' in FP64;
+    max-norm relative error against the FP64 oracle below 1e-5 (FP32 rounding of psi, L, R, T
+    and S with K-long FP32 sums)"""
+    from qcmaquis_amd.synthetic import TWOU1_SITE, fuse_index
+    ctx = ctx2u1 if sym == abi.MQ_SYM_TWOU1 else ctxsu2
+    print(f"fp32 hpsi sym={sym}", flush=True)
+    if sym == abi.MQ_SYM_TWOU1:
+        ch = make_chain(5, n_sites=4, total=(4, 4), max_m=90, D=6, phys=fuse_index(TWOU1_SITE, TWOU1_SITE),
+                        density=0.3)
+    else:
+        ch = make_chain(4, sym=sym, n_sites=5, total=(4, 0), max_m=8, D=9)
+    mpos, L_dev, L_or, R_dev, R_or = _grow_both(ctx, oracle_lib, ch)
+    rng = np.random.default_rng(5)
+    for k in range(len(ch.mpos)):
+        lay = ch.layouts[k]
+        plan = HpsiPlan(ctx, mpos[k], L_dev[k], R_dev[k + 1], lay)
+        psi = rng.normal(size=lay.n)
+        ref = oracle_lib.hpsi(ch.sym, ch.mpos[k], L_or[k], R_or[k + 1], lay, psi)
+        got = plan.apply(psi, precision=abi.MQ_PREC_FP32)
+        err = rel_err(got, ref)
+        print(f"fp32 hpsi site {k}: n={lay.n} err={err:.2e}", flush=True)
+        assert err < 1e-5
+        # FP64 after FP32 on the same plan is unchanged
+        assert rel_err(plan.apply(psi), ref) < TOL
+        plan.close()
+
+
 @pytest.mark.parametrize("seed,max_m,D", [(1, 3, 6), (4, 8, 9)])
 def test_su2u1_chain(ctxsu2, oracle_lib, seed, max_m, D):
     print(f"su2u1 chain seed={seed} max_m={max_m} D={D}", flush=True)
