@@ -21,7 +21,7 @@ namespace {
 // K chunk of 16 so that every reduction longer than one K slice goes through the split path.
 void run_gemms(std::vector<GemmDesc> const& G, double* const* base) {
     GemmSchedule sch = build_gemm_schedule(G, int64_t(1) << 40, 16);
-    std::vector<double> ws(sch.ws_elems + 1, 0.0);
+    std::vector<double> ws(sch.ws_elems + 1, std::nan(""));
     std::vector<char> seen(sch.ws_elems + 1, 0);
     for (int sh = 0; sh < GEMM_N_SHAPES; ++sh) {
         const int bm = gemm_bm(sh), bn = gemm_bn(sh);
@@ -161,7 +161,7 @@ int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const dou
         for (auto const& j : P.lpanel_jobs)
             require(uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= Ln &&
                     uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= P.lpanel_elems, "copy job out of range");
-        std::vector<double> T(P.t_elems + 1), S(P.s_elems + 1, std::nan("")), Lp(P.lpanel_elems + 1);
+        std::vector<double> T(P.t_elems + 1, std::nan("")), S(P.s_elems + 1, std::nan("")), Lp(P.lpanel_elems + 1, std::nan(""));
         run_copy(P.lpanel_jobs, Ldata, Lp.data());
         double* base[BASE_COUNT] = {nullptr};
         base[BASE_PSI_IN] = const_cast<double*>(psi);
@@ -202,7 +202,7 @@ static EmuBoundary* grow(bool left, int sym, const mq_mpo_desc* md, const mq_bin
     check_gemms(P.c_gemms, sizes);
     check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
     auto* res = new EmuBoundary{std::move(P.out), std::vector<double>(P.out_elems, 0.0)};
-    std::vector<double> T(P.t_elems + 1), S(P.s_elems + 1, std::nan(""));
+    std::vector<double> T(P.t_elems + 1, std::nan("")), S(P.s_elems + 1, std::nan(""));
     double* base[BASE_COUNT] = {nullptr};
     base[BASE_T] = T.data();
     base[BASE_S] = S.data();
