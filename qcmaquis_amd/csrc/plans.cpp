@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <memory>
 #include <unordered_map>
 #include <unordered_set>
@@ -125,6 +126,41 @@ void add_empty_lines(CohortAcc::Unit const& u, uint32_t nrows, int32_t term_pos,
             line.t0 = line.t1 = term_pos;
             out.push_back(line);
         }
+}
+
+// S lines and terms of all cohorts, built in parallel (emit(c, lines, terms) appends the lines of
+// cohort c with term ranges relative to its own term list) and concatenated in cohort order
+template <class Emit>
+void emit_lines(size_t n_cohorts, std::vector<SLine>& lines, std::vector<STerm>& terms, Emit emit) {
+    std::vector<std::vector<SLine>> cl(n_cohorts);
+    std::vector<std::vector<STerm>> ct(n_cohorts);
+    std::string err;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t c = 0; c < int64_t(n_cohorts); ++c) {
+        try {
+            emit(size_t(c), cl[size_t(c)], ct[size_t(c)]);
+        } catch (std::exception const& e) {
+#pragma omp critical
+            err = e.what();
+        }
+    }
+    if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
+    size_t nl = lines.size(), nt = terms.size();
+    for (size_t c = 0; c < n_cohorts; ++c) nl += cl[c].size(), nt += ct[c].size();
+    require(nt < size_t(INT32_MAX), "S table: more than 2^31 terms in one step");
+    lines.reserve(nl);
+    terms.reserve(nt);
+    for (size_t c = 0; c < n_cohorts; ++c) {
+        int32_t base = int32_t(terms.size());
+        for (SLine l : cl[c]) {
+            l.t0 += base;
+            l.t1 += base;
+            lines.push_back(l);
+        }
+        terms.insert(terms.end(), ct[c].begin(), ct[c].end());
+        std::vector<SLine>().swap(cl[c]);
+        std::vector<STerm>().swap(ct[c]);
+    }
 }
 
 inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
@@ -392,8 +428,19 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
         }
         if (uint64_t(rec.nb) * ls_in * ls_out) P.lpanel_jobs.push_back(job);
         l_used.insert(rec.ci_eff);
-        // S lines
-        for (auto& u : rec.acc.units) {
+        for (auto& u : rec.acc.units) P.flops.flops_total += 2ull * ls_in * u.ms * u.alpha.size();
+        uint64_t g = 2ull * ls_out * ls_in * rec.nb * rec.acc.stripe;
+        P.flops.flops_total += g;
+        P.flops.flops_gemm += g;
+        P.flops.flops_gemm_exec += g;
+        P.flops.n_terms += rec.acc.n_terms;
+    }
+    // S lines
+    emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c, std::vector<SLine>& lines, std::vector<STerm>& terms) {
+        auto const& rec = cohorts[c];
+        uint32_t ls_in = left_i.sizes[rec.lb_in];
+        uint64_t ldS = ktot[rec.lb_out];
+        for (auto const& u : rec.acc.units) {
             uint32_t seek = 0;
             for (size_t l = 0; l < u.line_count.size(); ++l) {
                 SLine line{};
@@ -401,17 +448,17 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                 line.lds = int32_t(ldS);
                 line.rows = int32_t(ls_in);
                 line.cols = int32_t(u.ms);
-                line.t0 = int32_t(P.terms.size());
+                line.t0 = int32_t(terms.size());
                 for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
                     TTask const& t = tlists[rec.lb_in].tasks[u.ti[k]];
                     check_tile(t, uint64_t(u.col[k]) * ls_in, ls_in, u.ms);
-                    P.terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * ls_in)});
+                    terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * ls_in)});
                 }
-                line.t1 = int32_t(P.terms.size());
+                line.t1 = int32_t(terms.size());
                 seek += u.line_count[l];
-                P.lines.push_back(line);
+                lines.push_back(line);
             }
-            add_empty_lines(u, rec.nb, int32_t(P.terms.size()), P.lines, [&](uint32_t b) {
+            add_empty_lines(u, rec.nb, int32_t(terms.size()), lines, [&](uint32_t b) {
                 SLine line{};
                 line.s_off = int64_t(s_off[rec.lb_out] + koff[c] + uint64_t(b) * ls_in + uint64_t(u.offset) * ldS);
                 line.lds = int32_t(ldS);
@@ -419,20 +466,8 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                 line.cols = int32_t(u.ms);
                 return line;
             });
-            P.flops.flops_total += 2ull * ls_in * u.ms * u.alpha.size();
         }
-        uint64_t g = 2ull * ls_out * ls_in * rec.nb * rec.acc.stripe;
-        P.flops.flops_total += g;
-        P.flops.flops_gemm += g;
-        P.flops.flops_gemm_exec += g;
-        P.flops.n_terms += rec.acc.n_terms;
-    }
-    // Execution order of the lines: by the T position of their first term, so that lines reading
-    // the T tasks of one input block run together and those tasks stay cache-resident (each line
-    // owns a disjoint S tile, so the order does not change any result).
-    auto first_t = [&](SLine const& l) { return l.t1 > l.t0 ? P.terms[l.t0].t_off : INT64_MAX; };
-    std::stable_sort(P.lines.begin(), P.lines.end(),
-                     [&](SLine const& a, SLine const& b) { return first_t(a) < first_t(b); });
+    });
     P.n_cohorts = cohorts.size();
     for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
         uint32_t ls = left_i.sizes[lb];
@@ -632,51 +667,58 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
     }
 
     // S buffers and final GEMMs: L'(rs_bra x nS*rs_ket) = A_bra^T (stripe x rs_bra)^T * S (prop_l, tasks.cpp:198-241)
+    std::vector<uint64_t> sbase(cohorts.size());
     for (size_t c = 0; c < cohorts.size(); ++c) {
         auto& rec = cohorts[c];
         uint32_t rs_ket = ket.right.sizes[rec.rb_ket], rs_bra = bra.right.sizes[rec.rb_bra];
         uint64_t stripe = rec.acc.stripe;
         require(stripe == bra.lp_rows[rec.rb_bra], "lstep: cohort stripe != bra left-paired rows");
-        uint64_t sbase = P.s_elems;
+        sbase[c] = P.s_elems;
         P.s_elems += round_up(stripe * rec.nrows * rs_ket, 32);
-        for (auto& u : rec.acc.units) {
-            uint32_t seek = 0;
-            for (size_t l = 0; l < u.line_count.size(); ++l) {
-                SLine line{};
-                line.s_off = int64_t(sbase + u.offset + uint64_t(u.line_b[l]) * rs_ket * stripe);
-                line.lds = int32_t(stripe);
-                line.rows = int32_t(u.ms);
-                line.cols = int32_t(rs_ket);
-                line.t0 = int32_t(P.terms.size());
-                for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
-                    TTask const& t = tlists[rec.rb_ket].tasks[u.ti[k]];
-                    check_tile(t, uint64_t(u.col[k]) * rs_ket, u.ms, rs_ket);
-                    P.terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * rs_ket)});
-                }
-                line.t1 = int32_t(P.terms.size());
-                seek += u.line_count[l];
-                P.lines.push_back(line);
-            }
-            add_empty_lines(u, rec.nrows, int32_t(P.terms.size()), P.lines, [&](uint32_t b) {
-                SLine line{};
-                line.s_off = int64_t(sbase + u.offset + uint64_t(b) * rs_ket * stripe);
-                line.lds = int32_t(stripe);
-                line.rows = int32_t(u.ms);
-                line.cols = int32_t(rs_ket);
-                return line;
-            });
-            P.flops.flops_total += 2ull * rs_ket * u.ms * u.alpha.size();
-        }
+        for (auto& u : rec.acc.units) P.flops.flops_total += 2ull * rs_ket * u.ms * u.alpha.size();
         uint32_t ci_new = new_ci[c];
         P.c_gemms.push_back(gemm(int32_t(rs_bra), int32_t(rec.nrows * rs_ket), int32_t(stripe), BASE_BRA,
-                                 int64_t(bra.lp_off[rec.rb_bra]), int32_t(stripe), BASE_S, int64_t(sbase), int32_t(stripe),
-                                 BASE_NEW, int64_t(P.out.data_off[ci_new]), int32_t(rs_bra), GEMM_TRANS_A));
+                                 int64_t(bra.lp_off[rec.rb_bra]), int32_t(stripe), BASE_S, int64_t(sbase[c]),
+                                 int32_t(stripe), BASE_NEW, int64_t(P.out.data_off[ci_new]), int32_t(rs_bra),
+                                 GEMM_TRANS_A));
         uint64_t g = 2ull * rs_bra * rec.nrows * rs_ket * stripe;
         P.flops.flops_total += g;
         P.flops.flops_gemm += g;
         P.flops.flops_gemm_exec += g;
         P.flops.n_terms += rec.acc.n_terms;
     }
+    emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c, std::vector<SLine>& lines, std::vector<STerm>& terms) {
+        auto const& rec = cohorts[c];
+        uint32_t rs_ket = ket.right.sizes[rec.rb_ket];
+        uint64_t stripe = rec.acc.stripe;
+        for (auto const& u : rec.acc.units) {
+            uint32_t seek = 0;
+            for (size_t l = 0; l < u.line_count.size(); ++l) {
+                SLine line{};
+                line.s_off = int64_t(sbase[c] + u.offset + uint64_t(u.line_b[l]) * rs_ket * stripe);
+                line.lds = int32_t(stripe);
+                line.rows = int32_t(u.ms);
+                line.cols = int32_t(rs_ket);
+                line.t0 = int32_t(terms.size());
+                for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
+                    TTask const& t = tlists[rec.rb_ket].tasks[u.ti[k]];
+                    check_tile(t, uint64_t(u.col[k]) * rs_ket, u.ms, rs_ket);
+                    terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * rs_ket)});
+                }
+                line.t1 = int32_t(terms.size());
+                seek += u.line_count[l];
+                lines.push_back(line);
+            }
+            add_empty_lines(u, rec.nrows, int32_t(terms.size()), lines, [&](uint32_t b) {
+                SLine line{};
+                line.s_off = int64_t(sbase[c] + u.offset + uint64_t(b) * rs_ket * stripe);
+                line.lds = int32_t(stripe);
+                line.rows = int32_t(u.ms);
+                line.cols = int32_t(rs_ket);
+                return line;
+            });
+        }
+    });
     return P;
 }
 
@@ -856,45 +898,19 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
     }
 
     // S (nS*ls_ket x stripe) and R'_b = S_b * B_bra^T per MPO row (prop_r, tasks.cpp:243-284)
+    std::vector<uint64_t> sbase(cohorts.size());
     for (size_t c = 0; c < cohorts.size(); ++c) {
         auto& rec = cohorts[c];
         uint32_t ls_ket = ket.left.sizes[rec.lb_ket], ls_bra = bra.left.sizes[rec.lb_bra];
         uint64_t stripe = rec.acc.stripe;
         require(stripe == bra.rp_cols[rec.lb_bra], "rstep: cohort stripe != bra right-paired columns");
         uint64_t lds = uint64_t(rec.nrows) * ls_ket;
-        uint64_t sbase = P.s_elems;
+        sbase[c] = P.s_elems;
         P.s_elems += round_up(lds * stripe, 32);
-        for (auto& u : rec.acc.units) {
-            uint32_t seek = 0;
-            for (size_t l = 0; l < u.line_count.size(); ++l) {
-                SLine line{};
-                line.s_off = int64_t(sbase + uint64_t(u.line_b[l]) * ls_ket + uint64_t(u.offset) * lds);
-                line.lds = int32_t(lds);
-                line.rows = int32_t(ls_ket);
-                line.cols = int32_t(u.ms);
-                line.t0 = int32_t(P.terms.size());
-                for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
-                    TTask const& t = tlists[rec.lb_ket].tasks[u.ti[k]];
-                    check_tile(t, uint64_t(u.col[k]) * ls_ket, ls_ket, u.ms);
-                    P.terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * ls_ket)});
-                }
-                line.t1 = int32_t(P.terms.size());
-                seek += u.line_count[l];
-                P.lines.push_back(line);
-            }
-            add_empty_lines(u, rec.nrows, int32_t(P.terms.size()), P.lines, [&](uint32_t b) {
-                SLine line{};
-                line.s_off = int64_t(sbase + uint64_t(b) * ls_ket + uint64_t(u.offset) * lds);
-                line.lds = int32_t(lds);
-                line.rows = int32_t(ls_ket);
-                line.cols = int32_t(u.ms);
-                return line;
-            });
-            P.flops.flops_total += 2ull * ls_ket * u.ms * u.alpha.size();
-        }
+        for (auto& u : rec.acc.units) P.flops.flops_total += 2ull * ls_ket * u.ms * u.alpha.size();
         uint32_t ci_new = new_ci[c];
-        P.c_gemms.push_back(gemm(int32_t(ls_ket), int32_t(ls_bra), int32_t(stripe), BASE_S, int64_t(sbase), int32_t(lds),
-                                 BASE_BRA, int64_t(bra.rp_off[rec.lb_bra]), int32_t(ls_bra), BASE_NEW,
+        P.c_gemms.push_back(gemm(int32_t(ls_ket), int32_t(ls_bra), int32_t(stripe), BASE_S, int64_t(sbase[c]),
+                                 int32_t(lds), BASE_BRA, int64_t(bra.rp_off[rec.lb_bra]), int32_t(ls_bra), BASE_NEW,
                                  int64_t(P.out.data_off[ci_new]), int32_t(ls_ket), GEMM_TRANS_B, int32_t(rec.nrows),
                                  int64_t(ls_ket), 0, int64_t(ls_ket) * ls_bra));
         uint64_t g = 2ull * rec.nrows * ls_ket * stripe * ls_bra;
@@ -903,74 +919,131 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
         P.flops.flops_gemm_exec += g;
         P.flops.n_terms += rec.acc.n_terms;
     }
+    emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c, std::vector<SLine>& lines, std::vector<STerm>& terms) {
+        auto const& rec = cohorts[c];
+        uint32_t ls_ket = ket.left.sizes[rec.lb_ket];
+        uint64_t lds = uint64_t(rec.nrows) * ls_ket;
+        for (auto const& u : rec.acc.units) {
+            uint32_t seek = 0;
+            for (size_t l = 0; l < u.line_count.size(); ++l) {
+                SLine line{};
+                line.s_off = int64_t(sbase[c] + uint64_t(u.line_b[l]) * ls_ket + uint64_t(u.offset) * lds);
+                line.lds = int32_t(lds);
+                line.rows = int32_t(ls_ket);
+                line.cols = int32_t(u.ms);
+                line.t0 = int32_t(terms.size());
+                for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
+                    TTask const& t = tlists[rec.lb_ket].tasks[u.ti[k]];
+                    check_tile(t, uint64_t(u.col[k]) * ls_ket, ls_ket, u.ms);
+                    terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * ls_ket)});
+                }
+                line.t1 = int32_t(terms.size());
+                seek += u.line_count[l];
+                lines.push_back(line);
+            }
+            add_empty_lines(u, rec.nrows, int32_t(terms.size()), lines, [&](uint32_t b) {
+                SLine line{};
+                line.s_off = int64_t(sbase[c] + uint64_t(b) * ls_ket + uint64_t(u.offset) * lds);
+                line.lds = int32_t(lds);
+                line.rows = int32_t(ls_ket);
+                line.cols = int32_t(u.ms);
+                return line;
+            });
+        }
+    });
     return P;
 }
 
 // ---- block form of the MPO-term sum ------------------------------------------------------------
 SBlockTable build_sblocks(std::vector<SLine> const& lines, std::vector<STerm> const& terms) {
     SBlockTable B;
+    const int64_t nl = int64_t(lines.size());
+    // first T tile of every line (lines without terms, i.e. zero tiles, sort last)
+    std::vector<int64_t> tmin(lines.size(), INT64_MAX);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nl; ++i) {
+        SLine const& l = lines[size_t(i)];
+        int64_t mn = INT64_MAX;
+        for (int32_t k = l.t0; k < l.t1; ++k) mn = std::min(mn, terms[size_t(k)].t_off);
+        tmin[size_t(i)] = mn;
+    }
     // lines by tile shape; inside a shape by their first T tile (lines reading the same T region
     // share most of their tiles), longer lines first
-    std::vector<uint32_t> order;
-    std::vector<int64_t> tmin(lines.size(), 0);
-    for (uint32_t i = 0; i < lines.size(); ++i) {
-        SLine const& l = lines[i];
-        if (l.rows <= 0 || l.cols <= 0) continue;
-        int64_t mn = INT64_MAX;  // lines without terms (zero tiles) sort last
-        for (int32_t k = l.t0; k < l.t1; ++k) mn = std::min(mn, terms[k].t_off);
-        tmin[i] = mn;
-        order.push_back(i);
+    std::map<std::pair<int32_t, int32_t>, std::vector<uint32_t>> by_shape;
+    for (uint32_t i = 0; i < lines.size(); ++i)
+        if (lines[i].rows > 0 && lines[i].cols > 0) by_shape[{lines[i].rows, lines[i].cols}].push_back(i);
+    std::vector<std::vector<uint32_t>*> shapes;
+    for (auto& kv : by_shape) shapes.push_back(&kv.second);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t q = 0; q < int64_t(shapes.size()); ++q) {
+        std::vector<uint32_t>& v = *shapes[size_t(q)];
+        std::stable_sort(v.begin(), v.end(), [&](uint32_t x, uint32_t y) {
+            if (tmin[x] != tmin[y]) return tmin[x] < tmin[y];
+            return lines[x].t1 - lines[x].t0 > lines[y].t1 - lines[y].t0;
+        });
     }
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        SLine const& x = lines[a];
-        SLine const& y = lines[b];
-        if (x.rows != y.rows) return x.rows < y.rows;
-        if (x.cols != y.cols) return x.cols < y.cols;
-        if (tmin[a] != tmin[b]) return tmin[a] < tmin[b];
-        return x.t1 - x.t0 > y.t1 - y.t0;
-    });
-    std::vector<int64_t> u;
-    for (size_t s0 = 0; s0 < order.size();) {
-        SLine const& first = lines[order[s0]];
-        size_t s1 = s0;
-        while (s1 < order.size() && s1 - s0 < size_t(SBLOCK_LINES) && lines[order[s1]].rows == first.rows &&
-               lines[order[s1]].cols == first.cols)
-            ++s1;
-        u.clear();
-        for (size_t q = s0; q < s1; ++q) {
-            SLine const& l = lines[order[q]];
-            for (int32_t k = l.t0; k < l.t1; ++k) u.push_back(terms[k].t_off);
+    // blocks: up to SBLOCK_LINES consecutive lines of one shape
+    struct Span { std::vector<uint32_t> const* v; uint32_t s0, n; };
+    std::vector<Span> spans;
+    for (auto* v : shapes)
+        for (uint32_t s0 = 0; s0 < v->size(); s0 += SBLOCK_LINES)
+            spans.push_back(Span{v, s0, std::min<uint32_t>(SBLOCK_LINES, uint32_t(v->size()) - s0)});
+    const int64_t nb = int64_t(spans.size());
+    std::vector<std::vector<int64_t>> unions(spans.size());
+    std::vector<std::vector<double>> alphas(spans.size());
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t bi = 0; bi < nb; ++bi) {
+        Span const& sp = spans[size_t(bi)];
+        std::vector<int64_t>& u = unions[size_t(bi)];
+        for (uint32_t q = 0; q < sp.n; ++q) {
+            SLine const& l = lines[(*sp.v)[sp.s0 + q]];
+            for (int32_t k = l.t0; k < l.t1; ++k) u.push_back(terms[size_t(k)].t_off);
         }
         std::sort(u.begin(), u.end());
         u.erase(std::unique(u.begin(), u.end()), u.end());
-        SBlock blk{};
-        blk.u_off = int64_t(B.utile.size());
-        blk.nu = int32_t(u.size());
-        blk.rows = first.rows;
-        blk.E = first.rows * first.cols;
-        blk.nlines = int32_t(s1 - s0);
-        blk.line0 = int32_t(B.lsoff.size());
-        size_t a0 = B.ualpha.size();
-        B.ualpha.resize(a0 + size_t(SBLOCK_LINES) * u.size(), 0.0);
-        for (size_t q = s0; q < s1; ++q) {
-            SLine const& l = lines[order[q]];
-            B.lsoff.push_back(l.s_off);
-            B.llds.push_back(l.lds);
+        std::vector<double>& al = alphas[size_t(bi)];
+        al.assign(size_t(SBLOCK_LINES) * u.size(), 0.0);
+        for (uint32_t q = 0; q < sp.n; ++q) {
+            SLine const& l = lines[(*sp.v)[sp.s0 + q]];
             for (int32_t k = l.t0; k < l.t1; ++k) {
-                size_t pos = size_t(std::lower_bound(u.begin(), u.end(), terms[k].t_off) - u.begin());
-                B.ualpha[a0 + pos * SBLOCK_LINES + (q - s0)] += terms[k].alpha;
+                size_t pos = size_t(std::lower_bound(u.begin(), u.end(), terms[size_t(k)].t_off) - u.begin());
+                al[pos * SBLOCK_LINES + q] += terms[size_t(k)].alpha;
             }
         }
-        for (size_t q = s1 - s0; q < size_t(SBLOCK_LINES); ++q) {
-            B.lsoff.push_back(0);
-            B.llds.push_back(1);
+    }
+    // concatenate in block order
+    std::vector<int64_t> u_off(spans.size() + 1, 0);
+    for (size_t bi = 0; bi < spans.size(); ++bi) u_off[bi + 1] = u_off[bi] + int64_t(unions[bi].size());
+    B.utile.resize(size_t(u_off.back()));
+    B.ualpha.resize(size_t(u_off.back()) * SBLOCK_LINES);
+    B.lsoff.assign(spans.size() * SBLOCK_LINES, 0);
+    B.llds.assign(spans.size() * SBLOCK_LINES, 1);
+    B.blocks.resize(spans.size());
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t bi = 0; bi < nb; ++bi) {
+        Span const& sp = spans[size_t(bi)];
+        SLine const& first = lines[(*sp.v)[sp.s0]];
+        std::copy(unions[size_t(bi)].begin(), unions[size_t(bi)].end(), B.utile.begin() + u_off[size_t(bi)]);
+        std::copy(alphas[size_t(bi)].begin(), alphas[size_t(bi)].end(),
+                  B.ualpha.begin() + u_off[size_t(bi)] * SBLOCK_LINES);
+        for (uint32_t q = 0; q < sp.n; ++q) {
+            SLine const& l = lines[(*sp.v)[sp.s0 + q]];
+            B.lsoff[size_t(bi) * SBLOCK_LINES + q] = l.s_off;
+            B.llds[size_t(bi) * SBLOCK_LINES + q] = l.lds;
         }
-        B.utile.insert(B.utile.end(), u.begin(), u.end());
-        int32_t bi = int32_t(B.blocks.size());
-        B.blocks.push_back(blk);
+        SBlock& blk = B.blocks[size_t(bi)];
+        blk.u_off = u_off[size_t(bi)];
+        blk.nu = int32_t(unions[size_t(bi)].size());
+        blk.rows = first.rows;
+        blk.E = first.rows * first.cols;
+        blk.nlines = int32_t(sp.n);
+        blk.line0 = int32_t(bi * SBLOCK_LINES);
+        blk.pad = 0;
+    }
+    for (int32_t bi = 0; bi < int32_t(nb); ++bi) {
+        SBlock const& blk = B.blocks[size_t(bi)];
         for (int32_t e = 0; e < blk.E; e += SBLOCK_ELEMS) B.work.push_back(SWork{bi, e});
-        B.gathered += uint64_t(u.size()) * uint64_t(blk.E);
-        s0 = s1;
+        B.gathered += uint64_t(blk.nu) * uint64_t(blk.E);
     }
     B.mac = B.gathered * SBLOCK_LINES;
     return B;
