@@ -309,9 +309,10 @@ __global__ void axpy_dev_kernel(double* __restrict__ y, const double* __restrict
         y[i] += coef * x[i];
 }
 
+// y = a*y + b*x; with a == 0 y is not read (it may hold anything, NaN included: BLAS beta = 0)
 __global__ void axpby_kernel(double* __restrict__ y, const double* __restrict__ x, uint64_t n, double a, double b) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-        y[i] = a * y[i] + b * x[i];
+        y[i] = a == 0.0 ? b * x[i] : a * y[i] + b * x[i];
 }
 
 __global__ void scale_rsqrt_kernel(double* __restrict__ x, uint64_t n, const double* __restrict__ nrm2) {

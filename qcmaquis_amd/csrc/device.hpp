@@ -48,7 +48,7 @@ hipError_t launch_dot(const double* x, const double* y, uint64_t n, double* work
 // y += coef * x with coef = sign * num[0] / (den ? den[0] : 1)  (coefficients stay on the device)
 hipError_t launch_axpy_dev(double* y, const double* x, uint64_t n, const double* num, const double* den, double sign,
                            hipStream_t stream);
-// y = a*y + b*x (host scalars)
+// y = a*y + b*x (host scalars); a == 0 assigns b*x without reading y
 hipError_t launch_axpby(double* y, const double* x, uint64_t n, double a, double b, hipStream_t stream);
 // x *= 1/sqrt(nrm2[0])
 hipError_t launch_scale_rsqrt(double* x, uint64_t n, const double* nrm2, hipStream_t stream);
