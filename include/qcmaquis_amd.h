@@ -129,6 +129,8 @@ typedef struct {
     uint64_t gathered_elements;       /* sum over S terms of rows * cols (T elements of the term sum) */
     uint64_t n_sblocks;               /* 16-line blocks of the matrix-core term sum */
     uint64_t union_elements;          /* sum over blocks of union tiles * tile elements (T elements read) */
+    double   seconds_tables;          /* host: schedule tables built and checked (part of build_seconds) */
+    double   seconds_blocks;          /* host: term-sum block tables (part of build_seconds) */
 } mq_hpsi_stats_t;
 
 typedef struct {

@@ -94,6 +94,8 @@ class HpsiStats(C.Structure):
         ("gathered_elements", C.c_uint64),
         ("n_sblocks", C.c_uint64),
         ("union_elements", C.c_uint64),
+        ("seconds_tables", C.c_double),
+        ("seconds_blocks", C.c_double),
     ]
 
 

@@ -222,7 +222,7 @@ struct mq_hpsi_plan_s {
     // FP32 operands of the mixed-precision H*psi, made on first use
     float *T32 = nullptr, *S32 = nullptr, *Lpanel32 = nullptr, *R32 = nullptr, *psi32 = nullptr;
     uint64_t r_elems = 0;
-    double build_seconds = 0;
+    double build_seconds = 0, seconds_tables = 0, seconds_blocks = 0;
 };
 
 // ================================================================================================
@@ -524,6 +524,9 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
     return guarded(ctx, [&] {
         require(mpo && left && right && out, "mq_hpsi_plan_create: null argument");
         auto t0 = std::chrono::steady_clock::now();
+        auto seconds_since = [](std::chrono::steady_clock::time_point t) {
+            return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+        };
         auto* p = new mq_hpsi_plan_s{ctx, build_hpsi_plan(ctx->sym, mpo->mpo, left->idx, right->idx, index_from(left_i),
                                                           index_from(phys_i), index_from(right_i))};
         std::unique_ptr<mq_hpsi_plan_s> guard(p);
@@ -539,8 +542,11 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
         check_gemms(H.t_gemms, sizes);
         check_gemms(H.c_gemms, sizes);
         check_lines(H.lines, H.terms, H.t_elems, H.s_elems);
+        p->seconds_tables = seconds_since(t0);
+        auto t1 = std::chrono::steady_clock::now();
         SBlockTable SB = build_sblocks(H.lines, H.terms);
         check_sblocks(SB, H.t_elems, H.s_elems);
+        p->seconds_blocks = seconds_since(t1);
         for (auto const& j : H.lpanel_jobs)
             require(j.src >= 0 && j.dst >= 0 && uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= left->n &&
                         uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= H.lpanel_elems,
@@ -595,6 +601,8 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
     s->s_line_elements = p->s_line_elements;
     s->gathered_elements = p->gathered_elements;
     s->n_sblocks = uint64_t(p->n_sblocks);
+    s->seconds_tables = p->seconds_tables;
+    s->seconds_blocks = p->seconds_blocks;
     s->union_elements = p->union_elements;
     return MQ_OK;
 }

@@ -235,7 +235,8 @@ class TwoSiteDMRG:
             m = DeviceMPS(self.ctx, self.lay[k], self.data[k])
             self.right[k] = rstep(self.ctx, self.mpo1[k], self.right[k + 1], m, m)
         self.energies, self.trunc = [], []
-        self.timing = {"davidson": 0.0, "hpsi": 0.0, "plan": 0.0, "svd": 0.0, "grow": 0.0, "merge": 0.0}
+        self.timing = {"davidson": 0.0, "hpsi": 0.0, "plan": 0.0, "svd": 0.0, "grow": 0.0, "merge": 0.0,
+                       "mpo": 0.0, "plan_tables": 0.0, "plan_blocks": 0.0, "plan_total": 0.0}
         self.hpsi_flops = 0
 
     def ts_mpo(self, i):
@@ -246,12 +247,19 @@ class TwoSiteDMRG:
     def step(self, i, to_right):
         t0 = time.perf_counter()
         lay2, psi2 = merge_two(self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1], self.phys2, self.off12)
+        tm = time.perf_counter()
+        mpo = self.ts_mpo(i)
         t1 = time.perf_counter()
-        plan = HpsiPlan(self.ctx, self.ts_mpo(i), self.left[i], self.right[i + 2], lay2)
+        self.timing["mpo"] += t1 - tm
+        plan = HpsiPlan(self.ctx, mpo, self.left[i], self.right[i + 2], lay2)
         t2 = time.perf_counter()
         e, psi, st = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter)
         t3 = time.perf_counter()
-        self.hpsi_flops += plan.stats()["flops_total"] * st["iterations"]
+        pst = plan.stats()
+        self.hpsi_flops += pst["flops_total"] * st["iterations"]
+        self.timing["plan_tables"] += pst["seconds_tables"]
+        self.timing["plan_blocks"] += pst["seconds_blocks"]
+        self.timing["plan_total"] += pst["build_seconds"]
         self.timing["hpsi"] += st["hpsi_seconds"]
         plan.close()
         la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right)
@@ -264,7 +272,7 @@ class TwoSiteDMRG:
             m = DeviceMPS(self.ctx, lb, b)
             self.right[i + 1] = rstep(self.ctx, self.mpo1[i + 1], self.right[i + 2], m, m)
         t5 = time.perf_counter()
-        for k, v in (("merge", t1 - t0), ("plan", t2 - t1), ("davidson", t3 - t2), ("svd", t4 - t3), ("grow", t5 - t4)):
+        for k, v in (("merge", tm - t0), ("plan", t2 - t1), ("davidson", t3 - t2), ("svd", t4 - t3), ("grow", t5 - t4)):
             self.timing[k] += v
         self.energies.append(e + self.ham.ecore)
         self.trunc.append(tw)
