@@ -153,7 +153,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmDesc* __restrict__ 
         }
 }
 
-// C tile = sum of its split partials, added in split order
+// C tile = sum of its split partials, added in split order; blockIdx.y selects 256 elements of
+// the tile, so one output tile is reduced by up to 32 workgroups
+constexpr int REDUCE_CHUNKS = 32;  // 8192 (largest tile) / 256
+static_assert(gemm_bm(GEMM_128x64) * gemm_bn(GEMM_128x64) <= REDUCE_CHUNKS * 256 &&
+                  gemm_bm(GEMM_32x128) * gemm_bn(GEMM_32x128) <= REDUCE_CHUNKS * 256 &&
+                  gemm_bm(GEMM_64x64) * gemm_bn(GEMM_64x64) <= REDUCE_CHUNKS * 256,
+              "reduce grid covers every tile shape");
 template <class TW, class TO>
 __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __restrict__ probs,
                                                           const GemmReduce* __restrict__ jobs, Bases bases,
@@ -164,13 +170,15 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __rest
     const int m0 = job.tm * bm, n0 = job.tn * bn;
     TO* C = static_cast<TO*>(bases.p[g.base_c]) + g.c + int64_t(job.batch) * g.sc;
     const int64_t tile_elems = int64_t(bm) * bn;
-    for (int idx = threadIdx.x; idx < bm * bn; idx += 256) {
+    const int idx = int(blockIdx.y) * 256 + int(threadIdx.x);
+    if (idx < bm * bn) {
         const int mm = idx % bm, nn = idx / bm;
         const int m = m0 + mm, n = n0 + nn;
-        if (m >= g.M || n >= g.N) continue;
-        TW sum = 0;
-        for (int s = 0; s < job.nsplit; ++s) sum += ws[job.ws + s * tile_elems + idx];
-        C[m + int64_t(n) * g.ldc] = TO(sum);
+        if (m < g.M && n < g.N) {
+            TW sum = 0;
+            for (int s = 0; s < job.nsplit; ++s) sum += ws[job.ws + s * tile_elems + idx];
+            C[m + int64_t(n) * g.ldc] = TO(sum);
+        }
     }
 }
 
@@ -351,8 +359,8 @@ void launch_gemm_t(const GemmDesc* probs, const GemmTile* const* tiles, const in
     }
     for (int64_t off = 0; off < n_reduces; off += 0x7fffffff) {
         int64_t cnt = n_reduces - off < 0x7fffffff ? n_reduces - off : 0x7fffffff;
-        hipLaunchKernelGGL((gemm_reduce_kernel<TI, TO>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
-                           reduces + off, bases, w);
+        hipLaunchKernelGGL((gemm_reduce_kernel<TI, TO>), dim3(unsigned(cnt), REDUCE_CHUNKS), dim3(256), 0, stream,
+                           probs, reduces + off, bases, w);
     }
 }
 

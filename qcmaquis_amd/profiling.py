@@ -16,9 +16,24 @@ import sys
 
 
 def short_name(name):
-    m = re.search(r"([A-Za-z_0-9]+)\(", name)
-    base = m.group(1) if m else name
-    return base if base != "kernel" else name[:60]
+    """kernel name without return type, namespaces and argument list; template arguments kept"""
+    head = name.strip()
+    if head.endswith(")"):  # drop the trailing argument list
+        depth = 0
+        for i in range(len(head) - 1, -1, -1):
+            depth += {")": 1, "(": -1}.get(head[i], 0)
+            if depth == 0:
+                head = head[:i]
+                break
+    head = head.replace("(anonymous namespace)::", "")
+    if head.startswith("void "):
+        head = head[5:]
+    depth, cut = 0, 0
+    for i, ch in enumerate(head):  # last "::" outside template brackets
+        depth += {"<": 1, ">": -1}.get(ch, 0)
+        if depth == 0 and head[i:i + 2] == "::":
+            cut = i + 2
+    return head[cut:]
 
 
 def load(path):
