@@ -255,6 +255,16 @@ def main():
                    s_bytes, st["n_sblocks"], st["union_elements"], st["s_line_elements"], st["n_terms"],
                    st["gathered_elements"])}
     dominant, other = (rl_sasm, rl_gemm) if stage[2] >= gemm_ms else (rl_gemm, rl_sasm)
+    # HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter runs of this
+    # default workload (FETCH_SIZE / WRITE_SIZE scaled by a device-copy calibration)
+    try:
+        prof = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))
+        rec = prof.get(dominant["kernel"])
+        if rec and args.workload == "qc" and args.warm_sweeps == 1:
+            dominant["traffic"] = round((rec["fetch_gb"] + rec["write_gb"]) * 1e9)
+            dominant["traffic_source"] = rec["source"]
+    except (OSError, ValueError, KeyError):
+        pass
     line = {
         "metric": METRIC,
         "value": round(value, 4),
