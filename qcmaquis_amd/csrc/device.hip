@@ -47,14 +47,15 @@ template <class T> __device__ inline T ld_or_zero(const T* p, bool ok) { return 
 // 2 x 2 grid of 16 FM x 16 FN wave tiles, each FM x FN matrix-core tiles.  Operands of type TI,
 // computed in TI, output converted to TO.  The tile reduces over [k0, k1) and writes C, or its
 // partial tile (type TI) into the workspace (split K).
-template <int FM, int FN, class TI, class TO>
-__global__ __launch_bounds__(256) void gemm_kernel(const GemmDesc* __restrict__ probs,
-                                                   const GemmTile* __restrict__ tiles, Bases bases,
-                                                   TI* __restrict__ ws) {
+template <int FM, int FN, class TI, class TO, int KS = BK, int MINB = 1>
+__global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restrict__ probs,
+                                                         const GemmTile* __restrict__ tiles, Bases bases,
+                                                         TI* __restrict__ ws) {
     constexpr int BM = 32 * FM, BN = 32 * FN;
-    constexpr int NA = 2 * FM, NB = 2 * FN;  // operand elements staged per thread and K slice
-    __shared__ TI As[BK][BM + LDS_PAD];
-    __shared__ TI Bs[BK][BN + LDS_PAD];
+    constexpr int NA = BM * KS / 256, NB = BN * KS / 256;  // operand elements staged per thread and K slice
+    static_assert(KS % 4 == 0 && NA * 256 == BM * KS && NB * 256 == BN * KS, "K slice geometry");
+    __shared__ TI As[KS][BM + LDS_PAD];
+    __shared__ TI Bs[KS][BN + LDS_PAD];
     typedef typename Mma<TI>::acc_t acc_t;
 
     const GemmTile tile = tiles[blockIdx.x];
@@ -74,11 +75,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmDesc* __restrict__ 
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
         if (!ta) { a_m[i] = tid % BM; a_k[i] = tid / BM + (256 / BM) * i; }
-        else     { a_k[i] = tid & 15; a_m[i] = (tid >> 4) + 16 * i; }
+        else     { a_k[i] = tid % KS; a_m[i] = tid / KS + (256 / KS) * i; }
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-        if (!tb) { b_k[i] = tid & 15; b_n[i] = (tid >> 4) + 16 * i; }
+        if (!tb) { b_k[i] = tid % KS; b_n[i] = tid / KS + (256 / KS) * i; }
         else     { b_n[i] = tid % BN; b_k[i] = tid / BN + (256 / BN) * i; }
     }
     const int k_end = tile.k1;
@@ -110,16 +111,16 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmDesc* __restrict__ 
     const int fr = lane & 15, fk = lane >> 4;
 
     if (tile.k0 < k_end) load(tile.k0);
-    for (int k0 = tile.k0; k0 < k_end; k0 += BK) {
+    for (int k0 = tile.k0; k0 < k_end; k0 += KS) {
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < NA; ++i) As[a_k[i]][a_m[i]] = ra[i];
 #pragma unroll
         for (int i = 0; i < NB; ++i) Bs[b_k[i]][b_n[i]] = rb[i];
         __syncthreads();
-        if (k0 + BK < k_end) load(k0 + BK);
+        if (k0 + KS < k_end) load(k0 + KS);
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
+        for (int kk = 0; kk < KS; kk += 4) {
             TI af[FM], bf[FN];
 #pragma unroll
             for (int i = 0; i < FM; ++i) af[i] = As[kk + fk][wm + 16 * i + fr];
