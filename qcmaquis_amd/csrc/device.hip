@@ -1,9 +1,9 @@
 // gfx950 kernels of the contraction path.
 //
-//  * gemm_f64_kernel: grouped, batched FP64 GEMM over a host-built tile table.  32x128, 64x64 or
-//    128x64 output tiles per 256-thread workgroup (four wave64s in a 2x2 grid, each a block of
-//    v_mfma_f64_16x16x4f64 tiles), operands staged through LDS in K-slices of 16 with the next
-//    slice prefetched into registers.  Each output tile is owned by one workgroup or, when K is
+//  * gemm_kernel: grouped, batched FP64 / FP32 GEMM over a host-built tile table.  32x128, 64x64
+//    or 128x64 output tiles per 256-thread workgroup (four wave64s in a 2x2 grid, each a block of
+//    16x16x4 matrix-core tiles), operands staged through LDS in K-slices of 8 with the next slice
+//    prefetched into registers.  Each output tile is owned by one workgroup or, when K is
 //    split, by a fixed set of partial tiles summed in order by gemm_reduce_kernel: no atomics.
 //  * sblock_kernel: the MPO-term sum (reference create_s / create_s_r, solver/tasks.cpp:425-494)
 //    as a block-sparse product on the matrix cores: 16 output tiles ("lines") of equal shape at a
@@ -340,6 +340,10 @@ inline unsigned grid_for(uint64_t n, int threads, unsigned cap = 8192) {
 
 namespace {
 
+// K slices of 8 with at most 128 VGPRs (two workgroups per SIMD): measured fastest for every tile
+// shape on the H*psi stages of a 24-orbital, M = 512 site problem (short and long reductions alike)
+constexpr int GEMM_KS = 8, GEMM_MINB = 2;
+
 template <class TI, class TO>
 void launch_gemm_t(const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
                    const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases, hipStream_t stream) {
@@ -348,13 +352,13 @@ void launch_gemm_t(const GemmDesc* probs, const GemmTile* const* tiles, const in
         for (int64_t off = 0; off < n_tiles[sh]; off += 0x7fffffff) {
             int64_t cnt = n_tiles[sh] - off < 0x7fffffff ? n_tiles[sh] - off : 0x7fffffff;
             if (sh == GEMM_32x128)
-                hipLaunchKernelGGL((gemm_kernel<1, 4, TI, TO>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                hipLaunchKernelGGL((gemm_kernel<1, 4, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    tiles[sh] + off, bases, w);
             else if (sh == GEMM_64x64)
-                hipLaunchKernelGGL((gemm_kernel<2, 2, TI, TO>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                hipLaunchKernelGGL((gemm_kernel<2, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    tiles[sh] + off, bases, w);
             else
-                hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    tiles[sh] + off, bases, w);
         }
     }
