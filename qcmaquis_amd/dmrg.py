@@ -278,7 +278,9 @@ class TwoSiteDMRG:
         self.trunc.append(tw)
         if self.verbose:
             print(f"  site {i} {'->' if to_right else '<-'} E={e + self.ham.ecore:.12f} M={sum(lb.left.sizes)} "
-                  f"tw={tw:.2e} it={st['iterations']}", flush=True)
+                  f"tw={tw:.2e} it={st['iterations']} plan {t2 - t1:.3f} s (tables {pst['seconds_tables']:.3f}, "
+                  f"blocks {pst['seconds_blocks']:.3f}, terms {pst['n_terms']}) davidson {t3 - t2:.3f} s "
+                  f"svd {t4 - t3:.3f} s grow {t5 - t4:.3f} s", flush=True)
         return e + self.ham.ecore
 
     def sweep(self):
