@@ -71,8 +71,8 @@ T* dev_alloc(uint64_t n) {
 // Copies a host table to the device and waits for the copy: callers pass vectors that may be
 // destroyed as soon as this returns, and an asynchronous copy from pageable memory may read its
 // source after the call.
-template <class T>
-T* dev_upload(std::vector<T> const& v, hipStream_t s) {
+template <class T, class A>
+T* dev_upload(std::vector<T, A> const& v, hipStream_t s) {
     T* p = dev_alloc<T>(v.size());
     if (!v.empty()) {
         hipcheck(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");

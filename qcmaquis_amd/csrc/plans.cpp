@@ -131,7 +131,7 @@ void add_empty_lines(CohortAcc::Unit const& u, uint32_t nrows, int32_t term_pos,
 // S lines and terms of all cohorts, built in parallel (emit(c, lines, terms) appends the lines of
 // cohort c with term ranges relative to its own term list) and concatenated in cohort order
 template <class Emit>
-void emit_lines(size_t n_cohorts, std::vector<SLine>& lines, std::vector<STerm>& terms, Emit emit) {
+void emit_lines(size_t n_cohorts, HostTable<SLine>& lines, HostTable<STerm>& terms, Emit emit) {
     std::vector<std::vector<SLine>> cl(n_cohorts);
     std::vector<std::vector<STerm>> ct(n_cohorts);
     std::string err;
@@ -145,21 +145,26 @@ void emit_lines(size_t n_cohorts, std::vector<SLine>& lines, std::vector<STerm>&
         }
     }
     if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
-    size_t nl = lines.size(), nt = terms.size();
-    for (size_t c = 0; c < n_cohorts; ++c) nl += cl[c].size(), nt += ct[c].size();
-    require(nt < size_t(INT32_MAX), "S table: more than 2^31 terms in one step");
-    lines.reserve(nl);
-    terms.reserve(nt);
+    std::vector<size_t> loff(n_cohorts + 1, lines.size()), toff(n_cohorts + 1, terms.size());
     for (size_t c = 0; c < n_cohorts; ++c) {
-        int32_t base = int32_t(terms.size());
-        for (SLine l : cl[c]) {
+        loff[c + 1] = loff[c] + cl[c].size();
+        toff[c + 1] = toff[c] + ct[c].size();
+    }
+    require(toff[n_cohorts] < size_t(INT32_MAX), "S table: more than 2^31 terms in one step");
+    lines.resize(loff[n_cohorts]);
+    terms.resize(toff[n_cohorts]);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t c = 0; c < int64_t(n_cohorts); ++c) {
+        const int32_t base = int32_t(toff[size_t(c)]);
+        SLine* dst = lines.data() + loff[size_t(c)];
+        for (SLine l : cl[size_t(c)]) {
             l.t0 += base;
             l.t1 += base;
-            lines.push_back(l);
+            *dst++ = l;
         }
-        terms.insert(terms.end(), ct[c].begin(), ct[c].end());
-        std::vector<SLine>().swap(cl[c]);
-        std::vector<STerm>().swap(ct[c]);
+        std::copy(ct[size_t(c)].begin(), ct[size_t(c)].end(), terms.begin() + int64_t(toff[size_t(c)]));
+        std::vector<SLine>().swap(cl[size_t(c)]);
+        std::vector<STerm>().swap(ct[size_t(c)]);
     }
 }
 
@@ -955,7 +960,7 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
 }
 
 // ---- block form of the MPO-term sum ------------------------------------------------------------
-SBlockTable build_sblocks(std::vector<SLine> const& lines, std::vector<STerm> const& terms) {
+SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const& terms) {
     SBlockTable B;
     const int64_t nl = int64_t(lines.size());
     // first T tile of every line (lines without terms, i.e. zero tiles, sort last)
@@ -991,23 +996,32 @@ SBlockTable build_sblocks(std::vector<SLine> const& lines, std::vector<STerm> co
     const int64_t nb = int64_t(spans.size());
     std::vector<std::vector<int64_t>> unions(spans.size());
     std::vector<std::vector<double>> alphas(spans.size());
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t bi = 0; bi < nb; ++bi) {
-        Span const& sp = spans[size_t(bi)];
-        std::vector<int64_t>& u = unions[size_t(bi)];
-        for (uint32_t q = 0; q < sp.n; ++q) {
-            SLine const& l = lines[(*sp.v)[sp.s0 + q]];
-            for (int32_t k = l.t0; k < l.t1; ++k) u.push_back(terms[size_t(k)].t_off);
-        }
-        std::sort(u.begin(), u.end());
-        u.erase(std::unique(u.begin(), u.end()), u.end());
-        std::vector<double>& al = alphas[size_t(bi)];
-        al.assign(size_t(SBLOCK_LINES) * u.size(), 0.0);
-        for (uint32_t q = 0; q < sp.n; ++q) {
-            SLine const& l = lines[(*sp.v)[sp.s0 + q]];
-            for (int32_t k = l.t0; k < l.t1; ++k) {
-                size_t pos = size_t(std::lower_bound(u.begin(), u.end(), terms[size_t(k)].t_off) - u.begin());
-                al[pos * SBLOCK_LINES + q] += terms[size_t(k)].alpha;
+#pragma omp parallel
+    {
+        // (T offset, line in block, term) of one block, sorted by offset then term: the union is the
+        // distinct offsets and each coefficient is summed in the line's term order
+        struct Ref { int64_t t_off; uint32_t q; int32_t k; };
+        std::vector<Ref> refs;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t bi = 0; bi < nb; ++bi) {
+            Span const& sp = spans[size_t(bi)];
+            refs.clear();
+            for (uint32_t q = 0; q < sp.n; ++q) {
+                SLine const& l = lines[(*sp.v)[sp.s0 + q]];
+                for (int32_t k = l.t0; k < l.t1; ++k) refs.push_back(Ref{terms[size_t(k)].t_off, q, k});
+            }
+            std::sort(refs.begin(), refs.end(), [](Ref const& a, Ref const& b) {
+                return a.t_off != b.t_off ? a.t_off < b.t_off : a.k < b.k;
+            });
+            std::vector<int64_t>& u = unions[size_t(bi)];
+            for (size_t r = 0; r < refs.size(); ++r)
+                if (r == 0 || refs[r].t_off != refs[r - 1].t_off) u.push_back(refs[r].t_off);
+            std::vector<double>& al = alphas[size_t(bi)];
+            al.assign(size_t(SBLOCK_LINES) * u.size(), 0.0);
+            size_t pos = 0;
+            for (size_t r = 0; r < refs.size(); ++r) {
+                if (r > 0 && refs[r].t_off != refs[r - 1].t_off) ++pos;
+                al[pos * SBLOCK_LINES + refs[r].q] += terms[size_t(refs[r].k)].alpha;
             }
         }
     }
@@ -1052,25 +1066,35 @@ SBlockTable build_sblocks(std::vector<SLine> const& lines, std::vector<STerm> co
 void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size) {
     require(B.lsoff.size() == B.blocks.size() * SBLOCK_LINES && B.llds.size() == B.lsoff.size() &&
                 B.ualpha.size() == B.utile.size() * SBLOCK_LINES, "S blocks: malformed tables");
-    for (SBlock const& b : B.blocks) {
-        require(b.nu >= 0 && b.E > 0 && b.rows > 0 && b.E % b.rows == 0 && b.nlines > 0 && b.nlines <= SBLOCK_LINES &&
-                    b.u_off >= 0 && uint64_t(b.u_off) + uint64_t(b.nu) <= B.utile.size() && b.line0 >= 0 &&
-                    uint64_t(b.line0) + SBLOCK_LINES <= B.lsoff.size(), "S blocks: malformed block");
+    int bad = 0;  // 1 malformed block, 2 tile outside T, 3 line outside S, 4 malformed work item
+#pragma omp parallel for schedule(dynamic, 256) reduction(max : bad)
+    for (int64_t bi = 0; bi < int64_t(B.blocks.size()); ++bi) {
+        SBlock const& b = B.blocks[size_t(bi)];
+        if (!(b.nu >= 0 && b.E > 0 && b.rows > 0 && b.E % b.rows == 0 && b.nlines > 0 && b.nlines <= SBLOCK_LINES &&
+              b.u_off >= 0 && uint64_t(b.u_off) + uint64_t(b.nu) <= B.utile.size() && b.line0 >= 0 &&
+              uint64_t(b.line0) + SBLOCK_LINES <= B.lsoff.size())) {
+            bad = std::max(bad, 1);
+            continue;
+        }
         for (int32_t k = 0; k < b.nu; ++k) {
             int64_t o = B.utile[size_t(b.u_off) + k];
-            require(o >= 0 && uint64_t(o) + uint64_t(b.E) <= t_size, "S blocks: tile outside the T buffer");
+            if (!(o >= 0 && uint64_t(o) + uint64_t(b.E) <= t_size)) bad = std::max(bad, 2);
         }
         int32_t cols = b.E / b.rows;
         for (int32_t i = 0; i < b.nlines; ++i) {
             int64_t so = B.lsoff[size_t(b.line0) + i];
             int32_t ld = B.llds[size_t(b.line0) + i];
-            require(so >= 0 && ld >= b.rows && uint64_t(so + (b.rows - 1) + int64_t(cols - 1) * ld) < s_size,
-                    "S blocks: line outside the S buffer");
+            if (!(so >= 0 && ld >= b.rows && uint64_t(so + (b.rows - 1) + int64_t(cols - 1) * ld) < s_size))
+                bad = std::max(bad, 3);
         }
     }
     for (SWork const& w : B.work)
-        require(w.blk >= 0 && size_t(w.blk) < B.blocks.size() && w.n0 >= 0 && w.n0 < B.blocks[size_t(w.blk)].E,
-                "S blocks: malformed work item");
+        if (!(w.blk >= 0 && size_t(w.blk) < B.blocks.size() && w.n0 >= 0 && w.n0 < B.blocks[size_t(w.blk)].E))
+            bad = std::max(bad, 4);
+    require(bad != 1, "S blocks: malformed block");
+    require(bad != 2, "S blocks: tile outside the T buffer");
+    require(bad != 3, "S blocks: line outside the S buffer");
+    require(bad != 4, "S blocks: malformed work item");
 }
 
 // ---- grouped GEMM schedule ---------------------------------------------------------------------
@@ -1159,17 +1183,27 @@ void check_gemms(std::vector<GemmDesc> const& v, const uint64_t* sizes) {
     }
 }
 
-void check_lines(std::vector<SLine> const& lines, std::vector<STerm> const& terms, uint64_t t_size, uint64_t s_size) {
-    for (auto const& l : lines) {
-        require(l.rows >= 0 && l.cols >= 0 && l.lds >= l.rows && l.t0 >= 0 && l.t1 >= l.t0 &&
-                    uint64_t(l.t1) <= terms.size(), "S table: malformed line");
+void check_lines(HostTable<SLine> const& lines, HostTable<STerm> const& terms, uint64_t t_size, uint64_t s_size) {
+    int bad = 0;  // 1 malformed line, 2 line outside S, 3 term outside T
+#pragma omp parallel for schedule(static) reduction(max : bad)
+    for (int64_t i = 0; i < int64_t(lines.size()); ++i) {
+        SLine const& l = lines[size_t(i)];
+        if (!(l.rows >= 0 && l.cols >= 0 && l.lds >= l.rows && l.t0 >= 0 && l.t1 >= l.t0 &&
+              uint64_t(l.t1) <= terms.size())) {
+            bad = std::max(bad, 1);
+            continue;
+        }
         if (!l.rows || !l.cols) continue;
         int64_t last = l.s_off + (l.rows - 1) + int64_t(l.cols - 1) * l.lds;
-        require(l.s_off >= 0 && uint64_t(last) < s_size, "S table: line outside the S buffer");
+        if (!(l.s_off >= 0 && uint64_t(last) < s_size)) bad = std::max(bad, 2);
         for (int t = l.t0; t < l.t1; ++t)
-            require(terms[t].t_off >= 0 && uint64_t(terms[t].t_off) + uint64_t(l.rows) * l.cols <= t_size,
-                    "S table: term outside the T buffer");
+            if (!(terms[size_t(t)].t_off >= 0 &&
+                  uint64_t(terms[size_t(t)].t_off) + uint64_t(l.rows) * l.cols <= t_size))
+                bad = std::max(bad, 3);
     }
+    require(bad != 1, "S table: malformed line");
+    require(bad != 2, "S table: line outside the S buffer");
+    require(bad != 3, "S table: term outside the T buffer");
 }
 
 

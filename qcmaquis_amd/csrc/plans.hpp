@@ -11,11 +11,39 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "core.hpp"
 
 namespace mq {
+
+// Host table whose resize leaves new elements uninitialised: the big tables are written in full
+// right after they are sized, in parallel, so no serial zero-fill (and no first touch by one
+// thread) is paid.
+template <class T>
+struct DefaultInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = DefaultInitAlloc<U>;
+    };
+    DefaultInitAlloc() = default;
+    template <class U>
+    DefaultInitAlloc(DefaultInitAlloc<U> const&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+        ::new (static_cast<void*>(p)) U;
+    }
+    template <class U, class... A>
+    void construct(U* p, A&&... a) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using HostTable = std::vector<T, DefaultInitAlloc<T>>;
+
 
 // One GEMM problem: C = op(A) * op(B), column-major, optionally batched with element strides.
 // Address fields hold an element offset into the role's base pointer given at launch.
@@ -57,8 +85,8 @@ struct SWork {
 struct SBlockTable {
     std::vector<SBlock> blocks;
     std::vector<SWork> work;
-    std::vector<int64_t> utile;   // T element offset of each union tile
-    std::vector<double> ualpha;
+    HostTable<int64_t> utile;   // T element offset of each union tile
+    HostTable<double> ualpha;
     std::vector<int64_t> lsoff;   // S offset of each line (SBLOCK_LINES entries per block)
     std::vector<int32_t> llds;    // its leading dimension
     uint64_t gathered = 0;        // sum over blocks of nu * E (T elements read)
@@ -112,7 +140,7 @@ struct STerm {
 };
 
 // block table of a list of S lines (lines without terms become zero tiles)
-SBlockTable build_sblocks(std::vector<SLine> const& lines, std::vector<STerm> const& terms);
+SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const& terms);
 void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size);
 
 struct FlopStats {
@@ -129,8 +157,8 @@ struct CopyJob {
 struct HpsiPlanHost {
     MPSLayout mps;
     std::vector<GemmDesc> t_gemms, c_gemms;
-    std::vector<SLine> lines;
-    std::vector<STerm> terms;
+    HostTable<SLine> lines;
+    HostTable<STerm> terms;
     std::vector<CopyJob> lpanel_jobs;
     uint64_t t_elems = 0, s_elems = 0, lpanel_elems = 0;
     uint64_t n_cohorts = 0, n_t_tasks = 0;
@@ -143,8 +171,8 @@ struct GrowPlanHost {
     BoundaryIndex out;      // index of the new boundary (with data_off layout)
     uint64_t out_elems = 0;
     std::vector<GemmDesc> t_gemms, c_gemms;
-    std::vector<SLine> lines;
-    std::vector<STerm> terms;
+    HostTable<SLine> lines;
+    HostTable<STerm> terms;
     uint64_t t_elems = 0, s_elems = 0;
     FlopStats flops;
 };
@@ -158,7 +186,7 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
 // Host-side bounds validation of device tables: every operand span of every GEMM problem, S line
 // and T term must lie inside its buffer (sizes indexed by BASE_*); throws mq::Error otherwise.
 void check_gemms(std::vector<GemmDesc> const& v, const uint64_t* sizes);
-void check_lines(std::vector<SLine> const& lines, std::vector<STerm> const& terms, uint64_t t_size, uint64_t s_size);
+void check_lines(HostTable<SLine> const& lines, HostTable<STerm> const& terms, uint64_t t_size, uint64_t s_size);
 
 GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, MPSLayout const& bra,
                               MPSLayout const& ket, bool symmetric);

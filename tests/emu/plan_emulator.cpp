@@ -74,7 +74,7 @@ void run_gemms(std::vector<GemmDesc> const& G, double* const* base) {
 // the MPO-term sum as the device runs it: the block table built from the lines, each work item
 // covering SBLOCK_ELEMS tile elements of one block (four waves of 16 * SBLOCK_ELEMS / 64); every
 // (line, element) of the table must be written exactly once
-void run_lines(std::vector<SLine> const& lines, std::vector<STerm> const& terms, const double* T, double* S,
+void run_lines(HostTable<SLine> const& lines, HostTable<STerm> const& terms, const double* T, double* S,
                uint64_t t_size, uint64_t s_size) {
     SBlockTable B = build_sblocks(lines, terms);
     check_sblocks(B, t_size, s_size);
