@@ -103,7 +103,8 @@ inline void op_iterate(OpBlock const& blk, const double cpl[4], CohortAcc& coh, 
         double alpha = e.coeff * cpl[casenr(e.row_spin, e.col_spin)];
         uint32_t ti = tl.get(mps_offset + uint64_t(e.row) * m2_size, ci);
         if (ti != UINT32_MAX) {
-            tl.tasks[ti].used = true;
+            // cohorts of one T list may be built concurrently: every writer stores the same flag
+            __atomic_store_n(&tl.tasks[ti].used, true, __ATOMIC_RELAXED);
             coh.push(s, e.col, alpha, ti, boundary_col);
         }
     }
@@ -539,33 +540,53 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
         std::vector<int64_t> offs;
         uint32_t nrows;
     };
-    std::vector<std::vector<CohRec>> coh_by(NKR);
     std::vector<TList> tlists(NKR);
     std::string err;
 
+    // T work lists, one per right ket sector (lshtm_t_tasks)
 #pragma omp parallel for schedule(dynamic, 1)
     for (uint32_t rb_ket = 0; rb_ket < NKR; ++rb_ket) {
         try {
-        Charge rc_ket = ket.right.charges[rb_ket];
-        uint32_t rs_ket = ket.right.sizes[rb_ket];
-        TList& tl = tlists[rb_ket];
-        for (size_t s = 0; s < NP; ++s) {
-            size_t lb_ket = kl_of[rb_ket * NP + s];
-            if (lb_ket == NKL) continue;
-            uint64_t ket_offset = kpb[s * NKR + rb_ket];
-            for (size_t lb_bra = 0; lb_bra < NKL; ++lb_bra) {
-                uint32_t ci = l_ci_kk[lb_bra * NKL + lb_ket];
-                if (ci == L.n_cohorts()) continue;
-                uint32_t ci_eff = L.tr[ci] ? l_ci_kk[lb_ket * NKL + lb_bra] : ci;
-                require(ci_eff < L.n_cohorts(), "lstep: transposed cohort without partner");
-                for (uint32_t ss = 0; ss < phys.sizes[s]; ++ss)
-                    tl.add(ket_offset + ss * rs_ket, ci, ci_eff, uint32_t(lb_ket));
+            uint32_t rs_ket = ket.right.sizes[rb_ket];
+            TList& tl = tlists[rb_ket];
+            for (size_t s = 0; s < NP; ++s) {
+                size_t lb_ket = kl_of[rb_ket * NP + s];
+                if (lb_ket == NKL) continue;
+                uint64_t ket_offset = kpb[s * NKR + rb_ket];
+                for (size_t lb_bra = 0; lb_bra < NKL; ++lb_bra) {
+                    uint32_t ci = l_ci_kk[lb_bra * NKL + lb_ket];
+                    if (ci == L.n_cohorts()) continue;
+                    uint32_t ci_eff = L.tr[ci] ? l_ci_kk[lb_ket * NKL + lb_bra] : ci;
+                    require(ci_eff < L.n_cohorts(), "lstep: transposed cohort without partner");
+                    for (uint32_t ss = 0; ss < phys.sizes[s]; ++ss)
+                        tl.add(ket_offset + ss * rs_ket, ci, ci_eff, uint32_t(lb_ket));
+                }
             }
+        } catch (std::exception const& e) {
+#pragma omp critical
+            err = e.what();
         }
-        for (uint32_t rb_bra = 0; rb_bra < NBR; ++rb_bra) {
+    }
+    if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
+
+    // cohorts, one per (right ket, right bra) sector pair (lshtm_tasks), in parallel over the pairs
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    for (uint32_t rb_ket = 0; rb_ket < NKR; ++rb_ket)
+        for (uint32_t rb_bra = 0; rb_bra < NBR; ++rb_bra)
+            if (std::abs(particle_number(sym, bra.right.charges[rb_bra]) -
+                         particle_number(sym, ket.right.charges[rb_ket])) <= 2)
+                pairs.emplace_back(rb_ket, rb_bra);
+    std::vector<std::unique_ptr<CohRec>> recs(pairs.size());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t pi = 0; pi < int64_t(pairs.size()); ++pi) {
+        try {
+            const uint32_t rb_ket = pairs[size_t(pi)].first, rb_bra = pairs[size_t(pi)].second;
+            Charge rc_ket = ket.right.charges[rb_ket];
+            uint32_t rs_ket = ket.right.sizes[rb_ket];
+            TList& tl = tlists[rb_ket];
             Charge rc_bra = bra.right.charges[rb_bra];
-            if (std::abs(particle_number(sym, rc_bra) - particle_number(sym, rc_ket)) > 2) continue;
-            CohRec rec{rb_bra, rb_ket, CohortAcc(phys, mpo.col_dim), {}, 0};
+            std::unique_ptr<CohRec> rp(new CohRec{rb_bra, rb_ket, CohortAcc(phys, mpo.col_dim), {}, 0});
+            CohRec& rec = *rp;
             CohortAcc& coh = rec.acc;
             for (size_t s = 0; s < NP; ++s) {
                 size_t lb_bra = bl_of[rb_bra * NP + s];
@@ -619,8 +640,7 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
             rec.nrows = cnt;
             for (auto& u : coh.units)
                 for (auto& b : u.line_b) b = pos[b];
-            coh_by[rb_ket].push_back(std::move(rec));
-        }
+            recs[size_t(pi)] = std::move(rp);
         } catch (std::exception const& e) {
 #pragma omp critical
             err = e.what();
@@ -628,8 +648,8 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
     }
     if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
     std::vector<CohRec> cohorts;
-    for (auto& v : coh_by)
-        for (auto& c : v) cohorts.push_back(std::move(c));
+    for (auto& r : recs)
+        if (r) cohorts.push_back(std::move(*r));
 
     // new boundary index (move_boundary.hpp:202-208)
     P.out = BoundaryIndex(bra.right, ket.right, mpo.col_dim);
@@ -782,33 +802,52 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
         std::vector<int64_t> offs;
         uint32_t nrows;
     };
-    std::vector<std::vector<CohRec>> coh_by(NKL);
     std::vector<TList> tlists(NKL);
     std::string err;
 
+    // T work lists, one per left ket sector (rshtm_t_tasks)
 #pragma omp parallel for schedule(dynamic, 1)
     for (uint32_t lb_ket = 0; lb_ket < NKL; ++lb_ket) {
         try {
-        Charge lc_ket = ket.left.charges[lb_ket];
-        uint32_t ls_ket = ket.left.sizes[lb_ket];
-        TList& tl = tlists[lb_ket];
-        for (size_t s = 0; s < NP; ++s) {
-            size_t rb_ket = kr_of[lb_ket * NP + s];
-            if (rb_ket == NKR) continue;
-            uint32_t rs_ket = ket.right.sizes[rb_ket];
-            uint64_t ket_offset = kpb[lb_ket * NP + s];
-            for (size_t rb_bra = 0; rb_bra < NBR; ++rb_bra) {
-                uint32_t ci = r_ci_kb[rb_ket * NBR + rb_bra];
-                if (ci == R.n_cohorts()) continue;
-                uint32_t ci_eff = R.tr[ci] ? r_ci_bk[rb_bra * NKR + rb_ket] : ci;
-                require(ci_eff < R.n_cohorts(), "rstep: transposed cohort without partner");
-                for (uint32_t ss = 0; ss < phys.sizes[s]; ++ss) tl.add(ket_offset + ss * rs_ket, ci, ci_eff, lb_ket);
+            TList& tl = tlists[lb_ket];
+            for (size_t s = 0; s < NP; ++s) {
+                size_t rb_ket = kr_of[lb_ket * NP + s];
+                if (rb_ket == NKR) continue;
+                uint32_t rs_ket = ket.right.sizes[rb_ket];
+                uint64_t ket_offset = kpb[lb_ket * NP + s];
+                for (size_t rb_bra = 0; rb_bra < NBR; ++rb_bra) {
+                    uint32_t ci = r_ci_kb[rb_ket * NBR + rb_bra];
+                    if (ci == R.n_cohorts()) continue;
+                    uint32_t ci_eff = R.tr[ci] ? r_ci_bk[rb_bra * NKR + rb_ket] : ci;
+                    require(ci_eff < R.n_cohorts(), "rstep: transposed cohort without partner");
+                    for (uint32_t ss = 0; ss < phys.sizes[s]; ++ss) tl.add(ket_offset + ss * rs_ket, ci, ci_eff, lb_ket);
+                }
             }
+        } catch (std::exception const& e) {
+#pragma omp critical
+            err = e.what();
         }
-        for (uint32_t lb_bra = 0; lb_bra < NBL; ++lb_bra) {
+    }
+    if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
+
+    // cohorts, one per (left ket, left bra) sector pair (rshtm_tasks), in parallel over the pairs
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    for (uint32_t lb_ket = 0; lb_ket < NKL; ++lb_ket)
+        for (uint32_t lb_bra = 0; lb_bra < NBL; ++lb_bra)
+            if (std::abs(particle_number(sym, ket.left.charges[lb_ket]) -
+                         particle_number(sym, bra.left.charges[lb_bra])) <= 2)
+                pairs.emplace_back(lb_ket, lb_bra);
+    std::vector<std::unique_ptr<CohRec>> recs(pairs.size());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t pi = 0; pi < int64_t(pairs.size()); ++pi) {
+        try {
+            const uint32_t lb_ket = pairs[size_t(pi)].first, lb_bra = pairs[size_t(pi)].second;
+            Charge lc_ket = ket.left.charges[lb_ket];
+            uint32_t ls_ket = ket.left.sizes[lb_ket];
+            TList& tl = tlists[lb_ket];
             Charge lc_bra = bra.left.charges[lb_bra];
-            if (std::abs(particle_number(sym, lc_ket) - particle_number(sym, lc_bra)) > 2) continue;
-            CohRec rec{lb_ket, lb_bra, CohortAcc(phys, mpo.row_dim), {}, 0};
+            std::unique_ptr<CohRec> rp(new CohRec{lb_ket, lb_bra, CohortAcc(phys, mpo.row_dim), {}, 0});
+            CohRec& rec = *rp;
             CohortAcc& coh = rec.acc;
             for (size_t s = 0; s < NP; ++s) {
                 size_t rb_bra = br_of[lb_bra * NP + s];
@@ -861,8 +900,7 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
             rec.nrows = cnt;
             for (auto& u : coh.units)
                 for (auto& b : u.line_b) b = pos[b];
-            coh_by[lb_ket].push_back(std::move(rec));
-        }
+            recs[size_t(pi)] = std::move(rp);
         } catch (std::exception const& e) {
 #pragma omp critical
             err = e.what();
@@ -870,8 +908,8 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
     }
     if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
     std::vector<CohRec> cohorts;
-    for (auto& v : coh_by)
-        for (auto& c : v) cohorts.push_back(std::move(c));
+    for (auto& r : recs)
+        if (r) cohorts.push_back(std::move(*r));
 
     // new boundary index (move_boundary.hpp:332-338): complement_transpose is always applied
     P.out = BoundaryIndex(ket.left, bra.left, mpo.row_dim);
