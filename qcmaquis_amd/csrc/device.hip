@@ -189,6 +189,7 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __rest
 // over the block's union of T tiles.  Lane l loads alpha of line l % 16 and union tile l / 16, and
 // element l % 16 of that tile: 16 consecutive values per tile.
 constexpr int SB_NC = SBLOCK_ELEMS / 64;
+constexpr int SB_UNROLL = 4;
 static_assert(SBLOCK_LINES == 16 && SB_NC * 64 == SBLOCK_ELEMS, "S block geometry");
 
 template <class T>
@@ -209,19 +210,26 @@ __global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ 
     for (int j = 0; j < SB_NC; ++j) acc[j] = acc_t{0, 0, 0, 0};
     const int64_t* ut = utile + b.u_off;
     const T* ua = ualpha + b.u_off * SBLOCK_LINES;
-    for (int ks = 0; ks < b.nu; ks += 4) {
-        const int k = ks + lk;
-        const bool kv = k < b.nu;
-        const T a = kv ? ua[k * SBLOCK_LINES + li] : T(0);  // A[line li][tile k]
-        const int64_t to = kv ? ut[k] : 0;
-        T x[SB_NC];
+    // SB_UNROLL k-steps of loads are issued before their matrix-core steps (more loads in flight
+    // per wave: the kernel is bound by the latency of the dependent offset -> tile loads)
+    for (int ks = 0; ks < b.nu; ks += 4 * SB_UNROLL) {
+        T a[SB_UNROLL], x[SB_UNROLL][SB_NC];
 #pragma unroll
-        for (int j = 0; j < SB_NC; ++j) {
-            const int e = n0 + 16 * j + li;
-            x[j] = (kv && e < b.E) ? X[to + e] : T(0);  // B[tile k][element e]
+        for (int u = 0; u < SB_UNROLL; ++u) {
+            const int k = ks + 4 * u + lk;
+            const bool kv = k < b.nu;
+            a[u] = kv ? ua[k * SBLOCK_LINES + li] : T(0);  // A[line li][tile k]
+            const int64_t to = kv ? ut[k] : 0;
+#pragma unroll
+            for (int j = 0; j < SB_NC; ++j) {
+                const int e = n0 + 16 * j + li;
+                x[u][j] = (kv && e < b.E) ? X[to + e] : T(0);  // B[tile k][element e]
+            }
         }
 #pragma unroll
-        for (int j = 0; j < SB_NC; ++j) acc[j] = Mma<T>::step(a, x[j], acc[j]);
+        for (int u = 0; u < SB_UNROLL; ++u)
+#pragma unroll
+            for (int j = 0; j < SB_NC; ++j) acc[j] = Mma<T>::step(a[u], x[u][j], acc[j]);
     }
     // register r of lane l holds Y[line row(l, r)][element l % 16]
 #pragma unroll
