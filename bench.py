@@ -120,6 +120,31 @@ class ProfilerRegion:
         return False
 
 
+def sweep_compare(ctx, L=24, M=512, seed=1002, n_sweeps=2):
+    """Sweep time and energy of the FP64 path and of the mixed path (FP32 contraction, FP64
+    Davidson refinement) from the same seeded start; the last sweep of each is reported."""
+    from qcmaquis_amd import abi
+    from qcmaquis_amd.dmrg import TwoSiteDMRG
+    from qcmaquis_amd.qc import QCHamiltonian, make_integrals
+    h, eri, ec = make_integrals(L, seed, 16)
+    ham = QCHamiltonian(h, eri, ec)
+    out = {}
+    for name, prec in (("fp64", abi.MQ_PREC_FP64), ("mixed_fp32", abi.MQ_PREC_FP32)):
+        d = TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8, precision=prec)
+        for s in range(n_sweeps):
+            t0 = dict(d.timing)
+            e, dt = d.sweep()
+        out[name] = {"energy": e, "sweep_seconds": round(dt, 3),
+                     "timing_s": {k: round(v - t0[k], 3) for k, v in d.timing.items()}}
+        log("%s sweep %d: E = %.10f, %.2f s %s" % (name, n_sweeps - 1, e, dt, out[name]["timing_s"]))
+        del d
+    out["delta_e"] = abs(out["mixed_fp32"]["energy"] - out["fp64"]["energy"])
+    out["speedup"] = round(out["fp64"]["sweep_seconds"] / out["mixed_fp32"]["sweep_seconds"], 3)
+    out["note"] = (f"{L} orbitals, M={M}: sweep {n_sweeps - 1} of {n_sweeps} from the same seeded random MPS; "
+                   "Davidson tol 1e-8, at most 8 iterations")
+    return out
+
+
 def cpu_baseline(w, flops, budget_s=30.0):
     """the oracle (restated reference CPU path) on the same site problem, all host cores"""
     import oracle
@@ -153,6 +178,7 @@ def main():
     ap.add_argument("--warm-sweeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dump-bonds", default=None, help="write the MPS bond indices of the workload (JSON)")
+    ap.add_argument("--no-sweeps", action="store_true", help="skip the FP64 / mixed sweep comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy inside the profiler region (HBM counter scale)")
     args = ap.parse_args()
@@ -289,6 +315,9 @@ def main():
         "roofline": dominant,
         "roofline_other": other,
     }
+    plan.close()
+    if args.workload == "qc" and not args.no_sweeps:
+        line["config"]["sweeps"] = sweep_compare(ctx)
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(w, st["flops_total"])
@@ -297,7 +326,6 @@ def main():
             log("cpu baseline failed:", repr(e))
             line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
-    plan.close()
     ctx.close()
 
 

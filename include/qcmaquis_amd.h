@@ -204,6 +204,11 @@ int mq_hpsi(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out_dev, 
  * (reference timing: cudaEvent around the GPU H*psi, super_hamil_mv.hpp:61-84) */
 int mq_hpsi_timed(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out_dev, int precision,
                   float* stage_ms);
+/* enable = 1: mq_hpsi, mq_hpsi_host and mq_davidson run H*psi as a HIP graph captured once per
+ * precision on buffers owned by the plan (psi copied in and out on the context stream); the
+ * launch sequence is the one of the default path.  Reference: no counterpart (one launch per GPU
+ * task in super_hamil_mv.hpp:100-155). */
+int mq_hpsi_plan_use_graphs(mq_hpsi_plan_t plan, int enable);
 /* host pointers, synchronous */
 int mq_hpsi_host(mq_hpsi_plan_t plan, const double* psi_in, double* psi_out, int precision);
 int mq_hpsi_plan_destroy(mq_hpsi_plan_t plan);

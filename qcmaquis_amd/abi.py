@@ -138,6 +138,7 @@ SIGNATURES = [
     ("mq_hpsi", i32, [vp, vp, vp, i32]),
     ("mq_hpsi_timed", i32, [vp, vp, vp, i32, P(C.c_float)]),
     ("mq_hpsi_host", i32, [vp, dp, dp, i32]),
+    ("mq_hpsi_plan_use_graphs", i32, [vp, i32]),
     ("mq_hpsi_plan_destroy", i32, [vp]),
     ("mq_davidson", i32, [vp, dp, i32, P(dp), C.c_double, i32, i32, dp, dp, P(DavidsonStats)]),
     ("mq_qc_build", i32, [i32, i32, dp, dp, C.c_double, C.c_double, P(vp)]),

@@ -174,11 +174,14 @@ struct SAsmStage {
     void run(const double* T, double* S, hipStream_t s) const {
         hipcheck(launch_sblocks<double>(blocks, work, n_work, utile, ualpha, lsoff, llds, T, S, s), "S-assembly");
     }
-    void run32(const float* T, float* S, hipStream_t s) {
+    void ensure32(hipStream_t s) {
         if (!ualpha32) {
             ualpha32 = dev_alloc<float>(n_alpha);
             hipcheck(launch_convert(ualpha, ualpha32, n_alpha, s), "convert");
         }
+    }
+    void run32(const float* T, float* S, hipStream_t s) {
+        ensure32(s);
         hipcheck(launch_sblocks<float>(blocks, work, n_work, utile, ualpha32, lsoff, llds, T, S, s), "S-assembly");
     }
     void release() {
@@ -221,6 +224,12 @@ struct mq_hpsi_plan_s {
     double *T = nullptr, *S = nullptr, *Lpanel = nullptr;
     // FP32 operands of the mixed-precision H*psi, made on first use
     float *T32 = nullptr, *S32 = nullptr, *Lpanel32 = nullptr, *R32 = nullptr, *psi32 = nullptr;
+    // captured H*psi per precision (FP64, FP32) on plan-owned psi buffers
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        double *in = nullptr, *out = nullptr;
+    } graphs[2];
+    bool use_graphs = false;
     uint64_t r_elems = 0;
     double build_seconds = 0, seconds_tables = 0, seconds_blocks = 0;
 };
@@ -624,6 +633,7 @@ static void ensure_fp32(mq_hpsi_plan_t p) {
     p->psi32 = dev_alloc<float>(H.mps.elements());
     hipcheck(launch_convert(p->Lpanel, p->Lpanel32, H.lpanel_elems, s), "convert");
     hipcheck(launch_convert(p->rdata, p->R32, p->r_elems, s), "convert");
+    p->sa.ensure32(s);
     p->T32 = dev_alloc<float>(H.t_elems);
 }
 
@@ -672,12 +682,56 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
     }
 }
 
+// H*psi through a HIP graph: the stage launches are captured once per precision on plan-owned
+// input / output buffers (every allocation happens before the capture, and no memset node is
+// recorded); a call copies psi in, launches the graph and copies psi' out, all on the context
+// stream.
+static void hpsi_graph_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
+    require(precision == MQ_PREC_FP64 || precision == MQ_PREC_FP32, "H*psi graph: unsupported precision");
+    hipStream_t s = p->ctx->stream;
+    const uint64_t n = p->host.mps.elements();
+    auto& g = p->graphs[precision == MQ_PREC_FP64 ? 0 : 1];
+    if (!g.exec) {
+        if (precision == MQ_PREC_FP32) ensure_fp32(p);
+        if (!g.in) g.in = dev_alloc<double>(n);
+        if (!g.out) g.out = dev_alloc<double>(n);
+        hipcheck(hipStreamSynchronize(s), "graph set-up");
+        hipcheck(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        hipGraph_t graph = nullptr;
+        try {
+            hpsi_enqueue(p, g.in, g.out, precision);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        hipcheck(hipStreamEndCapture(s, &graph), "end capture");
+        hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hipcheck(e, "graph instantiate");
+    }
+    hipcheck(hipMemcpyAsync(g.in, in, n * sizeof(double), hipMemcpyDeviceToDevice, s), "graph input");
+    hipcheck(hipGraphLaunch(g.exec, s), "graph launch");
+    hipcheck(hipMemcpyAsync(out, g.out, n * sizeof(double), hipMemcpyDeviceToDevice, s), "graph output");
+}
+
+static void hpsi_run(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
+    if (p->use_graphs) hpsi_graph_enqueue(p, in, out, precision);
+    else hpsi_enqueue(p, in, out, precision);
+}
+
 int mq_hpsi(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
     if (!p) return MQ_ERR_INVALID;
     return guarded(p->ctx, [&] {
         require(in && out && in != out, "mq_hpsi: input and output must be distinct device buffers");
-        hpsi_enqueue(p, in, out, precision);
+        hpsi_run(p, in, out, precision);
     });
+}
+
+int mq_hpsi_plan_use_graphs(mq_hpsi_plan_t p, int enable) {
+    if (!p) return MQ_ERR_INVALID;
+    p->use_graphs = enable != 0;
+    return MQ_OK;
 }
 
 int mq_hpsi_timed(mq_hpsi_plan_t p, const double* in, double* out, int precision, float* stage_ms) {
@@ -710,7 +764,7 @@ int mq_hpsi_host(mq_hpsi_plan_t p, const double* in, double* out, int precision)
         double* dout = dev_alloc<double>(n);
         try {
             hipcheck(hipMemcpyAsync(din, in, n * sizeof(double), hipMemcpyHostToDevice, p->ctx->stream), "upload");
-            hpsi_enqueue(p, din, dout, precision);
+            hpsi_run(p, din, dout, precision);
             hipcheck(hipMemcpyAsync(out, dout, n * sizeof(double), hipMemcpyDeviceToHost, p->ctx->stream), "download");
             hipcheck(hipStreamSynchronize(p->ctx->stream), "hpsi");
         } catch (...) {
@@ -734,6 +788,11 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
         dev_free(p->T);
         dev_free(p->S);
         dev_free(p->Lpanel);
+        for (auto& g : p->graphs) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            dev_free(g.in);
+            dev_free(g.out);
+        }
         dev_free(p->T32);
         dev_free(p->S32);
         dev_free(p->Lpanel32);
@@ -857,7 +916,7 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
                 hipcheck(launch_dot(t, t, n, work, sc, s), "dot");
                 hipcheck(launch_scale_rsqrt(t, n, sc, s), "scale");
                 auto h0 = std::chrono::steady_clock::now();
-                hpsi_enqueue(p, t, VA[it], prec);
+                hpsi_run(p, t, VA[it], prec);
                 hipcheck(hipStreamSynchronize(s), "hpsi");
                 hpsi_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
                 for (int i = 0; i <= it; ++i) M[size_t(i) * max_iter + it] = dot_host(V[i], VA[it]);

@@ -213,8 +213,11 @@ class TwoSiteDMRG:
     """Two-site DMRG for a TwoU1 quantum-chemistry Hamiltonian (QCHamiltonian)."""
 
     def __init__(self, ham, n_up, n_dn, M, ctx=None, init_m=8, seed=42, tol=1e-8, max_iter=8, cutoff=1e-16,
-                 verbose=False):
+                 verbose=False, precision=abi.MQ_PREC_FP64):
+        """precision: arithmetic of the Davidson H*psi (MQ_PREC_FP32 = FP32 contraction with an FP64
+        refinement of each eigenvector, see mq_davidson)"""
         self.ham, self.M, self.tol, self.max_iter, self.cutoff = ham, M, tol, max_iter, cutoff
+        self.precision = precision
         self.verbose = verbose
         self.ctx = ctx or Context(0, abi.MQ_SYM_TWOU1)
         self.L = ham.L
@@ -253,7 +256,7 @@ class TwoSiteDMRG:
         self.timing["mpo"] += t1 - tm
         plan = HpsiPlan(self.ctx, mpo, self.left[i], self.right[i + 2], lay2)
         t2 = time.perf_counter()
-        e, psi, st = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter)
+        e, psi, st = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter, precision=self.precision)
         t3 = time.perf_counter()
         pst = plan.stats()
         self.hpsi_flops += pst["flops_total"] * st["iterations"]

@@ -196,6 +196,10 @@ class HpsiPlan(_Handle):
         self.ctx.check(self.ctx.lib.mq_hpsi_host(self.h, _dp(psi), _dp(out), int(precision)))
         return out
 
+    def use_graphs(self, enable=True):
+        """run H*psi (apply, apply_device, davidson) as a captured HIP graph"""
+        self.ctx.check(self.ctx.lib.mq_hpsi_plan_use_graphs(self.h, int(bool(enable))))
+
     def apply_device(self, in_ptr, out_ptr, precision=abi.MQ_PREC_FP64):
         """Stream-ordered H*psi on device pointers (context stream)."""
         self.ctx.check(self.ctx.lib.mq_hpsi(self.h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), int(precision)))
