@@ -259,6 +259,22 @@ def main():
     stage32 /= nrep
     ms32 = wall32 / args.steps * 1e3
     err32 = float((out32 - ref).norm() / ref.norm())
+    # the same H*psi as captured HIP graphs (psi copied into and out of plan-owned buffers)
+    plan.use_graphs(True)
+    graph_ms = {}
+    outg = torch.empty_like(psi)
+    for name, prec in (("fp64", abi.MQ_PREC_FP64), ("mixed_fp32", abi.MQ_PREC_FP32)):
+        for _ in range(args.warmup):
+            plan.apply_device(psi.data_ptr(), outg.data_ptr(), prec)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.apply_device(psi.data_ptr(), outg.data_ptr(), prec)
+        ctx.sync()
+        graph_ms[name] = round((time.perf_counter() - t0) / args.steps * 1e3, 4)
+    plan.use_graphs(False)
+    graph_ok = bool(torch.equal(outg, out32))  # last graph call was FP32 on the same psi
+    log("graph H*psi ms: %s (FP32 graph output equals eager: %s)" % (graph_ms, graph_ok))
     log("FP32 H*psi: %.3f ms, stages %s, relative error vs FP64 %.2e" % (ms32, np.round(stage32, 4).tolist(), err32))
     ms = wall / args.steps * 1e3
     log("timed %d H*psi: %.3f ms each" % (args.steps, ms))
@@ -308,6 +324,7 @@ def main():
                        flops_per_step_reference_model=st["flops_total"], gemm_flops_executed=st["flops_gemm_exec"],
                        stage_ms={"setup": round(stage[0], 4), "t_gemm": round(stage[1], 4),
                                  "s_assembly": round(stage[2], 4), "ls_gemm": round(stage[3], 4)},
+                       hip_graph_ms_per_step=dict(graph_ms, fp32_output_equals_eager=graph_ok),
                        mixed_fp32={"ms_per_step": round(ms32, 4),
                                    "tflops": round(st["flops_total"] / (ms32 * 1e-3) / 1e12, 4),
                                    "speedup_vs_fp64": round(ms / ms32, 3), "rel_error_vs_fp64": err32,
