@@ -120,9 +120,10 @@ class ProfilerRegion:
         return False
 
 
-def sweep_compare(ctx, L=24, M=512, seed=1002, n_sweeps=2):
+def sweep_compare(ctx, L=24, M=512, seed=1002, n_sweeps=4):
     """Sweep time and energy of the FP64 path and of the mixed path (FP32 contraction, FP64
-    Davidson refinement) from the same seeded start; the last sweep of each is reported."""
+    Davidson refinement within the same 8-iteration budget) from the same seeded start; the
+    energies of every sweep and the time of the last are reported."""
     from qcmaquis_amd import abi
     from qcmaquis_amd.dmrg import TwoSiteDMRG
     from qcmaquis_amd.qc import QCHamiltonian, make_integrals
@@ -131,10 +132,12 @@ def sweep_compare(ctx, L=24, M=512, seed=1002, n_sweeps=2):
     out = {}
     for name, prec in (("fp64", abi.MQ_PREC_FP64), ("mixed_fp32", abi.MQ_PREC_FP32)):
         d = TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8, precision=prec)
+        energies = []
         for s in range(n_sweeps):
             t0 = dict(d.timing)
             e, dt = d.sweep()
-        out[name] = {"energy": e, "sweep_seconds": round(dt, 3),
+            energies.append(e)
+        out[name] = {"energy": e, "sweep_energies": energies, "sweep_seconds": round(dt, 3),
                      "timing_s": {k: round(v - t0[k], 3) for k, v in d.timing.items()}}
         log("%s sweep %d: E = %.10f, %.2f s %s" % (name, n_sweeps - 1, e, dt, out[name]["timing_s"]))
         del d

@@ -215,6 +215,9 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t plan);
 
 /* ---- Jacobi-Davidson (reference: solve_ietl_jcd, ietl_jacobi_davidson.h:35-76,
  *      jacobi_davidson::calculate_eigenvalue, ietl/jacobi.h:386-474) ------------------------ */
+/* precision MQ_PREC_FP32: the first max_iter - R iterations use FP32 H*psi (stopping at a relative
+ * residual of max(tol, 1e-6)), then at most R = max(1, max_iter / 4) iterations refine the Ritz
+ * vector with FP64 H*psi; orthogonalisation, Rayleigh matrix and residuals are FP64 throughout */
 int mq_davidson(mq_hpsi_plan_t plan, const double* psi0, int n_ortho, const double* const* ortho,
                 double tol, int max_iter, int precision, double* eval, double* psi_out,
                 mq_davidson_stats_t* stats);

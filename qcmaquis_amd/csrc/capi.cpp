@@ -902,7 +902,7 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
             int it = 0;
             // one Jacobi-Davidson run from V[0] with H*psi in precision `prec`; leaves the Ritz
             // vector in u
-            auto jd = [&](int prec, double tol_run) {
+            auto jd = [&](int prec, double tol_run, int iters) {
             project(V[0]);
             it = 0;
             while (true) {
@@ -938,7 +938,7 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
                 ++it;
                 rnorm = std::sqrt(dot_host(r, r));
                 bool conv = rnorm <= tol_run * std::fabs(theta) || rnorm < tol_run;  // iteration.h:57
-                if (conv || it >= max_iter) break;
+                if (conv || it >= iters) break;
                 // correction t = -r + (r.u)/(u.u) u  (jcd_gmres_solver with no GMRES steps, jacobi.h:152-156)
                 double dru = dot_host(r, u), duu = dot_host(u, u);
                 hipcheck(launch_axpby(V[it], r, n, 0.0, -1.0, s), "axpby");
@@ -946,16 +946,19 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
             }
             };
             int refine = 0;
-            if (precision == MQ_PREC_FP64) {
-                jd(MQ_PREC_FP64, tol);
+            require(precision == MQ_PREC_FP64 || precision == MQ_PREC_FP32,
+                    "mq_davidson: precision must be MQ_PREC_FP64 or MQ_PREC_FP32");
+            if (precision == MQ_PREC_FP64 || max_iter < 2) {
+                jd(MQ_PREC_FP64, tol, max_iter);
             } else {
-                // mixed precision: converge with FP32 H*psi to the FP32 floor, then refine the
-                // Ritz vector with FP64 H*psi until the requested tolerance
-                require(precision == MQ_PREC_FP32, "mq_davidson: precision must be MQ_PREC_FP64 or MQ_PREC_FP32");
-                jd(MQ_PREC_FP32, std::max(tol, 1e-6));
+                // mixed precision within the same iteration budget: FP32 H*psi for the first
+                // max_iter - R iterations (stopping at the FP32 floor), then the Ritz vector is
+                // refined with FP64 H*psi for at most R = max(1, max_iter / 4) iterations
+                const int R = std::max(1, max_iter / 4);
+                jd(MQ_PREC_FP32, std::max(tol, 1e-6), max_iter - R);
                 int it32 = it;
                 hipcheck(hipMemcpyAsync(V[0], u, n * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
-                jd(MQ_PREC_FP64, tol);
+                jd(MQ_PREC_FP64, tol, R);
                 refine = it;
                 it += it32;
             }
