@@ -218,6 +218,7 @@ class TwoSiteDMRG:
         refinement of each eigenvector, see mq_davidson)"""
         self.ham, self.M, self.tol, self.max_iter, self.cutoff = ham, M, tol, max_iter, cutoff
         self.precision = precision
+        self.compare_precision = None  # a list: every step also runs Davidson in the other precision
         self.verbose = verbose
         self.ctx = ctx or Context(0, abi.MQ_SYM_TWOU1)
         self.L = ham.L
@@ -258,6 +259,14 @@ class TwoSiteDMRG:
         t2 = time.perf_counter()
         e, psi, st = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter, precision=self.precision)
         t3 = time.perf_counter()
+        if self.compare_precision is not None:
+            # the same site problem, start vector and iteration budget in the other precision
+            other = abi.MQ_PREC_FP32 if self.precision == abi.MQ_PREC_FP64 else abi.MQ_PREC_FP64
+            c0 = time.perf_counter()
+            e2, _, st2 = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter, precision=other)
+            self.compare_precision.append({"site": i, "n": lay2.n, "e": e, "e_other": e2,
+                                           "seconds": t3 - t2, "seconds_other": time.perf_counter() - c0,
+                                           "iterations": st["iterations"], "iterations_other": st2["iterations"]})
         pst = plan.stats()
         self.hpsi_flops += pst["flops_total"] * st["iterations"]
         self.timing["plan_tables"] += pst["seconds_tables"]
