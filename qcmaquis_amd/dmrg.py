@@ -264,9 +264,17 @@ class TwoSiteDMRG:
             other = abi.MQ_PREC_FP32 if self.precision == abi.MQ_PREC_FP64 else abi.MQ_PREC_FP64
             c0 = time.perf_counter()
             e2, _, st2 = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter, precision=other)
+            c1 = time.perf_counter()
+            # both precisions converged tightly from the same start: the difference is the
+            # precision effect alone (the budgeted runs above also differ in convergence)
+            et64, _, s64 = plan.davidson(psi2, tol=1e-10, max_iter=40, precision=abi.MQ_PREC_FP64)
+            et32, _, s32 = plan.davidson(psi2, tol=1e-10, max_iter=40, precision=abi.MQ_PREC_FP32)
             self.compare_precision.append({"site": i, "n": lay2.n, "e": e, "e_other": e2,
-                                           "seconds": t3 - t2, "seconds_other": time.perf_counter() - c0,
-                                           "iterations": st["iterations"], "iterations_other": st2["iterations"]})
+                                           "seconds": t3 - t2, "seconds_other": c1 - c0,
+                                           "iterations": st["iterations"], "iterations_other": st2["iterations"],
+                                           "e_tight_fp64": et64, "e_tight_mixed": et32,
+                                           "residual_tight_fp64": s64["residual_norm"],
+                                           "residual_tight_mixed": s32["residual_norm"]})
         pst = plan.stats()
         self.hpsi_flops += pst["flops_total"] * st["iterations"]
         self.timing["plan_tables"] += pst["seconds_tables"]
