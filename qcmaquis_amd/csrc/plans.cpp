@@ -543,6 +543,12 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
     std::vector<TList> tlists(NKR);
     std::string err;
 
+    // MPO rows with a block in each left cohort
+    std::vector<std::vector<uint32_t>> lblk(L.n_cohorts());
+    for (uint32_t ci = 0; ci < L.n_cohorts(); ++ci)
+        for (uint32_t b = 0; b < L.aux_dim; ++b)
+            if (L.has_block(ci, b)) lblk[ci].push_back(b);
+
     // T work lists, one per right ket sector (lshtm_t_tasks)
 #pragma omp parallel for schedule(dynamic, 1)
     for (uint32_t rb_ket = 0; rb_ket < NKR; ++rb_ket) {
@@ -594,39 +600,59 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
                 Charge lc_bra = bra.left.charges[lb_bra];
                 uint32_t ls_bra = bra.left.sizes[lb_bra];
                 coh.add_unit(s, phys.sizes[s], ls_bra, bpb[s * NBL + lb_bra]);
-                for (uint32_t b2 = 0; b2 < mpo.col_dim; ++b2) {
-                    if (symmetric && mpo.right_herm.skip(b2, rc_bra, rc_ket)) continue;
-                    int Ap = mpo.right_spin[b2];
-                    if (su2 && !triangle(spin_of(rc_ket), Ap, spin_of(rc_bra))) continue;
-                    for (auto const& cell : mpo.cols[b2]) {
-                        uint32_t b1 = cell.other;
+                // left cohorts reachable from this unit, one per input sector (or none)
+                uint32_t ci_of[16];
+                require(NP <= 16, "lstep: more than 16 physical sectors");
+                std::vector<uint32_t> cand;
+                for (size_t si = 0; si < NP; ++si) {
+                    size_t lk = kl_of[rb_ket * NP + si];
+                    ci_of[si] = lk == NKL ? L.n_cohorts() : l_ci_bk[lb_bra * NKL + lk];
+                    if (ci_of[si] < L.n_cohorts()) cand.insert(cand.end(), lblk[ci_of[si]].begin(), lblk[ci_of[si]].end());
+                }
+                if (cand.empty()) continue;
+                std::sort(cand.begin(), cand.end());
+                cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+                // the terms of this unit, enumerated from the MPO rows b1 that have a left block and
+                // then put in the reference's order: MPO column b2, then b1, term, operator block
+                // (the b2-outer loop of lshtm_tasks, left_tasks.hpp:38-172)
+                struct Item { uint32_t b2, b1, t, k; };
+                std::vector<Item> items;
+                for (uint32_t b1 : cand)
+                    for (auto const& cell : mpo.rows[b1]) {
+                        uint32_t b2 = cell.other;
+                        if (symmetric && mpo.right_herm.skip(b2, rc_bra, rc_ket)) continue;
+                        if (su2 && !triangle(spin_of(rc_ket), mpo.right_spin[b2], spin_of(rc_bra))) continue;
                         for (uint32_t t = cell.t0; t < cell.t1; ++t) {
-                            uint32_t op = mpo.terms[t].op;
-                            SiteOp const& W = mpo.ops[op];
-                            int K = W.spin, A = mpo.left_spin[b1];
-                            for (BlkRef const& br : blk_by_out[size_t(op) * NP + s]) {
-                                OpBlock const& blk = W.blocks[br.blk];
-                                size_t lb_ket = kl_of[rb_ket * NP + br.s_in];
-                                if (lb_ket == NKL) continue;
-                                uint32_t ci = l_ci_bk[lb_bra * NKL + lb_ket];
-                                if (!L.has_block(ci, b1)) continue;
-                                Charge lc_ket = left_i.charges[lb_ket];
-                                uint32_t ls_ket = left_i.sizes[lb_ket];
-                                uint64_t ket_offset = kpb[size_t(br.s_in) * NKR + rb_ket];
-                                int64_t left_offset = L.offset(ci, b1);
-                                double cpl[4];
-                                double scale = L.conjugate_scale(ci, b1) * mpo.terms[t].scale;
-                                if (su2)
-                                    set_coupling(spin_of(lc_ket), std::abs(spin_of(rc_ket - lc_ket)), spin_of(rc_ket), A,
-                                                 K, Ap, spin_of(lc_bra), std::abs(spin_of(rc_bra - lc_bra)),
-                                                 spin_of(rc_bra), scale, cpl);
-                                else
-                                    cpl[0] = cpl[1] = cpl[2] = cpl[3] = scale;
-                                op_iterate(blk, cpl, coh, s, rs_ket, tl, ket_offset, ci, uint32_t(left_offset / ls_ket));
+                            auto const& brs = blk_by_out[size_t(mpo.terms[t].op) * NP + s];
+                            for (uint32_t k = 0; k < brs.size(); ++k) {
+                                uint32_t ci = ci_of[brs[k].s_in];
+                                if (ci < L.n_cohorts() && L.has_block(ci, b1)) items.push_back(Item{b2, b1, t, k});
                             }
                         }
                     }
-                    coh.add_line(b2);
+                std::stable_sort(items.begin(), items.end(), [](Item const& x, Item const& y) { return x.b2 < y.b2; });
+                for (size_t q = 0; q < items.size(); ++q) {
+                    Item const& it = items[q];
+                    uint32_t op = mpo.terms[it.t].op;
+                    SiteOp const& W = mpo.ops[op];
+                    BlkRef const& br = blk_by_out[size_t(op) * NP + s][it.k];
+                    OpBlock const& blk = W.blocks[br.blk];
+                    size_t lb_ket = kl_of[rb_ket * NP + br.s_in];
+                    uint32_t ci = ci_of[br.s_in];
+                    Charge lc_ket = left_i.charges[lb_ket];
+                    uint32_t ls_ket = left_i.sizes[lb_ket];
+                    uint64_t ket_offset = kpb[size_t(br.s_in) * NKR + rb_ket];
+                    int64_t left_offset = L.offset(ci, it.b1);
+                    double cpl[4];
+                    double scale = L.conjugate_scale(ci, it.b1) * mpo.terms[it.t].scale;
+                    if (su2)
+                        set_coupling(spin_of(lc_ket), std::abs(spin_of(rc_ket - lc_ket)), spin_of(rc_ket),
+                                     mpo.left_spin[it.b1], W.spin, mpo.right_spin[it.b2], spin_of(lc_bra),
+                                     std::abs(spin_of(rc_bra - lc_bra)), spin_of(rc_bra), scale, cpl);
+                    else
+                        cpl[0] = cpl[1] = cpl[2] = cpl[3] = scale;
+                    op_iterate(blk, cpl, coh, s, rs_ket, tl, ket_offset, ci, uint32_t(left_offset / ls_ket));
+                    if (q + 1 == items.size() || items[q + 1].b2 != it.b2) coh.add_line(it.b2);
                 }
             }
             if (!coh.n_terms) continue;
@@ -805,6 +831,12 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
     std::vector<TList> tlists(NKL);
     std::string err;
 
+    // MPO columns with a block in each right cohort
+    std::vector<std::vector<uint32_t>> rblk(R.n_cohorts());
+    for (uint32_t ci = 0; ci < R.n_cohorts(); ++ci)
+        for (uint32_t b = 0; b < R.aux_dim; ++b)
+            if (R.has_block(ci, b)) rblk[ci].push_back(b);
+
     // T work lists, one per left ket sector (rshtm_t_tasks)
 #pragma omp parallel for schedule(dynamic, 1)
     for (uint32_t lb_ket = 0; lb_ket < NKL; ++lb_ket) {
@@ -855,39 +887,59 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
                 Charge rc_bra = bra.right.charges[rb_bra];
                 uint32_t rs_bra = bra.right.sizes[rb_bra];
                 coh.add_unit(s, phys.sizes[s], rs_bra, bpb[lb_bra * NP + s]);
+                // right cohorts reachable from this unit, one per input sector (or none)
+                uint32_t ci_of[16];
+                require(NP <= 16, "rstep: more than 16 physical sectors");
+                std::vector<uint32_t> cand;
+                for (size_t si = 0; si < NP; ++si) {
+                    size_t rk = kr_of[lb_ket * NP + si];
+                    ci_of[si] = rk == NKR ? R.n_cohorts() : r_ci_kb[rk * NBR + rb_bra];
+                    if (ci_of[si] < R.n_cohorts()) cand.insert(cand.end(), rblk[ci_of[si]].begin(), rblk[ci_of[si]].end());
+                }
+                if (cand.empty()) continue;
+                std::sort(cand.begin(), cand.end());
+                cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
                 std::unique_ptr<Wigner9jCache> w9j;
                 if (su2) w9j.reset(new Wigner9jCache(spin_of(lc_ket), spin_of(lc_bra), spin_of(rc_bra)));
-                for (uint32_t b1 = 0; b1 < mpo.row_dim; ++b1) {
-                    if (symmetric && mpo.left_herm.skip(b1, lc_ket, lc_bra)) continue;
-                    int A = mpo.left_spin[b1];
-                    if (su2 && !triangle(spin_of(lc_ket), A, spin_of(lc_bra))) continue;
-                    for (auto const& cell : mpo.rows[b1]) {
-                        uint32_t b2 = cell.other;
+                // the terms of this unit, enumerated from the MPO columns b2 that have a right block
+                // and then put in the reference's order: MPO row b1, then b2, term, operator block
+                // (the b1-outer loop of rshtm_tasks, right_tasks.hpp:79-167)
+                struct Item { uint32_t b1, b2, t, k; };
+                std::vector<Item> items;
+                for (uint32_t b2 : cand)
+                    for (auto const& cell : mpo.cols[b2]) {
+                        uint32_t b1 = cell.other;
+                        if (symmetric && mpo.left_herm.skip(b1, lc_ket, lc_bra)) continue;
+                        if (su2 && !triangle(spin_of(lc_ket), mpo.left_spin[b1], spin_of(lc_bra))) continue;
                         for (uint32_t t = cell.t0; t < cell.t1; ++t) {
-                            uint32_t op = mpo.terms[t].op;
-                            SiteOp const& W = mpo.ops[op];
-                            int K = W.spin, Ap = mpo.right_spin[b2];
-                            for (BlkRef const& br : blk_by_out[size_t(op) * NP + s]) {
-                                OpBlock const& blk = W.blocks[br.blk];
-                                size_t rb_ket = kr_of[lb_ket * NP + br.s_in];
-                                if (rb_ket == NKR) continue;
-                                uint32_t ci = r_ci_kb[rb_ket * NBR + rb_bra];
-                                if (!R.has_block(ci, b2)) continue;
-                                Charge rc_ket = ket.right.charges[rb_ket];
-                                uint32_t rs_ket = ket.right.sizes[rb_ket];
-                                uint64_t ket_offset = kpb[lb_ket * NP + br.s_in];
-                                int64_t right_offset = R.offset(ci, b2);
-                                double cpl[4];
-                                double scale = R.conjugate_scale(ci, b2) * mpo.terms[t].scale;
-                                if (su2)
-                                    w9j->set_scale(A, K, Ap, spin_of(rc_ket), scale, cpl);
-                                else
-                                    cpl[0] = cpl[1] = cpl[2] = cpl[3] = scale;
-                                op_iterate(blk, cpl, coh, s, rs_ket, tl, ket_offset, ci, uint32_t(right_offset / rs_ket));
+                            auto const& brs = blk_by_out[size_t(mpo.terms[t].op) * NP + s];
+                            for (uint32_t k = 0; k < brs.size(); ++k) {
+                                uint32_t ci = ci_of[brs[k].s_in];
+                                if (ci < R.n_cohorts() && R.has_block(ci, b2)) items.push_back(Item{b1, b2, t, k});
                             }
                         }
                     }
-                    coh.add_line(b1);
+                std::stable_sort(items.begin(), items.end(), [](Item const& x, Item const& y) { return x.b1 < y.b1; });
+                for (size_t q = 0; q < items.size(); ++q) {
+                    Item const& it = items[q];
+                    uint32_t op = mpo.terms[it.t].op;
+                    SiteOp const& W = mpo.ops[op];
+                    BlkRef const& br = blk_by_out[size_t(op) * NP + s][it.k];
+                    OpBlock const& blk = W.blocks[br.blk];
+                    size_t rb_ket = kr_of[lb_ket * NP + br.s_in];
+                    uint32_t ci = ci_of[br.s_in];
+                    Charge rc_ket = ket.right.charges[rb_ket];
+                    uint32_t rs_ket = ket.right.sizes[rb_ket];
+                    uint64_t ket_offset = kpb[lb_ket * NP + br.s_in];
+                    int64_t right_offset = R.offset(ci, it.b2);
+                    double cpl[4];
+                    double scale = R.conjugate_scale(ci, it.b2) * mpo.terms[it.t].scale;
+                    if (su2)
+                        w9j->set_scale(mpo.left_spin[it.b1], W.spin, mpo.right_spin[it.b2], spin_of(rc_ket), scale, cpl);
+                    else
+                        cpl[0] = cpl[1] = cpl[2] = cpl[3] = scale;
+                    op_iterate(blk, cpl, coh, s, rs_ket, tl, ket_offset, ci, uint32_t(right_offset / rs_ket));
+                    if (q + 1 == items.size() || items[q + 1].b1 != it.b1) coh.add_line(it.b1);
                 }
             }
             if (!coh.n_terms) continue;
