@@ -150,7 +150,7 @@ def sweep_compare(ctx, L=24, M=512, seed=1002, n_sweeps=4):
            "sweep_seconds_mixed": round(dt - dav64 + dav32, 3),
            "davidson_seconds_fp64": round(dav64, 3), "davidson_seconds_mixed": round(dav32, 3),
            "max_abs_delta_e_converged": de_tight, "max_abs_delta_e_8_iterations": de, "worst_site": worst,
-           "timing_s": {k: round(v - t0[k], 3) for k, v in d.timing.items()},
+           "timing_s": {k: round(v - t0.get(k, 0.0), 3) for k, v in d.timing.items()},
            "note": (f"{L} orbitals, M={M}: sweep {n_sweeps - 1} of {n_sweeps} FP64 sweeps from a seeded random MPS; "
                     "each of its 46 two-site problems also solved with the mixed Davidson in the same "
                     "8-iteration budget (sweep time) and, for the energy difference, with both Davidsons "

@@ -141,6 +141,19 @@ typedef struct {
     int32_t  refine_iterations;       /* FP64 iterations after an FP32 run (precision MQ_PREC_FP32) */
 } mq_davidson_stats_t;
 
+/* How a boundary made by mq_lstep / mq_rstep was built (all zero for uploaded and initial
+ * boundaries).  Flop model as for H*psi, over the growth schedule (tasks.cpp:389-398). */
+typedef struct {
+    uint64_t n_lines, n_terms, n_sblocks;
+    uint64_t t_elements, s_elements, out_elements;
+    uint64_t flops_total, flops_gemm;
+    double   seconds_tables;          /* host: schedule tables built and checked */
+    double   seconds_blocks;          /* host: term-sum block tables built and checked */
+    double   seconds_upload;          /* device buffers allocated, tables uploaded */
+    double   seconds_device;          /* the step's kernels, to completion */
+    double   seconds_total;           /* the whole call */
+} mq_growth_stats_t;
+
 /* ---- context ---------------------------------------------------------------------------- */
 int         mq_ctx_create(int device, int symmetry, mq_ctx_t* out);
 int         mq_ctx_destroy(mq_ctx_t ctx);
@@ -149,8 +162,8 @@ int         mq_ctx_sync(mq_ctx_t ctx);
 void*       mq_ctx_stream(mq_ctx_t ctx);      /* hipStream_t of the context */
 int         mq_version(void);
 /* sizes in bytes of the ABI structs, in this order: mq_index_desc, mq_mpo_desc, mq_bindex_desc,
- * mq_boundary_info_t, mq_hpsi_stats_t, mq_davidson_stats_t; returns how many were written (a
- * binding checks its own struct layouts against these before the first call) */
+ * mq_boundary_info_t, mq_hpsi_stats_t, mq_davidson_stats_t, mq_growth_stats_t; returns how many
+ * were written (a binding checks its own struct layouts against these before the first call) */
 int         mq_abi_struct_sizes(uint32_t* sizes, int n);
 
 /* ---- MPO ---------------------------------------------------------------------------------- */
@@ -169,6 +182,8 @@ int mq_boundary_export_index(mq_boundary_t b, int32_t* bra_charges, uint32_t* br
                              int32_t* ket_charges, uint32_t* ket_sizes, uint32_t* cohort_lr,
                              int64_t* offsets, double* conj_scales, uint8_t* transposes, uint8_t* tr);
 int mq_boundary_download(mq_boundary_t b, double* data);
+/* build statistics of a boundary returned by mq_lstep / mq_rstep */
+int mq_boundary_growth_stats(mq_boundary_t b, mq_growth_stats_t* stats);
 int mq_boundary_destroy(mq_boundary_t b);
 
 /* ---- MPS site tensors in HBM ------------------------------------------------------------ */

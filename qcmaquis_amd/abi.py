@@ -104,6 +104,13 @@ class DavidsonStats(C.Structure):
                 ("hpsi_seconds", C.c_double), ("refine_iterations", C.c_int32)]
 
 
+class GrowthStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("n_lines", "n_terms", "n_sblocks", "t_elements", "s_elements",
+                                          "out_elements", "flops_total", "flops_gemm")] + \
+               [(f, C.c_double) for f in ("seconds_tables", "seconds_blocks", "seconds_upload", "seconds_device",
+                                          "seconds_total")]
+
+
 vp = C.c_void_p
 i32 = C.c_int
 dp = P(C.c_double)
@@ -126,6 +133,7 @@ SIGNATURES = [
                                        P(C.c_uint32), P(C.c_int64), dp, P(C.c_uint8), P(C.c_uint8)]),
     ("mq_boundary_download", i32, [vp, dp]),
     ("mq_boundary_destroy", i32, [vp]),
+    ("mq_boundary_growth_stats", i32, [vp, P(GrowthStats)]),
     ("mq_mps_upload", i32, [vp, P(IndexDesc), P(IndexDesc), P(IndexDesc), i32, dp, P(vp)]),
     ("mq_mps_elements", i32, [vp, P(C.c_uint64)]),
     ("mq_mps_download", i32, [vp, i32, dp]),
@@ -166,7 +174,8 @@ def lib():
             f.restype = res
             f.argtypes = args
         # the library writes these structs: a layout mismatch would overwrite Python memory
-        mine = [C.sizeof(t) for t in (IndexDesc, MPODesc, BIndexDesc, BoundaryInfo, HpsiStats, DavidsonStats)]
+        mine = [C.sizeof(t) for t in (IndexDesc, MPODesc, BIndexDesc, BoundaryInfo, HpsiStats, DavidsonStats,
+                                       GrowthStats)]
         theirs = (C.c_uint32 * len(mine))()
         got = L.mq_abi_struct_sizes(theirs, len(mine))
         if got != len(mine) or list(theirs) != mine:

@@ -2,10 +2,12 @@
 // steps and the Jacobi-Davidson eigensolver.  No C++ type crosses the ABI; errors are returned as
 // codes with the message kept on the context.
 #include <hip/hip_runtime.h>
+#include <omp.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -38,6 +40,7 @@ struct mq_boundary_s {
     BoundaryIndex idx;
     double* data = nullptr;
     uint64_t n = 0;
+    mq_growth_stats_t grow{};  // how mq_lstep / mq_rstep built it
 };
 
 struct mq_mps_s {
@@ -242,7 +245,8 @@ int mq_version(void) { return 1; }
 int mq_abi_struct_sizes(uint32_t* sizes, int n) {
     const uint32_t v[] = {uint32_t(sizeof(mq_index_desc)), uint32_t(sizeof(mq_mpo_desc)),
                           uint32_t(sizeof(mq_bindex_desc)), uint32_t(sizeof(mq_boundary_info_t)),
-                          uint32_t(sizeof(mq_hpsi_stats_t)), uint32_t(sizeof(mq_davidson_stats_t))};
+                          uint32_t(sizeof(mq_hpsi_stats_t)), uint32_t(sizeof(mq_davidson_stats_t)),
+                          uint32_t(sizeof(mq_growth_stats_t))};
     int k = 0;
     for (; sizes && k < n && k < int(sizeof(v) / sizeof(v[0])); ++k) sizes[k] = v[k];
     return k;
@@ -268,6 +272,11 @@ int mq_ctx_create(int device, int symmetry, mq_ctx_t* out) {
     if (symmetry != MQ_SYM_TWOU1 && symmetry != MQ_SYM_SU2U1) return MQ_ERR_INVALID;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MQ_ERR_HIP;
+    // The schedule builders' OpenMP regions alternate with the caller's own multithreaded host
+    // work (LAPACK in a sweep's SVD split, for one).  libomp keeps idle workers spinning for 200 ms
+    // after each region by default, which takes those cores from that work; here they sleep at
+    // once, unless the caller has set KMP_BLOCKTIME.
+    if (!std::getenv("KMP_BLOCKTIME")) kmp_set_blocktime(0);
     auto* c = new mq_ctx_s();
     c->device = device;
     c->sym = Sym(symmetry);
@@ -455,8 +464,19 @@ int mq_mps_destroy(mq_mps_t m) {
 }
 
 // ---- growth steps ------------------------------------------------------------------------------
+static double seconds_since(std::chrono::steady_clock::time_point& t) {
+    auto now = std::chrono::steady_clock::now();
+    double s = std::chrono::duration<double>(now - t).count();
+    t = now;
+    return s;
+}
+
+// t0: start of the call; the schedule tables were built since then
 static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double const* bra, uint64_t bra_n,
-                       double const* ket, uint64_t ket_n, uint64_t bnd_n, int precision, mq_boundary_t* out) {
+                       double const* ket, uint64_t ket_n, uint64_t bnd_n, int precision, mq_boundary_t* out,
+                       std::chrono::steady_clock::time_point t0) {
+    mq_growth_stats_t st{};
+    auto t = t0;
     uint64_t sizes[BASE_COUNT] = {0};
     sizes[BASE_T] = P.t_elems;
     sizes[BASE_S] = P.s_elems;
@@ -467,8 +487,18 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
     check_gemms(P.t_gemms, sizes);
     check_gemms(P.c_gemms, sizes);
     check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
+    st.seconds_tables = seconds_since(t);
     SBlockTable SB = build_sblocks(P.lines, P.terms);
     check_sblocks(SB, P.t_elems, P.s_elems);
+    st.seconds_blocks = seconds_since(t);
+    st.n_lines = P.lines.size();
+    st.n_terms = P.terms.size();
+    st.n_sblocks = SB.blocks.size();
+    st.t_elements = P.t_elems;
+    st.s_elements = P.s_elems;
+    st.out_elements = P.out_elems;
+    st.flops_total = P.flops.flops_total;
+    st.flops_gemm = P.flops.flops_gemm;
 
     auto* nb = new mq_boundary_s{ctx, std::move(P.out)};
     std::unique_ptr<mq_boundary_s> guard(nb);
@@ -482,6 +512,7 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
         tg.upload(P.t_gemms, ctx->stream);
         cg.upload(P.c_gemms, ctx->stream);
         sa.upload(SB, ctx->stream);
+        st.seconds_upload = seconds_since(t);
         Bases b{};
         b.p[BASE_T] = T;
         b.p[BASE_S] = S;
@@ -493,12 +524,15 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
         sa.run(T, S, ctx->stream);
         cg.run(b, ctx->stream, GEMM_F64);
         hipcheck(hipStreamSynchronize(ctx->stream), "growth step");
+        st.seconds_device = seconds_since(t);
     } catch (...) {
         (void)hipStreamSynchronize(ctx->stream);
         tg.release(); cg.release(); sa.release(); dev_free(T); dev_free(S);
         throw;
     }
     tg.release(); cg.release(); sa.release(); dev_free(T); dev_free(S);
+    st.seconds_total = seconds_since(t0);
+    nb->grow = st;
     *out = guard.release();
 }
 
@@ -506,11 +540,12 @@ int mq_lstep(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_mps_t bra, mq_mp
              mq_boundary_t* out) {
     return guarded(ctx, [&] {
         require(mpo && left && bra && ket && out, "mq_lstep: null argument");
+        auto t0 = std::chrono::steady_clock::now();
         GrowPlanHost P = build_lstep_plan(ctx->sym, mpo->mpo, left->idx, bra->lay, ket->lay, symmetric != 0);
         ensure_lp(bra);
         ensure_rp(ket);
         run_growth(ctx, P, left->data, bra->lp, bra->lay.elements(), ket->rp, ket->lay.elements(), left->n,
-                   precision, out);
+                   precision, out, t0);
     });
 }
 
@@ -518,12 +553,19 @@ int mq_rstep(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t right, mq_mps_t bra, mq_m
              mq_boundary_t* out) {
     return guarded(ctx, [&] {
         require(mpo && right && bra && ket && out, "mq_rstep: null argument");
+        auto t0 = std::chrono::steady_clock::now();
         GrowPlanHost P = build_rstep_plan(ctx->sym, mpo->mpo, right->idx, bra->lay, ket->lay, symmetric != 0);
         ensure_rp(bra);
         ensure_rp(ket);
         run_growth(ctx, P, right->data, bra->rp, bra->lay.elements(), ket->rp, ket->lay.elements(), right->n,
-                   precision, out);
+                   precision, out, t0);
     });
+}
+
+int mq_boundary_growth_stats(mq_boundary_t b, mq_growth_stats_t* stats) {
+    if (!b || !stats) return MQ_ERR_INVALID;
+    *stats = b->grow;
+    return MQ_OK;
 }
 
 // ---- H*psi ------------------------------------------------------------------------------------

@@ -297,6 +297,9 @@ class TwoSiteDMRG:
             m = DeviceMPS(self.ctx, lb, b)
             self.right[i + 1] = rstep(self.ctx, self.mpo1[i + 1], self.right[i + 2], m, m)
         t5 = time.perf_counter()
+        g = (self.left[i + 1] if to_right else self.right[i + 1]).growth_stats()
+        for k in ("tables", "blocks", "upload", "device"):
+            self.timing["grow_" + k] = self.timing.get("grow_" + k, 0.0) + g["seconds_" + k]
         for k, v in (("merge", tm - t0), ("plan", t2 - t1), ("davidson", t3 - t2), ("svd", t4 - t3s), ("grow", t5 - t4)):
             self.timing[k] += v
         self.energies.append(e + self.ham.ecore)

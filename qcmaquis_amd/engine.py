@@ -113,6 +113,12 @@ class DeviceBoundary(_Handle):
         self.ctx.check(self.ctx.lib.mq_boundary_info(self.h, C.byref(i)))
         return i
 
+    def growth_stats(self):
+        """how mq_lstep / mq_rstep built this boundary (zeros otherwise)"""
+        g = abi.GrowthStats()
+        self.ctx.check(self.ctx.lib.mq_boundary_growth_stats(self.h, C.byref(g)))
+        return {f: getattr(g, f) for f, _ in abi.GrowthStats._fields_}
+
     def download(self):
         """BoundarySpec with the index and data of this boundary."""
         i = self.info()

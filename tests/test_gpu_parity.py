@@ -33,6 +33,9 @@ def _grow_both(ctx, oracle, ch, symmetric=True):
         print(f"lstep site {k}: cohorts={len(got.cohorts)} elems={got.data.size}", flush=True)
         same_index(got, L_or[-1])
         assert rel_err(got.data, L_or[-1].data) < TOL
+        g = L_dev[-1].growth_stats()
+        assert g["out_elements"] == got.data.size and g["seconds_total"] > 0
+        assert g["seconds_total"] >= g["seconds_tables"] + g["seconds_blocks"] + g["seconds_device"]
     R_dev = {n: DeviceBoundary.initial(ctx, ch.bonds[n], ch.mpos[n - 1].col_dim)}
     R_or = {n: initial_boundary(ch.bonds[n], ch.mpos[n - 1].col_dim)}
     for k in range(n - 1, 0, -1):
