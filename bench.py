@@ -4,9 +4,10 @@
 One "step" = one H*psi of the site problem with psi, psi', boundaries and work lists resident in
 HBM (one Davidson iteration's operator application, reference super_hamil_mv).  The value is the
 reference's own flop model (ScheduleNew::total_flops, solver/tasks.cpp:389-398, 778-791) per
-second.  Also reported: the roofline of the dominant kernel (the grouped FP64 MFMA GEMM) from HIP
-events recorded on the launch stream, and the CPU oracle (restated reference CPU path, OpenMP +
-OpenBLAS) timed on the host cores on the same site problem.
+second.  Also reported: the roofline of the dominant kernel (the block-form MPO-term sum,
+sblock_kernel<double>) and of the grouped FP64 GEMMs, from HIP events recorded on the launch stream,
+and the CPU oracle (restated reference CPU path, OpenMP + OpenBLAS) timed on the host cores on the
+same site problem.
 
   python bench.py [--steps K] [--warmup W] [--workload synthetic] [--no-cpu]
 """
@@ -24,6 +25,7 @@ sys.path.insert(0, ROOT)
 METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"]
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+GEMM_FAMILY = "gemm_kernel<*, double, double> (T-GEMM and L*S GEMM launches, split-K reduce)"
 
 
 def log(*a):
@@ -301,25 +303,29 @@ def main():
     # its S tiles (FP64)
     s_bytes = 8 * (st["union_elements"] + st["s_line_elements"])
     s_gbs = s_bytes / (stage[2] * 1e-3) / 1e9
-    rl_gemm = {"bound": "mfma", "kernel": "gemm_f64_kernel (T-GEMM + L*S GEMM launches)",
+    rl_gemm = {"bound": "mfma", "kernel": GEMM_FAMILY,
                "achieved": round(gemm_tf, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                "frac": round(gemm_tf / PEAK_FP64_TFLOPS, 5), "traffic": None,
-               "work_per_launch": "%d FP64 flops" % st["flops_gemm_exec"]}
+               "work_per_launch": "%d FP64 flops over the T-GEMM and L*S GEMM stages (%.4f ms)" % (
+                   st["flops_gemm_exec"], gemm_ms)}
     rl_sasm = {"bound": "hbm", "kernel": "sblock_kernel<double>", "achieved": round(s_gbs, 2), "peak": PEAK_HBM_GBS,
                "unit": "GB/s", "frac": round(s_gbs / PEAK_HBM_GBS, 5), "traffic": None,
                "work_per_launch": "%d bytes (%d blocks of 16 lines: union T tile reads %d el, S writes %d el; "
                                   "%d terms, %d el in the per-term formulation)" % (
                    s_bytes, st["n_sblocks"], st["union_elements"], st["s_line_elements"], st["n_terms"],
                    st["gathered_elements"])}
-    dominant, other = (rl_sasm, rl_gemm) if stage[2] >= gemm_ms else (rl_gemm, rl_sasm)
-    # HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter runs of this
-    # default workload (FETCH_SIZE / WRITE_SIZE scaled by a device-copy calibration)
+    # the dominant kernel: the term sum is one launch per H*psi and the largest single share of its
+    # time; the GEMM stages are spread over three tile-shape instances per stage and a reduce
+    dominant, other = rl_sasm, rl_gemm
+    # HBM bytes per launch (per H*psi for the GEMM family) from the committed rocprofv3 counter runs
+    # of this default workload (FETCH_SIZE / WRITE_SIZE scaled by a device-copy calibration)
     try:
         prof = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))
-        rec = prof.get(dominant["kernel"])
-        if rec and args.workload == "qc" and args.warm_sweeps == 1:
-            dominant["traffic"] = round((rec["fetch_gb"] + rec["write_gb"]) * 1e9)
-            dominant["traffic_source"] = rec["source"]
+        for rl in (dominant, other):
+            rec = prof.get(rl["kernel"])
+            if rec and args.workload == "qc" and args.warm_sweeps == 1:
+                rl["traffic"] = round((rec["fetch_gb"] + rec["write_gb"]) * 1e9)
+                rl["traffic_source"] = rec["source"]
     except (OSError, ValueError, KeyError):
         pass
     line = {

@@ -1,12 +1,15 @@
 """Summaries of rocprofv3 CSV output for the contraction kernels.
 
     python -m qcmaquis_amd.profiling KERNEL_TRACE_OR_COUNTER_CSV [--last N] [--scale COUNTER=F ...]
+    python -m qcmaquis_amd.profiling CSV --around REGEX [--before B] [--after A] [--scale ...]
 
 Groups the dispatches of a ``*_kernel_trace.csv`` or ``*_counter_collection.csv`` by kernel name,
 keeps the last N dispatches of each kernel (the timed region of a benchmark is the end of its
 run: a workload set-up that launches the same kernels comes first) and prints a markdown table of
 their count, mean / min / max duration and, for counter files, the mean counter value (KiB for
-FETCH_SIZE / WRITE_SIZE) multiplied by the given calibration factor.
+FETCH_SIZE / WRITE_SIZE) multiplied by the given calibration factor.  With --around, lists the
+dispatches in launch order from B before to A after the last dispatch whose name matches REGEX
+(one H*psi's launch sequence, for instance).
 """
 import argparse
 import collections
@@ -65,6 +68,27 @@ def summarize(path, last=0, scale=None, match=None):
     return table
 
 
+def around(path, regex, before=3, after=4, scale=None):
+    """the dispatches in launch order around the last one matching regex"""
+    scale = scale or {}
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    hits = [i for i, r in enumerate(rows) if re.search(regex, r["Kernel_Name"])]
+    if not hits:
+        return []
+    i = hits[-1]
+    out = []
+    for r in rows[max(0, i - before):i + after + 1]:
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6
+        ent = {"kernel": short_name(r["Kernel_Name"]), "dispatches": 1, "of": 1, "mean_ms": dur, "min_ms": dur,
+               "max_ms": dur}
+        if "Counter_Name" in r:
+            ent["counter"] = r["Counter_Name"]
+            ent["mean_kib"] = float(r["Counter_Value"])
+            ent["scaled_gb"] = ent["mean_kib"] * 1024 * scale.get(r["Counter_Name"], 1.0) / 1e9
+        out.append(ent)
+    return out
+
+
 def markdown(table):
     has_c = any("counter" in t for t in table)
     head = "| kernel | dispatches (of) | mean ms | min ms | max ms |" + (" counter | mean KiB | scaled GB |" if has_c else "")
@@ -84,9 +108,15 @@ def main(argv=None):
     ap.add_argument("--last", type=int, default=0, help="keep the last N dispatches of each kernel (0 = all)")
     ap.add_argument("--match", default=None, help="regex on the kernel name")
     ap.add_argument("--scale", nargs="*", default=[], help="COUNTER=factor calibration, e.g. FETCH_SIZE=2.0")
+    ap.add_argument("--around", default=None, help="regex: list the launch sequence around its last dispatch")
+    ap.add_argument("--before", type=int, default=3)
+    ap.add_argument("--after", type=int, default=4)
     a = ap.parse_args(argv)
     scale = {k: float(v) for k, v in (s.split("=") for s in a.scale)}
-    print(markdown(summarize(a.csv, a.last, scale, a.match)))
+    if a.around:
+        print(markdown(around(a.csv, a.around, a.before, a.after, scale)))
+    else:
+        print(markdown(summarize(a.csv, a.last, scale, a.match)))
 
 
 if __name__ == "__main__":
