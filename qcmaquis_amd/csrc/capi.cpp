@@ -123,14 +123,16 @@ int guarded(mq_ctx_t ctx, F&& f) {
 // the workspace of the partial tiles, sized for FP64 and shared by the FP32 launches)
 struct GemmStage {
     GemmDesc* probs = nullptr;
+    GemmRow* rows = nullptr;
     GemmTile* tiles[GEMM_N_SHAPES] = {nullptr, nullptr, nullptr};
     int64_t n_tiles[GEMM_N_SHAPES] = {0, 0, 0};
     GemmReduce* reduces = nullptr;
     int64_t n_reduces = 0;
     double* ws = nullptr;
-    void upload(std::vector<GemmDesc> const& v, hipStream_t s) {
+    void upload(std::vector<GemmDesc> const& v, std::vector<GemmRow> const& r, hipStream_t s) {
         GemmSchedule sch = build_gemm_schedule(v);
         probs = dev_upload(v, s);
+        rows = dev_upload(r, s);
         for (int sh = 0; sh < GEMM_N_SHAPES; ++sh) {
             tiles[sh] = dev_upload(sch.tiles[sh], s);
             n_tiles[sh] = int64_t(sch.tiles[sh].size());
@@ -140,10 +142,12 @@ struct GemmStage {
         ws = dev_alloc<double>(sch.ws_elems);
     }
     void run(Bases const& b, hipStream_t s, GemmPrec prec) const {
-        hipcheck(launch_gemm(prec, probs, tiles, n_tiles, reduces, n_reduces, ws, b, s), "gemm launch");
+        hipcheck(launch_gemm(prec, probs, rows, tiles, n_tiles, reduces, n_reduces, ws, b, s), "gemm launch");
     }
     void release() {
         dev_free(probs);
+        dev_free(rows);
+        rows = nullptr;
         for (auto& t : tiles) dev_free(t), t = nullptr;
         dev_free(reduces);
         dev_free(ws);
@@ -484,7 +488,7 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
     sizes[BASE_BRA] = bra_n;
     sizes[BASE_KET] = ket_n;
     sizes[BASE_NEW] = P.out_elems;
-    check_gemms(P.t_gemms, sizes);
+    check_gemms(P.t_gemms, sizes, P.t_rows);
     check_gemms(P.c_gemms, sizes);
     check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
     st.seconds_tables = seconds_since(t);
@@ -509,8 +513,8 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
     GemmStage tg, cg;
     SAsmStage sa;
     try {
-        tg.upload(P.t_gemms, ctx->stream);
-        cg.upload(P.c_gemms, ctx->stream);
+        tg.upload(P.t_gemms, P.t_rows, ctx->stream);
+        cg.upload(P.c_gemms, {}, ctx->stream);
         sa.upload(SB, ctx->stream);
         st.seconds_upload = seconds_since(t);
         Bases b{};
@@ -590,7 +594,7 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
         sizes[BASE_S] = H.s_elems;
         sizes[BASE_LPANEL] = H.lpanel_elems;
         sizes[BASE_BOUNDARY] = right->n;
-        check_gemms(H.t_gemms, sizes);
+        check_gemms(H.t_gemms, sizes, H.t_rows);
         check_gemms(H.c_gemms, sizes);
         check_lines(H.lines, H.terms, H.t_elems, H.s_elems);
         p->seconds_tables = seconds_since(t0);
@@ -607,8 +611,8 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
         p->T = dev_alloc<double>(H.t_elems);
         p->S = dev_alloc<double>(H.s_elems);
         p->Lpanel = dev_alloc<double>(H.lpanel_elems);
-        p->tg.upload(H.t_gemms, ctx->stream);
-        p->cg.upload(H.c_gemms, ctx->stream);
+        p->tg.upload(H.t_gemms, H.t_rows, ctx->stream);
+        p->cg.upload(H.c_gemms, {}, ctx->stream);
         p->sa.upload(SB, ctx->stream);
         p->n_lines = int64_t(H.lines.size());
         p->n_sblocks = int64_t(SB.blocks.size());

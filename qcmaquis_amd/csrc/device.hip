@@ -49,6 +49,7 @@ template <class T> __device__ inline T ld_or_zero(const T* p, bool ok) { return 
 // partial tile (type TI) into the workspace (split K).
 template <int FM, int FN, class TI, class TO, int KS = BK, int MINB = 1>
 __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restrict__ probs,
+                                                         const GemmRow* __restrict__ rows,
                                                          const GemmTile* __restrict__ tiles, Bases bases,
                                                          TI* __restrict__ ws) {
     constexpr int BM = 32 * FM, BN = 32 * FN;
@@ -64,10 +65,18 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const int m0 = tile.tm * BM, n0 = tile.tn * BN;
-    const bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B;
+    const bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B, rmap = g.flags & GEMM_ROWMAP;
 
-    const TI* A = static_cast<const TI*>(bases.p[g.base_a]) + g.a + int64_t(tile.batch) * g.sa;
+    const TI* A = static_cast<const TI*>(bases.p[g.base_a]) + (rmap ? 0 : g.a + int64_t(tile.batch) * g.sa);
     const TI* B = static_cast<const TI*>(bases.p[g.base_b]) + g.b + int64_t(tile.batch) * g.sb;
+    // stacked problem: A is not transposed, so each thread loads from one row (tid % BM) of the tile
+    const TI* a_row = A;
+    int64_t a_rld = 0;
+    if (rmap && m0 + tid % BM < g.M) {
+        const GemmRow r = rows[g.a + m0 + tid % BM];
+        a_row = A + r.a;
+        a_rld = r.ld;
+    }
 
     // per-thread load coordinates inside the BM x 16 A slice and the 16 x BN B slice, with the
     // contiguous dimension of the operand across consecutive threads
@@ -89,7 +98,8 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
         for (int i = 0; i < NA; ++i) {
             int m = m0 + a_m[i], k = k0 + a_k[i];
             bool ok = (m < g.M) && (k < k_end);
-            const TI* pa = ta ? A + int64_t(m) * g.lda + k : A + m + int64_t(k) * g.lda;
+            const TI* pa = rmap ? a_row + int64_t(k) * a_rld
+                                : ta ? A + int64_t(m) * g.lda + k : A + m + int64_t(k) * g.lda;
             ra[i] = ld_or_zero(pa, ok);
         }
 #pragma unroll
@@ -136,22 +146,31 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
 
     // D = C^T: register r of lane l holds C[l % 16][row(l, r)] of its 16 x 16 tile, so
     // consecutive lanes store consecutive rows of a column
-    TO* C = static_cast<TO*>(bases.p[g.base_c]) + g.c + int64_t(tile.batch) * g.sc;
-    const bool split = tile.ws >= 0;
+    TO* C = static_cast<TO*>(bases.p[g.base_c]) + (rmap ? 0 : g.c + int64_t(tile.batch) * g.sc);
+    const bool split = tile.ws >= 0;  // never for stacked problems
 #pragma unroll
-    for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i) {
+        const int mm = wm + 16 * i + fr, m = m0 + mm;
+        if (m >= g.M) continue;
+        // column n of this row at c_row + n * c_ld
+        TO* c_row = C + m;
+        int64_t c_ld = g.ldc;
+        if (rmap) {
+            const GemmRow r = rows[g.a + m];
+            c_row = C + r.c + int64_t(tile.batch) * g.sc * r.ld;
+            c_ld = r.ld;
+        }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-            int mm = wm + 16 * i + fr, m = m0 + mm;
-            if (m >= g.M) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 int nn = wn + 16 * j + Mma<TI>::row(lane, r), n = n0 + nn;
                 if (n >= g.N) continue;
                 if (split) ws[tile.ws + mm + int64_t(nn) * BM] = acc[i][j][r];
-                else C[m + int64_t(n) * g.ldc] = TO(acc[i][j][r]);
+                else c_row[int64_t(n) * c_ld] = TO(acc[i][j][r]);
             }
         }
+    }
 }
 
 // C tile = sum of its split partials, added in split order; blockIdx.y selects 256 elements of
@@ -353,7 +372,7 @@ namespace {
 constexpr int GEMM_KS = 8, GEMM_MINB = 2;
 
 template <class TI, class TO>
-void launch_gemm_t(const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
+void launch_gemm_t(const GemmDesc* probs, const GemmRow* rows, const GemmTile* const* tiles, const int64_t* n_tiles,
                    const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases, hipStream_t stream) {
     TI* w = static_cast<TI*>(ws);
     for (int sh = 0; sh < GEMM_N_SHAPES; ++sh) {
@@ -361,13 +380,13 @@ void launch_gemm_t(const GemmDesc* probs, const GemmTile* const* tiles, const in
             int64_t cnt = n_tiles[sh] - off < 0x7fffffff ? n_tiles[sh] - off : 0x7fffffff;
             if (sh == GEMM_32x128)
                 hipLaunchKernelGGL((gemm_kernel<1, 4, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
-                                   tiles[sh] + off, bases, w);
+                                   rows, tiles[sh] + off, bases, w);
             else if (sh == GEMM_64x64)
                 hipLaunchKernelGGL((gemm_kernel<2, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
-                                   tiles[sh] + off, bases, w);
+                                   rows, tiles[sh] + off, bases, w);
             else
                 hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
-                                   tiles[sh] + off, bases, w);
+                                   rows, tiles[sh] + off, bases, w);
         }
     }
     for (int64_t off = 0; off < n_reduces; off += 0x7fffffff) {
@@ -379,13 +398,18 @@ void launch_gemm_t(const GemmDesc* probs, const GemmTile* const* tiles, const in
 
 }  // namespace
 
-hipError_t launch_gemm(GemmPrec prec, const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
-                       const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases, hipStream_t stream) {
+hipError_t launch_gemm(GemmPrec prec, const GemmDesc* probs, const GemmRow* rows, const GemmTile* const* tiles,
+                       const int64_t* n_tiles, const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases,
+                       hipStream_t stream) {
     switch (prec) {
-        case GEMM_F64: launch_gemm_t<double, double>(probs, tiles, n_tiles, reduces, n_reduces, ws, bases, stream); break;
-        case GEMM_F32: launch_gemm_t<float, float>(probs, tiles, n_tiles, reduces, n_reduces, ws, bases, stream); break;
+        case GEMM_F64:
+            launch_gemm_t<double, double>(probs, rows, tiles, n_tiles, reduces, n_reduces, ws, bases, stream);
+            break;
+        case GEMM_F32:
+            launch_gemm_t<float, float>(probs, rows, tiles, n_tiles, reduces, n_reduces, ws, bases, stream);
+            break;
         case GEMM_F32_OUT_F64:
-            launch_gemm_t<float, double>(probs, tiles, n_tiles, reduces, n_reduces, ws, bases, stream);
+            launch_gemm_t<float, double>(probs, rows, tiles, n_tiles, reduces, n_reduces, ws, bases, stream);
             break;
         default: return hipErrorInvalidValue;
     }

@@ -21,9 +21,11 @@ struct Bases {
 enum GemmPrec { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_F32_OUT_F64 = 2 };
 
 // grouped GEMM stage: tiles of each shape (GemmShape) then the split-K reduce jobs; ws is the
-// workspace of the split partials (GemmSchedule::ws_elems elements of the compute type)
-hipError_t launch_gemm(GemmPrec prec, const GemmDesc* probs, const GemmTile* const* tiles, const int64_t* n_tiles,
-                       const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases, hipStream_t stream);
+// workspace of the split partials (GemmSchedule::ws_elems elements of the compute type); rows is
+// the row table of the stage's stacked (GEMM_ROWMAP) problems
+hipError_t launch_gemm(GemmPrec prec, const GemmDesc* probs, const GemmRow* rows, const GemmTile* const* tiles,
+                       const int64_t* n_tiles, const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases,
+                       hipStream_t stream);
 // block-form MPO-term sum (SBlockTable uploaded; alpha, T and S of type T): writes every S tile
 // of the table's lines
 template <class T>

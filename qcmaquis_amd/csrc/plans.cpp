@@ -196,24 +196,55 @@ GemmDesc gemm(int32_t M, int32_t N, int32_t K, int base_a, int64_t a, int32_t ld
     return g;
 }
 
-// right-boundary T-GEMMs shared by H*psi and the right step:
-// T(ls_ket x nb*rs_bra) = psi_ket(:, col : col+rs_ket) * R[ci_eff], per-block transposed when the
-// cohort is stored as its Hermitian partner (reference: MPSBlock::create_T, tasks.cpp:650-752)
-void right_t_gemm(BoundaryIndex const& R, TTask const& t, uint32_t ls_ket, uint32_t rs_ket, uint32_t rs_bra,
-                  int64_t psi_block_off, int psi_base, std::vector<GemmDesc>& out) {
-    uint32_t nb = R.nblk[t.ci_eff];
-    int64_t a = psi_block_off + int64_t(t.psi_col) * ls_ket;
-    if (t.ci == t.ci_eff) {
-        check_cohort_shape(R, t.ci_eff, rs_ket, rs_bra, "T-GEMM");
-        out.push_back(gemm(ls_ket, int32_t(nb * rs_bra), rs_ket, psi_base, a, ls_ket, BASE_BOUNDARY,
-                           int64_t(R.data_off[t.ci_eff]), rs_ket, BASE_T, int64_t(t.t_off), ls_ket));
-    } else {
-        check_cohort_shape(R, t.ci_eff, rs_bra, rs_ket, "T-GEMM");
-        out.push_back(gemm(ls_ket, rs_bra, rs_ket, psi_base, a, ls_ket, BASE_BOUNDARY, int64_t(R.data_off[t.ci_eff]),
-                           rs_bra, BASE_T, int64_t(t.t_off), ls_ket, GEMM_TRANS_B, int32_t(nb), 0,
-                           int64_t(rs_bra) * rs_ket, int64_t(ls_ket) * rs_bra));
+// Right-boundary T-GEMMs shared by H*psi and the right step (reference: MPSBlock::create_T,
+// tasks.cpp:650-752, one dgemm per task): T(ls_ket x nb*rs_bra) = psi_ket(:, col : col+rs_ket) *
+// R[ci_eff], per block transposed when the cohort is stored as its Hermitian partner.  The tasks
+// that read one cohort in one orientation are stacked into one problem (rows from a row table),
+// so each cohort is read once per row tile instead of once per task; stacks in order of first use.
+struct RightTStacks {
+    struct Ref {
+        TTask const* t;
+        uint32_t ls_ket, rs_ket, rs_bra;
+        int64_t psi_block_off;
+    };
+    std::vector<std::vector<Ref>> stacks;
+    std::unordered_map<uint64_t, size_t> pos;
+    void add(TTask const& t, uint32_t ls_ket, uint32_t rs_ket, uint32_t rs_bra, int64_t psi_block_off) {
+        uint64_t key = uint64_t(t.ci_eff) * 2 + (t.ci != t.ci_eff);
+        auto it = pos.find(key);
+        if (it == pos.end()) {
+            it = pos.emplace(key, stacks.size()).first;
+            stacks.emplace_back();
+        }
+        stacks[it->second].push_back(Ref{&t, ls_ket, rs_ket, rs_bra, psi_block_off});
     }
-}
+    void emit(BoundaryIndex const& R, int psi_base, std::vector<GemmDesc>& out, std::vector<GemmRow>& rows) const {
+        for (auto const& st : stacks) {
+            Ref const& f = st.front();
+            const uint32_t ci_eff = f.t->ci_eff, nb = R.nblk[ci_eff], rs_ket = f.rs_ket, rs_bra = f.rs_bra;
+            const bool trans = f.t->ci != ci_eff;
+            if (!trans) check_cohort_shape(R, ci_eff, rs_ket, rs_bra, "T-GEMM");
+            else check_cohort_shape(R, ci_eff, rs_bra, rs_ket, "T-GEMM");
+            const int64_t row0 = int64_t(rows.size());
+            int64_t M = 0;
+            for (Ref const& r : st) {
+                require(r.rs_ket == rs_ket && r.rs_bra == rs_bra, "T-GEMM: stacked tasks disagree on the cohort shape");
+                int64_t a = r.psi_block_off + int64_t(r.t->psi_col) * r.ls_ket;
+                for (uint32_t i = 0; i < r.ls_ket; ++i)
+                    rows.push_back(GemmRow{a + i, int64_t(r.t->t_off) + i, int32_t(r.ls_ket), 0});
+                M += r.ls_ket;
+            }
+            require(M < INT32_MAX, "T-GEMM: stacked problem too tall");
+            if (!trans)
+                out.push_back(gemm(int32_t(M), int32_t(nb * rs_bra), rs_ket, psi_base, row0, 1, BASE_BOUNDARY,
+                                   int64_t(R.data_off[ci_eff]), rs_ket, BASE_T, 0, 1, GEMM_ROWMAP));
+            else
+                out.push_back(gemm(int32_t(M), rs_bra, rs_ket, psi_base, row0, 1, BASE_BOUNDARY,
+                                   int64_t(R.data_off[ci_eff]), rs_bra, BASE_T, 0, 1, GEMM_ROWMAP | GEMM_TRANS_B,
+                                   int32_t(nb), 0, int64_t(rs_bra) * rs_ket, int64_t(rs_bra)));
+        }
+    }
+};
 
 void validate_boundary_offsets(BoundaryIndex const& B) {
     for (uint32_t ci = 0; ci < B.n_cohorts(); ++ci) {
@@ -372,6 +403,7 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
         for (auto& c : v) cohorts.push_back(std::move(c));
 
     // --- T buffer layout (only tasks referenced by some alpha are executed) ---
+    RightTStacks tstacks;
     for (uint32_t lb_in = 0; lb_in < left_i.size(); ++lb_in) {
         uint32_t ls_in = left_i.sizes[lb_in];
         for (auto& t : tlists[lb_in].tasks) {
@@ -391,10 +423,11 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
             t.t_off = P.t_elems;
             P.t_elems += round_up(t.size, 32);
             r_used.insert(t.ci_eff);
-            right_t_gemm(R, t, ls_in, rs_ket, rs_bra, int64_t(P.mps.rp_off[lb_in]), BASE_PSI_IN, P.t_gemms);
+            tstacks.add(t, ls_in, rs_ket, rs_bra, int64_t(P.mps.rp_off[lb_in]));
             P.flops.flops_gemm_exec += 2ull * ls_in * csz;
         }
     }
+    tstacks.emit(R, BASE_PSI_IN, P.t_gemms, P.t_rows);
 
     // --- S panels / L panels per output block: cohorts of one lb_out stacked along K ---
     std::vector<uint64_t> ktot(left_i.size(), 0);
@@ -971,6 +1004,7 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
     P.out.complement_transpose(sym, mpo.left_herm, false);
     P.out_elems = P.out.layout();
 
+    RightTStacks tstacks;
     for (uint32_t lb_ket = 0; lb_ket < ket.left.size(); ++lb_ket) {
         uint32_t ls_ket = ket.left.sizes[lb_ket];
         for (auto& t : tlists[lb_ket].tasks) {
@@ -987,10 +1021,11 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
             require(ket.right.position(R.bra.charges[lr.first]) < ket.right.size() &&
                         ket.right.sizes[ket.right.position(R.bra.charges[lr.first])] == rs_ket,
                     "rstep: right boundary index does not match the ket right index");
-            right_t_gemm(R, t, ls_ket, rs_ket, rs_bra, int64_t(ket.rp_off[lb_ket]), BASE_KET, P.t_gemms);
+            tstacks.add(t, ls_ket, rs_ket, rs_bra, int64_t(ket.rp_off[lb_ket]));
             P.flops.flops_gemm_exec += flops;
         }
     }
+    tstacks.emit(R, BASE_KET, P.t_gemms, P.t_rows);
 
     // S (nS*ls_ket x stripe) and R'_b = S_b * B_bra^T per MPO row (prop_r, tasks.cpp:243-284)
     std::vector<uint64_t> sbase(cohorts.size());
@@ -1216,7 +1251,7 @@ GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t tar
         int bm = gemm_bm(sh), bn = gemm_bn(sh);
         int32_t tm = (g.M + bm - 1) / bm, tn = (g.N + bn - 1) / bn;
         int64_t splits = 1, step = g.K;
-        if (g.K > chunk) {
+        if (g.K > chunk && !(g.flags & GEMM_ROWMAP)) {  // stacked problems have no split path
             splits = (g.K + chunk - 1) / chunk;
             step = ((g.K + splits - 1) / splits + 15) / 16 * 16;
             splits = (g.K + step - 1) / step;
@@ -1245,7 +1280,7 @@ GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t tar
 
 // ---- host-side bounds validation of every device table (a table entry that would leave its
 //      buffer is rejected before anything is launched) -----------------------------------------
-void check_gemms(std::vector<GemmDesc> const& v, const uint64_t* sizes) {
+void check_gemms(std::vector<GemmDesc> const& v, const uint64_t* sizes, std::vector<GemmRow> const& rows) {
     auto span = [](int64_t off, int64_t stride, int32_t batch, int64_t rows, int64_t cols, int64_t ld) {
         // [lo, hi] accessed element offsets of a rows x cols column-major operand
         int64_t last = off + int64_t(batch - 1) * stride + (rows - 1) + (cols - 1) * ld;
@@ -1257,6 +1292,26 @@ void check_gemms(std::vector<GemmDesc> const& v, const uint64_t* sizes) {
         for (int b : {g.base_a, g.base_b, g.base_c}) require(b >= 0 && b < BASE_COUNT, "gemm table: bad base");
         if (g.M == 0 || g.N == 0) continue;
         bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B;
+        if (g.flags & GEMM_ROWMAP) {
+            require(!ta && g.sa == 0 && g.a >= 0 && uint64_t(g.a) + uint64_t(g.M) <= rows.size(),
+                    "gemm table: stacked problem outside its row table");
+            for (int64_t m = g.a; m < g.a + g.M; ++m) {
+                GemmRow const& r = rows[size_t(m)];
+                require(r.ld >= 1, "gemm table: row with ld < 1");
+                if (g.K > 0)
+                    require(r.a >= 0 && uint64_t(r.a + int64_t(g.K - 1) * r.ld) < sizes[g.base_a],
+                            "gemm table: A row outside its buffer");
+                require(r.c >= 0 && uint64_t(r.c + (int64_t(g.batch - 1) * g.sc + g.N - 1) * r.ld) < sizes[g.base_c],
+                        "gemm table: C row outside its buffer");
+            }
+            if (g.K > 0) {
+                int64_t br = tb ? g.N : g.K, bc = tb ? g.K : g.N;
+                require(g.ldb >= br, "gemm table: ldb too small");
+                auto sb = span(g.b, g.sb, g.batch, br, bc, g.ldb);
+                require(sb.first >= 0 && uint64_t(sb.second) < sizes[g.base_b], "gemm table: B outside its buffer");
+            }
+            continue;
+        }
         if (g.K > 0) {
             int64_t ar = ta ? g.K : g.M, ac = ta ? g.M : g.K;
             require(g.lda >= ar, "gemm table: lda too small");

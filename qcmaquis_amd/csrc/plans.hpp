@@ -50,6 +50,7 @@ using HostTable = std::vector<T, DefaultInitAlloc<T>>;
 enum : int32_t {
     GEMM_TRANS_A = 1,
     GEMM_TRANS_B = 2,
+    GEMM_ROWMAP = 4,  // stacked problem: rows from a row table (A not transposed, never split along K)
 };
 enum : int32_t {  // operand base selectors
     BASE_PSI_IN = 0, BASE_PSI_OUT, BASE_T, BASE_S, BASE_LPANEL, BASE_BOUNDARY, BASE_BRA, BASE_KET,
@@ -65,6 +66,17 @@ struct GemmDesc {
     int32_t pad;
 };
 static_assert(sizeof(GemmDesc) == 96, "GemmDesc layout");
+
+// Row of a stacked problem (GEMM_ROWMAP).  Several products that share their B operand (T-GEMMs
+// reading one right-boundary cohort) run as one problem whose M rows are the rows of their A
+// operands.  Row m of problem g is entry g.a + m of the row table: it reads A at element
+// a + k * ld of base_a and writes C at element c + (n + batch * g.sc) * ld of base_c (g.sc counts
+// columns for these problems; g.lda, g.ldc and g.sa are unused).
+struct GemmRow {
+    int64_t a, c;
+    int32_t ld, pad;
+};
+static_assert(sizeof(GemmRow) == 24, "GemmRow layout");
 
 // Block form of the MPO-term sum for the matrix cores.  Lines of equal tile shape are grouped in
 // blocks of SBLOCK_LINES; a block stores the sorted union of the T tiles its lines read and a dense
@@ -157,6 +169,7 @@ struct CopyJob {
 struct HpsiPlanHost {
     MPSLayout mps;
     std::vector<GemmDesc> t_gemms, c_gemms;
+    std::vector<GemmRow> t_rows;  // row table of the stacked T-GEMMs
     HostTable<SLine> lines;
     HostTable<STerm> terms;
     std::vector<CopyJob> lpanel_jobs;
@@ -171,6 +184,7 @@ struct GrowPlanHost {
     BoundaryIndex out;      // index of the new boundary (with data_off layout)
     uint64_t out_elems = 0;
     std::vector<GemmDesc> t_gemms, c_gemms;
+    std::vector<GemmRow> t_rows;  // row table of stacked T-GEMMs (right step)
     HostTable<SLine> lines;
     HostTable<STerm> terms;
     uint64_t t_elems = 0, s_elems = 0;
@@ -183,9 +197,11 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
 GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, MPSLayout const& bra,
                               MPSLayout const& ket, bool symmetric);
 
-// Host-side bounds validation of device tables: every operand span of every GEMM problem, S line
-// and T term must lie inside its buffer (sizes indexed by BASE_*); throws mq::Error otherwise.
-void check_gemms(std::vector<GemmDesc> const& v, const uint64_t* sizes);
+// Host-side bounds validation of device tables: every operand span of every GEMM problem (and
+// every row of a stacked one), S line and T term must lie inside its buffer (sizes indexed by
+// BASE_*); throws mq::Error otherwise.
+void check_gemms(std::vector<GemmDesc> const& v, const uint64_t* sizes,
+                 std::vector<GemmRow> const& rows = std::vector<GemmRow>());
 void check_lines(HostTable<SLine> const& lines, HostTable<STerm> const& terms, uint64_t t_size, uint64_t s_size);
 
 GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, MPSLayout const& bra,

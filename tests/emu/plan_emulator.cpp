@@ -19,7 +19,7 @@ namespace {
 // executes the stage as the device does: the tiles of the GEMM schedule, each over its K range,
 // writing C or a workspace partial, then the split-K reduce jobs.  The schedule is built with a
 // K chunk of 16 so that every reduction longer than one K slice goes through the split path.
-void run_gemms(std::vector<GemmDesc> const& G, double* const* base) {
+void run_gemms(std::vector<GemmDesc> const& G, std::vector<GemmRow> const& rows, double* const* base) {
     GemmSchedule sch = build_gemm_schedule(G, int64_t(1) << 40, 16);
     std::vector<double> ws(sch.ws_elems + 1, std::nan(""));
     std::vector<char> seen(sch.ws_elems + 1, 0);
@@ -27,21 +27,27 @@ void run_gemms(std::vector<GemmDesc> const& G, double* const* base) {
         const int bm = gemm_bm(sh), bn = gemm_bn(sh);
         for (GemmTile const& t : sch.tiles[sh]) {
             GemmDesc const& g = G[t.prob];
-            bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B;
-            const double* A = base[g.base_a] + g.a + int64_t(t.batch) * g.sa;
+            bool ta = g.flags & GEMM_TRANS_A, tb = g.flags & GEMM_TRANS_B, rmap = g.flags & GEMM_ROWMAP;
+            require(!rmap || t.ws < 0, "emulated gemm: stacked problem split along K");
+            const double* A = base[g.base_a] + (rmap ? 0 : g.a + int64_t(t.batch) * g.sa);
             const double* B = base[g.base_b] + g.b + int64_t(t.batch) * g.sb;
-            double* Cm = base[g.base_c] + g.c + int64_t(t.batch) * g.sc;
+            double* Cm = base[g.base_c] + (rmap ? 0 : g.c + int64_t(t.batch) * g.sc);
             for (int nn = 0; nn < bn; ++nn)
                 for (int mm = 0; mm < bm; ++mm) {
                     int m = t.tm * bm + mm, n = t.tn * bn + nn;
                     if (m >= g.M || n >= g.N) continue;
+                    GemmRow row{};
+                    if (rmap) row = rows.at(size_t(g.a + m));
                     double s = 0.0;
                     for (int k = t.k0; k < t.k1; ++k) {
-                        double a = ta ? A[int64_t(m) * g.lda + k] : A[m + int64_t(k) * g.lda];
+                        double a = rmap ? A[row.a + int64_t(k) * row.ld]
+                                        : ta ? A[int64_t(m) * g.lda + k] : A[m + int64_t(k) * g.lda];
                         double v = tb ? B[n + int64_t(k) * g.ldb] : B[k + int64_t(n) * g.ldb];
                         s += a * v;
                     }
-                    if (t.ws < 0) {
+                    if (rmap) {
+                        Cm[row.c + (n + int64_t(t.batch) * g.sc) * row.ld] = s;
+                    } else if (t.ws < 0) {
                         Cm[m + int64_t(n) * g.ldc] = s;
                     } else {
                         uint64_t w = uint64_t(t.ws) + mm + uint64_t(nn) * bm;
@@ -155,7 +161,7 @@ int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const dou
         sizes[BASE_S] = P.s_elems;
         sizes[BASE_LPANEL] = P.lpanel_elems;
         sizes[BASE_BOUNDARY] = Rn;
-        check_gemms(P.t_gemms, sizes);
+        check_gemms(P.t_gemms, sizes, P.t_rows);
         check_gemms(P.c_gemms, sizes);
         check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
         for (auto const& j : P.lpanel_jobs)
@@ -170,9 +176,9 @@ int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const dou
         base[BASE_S] = S.data();
         base[BASE_LPANEL] = Lp.data();
         base[BASE_BOUNDARY] = const_cast<double*>(Rdata);
-        run_gemms(P.t_gemms, base);
+        run_gemms(P.t_gemms, P.t_rows, base);
         run_lines(P.lines, P.terms, T.data(), S.data(), P.t_elems, P.s_elems);
-        run_gemms(P.c_gemms, base);
+        run_gemms(P.c_gemms, {}, base);
         return 0;
     } catch (std::exception const& e) {
         fprintf(stderr, "emu_hpsi: %s\n", e.what());
@@ -198,7 +204,7 @@ static EmuBoundary* grow(bool left, int sym, const mq_mpo_desc* md, const mq_bin
     sizes[BASE_BRA] = bra.elements();
     sizes[BASE_KET] = ket.elements();
     sizes[BASE_NEW] = P.out_elems;
-    check_gemms(P.t_gemms, sizes);
+    check_gemms(P.t_gemms, sizes, P.t_rows);
     check_gemms(P.c_gemms, sizes);
     check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
     auto* res = new EmuBoundary{std::move(P.out), std::vector<double>(P.out_elems, 0.0)};
@@ -210,9 +216,9 @@ static EmuBoundary* grow(bool left, int sym, const mq_mpo_desc* md, const mq_bin
     base[BASE_BRA] = bra_data.data();
     base[BASE_KET] = const_cast<double*>(ket_rp);
     base[BASE_NEW] = res->data.data();
-    run_gemms(P.t_gemms, base);
+    run_gemms(P.t_gemms, P.t_rows, base);
     run_lines(P.lines, P.terms, T.data(), S.data(), P.t_elems, P.s_elems);
-    run_gemms(P.c_gemms, base);
+    run_gemms(P.c_gemms, {}, base);
     return res;
 }
 
