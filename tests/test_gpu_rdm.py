@@ -1,0 +1,60 @@
+"""1-RDM and integral gradients of a converged DMRG state through device boundary growth.
+
+The seeded 6-orbital problem is swept untruncated (M = 400) to the FCI energy; then
+qcmaquis_amd.rdm evaluates derivative-MPO expectation values on the device and is compared with
+the determinant FCI state (tests/fci_rdm.py) and with central finite differences of the FCI
+energy (step 1e-5, 1e-6 tolerance, the check the differentiation configuration asks for).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from qcmaquis_amd.dmrg import TwoSiteDMRG
+from qcmaquis_amd.qc import QCHamiltonian, fci_energy, make_integrals
+from qcmaquis_amd.rdm import eri_gradient, expval, h_gradient, one_rdm, unit_eri, unit_h
+from tests.fci_rdm import fci_ground
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fci_golden.json")))
+
+
+def test_rdm_and_gradients_small6(ctx2u1):
+    g = GOLD["small6"]
+    L, nu, nd = g["L"], g["n_up"], g["n_dn"]
+    print("small6 rdm: sweeps", flush=True)
+    h, eri, ec = make_integrals(L, g["seed"], g["rank"])
+    d = TwoSiteDMRG(QCHamiltonian(h, eri, ec), nu, nd, M=400, ctx=ctx2u1, tol=1e-11, max_iter=20)
+    for _ in range(4):
+        e, _ = d.sweep()
+    e_fci, gam_fci, Gam_fci = fci_ground(h, eri, nu, nd, ec)
+    print(f"  E dmrg {e:.12f} fci {e_fci:.12f}", flush=True)
+    assert abs(e - e_fci) < 1e-9
+
+    print("small6 rdm: 1-RDM", flush=True)
+    gamma = one_rdm(ctx2u1, d.lay, d.data)
+    print(f"  trace {np.trace(gamma):.12f}, max |gamma - gamma_fci| {np.abs(gamma - gam_fci).max():.2e}", flush=True)
+    assert abs(np.trace(gamma) - (nu + nd)) < 1e-9
+    assert np.abs(gamma - gam_fci).max() < 1e-6
+    # energy from its parts: one-body part from gamma, two-body part as one expectation value
+    e2 = expval(ctx2u1, d.lay, d.data, QCHamiltonian(np.zeros((L, L)), eri))
+    assert abs(np.einsum("pq,pq", h, gamma) + e2 + ec - e) < 1e-8
+
+    print("small6 rdm: gradients vs finite differences", flush=True)
+    rng = np.random.default_rng(5)
+    eri_entries = [tuple(int(x) for x in rng.integers(0, L, 4)) for _ in range(6)]
+    h_entries = [tuple(int(x) for x in rng.integers(0, L, 2)) for _ in range(4)]
+    ge = eri_gradient(ctx2u1, d.lay, d.data, eri_entries)
+    gh = h_gradient(ctx2u1, d.lay, d.data, h_entries)
+    eps = 1e-5
+    for (p, q, r, s), v in zip(eri_entries, ge):
+        de = unit_eri(L, p, q, r, s)
+        fd = (fci_energy(h, eri + eps * de, nu, nd)[0] - fci_energy(h, eri - eps * de, nu, nd)[0]) / (2 * eps)
+        print(f"  d/d({p}{q}|{r}{s}) device {v:.10f} fd {fd:.10f} fci-rdm {0.5 * np.einsum('pqrs,pqrs', de, Gam_fci):.10f}",
+              flush=True)
+        assert abs(v - fd) < 1e-6
+    for (p, q), v in zip(h_entries, gh):
+        dh = unit_h(L, p, q)
+        fd = (fci_energy(h + eps * dh, eri, nu, nd)[0] - fci_energy(h - eps * dh, eri, nu, nd)[0]) / (2 * eps)
+        assert abs(v - fd) < 1e-6
