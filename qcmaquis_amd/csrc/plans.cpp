@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <map>
 #include <memory>
 #include <unordered_map>
@@ -112,61 +113,98 @@ inline void op_iterate(OpBlock const& blk, const double cpl[4], CohortAcc& coh, 
 
 // MPO rows of one unit that received no terms: emitted as lines without terms, so the term sum
 // writes their S tiles as zeros and S needs no separate clear
-template <class MakeLine>
-void add_empty_lines(CohortAcc::Unit const& u, uint32_t nrows, int32_t term_pos, std::vector<SLine>& out,
-                     MakeLine make_line) {
+template <class Out, class MakeLine>
+void add_empty_lines(CohortAcc::Unit const& u, uint32_t nrows, int32_t term_pos, Out& out, MakeLine make_line) {
     if (!u.ms) return;
+    auto empty = [&](uint32_t b) {
+        SLine line = make_line(b);
+        line.t0 = line.t1 = term_pos;
+        out.push_back(line);
+    };
+    if (std::is_sorted(u.line_b.begin(), u.line_b.end(), std::less_equal<uint32_t>())) {
+        // rows with terms strictly increasing (the builders add lines in MPO-row order)
+        require(u.line_b.empty() || u.line_b.back() < nrows, "S lines: MPO row outside its cohort");
+        size_t i = 0;
+        for (uint32_t b = 0; b < nrows; ++b) {
+            if (i < u.line_b.size() && u.line_b[i] == b) ++i;
+            else empty(b);
+        }
+        return;
+    }
     std::vector<char> has(nrows, 0);
     for (uint32_t b : u.line_b) {
         require(b < nrows, "S lines: MPO row outside its cohort");
         has[b] = 1;
     }
     for (uint32_t b = 0; b < nrows; ++b)
-        if (!has[b]) {
-            SLine line = make_line(b);
-            line.t0 = line.t1 = term_pos;
-            out.push_back(line);
-        }
+        if (!has[b]) empty(b);
 }
 
-// S lines and terms of all cohorts, built in parallel (emit(c, lines, terms) appends the lines of
-// cohort c with term ranges relative to its own term list) and concatenated in cohort order
-template <class Emit>
-void emit_lines(size_t n_cohorts, HostTable<SLine>& lines, HostTable<STerm>& terms, Emit emit) {
-    std::vector<std::vector<SLine>> cl(n_cohorts);
-    std::vector<std::vector<STerm>> ct(n_cohorts);
+// lines a unit emits: one per MPO row with terms, plus the empty lines of add_empty_lines
+size_t unit_lines(CohortAcc::Unit const& u, uint32_t nrows) {
+    if (!u.ms) return u.line_count.size();
+    if (std::is_sorted(u.line_b.begin(), u.line_b.end(), std::less_equal<uint32_t>()))
+        return u.line_count.size() + (nrows - std::min<size_t>(nrows, u.line_b.size()));
+    std::vector<char> has(nrows, 0);
+    size_t n = 0;
+    for (uint32_t b : u.line_b)
+        if (b < nrows && !has[b]) has[b] = 1, ++n;
+    return u.line_count.size() + (nrows - n);
+}
+
+// fixed-size output window of one cohort in the plan's line or term table; size() is the absolute
+// index of the next entry
+template <class T>
+struct TableSink {
+    T* p;
+    size_t n, cap, base;
+    void push_back(T const& v) {
+        if (n == cap) throw Error(MQ_ERR_INVALID, "S table: cohort emitted more entries than counted");
+        p[n++] = v;
+    }
+    size_t size() const { return base + n; }
+};
+
+// S lines and terms of all cohorts, written in parallel straight into the tables in cohort order:
+// the lines and terms of every cohort are counted first (units_of(c): its units, nrows_of(c): its
+// MPO rows), then emit(c, lines, terms) appends cohort c's lines (term ranges absolute) and terms
+template <class Units, class Rows, class Emit>
+void emit_lines(size_t n_cohorts, HostTable<SLine>& lines, HostTable<STerm>& terms, Units units_of, Rows nrows_of,
+                Emit emit) {
+    std::vector<size_t> nl(n_cohorts), nt(n_cohorts);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t c = 0; c < int64_t(n_cohorts); ++c) {
+        size_t a = 0, b = 0;
+        for (auto const& u : units_of(size_t(c))) {
+            a += unit_lines(u, nrows_of(size_t(c)));
+            b += u.ti.size();
+        }
+        nl[size_t(c)] = a;
+        nt[size_t(c)] = b;
+    }
+    std::vector<size_t> loff(n_cohorts + 1, lines.size()), toff(n_cohorts + 1, terms.size());
+    for (size_t c = 0; c < n_cohorts; ++c) {
+        loff[c + 1] = loff[c] + nl[c];
+        toff[c + 1] = toff[c] + nt[c];
+    }
+    require(toff[n_cohorts] < size_t(INT32_MAX), "S table: more than 2^31 terms in one step");
+    lines.resize(loff[n_cohorts]);
+    terms.resize(toff[n_cohorts]);
     std::string err;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t c = 0; c < int64_t(n_cohorts); ++c) {
         try {
-            emit(size_t(c), cl[size_t(c)], ct[size_t(c)]);
+            TableSink<SLine> ls{lines.data() + loff[size_t(c)], 0, nl[size_t(c)], loff[size_t(c)]};
+            TableSink<STerm> ts{terms.data() + toff[size_t(c)], 0, nt[size_t(c)], toff[size_t(c)]};
+            emit(size_t(c), ls, ts);
+            if (ls.n != ls.cap || ts.n != ts.cap)
+                throw Error(MQ_ERR_INVALID, "S table: cohort emitted fewer entries than counted");
         } catch (std::exception const& e) {
 #pragma omp critical
             err = e.what();
         }
     }
     if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
-    std::vector<size_t> loff(n_cohorts + 1, lines.size()), toff(n_cohorts + 1, terms.size());
-    for (size_t c = 0; c < n_cohorts; ++c) {
-        loff[c + 1] = loff[c] + cl[c].size();
-        toff[c + 1] = toff[c] + ct[c].size();
-    }
-    require(toff[n_cohorts] < size_t(INT32_MAX), "S table: more than 2^31 terms in one step");
-    lines.resize(loff[n_cohorts]);
-    terms.resize(toff[n_cohorts]);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int64_t c = 0; c < int64_t(n_cohorts); ++c) {
-        const int32_t base = int32_t(toff[size_t(c)]);
-        SLine* dst = lines.data() + loff[size_t(c)];
-        for (SLine l : cl[size_t(c)]) {
-            l.t0 += base;
-            l.t1 += base;
-            *dst++ = l;
-        }
-        std::copy(ct[size_t(c)].begin(), ct[size_t(c)].end(), terms.begin() + int64_t(toff[size_t(c)]));
-        std::vector<SLine>().swap(cl[size_t(c)]);
-        std::vector<STerm>().swap(ct[size_t(c)]);
-    }
 }
 
 inline uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
@@ -475,7 +513,8 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
         P.flops.n_terms += rec.acc.n_terms;
     }
     // S lines
-    emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c, std::vector<SLine>& lines, std::vector<STerm>& terms) {
+    emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c) -> auto const& { return cohorts[c].acc.units; },
+               [&](size_t c) { return cohorts[c].nb; }, [&](size_t c, auto& lines, auto& terms) {
         auto const& rec = cohorts[c];
         uint32_t ls_in = left_i.sizes[rec.lb_in];
         uint64_t ldS = ktot[rec.lb_out];
@@ -771,7 +810,8 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
         P.flops.flops_gemm_exec += g;
         P.flops.n_terms += rec.acc.n_terms;
     }
-    emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c, std::vector<SLine>& lines, std::vector<STerm>& terms) {
+    emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c) -> auto const& { return cohorts[c].acc.units; },
+               [&](size_t c) { return cohorts[c].nrows; }, [&](size_t c, auto& lines, auto& terms) {
         auto const& rec = cohorts[c];
         uint32_t rs_ket = ket.right.sizes[rec.rb_ket];
         uint64_t stripe = rec.acc.stripe;
@@ -1049,7 +1089,8 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
         P.flops.flops_gemm_exec += g;
         P.flops.n_terms += rec.acc.n_terms;
     }
-    emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c, std::vector<SLine>& lines, std::vector<STerm>& terms) {
+    emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c) -> auto const& { return cohorts[c].acc.units; },
+               [&](size_t c) { return cohorts[c].nrows; }, [&](size_t c, auto& lines, auto& terms) {
         auto const& rec = cohorts[c];
         uint32_t ls_ket = ket.left.sizes[rec.lb_ket];
         uint64_t lds = uint64_t(rec.nrows) * ls_ket;
@@ -1123,30 +1164,39 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
     std::vector<std::vector<double>> alphas(spans.size());
 #pragma omp parallel
     {
-        // (T offset, line in block, term) of one block, sorted by offset then term: the union is the
-        // distinct offsets and each coefficient is summed in the line's term order
-        struct Ref { int64_t t_off; uint32_t q; int32_t k; };
-        std::vector<Ref> refs;
+        // union of one block's T tiles in order of first use (lines in block order, each line's terms
+        // in term order), found with an open-addressing table; each coefficient is summed in the
+        // line's term order
+        std::vector<int64_t> key;
+        std::vector<int32_t> slot;
 #pragma omp for schedule(dynamic, 64)
         for (int64_t bi = 0; bi < nb; ++bi) {
             Span const& sp = spans[size_t(bi)];
-            refs.clear();
+            size_t nref = 0;
             for (uint32_t q = 0; q < sp.n; ++q) {
                 SLine const& l = lines[(*sp.v)[sp.s0 + q]];
-                for (int32_t k = l.t0; k < l.t1; ++k) refs.push_back(Ref{terms[size_t(k)].t_off, q, k});
+                nref += size_t(l.t1 - l.t0);
             }
-            std::sort(refs.begin(), refs.end(), [](Ref const& a, Ref const& b) {
-                return a.t_off != b.t_off ? a.t_off < b.t_off : a.k < b.k;
-            });
+            size_t cap = 16;
+            while (cap < 2 * nref) cap *= 2;
+            key.assign(cap, -1);
+            slot.resize(cap);
             std::vector<int64_t>& u = unions[size_t(bi)];
-            for (size_t r = 0; r < refs.size(); ++r)
-                if (r == 0 || refs[r].t_off != refs[r - 1].t_off) u.push_back(refs[r].t_off);
             std::vector<double>& al = alphas[size_t(bi)];
-            al.assign(size_t(SBLOCK_LINES) * u.size(), 0.0);
-            size_t pos = 0;
-            for (size_t r = 0; r < refs.size(); ++r) {
-                if (r > 0 && refs[r].t_off != refs[r - 1].t_off) ++pos;
-                al[pos * SBLOCK_LINES + refs[r].q] += terms[size_t(refs[r].k)].alpha;
+            for (uint32_t q = 0; q < sp.n; ++q) {
+                SLine const& l = lines[(*sp.v)[sp.s0 + q]];
+                for (int32_t k = l.t0; k < l.t1; ++k) {
+                    const int64_t t = terms[size_t(k)].t_off;
+                    size_t h = size_t((uint64_t(t) * 0x9E3779B97F4A7C15ull) >> 20) & (cap - 1);
+                    while (key[h] != -1 && key[h] != t) h = (h + 1) & (cap - 1);
+                    if (key[h] == -1) {
+                        key[h] = t;
+                        slot[h] = int32_t(u.size());
+                        u.push_back(t);
+                        al.resize(al.size() + SBLOCK_LINES, 0.0);
+                    }
+                    al[size_t(slot[h]) * SBLOCK_LINES + q] += terms[size_t(k)].alpha;
+                }
             }
         }
     }
