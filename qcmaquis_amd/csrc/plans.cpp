@@ -349,37 +349,49 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
         uint32_t lb_in, lb_out, ci, ci_eff, nb;
         CohortAcc acc;
     };
-    std::vector<std::vector<CohRec>> coh_by_in(NL);
     std::vector<TList> tlists(NL);
     std::unordered_set<uint32_t> l_used, r_used;
     std::string err;
 
+    // T work lists (rshtm_t_tasks), one per input block
 #pragma omp parallel for schedule(dynamic, 1)
     for (uint32_t lb_in = 0; lb_in < NL; ++lb_in) {
         try {
-        Charge lc_in = left_i.charges[lb_in];
-        uint32_t ls_in = left_i.sizes[lb_in];
-        TList& tl = tlists[lb_in];
-        // T work list (rshtm_t_tasks)
-        for (size_t s = 0; s < NP; ++s) {
-            size_t rb_ket = pos_lp[lb_in * NP + s];
-            if (rb_ket == NR) continue;
-            uint32_t rs_ket = right_i.sizes[rb_ket];
-            uint64_t ket_offset = pb_off[s * NR + rb_ket];
-            for (size_t rb_bra = 0; rb_bra < NR; ++rb_bra) {
-                uint32_t ci = r_ci[rb_ket * NR + rb_bra];
-                if (ci == R.n_cohorts()) continue;
-                uint32_t ci_eff = R.tr[ci] ? r_ci[rb_bra * NR + rb_ket] : ci;
-                require(ci_eff < R.n_cohorts(), "hpsi: transposed right cohort without partner");
-                for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) tl.add(ket_offset + ss * rs_ket, ci, ci_eff, lb_in);
+            TList& tl = tlists[lb_in];
+            for (size_t s = 0; s < NP; ++s) {
+                size_t rb_ket = pos_lp[lb_in * NP + s];
+                if (rb_ket == NR) continue;
+                uint32_t rs_ket = right_i.sizes[rb_ket];
+                uint64_t ket_offset = pb_off[s * NR + rb_ket];
+                for (size_t rb_bra = 0; rb_bra < NR; ++rb_bra) {
+                    uint32_t ci = r_ci[rb_ket * NR + rb_bra];
+                    if (ci == R.n_cohorts()) continue;
+                    uint32_t ci_eff = R.tr[ci] ? r_ci[rb_bra * NR + rb_ket] : ci;
+                    require(ci_eff < R.n_cohorts(), "hpsi: transposed right cohort without partner");
+                    for (uint32_t ss = 0; ss < phys_i.sizes[s]; ++ss) tl.add(ket_offset + ss * rs_ket, ci, ci_eff, lb_in);
+                }
             }
+        } catch (std::exception const& e) {
+#pragma omp critical
+            err = e.what();
         }
-        // cohorts (shtm_tasks)
-        for (uint32_t lb_out = 0; lb_out < NL; ++lb_out) {
+    }
+    if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
+
+    // cohorts (shtm_tasks), one per (lb_in, lb_out) pair with a left cohort, built in parallel; the
+    // T lists are only read (a task's used flag is set atomically)
+    std::vector<std::unique_ptr<CohRec>> recs(size_t(NL) * NL);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t pr = 0; pr < int64_t(NL) * int64_t(NL); ++pr) {
+        const uint32_t lb_in = uint32_t(pr / int64_t(NL)), lb_out = uint32_t(pr % int64_t(NL));
+        try {
+            Charge lc_in = left_i.charges[lb_in];
+            uint32_t ls_in = left_i.sizes[lb_in];
+            TList& tl = tlists[lb_in];
             Charge lc_out = left_i.charges[lb_out];
             uint32_t ls_out = left_i.sizes[lb_out];
             uint32_t ci = L.cohort_index(lc_out, lc_in);
-            if (ci == L.n_cohorts()) continue;
+            if (ci == L.n_cohorts()) continue;  // no left cohort for this pair
             uint32_t ci_eff = L.tr[ci] ? L.cohort_index(lc_in, lc_out) : ci;
             require(ci_eff < L.n_cohorts(), "hpsi: transposed left cohort without partner");
             CohRec rec{lb_in, lb_out, ci, ci_eff, L.nblk[ci_eff], CohortAcc(phys_i, 0)};
@@ -428,8 +440,7 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                     coh.add_line(left_idx);
                 }
             }
-            if (coh.n_terms) coh_by_in[lb_in].push_back(std::move(rec));
-        }
+            if (coh.n_terms) recs[size_t(pr)].reset(new CohRec(std::move(rec)));
         } catch (std::exception const& e) {
 #pragma omp critical
             err = e.what();
@@ -437,8 +448,8 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
     }
     if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
     std::vector<CohRec> cohorts;
-    for (auto& v : coh_by_in)
-        for (auto& c : v) cohorts.push_back(std::move(c));
+    for (auto& r : recs)
+        if (r) cohorts.push_back(std::move(*r));
 
     // --- T buffer layout (only tasks referenced by some alpha are executed) ---
     RightTStacks tstacks;
