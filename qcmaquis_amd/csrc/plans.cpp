@@ -1175,11 +1175,13 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
     std::vector<std::vector<double>> alphas(spans.size());
 #pragma omp parallel
     {
-        // union of one block's T tiles in order of first use (lines in block order, each line's terms
-        // in term order), found with an open-addressing table; each coefficient is summed in the
-        // line's term order
+        // union of one block's T tiles, found with an open-addressing table in order of first use
+        // (lines in block order, each line's terms in term order), then sorted by T offset; each
+        // coefficient is summed in the line's term order
         std::vector<int64_t> key;
         std::vector<int32_t> slot;
+        std::vector<std::pair<int64_t, int32_t>> order;
+        std::vector<double> sal;
 #pragma omp for schedule(dynamic, 64)
         for (int64_t bi = 0; bi < nb; ++bi) {
             Span const& sp = spans[size_t(bi)];
@@ -1209,6 +1211,18 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
                     al[size_t(slot[h]) * SBLOCK_LINES + q] += terms[size_t(k)].alpha;
                 }
             }
+            // union tiles in T order (the term-sum kernel then reads T in ascending address order)
+            if (std::is_sorted(u.begin(), u.end())) continue;
+            order.resize(u.size());
+            for (size_t i = 0; i < u.size(); ++i) order[i] = std::make_pair(u[i], int32_t(i));
+            std::sort(order.begin(), order.end());
+            sal.resize(al.size());
+            for (size_t i = 0; i < u.size(); ++i) {
+                u[i] = order[i].first;
+                std::copy_n(al.begin() + int64_t(order[i].second) * SBLOCK_LINES, SBLOCK_LINES,
+                            sal.begin() + int64_t(i) * SBLOCK_LINES);
+            }
+            al.swap(sal);
         }
     }
     // concatenate in block order

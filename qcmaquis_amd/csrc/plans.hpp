@@ -79,8 +79,8 @@ struct GemmRow {
 static_assert(sizeof(GemmRow) == 24, "GemmRow layout");
 
 // Block form of the MPO-term sum for the matrix cores.  Lines of equal tile shape are grouped in
-// blocks of SBLOCK_LINES; a block stores the union of the T tiles its lines read (in order of first
-// use) and a dense SBLOCK_LINES x nu coefficient matrix (tile-major: alpha of line i for union
+// blocks of SBLOCK_LINES; a block stores the sorted union of the T tiles its lines read and a
+// dense SBLOCK_LINES x nu coefficient matrix (tile-major: alpha of line i for union
 // tile k at ualpha[16 (u_off + k) + i], zero where line i does not read tile k, summed where it
 // reads it twice), so the block's S tiles are Y = alpha * X with X the nu x E matrix of the tiles'
 // elements.
