@@ -361,15 +361,22 @@ struct BoundaryIndex {
                     tr[ciB] = 1;
                     fresh = true;
                 }
+                // herm.skip(b, l, r) = l < r and b has a partner: nothing to fill unless l < r
+                if (!(ket.charges[rb] < bra.charges[lb])) continue;
+                const double cc = conjugate_correction(sym, ket.charges[rb], bra.charges[lb]);
+                const bool target_ok = fresh || tr[ciB];
+                const int64_t* offA = offsets[ciA].data();
+                int64_t* offB = offsets[ciB].data();
+                double* csB = conj_scales[ciB].data();
+                uint8_t* tpB = transposes[ciB].data();
                 for (uint32_t b = 0; b < herm.size(); ++b) {
-                    if (herm.skip(b, ket.charges[rb], bra.charges[lb])) {
-                        require(offsets[ciB][b] == -1, "complement_transpose: block already present");
-                        require(fresh || tr[ciB], "complement_transpose: cohort is not b-index complete");
-                        offsets[ciB][b] = offsets[ciA][herm.conj(b)];
-                        conj_scales[ciB][b] = conjugate_correction(sym, ket.charges[rb], bra.charges[lb]) *
-                                              double(herm.ph(forward ? b : herm.conj(b)));
-                        transposes[ciB][b] = 1;
-                    }
+                    const uint32_t pb = herm.herm[b];
+                    if (pb == MQ_NO_PARTNER) continue;  // herm.skip(b, l, r) with l < r
+                    require(offB[b] == -1, "complement_transpose: block already present");
+                    require(target_ok, "complement_transpose: cohort is not b-index complete");
+                    offB[b] = offA[pb];
+                    csB[b] = cc * double(herm.phase[forward ? b : pb]);
+                    tpB[b] = 1;
                 }
             }
         // as in the reference, n_blocks of a transposed cohort stays 0: consumers always size
