@@ -206,10 +206,13 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __rest
 // elements of one block of 16 lines; each of the four waves owns 16 * SB_NC of them and computes
 // Y(16 lines x 16 elements) += alpha(16 x 4 tiles) * X(4 tiles x 16 elements) per matrix-core step
 // over the block's union of T tiles.  Lane l loads alpha of line l % 16 and union tile l / 16, and
-// element l % 16 of that tile: 16 consecutive values per tile.
+// element l % 16 of that tile: 16 consecutive values per tile.  The union's tile offsets are
+// staged in LDS, SB_UT_CHUNK at a time, so the tile loads do not wait on a global offset load.
 constexpr int SB_NC = SBLOCK_ELEMS / 64;
 constexpr int SB_UNROLL = 4;
+constexpr int SB_UT_CHUNK = 256;
 static_assert(SBLOCK_LINES == 16 && SB_NC * 64 == SBLOCK_ELEMS, "S block geometry");
+static_assert(SB_UT_CHUNK % (4 * SB_UNROLL) == 0, "offset chunks hold whole unrolled k-steps");
 
 template <class T>
 __global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ blocks, const SWork* __restrict__ work,
@@ -218,38 +221,47 @@ __global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ 
                                                      const int32_t* __restrict__ llds, const T* __restrict__ X,
                                                      T* __restrict__ S) {
     typedef typename Mma<T>::acc_t acc_t;
+    __shared__ int64_t uts[SB_UT_CHUNK];
     const SWork w = work[blockIdx.x];
     const SBlock b = blocks[w.blk];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n0 = w.n0 + wave * 16 * SB_NC;
-    if (n0 >= b.E) return;  // no barriers below: waves past the tile end leave early
+    // a wave past the tile end loads and stores nothing but takes part in the barriers
+    const bool active = n0 < b.E;
     const int li = lane & 15, lk = lane >> 4;
     acc_t acc[SB_NC];
 #pragma unroll
     for (int j = 0; j < SB_NC; ++j) acc[j] = acc_t{0, 0, 0, 0};
     const int64_t* ut = utile + b.u_off;
     const T* ua = ualpha + b.u_off * SBLOCK_LINES;
-    // SB_UNROLL k-steps of loads are issued before their matrix-core steps (more loads in flight
-    // per wave: the kernel is bound by the latency of the dependent offset -> tile loads)
-    for (int ks = 0; ks < b.nu; ks += 4 * SB_UNROLL) {
-        T a[SB_UNROLL], x[SB_UNROLL][SB_NC];
+    for (int c0 = 0; c0 < b.nu; c0 += SB_UT_CHUNK) {
+        const int cn = b.nu - c0 < SB_UT_CHUNK ? b.nu - c0 : SB_UT_CHUNK;
+        __syncthreads();  // the previous chunk's offsets are consumed
+        for (int i = threadIdx.x; i < cn; i += blockDim.x) uts[i] = ut[c0 + i];
+        __syncthreads();
+        if (!active) continue;
+        // SB_UNROLL k-steps of loads are issued before their matrix-core steps
+        for (int ks = 0; ks < cn; ks += 4 * SB_UNROLL) {
+            T a[SB_UNROLL], x[SB_UNROLL][SB_NC];
 #pragma unroll
-        for (int u = 0; u < SB_UNROLL; ++u) {
-            const int k = ks + 4 * u + lk;
-            const bool kv = k < b.nu;
-            a[u] = kv ? ua[k * SBLOCK_LINES + li] : T(0);  // A[line li][tile k]
-            const int64_t to = kv ? ut[k] : 0;
+            for (int u = 0; u < SB_UNROLL; ++u) {
+                const int k = ks + 4 * u + lk;
+                const bool kv = k < cn;
+                a[u] = kv ? ua[int64_t(c0 + k) * SBLOCK_LINES + li] : T(0);  // A[line li][tile k]
+                const int64_t to = kv ? uts[k] : 0;
 #pragma unroll
-            for (int j = 0; j < SB_NC; ++j) {
-                const int e = n0 + 16 * j + li;
-                x[u][j] = (kv && e < b.E) ? X[to + e] : T(0);  // B[tile k][element e]
+                for (int j = 0; j < SB_NC; ++j) {
+                    const int e = n0 + 16 * j + li;
+                    x[u][j] = (kv && e < b.E) ? X[to + e] : T(0);  // B[tile k][element e]
+                }
             }
+#pragma unroll
+            for (int u = 0; u < SB_UNROLL; ++u)
+#pragma unroll
+                for (int j = 0; j < SB_NC; ++j) acc[j] = Mma<T>::step(a[u], x[u][j], acc[j]);
         }
-#pragma unroll
-        for (int u = 0; u < SB_UNROLL; ++u)
-#pragma unroll
-            for (int j = 0; j < SB_NC; ++j) acc[j] = Mma<T>::step(a[u], x[u][j], acc[j]);
     }
+    if (!active) return;
     // register r of lane l holds Y[line row(l, r)][element l % 16]
 #pragma unroll
     for (int j = 0; j < SB_NC; ++j) {
