@@ -206,11 +206,12 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __rest
 // elements of one block of 16 lines; each of the four waves owns 16 * SB_NC of them and computes
 // Y(16 lines x 16 elements) += alpha(16 x 4 tiles) * X(4 tiles x 16 elements) per matrix-core step
 // over the block's union of T tiles.  Lane l loads alpha of line l % 16 and union tile l / 16, and
-// element l % 16 of that tile: 16 consecutive values per tile.  The union's tile offsets are
-// staged in LDS, SB_UT_CHUNK at a time, so the tile loads do not wait on a global offset load.
+// element l % 16 of that tile: 16 consecutive values per tile.  The union's tile offsets and
+// coefficients are staged in LDS, SB_UT_CHUNK tiles at a time: the tile loads do not wait on a
+// global offset load, and the four waves read the coefficients from LDS, not four times from L2.
 constexpr int SB_NC = SBLOCK_ELEMS / 64;
 constexpr int SB_UNROLL = 4;
-constexpr int SB_UT_CHUNK = 256;
+constexpr int SB_UT_CHUNK = 64;
 static_assert(SBLOCK_LINES == 16 && SB_NC * 64 == SBLOCK_ELEMS, "S block geometry");
 static_assert(SB_UT_CHUNK % (4 * SB_UNROLL) == 0, "offset chunks hold whole unrolled k-steps");
 
@@ -222,6 +223,7 @@ __global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ 
                                                      T* __restrict__ S) {
     typedef typename Mma<T>::acc_t acc_t;
     __shared__ int64_t uts[SB_UT_CHUNK];
+    __shared__ T uas[SB_UT_CHUNK * SBLOCK_LINES];
     const SWork w = work[blockIdx.x];
     const SBlock b = blocks[w.blk];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -238,6 +240,7 @@ __global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ 
         const int cn = b.nu - c0 < SB_UT_CHUNK ? b.nu - c0 : SB_UT_CHUNK;
         __syncthreads();  // the previous chunk's offsets are consumed
         for (int i = threadIdx.x; i < cn; i += blockDim.x) uts[i] = ut[c0 + i];
+        for (int i = threadIdx.x; i < cn * SBLOCK_LINES; i += blockDim.x) uas[i] = ua[int64_t(c0) * SBLOCK_LINES + i];
         __syncthreads();
         if (!active) continue;
         // SB_UNROLL k-steps of loads are issued before their matrix-core steps
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ 
             for (int u = 0; u < SB_UNROLL; ++u) {
                 const int k = ks + 4 * u + lk;
                 const bool kv = k < cn;
-                a[u] = kv ? ua[int64_t(c0 + k) * SBLOCK_LINES + li] : T(0);  // A[line li][tile k]
+                a[u] = kv ? uas[k * SBLOCK_LINES + li] : T(0);  // A[line li][tile k]
                 const int64_t to = kv ? uts[k] : 0;
 #pragma unroll
                 for (int j = 0; j < SB_NC; ++j) {
