@@ -69,13 +69,19 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
 
     const TI* A = static_cast<const TI*>(bases.p[g.base_a]) + (rmap ? 0 : g.a + int64_t(tile.batch) * g.sa);
     const TI* B = static_cast<const TI*>(bases.p[g.base_b]) + g.b + int64_t(tile.batch) * g.sb;
-    // stacked problem: A is not transposed, so each thread loads from one row (tid % BM) of the tile
+    // stacked problem: the tile's rows are staged in LDS once (the A loads and the C stores look them
+    // up there); A is not transposed, so each thread loads from one row (tid % BM) of the tile
+    __shared__ GemmRow Rs[BM];
     const TI* a_row = A;
     int64_t a_rld = 0;
-    if (rmap && m0 + tid % BM < g.M) {
-        const GemmRow r = rows[g.a + m0 + tid % BM];
-        a_row = A + r.a;
-        a_rld = r.ld;
+    if (rmap) {
+        for (int i = tid; i < BM; i += 256)
+            if (m0 + i < g.M) Rs[i] = rows[g.a + m0 + i];
+        __syncthreads();
+        if (m0 + tid % BM < g.M) {
+            a_row = A + Rs[tid % BM].a;
+            a_rld = Rs[tid % BM].ld;
+        }
     }
 
     // per-thread load coordinates inside the BM x 16 A slice and the 16 x BN B slice, with the
@@ -156,7 +162,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
         TO* c_row = C + m;
         int64_t c_ld = g.ldc;
         if (rmap) {
-            const GemmRow r = rows[g.a + m];
+            const GemmRow r = Rs[mm];
             c_row = C + r.c + int64_t(tile.batch) * g.sc * r.ld;
             c_ld = r.ld;
         }
