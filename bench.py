@@ -138,27 +138,40 @@ def sweep_compare(ctx, L=24, M=512, seed=1002, n_sweeps=4):
     for s in range(n_sweeps):
         if s == n_sweeps - 1:
             d.compare_precision = []
+            d.compare_plain = True
+            ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 1)
         t0 = dict(d.timing)
         e, dt = d.sweep()
         energies.append(e)
+    ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
     cmp = d.compare_precision
     dav64 = sum(c["seconds"] for c in cmp)
     dav32 = sum(c["seconds_other"] for c in cmp)
+    davplain = sum(c["seconds_plain"] for c in cmp)
+    # the plain tables the comparison needs are not part of the sweep either
+    dt -= d.timing.get("plan_plain", 0.0) - t0.get("plan_plain", 0.0)
     de = max(abs(c["e_other"] - c["e"]) for c in cmp)
     de_tight = max(abs(c["e_tight_mixed"] - c["e_tight_fp64"]) for c in cmp)
     worst = max(cmp, key=lambda c: abs(c["e_tight_mixed"] - c["e_tight_fp64"]))
     dt -= sum(c["seconds_extra"] for c in cmp)  # the comparison solves are not part of the sweep
     out = {"fp64_sweep_energies": energies, "sweep_seconds_fp64": round(dt, 3),
            "sweep_seconds_mixed": round(dt - dav64 + dav32, 3),
+           "sweep_seconds_fp64_plain": round(dt - dav64 + davplain, 3),
            "davidson_seconds_fp64": round(dav64, 3), "davidson_seconds_mixed": round(dav32, 3),
+           "davidson_seconds_fp64_plain": round(davplain, 3),
+           "max_abs_delta_e_plain_vs_fp64": max(abs(c["e_plain"] - c["e"]) for c in cmp),
            "max_abs_delta_e_converged": de_tight, "max_abs_delta_e_8_iterations": de, "worst_site": worst,
            "timing_s": {k: round(v - t0.get(k, 0.0), 3) for k, v in d.timing.items()},
            "note": (f"{L} orbitals, M={M}: sweep {n_sweeps - 1} of {n_sweeps} FP64 sweeps from a seeded random MPS; "
                     "each of its 46 two-site problems also solved with the mixed Davidson in the same "
                     "8-iteration budget (sweep time) and, for the energy difference, with both Davidsons "
-                    "converged to a residual of 1e-10 |E| (at most 40 iterations)")}
-    log("sweep comparison: |dE| converged %.3e (8 iterations: %.3e), sweep %.2f s (fp64) / %.2f s (mixed), "
-        "davidson %.2f / %.2f s" % (de_tight, de, out["sweep_seconds_fp64"], out["sweep_seconds_mixed"], dav64, dav32))
+                    "converged to a residual of 1e-10 |E| (at most 40 iterations); fp64_plain = the same FP64 "
+                    "Davidson with the reference's launch structure (one library dgemm per T task and per "
+                    "cohort, one gather per term)")}
+    log("sweep comparison: |dE| converged %.3e (8 iterations: %.3e), sweep %.2f s (fp64) / %.2f s (mixed) / "
+        "%.2f s (fp64 plain), davidson %.2f / %.2f / %.2f s" % (
+            de_tight, de, out["sweep_seconds_fp64"], out["sweep_seconds_mixed"], out["sweep_seconds_fp64_plain"],
+            dav64, dav32, davplain))
     return out
 
 
@@ -222,7 +235,9 @@ def main():
     t0 = time.perf_counter()
     w = qc_workload(ctx, warm_sweeps=args.warm_sweeps, dump_bonds=args.dump_bonds) if args.workload == "qc" else synthetic_workload(ctx)
     log("workload built in %.1f s: %s" % (time.perf_counter() - t0, w["desc"]))
+    ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 1)  # the plan can also run the reference-style launches
     plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
+    ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
     st = plan.stats()
     log("plan:", {k: st[k] for k in ("n_t_tasks", "n_cohorts", "n_lines", "n_terms", "flops_total", "flops_gemm",
                                       "flops_gemm_exec", "bytes_compulsory", "t_elements", "s_elements",
@@ -291,6 +306,26 @@ def main():
         graph_ms[name] = round((time.perf_counter() - t0) / args.steps * 1e3, 4)
     plan.use_graphs(False)
     graph_ok = bool(torch.equal(outg, out32))  # last graph call was FP32 on the same psi
+    # FP64 with the reference's launch structure: one library dgemm per T task and per output
+    # block, one gather per term (MQ_PREC_FP64_PLAIN), on the same plan and psi
+    outp = torch.empty_like(psi)
+    plan.apply_device(psi.data_ptr(), outp.data_ptr(), abi.MQ_PREC_FP64_PLAIN)
+    ctx.sync()
+    nplain = max(2, min(args.steps, 5))
+    stagep = np.zeros(4)
+    t0 = time.perf_counter()
+    for _ in range(nplain):
+        plan.apply_device(psi.data_ptr(), outp.data_ptr(), abi.MQ_PREC_FP64_PLAIN)
+    ctx.sync()
+    msp = (time.perf_counter() - t0) / nplain * 1e3
+    for _ in range(nplain):
+        ctx.check(ctx.lib.mq_hpsi_timed(plan.h, C.c_void_p(psi.data_ptr()), C.c_void_p(outp.data_ptr()),
+                                        abi.MQ_PREC_FP64_PLAIN, sms))
+        stagep += np.array(list(sms))
+    stagep /= nplain
+    errp = float((outp - ref).norm() / ref.norm())
+    log("FP64 plain H*psi: %.3f ms, stages %s, relative difference vs grouped FP64 %.2e" % (
+        msp, np.round(stagep, 4).tolist(), errp))
     log("graph H*psi ms: %s (FP32 graph output equals eager: %s)" % (graph_ms, graph_ok))
     log("FP32 H*psi: %.3f ms, stages %s, relative error vs FP64 %.2e" % (ms32, np.round(stage32, 4).tolist(), err32))
     ms = wall / args.steps * 1e3
@@ -349,7 +384,17 @@ def main():
                        mixed_fp32={"ms_per_step": round(ms32, 4),
                                    "tflops": round(st["flops_total"] / (ms32 * 1e-3) / 1e12, 4),
                                    "speedup_vs_fp64": round(ms / ms32, 3), "rel_error_vs_fp64": err32,
-                                   "stage_ms": [round(x, 4) for x in stage32]}),
+                                   "stage_ms": [round(x, 4) for x in stage32]},
+                       fp64_plain={"ms_per_step": round(msp, 4),
+                                   "tflops": round(st["flops_total"] / (msp * 1e-3) / 1e12, 4),
+                                   "grouped_fp64_speedup": round(msp / ms, 3),
+                                   "mixed_grouped_speedup": round(msp / ms32, 3),
+                                   "rel_difference_vs_grouped_fp64": errp,
+                                   "stage_ms": [round(x, 4) for x in stagep],
+                                   "launches": "one rocBLAS dgemm per executed T task, one S-assembly "
+                                               "workgroup per line (%d) gathering each term, one rocBLAS dgemm "
+                                               "per cohort (%d) added into its output block" % (
+                                                   st["n_lines"], st["n_cohorts"])}),
         "roofline": dominant,
         "roofline_other": other,
     }

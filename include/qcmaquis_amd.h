@@ -45,6 +45,16 @@ extern "C" {
 #define MQ_PREC_FP64   0
 #define MQ_PREC_FP32   1
 #define MQ_PREC_BF16X3 2
+/* FP64 with the reference's launch structure, for comparison with the grouped path: one library
+ * (rocBLAS) dgemm per T task, one S-assembly workgroup per S tile that gathers every term, and
+ * one dgemm per cohort added into its output block (reference: MPSBlock::create_T_gpu,
+ * Cohort::create_s_r_gpu / compute_s_stackedv, Cohort::contract_gpu; tasks.cpp:286-344, 466-504,
+ * 650-752).  Needs a plan made while the context option MQ_OPT_PLAIN_TABLES is set. */
+#define MQ_PREC_FP64_PLAIN 3
+
+/* context options (mq_ctx_set_option) */
+#define MQ_OPT_PLAIN_TABLES 1  /* value 1: H*psi plans made afterwards also keep the per-task tables
+                                  that MQ_PREC_FP64_PLAIN runs (default 0) */
 
 /* MPS pairing (reference: MPSTensor::make_left_paired / make_right_paired, mpstensor.hpp:211-237) */
 #define MQ_RIGHT_PAIRED 0      /* blocks per left charge:  ls x (sum_sigma d_sigma * rs(lc+sigma)) */
@@ -131,6 +141,7 @@ typedef struct {
     uint64_t union_elements;          /* sum over blocks of union tiles * tile elements (T elements read) */
     double   seconds_tables;          /* host: schedule tables built and checked (part of build_seconds) */
     double   seconds_blocks;          /* host: term-sum block tables (part of build_seconds) */
+    double   seconds_plain;           /* per-task tables of MQ_PREC_FP64_PLAIN kept (part of build_seconds) */
 } mq_hpsi_stats_t;
 
 typedef struct {
@@ -160,6 +171,7 @@ int         mq_ctx_destroy(mq_ctx_t ctx);
 const char* mq_last_error(mq_ctx_t ctx);
 int         mq_ctx_sync(mq_ctx_t ctx);
 void*       mq_ctx_stream(mq_ctx_t ctx);      /* hipStream_t of the context */
+int         mq_ctx_set_option(mq_ctx_t ctx, int option, int value);
 int         mq_version(void);
 /* sizes in bytes of the ABI structs, in this order: mq_index_desc, mq_mpo_desc, mq_bindex_desc,
  * mq_boundary_info_t, mq_hpsi_stats_t, mq_davidson_stats_t, mq_growth_stats_t; returns how many
@@ -211,8 +223,9 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t plan, mq_hpsi_stats_t* stats);
 int mq_hpsi_plan_psi_elements(mq_hpsi_plan_t plan, uint64_t* n);
 /* device pointers (on the context's device), stream-ordered on the context stream */
 int mq_hpsi(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out_dev, int precision);
-/* precision: MQ_PREC_FP64, or MQ_PREC_FP32 (psi, L and R rounded to FP32, T-GEMMs, term sum and
- * output GEMMs on the FP32 matrix cores with FP32 intermediates, psi' returned in FP64) */
+/* precision: MQ_PREC_FP64, MQ_PREC_FP32 (psi, L and R rounded to FP32, T-GEMMs, term sum and
+ * output GEMMs on the FP32 matrix cores with FP32 intermediates, psi' returned in FP64), or
+ * MQ_PREC_FP64_PLAIN (never captured in a graph) */
 /* as mq_hpsi, synchronous, with the duration of each stage measured by HIP events recorded on
  * the context stream: stage_ms[0] set-up (FP32: rounding psi), [1] T-GEMMs, [2] S-assembly
  * (MPO-term sum), [3] output GEMMs

@@ -19,6 +19,8 @@ MQ_SYM_SU2U1 = 2
 MQ_PREC_FP64 = 0
 MQ_PREC_FP32 = 1
 MQ_PREC_BF16X3 = 2
+MQ_PREC_FP64_PLAIN = 3
+MQ_OPT_PLAIN_TABLES = 1
 MQ_RIGHT_PAIRED = 0
 MQ_LEFT_PAIRED = 1
 MQ_NO_PARTNER = 0xFFFFFFFF
@@ -96,6 +98,7 @@ class HpsiStats(C.Structure):
         ("union_elements", C.c_uint64),
         ("seconds_tables", C.c_double),
         ("seconds_blocks", C.c_double),
+        ("seconds_plain", C.c_double),
     ]
 
 
@@ -124,6 +127,7 @@ SIGNATURES = [
     ("mq_last_error", C.c_char_p, [vp]),
     ("mq_ctx_sync", i32, [vp]),
     ("mq_ctx_stream", vp, [vp]),
+    ("mq_ctx_set_option", i32, [vp, i32, i32]),
     ("mq_mpo_create", i32, [vp, P(MPODesc), P(vp)]),
     ("mq_mpo_destroy", i32, [vp]),
     ("mq_boundary_create_initial", i32, [vp, P(IndexDesc), C.c_uint32, P(vp)]),

@@ -44,7 +44,8 @@ def build_library(verbose=False):
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(lambda j: _run(j[0]), jobs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-fopenmp", "-o", tmp] + [o for _, o in jobs])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-fopenmp", "-o", tmp] + [o for _, o in jobs] +
+         ["-L/opt/rocm/lib", "-lrocblas"])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -2,6 +2,7 @@
 // steps and the Jacobi-Davidson eigensolver.  No C++ type crosses the ABI; errors are returned as
 // codes with the message kept on the context.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 #include <omp.h>
 
 #include <algorithm>
@@ -28,6 +29,8 @@ struct mq_ctx_s {
     hipStream_t stream = nullptr;
     std::string err;
     double* dwork = nullptr;  // DOT_BLOCKS partials + 64 scalars
+    bool plain_tables = false;         // MQ_OPT_PLAIN_TABLES
+    rocblas_handle blas = nullptr;     // library GEMMs of MQ_PREC_FP64_PLAIN, made on first use
 };
 
 struct mq_mpo_s {
@@ -238,7 +241,15 @@ struct mq_hpsi_plan_s {
     } graphs[2];
     bool use_graphs = false;
     uint64_t r_elems = 0;
-    double build_seconds = 0, seconds_tables = 0, seconds_blocks = 0;
+    double build_seconds = 0, seconds_tables = 0, seconds_blocks = 0, seconds_plain = 0;
+    // per-task tables of MQ_PREC_FP64_PLAIN (MQ_OPT_PLAIN_TABLES)
+    struct Plain {
+        bool ok = false;
+        std::vector<GemmDesc> t, c;
+        SLine* lines = nullptr;
+        STerm* terms = nullptr;
+        int64_t n_lines = 0;
+    } plain;
 };
 
 // ================================================================================================
@@ -301,6 +312,7 @@ int mq_ctx_destroy(mq_ctx_t ctx) {
     int rc = guarded(ctx, [&] {
         (void)hipStreamSynchronize(ctx->stream);
         dev_free(ctx->dwork);
+        if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
         (void)hipStreamDestroy(ctx->stream);
     });
     delete ctx;
@@ -308,6 +320,13 @@ int mq_ctx_destroy(mq_ctx_t ctx) {
 }
 
 const char* mq_last_error(mq_ctx_t ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mq_ctx_set_option(mq_ctx_t ctx, int option, int value) {
+    return guarded(ctx, [&] {
+        require(option == MQ_OPT_PLAIN_TABLES, "mq_ctx_set_option: unknown option");
+        ctx->plain_tables = value != 0;
+    });
+}
 
 int mq_ctx_sync(mq_ctx_t ctx) {
     return guarded(ctx, [&] { hipcheck(hipStreamSynchronize(ctx->stream), "stream synchronize"); });
@@ -617,6 +636,20 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
         p->n_lines = int64_t(H.lines.size());
         p->n_sblocks = int64_t(SB.blocks.size());
         p->union_elements = SB.gathered;
+        if (ctx->plain_tables) {
+            auto tp = std::chrono::steady_clock::now();
+            check_gemms(H.t_gemms_task, sizes);
+            check_gemms(H.c_gemms_cohort, sizes);
+            p->plain.t = H.t_gemms_task;
+            p->plain.c = H.c_gemms_cohort;
+            p->plain.lines = dev_upload(H.lines, ctx->stream);
+            p->plain.terms = dev_upload(H.terms, ctx->stream);
+            p->plain.n_lines = int64_t(H.lines.size());
+            p->plain.ok = true;
+            p->seconds_plain = seconds_since(tp);
+        }
+        std::vector<GemmDesc>().swap(H.t_gemms_task);
+        std::vector<GemmDesc>().swap(H.c_gemms_cohort);
         CopyJob* jobs = dev_upload(H.lpanel_jobs, ctx->stream);
         hipError_t e = launch_copy_jobs(jobs, int32_t(H.lpanel_jobs.size()), left->data, p->Lpanel, ctx->stream);
         hipError_t e2 = hipStreamSynchronize(ctx->stream);
@@ -659,6 +692,7 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
     s->seconds_tables = p->seconds_tables;
     s->seconds_blocks = p->seconds_blocks;
     s->union_elements = p->union_elements;
+    s->seconds_plain = p->seconds_plain;
     return MQ_OK;
 }
 
@@ -686,6 +720,31 @@ static void ensure_fp32(mq_hpsi_plan_t p) {
 // H*psi on the context stream; ev (optional, 5 events) brackets the stages set-up, T-GEMMs,
 // term sum and output GEMMs.  FP32: psi, L panels and R are rounded to FP32, the three stages
 // run on the FP32 matrix cores with FP32 T and S, and psi' is written in FP64.
+// One rocBLAS dgemm (or strided-batched dgemm) per problem on the context stream: the launch
+// structure of the reference's per-task cuBLAS calls (tasks.cpp:286-344, 650-752).
+static void library_gemms(mq_ctx_t ctx, std::vector<GemmDesc> const& probs, Bases const& b, double beta = 0.0) {
+    if (!ctx->blas) {
+        require(rocblas_create_handle(&ctx->blas) == rocblas_status_success, "rocblas_create_handle failed");
+        require(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host) == rocblas_status_success,
+                "rocblas_set_pointer_mode failed");
+    }
+    require(rocblas_set_stream(ctx->blas, ctx->stream) == rocblas_status_success, "rocblas_set_stream failed");
+    const double one = 1.0;
+    for (GemmDesc const& g : probs) {
+        if (g.M <= 0 || g.N <= 0) continue;
+        const double* A = static_cast<const double*>(b.p[g.base_a]) + g.a;
+        const double* B = static_cast<const double*>(b.p[g.base_b]) + g.b;
+        double* C = static_cast<double*>(b.p[g.base_c]) + g.c;
+        const rocblas_operation oa = (g.flags & GEMM_TRANS_A) ? rocblas_operation_transpose : rocblas_operation_none;
+        const rocblas_operation ob = (g.flags & GEMM_TRANS_B) ? rocblas_operation_transpose : rocblas_operation_none;
+        rocblas_status st = g.batch == 1
+            ? rocblas_dgemm(ctx->blas, oa, ob, g.M, g.N, g.K, &one, A, g.lda, B, g.ldb, &beta, C, g.ldc)
+            : rocblas_dgemm_strided_batched(ctx->blas, oa, ob, g.M, g.N, g.K, &one, A, g.lda, g.sa, B, g.ldb, g.sb,
+                                            &beta, C, g.ldc, g.sc, g.batch);
+        require(st == rocblas_status_success, "rocblas dgemm failed");
+    }
+}
+
 static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int precision, hipEvent_t* ev = nullptr) {
     hipStream_t s = p->ctx->stream;
     auto mark = [&](int i) {
@@ -723,8 +782,27 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         mark(3);
         p->cg.run(b, s, GEMM_F32_OUT_F64);
         mark(4);
+    } else if (precision == MQ_PREC_FP64_PLAIN) {
+        require(p->plain.ok, "H*psi MQ_PREC_FP64_PLAIN: the plan was made without MQ_OPT_PLAIN_TABLES");
+        b.p[BASE_PSI_IN] = const_cast<double*>(in);
+        b.p[BASE_T] = p->T;
+        b.p[BASE_S] = p->S;
+        b.p[BASE_LPANEL] = p->Lpanel;
+        b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
+        mark(0);
+        mark(1);
+        library_gemms(p->ctx, p->plain.t, b);
+        mark(2);
+        hipcheck(launch_slines(p->plain.lines, p->plain.n_lines, p->plain.terms, p->T, p->S, s), "S lines");
+        mark(3);
+        // psi' = 0, then every cohort's product is added in stream order (the reference's mutex /
+        // atomic accumulation into the output block)
+        hipcheck(hipMemsetAsync(out, 0, p->host.mps.elements() * sizeof(double), s), "psi' clear");
+        library_gemms(p->ctx, p->plain.c, b, 1.0);
+        mark(4);
     } else {
-        throw Error(MQ_ERR_UNSUPPORTED, "H*psi precision: MQ_PREC_FP64 and MQ_PREC_FP32 are available");
+        throw Error(MQ_ERR_UNSUPPORTED,
+                    "H*psi precision: MQ_PREC_FP64, MQ_PREC_FP32 and MQ_PREC_FP64_PLAIN are available");
     }
 }
 
@@ -762,7 +840,7 @@ static void hpsi_graph_enqueue(mq_hpsi_plan_t p, const double* in, double* out, 
 }
 
 static void hpsi_run(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
-    if (p->use_graphs) hpsi_graph_enqueue(p, in, out, precision);
+    if (p->use_graphs && precision != MQ_PREC_FP64_PLAIN) hpsi_graph_enqueue(p, in, out, precision);
     else hpsi_enqueue(p, in, out, precision);
 }
 
@@ -844,6 +922,8 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
         dev_free(p->Lpanel32);
         dev_free(p->R32);
         dev_free(p->psi32);
+        dev_free(p->plain.lines);
+        dev_free(p->plain.terms);
     });
     delete p;
     return rc;
@@ -992,9 +1072,11 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
             }
             };
             int refine = 0;
-            require(precision == MQ_PREC_FP64 || precision == MQ_PREC_FP32,
-                    "mq_davidson: precision must be MQ_PREC_FP64 or MQ_PREC_FP32");
-            if (precision == MQ_PREC_FP64 || max_iter < 2) {
+            require(precision == MQ_PREC_FP64 || precision == MQ_PREC_FP32 || precision == MQ_PREC_FP64_PLAIN,
+                    "mq_davidson: precision must be MQ_PREC_FP64, MQ_PREC_FP32 or MQ_PREC_FP64_PLAIN");
+            if (precision == MQ_PREC_FP64_PLAIN) {
+                jd(MQ_PREC_FP64_PLAIN, tol, max_iter);
+            } else if (precision == MQ_PREC_FP64 || max_iter < 2) {
                 jd(MQ_PREC_FP64, tol, max_iter);
             } else {
                 // mixed precision within the same iteration budget: FP32 H*psi for the first

@@ -285,6 +285,21 @@ __global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ 
     }
 }
 
+// One S line (rows x cols tile at s_off, leading dimension lds) per workgroup; element e of the
+// tile is the sum over the line's terms of alpha * T[t_off + e], in term order (reference:
+// compute_s_stackedv, gpu_dgemm_ddot.cu:282-359, one gather per term).  Lines without terms
+// write zeros.
+__global__ __launch_bounds__(256) void sline_kernel(const SLine* __restrict__ lines, const STerm* __restrict__ terms,
+                                                    const double* __restrict__ T, double* __restrict__ S) {
+    const SLine l = lines[blockIdx.x];
+    const int E = l.rows * l.cols;
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+        double acc = 0.0;
+        for (int k = l.t0; k < l.t1; ++k) acc += terms[k].alpha * T[terms[k].t_off + e];
+        S[l.s_off + e % l.rows + int64_t(e / l.rows) * l.lds] = acc;
+    }
+}
+
 template <class TI, class TO>
 __global__ void convert_kernel(const TI* __restrict__ src, TO* __restrict__ dst, uint64_t n) {
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
@@ -433,6 +448,15 @@ hipError_t launch_gemm(GemmPrec prec, const GemmDesc* probs, const GemmRow* rows
             launch_gemm_t<float, double>(probs, rows, tiles, n_tiles, reduces, n_reduces, ws, bases, stream);
             break;
         default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_slines(const SLine* lines, int64_t n_lines, const STerm* terms, const double* T, double* S,
+                         hipStream_t stream) {
+    for (int64_t off = 0; off < n_lines; off += 0x7fffffff) {
+        int64_t cnt = n_lines - off < 0x7fffffff ? n_lines - off : 0x7fffffff;
+        hipLaunchKernelGGL(sline_kernel, dim3(unsigned(cnt)), dim3(256), 0, stream, lines + off, terms, T, S);
     }
     return hipGetLastError();
 }

@@ -32,6 +32,10 @@ template <class T>
 hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_work, const int64_t* utile,
                           const T* ualpha, const int64_t* lsoff, const int32_t* llds, const T* X, T* S,
                           hipStream_t stream);
+// reference-style MPO-term sum: one workgroup per S line, every element gathers all of the line's
+// terms (the per-term formulation the block-form kernel replaces)
+hipError_t launch_slines(const SLine* lines, int64_t n_lines, const STerm* terms, const double* T, double* S,
+                         hipStream_t stream);
 // dst = float(src)
 hipError_t launch_convert(const double* src, float* dst, uint64_t n, hipStream_t stream);
 hipError_t launch_fill(double* p, uint64_t n, double v, hipStream_t stream);

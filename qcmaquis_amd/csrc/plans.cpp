@@ -256,7 +256,9 @@ struct RightTStacks {
         }
         stacks[it->second].push_back(Ref{&t, ls_ket, rs_ket, rs_bra, psi_block_off});
     }
-    void emit(BoundaryIndex const& R, int psi_base, std::vector<GemmDesc>& out, std::vector<GemmRow>& rows) const {
+    // per_task (optional): the same products as one problem per task, as the reference issues them
+    void emit(BoundaryIndex const& R, int psi_base, std::vector<GemmDesc>& out, std::vector<GemmRow>& rows,
+              std::vector<GemmDesc>* per_task = nullptr) const {
         for (auto const& st : stacks) {
             Ref const& f = st.front();
             const uint32_t ci_eff = f.t->ci_eff, nb = R.nblk[ci_eff], rs_ket = f.rs_ket, rs_bra = f.rs_bra;
@@ -271,6 +273,16 @@ struct RightTStacks {
                 for (uint32_t i = 0; i < r.ls_ket; ++i)
                     rows.push_back(GemmRow{a + i, int64_t(r.t->t_off) + i, int32_t(r.ls_ket), 0});
                 M += r.ls_ket;
+                if (!per_task) continue;
+                if (!trans)
+                    per_task->push_back(gemm(int32_t(r.ls_ket), int32_t(nb * rs_bra), rs_ket, psi_base, a,
+                                             int32_t(r.ls_ket), BASE_BOUNDARY, int64_t(R.data_off[ci_eff]), rs_ket,
+                                             BASE_T, int64_t(r.t->t_off), int32_t(r.ls_ket)));
+                else
+                    per_task->push_back(gemm(int32_t(r.ls_ket), rs_bra, rs_ket, psi_base, a, int32_t(r.ls_ket),
+                                             BASE_BOUNDARY, int64_t(R.data_off[ci_eff]), rs_bra, BASE_T,
+                                             int64_t(r.t->t_off), int32_t(r.ls_ket), GEMM_TRANS_B, int32_t(nb), 0,
+                                             int64_t(rs_bra) * rs_ket, int64_t(r.ls_ket) * rs_bra));
             }
             require(M < INT32_MAX, "T-GEMM: stacked problem too tall");
             if (!trans)
@@ -476,7 +488,7 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
             P.flops.flops_gemm_exec += 2ull * ls_in * csz;
         }
     }
-    tstacks.emit(R, BASE_PSI_IN, P.t_gemms, P.t_rows);
+    tstacks.emit(R, BASE_PSI_IN, P.t_gemms, P.t_rows, &P.t_gemms_task);
 
     // --- S panels / L panels per output block: cohorts of one lb_out stacked along K ---
     std::vector<uint64_t> ktot(left_i.size(), 0);
@@ -563,6 +575,17 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
         P.c_gemms.push_back(gemm(int32_t(ls), int32_t(P.mps.rp_cols[lb]), int32_t(ktot[lb]), BASE_LPANEL,
                                  int64_t(l_off[lb]), int32_t(ls), BASE_S, int64_t(s_off[lb]), int32_t(ktot[lb]),
                                  BASE_PSI_OUT, int64_t(P.mps.rp_off[lb]), int32_t(ls)));
+    }
+    // the reference's L*S products, one per cohort, each added into its output block (Cohort::contract,
+    // tasks.cpp:286-344): the K range of the cohort in its block's stacked L panel and S panel
+    for (size_t c = 0; c < cohorts.size(); ++c) {
+        auto const& rec = cohorts[c];
+        uint32_t ls_out = left_i.sizes[rec.lb_out];
+        int32_t kc = int32_t(uint64_t(rec.nb) * left_i.sizes[rec.lb_in]);
+        P.c_gemms_cohort.push_back(gemm(int32_t(ls_out), int32_t(P.mps.rp_cols[rec.lb_out]), kc, BASE_LPANEL,
+                                        int64_t(l_off[rec.lb_out] + uint64_t(ls_out) * koff[c]), int32_t(ls_out),
+                                        BASE_S, int64_t(s_off[rec.lb_out] + koff[c]), int32_t(ktot[rec.lb_out]),
+                                        BASE_PSI_OUT, int64_t(P.mps.rp_off[rec.lb_out]), int32_t(ls_out)));
     }
     uint64_t bytes = 0;
     for (auto ci : l_used) bytes += L.cohort_size(ci);

@@ -171,6 +171,8 @@ struct HpsiPlanHost {
     MPSLayout mps;
     std::vector<GemmDesc> t_gemms, c_gemms;
     std::vector<GemmRow> t_rows;  // row table of the stacked T-GEMMs
+    std::vector<GemmDesc> t_gemms_task;  // the same T-GEMMs, one problem per task (reference layout)
+    std::vector<GemmDesc> c_gemms_cohort;  // the L*S products per cohort, to be summed into psi'
     HostTable<SLine> lines;
     HostTable<STerm> terms;
     std::vector<CopyJob> lpanel_jobs;

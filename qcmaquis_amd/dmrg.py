@@ -219,6 +219,7 @@ class TwoSiteDMRG:
         self.ham, self.M, self.tol, self.max_iter, self.cutoff = ham, M, tol, max_iter, cutoff
         self.precision = precision
         self.compare_precision = None  # a list: every step also runs Davidson in the other precision
+        self.compare_plain = False  # with compare_precision: also the MQ_PREC_FP64_PLAIN Davidson
         self.verbose = verbose
         self.ctx = ctx or Context(0, abi.MQ_SYM_TWOU1)
         self.L = ham.L
@@ -270,13 +271,18 @@ class TwoSiteDMRG:
             # precision effect alone (the budgeted runs above also differ in convergence)
             et64, _, s64 = plan.davidson(psi2, tol=1e-10, max_iter=40, precision=abi.MQ_PREC_FP64)
             et32, _, s32 = plan.davidson(psi2, tol=1e-10, max_iter=40, precision=abi.MQ_PREC_FP32)
-            self.compare_precision.append({"site": i, "n": lay2.n, "e": e, "e_other": e2,
-                                           "seconds": t3 - t2, "seconds_other": c1 - c0,
-                                           "iterations": st["iterations"], "iterations_other": st2["iterations"],
-                                           "e_tight_fp64": et64, "e_tight_mixed": et32,
-                                           "residual_tight_fp64": s64["residual_norm"],
-                                           "residual_tight_mixed": s32["residual_norm"],
-                                           "seconds_extra": time.perf_counter() - c0})
+            rec = {"site": i, "n": lay2.n, "e": e, "e_other": e2, "seconds": t3 - t2, "seconds_other": c1 - c0,
+                   "iterations": st["iterations"], "iterations_other": st2["iterations"],
+                   "e_tight_fp64": et64, "e_tight_mixed": et32, "residual_tight_fp64": s64["residual_norm"],
+                   "residual_tight_mixed": s32["residual_norm"]}
+            if self.compare_plain:
+                # the reference's launch structure (needs a plan made with MQ_OPT_PLAIN_TABLES)
+                p0 = time.perf_counter()
+                ep, _, stp = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter,
+                                           precision=abi.MQ_PREC_FP64_PLAIN)
+                rec.update(e_plain=ep, seconds_plain=time.perf_counter() - p0, iterations_plain=stp["iterations"])
+            rec["seconds_extra"] = time.perf_counter() - c0
+            self.compare_precision.append(rec)
             extra = time.perf_counter() - c0
             self.timing["compare"] = self.timing.get("compare", 0.0) + extra
             t3s = t3 + extra  # the comparison solves are not charged to the split
@@ -285,6 +291,7 @@ class TwoSiteDMRG:
         self.timing["plan_tables"] += pst["seconds_tables"]
         self.timing["plan_blocks"] += pst["seconds_blocks"]
         self.timing["plan_total"] += pst["build_seconds"]
+        self.timing["plan_plain"] = self.timing.get("plan_plain", 0.0) + pst["seconds_plain"]
         self.timing["hpsi"] += st["hpsi_seconds"]
         plan.close()
         la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right)

@@ -45,6 +45,10 @@ class Context:
     def sync(self):
         self.check(self.lib.mq_ctx_sync(self.h))
 
+    def set_option(self, option, value):
+        """mq_ctx_set_option (abi.MQ_OPT_PLAIN_TABLES: plans made afterwards can run MQ_PREC_FP64_PLAIN)"""
+        self.check(self.lib.mq_ctx_set_option(self.h, int(option), int(value)))
+
     @property
     def stream(self):
         return self.lib.mq_ctx_stream(self.h)

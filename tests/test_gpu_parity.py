@@ -113,6 +113,45 @@ def test_fp32_hpsi(ctx2u1, ctxsu2, oracle_lib, sym):
         plan.close()
 
 
+@pytest.mark.parametrize("sym", [abi.MQ_SYM_TWOU1, abi.MQ_SYM_SU2U1])
+def test_plain_hpsi(ctx2u1, ctxsu2, oracle_lib, sym):
+    """the reference-style launch structure (one rocBLAS dgemm per T task and per output block,
+    one gather per term) gives the oracle's H*psi to the FP64 tolerance; a plan made without the
+    plain tables refuses MQ_PREC_FP64_PLAIN with an error code"""
+    from qcmaquis_amd.engine import EngineError
+    from qcmaquis_amd.synthetic import TWOU1_SITE, fuse_index
+    ctx = ctx2u1 if sym == abi.MQ_SYM_TWOU1 else ctxsu2
+    print(f"plain hpsi sym={sym}", flush=True)
+    if sym == abi.MQ_SYM_TWOU1:
+        ch = make_chain(5, n_sites=4, total=(4, 4), max_m=90, D=6, phys=fuse_index(TWOU1_SITE, TWOU1_SITE),
+                        density=0.3)
+    else:
+        ch = make_chain(4, sym=sym, n_sites=5, total=(4, 0), max_m=8, D=9)
+    mpos, L_dev, L_or, R_dev, R_or = _grow_both(ctx, oracle_lib, ch)
+    rng = np.random.default_rng(6)
+    k = len(ch.mpos) // 2
+    lay = ch.layouts[k]
+    psi = rng.normal(size=lay.n)
+    ref = oracle_lib.hpsi(ch.sym, ch.mpos[k], L_or[k], R_or[k + 1], lay, psi)
+    plan = HpsiPlan(ctx, mpos[k], L_dev[k], R_dev[k + 1], lay)
+    with pytest.raises(EngineError):
+        plan.apply(psi, precision=abi.MQ_PREC_FP64_PLAIN)
+    plan.close()
+    ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 1)
+    try:
+        plan = HpsiPlan(ctx, mpos[k], L_dev[k], R_dev[k + 1], lay)
+        got = plan.apply(psi, precision=abi.MQ_PREC_FP64_PLAIN)
+        print(f"plain hpsi site {k}: n={lay.n} err={rel_err(got, ref):.2e}", flush=True)
+        assert rel_err(got, ref) < TOL
+        assert rel_err(plan.apply(psi), ref) < TOL
+        e_p, _, _ = plan.davidson(psi, tol=1e-10, max_iter=30, precision=abi.MQ_PREC_FP64_PLAIN)
+        e_g, _, _ = plan.davidson(psi, tol=1e-10, max_iter=30)
+        assert abs(e_p - e_g) < 1e-9 * max(1.0, abs(e_g))
+        plan.close()
+    finally:
+        ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
+
+
 @pytest.mark.parametrize("seed,max_m,D", [(1, 3, 6), (4, 8, 9)])
 def test_su2u1_chain(ctxsu2, oracle_lib, seed, max_m, D):
     print(f"su2u1 chain seed={seed} max_m={max_m} D={D}", flush=True)
