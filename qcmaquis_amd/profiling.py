@@ -40,17 +40,19 @@ def short_name(name):
 
 
 def load(path):
+    """dispatch rows grouped by kernel name (and counter, for counter files with several)"""
     rows = list(csv.DictReader(open(path)))
     out = collections.OrderedDict()
     for r in rows:
-        out.setdefault(r["Kernel_Name"], []).append(r)
+        key = (r["Kernel_Name"], r.get("Counter_Name", ""))
+        out.setdefault(key, []).append(r)
     return out
 
 
 def summarize(path, last=0, scale=None, match=None):
     scale = scale or {}
     table = []
-    for name, rows in load(path).items():
+    for (name, _), rows in load(path).items():
         if match and not re.search(match, name):
             continue
         sel = rows[-last:] if last else rows
