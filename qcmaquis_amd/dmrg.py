@@ -10,6 +10,7 @@ Per two-site step (reference ts_optimize::sweep, ts_optimize.hpp:114-362):
 The SVD split runs on the host (numpy/LAPACK) in this version; boundaries and the contraction live
 in HBM.  MPS site tensors are kept as right-paired vectors with their MPSLayout.
 """
+import os
 import time
 
 import numpy as np
@@ -114,11 +115,31 @@ def estimate_truncation(svals, Mmax, cutoff):
     return keeps, tw
 
 
+_svd_pool = None
+
+
+def block_svds(blocks):
+    """thin SVD of every block: the blocks in parallel on the host cores, each with one BLAS
+    thread (the blocks are at most a few hundred rows, too small for a threaded LAPACK call)"""
+    global _svd_pool
+    if len(blocks) < 2:
+        return {m: np.linalg.svd(X, full_matrices=False) for m, X in blocks.items()}
+    from concurrent.futures import ThreadPoolExecutor
+    from threadpoolctl import threadpool_limits
+    if _svd_pool is None:
+        _svd_pool = ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1))
+    order = sorted(blocks, key=lambda m: -blocks[m].size)  # largest first
+    with threadpool_limits(1):
+        res = list(_svd_pool.map(lambda m: np.linalg.svd(blocks[m], full_matrices=False), order))
+    out = dict(zip(order, res))
+    return {m: out[m] for m in blocks}
+
+
 def split_two(lay2, psi2, phys, off12, Mmax, cutoff, to_right):
     """SVD split of a two-site tensor; to_right: A = U left-normalised, B = S V^T (split_mps_l2r),
     else A = U S, B = V^T right-normalised (split_mps_r2l)"""
     mats = split_matrices(lay2, psi2, phys, off12)
-    svd = {m: np.linalg.svd(X, full_matrices=False) for m, (X, _, _) in mats.items()}
+    svd = block_svds({m: X for m, (X, _, _) in mats.items()})
     keeps, tw = estimate_truncation({m: s[1] for m, s in svd.items()}, Mmax, cutoff)
     mid = Index([m for m in keeps if keeps[m] > 0], [keeps[m] for m in keeps if keeps[m] > 0])
     la = MPSLayout(lay2.left, phys, mid)
