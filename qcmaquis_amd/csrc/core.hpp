@@ -347,6 +347,11 @@ struct BoundaryIndex {
     // the partner's offsets into the cohort's own offset table, which is only consistent when the
     // completed cohort has no blocks of its own ("b-index complete cohorts"); that is asserted.
     void complement_transpose(Sym sym, Hermitian const& herm, bool forward) {
+        // pass 1 (serial, cohort numbering as in the reference's loop): partner cohorts, new ones
+        // appended as transposed aliases with their per-index tables still empty
+        require(herm.size() == aux_dim, "complement_transpose: Hermitian table size != aux_dim");
+        struct Pair { uint32_t ciA, ciB; uint32_t lb, rb; bool fresh; };
+        std::vector<Pair> pairs;
         for (size_t rb = 0; rb < ket.size(); ++rb)
             for (size_t lb = 0; lb < bra.size(); ++lb) {
                 uint32_t ciA = lbrb[lb * ket.size() + rb];
@@ -357,28 +362,58 @@ struct BoundaryIndex {
                     size_t pl = bra.position(ket.charges[rb]), pr = ket.position(bra.charges[lb]);
                     require(pl < bra.size() && pr < ket.size(),
                             "complement_transpose: partner charges missing from the index");
-                    ciB = add_cohort(pl, pr, std::vector<int64_t>(herm.size(), -1));
-                    tr[ciB] = 1;
+                    ciB = n_cohorts();
+                    lbrb[pl * ket.size() + pr] = ciB;
+                    offsets.emplace_back();
+                    conj_scales.emplace_back();
+                    transposes.emplace_back();
+                    tr.push_back(1);
+                    clr.push_back({uint32_t(pl), uint32_t(pr)});
+                    lsz.push_back(bra.sizes[pl]);
+                    rsz.push_back(ket.sizes[pr]);
+                    nblk.push_back(0);
                     fresh = true;
                 }
-                // herm.skip(b, l, r) = l < r and b has a partner: nothing to fill unless l < r
-                if (!(ket.charges[rb] < bra.charges[lb])) continue;
-                const double cc = conjugate_correction(sym, ket.charges[rb], bra.charges[lb]);
-                const bool target_ok = fresh || tr[ciB];
-                const int64_t* offA = offsets[ciA].data();
-                int64_t* offB = offsets[ciB].data();
-                double* csB = conj_scales[ciB].data();
-                uint8_t* tpB = transposes[ciB].data();
-                for (uint32_t b = 0; b < herm.size(); ++b) {
-                    const uint32_t pb = herm.herm[b];
-                    if (pb == MQ_NO_PARTNER) continue;  // herm.skip(b, l, r) with l < r
-                    require(offB[b] == -1, "complement_transpose: block already present");
-                    require(target_ok, "complement_transpose: cohort is not b-index complete");
+                pairs.push_back(Pair{ciA, ciB, uint32_t(lb), uint32_t(rb), fresh});
+            }
+        // pass 2 (in parallel; every pair writes only its own partner cohort): tables of the new
+        // cohorts, then the Hermitian-partner blocks by reference.  herm.skip(b, l, r) = l < r
+        // and b has a partner, so nothing is filled unless l < r
+        std::string err;
+#pragma omp parallel for schedule(dynamic, 8)
+        for (int64_t q = 0; q < int64_t(pairs.size()); ++q) {
+            Pair const& pr = pairs[size_t(q)];
+            if (pr.fresh) {
+                offsets[pr.ciB].assign(aux_dim, -1);
+                conj_scales[pr.ciB].assign(aux_dim, 1.0);
+                transposes[pr.ciB].assign(aux_dim, 0);
+            }
+            const Charge lc = ket.charges[pr.rb], rc = bra.charges[pr.lb];
+            if (!(lc < rc)) continue;
+            const double cc = conjugate_correction(sym, lc, rc);
+            const bool target_ok = pr.fresh || tr[pr.ciB];
+            const int64_t* offA = offsets[pr.ciA].data();
+            int64_t* offB = offsets[pr.ciB].data();
+            double* csB = conj_scales[pr.ciB].data();
+            uint8_t* tpB = transposes[pr.ciB].data();
+            const char* bad = nullptr;
+            for (uint32_t b = 0; b < herm.size() && !bad; ++b) {
+                const uint32_t pb = herm.herm[b];
+                if (pb == MQ_NO_PARTNER) continue;
+                if (offB[b] != -1) bad = "complement_transpose: block already present";
+                else if (!target_ok) bad = "complement_transpose: cohort is not b-index complete";
+                else {
                     offB[b] = offA[pb];
                     csB[b] = cc * double(herm.phase[forward ? b : pb]);
                     tpB[b] = 1;
                 }
             }
+            if (bad) {
+#pragma omp critical
+                err = bad;
+            }
+        }
+        if (!err.empty()) throw Error(MQ_ERR_INVALID, err);
         // as in the reference, n_blocks of a transposed cohort stays 0: consumers always size
         // work from the partner cohort ci_eff
     }
