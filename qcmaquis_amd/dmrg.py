@@ -347,3 +347,109 @@ class TwoSiteDMRG:
         for i in range(self.L - 2, -1, -1):
             self.step(i, False)
         return self.energies[-1], time.perf_counter() - t0
+
+
+def _absorb_into_right_neighbour(lay, data, mid, Rs):
+    """site right of a moved centre: its right-paired block of left sector m becomes
+    R_m (k x rs) @ block (rs x cols) (MPSTensor::multiply_from_left, mpstensor.hpp:345)"""
+    nl = MPSLayout(mid, lay.phys, lay.right)
+    out = np.zeros(nl.n)
+    for mb in range(len(mid)):
+        lb = lay.left.position(mid.charges[mb])
+        blk = rp_block(lay, data, lb)
+        out[nl.rp_off[mb]:nl.rp_off[mb + 1]] = (Rs[mb] @ blk).T.reshape(-1)
+    return nl, out
+
+
+def _absorb_into_left_neighbour(lay, data, mid, Ls):
+    """site left of a moved centre: its left-paired block of right sector m becomes
+    block (rows x ls) @ L_m (ls x k) (MPSTensor::multiply_from_right)"""
+    lp = lay.to_left(data)
+    nl = MPSLayout(lay.left, lay.phys, mid)
+    nlp = np.zeros(nl.n)
+    for mb in range(len(mid)):
+        rb = lay.right.position(mid.charges[mb])
+        rows = lay.lp_rows[rb]
+        blk = lp[lay.lp_off[rb]:lay.lp_off[rb + 1]].reshape(lay.right.sizes[rb], rows).T
+        nlp[nl.lp_off[mb]:nl.lp_off[mb + 1]] = (blk @ Ls[mb]).T.reshape(-1)
+    return nl, nl.to_right(nlp)
+
+
+class SingleSiteDMRG:
+    """Single-site DMRG (ss_optimize.hpp without the subspace-expansion noise): each step solves the
+    one-site problem H_eff(L_i, W_i, R_{i+1}) with the device Jacobi-Davidson, then moves the
+    orthogonality centre with an SVD of the site tensor (normalize_left / normalize_right,
+    mpstensor.hpp:242) and multiplies the remainder into the neighbour; the left or right boundary
+    of the moved bond is grown on the device.  Bond dimensions do not grow, so the bonds the state
+    starts from (allowed sectors capped at M) are the variational space."""
+
+    def __init__(self, ham, n_up, n_dn, M, ctx=None, seed=42, tol=1e-8, max_iter=8, precision=abi.MQ_PREC_FP64):
+        self.ham, self.tol, self.max_iter, self.precision = ham, tol, max_iter, precision
+        self.ctx = ctx or Context(0, abi.MQ_SYM_TWOU1)
+        self.L = ham.L
+        self.phys = ham.phys()
+        rng = np.random.default_rng(seed)
+        bonds = allowed_bonds([self.phys] * self.L, (n_up, n_dn), M)
+        self.lay = [MPSLayout(bonds[k], self.phys, bonds[k + 1]) for k in range(self.L)]
+        self.data = [rng.normal(size=l.n) for l in self.lay]
+        right_normalize(self.lay, self.data)
+        self.mpo1 = [DeviceMPO(self.ctx, ham.mpo(k)) for k in range(self.L)]
+        self.left = {0: DeviceBoundary.initial(self.ctx, self.lay[0].left, self.mpo1[0].spec.row_dim)}
+        self.right = {self.L: DeviceBoundary.initial(self.ctx, self.lay[-1].right, self.mpo1[-1].spec.col_dim)}
+        for k in range(self.L - 1, 0, -1):
+            m = DeviceMPS(self.ctx, self.lay[k], self.data[k])
+            self.right[k] = rstep(self.ctx, self.mpo1[k], self.right[k + 1], m, m)
+        self.energies = []
+
+    def step(self, i, to_right):
+        plan = HpsiPlan(self.ctx, self.mpo1[i], self.left[i], self.right[i + 1], self.lay[i])
+        e, psi, _ = plan.davidson(self.data[i], tol=self.tol, max_iter=self.max_iter, precision=self.precision)
+        plan.close()
+        lay = self.lay[i]
+        if to_right:
+            lp = lay.to_left(psi)
+            us, carry, sizes = {}, [], []
+            for rb in range(len(lay.right)):
+                rows, rs = lay.lp_rows[rb], lay.right.sizes[rb]
+                X = lp[lay.lp_off[rb]:lay.lp_off[rb + 1]].reshape(rs, rows).T
+                U, S, Vt = np.linalg.svd(X, full_matrices=False)
+                us[rb] = U
+                carry.append(S[:, None] * Vt)
+                sizes.append(U.shape[1])
+            mid = Index(lay.right.charges, sizes)
+            na = MPSLayout(lay.left, lay.phys, mid)
+            a_lp = np.zeros(na.n)
+            for mb in range(len(mid)):
+                a_lp[na.lp_off[mb]:na.lp_off[mb + 1]] = us[mb].T.reshape(-1)
+            self.lay[i], self.data[i] = na, na.to_right(a_lp)
+            self.lay[i + 1], self.data[i + 1] = _absorb_into_right_neighbour(self.lay[i + 1], self.data[i + 1], mid, carry)
+            m = DeviceMPS(self.ctx, self.lay[i], self.data[i])
+            self.left[i + 1] = lstep(self.ctx, self.mpo1[i], self.left[i], m, m)
+        else:
+            vts, carry, sizes = {}, [], []
+            for lb in range(len(lay.left)):
+                X = rp_block(lay, psi, lb)
+                U, S, Vt = np.linalg.svd(X, full_matrices=False)
+                vts[lb] = Vt
+                carry.append(U * S[None, :])
+                sizes.append(Vt.shape[0])
+            mid = Index(lay.left.charges, sizes)
+            nb = MPSLayout(mid, lay.phys, lay.right)
+            b_rp = np.zeros(nb.n)
+            for mb in range(len(mid)):
+                b_rp[nb.rp_off[mb]:nb.rp_off[mb + 1]] = vts[mb].T.reshape(-1)
+            self.lay[i], self.data[i] = nb, b_rp
+            self.lay[i - 1], self.data[i - 1] = _absorb_into_left_neighbour(self.lay[i - 1], self.data[i - 1], mid, carry)
+            m = DeviceMPS(self.ctx, self.lay[i], self.data[i])
+            self.right[i] = rstep(self.ctx, self.mpo1[i], self.right[i + 1], m, m)
+        self.energies.append(e + self.ham.ecore)
+        return e + self.ham.ecore
+
+    def sweep(self):
+        """left to right over sites 0..L-2, then right to left over L-1..1 (ss_optimize.hpp:100-190)"""
+        t0 = time.perf_counter()
+        for i in range(0, self.L - 1):
+            self.step(i, True)
+        for i in range(self.L - 1, 0, -1):
+            self.step(i, False)
+        return self.energies[-1], time.perf_counter() - t0

@@ -83,3 +83,25 @@ def test_sweeps_config1(ctx2u1):
         print(f"  sweep {s}: E={e:.12f} dE_fci={e - g['e0']:.3e} ({dt:.2f} s)", flush=True)
     assert e > g["e0"] - 1e-10
     assert e - g["e0"] < 1e-3
+
+
+def test_single_site_sweeps_small6(ctx2u1):
+    """single-site DMRG (ss_optimize without noise) on bonds that span the whole space reaches the
+    FCI energy; the energy never rises from one site problem to the next (variational)"""
+    from qcmaquis_amd.dmrg import SingleSiteDMRG
+    g = GOLD["small6"]
+    print("small6 single-site sweeps", flush=True)
+    h, eri, ec = make_integrals(g["L"], g["seed"], g["rank"])
+    d = SingleSiteDMRG(QCHamiltonian(h, eri, ec), g["n_up"], g["n_dn"], M=400, ctx=ctx2u1, tol=1e-11, max_iter=20)
+    for s in range(6):
+        e, dt = d.sweep()
+        print(f"  sweep {s}: E={e:.12f} ({dt:.2f} s)", flush=True)
+    assert abs(e - g["e0"]) < 1e-8
+    assert np.all(np.diff(d.energies) < 1e-9)
+    # truncated bonds (at most 4 states per sector): variational, monotone, above FCI
+    d = SingleSiteDMRG(QCHamiltonian(h, eri, ec), g["n_up"], g["n_dn"], M=4, ctx=ctx2u1, tol=1e-10, max_iter=20)
+    for s in range(4):
+        e, _ = d.sweep()
+    print(f"  M=4 per sector: E={e:.12f} (FCI {g['e0']:.12f})", flush=True)
+    assert e > g["e0"] - 1e-10 and e - g["e0"] < 1.0
+    assert np.all(np.diff(d.energies) < 1e-9)
