@@ -209,20 +209,21 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __rest
 }
 
 // Block-form MPO-term sum on the matrix cores.  A work item is SBLOCK_ELEMS consecutive tile
-// elements of one block of 16 lines; each of the four waves owns 16 * SB_NC of them and computes
+// elements of one block of 16 lines; each of the SB_WAVES waves owns 16 * SB_NC of them and computes
 // Y(16 lines x 16 elements) += alpha(16 x 4 tiles) * X(4 tiles x 16 elements) per matrix-core step
 // over the block's union of T tiles.  Lane l loads alpha of line l % 16 and union tile l / 16, and
 // element l % 16 of that tile: 16 consecutive values per tile.  The union's tile offsets and
 // coefficients are staged in LDS, SB_UT_CHUNK tiles at a time: the tile loads do not wait on a
 // global offset load, and the four waves read the coefficients from LDS, not four times from L2.
-constexpr int SB_NC = SBLOCK_ELEMS / 64;
+constexpr int SB_NC = 2;                                // 16-element column tiles per wave
+constexpr int SB_WAVES = SBLOCK_ELEMS / (16 * SB_NC);   // waves per workgroup
 constexpr int SB_UNROLL = 4;
 constexpr int SB_UT_CHUNK = 64;
-static_assert(SBLOCK_LINES == 16 && SB_NC * 64 == SBLOCK_ELEMS, "S block geometry");
+static_assert(SBLOCK_LINES == 16 && SB_WAVES * 16 * SB_NC == SBLOCK_ELEMS && SB_WAVES <= 16, "S block geometry");
 static_assert(SB_UT_CHUNK % (4 * SB_UNROLL) == 0, "offset chunks hold whole unrolled k-steps");
 
 template <class T>
-__global__ __launch_bounds__(256) void sblock_kernel(const SBlock* __restrict__ blocks, const SWork* __restrict__ work,
+__global__ __launch_bounds__(64 * SB_WAVES) void sblock_kernel(const SBlock* __restrict__ blocks, const SWork* __restrict__ work,
                                                      const int64_t* __restrict__ utile, const T* __restrict__ ualpha,
                                                      const int64_t* __restrict__ lsoff,
                                                      const int32_t* __restrict__ llds, const T* __restrict__ X,
@@ -467,7 +468,7 @@ hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_wor
                           hipStream_t stream) {
     for (int64_t off = 0; off < n_work; off += 0x7fffffff) {
         int64_t cnt = n_work - off < 0x7fffffff ? n_work - off : 0x7fffffff;
-        hipLaunchKernelGGL(sblock_kernel<T>, dim3(unsigned(cnt)), dim3(256), 0, stream, blocks, work + off, utile,
+        hipLaunchKernelGGL(sblock_kernel<T>, dim3(unsigned(cnt)), dim3(64 * SB_WAVES), 0, stream, blocks, work + off, utile,
                            ualpha, lsoff, llds, X, S);
     }
     return hipGetLastError();

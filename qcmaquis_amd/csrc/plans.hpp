@@ -106,7 +106,7 @@ struct SBlockTable {
     uint64_t mac = 0;             // SBLOCK_LINES * gathered (matrix-core multiply-adds)
 };
 // work items cover SBLOCK_ELEMS consecutive tile elements of one block
-constexpr int SBLOCK_ELEMS = 128;
+constexpr int SBLOCK_ELEMS = 256;
 
 // Execution schedule of a grouped GEMM stage.  Output tiles come in three shapes (BM x BN):
 // 32x128 for M <= 32, 64x64 for M <= 64, 128x64 above; so a problem with a short M edge is not
