@@ -127,8 +127,8 @@ int guarded(mq_ctx_t ctx, F&& f) {
 struct GemmStage {
     GemmDesc* probs = nullptr;
     GemmRow* rows = nullptr;
-    GemmTile* tiles[GEMM_N_SHAPES] = {nullptr, nullptr, nullptr};
-    int64_t n_tiles[GEMM_N_SHAPES] = {0, 0, 0};
+    GemmTile* tiles[GEMM_N_SHAPES] = {};
+    int64_t n_tiles[GEMM_N_SHAPES] = {};
     GemmReduce* reduces = nullptr;
     int64_t n_reduces = 0;
     double* ws = nullptr;

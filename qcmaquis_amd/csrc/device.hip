@@ -1,7 +1,7 @@
 // gfx950 kernels of the contraction path.
 //
-//  * gemm_kernel: grouped, batched FP64 / FP32 GEMM over a host-built tile table.  32x128, 64x64
-//    or 128x64 output tiles per 256-thread workgroup (four wave64s in a 2x2 grid, each a block of
+//  * gemm_kernel: grouped, batched FP64 / FP32 GEMM over a host-built tile table.  32x128, 64x64,
+//    96x64 or 128x64 output tiles per 256-thread workgroup (four wave64s in a 2x2 grid, each a block of
 //    16x16x4 matrix-core tiles), operands staged through LDS in K-slices of 8 with the next slice
 //    prefetched into registers.  Each output tile is owned by one workgroup or, when K is
 //    split, by a fixed set of partial tiles summed in order by gemm_reduce_kernel: no atomics.
@@ -71,6 +71,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
     const TI* B = static_cast<const TI*>(bases.p[g.base_b]) + g.b + int64_t(tile.batch) * g.sb;
     // stacked problem: the tile's rows are staged in LDS once (the A loads and the C stores look them
     // up there); A is not transposed, so each thread loads from one row (tid % BM) of the tile
+    // (stacked problems only get tile heights that divide 256)
     __shared__ GemmRow Rs[BM];
     const TI* a_row = A;
     int64_t a_rld = 0;
@@ -89,13 +90,15 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
     int a_m[NA], a_k[NA], b_k[NB], b_n[NB];
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
-        if (!ta) { a_m[i] = tid % BM; a_k[i] = tid / BM + (256 / BM) * i; }
-        else     { a_k[i] = tid % KS; a_m[i] = tid / KS + (256 / KS) * i; }
+        const int idx = tid + 256 * i;
+        if (!ta) { a_m[i] = idx % BM; a_k[i] = idx / BM; }
+        else     { a_k[i] = idx % KS; a_m[i] = idx / KS; }
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-        if (!tb) { b_k[i] = tid % KS; b_n[i] = tid / KS + (256 / KS) * i; }
-        else     { b_n[i] = tid % BN; b_k[i] = tid / BN + (256 / BN) * i; }
+        const int idx = tid + 256 * i;
+        if (!tb) { b_k[i] = idx % KS; b_n[i] = idx / KS; }
+        else     { b_n[i] = idx % BN; b_k[i] = idx / BN; }
     }
     const int k_end = tile.k1;
     TI ra[NA], rb[NB];
@@ -184,7 +187,8 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
 constexpr int REDUCE_CHUNKS = 32;  // 8192 (largest tile) / 256
 static_assert(gemm_bm(GEMM_128x64) * gemm_bn(GEMM_128x64) <= REDUCE_CHUNKS * 256 &&
                   gemm_bm(GEMM_32x128) * gemm_bn(GEMM_32x128) <= REDUCE_CHUNKS * 256 &&
-                  gemm_bm(GEMM_64x64) * gemm_bn(GEMM_64x64) <= REDUCE_CHUNKS * 256,
+                  gemm_bm(GEMM_64x64) * gemm_bn(GEMM_64x64) <= REDUCE_CHUNKS * 256 &&
+                  gemm_bm(GEMM_96x64) * gemm_bn(GEMM_96x64) <= REDUCE_CHUNKS * 256,
               "reduce grid covers every tile shape");
 template <class TW, class TO>
 __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __restrict__ probs,
@@ -420,6 +424,9 @@ void launch_gemm_t(const GemmDesc* probs, const GemmRow* rows, const GemmTile* c
                                    rows, tiles[sh] + off, bases, w);
             else if (sh == GEMM_64x64)
                 hipLaunchKernelGGL((gemm_kernel<2, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                                   rows, tiles[sh] + off, bases, w);
+            else if (sh == GEMM_96x64)
+                hipLaunchKernelGGL((gemm_kernel<3, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    rows, tiles[sh] + off, bases, w);
             else
                 hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,

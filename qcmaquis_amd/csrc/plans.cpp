@@ -1324,7 +1324,13 @@ void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size) {
 GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t target_items, int32_t min_chunk) {
     require(min_chunk >= 16 && min_chunk % 16 == 0, "gemm schedule: chunk must be a multiple of 16");
     GemmSchedule S;
-    auto shape_of = [](GemmDesc const& g) { return g.M <= 32 ? GEMM_32x128 : g.M <= 64 ? GEMM_64x64 : GEMM_128x64; };
+    // 96-row tiles (for 64 < M <= 96) only for problems with a plain A operand: a stacked problem's
+    // threads each load one row, which needs a tile height that divides 256
+    auto shape_of = [](GemmDesc const& g) {
+        return g.M <= 32 ? GEMM_32x128
+             : g.M <= 64 ? GEMM_64x64
+             : (g.M <= 96 && !(g.flags & GEMM_ROWMAP)) ? GEMM_96x64 : GEMM_128x64;
+    };
     auto tiles_of = [&](GemmDesc const& g) {
         int sh = shape_of(g);
         return int64_t((g.M + gemm_bm(sh) - 1) / gemm_bm(sh)) * ((g.N + gemm_bn(sh) - 1) / gemm_bn(sh)) * g.batch;

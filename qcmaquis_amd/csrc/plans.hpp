@@ -109,13 +109,16 @@ struct SBlockTable {
 constexpr int SBLOCK_ELEMS = 256;
 
 // Execution schedule of a grouped GEMM stage.  Output tiles come in three shapes (BM x BN):
-// 32x128 for M <= 32, 64x64 for M <= 64, 128x64 above; so a problem with a short M edge is not
-// padded to 64 rows and the B panel of a tall problem is read once.  When the stage has too few
+// 32x128 for M <= 32, 64x64 for M <= 64, 96x64 for M <= 96 (problems without a row table), 128x64
+// above; so a problem with a short M edge is not padded to a taller tile and the B panel of a tall
+// problem is read once.  When the stage has too few
 // tiles to fill the device, long reductions are split along K: each split tile writes its
 // partial BM x BN tile (column-major, ld BM) into a workspace and a reduce job adds the
 // partials in split order into C, so the result does not depend on scheduling.
-enum GemmShape { GEMM_32x128 = 0, GEMM_64x64 = 1, GEMM_128x64 = 2, GEMM_N_SHAPES = 3 };
-constexpr int gemm_bm(int shape) { return shape == GEMM_32x128 ? 32 : shape == GEMM_64x64 ? 64 : 128; }
+enum GemmShape { GEMM_32x128 = 0, GEMM_64x64 = 1, GEMM_128x64 = 2, GEMM_96x64 = 3, GEMM_N_SHAPES = 4 };
+constexpr int gemm_bm(int shape) {
+    return shape == GEMM_32x128 ? 32 : shape == GEMM_64x64 ? 64 : shape == GEMM_96x64 ? 96 : 128;
+}
 constexpr int gemm_bn(int shape) { return shape == GEMM_32x128 ? 128 : 64; }
 struct GemmTile {
     int32_t prob, batch, tm, tn;
@@ -133,7 +136,11 @@ struct GemmSchedule {
     std::vector<GemmTile> tiles[GEMM_N_SHAPES];
     std::vector<GemmReduce> reduces;
     uint64_t ws_elems = 0;
-    uint64_t n_tiles() const { return tiles[0].size() + tiles[1].size() + tiles[2].size(); }
+    uint64_t n_tiles() const {
+        uint64_t n = 0;
+        for (auto const& t : tiles) n += t.size();
+        return n;
+    }
 };
 // target_items: number of workgroups a stage should provide before K is split;
 // min_chunk: shortest K range of a split tile (a multiple of the K step 16)
