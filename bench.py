@@ -209,6 +209,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dump-bonds", default=None, help="write the MPS bond indices of the workload (JSON)")
     ap.add_argument("--no-sweeps", action="store_true", help="skip the FP64 / mixed sweep comparison")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the reference-style FP64 H*psi (thousands of library launches: slow under counters)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy inside the profiler region (HBM counter scale)")
     args = ap.parse_args()
@@ -235,7 +237,7 @@ def main():
     t0 = time.perf_counter()
     w = qc_workload(ctx, warm_sweeps=args.warm_sweeps, dump_bonds=args.dump_bonds) if args.workload == "qc" else synthetic_workload(ctx)
     log("workload built in %.1f s: %s" % (time.perf_counter() - t0, w["desc"]))
-    ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 1)  # the plan can also run the reference-style launches
+    ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, int(not args.no_plain))  # also the reference-style launches
     plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
     ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
     st = plan.stats()
@@ -308,24 +310,33 @@ def main():
     graph_ok = bool(torch.equal(outg, out32))  # last graph call was FP32 on the same psi
     # FP64 with the reference's launch structure: one library dgemm per T task and per output
     # block, one gather per term (MQ_PREC_FP64_PLAIN), on the same plan and psi
-    outp = torch.empty_like(psi)
-    plan.apply_device(psi.data_ptr(), outp.data_ptr(), abi.MQ_PREC_FP64_PLAIN)
-    ctx.sync()
-    nplain = max(2, min(args.steps, 5))
-    stagep = np.zeros(4)
-    t0 = time.perf_counter()
-    for _ in range(nplain):
+    plain = None
+    if not args.no_plain:
+        outp = torch.empty_like(psi)
         plan.apply_device(psi.data_ptr(), outp.data_ptr(), abi.MQ_PREC_FP64_PLAIN)
-    ctx.sync()
-    msp = (time.perf_counter() - t0) / nplain * 1e3
-    for _ in range(nplain):
-        ctx.check(ctx.lib.mq_hpsi_timed(plan.h, C.c_void_p(psi.data_ptr()), C.c_void_p(outp.data_ptr()),
-                                        abi.MQ_PREC_FP64_PLAIN, sms))
-        stagep += np.array(list(sms))
-    stagep /= nplain
-    errp = float((outp - ref).norm() / ref.norm())
-    log("FP64 plain H*psi: %.3f ms, stages %s, relative difference vs grouped FP64 %.2e" % (
-        msp, np.round(stagep, 4).tolist(), errp))
+        ctx.sync()
+        nplain = max(2, min(args.steps, 5))
+        stagep = np.zeros(4)
+        t0 = time.perf_counter()
+        for _ in range(nplain):
+            plan.apply_device(psi.data_ptr(), outp.data_ptr(), abi.MQ_PREC_FP64_PLAIN)
+        ctx.sync()
+        msp = (time.perf_counter() - t0) / nplain * 1e3
+        for _ in range(nplain):
+            ctx.check(ctx.lib.mq_hpsi_timed(plan.h, C.c_void_p(psi.data_ptr()), C.c_void_p(outp.data_ptr()),
+                                            abi.MQ_PREC_FP64_PLAIN, sms))
+            stagep += np.array(list(sms))
+        stagep /= nplain
+        errp = float((outp - ref).norm() / ref.norm())
+        log("FP64 plain H*psi: %.3f ms, stages %s, relative difference vs grouped FP64 %.2e" % (
+            msp, np.round(stagep, 4).tolist(), errp))
+        plain = {"ms_per_step": round(msp, 4), "tflops": round(st["flops_total"] / (msp * 1e-3) / 1e12, 4),
+                 "grouped_fp64_speedup": round(msp / (wall / args.steps * 1e3), 3),
+                 "mixed_grouped_speedup": round(msp / ms32, 3), "rel_difference_vs_grouped_fp64": errp,
+                 "stage_ms": [round(x, 4) for x in stagep],
+                 "launches": "one rocBLAS dgemm per executed T task, one S-assembly workgroup per line (%d) "
+                             "gathering each term, one rocBLAS dgemm per cohort (%d) added into its output "
+                             "block" % (st["n_lines"], st["n_cohorts"])}
     log("graph H*psi ms: %s (FP32 graph output equals eager: %s)" % (graph_ms, graph_ok))
     log("FP32 H*psi: %.3f ms, stages %s, relative error vs FP64 %.2e" % (ms32, np.round(stage32, 4).tolist(), err32))
     ms = wall / args.steps * 1e3
@@ -385,16 +396,7 @@ def main():
                                    "tflops": round(st["flops_total"] / (ms32 * 1e-3) / 1e12, 4),
                                    "speedup_vs_fp64": round(ms / ms32, 3), "rel_error_vs_fp64": err32,
                                    "stage_ms": [round(x, 4) for x in stage32]},
-                       fp64_plain={"ms_per_step": round(msp, 4),
-                                   "tflops": round(st["flops_total"] / (msp * 1e-3) / 1e12, 4),
-                                   "grouped_fp64_speedup": round(msp / ms, 3),
-                                   "mixed_grouped_speedup": round(msp / ms32, 3),
-                                   "rel_difference_vs_grouped_fp64": errp,
-                                   "stage_ms": [round(x, 4) for x in stagep],
-                                   "launches": "one rocBLAS dgemm per executed T task, one S-assembly "
-                                               "workgroup per line (%d) gathering each term, one rocBLAS dgemm "
-                                               "per cohort (%d) added into its output block" % (
-                                                   st["n_lines"], st["n_cohorts"])}),
+                       fp64_plain=plain),
         "roofline": dominant,
         "roofline_other": other,
     }
