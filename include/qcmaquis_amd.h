@@ -218,6 +218,15 @@ int mq_rstep(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t right, mq_mps_t bra, mq_m
 int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_boundary_t right,
                         const mq_index_desc* left_i, const mq_index_desc* phys_i,
                         const mq_index_desc* right_i, mq_hpsi_plan_t* out);
+/* n independent site problems as one plan (instance i: mpos[i], lefts[i], rights[i] and the MPS
+ * indices left_is[i], phys_is[i], right_is[i]); psi is the concatenation of the instances' psi, and
+ * every stage of an H*psi over all instances is one grouped launch.  mq_davidson refuses batch
+ * plans (their operator is block-diagonal).  Reference: no counterpart (one schedule per site
+ * problem). */
+int mq_hpsi_plan_create_batch(mq_ctx_t ctx, int n, const mq_mpo_t* mpos, const mq_boundary_t* lefts,
+                              const mq_boundary_t* rights, const mq_index_desc* left_is,
+                              const mq_index_desc* phys_is, const mq_index_desc* right_is,
+                              mq_hpsi_plan_t* out);
 int mq_hpsi_plan_stats(mq_hpsi_plan_t plan, mq_hpsi_stats_t* stats);
 /* psi: right-paired blocks of left_i concatenated, each ls x stripe column-major, no padding */
 int mq_hpsi_plan_psi_elements(mq_hpsi_plan_t plan, uint64_t* n);

@@ -145,6 +145,8 @@ SIGNATURES = [
     ("mq_lstep", i32, [vp, vp, vp, vp, vp, i32, i32, P(vp)]),
     ("mq_rstep", i32, [vp, vp, vp, vp, vp, i32, i32, P(vp)]),
     ("mq_hpsi_plan_create", i32, [vp, vp, vp, vp, P(IndexDesc), P(IndexDesc), P(IndexDesc), P(vp)]),
+    ("mq_hpsi_plan_create_batch", i32, [vp, i32, P(vp), P(vp), P(vp), P(IndexDesc), P(IndexDesc), P(IndexDesc),
+                                         P(vp)]),
     ("mq_hpsi_plan_stats", i32, [vp, P(HpsiStats)]),
     ("mq_hpsi_plan_psi_elements", i32, [vp, P(C.c_uint64)]),
     ("mq_hpsi", i32, [vp, vp, vp, i32]),

@@ -241,6 +241,9 @@ struct mq_hpsi_plan_s {
     } graphs[2];
     bool use_graphs = false;
     uint64_t r_elems = 0;
+    uint64_t psi_n = 0;          // psi elements (all instances of a batch plan)
+    int n_instances = 1;         // independent site problems in the plan
+    double* rown = nullptr;      // right boundaries copied into the plan (batch plans)
     double build_seconds = 0, seconds_tables = 0, seconds_blocks = 0, seconds_plain = 0;
     // per-task tables of MQ_PREC_FP64_PLAIN (MQ_OPT_PLAIN_TABLES)
     struct Plain {
@@ -592,79 +595,192 @@ int mq_boundary_growth_stats(mq_boundary_t b, mq_growth_stats_t* stats) {
 }
 
 // ---- H*psi ------------------------------------------------------------------------------------
+// L panel set-up of one site problem: copy jobs from its left boundary into the plan's panels
+struct LpanelPart {
+    const double* data;
+    uint64_t n;
+    std::vector<CopyJob> jobs;
+};
+
+// device side of an H*psi plan whose host tables are in p->host (p->psi_n, p->rdata set): checks,
+// term-sum blocks, buffers, uploads, L panels; frees the host term tables
+static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart> const& lparts, uint64_t r_n,
+                        std::chrono::steady_clock::time_point t0) {
+    auto seconds_since = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    };
+    HpsiPlanHost& H = p->host;
+    uint64_t sizes[BASE_COUNT] = {0};
+    sizes[BASE_PSI_IN] = p->psi_n;
+    sizes[BASE_PSI_OUT] = p->psi_n;
+    sizes[BASE_T] = H.t_elems;
+    sizes[BASE_S] = H.s_elems;
+    sizes[BASE_LPANEL] = H.lpanel_elems;
+    sizes[BASE_BOUNDARY] = r_n;
+    check_gemms(H.t_gemms, sizes, H.t_rows);
+    check_gemms(H.c_gemms, sizes);
+    check_lines(H.lines, H.terms, H.t_elems, H.s_elems);
+    p->seconds_tables = seconds_since(t0);
+    auto t1 = std::chrono::steady_clock::now();
+    SBlockTable SB = build_sblocks(H.lines, H.terms);
+    check_sblocks(SB, H.t_elems, H.s_elems);
+    p->seconds_blocks = seconds_since(t1);
+    for (auto const& part : lparts)
+        for (auto const& j : part.jobs)
+            require(j.src >= 0 && j.dst >= 0 && uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= part.n &&
+                        uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= H.lpanel_elems,
+                    "L panel job outside its buffers");
+    p->r_elems = r_n;
+    p->T = dev_alloc<double>(H.t_elems);
+    p->S = dev_alloc<double>(H.s_elems);
+    p->Lpanel = dev_alloc<double>(H.lpanel_elems);
+    p->tg.upload(H.t_gemms, H.t_rows, ctx->stream);
+    p->cg.upload(H.c_gemms, {}, ctx->stream);
+    p->sa.upload(SB, ctx->stream);
+    p->n_lines = int64_t(H.lines.size());
+    p->n_sblocks = int64_t(SB.blocks.size());
+    p->union_elements = SB.gathered;
+    if (ctx->plain_tables && p->n_instances == 1) {
+        auto tp = std::chrono::steady_clock::now();
+        check_gemms(H.t_gemms_task, sizes);
+        check_gemms(H.c_gemms_cohort, sizes);
+        p->plain.t = H.t_gemms_task;
+        p->plain.c = H.c_gemms_cohort;
+        p->plain.lines = dev_upload(H.lines, ctx->stream);
+        p->plain.terms = dev_upload(H.terms, ctx->stream);
+        p->plain.n_lines = int64_t(H.lines.size());
+        p->plain.ok = true;
+        p->seconds_plain = seconds_since(tp);
+    }
+    std::vector<GemmDesc>().swap(H.t_gemms_task);
+    std::vector<GemmDesc>().swap(H.c_gemms_cohort);
+    for (auto const& part : lparts) {
+        CopyJob* jobs = dev_upload(part.jobs, ctx->stream);
+        hipError_t e = launch_copy_jobs(jobs, int32_t(part.jobs.size()), part.data, p->Lpanel, ctx->stream);
+        hipError_t e2 = hipStreamSynchronize(ctx->stream);
+        dev_free(jobs);
+        hipcheck(e, "L panel copy");
+        hipcheck(e2, "L panel copy");
+    }
+    // the host tables are on the device now; keep only what the statistics need
+    for (SLine const& l : H.lines) {
+        uint64_t e = uint64_t(l.rows) * uint64_t(l.cols);
+        p->s_line_elements += e;
+        p->gathered_elements += e * uint64_t(l.t1 - l.t0);
+    }
+    H.terms.clear(); H.terms.shrink_to_fit();
+    H.lines.clear(); H.lines.shrink_to_fit();
+    H.lpanel_jobs.clear(); H.lpanel_jobs.shrink_to_fit();
+    p->build_seconds = seconds_since(t0);
+}
+
 int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_boundary_t right,
                         const mq_index_desc* left_i, const mq_index_desc* phys_i, const mq_index_desc* right_i,
                         mq_hpsi_plan_t* out) {
     return guarded(ctx, [&] {
         require(mpo && left && right && out, "mq_hpsi_plan_create: null argument");
         auto t0 = std::chrono::steady_clock::now();
-        auto seconds_since = [](std::chrono::steady_clock::time_point t) {
-            return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
-        };
         auto* p = new mq_hpsi_plan_s{ctx, build_hpsi_plan(ctx->sym, mpo->mpo, left->idx, right->idx, index_from(left_i),
                                                           index_from(phys_i), index_from(right_i))};
         std::unique_ptr<mq_hpsi_plan_s> guard(p);
-        HpsiPlanHost& H = p->host;
-        uint64_t psi_n = H.mps.elements();
-        uint64_t sizes[BASE_COUNT] = {0};
-        sizes[BASE_PSI_IN] = psi_n;
-        sizes[BASE_PSI_OUT] = psi_n;
-        sizes[BASE_T] = H.t_elems;
-        sizes[BASE_S] = H.s_elems;
-        sizes[BASE_LPANEL] = H.lpanel_elems;
-        sizes[BASE_BOUNDARY] = right->n;
-        check_gemms(H.t_gemms, sizes, H.t_rows);
-        check_gemms(H.c_gemms, sizes);
-        check_lines(H.lines, H.terms, H.t_elems, H.s_elems);
-        p->seconds_tables = seconds_since(t0);
-        auto t1 = std::chrono::steady_clock::now();
-        SBlockTable SB = build_sblocks(H.lines, H.terms);
-        check_sblocks(SB, H.t_elems, H.s_elems);
-        p->seconds_blocks = seconds_since(t1);
-        for (auto const& j : H.lpanel_jobs)
-            require(j.src >= 0 && j.dst >= 0 && uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= left->n &&
-                        uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= H.lpanel_elems,
-                    "L panel job outside its buffers");
+        p->psi_n = p->host.mps.elements();
         p->rdata = right->data;
-        p->r_elems = right->n;
-        p->T = dev_alloc<double>(H.t_elems);
-        p->S = dev_alloc<double>(H.s_elems);
-        p->Lpanel = dev_alloc<double>(H.lpanel_elems);
-        p->tg.upload(H.t_gemms, H.t_rows, ctx->stream);
-        p->cg.upload(H.c_gemms, {}, ctx->stream);
-        p->sa.upload(SB, ctx->stream);
-        p->n_lines = int64_t(H.lines.size());
-        p->n_sblocks = int64_t(SB.blocks.size());
-        p->union_elements = SB.gathered;
-        if (ctx->plain_tables) {
-            auto tp = std::chrono::steady_clock::now();
-            check_gemms(H.t_gemms_task, sizes);
-            check_gemms(H.c_gemms_cohort, sizes);
-            p->plain.t = H.t_gemms_task;
-            p->plain.c = H.c_gemms_cohort;
-            p->plain.lines = dev_upload(H.lines, ctx->stream);
-            p->plain.terms = dev_upload(H.terms, ctx->stream);
-            p->plain.n_lines = int64_t(H.lines.size());
-            p->plain.ok = true;
-            p->seconds_plain = seconds_since(tp);
+        finish_plan(ctx, p, {LpanelPart{left->data, left->n, p->host.lpanel_jobs}}, right->n, t0);
+        *out = guard.release();
+    });
+}
+
+// n independent site problems as one plan: the host tables of each are built as for one problem,
+// then concatenated with their buffer offsets shifted (psi, T, S, L panels, right boundaries, GEMM
+// row table, term list), so every stage of the batch is one grouped launch
+int mq_hpsi_plan_create_batch(mq_ctx_t ctx, int n, const mq_mpo_t* mpos, const mq_boundary_t* lefts,
+                              const mq_boundary_t* rights, const mq_index_desc* left_is,
+                              const mq_index_desc* phys_is, const mq_index_desc* right_is, mq_hpsi_plan_t* out) {
+    return guarded(ctx, [&] {
+        require(n >= 1 && mpos && lefts && rights && left_is && phys_is && right_is && out,
+                "mq_hpsi_plan_create_batch: null argument");
+        for (int i = 0; i < n; ++i)
+            require(mpos[i] && lefts[i] && rights[i], "mq_hpsi_plan_create_batch: null instance");
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<HpsiPlanHost> parts(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i)
+            parts[size_t(i)] = build_hpsi_plan(ctx->sym, mpos[i]->mpo, lefts[i]->idx, rights[i]->idx,
+                                               index_from(&left_is[i]), index_from(&phys_is[i]),
+                                               index_from(&right_is[i]));
+        auto* p = new mq_hpsi_plan_s{ctx, HpsiPlanHost{}};
+        std::unique_ptr<mq_hpsi_plan_s> guard(p);
+        HpsiPlanHost& H = p->host;
+        H.mps = parts[0].mps;
+        uint64_t r_n = 0;
+        std::vector<LpanelPart> lparts;
+        int64_t shift[BASE_COUNT] = {0};
+        for (int i = 0; i < n; ++i) {
+            HpsiPlanHost& Q = parts[size_t(i)];
+            shift[BASE_PSI_IN] = shift[BASE_PSI_OUT] = int64_t(p->psi_n);
+            shift[BASE_T] = int64_t(H.t_elems);
+            shift[BASE_S] = int64_t(H.s_elems);
+            shift[BASE_LPANEL] = int64_t(H.lpanel_elems);
+            shift[BASE_BOUNDARY] = int64_t(r_n);
+            const int64_t row0 = int64_t(H.t_rows.size()), term0 = int64_t(H.terms.size());
+            for (GemmDesc g : Q.t_gemms) {
+                if (g.flags & GEMM_ROWMAP) g.a += row0;
+                else g.a += shift[g.base_a];
+                g.b += shift[g.base_b];
+                if (!(g.flags & GEMM_ROWMAP)) g.c += shift[g.base_c];
+                H.t_gemms.push_back(g);
+            }
+            for (GemmRow r : Q.t_rows) {
+                r.a += shift[BASE_PSI_IN];
+                r.c += shift[BASE_T];
+                H.t_rows.push_back(r);
+            }
+            for (GemmDesc g : Q.c_gemms) {
+                g.a += shift[g.base_a];
+                g.b += shift[g.base_b];
+                g.c += shift[g.base_c];
+                H.c_gemms.push_back(g);
+            }
+            for (SLine l : Q.lines) {
+                l.s_off += shift[BASE_S];
+                l.t0 += int32_t(term0);
+                l.t1 += int32_t(term0);
+                H.lines.push_back(l);
+            }
+            require(term0 + int64_t(Q.terms.size()) < int64_t(INT32_MAX), "batch: more than 2^31 terms");
+            for (STerm t : Q.terms) {
+                t.t_off += shift[BASE_T];
+                H.terms.push_back(t);
+            }
+            LpanelPart lp{lefts[i]->data, lefts[i]->n, Q.lpanel_jobs};
+            for (auto& j : lp.jobs) j.dst += shift[BASE_LPANEL];
+            lparts.push_back(std::move(lp));
+            H.t_elems += Q.t_elems;
+            H.s_elems += Q.s_elems;
+            H.lpanel_elems += Q.lpanel_elems;
+            H.n_cohorts += Q.n_cohorts;
+            H.n_t_tasks += Q.n_t_tasks;
+            H.bytes_compulsory += Q.bytes_compulsory;
+            H.flops.flops_total += Q.flops.flops_total;
+            H.flops.flops_gemm += Q.flops.flops_gemm;
+            H.flops.flops_gemm_exec += Q.flops.flops_gemm_exec;
+            H.flops.n_terms += Q.flops.n_terms;
+            p->psi_n += Q.mps.elements();
+            r_n += rights[i]->n;
+            Q = HpsiPlanHost{};  // host tables merged: release them
         }
-        std::vector<GemmDesc>().swap(H.t_gemms_task);
-        std::vector<GemmDesc>().swap(H.c_gemms_cohort);
-        CopyJob* jobs = dev_upload(H.lpanel_jobs, ctx->stream);
-        hipError_t e = launch_copy_jobs(jobs, int32_t(H.lpanel_jobs.size()), left->data, p->Lpanel, ctx->stream);
-        hipError_t e2 = hipStreamSynchronize(ctx->stream);
-        dev_free(jobs);
-        hipcheck(e, "L panel copy");
-        hipcheck(e2, "L panel copy");
-        // the host tables are on the device now; keep only what the statistics need
-        for (SLine const& l : H.lines) {
-            uint64_t e = uint64_t(l.rows) * uint64_t(l.cols);
-            p->s_line_elements += e;
-            p->gathered_elements += e * uint64_t(l.t1 - l.t0);
+        p->n_instances = n;
+        // the right boundaries of all instances in one device buffer owned by the plan
+        p->rown = dev_alloc<double>(r_n);
+        uint64_t roff = 0;
+        for (int i = 0; i < n; ++i) {
+            if (rights[i]->n)
+                hipcheck(hipMemcpyAsync(p->rown + roff, rights[i]->data, rights[i]->n * sizeof(double),
+                                        hipMemcpyDeviceToDevice, ctx->stream), "batch right boundaries");
+            roff += rights[i]->n;
         }
-        H.terms.clear(); H.terms.shrink_to_fit();
-        H.lines.clear(); H.lines.shrink_to_fit();
-        p->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        hipcheck(hipStreamSynchronize(ctx->stream), "batch right boundaries");
+        p->rdata = p->rown;
+        finish_plan(ctx, p, lparts, r_n, t0);
         *out = guard.release();
     });
 }
@@ -672,7 +788,7 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
 int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
     if (!p || !s) return MQ_ERR_INVALID;
     HpsiPlanHost const& H = p->host;
-    s->psi_elements = H.mps.elements();
+    s->psi_elements = p->psi_n;
     s->n_t_tasks = H.n_t_tasks;
     s->n_cohorts = H.n_cohorts;
     s->n_lines = uint64_t(p->n_lines);
@@ -698,7 +814,7 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
 
 int mq_hpsi_plan_psi_elements(mq_hpsi_plan_t p, uint64_t* n) {
     if (!p || !n) return MQ_ERR_INVALID;
-    *n = p->host.mps.elements();
+    *n = p->psi_n;
     return MQ_OK;
 }
 
@@ -710,7 +826,7 @@ static void ensure_fp32(mq_hpsi_plan_t p) {
     p->S32 = dev_alloc<float>(H.s_elems);
     p->Lpanel32 = dev_alloc<float>(H.lpanel_elems);
     p->R32 = dev_alloc<float>(p->r_elems);
-    p->psi32 = dev_alloc<float>(H.mps.elements());
+    p->psi32 = dev_alloc<float>(p->psi_n);
     hipcheck(launch_convert(p->Lpanel, p->Lpanel32, H.lpanel_elems, s), "convert");
     hipcheck(launch_convert(p->rdata, p->R32, p->r_elems, s), "convert");
     p->sa.ensure32(s);
@@ -774,7 +890,7 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         b.p[BASE_LPANEL] = p->Lpanel32;
         b.p[BASE_BOUNDARY] = p->R32;
         mark(0);
-        hipcheck(launch_convert(in, p->psi32, p->host.mps.elements(), s), "convert");
+        hipcheck(launch_convert(in, p->psi32, p->psi_n, s), "convert");
         mark(1);
         p->tg.run(b, s, GEMM_F32);
         mark(2);
@@ -783,7 +899,7 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         p->cg.run(b, s, GEMM_F32_OUT_F64);
         mark(4);
     } else if (precision == MQ_PREC_FP64_PLAIN) {
-        require(p->plain.ok, "H*psi MQ_PREC_FP64_PLAIN: the plan was made without MQ_OPT_PLAIN_TABLES");
+        require(p->plain.ok, "H*psi MQ_PREC_FP64_PLAIN: the plan was made without MQ_OPT_PLAIN_TABLES or is a batch plan");
         b.p[BASE_PSI_IN] = const_cast<double*>(in);
         b.p[BASE_T] = p->T;
         b.p[BASE_S] = p->S;
@@ -797,7 +913,7 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         mark(3);
         // psi' = 0, then every cohort's product is added in stream order (the reference's mutex /
         // atomic accumulation into the output block)
-        hipcheck(hipMemsetAsync(out, 0, p->host.mps.elements() * sizeof(double), s), "psi' clear");
+        hipcheck(hipMemsetAsync(out, 0, p->psi_n * sizeof(double), s), "psi' clear");
         library_gemms(p->ctx, p->plain.c, b, 1.0);
         mark(4);
     } else {
@@ -813,7 +929,7 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
 static void hpsi_graph_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
     require(precision == MQ_PREC_FP64 || precision == MQ_PREC_FP32, "H*psi graph: unsupported precision");
     hipStream_t s = p->ctx->stream;
-    const uint64_t n = p->host.mps.elements();
+    const uint64_t n = p->psi_n;
     auto& g = p->graphs[precision == MQ_PREC_FP64 ? 0 : 1];
     if (!g.exec) {
         if (precision == MQ_PREC_FP32) ensure_fp32(p);
@@ -883,7 +999,7 @@ int mq_hpsi_host(mq_hpsi_plan_t p, const double* in, double* out, int precision)
     if (!p) return MQ_ERR_INVALID;
     return guarded(p->ctx, [&] {
         require(in && out, "mq_hpsi_host: null argument");
-        uint64_t n = p->host.mps.elements();
+        uint64_t n = p->psi_n;
         double* din = dev_alloc<double>(n);
         double* dout = dev_alloc<double>(n);
         try {
@@ -924,6 +1040,7 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
         dev_free(p->psi32);
         dev_free(p->plain.lines);
         dev_free(p->plain.terms);
+        dev_free(p->rown);
     });
     delete p;
     return rc;
@@ -981,11 +1098,12 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
                            int max_iter, int precision, double* eval, double* psi_out, mq_davidson_stats_t* stats) {
     if (!p) return MQ_ERR_INVALID;
     return guarded(p->ctx, [&] {
+        require(p->n_instances == 1, "mq_davidson: a batch plan holds several independent problems");
         require(psi0 && eval && psi_out && max_iter >= 1 && n_ortho >= 0, "mq_davidson: bad argument");
         require(n_ortho == 0 || ortho, "mq_davidson: null ortho vectors");
         auto t_start = std::chrono::steady_clock::now();
         hipStream_t s = p->ctx->stream;
-        const uint64_t n = p->host.mps.elements();
+        const uint64_t n = p->psi_n;
         require(n > 0, "mq_davidson: empty site problem");
         // the reference truncates the ortho list to at most n-1 vectors (ietl_jacobi_davidson.h:41-42)
         int no = std::min<int64_t>(n_ortho, int64_t(n) - 1);

@@ -225,3 +225,52 @@ class HpsiPlan(_Handle):
         self.ctx.check(self.ctx.lib.mq_davidson(self.h, _dp(psi0), len(orth), arr, float(tol), int(max_iter),
                                                 int(precision), C.byref(ev), _dp(out), C.byref(st)))
         return ev.value, out, {f: getattr(st, f) for f, _ in abi.DavidsonStats._fields_}
+
+
+class HpsiPlanBatch(_Handle):
+    """Independent site problems in one plan (mq_hpsi_plan_create_batch): psi is the concatenation
+    of the instances' psi and one H*psi over all of them is one grouped launch per stage.
+    items: (mpo, left, right, layout) per instance."""
+
+    _destroy = "mq_hpsi_plan_destroy"
+
+    def __init__(self, ctx, items):
+        n = len(items)
+        h = C.c_void_p()
+        vp = C.c_void_p * n
+        descs = [[it[3].left.desc() for it in items], [it[3].phys.desc() for it in items],
+                 [it[3].right.desc() for it in items]]
+        arr = [(abi.IndexDesc * n)(*d) for d in descs]
+        ctx.check(ctx.lib.mq_hpsi_plan_create_batch(ctx.h, n, vp(*[it[0].h for it in items]),
+                                                    vp(*[it[1].h for it in items]), vp(*[it[2].h for it in items]),
+                                                    arr[0], arr[1], arr[2], C.byref(h)))
+        super().__init__(ctx, h)
+        self.layouts = [it[3] for it in items]
+        self.offsets = np.concatenate([[0], np.cumsum([lay.n for lay in self.layouts])]).astype(np.int64)
+        self._refs = items
+
+    @property
+    def n(self):
+        return int(self.offsets[-1])
+
+    def split(self, vec):
+        """the instances' segments of a concatenated vector"""
+        return [vec[self.offsets[i]:self.offsets[i + 1]] for i in range(len(self.layouts))]
+
+    def stats(self):
+        s = abi.HpsiStats()
+        self.ctx.check(self.ctx.lib.mq_hpsi_plan_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in abi.HpsiStats._fields_}
+
+    def apply(self, psi, precision=abi.MQ_PREC_FP64):
+        """H*psi of every instance for a host vector (instances concatenated)"""
+        psi = np.ascontiguousarray(psi, np.float64)
+        if psi.size != self.n:
+            raise ValueError("psi size does not match the batch")
+        out = np.zeros_like(psi)
+        self.ctx.check(self.ctx.lib.mq_hpsi_host(self.h, _dp(psi), _dp(out), int(precision)))
+        return out
+
+    def apply_device(self, in_ptr, out_ptr, precision=abi.MQ_PREC_FP64):
+        self.ctx.check(self.ctx.lib.mq_hpsi(self.h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), int(precision)))
+
