@@ -9,7 +9,10 @@ sblock_kernel<double>) and of the grouped FP64 GEMMs, from HIP events recorded o
 and the CPU oracle (restated reference CPU path, OpenMP + OpenBLAS) timed on the host cores on the
 same site problem.
 
-  python bench.py [--steps K] [--warmup W] [--workload synthetic] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--workload synthetic|batch] [--no-cpu]
+
+--workload batch measures configuration 4 (many independent site problems in one batch plan) and
+prints its own line.
 """
 import argparse
 import json
@@ -95,6 +98,182 @@ def qc_workload(ctx, L=24, M=512, seed=1002, warm_sweeps=1, dump_bonds=None):
             "sweep_seconds": sweeps, "sweep_timing_s": {k: round(v, 3) for k, v in d.timing.items()}}
     return dict(mpo=mpo, mpo_spec=ham.mpo(mid, True), L=d.left[mid], R=d.right[mid + 2], layout=lay2,
                 sym=abi.MQ_SYM_TWOU1, desc=desc, keep=(d, ham))
+
+
+def scaled_bonds(phys_list, total, M):
+    """bond indices of a random MPS of bond dimension about M: every sector allowed by the total
+    quantum numbers, sizes proportional to the number of states the sector can hold (min of the
+    left and right counts), scaled down to about M states per bond"""
+    from qcmaquis_amd.dmrg import allowed_bonds
+    from qcmaquis_amd.model import Index
+    out = []
+    for b in allowed_bonds(phys_list, total, 1 << 40):
+        sizes = np.array(b.sizes, np.float64)
+        if sizes.sum() > M:
+            sizes = np.maximum(1.0, np.floor(sizes * M / sizes.sum()))
+        out.append(Index.from_dict({tuple(int(x) for x in q): int(m) for q, m in zip(b.charges, sizes)}))
+    return out
+
+
+def batch_instance(ctx, L, M, seed):
+    """one instance of configuration 4: seeded integrals (PCG64(seed), L-rank 16), half filling,
+    a right-normalised random MPS of bond dimension about M, boundaries grown on the device to the
+    middle two-site problem (sites L/2-1, L/2); returns its (mpo, left, right, layout) and psi"""
+    from qcmaquis_amd.dmrg import merge_two, right_normalize, twosite_offsets
+    from qcmaquis_amd.engine import DeviceBoundary, DeviceMPO, DeviceMPS, lstep, rstep
+    from qcmaquis_amd.model import MPSLayout
+    from qcmaquis_amd.qc import QCHamiltonian, make_integrals
+    h, eri, ec = make_integrals(L, seed, 16)
+    ham = QCHamiltonian(h, eri, ec)
+    phys = ham.phys()
+    bonds = scaled_bonds([phys] * L, (L // 2, L // 2), M)
+    lay = [MPSLayout(bonds[k], phys, bonds[k + 1]) for k in range(L)]
+    rng = np.random.default_rng(seed)
+    data = [rng.normal(size=x.n) for x in lay]
+    right_normalize(lay, data)
+    mid = L // 2 - 1
+    R = DeviceBoundary.initial(ctx, lay[-1].right, ham.mpo(L - 1).col_dim)
+    for k in range(L - 1, mid + 1, -1):
+        m = DeviceMPS(ctx, lay[k], data[k])
+        R = rstep(ctx, DeviceMPO(ctx, ham.mpo(k)), R, m, m)
+    Lb = DeviceBoundary.initial(ctx, lay[0].left, ham.mpo(0).row_dim)
+    for k in range(mid):
+        m = DeviceMPS(ctx, lay[k], data[k])
+        Lb = lstep(ctx, DeviceMPO(ctx, ham.mpo(k)), Lb, m, m)
+    lay2, psi2 = merge_two(lay[mid], data[mid], lay[mid + 1], data[mid + 1], ham.phys(True), twosite_offsets(phys))
+    return (DeviceMPO(ctx, ham.mpo(mid, True)), Lb, R, lay2), psi2 / np.linalg.norm(psi2)
+
+
+def batch_bench(args, ctx):
+    """Configuration 4: independent instances (orbitals ~ U{10..20}, M from {128, 256, 512}, sizes
+    from PCG64(1004), integrals of instance i from PCG64(1004 + 1 + i)), each at its middle
+    two-site problem.  Times, with psi resident in HBM: every instance's own H*psi one after the
+    other (sum over instances) and one H*psi of the batch plan over all instances (FP64 and FP32
+    contraction); the batch's output is checked against the instances' own; and the Davidson of
+    every instance (8 iterations, tol 1e-8) one after the other vs the lock-step Davidson over the
+    batch (one batched H*psi per iteration)."""
+    import ctypes as C
+
+    import torch
+    from qcmaquis_amd import abi
+    from qcmaquis_amd.batch import lockstep_davidson
+    from qcmaquis_amd.engine import HpsiPlan, HpsiPlanBatch
+    rng = np.random.default_rng(1004)
+    shapes = [(int(rng.integers(10, 21)), int(rng.choice([128, 256, 512]))) for _ in range(args.instances)]
+    items, psis, recs = [], [], []
+    sms = (C.c_float * 4)()
+    bytes_plans = 0
+    free0, total_hbm = torch.cuda.mem_get_info()
+    t_all = time.perf_counter()
+    for i, (L, M) in enumerate(shapes):
+        t0 = time.perf_counter()
+        item, psi0 = batch_instance(ctx, L, M, 1004 + 1 + i)
+        tb = time.perf_counter() - t0
+        plan = HpsiPlan(ctx, *item)
+        st = plan.stats()
+        x = torch.from_numpy(psi0).cuda()
+        y = torch.empty_like(x)
+        for _ in range(args.warmup):
+            plan.apply_device(x.data_ptr(), y.data_ptr())
+        ctx.sync()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.apply_device(x.data_ptr(), y.data_ptr())
+        ctx.sync()
+        ms = (time.perf_counter() - t1) / args.steps * 1e3
+        stage = np.zeros(4)
+        for _ in range(3):
+            ctx.check(ctx.lib.mq_hpsi_timed(plan.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), 0, sms))
+            stage += np.array(list(sms)) / 3
+        t2 = time.perf_counter()
+        e, _, dst = plan.davidson(psi0, tol=1e-8, max_iter=8)
+        dav = time.perf_counter() - t2
+        bytes_plans += 8 * (st["t_elements"] + st["s_elements"] + st["l_panel_elements"])
+        recs.append({"L": L, "M": M, "psi": int(item[3].n), "flops": int(st["flops_total"]), "ms": ms,
+                     "stage_ms": [round(v, 4) for v in stage], "n_terms": int(st["n_terms"]),
+                     "n_lines": int(st["n_lines"]), "t_elements": int(st["t_elements"]),
+                     "s_elements": int(st["s_elements"]), "union_elements": int(st["union_elements"]),
+                     "n_t_tasks": int(st["n_t_tasks"]), "small_lines": int(st["small_lines"]),
+                     "small_terms": int(st["small_terms"]), "davidson_s": dav, "e": e, "out": y.cpu().numpy()})
+        items.append(item)
+        psis.append(psi0)
+        plan.close()
+        del x, y
+        free, _ = torch.cuda.mem_get_info()
+        log("instance %d: L=%d M=%d psi %d, %.2f GFLOP, H*psi %.3f ms (stages %s), %d terms, davidson %.3f s "
+            "(built in %.1f s), HBM used %.1f GB" % (i, L, M, item[3].n, st["flops_total"] / 1e9, ms,
+                                                     recs[-1]["stage_ms"], st["n_terms"], dav, tb,
+                                                     (free0 - free) / 1e9))
+        if (free0 - free) + 1.3 * bytes_plans > 0.8 * free0:
+            log("HBM: stopping at %d instances (boundaries + batch buffers would exceed 80%% of free HBM)" % (i + 1))
+            break
+    n = len(items)
+    t0 = time.perf_counter()
+    batch = HpsiPlanBatch(ctx, items)
+    build = time.perf_counter() - t0
+    bst = batch.stats()
+    x = torch.from_numpy(np.concatenate(psis)).cuda()
+    y = torch.empty_like(x)
+    timing = {}
+    for name, prec in (("fp64", abi.MQ_PREC_FP64), ("mixed_fp32", abi.MQ_PREC_FP32)):
+        for _ in range(args.warmup):
+            batch.apply_device(x.data_ptr(), y.data_ptr(), prec)
+        ctx.sync()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.apply_device(x.data_ptr(), y.data_ptr(), prec)
+        ctx.sync()
+        timing[name] = (time.perf_counter() - t1) / args.steps * 1e3
+        if name == "fp64":
+            # the batch may split its GEMMs along K differently from the instances alone (the
+            # split depends on the number of output tiles), so the sums can differ in rounding
+            got = batch.split(y.cpu().numpy())
+            err = max(float(np.linalg.norm(g - r["out"]) / np.linalg.norm(r["out"])) for g, r in zip(got, recs))
+        bstage = np.zeros(4)
+        for _ in range(3):
+            ctx.check(ctx.lib.mq_hpsi_timed(batch.h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), prec, sms))
+            bstage += np.array(list(sms)) / 3
+        timing[name + "_stages"] = [round(v, 4) for v in bstage]
+    t1 = time.perf_counter()
+    e_ls, _, its, n_hpsi = lockstep_davidson(batch, psis, tol=1e-8, max_iter=8)
+    lock_s = time.perf_counter() - t1
+    flops = sum(r["flops"] for r in recs)
+    assert flops == bst["flops_total"]
+    seq_ms = sum(r["ms"] for r in recs)
+    seq_dav = sum(r["davidson_s"] for r in recs)
+    de = float(max(abs(a - r["e"]) for a, r in zip(e_ls, recs)))
+    log("batch of %d: plan %.1f s, H*psi %.3f ms (sum of instances %.3f ms), FP32 %.3f ms; stages %s / %s; "
+        "largest relative difference vs the instances %.2e; davidson lock-step %.3f s vs %.3f s one after the "
+        "other, max |dE| %.2e" % (n, build, timing["fp64"], seq_ms, timing["mixed_fp32"], timing["fp64_stages"],
+                                  timing["mixed_fp32_stages"], err, lock_s, seq_dav, de))
+    value = flops / (timing["fp64"] * 1e-3) / 1e12
+    line = {
+        "metric": METRIC, "value": round(value, 4), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(timing["fp64"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded integrals, random MPS)",
+        "config": {
+            "workload": "configuration 4: %d independent instances (orbitals U{10..20}, half filling, TwoU1, "
+                        "M from {128, 256, 512}), middle two-site problem of each, one batched H*psi" % n,
+            "step": "one H*psi over every instance (one batch plan), inputs resident in HBM",
+            "instances": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items() if k != "out"}
+                          for r in recs],
+            "flops_per_step_reference_model": flops, "batch_plan_seconds": round(build, 2),
+            "sum_of_instance_hpsi_ms": round(seq_ms, 4),
+            "batch_speedup_vs_one_after_the_other": round(seq_ms / timing["fp64"], 3),
+            "mixed_fp32": {"ms_per_step": round(timing["mixed_fp32"], 4),
+                           "tflops": round(flops / (timing["mixed_fp32"] * 1e-3) / 1e12, 4)},
+            "stage_ms_fp64": timing["fp64_stages"], "stage_ms_fp32": timing["mixed_fp32_stages"],
+            "sum_of_instance_stage_ms": [round(sum(r["stage_ms"][k] for r in recs), 4) for k in range(4)],
+            "max_rel_difference_vs_instances": err,
+            "davidson": {"lockstep_seconds": round(lock_s, 3), "one_after_the_other_seconds": round(seq_dav, 3),
+                         "batched_hpsi_calls": int(n_hpsi), "max_abs_delta_e": de,
+                         "note": "8 iterations, tol 1e-8; lock-step = host subspace algebra around one batched "
+                                 "H*psi (host vectors) per iteration; one after the other = each instance's "
+                                 "device Davidson (mq_davidson)"},
+            "hbm_total_gb": round(total_hbm / 1e9, 1),
+            "seconds_building_instances": round(time.perf_counter() - t_all, 1)},
+    }
+    print(json.dumps(line), flush=True)
 
 
 class ProfilerRegion:
@@ -204,7 +383,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="qc", choices=["qc", "synthetic"])
+    ap.add_argument("--workload", default="qc", choices=["qc", "synthetic", "batch"])
+    ap.add_argument("--instances", type=int, default=64, help="instances of the batch workload")
     ap.add_argument("--warm-sweeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dump-bonds", default=None, help="write the MPS bond indices of the workload (JSON)")
@@ -234,6 +414,10 @@ def main():
         print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}), flush=True)
         return
     ctx = Context(0, abi.MQ_SYM_TWOU1)
+    if args.workload == "batch":
+        batch_bench(args, ctx)
+        ctx.close()
+        return
     t0 = time.perf_counter()
     w = qc_workload(ctx, warm_sweeps=args.warm_sweeps, dump_bonds=args.dump_bonds) if args.workload == "qc" else synthetic_workload(ctx)
     log("workload built in %.1f s: %s" % (time.perf_counter() - t0, w["desc"]))

@@ -55,6 +55,12 @@ extern "C" {
 /* context options (mq_ctx_set_option) */
 #define MQ_OPT_PLAIN_TABLES 1  /* value 1: H*psi plans made afterwards also keep the per-task tables
                                   that MQ_PREC_FP64_PLAIN runs (default 0) */
+#define MQ_OPT_SMALL_TILE_MAX 2 /* value 0..256: H*psi plans made afterwards sum the MPO terms of S
+                                  tiles of at most this many elements term by term, one lane per
+                                  element, instead of in 16-tile blocks on the matrix cores (0: every
+                                  tile in blocks); MQ_SMALL_TILE_AUTO (default): per site problem,
+                                  64 when its mean S tile has at most 16 elements, else 0 */
+#define MQ_SMALL_TILE_AUTO (-1)
 
 /* MPS pairing (reference: MPSTensor::make_left_paired / make_right_paired, mpstensor.hpp:211-237) */
 #define MQ_RIGHT_PAIRED 0      /* blocks per left charge:  ls x (sum_sigma d_sigma * rs(lc+sigma)) */
@@ -142,6 +148,9 @@ typedef struct {
     double   seconds_tables;          /* host: schedule tables built and checked (part of build_seconds) */
     double   seconds_blocks;          /* host: term-sum block tables (part of build_seconds) */
     double   seconds_plain;           /* per-task tables of MQ_PREC_FP64_PLAIN kept (part of build_seconds) */
+    uint64_t small_lines;             /* S tiles summed term by term (MQ_OPT_SMALL_TILE_MAX) */
+    uint64_t small_terms;             /* their terms */
+    uint64_t small_gathered;          /* their T elements read: sum over terms of the tile elements */
 } mq_hpsi_stats_t;
 
 typedef struct {

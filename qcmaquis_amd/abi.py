@@ -21,6 +21,8 @@ MQ_PREC_FP32 = 1
 MQ_PREC_BF16X3 = 2
 MQ_PREC_FP64_PLAIN = 3
 MQ_OPT_PLAIN_TABLES = 1
+MQ_OPT_SMALL_TILE_MAX = 2
+MQ_SMALL_TILE_AUTO = -1
 MQ_RIGHT_PAIRED = 0
 MQ_LEFT_PAIRED = 1
 MQ_NO_PARTNER = 0xFFFFFFFF
@@ -99,6 +101,9 @@ class HpsiStats(C.Structure):
         ("seconds_tables", C.c_double),
         ("seconds_blocks", C.c_double),
         ("seconds_plain", C.c_double),
+        ("small_lines", C.c_uint64),
+        ("small_terms", C.c_uint64),
+        ("small_gathered", C.c_uint64),
     ]
 
 

@@ -30,6 +30,7 @@ struct mq_ctx_s {
     std::string err;
     double* dwork = nullptr;  // DOT_BLOCKS partials + 64 scalars
     bool plain_tables = false;         // MQ_OPT_PLAIN_TABLES
+    int32_t small_tile_max = MQ_SMALL_TILE_AUTO;  // MQ_OPT_SMALL_TILE_MAX
     rocblas_handle blas = nullptr;     // library GEMMs of MQ_PREC_FP64_PLAIN, made on first use
 };
 
@@ -171,6 +172,11 @@ struct SAsmStage {
     int32_t* llds = nullptr;
     int64_t n_work = 0;
     uint64_t n_alpha = 0;
+    // small-tile lines
+    SSmallWork* swork = nullptr;
+    SLine* slines = nullptr;
+    STerm* sterms = nullptr;
+    int64_t n_swork = 0;
     void upload(SBlockTable const& B, hipStream_t s) {
         blocks = dev_upload(B.blocks, s);
         work = dev_upload(B.work, s);
@@ -180,9 +186,15 @@ struct SAsmStage {
         llds = dev_upload(B.llds, s);
         n_work = int64_t(B.work.size());
         n_alpha = B.ualpha.size();
+        swork = dev_upload(B.small_work, s);
+        slines = dev_upload(B.small_lines, s);
+        sterms = dev_upload(B.small_terms, s);
+        n_swork = int64_t(B.small_work.size());
     }
     void run(const double* T, double* S, hipStream_t s) const {
-        hipcheck(launch_sblocks<double>(blocks, work, n_work, utile, ualpha, lsoff, llds, T, S, s), "S-assembly");
+        if (n_work)
+            hipcheck(launch_sblocks<double>(blocks, work, n_work, utile, ualpha, lsoff, llds, T, S, s), "S-assembly");
+        if (n_swork) hipcheck(launch_small_lines<double>(swork, n_swork, slines, sterms, T, S, s), "S-assembly");
     }
     void ensure32(hipStream_t s) {
         if (!ualpha32) {
@@ -192,14 +204,17 @@ struct SAsmStage {
     }
     void run32(const float* T, float* S, hipStream_t s) {
         ensure32(s);
-        hipcheck(launch_sblocks<float>(blocks, work, n_work, utile, ualpha32, lsoff, llds, T, S, s), "S-assembly");
+        if (n_work)
+            hipcheck(launch_sblocks<float>(blocks, work, n_work, utile, ualpha32, lsoff, llds, T, S, s), "S-assembly");
+        if (n_swork) hipcheck(launch_small_lines<float>(swork, n_swork, slines, sterms, T, S, s), "S-assembly");
     }
     void release() {
         dev_free(blocks); dev_free(work); dev_free(utile); dev_free(ualpha); dev_free(ualpha32); dev_free(lsoff);
-        dev_free(llds);
+        dev_free(llds); dev_free(swork); dev_free(slines); dev_free(sterms);
         blocks = nullptr; work = nullptr; utile = nullptr; ualpha = nullptr; ualpha32 = nullptr; lsoff = nullptr;
-        llds = nullptr;
+        llds = nullptr; swork = nullptr; slines = nullptr; sterms = nullptr;
         n_work = 0;
+        n_swork = 0;
     }
 };
 
@@ -245,6 +260,7 @@ struct mq_hpsi_plan_s {
     int n_instances = 1;         // independent site problems in the plan
     double* rown = nullptr;      // right boundaries copied into the plan (batch plans)
     double build_seconds = 0, seconds_tables = 0, seconds_blocks = 0, seconds_plain = 0;
+    uint64_t small_lines = 0, small_terms = 0, small_gathered = 0;
     // per-task tables of MQ_PREC_FP64_PLAIN (MQ_OPT_PLAIN_TABLES)
     struct Plain {
         bool ok = false;
@@ -326,8 +342,15 @@ const char* mq_last_error(mq_ctx_t ctx) { return ctx ? ctx->err.c_str() : "null 
 
 int mq_ctx_set_option(mq_ctx_t ctx, int option, int value) {
     return guarded(ctx, [&] {
-        require(option == MQ_OPT_PLAIN_TABLES, "mq_ctx_set_option: unknown option");
-        ctx->plain_tables = value != 0;
+        if (option == MQ_OPT_PLAIN_TABLES) {
+            ctx->plain_tables = value != 0;
+        } else if (option == MQ_OPT_SMALL_TILE_MAX) {
+            require(value == MQ_SMALL_TILE_AUTO || (value >= 0 && value <= SMALL_LINE_LANES),
+                    "mq_ctx_set_option: MQ_OPT_SMALL_TILE_MAX is MQ_SMALL_TILE_AUTO or 0..256");
+            ctx->small_tile_max = value;
+        } else {
+            throw Error(MQ_ERR_INVALID, "mq_ctx_set_option: unknown option");
+        }
     });
 }
 
@@ -605,7 +628,7 @@ struct LpanelPart {
 // device side of an H*psi plan whose host tables are in p->host (p->psi_n, p->rdata set): checks,
 // term-sum blocks, buffers, uploads, L panels; frees the host term tables
 static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart> const& lparts, uint64_t r_n,
-                        std::chrono::steady_clock::time_point t0) {
+                        std::vector<size_t> const& line_ranges, std::chrono::steady_clock::time_point t0) {
     auto seconds_since = [](std::chrono::steady_clock::time_point t) {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
     };
@@ -622,9 +645,19 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
     check_lines(H.lines, H.terms, H.t_elems, H.s_elems);
     p->seconds_tables = seconds_since(t0);
     auto t1 = std::chrono::steady_clock::now();
+    // small-tile lines, decided per site problem (line_ranges: the first line of each instance and
+    // the end)
+    for (size_t i = 0; i + 1 < line_ranges.size(); ++i) {
+        const size_t l0 = line_ranges[i], l1 = line_ranges[i + 1];
+        mark_small_lines(H.lines, l0, l1, ctx->small_tile_max == MQ_SMALL_TILE_AUTO
+                                              ? auto_small_tile_max(H.lines, l0, l1) : ctx->small_tile_max);
+    }
     SBlockTable SB = build_sblocks(H.lines, H.terms);
     check_sblocks(SB, H.t_elems, H.s_elems);
     p->seconds_blocks = seconds_since(t1);
+    p->small_lines = SB.small_lines.size();
+    p->small_terms = SB.small_terms.size();
+    p->small_gathered = SB.small_gathered;
     for (auto const& part : lparts)
         for (auto const& j : part.jobs)
             require(j.src >= 0 && j.dst >= 0 && uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= part.n &&
@@ -685,7 +718,8 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
         std::unique_ptr<mq_hpsi_plan_s> guard(p);
         p->psi_n = p->host.mps.elements();
         p->rdata = right->data;
-        finish_plan(ctx, p, {LpanelPart{left->data, left->n, p->host.lpanel_jobs}}, right->n, t0);
+        finish_plan(ctx, p, {LpanelPart{left->data, left->n, p->host.lpanel_jobs}}, right->n,
+                    {0, p->host.lines.size()}, t0);
         *out = guard.release();
     });
 }
@@ -702,20 +736,19 @@ int mq_hpsi_plan_create_batch(mq_ctx_t ctx, int n, const mq_mpo_t* mpos, const m
         for (int i = 0; i < n; ++i)
             require(mpos[i] && lefts[i] && rights[i], "mq_hpsi_plan_create_batch: null instance");
         auto t0 = std::chrono::steady_clock::now();
-        std::vector<HpsiPlanHost> parts(static_cast<size_t>(n));
-        for (int i = 0; i < n; ++i)
-            parts[size_t(i)] = build_hpsi_plan(ctx->sym, mpos[i]->mpo, lefts[i]->idx, rights[i]->idx,
-                                               index_from(&left_is[i]), index_from(&phys_is[i]),
-                                               index_from(&right_is[i]));
         auto* p = new mq_hpsi_plan_s{ctx, HpsiPlanHost{}};
         std::unique_ptr<mq_hpsi_plan_s> guard(p);
         HpsiPlanHost& H = p->host;
-        H.mps = parts[0].mps;
         uint64_t r_n = 0;
         std::vector<LpanelPart> lparts;
+        std::vector<size_t> line_ranges{0};
         int64_t shift[BASE_COUNT] = {0};
         for (int i = 0; i < n; ++i) {
-            HpsiPlanHost& Q = parts[size_t(i)];
+            // built and merged one at a time: the host holds the merged tables and one instance's
+            HpsiPlanHost Q = build_hpsi_plan(ctx->sym, mpos[i]->mpo, lefts[i]->idx, rights[i]->idx,
+                                             index_from(&left_is[i]), index_from(&phys_is[i]),
+                                             index_from(&right_is[i]));
+            if (i == 0) H.mps = Q.mps;
             shift[BASE_PSI_IN] = shift[BASE_PSI_OUT] = int64_t(p->psi_n);
             shift[BASE_T] = int64_t(H.t_elems);
             shift[BASE_S] = int64_t(H.s_elems);
@@ -766,7 +799,7 @@ int mq_hpsi_plan_create_batch(mq_ctx_t ctx, int n, const mq_mpo_t* mpos, const m
             H.flops.n_terms += Q.flops.n_terms;
             p->psi_n += Q.mps.elements();
             r_n += rights[i]->n;
-            Q = HpsiPlanHost{};  // host tables merged: release them
+            line_ranges.push_back(H.lines.size());
         }
         p->n_instances = n;
         // the right boundaries of all instances in one device buffer owned by the plan
@@ -780,7 +813,7 @@ int mq_hpsi_plan_create_batch(mq_ctx_t ctx, int n, const mq_mpo_t* mpos, const m
         }
         hipcheck(hipStreamSynchronize(ctx->stream), "batch right boundaries");
         p->rdata = p->rown;
-        finish_plan(ctx, p, lparts, r_n, t0);
+        finish_plan(ctx, p, lparts, r_n, line_ranges, t0);
         *out = guard.release();
     });
 }
@@ -809,6 +842,9 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
     s->seconds_blocks = p->seconds_blocks;
     s->union_elements = p->union_elements;
     s->seconds_plain = p->seconds_plain;
+    s->small_lines = p->small_lines;
+    s->small_terms = p->small_terms;
+    s->small_gathered = p->small_gathered;
     return MQ_OK;
 }
 

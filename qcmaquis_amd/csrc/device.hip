@@ -290,6 +290,35 @@ __global__ __launch_bounds__(64 * SB_WAVES) void sblock_kernel(const SBlock* __r
     }
 }
 
+// Small-tile MPO-term sum: a work item is up to 256 >> lg lines of 2^lg lanes each (tile elements
+// <= 2^lg); lane e of a line sums alpha_k * T[t_off_k + e] over the line's terms in term order,
+// four terms' loads ahead of their multiply-adds, and writes element e of the line's S tile.  The
+// lanes of a line read the same term entries; no LDS and no barrier, so lanes past their line's
+// tile simply return.
+template <class T>
+__global__ __launch_bounds__(SMALL_LINE_LANES) void small_lines_kernel(const SSmallWork* __restrict__ work,
+                                                                       const SLine* __restrict__ lines,
+                                                                       const STerm* __restrict__ terms,
+                                                                       const T* __restrict__ X, T* __restrict__ S) {
+    const SSmallWork w = work[blockIdx.x];
+    const int li = int(threadIdx.x) >> w.lg, e = int(threadIdx.x) & ((1 << w.lg) - 1);
+    if (li >= w.n) return;
+    const SLine l = lines[w.line0 + li];
+    if (e >= l.rows * l.cols) return;
+    T acc = T(0);
+    int k = l.t0;
+    for (; k + 4 <= l.t1; k += 4) {
+        const STerm a = terms[k], b = terms[k + 1], c = terms[k + 2], d = terms[k + 3];
+        const T xa = X[a.t_off + e], xb = X[b.t_off + e], xc = X[c.t_off + e], xd = X[d.t_off + e];
+        acc += T(a.alpha) * xa;
+        acc += T(b.alpha) * xb;
+        acc += T(c.alpha) * xc;
+        acc += T(d.alpha) * xd;
+    }
+    for (; k < l.t1; ++k) acc += T(terms[k].alpha) * X[terms[k].t_off + e];
+    S[l.s_off + e % l.rows + int64_t(e / l.rows) * l.lds] = acc;
+}
+
 // One S line (rows x cols tile at s_off, leading dimension lds) per workgroup; element e of the
 // tile is the sum over the line's terms of alpha * T[t_off + e], in term order (reference:
 // compute_s_stackedv, gpu_dgemm_ddot.cu:282-359, one gather per term).  Lines without terms
@@ -484,6 +513,21 @@ template hipError_t launch_sblocks<double>(const SBlock*, const SWork*, int64_t,
                                            const int64_t*, const int32_t*, const double*, double*, hipStream_t);
 template hipError_t launch_sblocks<float>(const SBlock*, const SWork*, int64_t, const int64_t*, const float*,
                                           const int64_t*, const int32_t*, const float*, float*, hipStream_t);
+
+template <class T>
+hipError_t launch_small_lines(const SSmallWork* work, int64_t n_work, const SLine* lines, const STerm* terms,
+                              const T* X, T* S, hipStream_t stream) {
+    for (int64_t off = 0; off < n_work; off += 0x7fffffff) {
+        int64_t cnt = n_work - off < 0x7fffffff ? n_work - off : 0x7fffffff;
+        hipLaunchKernelGGL(small_lines_kernel<T>, dim3(unsigned(cnt)), dim3(SMALL_LINE_LANES), 0, stream, work + off,
+                           lines, terms, X, S);
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_small_lines<double>(const SSmallWork*, int64_t, const SLine*, const STerm*, const double*,
+                                               double*, hipStream_t);
+template hipError_t launch_small_lines<float>(const SSmallWork*, int64_t, const SLine*, const STerm*, const float*,
+                                              float*, hipStream_t);
 
 hipError_t launch_convert(const double* src, float* dst, uint64_t n, hipStream_t stream) {
     if (!n) return hipSuccess;

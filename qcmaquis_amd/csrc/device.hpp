@@ -32,6 +32,11 @@ template <class T>
 hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_work, const int64_t* utile,
                           const T* ualpha, const int64_t* lsoff, const int32_t* llds, const T* X, T* S,
                           hipStream_t stream);
+// small-tile MPO-term sum (SBlockTable small_* tables uploaded; T and S of type T, coefficients
+// FP64, rounded to T): writes the S tiles of the small lines
+template <class T>
+hipError_t launch_small_lines(const SSmallWork* work, int64_t n_work, const SLine* lines, const STerm* terms,
+                              const T* X, T* S, hipStream_t stream);
 // reference-style MPO-term sum: one workgroup per S line, every element gathers all of the line's
 // terms (the per-term formulation the block-form kernel replaces)
 hipError_t launch_slines(const SLine* lines, int64_t n_lines, const STerm* terms, const double* T, double* S,

@@ -1160,8 +1160,69 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
 }
 
 // ---- block form of the MPO-term sum ------------------------------------------------------------
+// small-tile lines: ordered by lanes per line (the power of two >= rows * cols), then by term
+// count (longest first), with their terms copied in that order
+template <class Pred>
+static void build_small_lines(SBlockTable& B, HostTable<SLine> const& lines, HostTable<STerm> const& terms,
+                              Pred is_small) {
+    struct Key { int32_t lg, nt; uint32_t i; };
+    std::vector<Key> keys;
+    for (uint32_t i = 0; i < lines.size(); ++i) {
+        SLine const& l = lines[i];
+        if (!is_small(l)) continue;
+        require(int64_t(l.rows) * l.cols <= SMALL_LINE_LANES, "S blocks: small-tile line wider than 256 elements");
+        int32_t lg = 0;
+        while ((1 << lg) < l.rows * l.cols) ++lg;
+        keys.push_back(Key{lg, l.t1 - l.t0, i});
+    }
+    std::stable_sort(keys.begin(), keys.end(), [](Key const& a, Key const& b) {
+        return a.lg != b.lg ? a.lg < b.lg : a.nt > b.nt;
+    });
+    std::vector<int64_t> t_start(keys.size() + 1, 0);
+    for (size_t q = 0; q < keys.size(); ++q) t_start[q + 1] = t_start[q] + keys[q].nt;
+    require(t_start.back() < int64_t(INT32_MAX), "S blocks: more than 2^31 small-tile terms");
+    B.small_lines.resize(keys.size());
+    B.small_terms.resize(size_t(t_start.back()));
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t q = 0; q < int64_t(keys.size()); ++q) {
+        SLine l = lines[keys[size_t(q)].i];
+        std::copy(terms.begin() + l.t0, terms.begin() + l.t1, B.small_terms.begin() + t_start[size_t(q)]);
+        l.t0 = int32_t(t_start[size_t(q)]);
+        l.t1 = int32_t(t_start[size_t(q) + 1]);
+        B.small_lines[size_t(q)] = l;
+    }
+    for (size_t q = 0; q < keys.size();) {
+        const int32_t lg = keys[q].lg, per = SMALL_LINE_LANES >> lg;
+        int32_t n = 0;
+        while (q + size_t(n) < keys.size() && keys[q + size_t(n)].lg == lg && n < per) ++n;
+        B.small_work.push_back(SSmallWork{int32_t(q), n, lg, 0});
+        q += size_t(n);
+    }
+    for (SLine const& l : B.small_lines) B.small_gathered += uint64_t(l.t1 - l.t0) * uint64_t(l.rows) * l.cols;
+}
+
+int32_t auto_small_tile_max(HostTable<SLine> const& lines, size_t l0, size_t l1) {
+    uint64_t e = 0;
+    for (size_t i = l0; i < l1; ++i) e += uint64_t(lines[i].rows) * uint64_t(lines[i].cols);
+    return l1 > l0 && e <= 16 * uint64_t(l1 - l0) ? 64 : 0;
+}
+
+void mark_small_lines(HostTable<SLine>& lines, size_t l0, size_t l1, int32_t limit) {
+    require(limit >= 0 && limit <= SMALL_LINE_LANES, "S blocks: small-tile limit outside 0..256");
+    for (size_t i = l0; i < l1; ++i) {
+        SLine& l = lines[i];
+        const int64_t E = int64_t(l.rows) * l.cols;
+        l.flags = (l.flags & ~SLINE_SMALL) | (E > 0 && E <= limit ? SLINE_SMALL : 0);
+    }
+}
+
 SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const& terms) {
     SBlockTable B;
+    auto is_small = [&](SLine const& l) {
+        return (l.flags & SLINE_SMALL) && l.rows > 0 && l.cols > 0;
+    };
+    bool any_small = false;
+    for (SLine const& l : lines) any_small |= is_small(l);
     const int64_t nl = int64_t(lines.size());
     // first T tile of every line (lines without terms, i.e. zero tiles, sort last)
     std::vector<int64_t> tmin(lines.size(), INT64_MAX);
@@ -1176,7 +1237,8 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
     // share most of their tiles), longer lines first
     std::map<std::pair<int32_t, int32_t>, std::vector<uint32_t>> by_shape;
     for (uint32_t i = 0; i < lines.size(); ++i)
-        if (lines[i].rows > 0 && lines[i].cols > 0) by_shape[{lines[i].rows, lines[i].cols}].push_back(i);
+        if (lines[i].rows > 0 && lines[i].cols > 0 && !is_small(lines[i]))
+            by_shape[{lines[i].rows, lines[i].cols}].push_back(i);
     std::vector<std::vector<uint32_t>*> shapes;
     for (auto& kv : by_shape) shapes.push_back(&kv.second);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -1283,6 +1345,7 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
         B.gathered += uint64_t(blk.nu) * uint64_t(blk.E);
     }
     B.mac = B.gathered * SBLOCK_LINES;
+    if (any_small) build_small_lines(B, lines, terms, is_small);
     return B;
 }
 
@@ -1314,10 +1377,39 @@ void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size) {
     for (SWork const& w : B.work)
         if (!(w.blk >= 0 && size_t(w.blk) < B.blocks.size() && w.n0 >= 0 && w.n0 < B.blocks[size_t(w.blk)].E))
             bad = std::max(bad, 4);
+    // small-tile tables: 5 malformed line, 6 term tile outside T, 7 malformed work item
+    const int64_t nsl = int64_t(B.small_lines.size());
+#pragma omp parallel for schedule(dynamic, 1024) reduction(max : bad)
+    for (int64_t i = 0; i < nsl; ++i) {
+        SLine const& l = B.small_lines[size_t(i)];
+        const int64_t E = int64_t(l.rows) * l.cols;
+        if (!(l.rows > 0 && l.cols > 0 && E <= SMALL_LINE_LANES && l.lds >= l.rows && l.s_off >= 0 &&
+              uint64_t(l.s_off + (l.rows - 1) + int64_t(l.cols - 1) * l.lds) < s_size && l.t0 >= 0 && l.t0 <= l.t1 &&
+              uint64_t(l.t1) <= B.small_terms.size())) {
+            bad = std::max(bad, 5);
+            continue;
+        }
+        for (int32_t k = l.t0; k < l.t1; ++k) {
+            int64_t o = B.small_terms[size_t(k)].t_off;
+            if (!(o >= 0 && uint64_t(o) + uint64_t(E) <= t_size)) bad = std::max(bad, 6);
+        }
+    }
+    for (SSmallWork const& w : B.small_work) {
+        bool ok = w.lg >= 0 && (1 << w.lg) <= SMALL_LINE_LANES && w.n > 0 && w.n <= (SMALL_LINE_LANES >> w.lg) &&
+                  w.line0 >= 0 && int64_t(w.line0) + w.n <= nsl;
+        for (int32_t q = 0; ok && q < w.n; ++q) {
+            SLine const& l = B.small_lines[size_t(w.line0 + q)];
+            ok = int64_t(l.rows) * l.cols <= (int64_t(1) << w.lg);
+        }
+        if (!ok) bad = std::max(bad, 7);
+    }
     require(bad != 1, "S blocks: malformed block");
     require(bad != 2, "S blocks: tile outside the T buffer");
     require(bad != 3, "S blocks: line outside the S buffer");
     require(bad != 4, "S blocks: malformed work item");
+    require(bad != 5, "S blocks: malformed small-tile line");
+    require(bad != 6, "S blocks: small-tile term outside the T buffer");
+    require(bad != 7, "S blocks: malformed small-tile work item");
 }
 
 // ---- grouped GEMM schedule ---------------------------------------------------------------------

@@ -78,6 +78,20 @@ struct GemmRow {
 };
 static_assert(sizeof(GemmRow) == 24, "GemmRow layout");
 
+// S-assembly output tile (rows x cols at s_off with leading dimension lds) and its term range;
+// flags: SLINE_SMALL = summed by the small-tile term sum, not in a 16-line block.
+constexpr int32_t SLINE_SMALL = 1;
+struct SLine {
+    int64_t s_off;
+    int32_t lds, rows, cols, t0;
+    int32_t t1, flags;
+};
+static_assert(sizeof(SLine) == 32, "SLine layout");
+struct STerm {
+    double alpha;
+    int64_t t_off;  // element offset of a contiguous rows x cols tile in the T buffer
+};
+
 // Block form of the MPO-term sum for the matrix cores.  Lines of equal tile shape are grouped in
 // blocks of SBLOCK_LINES; a block stores the sorted union of the T tiles its lines read and a
 // dense SBLOCK_LINES x nu coefficient matrix (tile-major: alpha of line i for union
@@ -95,6 +109,14 @@ static_assert(sizeof(SBlock) == 32, "SBlock layout");
 struct SWork {
     int32_t blk, n0;          // block and first tile element of one workgroup
 };
+// Lines marked SLINE_SMALL (tiles of at most SMALL_LINE_LANES elements) are summed term by term:
+// each line gets a power-of-two group of lanes (one lane per tile element) and a workgroup of 256
+// lanes takes 256 / group consecutive lines of one group size (lines sorted by group size, then by
+// term count, so the lanes of a wave run loops of similar length).
+struct SSmallWork {
+    int32_t line0, n;         // first line (small line table) and number of lines
+    int32_t lg, pad;          // log2 of the lanes per line
+};
 struct SBlockTable {
     std::vector<SBlock> blocks;
     std::vector<SWork> work;
@@ -104,7 +126,14 @@ struct SBlockTable {
     std::vector<int32_t> llds;    // its leading dimension
     uint64_t gathered = 0;        // sum over blocks of nu * E (T elements read)
     uint64_t mac = 0;             // SBLOCK_LINES * gathered (matrix-core multiply-adds)
+    // small-tile lines: their own line and term tables (t0 / t1 index small_terms)
+    std::vector<SLine> small_lines;
+    HostTable<STerm> small_terms;
+    std::vector<SSmallWork> small_work;
+    uint64_t small_gathered = 0;  // sum over small lines of terms * tile elements
 };
+// lanes of a workgroup of the small-tile term sum; tiles up to this many elements can go there
+constexpr int SMALL_LINE_LANES = 256;
 // work items cover SBLOCK_ELEMS consecutive tile elements of one block
 constexpr int SBLOCK_ELEMS = 256;
 
@@ -147,20 +176,16 @@ struct GemmSchedule {
 GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t target_items = 4096,
                                  int32_t min_chunk = 256);
 
-// S-assembly output tile (rows x cols at s_off with leading dimension lds) and its term range.
-struct SLine {
-    int64_t s_off;
-    int32_t lds, rows, cols, t0;
-    int32_t t1, pad;
-};
-static_assert(sizeof(SLine) == 32, "SLine layout");
-struct STerm {
-    double alpha;
-    int64_t t_off;  // element offset of a contiguous rows x cols tile in the T buffer
-};
 
 // block table of a list of S lines (lines without terms become zero tiles)
+// lines marked SLINE_SMALL go to the small-tile tables, the others into 16-line blocks
 SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const& terms);
+// small-tile limit of the lines [l0, l1) of one site problem under the automatic rule: 64 when
+// their mean tile has at most 16 elements (then the 16-line blocks mostly hold tiles a few
+// elements wide, whose dense coefficient blocks cost more than the terms), else 0
+int32_t auto_small_tile_max(HostTable<SLine> const& lines, size_t l0, size_t l1);
+// marks the lines [l0, l1) whose tiles have 1 .. limit elements SLINE_SMALL (limit <= 256)
+void mark_small_lines(HostTable<SLine>& lines, size_t l0, size_t l1, int32_t limit);
 void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size);
 
 struct FlopStats {

@@ -30,7 +30,8 @@ def lib():
         P = C.POINTER
         dp = P(C.c_double)
         idx = P(abi.IndexDesc)
-        L.emu_hpsi.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, P(abi.BIndexDesc), dp, idx, idx, idx, dp, dp]
+        L.emu_hpsi.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, P(abi.BIndexDesc), dp, idx, idx, idx, dp, dp,
+                               C.c_int, P(C.c_uint64)]
         for f in (L.emu_lstep, L.emu_rstep):
             f.restype = C.c_void_p
             f.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, idx, idx, idx, dp, idx, idx, idx, dp, C.c_int]
@@ -45,16 +46,25 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def hpsi(sym, mpo, L, R, layout, psi):
+last_small_lines = 0
+
+
+def hpsi(sym, mpo, L, R, layout, psi, small_max=-1):
+    """H*psi through the product's tables; small_max: tiles of at most this many elements take the
+    small-tile term sum (MQ_OPT_SMALL_TILE_MAX; -1 = the automatic rule), their count is left in
+    last_small_lines"""
+    global last_small_lines
     psi = np.ascontiguousarray(psi, np.float64)
     out = np.zeros_like(psi)
     md, Ld, Rd = mpo.desc(), L.desc(), R.desc()
     li, pi, ri = layout.left.desc(), layout.phys.desc(), layout.right.desc()
     Ldat, Rdat = np.ascontiguousarray(L.data), np.ascontiguousarray(R.data)
+    ns = C.c_uint64()
     rc = lib().emu_hpsi(sym, C.byref(md), C.byref(Ld), _dp(Ldat), C.byref(Rd), _dp(Rdat), C.byref(li), C.byref(pi),
-                        C.byref(ri), _dp(psi), _dp(out))
+                        C.byref(ri), _dp(psi), _dp(out), int(small_max), C.byref(ns))
     if rc != 0:
         raise RuntimeError("emu_hpsi failed")
+    last_small_lines = ns.value
     return out
 
 

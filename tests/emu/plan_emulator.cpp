@@ -80,10 +80,33 @@ void run_gemms(std::vector<GemmDesc> const& G, std::vector<GemmRow> const& rows,
 // the MPO-term sum as the device runs it: the block table built from the lines, each work item
 // covering SBLOCK_ELEMS tile elements of one block (four waves of 16 * SBLOCK_ELEMS / 64); every
 // (line, element) of the table must be written exactly once
-void run_lines(HostTable<SLine> const& lines, HostTable<STerm> const& terms, const double* T, double* S,
-               uint64_t t_size, uint64_t s_size) {
+// (tiles of at most small_max elements: the small-tile work items, 2^lg lanes per line, lane e
+// summing element e over the line's terms)
+// Returns the number of small-tile lines.
+uint64_t run_lines(HostTable<SLine>& lines, HostTable<STerm> const& terms, const double* T, double* S,
+                   uint64_t t_size, uint64_t s_size, int32_t small_max = 0) {
+    mark_small_lines(lines, 0, lines.size(),
+                     small_max == MQ_SMALL_TILE_AUTO ? auto_small_tile_max(lines, 0, lines.size()) : small_max);
     SBlockTable B = build_sblocks(lines, terms);
     check_sblocks(B, t_size, s_size);
+    std::vector<char> sdone(B.small_lines.size(), 0);
+    uint64_t small_covered = 0;
+    for (SSmallWork const& w : B.small_work)
+        for (int lane = 0; lane < SMALL_LINE_LANES; ++lane) {
+            const int li = lane >> w.lg, e = lane & ((1 << w.lg) - 1);
+            if (li >= w.n) continue;
+            SLine const& l = B.small_lines[size_t(w.line0 + li)];
+            if (e >= l.rows * l.cols) continue;
+            double acc = 0.0;
+            for (int32_t k = l.t0; k < l.t1; ++k) acc += B.small_terms[size_t(k)].alpha * T[B.small_terms[size_t(k)].t_off + e];
+            S[l.s_off + e % l.rows + int64_t(e / l.rows) * l.lds] = acc;
+            if (e == 0) {
+                require(!sdone[size_t(w.line0 + li)], "emulated small lines: line written twice");
+                sdone[size_t(w.line0 + li)] = 1;
+            }
+            ++small_covered;
+        }
+    for (char d : sdone) require(d, "emulated small lines: line not written");
     std::vector<std::vector<char>> done(B.blocks.size());
     for (size_t i = 0; i < B.blocks.size(); ++i) done[i].assign(size_t(B.blocks[i].E) * SBLOCK_LINES, 0);
     const int per_wave = 16 * (SBLOCK_ELEMS / 64);
@@ -105,11 +128,12 @@ void run_lines(HostTable<SLine> const& lines, HostTable<STerm> const& terms, con
         for (int e = 0; e < B.blocks[i].E; ++e)
             for (int l = 0; l < B.blocks[i].nlines; ++l)
                 require(done[i][size_t(e) * SBLOCK_LINES + l], "emulated S blocks: element not written");
-    uint64_t covered = 0;
+    uint64_t covered = small_covered;
     for (auto const& b : B.blocks) covered += uint64_t(b.E) * b.nlines;
     uint64_t expected = 0;
     for (auto const& l : lines) expected += uint64_t(l.rows) * l.cols;
     require(covered == expected, "emulated S blocks: lines missing from the block table");
+    return B.small_lines.size();
 }
 
 void run_copy(std::vector<CopyJob> const& jobs, const double* src, double* dst) {
@@ -148,7 +172,7 @@ extern "C" {
 
 int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const double* Ldata, const mq_bindex_desc* Rd,
              const double* Rdata, const mq_index_desc* li, const mq_index_desc* pi, const mq_index_desc* ri,
-             const double* psi, double* out) {
+             const double* psi, double* out, int small_max, uint64_t* n_small_lines) {
     try {
         MPO mpo(*md);
         BoundaryIndex L = boundary_index_from(*Ld), R = boundary_index_from(*Rd);
@@ -177,7 +201,7 @@ int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const dou
         base[BASE_LPANEL] = Lp.data();
         base[BASE_BOUNDARY] = const_cast<double*>(Rdata);
         run_gemms(P.t_gemms, P.t_rows, base);
-        run_lines(P.lines, P.terms, T.data(), S.data(), P.t_elems, P.s_elems);
+        *n_small_lines = run_lines(P.lines, P.terms, T.data(), S.data(), P.t_elems, P.s_elems, small_max);
         run_gemms(P.c_gemms, {}, base);
         return 0;
     } catch (std::exception const& e) {

@@ -113,6 +113,37 @@ def test_fp32_hpsi(ctx2u1, ctxsu2, oracle_lib, sym):
         plan.close()
 
 
+@pytest.mark.parametrize("small_max", [0, 8, 256])
+def test_small_tile_limits(ctx2u1, oracle_lib, small_max):
+    """MPO-term sum with every S tile in 16-line blocks (0), split between blocks and the small-tile
+    term sum (8), or every tile of at most 256 elements term by term (256): FP64 to 1e-12 and FP32
+    to 1e-5 of the oracle at every site of a chain with sector dims 1..90"""
+    from qcmaquis_amd.synthetic import TWOU1_SITE, fuse_index
+    print(f"small-tile limit {small_max}", flush=True)
+    ch = make_chain(5, n_sites=4, total=(4, 4), max_m=90, D=6, phys=fuse_index(TWOU1_SITE, TWOU1_SITE), density=0.3)
+    mpos, L_dev, L_or, R_dev, R_or = _grow_both(ctx2u1, oracle_lib, ch)
+    rng = np.random.default_rng(6)
+    ctx2u1.set_option(abi.MQ_OPT_SMALL_TILE_MAX, small_max)
+    try:
+        n_small = 0
+        for k in range(len(ch.mpos)):
+            lay = ch.layouts[k]
+            plan = HpsiPlan(ctx2u1, mpos[k], L_dev[k], R_dev[k + 1], lay)
+            st = plan.stats()
+            n_small += st["small_lines"]
+            assert st["small_lines"] <= st["n_lines"]
+            psi = rng.normal(size=lay.n)
+            ref = oracle_lib.hpsi(ch.sym, ch.mpos[k], L_or[k], R_or[k + 1], lay, psi)
+            err, err32 = rel_err(plan.apply(psi), ref), rel_err(plan.apply(psi, abi.MQ_PREC_FP32), ref)
+            print(f"  site {k}: n={lay.n} lines {st['n_lines']} small {st['small_lines']} fp64 {err:.2e} "
+                  f"fp32 {err32:.2e}", flush=True)
+            assert err < TOL and err32 < 1e-5
+            plan.close()
+        assert (n_small == 0) == (small_max == 0)
+    finally:
+        ctx2u1.set_option(abi.MQ_OPT_SMALL_TILE_MAX, abi.MQ_SMALL_TILE_AUTO)
+
+
 @pytest.mark.parametrize("sym", [abi.MQ_SYM_TWOU1, abi.MQ_SYM_SU2U1])
 def test_plain_hpsi(ctx2u1, ctxsu2, oracle_lib, sym):
     """the reference-style launch structure (one rocBLAS dgemm per T task and per output block,

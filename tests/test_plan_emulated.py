@@ -39,11 +39,17 @@ def test_chain_emulated(oracle_lib, sym, seed, kw):
         same_index(Re[k], Ro[k])
         assert rel_err(Re[k].data, Ro[k].data) < 1e-12
     rng = np.random.default_rng(seed)
+    n_small = {}
     for k in range(n):
         psi = rng.normal(size=ch.layouts[k].n)
-        a = emu.hpsi(sym, ch.mpos[k], Lo[k], Ro[k + 1], ch.layouts[k], psi)
         b = oracle_lib.hpsi(sym, ch.mpos[k], Lo[k], Ro[k + 1], ch.layouts[k], psi)
-        assert rel_err(a, b) < 1e-12, k
+        # every tile in 16-line blocks, a mix of blocks and small-tile lines, the automatic rule,
+        # every tile term by term
+        for small_max in (0, 6, -1, 256):
+            a = emu.hpsi(sym, ch.mpos[k], Lo[k], Ro[k + 1], ch.layouts[k], psi, small_max)
+            assert rel_err(a, b) < 1e-12, (k, small_max)
+            n_small[small_max] = n_small.get(small_max, 0) + emu.last_small_lines
+    assert n_small[0] == 0 and 0 < n_small[6] < n_small[256]
 
 
 def test_nonsymmetric_left(oracle_lib):
