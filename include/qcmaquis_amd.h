@@ -267,6 +267,16 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t plan);
 int mq_davidson(mq_hpsi_plan_t plan, const double* psi0, int n_ortho, const double* const* ortho,
                 double tol, int max_iter, int precision, double* eval, double* psi_out,
                 mq_davidson_stats_t* stats);
+/* Lowest eigenpair of every instance of a batch plan (mq_hpsi_plan_create_batch), the iteration of
+ * mq_davidson run in lock step: one batched H*psi per iteration, the vector operations as
+ * segmented launches over all instances.  No counterpart in the reference (which solves one site
+ * problem at a time; the iteration is jacobi_davidson::calculate_eigenvalue, ietl/jacobi.h:386-474);
+ * for drivers of many independent problems.  psi0 / psi_out: the instances' vectors concatenated
+ * (host); evals, iterations: one entry per instance; stats->iterations counts lock-step iterations.
+ * No ortho vectors.  Precision MQ_PREC_FP64, or MQ_PREC_FP32 (FP32 contraction, FP64 refinement as
+ * in mq_davidson). */
+int mq_davidson_batch(mq_hpsi_plan_t plan, const double* psi0, double tol, int max_iter, int precision,
+                      double* evals, double* psi_out, int32_t* iterations, mq_davidson_stats_t* stats);
 
 /* ---- quantum-chemistry Hamiltonian MPO (host only; input producer of the path) -------------
  * reference: models/chem/2u1/model.hpp (create_terms), generate_mpo/tagged_mpo_maker_optim.hpp,

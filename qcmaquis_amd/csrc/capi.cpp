@@ -259,6 +259,8 @@ struct mq_hpsi_plan_s {
     uint64_t psi_n = 0;          // psi elements (all instances of a batch plan)
     int n_instances = 1;         // independent site problems in the plan
     double* rown = nullptr;      // right boundaries copied into the plan (batch plans)
+    std::vector<int64_t> inst_off;    // psi offset of each instance, and psi_n (batch plans)
+    int64_t* d_inst_off = nullptr;    // the same on the device
     double build_seconds = 0, seconds_tables = 0, seconds_blocks = 0, seconds_plain = 0;
     uint64_t small_lines = 0, small_terms = 0, small_gathered = 0;
     // per-task tables of MQ_PREC_FP64_PLAIN (MQ_OPT_PLAIN_TABLES)
@@ -731,8 +733,8 @@ int mq_hpsi_plan_create_batch(mq_ctx_t ctx, int n, const mq_mpo_t* mpos, const m
                               const mq_boundary_t* rights, const mq_index_desc* left_is,
                               const mq_index_desc* phys_is, const mq_index_desc* right_is, mq_hpsi_plan_t* out) {
     return guarded(ctx, [&] {
-        require(n >= 1 && mpos && lefts && rights && left_is && phys_is && right_is && out,
-                "mq_hpsi_plan_create_batch: null argument");
+        require(n >= 1 && n <= 65535 && mpos && lefts && rights && left_is && phys_is && right_is && out,
+                "mq_hpsi_plan_create_batch: null argument or more than 65535 instances");
         for (int i = 0; i < n; ++i)
             require(mpos[i] && lefts[i] && rights[i], "mq_hpsi_plan_create_batch: null instance");
         auto t0 = std::chrono::steady_clock::now();
@@ -744,6 +746,7 @@ int mq_hpsi_plan_create_batch(mq_ctx_t ctx, int n, const mq_mpo_t* mpos, const m
         std::vector<size_t> line_ranges{0};
         int64_t shift[BASE_COUNT] = {0};
         for (int i = 0; i < n; ++i) {
+            p->inst_off.push_back(int64_t(p->psi_n));
             // built and merged one at a time: the host holds the merged tables and one instance's
             HpsiPlanHost Q = build_hpsi_plan(ctx->sym, mpos[i]->mpo, lefts[i]->idx, rights[i]->idx,
                                              index_from(&left_is[i]), index_from(&phys_is[i]),
@@ -801,7 +804,9 @@ int mq_hpsi_plan_create_batch(mq_ctx_t ctx, int n, const mq_mpo_t* mpos, const m
             r_n += rights[i]->n;
             line_ranges.push_back(H.lines.size());
         }
+        p->inst_off.push_back(int64_t(p->psi_n));
         p->n_instances = n;
+        p->d_inst_off = dev_upload(p->inst_off, ctx->stream);
         // the right boundaries of all instances in one device buffer owned by the plan
         p->rown = dev_alloc<double>(r_n);
         uint64_t roff = 0;
@@ -1077,6 +1082,7 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
         dev_free(p->plain.lines);
         dev_free(p->plain.terms);
         dev_free(p->rown);
+        dev_free(p->d_inst_off);
     });
     delete p;
     return rc;
@@ -1251,6 +1257,192 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
                 stats->iterations = it;
                 stats->refine_iterations = refine;
                 stats->residual_norm = rnorm;
+                stats->hpsi_seconds = hpsi_s;
+                stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+            }
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            for (auto b : bufs) dev_free(b);
+            throw;
+        }
+        for (auto b : bufs) dev_free(b);
+    });
+}
+
+// Lock-step Jacobi-Davidson over the instances of a batch plan: the iteration of mq_davidson
+// (modified Gram-Schmidt with one refinement when the norm collapses, Rayleigh matrix, lowest Ritz
+// pair, residual, correction t = -r + (r.u / u.u) u) run for every instance at once, one batched
+// H*psi per iteration; every vector operation is one segmented launch over all instances, and the
+// per-instance scalars come back to the host together.  An instance that meets the residual rule
+// stops (its segments are no longer changed); the others go on.  No ortho vectors.
+extern "C" int mq_davidson_batch(mq_hpsi_plan_t p, const double* psi0, double tol, int max_iter, int precision,
+                                 double* evals, double* psi_out, int32_t* iterations, mq_davidson_stats_t* stats) {
+    if (!p) return MQ_ERR_INVALID;
+    return guarded(p->ctx, [&] {
+        require(p->n_instances > 1 || !p->inst_off.empty(), "mq_davidson_batch: not a batch plan");
+        require(psi0 && evals && psi_out && iterations && max_iter >= 1, "mq_davidson_batch: bad argument");
+        require(precision == MQ_PREC_FP64 || precision == MQ_PREC_FP32,
+                "mq_davidson_batch: precision must be MQ_PREC_FP64 or MQ_PREC_FP32");
+        auto t_start = std::chrono::steady_clock::now();
+        hipStream_t s = p->ctx->stream;
+        const int ns = p->n_instances;
+        const uint64_t n = p->psi_n;
+        const int64_t* off = p->d_inst_off;
+        for (int i = 0; i < ns; ++i) require(p->inst_off[size_t(i) + 1] > p->inst_off[size_t(i)], "mq_davidson_batch: empty instance");
+        std::vector<double*> bufs;
+        auto alloc = [&](uint64_t k) { bufs.push_back(dev_alloc<double>(k)); return bufs.back(); };
+        double hpsi_s = 0;
+        try {
+            std::vector<double*> V(max_iter), VA(max_iter);
+            for (int i = 0; i < max_iter; ++i) { V[i] = alloc(n); VA[i] = alloc(n); }
+            double* u = alloc(n);
+            double* r = alloc(n);
+            double* part = alloc(uint64_t(ns) * SEG_CHUNKS);
+            // dot results: up to max_iter + 1 rows of ns; coefficient arrays a, b
+            double* dres = alloc(uint64_t(max_iter + 1) * ns);
+            double* ca = alloc(uint64_t(ns));
+            double* cb = alloc(uint64_t(ns));
+            std::vector<double> hres(size_t(max_iter + 1) * ns);
+            hipcheck(hipMemcpyAsync(V[0], psi0, n * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+            hipcheck(hipStreamSynchronize(s), "upload");
+            // k dots (x_j . y_j, j < k) of every instance, one download
+            auto dots = [&](std::vector<std::pair<const double*, const double*>> const& xy) {
+                for (size_t j = 0; j < xy.size(); ++j)
+                    hipcheck(launch_seg_dot(xy[j].first, xy[j].second, off, ns, part, dres + j * ns, s), "dot");
+                hipcheck(hipMemcpyAsync(hres.data(), dres, xy.size() * ns * sizeof(double), hipMemcpyDeviceToHost, s),
+                         "dot");
+                hipcheck(hipStreamSynchronize(s), "dot");
+                std::vector<std::vector<double>> outv(xy.size());
+                for (size_t j = 0; j < xy.size(); ++j) outv[j].assign(hres.begin() + j * ns, hres.begin() + (j + 1) * ns);
+                return outv;
+            };
+            // y_i = a_i y_i + b_i x_i (the coefficient uploads complete before the host vectors go)
+            auto axpby = [&](double* y, const double* x, std::vector<double> const& a, std::vector<double> const& b) {
+                hipcheck(hipMemcpyAsync(ca, a.data(), ns * sizeof(double), hipMemcpyHostToDevice, s), "coef");
+                hipcheck(hipMemcpyAsync(cb, b.data(), ns * sizeof(double), hipMemcpyHostToDevice, s), "coef");
+                hipcheck(launch_seg_axpby(y, x, off, ns, ca, cb, s), "axpby");
+                hipcheck(hipStreamSynchronize(s), "axpby");
+            };
+            std::vector<double> theta(ns, 0.0), rnorm(ns, 0.0);
+            std::vector<int32_t> its(ns, 0);
+            std::vector<char> active(ns, 1);
+            std::vector<std::vector<double>> M(ns, std::vector<double>(size_t(max_iter) * max_iter, 0.0));
+            const std::vector<double> one(ns, 1.0), zero(ns, 0.0);
+            int it_run = 0;
+            // one lock-step run from V[0] in precision prec; leaves the Ritz vectors in u
+            auto jd = [&](int prec, double tol_run, int iters) {
+                std::fill(active.begin(), active.end(), 1);
+                std::fill(its.begin(), its.end(), 0);
+                int it = 0;
+                while (true) {
+                    double* t = V[it];
+                    std::vector<double> am(ns), bm(ns);  // masked coefficients
+                    auto gs_pass = [&](std::vector<char> const& mask) {
+                        for (int j = 0; j < it; ++j) {
+                            auto c = dots({{V[j], t}})[0];
+                            for (int i = 0; i < ns; ++i) bm[i] = mask[i] ? -c[i] : 0.0;
+                            axpby(t, V[j], one, bm);
+                        }
+                    };
+                    auto tau = dots({{t, t}})[0];
+                    gs_pass(active);
+                    auto nn = dots({{t, t}})[0];
+                    std::vector<char> again(ns, 0);
+                    bool any = false;
+                    for (int i = 0; i < ns; ++i) {
+                        again[i] = active[i] && std::sqrt(nn[i]) < 0.25 * std::sqrt(tau[i]);
+                        any |= again[i] != 0;
+                    }
+                    if (any) gs_pass(again);
+                    nn = dots({{t, t}})[0];
+                    for (int i = 0; i < ns; ++i) {
+                        am[i] = 0.0;
+                        bm[i] = active[i] ? 1.0 / std::sqrt(nn[i]) : 0.0;  // inactive: t = 0, so H t = 0
+                    }
+                    axpby(t, t, am, bm);
+                    auto h0 = std::chrono::steady_clock::now();
+                    hpsi_run(p, t, VA[it], prec);
+                    hipcheck(hipStreamSynchronize(s), "hpsi");
+                    hpsi_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
+                    std::vector<std::pair<const double*, const double*>> xy;
+                    for (int j = 0; j <= it; ++j) xy.push_back({V[j], VA[it]});
+                    auto mcol = dots(xy);
+                    const int dim = it + 1;
+                    std::vector<std::vector<double>> sv(ns);
+                    for (int i = 0; i < ns; ++i) {
+                        if (!active[i]) continue;
+                        for (int j = 0; j <= it; ++j) M[i][size_t(j) * max_iter + it] = mcol[j][i];
+                        std::vector<double> Ms(size_t(dim) * dim);
+                        for (int a = 0; a < dim; ++a)
+                            for (int b = a; b < dim; ++b)
+                                Ms[size_t(a) * dim + b] = Ms[size_t(b) * dim + a] = M[i][size_t(a) * max_iter + b];
+                        small_sym_lowest(Ms, dim, theta[i], sv[i]);
+                    }
+                    // u = V s, r = VA s (inactive instances keep their u and r)
+                    for (int j = 0; j < dim; ++j) {
+                        for (int i = 0; i < ns; ++i) {
+                            am[i] = active[i] ? (j == 0 ? 0.0 : 1.0) : 1.0;
+                            bm[i] = active[i] ? sv[i][size_t(j)] : 0.0;
+                        }
+                        axpby(u, V[j], am, bm);
+                        axpby(r, VA[j], am, bm);
+                    }
+                    for (int i = 0; i < ns; ++i) bm[i] = active[i] ? -theta[i] : 0.0;
+                    axpby(r, u, one, bm);  // r = uA - theta u
+                    ++it;
+                    auto rr = dots({{r, r}})[0];
+                    bool any_active = false;
+                    for (int i = 0; i < ns; ++i) {
+                        if (!active[i]) continue;
+                        rnorm[i] = std::sqrt(rr[i]);
+                        its[i] = it;
+                        if (rnorm[i] <= tol_run * std::fabs(theta[i]) || rnorm[i] < tol_run) active[i] = 0;  // iteration.h:57
+                        any_active |= active[i] != 0;
+                    }
+                    if (!any_active || it >= iters) break;
+                    // correction t = -r + (r.u)/(u.u) u of the active instances
+                    auto ru = dots({{r, u}, {u, u}});
+                    for (int i = 0; i < ns; ++i) {
+                        am[i] = 0.0;
+                        bm[i] = active[i] ? -1.0 : 0.0;
+                    }
+                    axpby(V[it], r, am, bm);
+                    for (int i = 0; i < ns; ++i) bm[i] = active[i] ? ru[0][i] / ru[1][i] : 0.0;
+                    axpby(V[it], u, one, bm);
+                }
+                it_run = it;
+            };
+            std::vector<int32_t> total(ns, 0);
+            int refine = 0, lockstep = 0;
+            if (precision == MQ_PREC_FP64 || max_iter < 2) {
+                jd(MQ_PREC_FP64, tol, max_iter);
+                total = its;
+                lockstep = it_run;
+            } else {
+                // FP32 H*psi first (to the FP32 floor), then FP64 refinement of every Ritz vector, as
+                // mq_davidson does for one instance
+                const int R = std::max(1, max_iter / 4);
+                jd(MQ_PREC_FP32, std::max(tol, 1e-6), max_iter - R);
+                total = its;
+                lockstep = it_run;
+                hipcheck(hipMemcpyAsync(V[0], u, n * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
+                jd(MQ_PREC_FP64, tol, R);
+                for (int i = 0; i < ns; ++i) total[i] += its[i];
+                refine = it_run;
+                lockstep += it_run;
+            }
+            hipcheck(hipMemcpyAsync(psi_out, u, n * sizeof(double), hipMemcpyDeviceToHost, s), "download");
+            hipcheck(hipStreamSynchronize(s), "davidson");
+            double rmax = 0;
+            for (int i = 0; i < ns; ++i) {
+                evals[i] = theta[i];
+                iterations[i] = total[i];
+                rmax = std::max(rmax, rnorm[i]);
+            }
+            if (stats) {
+                stats->iterations = lockstep;
+                stats->refine_iterations = refine;
+                stats->residual_norm = rmax;
                 stats->hpsi_seconds = hpsi_s;
                 stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
             }

@@ -415,6 +415,40 @@ __global__ void axpy_dev_kernel(double* __restrict__ y, const double* __restrict
         y[i] += coef * x[i];
 }
 
+// Segmented forms for a batch of independent vectors concatenated in one buffer (segment s is
+// elements off[s] .. off[s+1]-1; blockIdx.y = segment).  Dot: SEG_CHUNKS partial sums per
+// segment, then one thread per segment adds them in chunk order (deterministic).
+__global__ __launch_bounds__(DOT_THREADS) void seg_dot_partial_kernel(const double* __restrict__ x,
+                                                                      const double* __restrict__ y,
+                                                                      const int64_t* __restrict__ off,
+                                                                      double* __restrict__ part) {
+    const int seg = blockIdx.y;
+    const int64_t a = off[seg], b = off[seg + 1];
+    double s = 0.0;
+    for (int64_t i = a + int64_t(blockIdx.x) * DOT_THREADS + threadIdx.x; i < b; i += int64_t(SEG_CHUNKS) * DOT_THREADS)
+        s += x[i] * y[i];
+    double r = block_sum(s);
+    if (threadIdx.x == 0) part[int64_t(seg) * SEG_CHUNKS + blockIdx.x] = r;
+}
+
+__global__ void seg_dot_final_kernel(const double* __restrict__ part, int nseg, double* __restrict__ out) {
+    const int seg = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (seg >= nseg) return;
+    double s = 0.0;
+    for (int c = 0; c < SEG_CHUNKS; ++c) s += part[int64_t(seg) * SEG_CHUNKS + c];
+    out[seg] = s;
+}
+
+// segment s: y = a[s]*y + b[s]*x (y not read where a[s] == 0)
+__global__ void seg_axpby_kernel(double* __restrict__ y, const double* __restrict__ x, const int64_t* __restrict__ off,
+                                 const double* __restrict__ ca, const double* __restrict__ cb) {
+    const int seg = blockIdx.y;
+    const int64_t i0 = off[seg], i1 = off[seg + 1];
+    const double a = ca[seg], b = cb[seg];
+    for (int64_t i = i0 + int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < i1; i += int64_t(gridDim.x) * blockDim.x)
+        y[i] = a == 0.0 ? b * x[i] : a * y[i] + b * x[i];
+}
+
 // y = a*y + b*x; with a == 0 y is not read (it may hold anything, NaN included: BLAS beta = 0)
 __global__ void axpby_kernel(double* __restrict__ y, const double* __restrict__ x, uint64_t n, double a, double b) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
@@ -563,6 +597,22 @@ hipError_t launch_dot(const double* x, const double* y, uint64_t n, double* work
 hipError_t launch_axpy_dev(double* y, const double* x, uint64_t n, const double* num, const double* den, double sign,
                            hipStream_t stream) {
     hipLaunchKernelGGL(axpy_dev_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, y, x, n, num, den, sign);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_dot(const double* x, const double* y, const int64_t* off, int nseg, double* part, double* out,
+                          hipStream_t stream) {
+    if (nseg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(seg_dot_partial_kernel, dim3(SEG_CHUNKS, unsigned(nseg)), dim3(DOT_THREADS), 0, stream, x, y,
+                       off, part);
+    hipLaunchKernelGGL(seg_dot_final_kernel, dim3(unsigned((nseg + 255) / 256)), dim3(256), 0, stream, part, nseg, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_axpby(double* y, const double* x, const int64_t* off, int nseg, const double* a, const double* b,
+                            hipStream_t stream) {
+    if (nseg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(seg_axpby_kernel, dim3(SEG_CHUNKS, unsigned(nseg)), dim3(256), 0, stream, y, x, off, a, b);
     return hipGetLastError();
 }
 

@@ -61,6 +61,14 @@ hipError_t launch_axpy_dev(double* y, const double* x, uint64_t n, const double*
                            hipStream_t stream);
 // y = a*y + b*x (host scalars); a == 0 assigns b*x without reading y
 hipError_t launch_axpby(double* y, const double* x, uint64_t n, double a, double b, hipStream_t stream);
+// segmented forms over nseg (<= 65535) vectors concatenated in one buffer, segment s being
+// elements off[s] .. off[s+1]-1 (off on the device): out[s] = x_s . y_s (part holds
+// nseg * SEG_CHUNKS partials), and y_s = a[s]*y_s + b[s]*x_s (a, b on the device)
+constexpr int SEG_CHUNKS = 64;
+hipError_t launch_seg_dot(const double* x, const double* y, const int64_t* off, int nseg, double* part, double* out,
+                          hipStream_t stream);
+hipError_t launch_seg_axpby(double* y, const double* x, const int64_t* off, int nseg, const double* a, const double* b,
+                            hipStream_t stream);
 // x *= 1/sqrt(nrm2[0])
 hipError_t launch_scale_rsqrt(double* x, uint64_t n, const double* nrm2, hipStream_t stream);
 

@@ -274,3 +274,19 @@ class HpsiPlanBatch(_Handle):
     def apply_device(self, in_ptr, out_ptr, precision=abi.MQ_PREC_FP64):
         self.ctx.check(self.ctx.lib.mq_hpsi(self.h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), int(precision)))
 
+    def davidson(self, psi0s, tol=1e-8, max_iter=8, precision=abi.MQ_PREC_FP64):
+        """lowest eigenpair of every instance, lock-step on the device (mq_davidson_batch):
+        (energies, vectors, iterations per instance, stats)"""
+        psi0 = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64) for x in psi0s]))
+        if psi0.size != self.n:
+            raise ValueError("start vectors do not match the batch")
+        k = len(self.layouts)
+        out = np.zeros_like(psi0)
+        ev = np.zeros(k)
+        its = np.zeros(k, np.int32)
+        st = abi.DavidsonStats()
+        self.ctx.check(self.ctx.lib.mq_davidson_batch(self.h, _dp(psi0), float(tol), int(max_iter), int(precision),
+                                                      _dp(ev), _dp(out), its.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                      C.byref(st)))
+        return ev, self.split(out), its, {f: getattr(st, f) for f, _ in abi.DavidsonStats._fields_}
+

@@ -5,9 +5,10 @@ plan.
   1e-12 relative; FP32 to 1e-5 of the FP64 result), and the batch's tables and flop counts are the
   sums of the instances';
 * mq_davidson refuses a batch plan (the solver works on one instance);
-* the lock-step Davidson over the batch follows every instance's own device Davidson (same
-  iteration): the same eigenvalue to 1e-8 and vector to 1e-6, and the residual rule holds where it
-  stopped an instance.
+* the lock-step Davidson over the batch, on the host (batch.py) and on the device
+  (mq_davidson_batch), follows every instance's own device Davidson (same iteration): the same
+  eigenvalue and vector, the same iteration counts, and the residual rule holds where it stopped an
+  instance.
 """
 import ctypes as C
 
@@ -88,3 +89,22 @@ def test_lockstep_davidson(ctx2u1):
         if it < 40:         # stopped on the residual rule
             r = plan.apply(v) - ei * v
             assert np.linalg.norm(r) <= 1e-10 * max(1.0, abs(ei)) + 1e-11
+
+
+@pytest.mark.parametrize("precision", [abi.MQ_PREC_FP64, abi.MQ_PREC_FP32])
+def test_device_lockstep_davidson(ctx2u1, precision):
+    """mq_davidson_batch: every instance's eigenvalue, vector and iteration count equal those of
+    its own plan's mq_davidson (same iteration; FP64, and FP32 contraction with FP64 refinement)"""
+    inst = _instances(ctx2u1)
+    batch = HpsiPlanBatch(ctx2u1, [x["item"] for x in inst])
+    e, vecs, its, st = batch.davidson([x["psi"] for x in inst], tol=1e-10, max_iter=40, precision=precision)
+    print(f"device lock-step (precision {precision}): {e}, iterations {its}, lock-step {st['iterations']}", flush=True)
+    assert st["iterations"] == (its.max() if precision == abi.MQ_PREC_FP64 else st["iterations"])
+    for x, ei, v, it in zip(inst, e, vecs, its):
+        plan = HpsiPlan(ctx2u1, *x["item"])
+        ref, vref, rst = plan.davidson(x["psi"], tol=1e-10, max_iter=40, precision=precision)
+        print(f"  n={v.size}: {ei:.12f} device {ref:.12f}, iterations {it} / {rst['iterations']}", flush=True)
+        assert abs(ei - ref) < 1e-9 * max(1.0, abs(ref))
+        assert abs(abs(v @ vref) - 1) < 1e-6
+        if precision == abi.MQ_PREC_FP64:
+            assert it == rst["iterations"]
