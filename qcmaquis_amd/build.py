@@ -52,5 +52,27 @@ def build_library(verbose=False):
     return LIB
 
 
+H5_LIB = os.path.join(PKG, "libqcmaquis_amd_h5.so")
+H5_PREFIX = os.environ.get("QCMAQUIS_AMD_HDF5", "/opt/conda")
+
+
+def build_h5(verbose=False):
+    """the host-only result / checkpoint file library (HDF5 C library under H5_PREFIX); skipped,
+    returning None, when the HDF5 headers are absent"""
+    inc, lib = os.path.join(H5_PREFIX, "include"), os.path.join(H5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print("HDF5 headers not found under", H5_PREFIX, "- result files not built")
+        return None
+    tmp = H5_LIB + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", inc,
+          "-o", tmp, os.path.join(CSRC, "h5", "h5io.cpp"), "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib])
+    os.replace(tmp, H5_LIB)
+    if verbose:
+        print("built", H5_LIB)
+    return H5_LIB
+
+
 if __name__ == "__main__":
     build_library(verbose=True)
+    build_h5(verbose=True)

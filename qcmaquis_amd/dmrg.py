@@ -115,6 +115,19 @@ def estimate_truncation(svals, Mmax, cutoff):
     return keeps, tw
 
 
+def truncation_details(svals, Mmax, cutoff):
+    """the reference's truncation record (estimate_truncation, block_matrix_algorithms.h:135-181;
+    svd_truncate): kept states, truncated weight (discarded s^2 / all s^2), truncated fraction
+    (discarded s / all s) and smallest kept ratio (cut / s_max)"""
+    keeps, tw = estimate_truncation(svals, Mmax, cutoff)
+    allv = np.sort(np.concatenate([s for s in svals.values()]))[::-1]
+    cut = cutoff * allv[0]
+    if allv.size > Mmax:
+        cut = max(cut, allv[Mmax])
+    return {"BondDimension": int(sum(keeps.values())), "TruncatedWeight": tw,
+            "TruncatedFraction": float(np.sum(allv[allv < cut]) / np.sum(allv)), "SmallestEV": float(cut / allv[0])}
+
+
 _svd_pool = None
 
 
@@ -135,12 +148,15 @@ def block_svds(blocks):
     return {m: out[m] for m in blocks}
 
 
-def split_two(lay2, psi2, phys, off12, Mmax, cutoff, to_right):
+def split_two(lay2, psi2, phys, off12, Mmax, cutoff, to_right, details=None):
     """SVD split of a two-site tensor; to_right: A = U left-normalised, B = S V^T (split_mps_l2r),
-    else A = U S, B = V^T right-normalised (split_mps_r2l)"""
+    else A = U S, B = V^T right-normalised (split_mps_r2l); details (a dict) receives the
+    truncation record"""
     mats = split_matrices(lay2, psi2, phys, off12)
     svd = block_svds({m: X for m, (X, _, _) in mats.items()})
     keeps, tw = estimate_truncation({m: s[1] for m, s in svd.items()}, Mmax, cutoff)
+    if details is not None:
+        details.update(truncation_details({m: s[1] for m, s in svd.items()}, Mmax, cutoff))
     mid = Index([m for m in keeps if keeps[m] > 0], [keeps[m] for m in keeps if keeps[m] > 0])
     la = MPSLayout(lay2.left, phys, mid)
     lb = MPSLayout(mid, phys, lay2.right)
@@ -234,9 +250,11 @@ class TwoSiteDMRG:
     """Two-site DMRG for a TwoU1 quantum-chemistry Hamiltonian (QCHamiltonian)."""
 
     def __init__(self, ham, n_up, n_dn, M, ctx=None, init_m=8, seed=42, tol=1e-8, max_iter=8, cutoff=1e-16,
-                 verbose=False, precision=abi.MQ_PREC_FP64):
+                 verbose=False, precision=abi.MQ_PREC_FP64, mps=None, resultfile=None, first_sweep=0):
         """precision: arithmetic of the Davidson H*psi (MQ_PREC_FP32 = FP32 contraction with an FP64
-        refinement of each eigenvector, see mq_davidson)"""
+        refinement of each eigenvector, see mq_davidson); mps: (layouts, data) to start from (e.g.
+        results.load_checkpoint) instead of a random state; resultfile: HDF5 result file that every
+        sweep appends to (results.ResultArchive); first_sweep: number of the first sweep"""
         self.ham, self.M, self.tol, self.max_iter, self.cutoff = ham, M, tol, max_iter, cutoff
         self.precision = precision
         self.compare_precision = None  # a list: every step also runs Davidson in the other precision
@@ -247,11 +265,20 @@ class TwoSiteDMRG:
         self.phys = ham.phys()
         self.phys2 = ham.phys(True)
         self.off12 = twosite_offsets(self.phys)
-        rng = np.random.default_rng(seed)
-        bonds = allowed_bonds([self.phys] * self.L, (n_up, n_dn), init_m)
-        self.lay = [MPSLayout(bonds[k], self.phys, bonds[k + 1]) for k in range(self.L)]
-        self.data = [rng.normal(size=l.n) for l in self.lay]
+        if mps is None:
+            rng = np.random.default_rng(seed)
+            bonds = allowed_bonds([self.phys] * self.L, (n_up, n_dn), init_m)
+            self.lay = [MPSLayout(bonds[k], self.phys, bonds[k + 1]) for k in range(self.L)]
+            self.data = [rng.normal(size=l.n) for l in self.lay]
+        else:
+            self.lay = list(mps[0])
+            self.data = [np.array(d, np.float64) for d in mps[1]]
+            if len(self.lay) != self.L:
+                raise ValueError("the MPS has %d sites, the Hamiltonian %d" % (len(self.lay), self.L))
         right_normalize(self.lay, self.data)
+        self.resultfile = resultfile
+        self.sweep_number = first_sweep
+        self.iteration_results = {}
         self.mpo1 = [DeviceMPO(self.ctx, ham.mpo(k)) for k in range(self.L)]
         self.mpo2 = {}
         # boundaries: left[k] of bond k, right[k] of bond k
@@ -315,7 +342,8 @@ class TwoSiteDMRG:
         self.timing["plan_plain"] = self.timing.get("plan_plain", 0.0) + pst["seconds_plain"]
         self.timing["hpsi"] += st["hpsi_seconds"]
         plan.close()
-        la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right)
+        trunc = {}
+        la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right, trunc)
         t4 = time.perf_counter()
         self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1] = la, a, lb, b
         if to_right:
@@ -332,6 +360,10 @@ class TwoSiteDMRG:
             self.timing[k] += v
         self.energies.append(e + self.ham.ecore)
         self.trunc.append(tw)
+        # per-step results of the sweep (ts_optimize.hpp:272, 348-351)
+        self.iteration_results.setdefault("Energy", []).append(e + self.ham.ecore)
+        for k, v in trunc.items():
+            self.iteration_results.setdefault(k, []).append(v)
         if self.verbose:
             print(f"  site {i} {'->' if to_right else '<-'} E={e + self.ham.ecore:.12f} M={sum(lb.left.sizes)} "
                   f"tw={tw:.2e} it={st['iterations']} plan {t2 - t1:.3f} s (tables {pst['seconds_tables']:.3f}, "
@@ -340,13 +372,32 @@ class TwoSiteDMRG:
         return e + self.ham.ecore
 
     def sweep(self):
-        """one full sweep (2L-2 two-site steps, ts_optimize.hpp:128-131)"""
+        """one full sweep (2L-2 two-site steps, ts_optimize.hpp:128-131); with a result file, the
+        sweep's per-step results and the lowest energy are written (dmrg_sim.hpp:88-100)"""
         t0 = time.perf_counter()
+        self.iteration_results = {}
         for i in range(0, self.L - 1):
             self.step(i, True)
         for i in range(self.L - 2, -1, -1):
             self.step(i, False)
-        return self.energies[-1], time.perf_counter() - t0
+        dt = time.perf_counter() - t0
+        if self.resultfile:
+            from .results import ResultArchive
+            ar = ResultArchive(self.resultfile)
+            ar.append_iteration(self.sweep_number, self.iteration_results)
+            emin = min(self.iteration_results["Energy"])
+            prev = getattr(self, "_emin_prev", None)
+            # the first sweep's change is measured from the largest double, as in the reference
+            ar.write_sweep_energy(emin, abs(emin - (prev if prev is not None else np.finfo(np.float64).max)))
+            self._emin_prev = emin
+        self.sweep_number += 1
+        return self.energies[-1], dt
+
+    def checkpoint(self, directory, site=-1):
+        """the MPS and the status (last finished sweep) in the reference's checkpoint layout
+        (sim.hpp:223-235; results.save_checkpoint)"""
+        from .results import save_checkpoint
+        save_checkpoint(directory, self.lay, self.data, self.sweep_number - 1, site)
 
 
 def _absorb_into_right_neighbour(lay, data, mid, Rs):

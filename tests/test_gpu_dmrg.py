@@ -105,3 +105,35 @@ def test_single_site_sweeps_small6(ctx2u1):
     print(f"  M=4 per sector: E={e:.12f} (FCI {g['e0']:.12f})", flush=True)
     assert e > g["e0"] - 1e-10 and e - g["e0"] < 1.0
     assert np.all(np.diff(d.energies) < 1e-9)
+
+
+def test_result_file_and_restart_small6(ctx2u1, tmp_path):
+    """sweeps write the reference's result layout (per-step Energy, BondDimension and truncation
+    records per sweep, the lowest energy); a checkpoint after three sweeps restarts the run, whose
+    next sweep continues the numbering and stays at the FCI energy"""
+    from qcmaquis_amd import results as R
+    g = GOLD["small6"]
+    h, eri, ec = make_integrals(g["L"], g["seed"], g["rank"])
+    ham = QCHamiltonian(h, eri, ec)
+    rf, chk = str(tmp_path / "res.h5"), str(tmp_path / "chkp")
+    print("small6 result file and restart", flush=True)
+    d = TwoSiteDMRG(ham, g["n_up"], g["n_dn"], M=400, ctx=ctx2u1, tol=1e-10, max_iter=12, resultfile=rf)
+    for _ in range(3):
+        d.sweep()
+    d.checkpoint(chk)
+    ar = R.ResultArchive(rf)
+    assert ar.sweeps() == [0, 1, 2]
+    for k in range(3):
+        e = ar.iteration(k, "Energy")
+        assert e.size == 2 * (g["L"] - 1)
+        assert R.info(rf, "/spectrum/iteration/%d/results/BondDimension/mean/value" % k)[0] == R.INTEGER
+        assert ar.iteration(k, "TruncatedWeight").size == e.size
+    assert ar.energy() == ar.iteration(2, "Energy").min()
+    lay, data, status = R.load_checkpoint(chk)
+    assert status == {"sweep": 2, "site": -1}
+    d2 = TwoSiteDMRG(ham, g["n_up"], g["n_dn"], M=400, ctx=ctx2u1, tol=1e-10, max_iter=12, mps=(lay, data),
+                     resultfile=rf, first_sweep=status["sweep"] + 1)
+    e, _ = d2.sweep()
+    print(f"  restarted sweep: E={e:.12f} (FCI {g['e0']:.12f})", flush=True)
+    assert abs(e - g["e0"]) < 1e-9
+    assert R.ResultArchive(rf).sweeps() == [0, 1, 2, 3]
