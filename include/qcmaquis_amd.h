@@ -44,7 +44,8 @@ extern "C" {
 /* device arithmetic of the sector GEMMs */
 #define MQ_PREC_FP64   0
 #define MQ_PREC_FP32   1
-#define MQ_PREC_BF16X3 2
+#define MQ_PREC_BF16X3 2         /* reserved (split-BF16 products): not built; H*psi returns
+                                    MQ_ERR_UNSUPPORTED */
 /* FP64 with the reference's launch structure, for comparison with the grouped path: one library
  * (rocBLAS) dgemm per T task, one S-assembly workgroup per S tile that gathers every term, and
  * one dgemm per cohort added into its output block (reference: MPSBlock::create_T_gpu,
