@@ -10,8 +10,11 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "core.hpp"
@@ -63,15 +66,75 @@ void hipcheck(hipError_t e, const char* what) {
     if (e != hipSuccess) throw Error(MQ_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Device blocks of 1 MiB and more are kept for reuse when freed: a sweep creates and destroys
+// plans of similar sizes at every site, and hipMalloc / hipFree of gigabyte buffers cost
+// milliseconds each (hipFree also waits for the device).  A freed block serves a later request
+// of at most its size and at least 4/5 of it on the same device; at most CACHE_CAP bytes are kept
+// (the largest idle blocks are released first), and all idle blocks are released before an
+// allocation that the device cannot satisfy is given up.  A block enters the cache after a device
+// synchronisation (as hipFree waits for the device), so a reused block is idle; like a fresh
+// hipMalloc it holds undefined contents.
+struct DevCache {
+    std::mutex m;
+    std::unordered_map<void*, std::pair<int, uint64_t>> live;    // block -> (device, bytes)
+    std::multimap<uint64_t, std::pair<int, void*>> idle;         // bytes -> (device, block)
+    uint64_t idle_bytes = 0;
+};
+constexpr uint64_t CACHE_MIN = uint64_t(1) << 20;
+constexpr uint64_t CACHE_CAP = uint64_t(16) << 30;
+
+DevCache& devcache() {
+    static DevCache* c = new DevCache;  // never destroyed: frees may come from static destructors
+    return *c;
+}
+
+// releases the idle blocks (of one device, or all with dev < 0); caller holds the lock
+void cache_release_locked(DevCache& c, int dev) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto it = c.idle.begin(); it != c.idle.end();) {
+        if (dev >= 0 && it->second.first != dev) {
+            ++it;
+            continue;
+        }
+        if (it->second.first != cur) (void)hipSetDevice(it->second.first);
+        (void)hipFree(it->second.second);
+        if (it->second.first != cur) (void)hipSetDevice(cur);
+        c.idle_bytes -= it->first;
+        it = c.idle.erase(it);
+    }
+}
+
 template <class T>
 T* dev_alloc(uint64_t n) {
     if (n == 0) n = 1;
+    const uint64_t bytes = n * sizeof(T);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    DevCache& c = devcache();
+    std::lock_guard<std::mutex> lock(c.m);
+    if (bytes >= CACHE_MIN) {
+        for (auto it = c.idle.lower_bound(bytes); it != c.idle.end() && it->first <= bytes + bytes / 4; ++it)
+            if (it->second.first == dev) {
+                void* p = it->second.second;
+                c.live[p] = {dev, it->first};
+                c.idle_bytes -= it->first;
+                c.idle.erase(it);
+                return static_cast<T*>(p);
+            }
+    }
     void* p = nullptr;
-    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        throw Error(MQ_ERR_NOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed");
+        cache_release_locked(c, dev);
+        e = hipMalloc(&p, bytes);
     }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(MQ_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    c.live[p] = {dev, bytes};
     return static_cast<T*>(p);
 }
 
@@ -89,7 +152,30 @@ T* dev_upload(std::vector<T, A> const& v, hipStream_t s) {
 }
 
 void dev_free(void* p) {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    DevCache& c = devcache();
+    std::lock_guard<std::mutex> lock(c.m);
+    auto it = c.live.find(p);
+    if (it == c.live.end() || it->second.second < CACHE_MIN) {
+        if (it != c.live.end()) c.live.erase(it);
+        (void)hipFree(p);
+        return;
+    }
+    // work on any stream may still read the block (hipFree would wait for the device too)
+    (void)hipDeviceSynchronize();
+    c.idle.emplace(it->second.second, std::make_pair(it->second.first, p));
+    c.idle_bytes += it->second.second;
+    c.live.erase(it);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    while (c.idle_bytes > CACHE_CAP) {
+        auto last = std::prev(c.idle.end());
+        if (last->second.first != cur) (void)hipSetDevice(last->second.first);
+        (void)hipFree(last->second.second);
+        if (last->second.first != cur) (void)hipSetDevice(cur);
+        c.idle_bytes -= last->first;
+        c.idle.erase(last);
+    }
 }
 
 struct DeviceGuard {
@@ -333,6 +419,11 @@ int mq_ctx_destroy(mq_ctx_t ctx) {
     int rc = guarded(ctx, [&] {
         (void)hipStreamSynchronize(ctx->stream);
         dev_free(ctx->dwork);
+        {
+            DevCache& c = devcache();
+            std::lock_guard<std::mutex> lock(c.m);
+            cache_release_locked(c, ctx->device);
+        }
         if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
         (void)hipStreamDestroy(ctx->stream);
     });
