@@ -188,7 +188,8 @@ constexpr int REDUCE_CHUNKS = 32;  // 8192 (largest tile) / 256
 static_assert(gemm_bm(GEMM_128x64) * gemm_bn(GEMM_128x64) <= REDUCE_CHUNKS * 256 &&
                   gemm_bm(GEMM_32x128) * gemm_bn(GEMM_32x128) <= REDUCE_CHUNKS * 256 &&
                   gemm_bm(GEMM_64x64) * gemm_bn(GEMM_64x64) <= REDUCE_CHUNKS * 256 &&
-                  gemm_bm(GEMM_96x64) * gemm_bn(GEMM_96x64) <= REDUCE_CHUNKS * 256,
+                  gemm_bm(GEMM_96x64) * gemm_bn(GEMM_96x64) <= REDUCE_CHUNKS * 256 &&
+                  gemm_bm(GEMM_64x128) * gemm_bn(GEMM_64x128) <= REDUCE_CHUNKS * 256,
               "reduce grid covers every tile shape");
 template <class TW, class TO>
 __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __restrict__ probs,
@@ -490,6 +491,9 @@ void launch_gemm_t(const GemmDesc* probs, const GemmRow* rows, const GemmTile* c
                                    rows, tiles[sh] + off, bases, w);
             else if (sh == GEMM_96x64)
                 hipLaunchKernelGGL((gemm_kernel<3, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                                   rows, tiles[sh] + off, bases, w);
+            else if (sh == GEMM_64x128)
+                hipLaunchKernelGGL((gemm_kernel<2, 4, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    rows, tiles[sh] + off, bases, w);
             else
                 hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,

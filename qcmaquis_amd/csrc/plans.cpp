@@ -1420,7 +1420,7 @@ GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t tar
     // threads each load one row, which needs a tile height that divides 256
     auto shape_of = [](GemmDesc const& g) {
         return g.M <= 32 ? GEMM_32x128
-             : g.M <= 64 ? GEMM_64x64
+             : g.M <= 64 ? (g.N > 64 ? GEMM_64x128 : GEMM_64x64)
              : (g.M <= 96 && !(g.flags & GEMM_ROWMAP)) ? GEMM_96x64 : GEMM_128x64;
     };
     auto tiles_of = [&](GemmDesc const& g) {
