@@ -5,6 +5,8 @@ plan.
   1e-12 relative; FP32 to 1e-5 of the FP64 result), and the batch's tables and flop counts are the
   sums of the instances';
 * mq_davidson refuses a batch plan (the solver works on one instance);
+* SU2U1: a batch of every site of two random chains, with small-tile lines, equals the CPU
+  oracle's H*psi of each instance to 1e-12;
 * the lock-step Davidson over the batch, on the host (batch.py) and on the device
   (mq_davidson_batch), follows every instance's own device Davidson (same iteration): the same
   eigenvalue and vector, the same iteration counts, and the residual rule holds where it stopped an
@@ -108,3 +110,31 @@ def test_device_lockstep_davidson(ctx2u1, precision):
         assert abs(abs(v @ vref) - 1) < 1e-6
         if precision == abi.MQ_PREC_FP64:
             assert it == rst["iterations"]
+
+
+def test_batch_su2u1_chains_vs_oracle(ctxsu2, oracle_lib):
+    """a batch of SU2U1 site problems from two random chains (every site of each): each segment
+    of the batched H*psi equals the oracle's H*psi of its instance to 1e-12 (FP64), with the
+    small-tile term sum forced on for tiles of up to 8 elements"""
+    from tests.chains import make_chain, oracle_left, oracle_right, rel_err
+    from qcmaquis_amd.engine import DeviceBoundary, DeviceMPO
+    items, refs = [], []
+    for seed, max_m, D in ((1, 3, 6), (4, 8, 9)):
+        ch = make_chain(seed, sym=abi.MQ_SYM_SU2U1, n_sites=5, total=(4, 0), max_m=max_m, D=D)
+        n = len(ch.mpos)
+        L_or, R_or = oracle_left(oracle_lib, ch, n - 1), oracle_right(oracle_lib, ch, 1)
+        for k in range(n):
+            items.append((DeviceMPO(ctxsu2, ch.mpos[k]), DeviceBoundary.upload(ctxsu2, L_or[k]),
+                          DeviceBoundary.upload(ctxsu2, R_or[k + 1]), ch.layouts[k]))
+            refs.append((ch, k, L_or[k], R_or[k + 1]))
+    ctxsu2.set_option(abi.MQ_OPT_SMALL_TILE_MAX, 8)
+    try:
+        batch = HpsiPlanBatch(ctxsu2, items)
+    finally:
+        ctxsu2.set_option(abi.MQ_OPT_SMALL_TILE_MAX, abi.MQ_SMALL_TILE_AUTO)
+    print(f"su2u1 batch of {len(items)}: psi {batch.n}, small-tile lines {batch.stats()['small_lines']}", flush=True)
+    assert batch.stats()["small_lines"] > 0
+    psi = np.random.default_rng(3).normal(size=batch.n)
+    for x, g, (ch, k, L, R) in zip(batch.split(psi), batch.split(batch.apply(psi)), refs):
+        ref = oracle_lib.hpsi(ch.sym, ch.mpos[k], L, R, ch.layouts[k], x)
+        assert rel_err(g, ref) < 1e-12, k
