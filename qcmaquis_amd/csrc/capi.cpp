@@ -138,16 +138,63 @@ T* dev_alloc(uint64_t n) {
     return static_cast<T*>(p);
 }
 
+// Host-to-device copies of large tables go through two pinned staging halves: a piece is copied
+// into one half by the host threads while the previous piece moves from the other half by DMA
+// (pinned transfers run at about twice the rate of pageable ones).  Process-wide, one copy at a
+// time; smaller copies go directly from pageable memory.
+struct Staging {
+    std::mutex m;
+    char* half[2] = {nullptr, nullptr};
+};
+constexpr size_t STAGE_HALF = size_t(64) << 20;
+constexpr size_t STAGE_MIN = size_t(4) << 20;
+
+Staging& staging() {
+    static Staging* st = new Staging;  // pinned halves live until the process ends
+    return *st;
+}
+
+void upload_bytes(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes < STAGE_MIN) {
+        hipcheck(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "upload");
+        hipcheck(hipStreamSynchronize(s), "upload");
+        return;
+    }
+    Staging& st = staging();
+    std::lock_guard<std::mutex> lock(st.m);
+    for (auto& h : st.half)
+        if (!h) hipcheck(hipHostMalloc(reinterpret_cast<void**>(&h), STAGE_HALF, hipHostMallocPortable), "pinned staging");
+    const char* in = static_cast<const char*>(src);
+    char* out = static_cast<char*>(dst);
+    bool in_flight = false;
+    int h = 0;
+    try {
+        for (size_t off = 0; off < bytes; off += STAGE_HALF, h ^= 1) {
+            const size_t n = std::min(STAGE_HALF, bytes - off);
+            // fill this half while the other half's transfer runs
+            const int64_t pieces = int64_t((n + (size_t(1) << 20) - 1) >> 20);
+#pragma omp parallel for schedule(static)
+            for (int64_t q = 0; q < pieces; ++q) {
+                const size_t a = size_t(q) << 20, len = std::min(size_t(1) << 20, n - a);
+                std::memcpy(st.half[h] + a, in + off + a, len);
+            }
+            if (in_flight) hipcheck(hipStreamSynchronize(s), "upload");
+            hipcheck(hipMemcpyAsync(out + off, st.half[h], n, hipMemcpyHostToDevice, s), "upload");
+            in_flight = true;
+        }
+        hipcheck(hipStreamSynchronize(s), "upload");
+    } catch (...) {
+        (void)hipStreamSynchronize(s);  // no transfer may still read a half when the lock is released
+        throw;
+    }
+}
+
 // Copies a host table to the device and waits for the copy: callers pass vectors that may be
-// destroyed as soon as this returns, and an asynchronous copy from pageable memory may read its
-// source after the call.
+// destroyed as soon as this returns.
 template <class T, class A>
 T* dev_upload(std::vector<T, A> const& v, hipStream_t s) {
     T* p = dev_alloc<T>(v.size());
-    if (!v.empty()) {
-        hipcheck(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
-        hipcheck(hipStreamSynchronize(s), "upload");
-    }
+    if (!v.empty()) upload_bytes(p, v.data(), v.size() * sizeof(T), s);
     return p;
 }
 
