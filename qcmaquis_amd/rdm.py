@@ -11,6 +11,8 @@ of that entry (the entry and its permutational images set to one, everything els
     dE/dh_pq       = gamma_pq + gamma_qp            (p != q; gamma_pp for p == q)
     dE/d(pq|rs)    = 1/2 sum over the distinct images (ab|cd) of (pq|rs) of Gamma_abcd
 
+(two_rdm_sym collects the second for every class of (pq|rs): Gamma averaged over the images.)
+
 with gamma_pq = <E_pq> the spin-summed 1-RDM and Gamma_pqrs = <E_pq E_rs> - delta_qr <E_ps>.
 At an eigenstate these are the Hellmann-Feynman forces on the integrals (the gradients orbital
 optimisation consumes).  Each expectation value is one left-to-right boundary growth over all
@@ -76,6 +78,31 @@ def eri_gradient(ctx, layouts, data, entries):
     L = len(layouts)
     zero = np.zeros((L, L))
     return np.array([expval(ctx, layouts, data, QCHamiltonian(zero, unit_eri(L, *e))) for e in entries])
+
+
+def eri_orbits(L):
+    """one representative (the smallest index tuple) of every class of (pq|rs) under the eight
+    permutational symmetries"""
+    out = set()
+    for e in itertools.product(range(L), repeat=4):
+        out.add(eri_images(*e)[0])
+    return sorted(out)
+
+
+def two_rdm_sym(ctx, layouts, data):
+    """the spin-summed 2-RDM averaged over the eight permutations of (pq|rs):
+    Gamma^sym_pqrs = 2 dE/d(pq|rs) / (number of distinct images), so that the two-body energy is
+    1/2 sum (pq|rs) Gamma^sym_pqrs for any integrals with the eight-fold symmetry (the part of
+    Gamma that real integrals see); one expectation value per class"""
+    L = len(layouts)
+    orbits = eri_orbits(L)
+    g = eri_gradient(ctx, layouts, data, orbits)
+    gam = np.zeros((L, L, L, L))
+    for e, v in zip(orbits, g):
+        imgs = eri_images(*e)
+        for a, b, c, d in imgs:
+            gam[a, b, c, d] = 2.0 * v / len(imgs)
+    return gam
 
 
 def one_rdm(ctx, layouts, data):

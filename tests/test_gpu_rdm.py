@@ -13,7 +13,7 @@ import pytest
 
 from qcmaquis_amd.dmrg import TwoSiteDMRG
 from qcmaquis_amd.qc import QCHamiltonian, fci_energy, make_integrals
-from qcmaquis_amd.rdm import eri_gradient, expval, h_gradient, one_rdm, unit_eri, unit_h
+from qcmaquis_amd.rdm import two_rdm_sym, eri_gradient, expval, h_gradient, one_rdm, unit_eri, unit_h
 from tests.fci_rdm import fci_ground
 
 pytestmark = pytest.mark.gpu
@@ -58,3 +58,15 @@ def test_rdm_and_gradients_small6(ctx2u1):
         dh = unit_h(L, p, q)
         fd = (fci_energy(h + eps * dh, eri, nu, nd)[0] - fci_energy(h - eps * dh, eri, nu, nd)[0]) / (2 * eps)
         assert abs(v - fd) < 1e-6
+
+    print("small6 rdm: permutation-averaged 2-RDM", flush=True)
+    gam2 = two_rdm_sym(ctx2u1, d.lay, d.data)
+    perms = ["pqrs", "qprs", "pqsr", "qpsr", "rspq", "srpq", "rsqp", "srqp"]
+    gam2_fci = sum(np.einsum("pqrs->" + t, Gam_fci) for t in perms) / 8.0
+    print(f"  max |Gamma_sym - Gamma_sym(fci)| {np.abs(gam2 - gam2_fci).max():.2e}", flush=True)
+    assert np.abs(gam2 - gam2_fci).max() < 1e-6
+    # the two-body energy from it equals the two-body expectation value
+    assert abs(0.5 * np.einsum("pqrs,pqrs", eri, gam2) - e2) < 1e-8
+    # particle-number sum rule of the spin-summed 2-RDM: sum_pr Gamma_pprr = N (N - 1)
+    n = nu + nd
+    assert abs(np.einsum("pprr->", gam2) - n * (n - 1)) < 1e-8

@@ -43,3 +43,17 @@ def test_fci_rdms_reproduce_energy_and_gradients():
         de = unit_eri(L, p, q, r, s)
         fd = (fci_energy(h, eri + eps * de, nu, nd)[0] - fci_energy(h, eri - eps * de, nu, nd)[0]) / (2 * eps)
         assert abs(fd - 0.5 * np.einsum("pqrs,pqrs", de, G)) < 1e-6
+
+
+def test_eri_orbits_cover_every_index_once():
+    """the classes of (pq|rs) under the eight permutations: m (m + 1) / 2 of them with
+    m = L (L + 1) / 2, and their images partition all L^4 index tuples"""
+    import itertools
+
+    from qcmaquis_amd.rdm import eri_images, eri_orbits
+    for L in (1, 2, 3, 5):
+        orbits = eri_orbits(L)
+        m = L * (L + 1) // 2
+        assert len(orbits) == m * (m + 1) // 2
+        seen = [img for e in orbits for img in eri_images(*e)]
+        assert sorted(seen) == sorted(itertools.product(range(L), repeat=4))
