@@ -1,4 +1,5 @@
-"""Many independent site problems at once: lock-step Jacobi-Davidson over a batch plan.
+"""Many independent site problems at once: lock-step Jacobi-Davidson over a batch plan, and
+independent two-site DMRG runs swept concurrently (BatchTwoSiteDMRG).
 
 Every iteration applies H to the new search vector of every unconverged instance with one H*psi
 of the batch plan (one grouped launch per stage for all instances, engine.HpsiPlanBatch); the
@@ -9,9 +10,12 @@ Gram-Schmidt against the subspace with one refinement when the norm drops below 
 start, the Rayleigh matrix, its lowest eigenpair, the residual r = Hu - theta u; an instance stops
 when |r| <= tol |theta| or |r| < tol, or after max_iter iterations.
 """
+import time
+
 import numpy as np
 
 from . import abi
+from .engine import HpsiPlanBatch
 
 
 def lockstep_davidson(batch, psi0s, tol=1e-8, max_iter=8, precision=abi.MQ_PREC_FP64):
@@ -60,3 +64,46 @@ def lockstep_davidson(batch, psi0s, tol=1e-8, max_iter=8, precision=abi.MQ_PREC_
         if not active.any():
             break
     return theta.copy(), u, its, n_hpsi
+
+
+class BatchTwoSiteDMRG:
+    """Independent two-site DMRG runs (dmrg.TwoSiteDMRG, all on one context) swept concurrently.
+    At every step the current two-site problems of all runs that still have one form one batch
+    plan; the device lock-step Davidson (mq_davidson_batch: one batched H*psi per iteration) solves
+    them together, then each run splits its vector and grows its boundary as TwoSiteDMRG.step does.
+    A sweep of a run is its 2L-2 steps in the reference's order (ts_optimize.hpp:128-131); runs of
+    different lengths finish their sweeps at different steps."""
+
+    def __init__(self, runs, precision=abi.MQ_PREC_FP64):
+        if not runs:
+            raise ValueError("no runs")
+        ctx = runs[0].ctx
+        if any(r.ctx is not ctx for r in runs):
+            raise ValueError("the runs of a batch share one context")
+        self.runs, self.ctx, self.precision = list(runs), ctx, precision
+        self.timing = {"plan": 0.0, "davidson": 0.0, "accept": 0.0}
+
+    def sweep(self):
+        """one sweep of every run; returns the runs' last energies"""
+        seqs = [[(i, True) for i in range(r.L - 1)] + [(i, False) for i in range(r.L - 2, -1, -1)] for r in self.runs]
+        for r in self.runs:
+            r.iteration_results = {}
+        tol, max_iter = self.runs[0].tol, self.runs[0].max_iter
+        for t in range(max(len(q) for q in seqs)):
+            act = [k for k in range(len(self.runs)) if t < len(seqs[k])]
+            probs = [self.runs[k].site_problem(seqs[k][t][0]) for k in act]
+            t0 = time.perf_counter()
+            batch = HpsiPlanBatch(self.ctx, [p[:4] for p in probs])
+            t1 = time.perf_counter()
+            e, vecs, _, _ = batch.davidson([p[4] for p in probs], tol=tol, max_iter=max_iter, precision=self.precision)
+            batch.close()
+            t2 = time.perf_counter()
+            for k, ek, v, p in zip(act, e, vecs, probs):
+                i, to_right = seqs[k][t]
+                self.runs[k].accept(i, to_right, ek, v, p[3])
+            self.timing["plan"] += t1 - t0
+            self.timing["davidson"] += t2 - t1
+            self.timing["accept"] += time.perf_counter() - t2
+        for r in self.runs:
+            r.sweep_number += 1
+        return [r.energies[-1] for r in self.runs]

@@ -342,6 +342,26 @@ class TwoSiteDMRG:
         self.timing["plan_plain"] = self.timing.get("plan_plain", 0.0) + pst["seconds_plain"]
         self.timing["hpsi"] += st["hpsi_seconds"]
         plan.close()
+        tw, lb, t4, t5 = self.accept(i, to_right, e, psi, lay2)
+        for k, v in (("merge", tm - t0), ("plan", t2 - t1), ("davidson", t3 - t2), ("svd", t4 - t3s), ("grow", t5 - t4)):
+            self.timing[k] += v
+        if self.verbose:
+            print(f"  site {i} {'->' if to_right else '<-'} E={e + self.ham.ecore:.12f} M={sum(lb.left.sizes)} "
+                  f"tw={tw:.2e} it={st['iterations']} plan {t2 - t1:.3f} s (tables {pst['seconds_tables']:.3f}, "
+                  f"blocks {pst['seconds_blocks']:.3f}, terms {pst['n_terms']}) davidson {t3 - t2:.3f} s "
+                  f"svd {t4 - t3s:.3f} s grow {t5 - t4:.3f} s", flush=True)
+        return e + self.ham.ecore
+
+    def site_problem(self, i):
+        """the two-site problem at sites (i, i+1): (two-site MPO, left boundary, right boundary,
+        two-site layout, start vector)"""
+        lay2, psi2 = merge_two(self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1], self.phys2, self.off12)
+        return self.ts_mpo(i), self.left[i], self.right[i + 2], lay2, psi2
+
+    def accept(self, i, to_right, e, psi, lay2):
+        """the second half of a step: split the solved two-site vector psi (energy e, without the
+        core energy) into sites i, i+1, grow the boundary in the sweep direction and record the
+        step's results; returns (truncated weight, layout of site i+1, split end, growth end)"""
         trunc = {}
         la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right, trunc)
         t4 = time.perf_counter()
@@ -356,20 +376,13 @@ class TwoSiteDMRG:
         g = (self.left[i + 1] if to_right else self.right[i + 1]).growth_stats()
         for k in ("tables", "blocks", "upload", "device"):
             self.timing["grow_" + k] = self.timing.get("grow_" + k, 0.0) + g["seconds_" + k]
-        for k, v in (("merge", tm - t0), ("plan", t2 - t1), ("davidson", t3 - t2), ("svd", t4 - t3s), ("grow", t5 - t4)):
-            self.timing[k] += v
         self.energies.append(e + self.ham.ecore)
         self.trunc.append(tw)
         # per-step results of the sweep (ts_optimize.hpp:272, 348-351)
         self.iteration_results.setdefault("Energy", []).append(e + self.ham.ecore)
         for k, v in trunc.items():
             self.iteration_results.setdefault(k, []).append(v)
-        if self.verbose:
-            print(f"  site {i} {'->' if to_right else '<-'} E={e + self.ham.ecore:.12f} M={sum(lb.left.sizes)} "
-                  f"tw={tw:.2e} it={st['iterations']} plan {t2 - t1:.3f} s (tables {pst['seconds_tables']:.3f}, "
-                  f"blocks {pst['seconds_blocks']:.3f}, terms {pst['n_terms']}) davidson {t3 - t2:.3f} s "
-                  f"svd {t4 - t3s:.3f} s grow {t5 - t4:.3f} s", flush=True)
-        return e + self.ham.ecore
+        return tw, lb, t4, t5
 
     def sweep(self):
         """one full sweep (2L-2 two-site steps, ts_optimize.hpp:128-131); with a result file, the

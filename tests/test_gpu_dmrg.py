@@ -137,3 +137,35 @@ def test_result_file_and_restart_small6(ctx2u1, tmp_path):
     print(f"  restarted sweep: E={e:.12f} (FCI {g['e0']:.12f})", flush=True)
     assert abs(e - g["e0"]) < 1e-9
     assert R.ResultArchive(rf).sweeps() == [0, 1, 2, 3]
+
+
+def test_batch_sweeps_match_individual_runs(ctx2u1):
+    """three runs of different lengths swept concurrently (BatchTwoSiteDMRG: one batch plan and one
+    device lock-step Davidson per step) give, step by step, the energies of the same runs swept one
+    by one (same start states, same iteration); the 6-orbital run reaches its FCI energy"""
+    from qcmaquis_amd.batch import BatchTwoSiteDMRG
+    g = GOLD["small6"]
+    specs = [(g["L"], g["n_up"], g["n_dn"], g["seed"], g["rank"]), (5, 3, 2, 11, 6), (4, 2, 2, 12, 5)]
+
+    def runs():
+        out = []
+        for L, nu, nd, seed, rank in specs:
+            h, eri, ec = make_integrals(L, seed, rank)
+            out.append(TwoSiteDMRG(QCHamiltonian(h, eri, ec), nu, nd, M=400, ctx=ctx2u1, tol=1e-10, max_iter=12,
+                                   seed=seed))
+        return out
+
+    print("batch sweeps vs individual runs", flush=True)
+    single = runs()
+    for r in single:
+        for _ in range(3):
+            r.sweep()
+    batch = BatchTwoSiteDMRG(runs())
+    for _ in range(3):
+        batch.sweep()
+    for a, b in zip(single, batch.runs):
+        ea, eb = np.array(a.iteration_results["Energy"]), np.array(b.iteration_results["Energy"])
+        print(f"  L={a.L}: last sweep max |dE| {np.abs(ea - eb).max():.2e}, E={eb[-1]:.12f}", flush=True)
+        assert ea.shape == eb.shape and np.abs(ea - eb).max() < 1e-9
+        assert b.sweep_number == 3
+    assert abs(batch.runs[0].energies[-1] - g["e0"]) < 1e-9
