@@ -9,10 +9,10 @@ sblock_kernel<double>) and of the grouped FP64 GEMMs, from HIP events recorded o
 and the CPU oracle (restated reference CPU path, OpenMP + OpenBLAS) timed on the host cores on the
 same site problem.
 
-  python bench.py [--steps K] [--warmup W] [--workload synthetic|batch] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--workload synthetic|batch|batch-sweep] [--no-cpu]
 
 --workload batch measures configuration 4 (many independent site problems in one batch plan) and
-prints its own line.
+prints its own line; --workload batch-sweep sweeps such runs concurrently against one by one.
 """
 import argparse
 import json
@@ -142,6 +142,67 @@ def batch_instance(ctx, L, M, seed):
         Lb = lstep(ctx, DeviceMPO(ctx, ham.mpo(k)), Lb, m, m)
     lay2, psi2 = merge_two(lay[mid], data[mid], lay[mid + 1], data[mid + 1], ham.phys(True), twosite_offsets(phys))
     return (DeviceMPO(ctx, ham.mpo(mid, True)), Lb, R, lay2), psi2 / np.linalg.norm(psi2)
+
+
+def batch_sweep_bench(args, ctx):
+    """Configuration 4 as sweeps: `--instances` runs (sizes and integrals as in batch_bench), each
+    a two-site DMRG from a right-normalised random MPS of bond dimension about M; one sweep of
+    every run one after the other (TwoSiteDMRG.sweep), then one sweep of fresh copies swept
+    concurrently (BatchTwoSiteDMRG: one batch plan and one device lock-step Davidson per step).
+    FP64 Davidson, 8 iterations, tol 1e-8."""
+    from qcmaquis_amd import abi
+    from qcmaquis_amd.batch import BatchTwoSiteDMRG
+    from qcmaquis_amd.dmrg import TwoSiteDMRG, right_normalize
+    from qcmaquis_amd.model import MPSLayout
+    from qcmaquis_amd.qc import QCHamiltonian, make_integrals
+    rng = np.random.default_rng(1004)
+    shapes = [(int(rng.integers(10, 21)), int(rng.choice([128, 256, 512]))) for _ in range(args.instances)]
+
+    def make_runs():
+        runs = []
+        for i, (L, M) in enumerate(shapes):
+            h, eri, ec = make_integrals(L, 1004 + 1 + i, 16)
+            ham = QCHamiltonian(h, eri, ec)
+            phys = ham.phys()
+            bonds = scaled_bonds([phys] * L, (L // 2, L // 2), M)
+            lay = [MPSLayout(bonds[k], phys, bonds[k + 1]) for k in range(L)]
+            r = np.random.default_rng(1004 + 1 + i)
+            data = [r.normal(size=x.n) for x in lay]
+            right_normalize(lay, data)
+            runs.append(TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8, mps=(lay, data)))
+        return runs
+
+    t0 = time.perf_counter()
+    runs = make_runs()
+    log("%d runs built in %.1f s" % (len(runs), time.perf_counter() - t0))
+    single = []
+    t0 = time.perf_counter()
+    for k, r in enumerate(runs):
+        e, dt = r.sweep()
+        single.append((e, list(r.iteration_results["Energy"])))
+        log("run %d (L=%d M=%d): one sweep %.2f s, E=%.10f" % (k, r.L, r.M, dt, e))
+    t_single = time.perf_counter() - t0
+    del runs
+    batch_runs = make_runs()
+    b = BatchTwoSiteDMRG(batch_runs, abi.MQ_PREC_FP64)
+    t0 = time.perf_counter()
+    b.sweep()
+    t_batch = time.perf_counter() - t0
+    de = max(float(np.max(np.abs(np.array(s[1]) - np.array(r.iteration_results["Energy"]))))
+             for s, r in zip(single, batch_runs))
+    log("sweep of %d runs: one after the other %.2f s, concurrently %.2f s (plan %.2f, davidson %.2f, split and "
+        "growth %.2f), max |dE| per step %.2e" % (len(batch_runs), t_single, t_batch, b.timing["plan"],
+                                                  b.timing["davidson"], b.timing["accept"], de))
+    line = {"metric": METRIC, "value": round(t_single / t_batch, 4), "unit": "speedup (one sweep, concurrent vs one "
+            "after the other)", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(t_batch * 1e3, 1),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded integrals, random MPS)",
+            "config": {"workload": "configuration 4 sweeps: %d independent runs (orbitals U{10..20}, half filling, "
+                                   "TwoU1, M from {128, 256, 512}), one two-site sweep each" % len(batch_runs),
+                       "seconds_one_after_the_other": round(t_single, 3), "seconds_concurrent": round(t_batch, 3),
+                       "concurrent_timing_s": {k: round(v, 3) for k, v in b.timing.items()},
+                       "max_abs_delta_e_per_step": de, "runs": [{"L": L, "M": M} for L, M in shapes]}}
+    print(json.dumps(line), flush=True)
 
 
 def batch_bench(args, ctx):
@@ -396,7 +457,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="qc", choices=["qc", "synthetic", "batch"])
+    ap.add_argument("--workload", default="qc", choices=["qc", "synthetic", "batch", "batch-sweep"])
     ap.add_argument("--instances", type=int, default=64, help="instances of the batch workload")
     ap.add_argument("--warm-sweeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -427,8 +488,8 @@ def main():
         print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}), flush=True)
         return
     ctx = Context(0, abi.MQ_SYM_TWOU1)
-    if args.workload == "batch":
-        batch_bench(args, ctx)
+    if args.workload in ("batch", "batch-sweep"):
+        (batch_bench if args.workload == "batch" else batch_sweep_bench)(args, ctx)
         ctx.close()
         return
     t0 = time.perf_counter()

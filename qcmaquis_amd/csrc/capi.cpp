@@ -6,6 +6,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -879,55 +880,32 @@ int mq_hpsi_plan_create_batch(mq_ctx_t ctx, int n, const mq_mpo_t* mpos, const m
         auto* p = new mq_hpsi_plan_s{ctx, HpsiPlanHost{}};
         std::unique_ptr<mq_hpsi_plan_s> guard(p);
         HpsiPlanHost& H = p->host;
+        // every instance's tables (each build runs on all host cores), then their offsets in the
+        // merged tables, then the merged tables written in parallel with the offsets applied
+        std::vector<HpsiPlanHost> parts(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i)
+            parts[size_t(i)] = build_hpsi_plan(ctx->sym, mpos[i]->mpo, lefts[i]->idx, rights[i]->idx,
+                                               index_from(&left_is[i]), index_from(&phys_is[i]),
+                                               index_from(&right_is[i]));
+        H.mps = parts[0].mps;
+        const size_t un = size_t(n);
+        std::vector<std::array<int64_t, BASE_COUNT>> shift(un);
+        std::vector<int64_t> o_tg(un + 1, 0), o_rows(un + 1, 0), o_cg(un + 1, 0), o_lines(un + 1, 0), o_terms(un + 1, 0);
         uint64_t r_n = 0;
-        std::vector<LpanelPart> lparts;
-        std::vector<size_t> line_ranges{0};
-        int64_t shift[BASE_COUNT] = {0};
-        for (int i = 0; i < n; ++i) {
+        for (size_t i = 0; i < un; ++i) {
+            HpsiPlanHost const& Q = parts[i];
+            shift[i].fill(0);
+            shift[i][BASE_PSI_IN] = shift[i][BASE_PSI_OUT] = int64_t(p->psi_n);
+            shift[i][BASE_T] = int64_t(H.t_elems);
+            shift[i][BASE_S] = int64_t(H.s_elems);
+            shift[i][BASE_LPANEL] = int64_t(H.lpanel_elems);
+            shift[i][BASE_BOUNDARY] = int64_t(r_n);
+            o_tg[i + 1] = o_tg[i] + int64_t(Q.t_gemms.size());
+            o_rows[i + 1] = o_rows[i] + int64_t(Q.t_rows.size());
+            o_cg[i + 1] = o_cg[i] + int64_t(Q.c_gemms.size());
+            o_lines[i + 1] = o_lines[i] + int64_t(Q.lines.size());
+            o_terms[i + 1] = o_terms[i] + int64_t(Q.terms.size());
             p->inst_off.push_back(int64_t(p->psi_n));
-            // built and merged one at a time: the host holds the merged tables and one instance's
-            HpsiPlanHost Q = build_hpsi_plan(ctx->sym, mpos[i]->mpo, lefts[i]->idx, rights[i]->idx,
-                                             index_from(&left_is[i]), index_from(&phys_is[i]),
-                                             index_from(&right_is[i]));
-            if (i == 0) H.mps = Q.mps;
-            shift[BASE_PSI_IN] = shift[BASE_PSI_OUT] = int64_t(p->psi_n);
-            shift[BASE_T] = int64_t(H.t_elems);
-            shift[BASE_S] = int64_t(H.s_elems);
-            shift[BASE_LPANEL] = int64_t(H.lpanel_elems);
-            shift[BASE_BOUNDARY] = int64_t(r_n);
-            const int64_t row0 = int64_t(H.t_rows.size()), term0 = int64_t(H.terms.size());
-            for (GemmDesc g : Q.t_gemms) {
-                if (g.flags & GEMM_ROWMAP) g.a += row0;
-                else g.a += shift[g.base_a];
-                g.b += shift[g.base_b];
-                if (!(g.flags & GEMM_ROWMAP)) g.c += shift[g.base_c];
-                H.t_gemms.push_back(g);
-            }
-            for (GemmRow r : Q.t_rows) {
-                r.a += shift[BASE_PSI_IN];
-                r.c += shift[BASE_T];
-                H.t_rows.push_back(r);
-            }
-            for (GemmDesc g : Q.c_gemms) {
-                g.a += shift[g.base_a];
-                g.b += shift[g.base_b];
-                g.c += shift[g.base_c];
-                H.c_gemms.push_back(g);
-            }
-            for (SLine l : Q.lines) {
-                l.s_off += shift[BASE_S];
-                l.t0 += int32_t(term0);
-                l.t1 += int32_t(term0);
-                H.lines.push_back(l);
-            }
-            require(term0 + int64_t(Q.terms.size()) < int64_t(INT32_MAX), "batch: more than 2^31 terms");
-            for (STerm t : Q.terms) {
-                t.t_off += shift[BASE_T];
-                H.terms.push_back(t);
-            }
-            LpanelPart lp{lefts[i]->data, lefts[i]->n, Q.lpanel_jobs};
-            for (auto& j : lp.jobs) j.dst += shift[BASE_LPANEL];
-            lparts.push_back(std::move(lp));
             H.t_elems += Q.t_elems;
             H.s_elems += Q.s_elems;
             H.lpanel_elems += Q.lpanel_elems;
@@ -940,8 +918,64 @@ int mq_hpsi_plan_create_batch(mq_ctx_t ctx, int n, const mq_mpo_t* mpos, const m
             H.flops.n_terms += Q.flops.n_terms;
             p->psi_n += Q.mps.elements();
             r_n += rights[i]->n;
-            line_ranges.push_back(H.lines.size());
         }
+        require(o_terms[un] < int64_t(INT32_MAX), "batch: more than 2^31 terms");
+        H.t_gemms.resize(size_t(o_tg[un]));
+        H.t_rows.resize(size_t(o_rows[un]));
+        H.c_gemms.resize(size_t(o_cg[un]));
+        H.lines.resize(size_t(o_lines[un]));
+        H.terms.resize(size_t(o_terms[un]));
+        std::vector<LpanelPart> lparts;
+        std::vector<size_t> line_ranges;
+        for (size_t i = 0; i < un; ++i) {
+            HpsiPlanHost& Q = parts[i];
+            auto const& sh = shift[i];
+            const int64_t row0 = o_rows[i];
+            const int32_t term0 = int32_t(o_terms[i]);
+            for (size_t k = 0; k < Q.t_gemms.size(); ++k) {
+                GemmDesc g = Q.t_gemms[k];
+                if (g.flags & GEMM_ROWMAP) g.a += row0;
+                else g.a += sh[g.base_a];
+                g.b += sh[g.base_b];
+                if (!(g.flags & GEMM_ROWMAP)) g.c += sh[g.base_c];
+                H.t_gemms[size_t(o_tg[i]) + k] = g;
+            }
+            for (size_t k = 0; k < Q.c_gemms.size(); ++k) {
+                GemmDesc g = Q.c_gemms[k];
+                g.a += sh[g.base_a];
+                g.b += sh[g.base_b];
+                g.c += sh[g.base_c];
+                H.c_gemms[size_t(o_cg[i]) + k] = g;
+            }
+            const int64_t nr = int64_t(Q.t_rows.size()), nl = int64_t(Q.lines.size()), nt = int64_t(Q.terms.size());
+#pragma omp parallel for schedule(static)
+            for (int64_t k = 0; k < nr; ++k) {
+                GemmRow r = Q.t_rows[size_t(k)];
+                r.a += sh[BASE_PSI_IN];
+                r.c += sh[BASE_T];
+                H.t_rows[size_t(row0 + k)] = r;
+            }
+#pragma omp parallel for schedule(static)
+            for (int64_t k = 0; k < nl; ++k) {
+                SLine l = Q.lines[size_t(k)];
+                l.s_off += sh[BASE_S];
+                l.t0 += term0;
+                l.t1 += term0;
+                H.lines[size_t(o_lines[i] + k)] = l;
+            }
+#pragma omp parallel for schedule(static)
+            for (int64_t k = 0; k < nt; ++k) {
+                STerm t = Q.terms[size_t(k)];
+                t.t_off += sh[BASE_T];
+                H.terms[size_t(o_terms[i] + k)] = t;
+            }
+            LpanelPart lp{lefts[i]->data, lefts[i]->n, Q.lpanel_jobs};
+            for (auto& j : lp.jobs) j.dst += sh[BASE_LPANEL];
+            lparts.push_back(std::move(lp));
+            line_ranges.push_back(size_t(o_lines[i]));
+            Q = HpsiPlanHost{};  // merged: release the instance's tables
+        }
+        line_ranges.push_back(size_t(o_lines[un]));
         p->inst_off.push_back(int64_t(p->psi_n));
         p->n_instances = n;
         p->d_inst_off = dev_upload(p->inst_off, ctx->stream);
