@@ -1417,22 +1417,34 @@ GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t tar
     require(min_chunk >= 16 && min_chunk % 16 == 0, "gemm schedule: chunk must be a multiple of 16");
     GemmSchedule S;
     // 96-row tiles (for 64 < M <= 96) only for problems with a plain A operand: a stacked problem's
-    // threads each load one row, which needs a tile height that divides 256
-    auto shape_of = [](GemmDesc const& g) {
+    // threads each load one row, which needs a tile height that divides 256.  64x128 tiles (for
+    // 32 < M <= 64, N > 64) only when the stage has too few tiles to fill the device and is split
+    // along K anyway: then the wider tile halves the A re-reads at no cost in parallelism
+    bool wide = false;
+    auto shape_of = [&](GemmDesc const& g) {
         return g.M <= 32 ? GEMM_32x128
-             : g.M <= 64 ? (g.N > 64 ? GEMM_64x128 : GEMM_64x64)
+             : g.M <= 64 ? ((wide && g.N > 64) ? GEMM_64x128 : GEMM_64x64)
              : (g.M <= 96 && !(g.flags & GEMM_ROWMAP)) ? GEMM_96x64 : GEMM_128x64;
     };
     auto tiles_of = [&](GemmDesc const& g) {
         int sh = shape_of(g);
         return int64_t((g.M + gemm_bm(sh) - 1) / gemm_bm(sh)) * ((g.N + gemm_bn(sh) - 1) / gemm_bn(sh)) * g.batch;
     };
+    auto count = [&](int64_t& n_tiles, double& work) {
+        n_tiles = 0;
+        work = 0;
+        for (auto const& g : probs) {
+            if (g.M <= 0 || g.N <= 0) continue;
+            n_tiles += tiles_of(g);
+            work += double(tiles_of(g)) * g.K;
+        }
+    };
     int64_t n_tiles = 0;
     double work = 0;
-    for (auto const& g : probs) {
-        if (g.M <= 0 || g.N <= 0) continue;
-        n_tiles += tiles_of(g);
-        work += double(tiles_of(g)) * g.K;
+    count(n_tiles, work);
+    if (n_tiles < target_items && target_items > 0) {
+        wide = true;
+        count(n_tiles, work);
     }
     // K range per workgroup when the stage alone cannot fill the device
     int64_t chunk = INT64_MAX;

@@ -138,8 +138,8 @@ constexpr int SMALL_LINE_LANES = 256;
 constexpr int SBLOCK_ELEMS = 256;
 
 // Execution schedule of a grouped GEMM stage.  Output tiles come in five shapes (BM x BN):
-// 32x128 for M <= 32, 64x128 (N > 64) or 64x64 for M <= 64, 96x64 for M <= 96 (problems without a
-// row table), 128x64 above; so a problem with a short M edge is not padded to a taller tile and the B panel of a tall
+// 32x128 for M <= 32, 64x64 for M <= 64 (64x128 when N > 64 and the stage is split along K), 96x64
+// for M <= 96 (problems without a row table), 128x64 above; so a problem with a short M edge is not padded to a taller tile and the B panel of a tall
 // problem is read once.  When the stage has too few
 // tiles to fill the device, long reductions are split along K: each split tile writes its
 // partial BM x BN tile (column-major, ld BM) into a workspace and a reduce job adds the
