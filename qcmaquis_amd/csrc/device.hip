@@ -474,7 +474,7 @@ namespace {
 
 // K slices of 8 with at most 128 VGPRs (two workgroups per SIMD): measured fastest for every tile
 // shape on the H*psi stages of a 24-orbital, M = 512 site problem (short and long reductions alike)
-constexpr int GEMM_KS = 8, GEMM_MINB = 2;
+constexpr int GEMM_KS = 8, GEMM_MINB = 4;
 
 template <class TI, class TO>
 void launch_gemm_t(const GemmDesc* probs, const GemmRow* rows, const GemmTile* const* tiles, const int64_t* n_tiles,
