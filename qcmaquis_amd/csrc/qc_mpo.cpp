@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "core.hpp"
+#include "qc_mpo.hpp"
 #include "qcmaquis_amd.h"
 
 namespace mq {
@@ -139,46 +140,6 @@ struct Cell {
         for (auto& x : terms)
             if (x.first == t) { x.second += s; return; }
         terms.push_back({t, s});
-    }
-};
-
-// flattened MPO of one site (arrays of an mq_mpo_desc)
-struct FlatMPO {
-    uint32_t row_dim = 1, col_dim = 1;
-    std::vector<int32_t> op_spin;
-    std::vector<int64_t> op_block_ptr{0};
-    std::vector<int32_t> block_charges;
-    std::vector<int64_t> block_entry_ptr{0};
-    std::vector<uint32_t> entry_rc;
-    std::vector<double> entry_coeff;
-    std::vector<uint32_t> term_b, term_op;
-    std::vector<double> term_scale;
-    std::vector<int32_t> lspin, rspin, lphase, rphase;
-    std::vector<uint32_t> lherm, rherm;
-    std::vector<uint32_t> tag_to_op;  // local tag -> op index in this tensor
-
-    void fill_desc(mq_mpo_desc& d) const {
-        std::memset(&d, 0, sizeof(d));
-        d.n_ops = int32_t(op_spin.size());
-        d.op_spin = op_spin.data();
-        d.op_block_ptr = op_block_ptr.data();
-        d.block_charges = block_charges.data();
-        d.block_entry_ptr = block_entry_ptr.data();
-        d.entry_rc = entry_rc.data();
-        d.entry_spins = nullptr;
-        d.entry_coeff = entry_coeff.data();
-        d.row_dim = row_dim;
-        d.col_dim = col_dim;
-        d.n_terms = int64_t(term_op.size());
-        d.term_b = term_b.data();
-        d.term_op = term_op.data();
-        d.term_scale = term_scale.data();
-        d.left_spin = lspin.data();
-        d.right_spin = rspin.data();
-        d.left_herm = lherm.data();
-        d.left_phase = lphase.data();
-        d.right_herm = rherm.data();
-        d.right_phase = rphase.data();
     }
 };
 
@@ -607,33 +568,43 @@ struct QCBuilder {
 }  // namespace mq
 
 struct mq_qc_s {
-    mq::QCBuilder b;
+    int sym = MQ_SYM_TWOU1;
+    mq::QCBuilder b;                                          // TwoU1
+    std::unique_ptr<mq::SU2QC, mq::SU2QCDeleter> su2;         // SU2U1
     std::vector<std::unique_ptr<mq::FlatMPO>> two;
     std::vector<int32_t> phys_charges[2];
     std::vector<uint32_t> phys_sizes[2];
     std::string err;
+    int length() const { return sym == MQ_SYM_SU2U1 ? mq::su2qc_length(*su2) : b.L; }
 };
 
 extern "C" {
 
 int mq_qc_build(int symmetry, int L, const double* h1, const double* eri, double ecore, double cutoff, mq_qc_t* out) {
     if (!out || L < 2 || !h1 || !eri) return MQ_ERR_INVALID;
-    if (symmetry != MQ_SYM_TWOU1) return MQ_ERR_UNSUPPORTED;
+    if (symmetry != MQ_SYM_TWOU1 && symmetry != MQ_SYM_SU2U1) return MQ_ERR_UNSUPPORTED;
     try {
-        auto* q = new mq_qc_s();
-        q->b.L = L;
+        std::unique_ptr<mq_qc_s> q(new mq_qc_s());
+        q->sym = symmetry;
         q->b.ecore = ecore;
-        q->b.build(h1, eri, cutoff);
-        q->two.resize(L - 1);
-        for (int k = 0; k < 2; ++k) {
-            mq::PhysLayout P = k ? mq::two_site_layout() : mq::one_site_layout();
-            for (size_t i = 0; i < P.charges.size(); ++i) {
-                q->phys_charges[k].push_back(P.charges[i].q[0]);
-                q->phys_charges[k].push_back(P.charges[i].q[1]);
-                q->phys_sizes[k].push_back(P.sizes[i]);
+        if (symmetry == MQ_SYM_SU2U1) {
+            q->b.L = L;
+            q->su2 = mq::su2qc_build(L, h1, eri, cutoff);
+            for (int k = 0; k < 2; ++k) mq::su2qc_phys(k, q->phys_charges[k], q->phys_sizes[k]);
+        } else {
+            q->b.L = L;
+            q->b.build(h1, eri, cutoff);
+            q->two.resize(L - 1);
+            for (int k = 0; k < 2; ++k) {
+                mq::PhysLayout P = k ? mq::two_site_layout() : mq::one_site_layout();
+                for (size_t i = 0; i < P.charges.size(); ++i) {
+                    q->phys_charges[k].push_back(P.charges[i].q[0]);
+                    q->phys_charges[k].push_back(P.charges[i].q[1]);
+                    q->phys_sizes[k].push_back(P.sizes[i]);
+                }
             }
         }
-        *out = q;
+        *out = q.release();
         return MQ_OK;
     } catch (mq::Error const& e) {
         fprintf(stderr, "mq_qc_build: %s\n", e.what());
@@ -646,23 +617,29 @@ int mq_qc_build(int symmetry, int L, const double* h1, const double* eri, double
 
 int mq_qc_info(mq_qc_t q, int* L, uint32_t* bond_dims, double* ecore) {
     if (!q) return MQ_ERR_INVALID;
-    if (L) *L = q->b.L;
-    if (bond_dims) std::copy(q->b.bond_dims.begin(), q->b.bond_dims.end(), bond_dims);
+    if (L) *L = q->length();
+    if (bond_dims) {
+        auto const& bd = q->sym == MQ_SYM_SU2U1 ? mq::su2qc_bond_dims(*q->su2) : q->b.bond_dims;
+        std::copy(bd.begin(), bd.end(), bond_dims);
+    }
     if (ecore) *ecore = q->b.ecore;
     return MQ_OK;
 }
 
 int mq_qc_site_mpo(mq_qc_t q, int site, int twosite, mq_mpo_desc* out) {
-    if (!q || !out || site < 0 || site >= q->b.L || (twosite && site >= q->b.L - 1)) return MQ_ERR_INVALID;
+    if (!q || !out || site < 0 || site >= q->length() || (twosite && site >= q->length() - 1)) return MQ_ERR_INVALID;
     try {
-        if (!twosite) {
+        if (q->sym == MQ_SYM_SU2U1) {
+            (twosite ? mq::su2qc_twosite(*q->su2, site) : mq::su2qc_site(*q->su2, site)).fill_desc(*out);
+        } else if (!twosite) {
             q->b.site_mpo[site].fill_desc(*out);
         } else {
             if (!q->two[site]) q->two[site].reset(new mq::FlatMPO(q->b.twosite(site)));
             q->two[site]->fill_desc(*out);
         }
         return MQ_OK;
-    } catch (...) {
+    } catch (std::exception const& e) {
+        fprintf(stderr, "mq_qc_site_mpo: %s\n", e.what());
         return MQ_ERR_INVALID;
     }
 }

@@ -66,14 +66,18 @@ def merge_two(la, a, lb, b, phys2, off12):
     return lay2, out
 
 
-def split_matrices(lay2, psi2, phys, off12):
+def split_matrices(lay2, psi2, phys, off12, sym=abi.MQ_SYM_TWOU1):
     """X_m[(s1, l), (s2, r)] for every middle charge m, rows in ProductBasis(phys, left) order and
-    columns in ProductBasis(phys, right, -s2 + r) order"""
+    columns in ProductBasis(phys, right, -s2 + r) order; SU2U1 middle charges of negative spin
+    are not states and are left out"""
     left, right = lay2.left, lay2.right
     mids = {}
     for lc in left.charges:
         for s1 in phys.charges:
-            mids.setdefault(qadd(lc, s1), None)
+            m = qadd(lc, s1)
+            if sym == abi.MQ_SYM_SU2U1 and m[1] < 0:
+                continue
+            mids.setdefault(m, None)
     mats = {}
     for m in mids:
         rows, rows_n = [], 0
@@ -148,11 +152,11 @@ def block_svds(blocks):
     return {m: out[m] for m in blocks}
 
 
-def split_two(lay2, psi2, phys, off12, Mmax, cutoff, to_right, details=None):
-    """SVD split of a two-site tensor; to_right: A = U left-normalised, B = S V^T (split_mps_l2r),
-    else A = U S, B = V^T right-normalised (split_mps_r2l); details (a dict) receives the
-    truncation record"""
-    mats = split_matrices(lay2, psi2, phys, off12)
+def split_two(lay2, psi2, phys, off12, Mmax, cutoff, to_right, details=None, sym=abi.MQ_SYM_TWOU1):
+    """SVD split of a two-site tensor (sequentially coupled basis for SU2U1); to_right: A = U
+    left-normalised, B = S V^T (split_mps_l2r), else A = U S, B = V^T right-normalised
+    (split_mps_r2l); details (a dict) receives the truncation record"""
+    mats = split_matrices(lay2, psi2, phys, off12, sym)
     svd = block_svds({m: X for m, (X, _, _) in mats.items()})
     keeps, tw = estimate_truncation({m: s[1] for m, s in svd.items()}, Mmax, cutoff)
     if details is not None:
@@ -211,6 +215,42 @@ def allowed_bonds(phys_list, total, m_cap, full_right=False):
     return bonds
 
 
+def allowed_bonds_su2(phys_list, total, m_cap, full_right=False):
+    """SU2U1 bond indices: sectors (N, 2S) with 2S >= 0 reachable from both ends under plain charge
+    addition, sized by the number of spin-coupling paths (configuration state functions) from the
+    left end, the right end or m_cap, as allowed_bonds"""
+    n = len(phys_list)
+    fwd = [{(0, 0): 1}]
+    for p in phys_list:
+        nxt = {}
+        for c, m in fwd[-1].items():
+            for s in p.charges:
+                q = qadd(c, s)
+                if q[1] >= 0:
+                    nxt[q] = nxt.get(q, 0) + m
+        fwd.append(nxt)
+    bwd = [None] * (n + 1)
+    bwd[n] = {tuple(total): 1}
+    for k in range(n - 1, -1, -1):
+        prev = {}
+        for c, m in bwd[k + 1].items():
+            for s in phys_list[k].charges:
+                q = qsub(c, s)
+                if q[1] >= 0:
+                    prev[q] = prev.get(q, 0) + m
+        bwd[k] = prev
+    bonds = []
+    for k in range(n + 1):
+        d = {}
+        for c, left in fwd[k].items():
+            right = bwd[k].get(c, 0)
+            if right == 0:
+                continue
+            d[c] = (right if k > 0 else 1) if full_right else min(left, right, m_cap)
+        bonds.append(Index.from_dict(d))
+    return bonds
+
+
 def right_normalize(layouts, data):
     """right-canonical form by QR sweeps from the right (MPS::canonize)"""
     n = len(layouts)
@@ -246,8 +286,18 @@ def right_normalize(layouts, data):
     return layouts, data
 
 
+def initial_bonds(sym, phys, L, n_up, n_dn, m_cap, full_right=False):
+    """bond indices of a state with n_up + n_dn electrons: TwoU1 total charge (n_up, n_dn), SU2U1
+    total charge (N, 2S) = (n_up + n_dn, n_up - n_dn)"""
+    if sym == abi.MQ_SYM_SU2U1:
+        return allowed_bonds_su2([phys] * L, (n_up + n_dn, n_up - n_dn), m_cap, full_right)
+    return allowed_bonds([phys] * L, (n_up, n_dn), m_cap, full_right)
+
+
 class TwoSiteDMRG:
-    """Two-site DMRG for a TwoU1 quantum-chemistry Hamiltonian (QCHamiltonian)."""
+    """Two-site DMRG for a quantum-chemistry Hamiltonian (QCHamiltonian, TwoU1 or SU2U1).  With
+    SU2U1 the two-site vector is recoupled into the pair-coupled basis of the two-site MPO before
+    the Davidson solve and back before the split (su2.TwoSiteReduction)."""
 
     def __init__(self, ham, n_up, n_dn, M, ctx=None, init_m=8, seed=42, tol=1e-8, max_iter=8, cutoff=1e-16,
                  verbose=False, precision=abi.MQ_PREC_FP64, mps=None, resultfile=None, first_sweep=0):
@@ -260,14 +310,15 @@ class TwoSiteDMRG:
         self.compare_precision = None  # a list: every step also runs Davidson in the other precision
         self.compare_plain = False  # with compare_precision: also the MQ_PREC_FP64_PLAIN Davidson
         self.verbose = verbose
-        self.ctx = ctx or Context(0, abi.MQ_SYM_TWOU1)
+        self.sym = getattr(ham, "sym", abi.MQ_SYM_TWOU1)
+        self.ctx = ctx or Context(0, self.sym)
         self.L = ham.L
         self.phys = ham.phys()
         self.phys2 = ham.phys(True)
         self.off12 = twosite_offsets(self.phys)
         if mps is None:
             rng = np.random.default_rng(seed)
-            bonds = allowed_bonds([self.phys] * self.L, (n_up, n_dn), init_m)
+            bonds = initial_bonds(self.sym, self.phys, self.L, n_up, n_dn, init_m)
             self.lay = [MPSLayout(bonds[k], self.phys, bonds[k + 1]) for k in range(self.L)]
             self.data = [rng.normal(size=l.n) for l in self.lay]
         else:
@@ -297,9 +348,20 @@ class TwoSiteDMRG:
             self.mpo2[i] = DeviceMPO(self.ctx, self.ham.mpo(i, True))
         return self.mpo2[i]
 
+    def _merge(self, i):
+        """the two-site vector of sites (i, i+1) in the basis the two-site MPO acts on, and the
+        SU2U1 recoupling (None for TwoU1)"""
+        lay2, psi2 = merge_two(self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1], self.phys2, self.off12)
+        red = None
+        if self.sym == abi.MQ_SYM_SU2U1:
+            from .su2 import TwoSiteReduction
+            red = TwoSiteReduction(lay2, self.phys, self.off12)
+            psi2 = red.reduce(psi2)
+        return lay2, psi2, red
+
     def step(self, i, to_right):
         t0 = time.perf_counter()
-        lay2, psi2 = merge_two(self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1], self.phys2, self.off12)
+        lay2, psi2, red = self._merge(i)
         tm = time.perf_counter()
         mpo = self.ts_mpo(i)
         t1 = time.perf_counter()
@@ -342,7 +404,7 @@ class TwoSiteDMRG:
         self.timing["plan_plain"] = self.timing.get("plan_plain", 0.0) + pst["seconds_plain"]
         self.timing["hpsi"] += st["hpsi_seconds"]
         plan.close()
-        tw, lb, t4, t5 = self.accept(i, to_right, e, psi, lay2)
+        tw, lb, t4, t5 = self.accept(i, to_right, e, psi, lay2, red)
         for k, v in (("merge", tm - t0), ("plan", t2 - t1), ("davidson", t3 - t2), ("svd", t4 - t3s), ("grow", t5 - t4)):
             self.timing[k] += v
         if self.verbose:
@@ -354,16 +416,22 @@ class TwoSiteDMRG:
 
     def site_problem(self, i):
         """the two-site problem at sites (i, i+1): (two-site MPO, left boundary, right boundary,
-        two-site layout, start vector)"""
-        lay2, psi2 = merge_two(self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1], self.phys2, self.off12)
+        two-site layout, start vector; SU2U1: in the pair-coupled basis)"""
+        lay2, psi2, _ = self._merge(i)
         return self.ts_mpo(i), self.left[i], self.right[i + 2], lay2, psi2
 
-    def accept(self, i, to_right, e, psi, lay2):
+    def accept(self, i, to_right, e, psi, lay2, red=None):
         """the second half of a step: split the solved two-site vector psi (energy e, without the
-        core energy) into sites i, i+1, grow the boundary in the sweep direction and record the
-        step's results; returns (truncated weight, layout of site i+1, split end, growth end)"""
+        core energy; in the basis of the two-site MPO, recoupled back for SU2U1) into sites i, i+1,
+        grow the boundary in the sweep direction and record the step's results; returns (truncated
+        weight, layout of site i+1, split end, growth end)"""
+        if self.sym == abi.MQ_SYM_SU2U1:
+            if red is None:
+                from .su2 import TwoSiteReduction
+                red = TwoSiteReduction(lay2, self.phys, self.off12)
+            psi = red.unreduce(psi)
         trunc = {}
-        la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right, trunc)
+        la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right, trunc, self.sym)
         t4 = time.perf_counter()
         self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1] = la, a, lb, b
         if to_right:
@@ -449,11 +517,12 @@ class SingleSiteDMRG:
 
     def __init__(self, ham, n_up, n_dn, M, ctx=None, seed=42, tol=1e-8, max_iter=8, precision=abi.MQ_PREC_FP64):
         self.ham, self.tol, self.max_iter, self.precision = ham, tol, max_iter, precision
-        self.ctx = ctx or Context(0, abi.MQ_SYM_TWOU1)
+        self.sym = getattr(ham, "sym", abi.MQ_SYM_TWOU1)
+        self.ctx = ctx or Context(0, self.sym)
         self.L = ham.L
         self.phys = ham.phys()
         rng = np.random.default_rng(seed)
-        bonds = allowed_bonds([self.phys] * self.L, (n_up, n_dn), M)
+        bonds = initial_bonds(self.sym, self.phys, self.L, n_up, n_dn, M)
         self.lay = [MPSLayout(bonds[k], self.phys, bonds[k + 1]) for k in range(self.L)]
         self.data = [rng.normal(size=l.n) for l in self.lay]
         right_normalize(self.lay, self.data)

@@ -98,8 +98,12 @@ class RawMPO:
             for bl in range(d.op_block_ptr[o], d.op_block_ptr[o + 1]):
                 lc = (d.block_charges[4 * bl], d.block_charges[4 * bl + 1])
                 rc = (d.block_charges[4 * bl + 2], d.block_charges[4 * bl + 3])
-                ents = [(d.entry_rc[2 * e], d.entry_rc[2 * e + 1], d.entry_coeff[e])
-                        for e in range(d.block_entry_ptr[bl], d.block_entry_ptr[bl + 1])]
+                if d.entry_spins:
+                    ents = [(d.entry_rc[2 * e], d.entry_rc[2 * e + 1], d.entry_coeff[e], d.entry_spins[2 * e],
+                             d.entry_spins[2 * e + 1]) for e in range(d.block_entry_ptr[bl], d.block_entry_ptr[bl + 1])]
+                else:
+                    ents = [(d.entry_rc[2 * e], d.entry_rc[2 * e + 1], d.entry_coeff[e])
+                            for e in range(d.block_entry_ptr[bl], d.block_entry_ptr[bl + 1])]
                 blocks.append((lc, rc, ents))
             ops.append(SiteOp(blocks, d.op_spin[o] if d.op_spin else 0))
         terms = {}
@@ -112,16 +116,18 @@ class RawMPO:
 
 
 class QCHamiltonian:
-    """TwoU1 quantum-chemistry MPO chain (mq_qc_build)."""
+    """Quantum-chemistry MPO chain (mq_qc_build): TwoU1 (charges (N_up, N_down)) or SU2U1
+    (charges (N, 2S), spin-adapted operators)."""
 
-    def __init__(self, h, eri, ecore=0.0, cutoff=1e-20):
+    def __init__(self, h, eri, ecore=0.0, cutoff=1e-20, symmetry=abi.MQ_SYM_TWOU1):
         self.lib = abi.lib()
+        self.sym = int(symmetry)
         self.L = h.shape[0]
         self.h = np.ascontiguousarray(h, np.float64)
         self.eri = np.ascontiguousarray(eri, np.float64)
         hd = C.c_void_p()
         dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
-        rc = self.lib.mq_qc_build(abi.MQ_SYM_TWOU1, self.L, dp(self.h), dp(self.eri), float(ecore), float(cutoff),
+        rc = self.lib.mq_qc_build(self.sym, self.L, dp(self.h), dp(self.eri), float(ecore), float(cutoff),
                                   C.byref(hd))
         if rc != 0:
             raise RuntimeError(f"mq_qc_build failed ({rc})")
@@ -210,3 +216,27 @@ def fci_energy(h, eri, n_up, n_dn, ecore=0.0, k=1):
     else:
         w = np.sort(sla.eigsh(H, k=k, which="SA", tol=1e-12)[0])
     return w + ecore
+
+
+def fci_spectrum(h, eri, n_up, n_dn, ecore=0.0):
+    """all eigenvalues of H in the (n_up, n_dn) determinant sector (dense, small L only)"""
+    dim = len(_strings(h.shape[0], n_up)) * len(_strings(h.shape[0], n_dn))
+    return fci_energy(h, eri, n_up, n_dn, ecore=ecore, k=dim) if dim <= 400 else None
+
+
+def fci_spin_spectrum(h, eri, n, two_s, ecore=0.0, tol=1e-8):
+    """eigenvalues of the spin-S multiplets of n electrons (one per multiplet): the M_s = S
+    spectrum with the M_s = S + 1 spectrum taken out (every multiplet with S' > S has both)"""
+    lo = np.sort(fci_spectrum(h, eri, (n + two_s) // 2, (n - two_s) // 2, ecore))
+    if (n + two_s) // 2 + 1 > h.shape[0] or (n - two_s) // 2 - 1 < 0:
+        return lo
+    hi = list(np.sort(fci_spectrum(h, eri, (n + two_s) // 2 + 1, (n - two_s) // 2 - 1, ecore)))
+    out = []
+    for e in lo:
+        j = next((x for x, v in enumerate(hi) if abs(v - e) < tol), None)
+        if j is None:
+            out.append(e)
+        else:
+            hi.pop(j)
+    assert not hi, "unmatched higher-spin eigenvalues"
+    return np.array(out)
