@@ -100,48 +100,90 @@ def qc_workload(ctx, L=24, M=512, seed=1002, warm_sweeps=1, dump_bonds=None):
                 sym=abi.MQ_SYM_TWOU1, desc=desc, keep=(d, ham))
 
 
-def scaled_bonds(phys_list, total, M):
-    """bond indices of a random MPS of bond dimension about M: every sector allowed by the total
-    quantum numbers, sizes proportional to the number of states the sector can hold (min of the
-    left and right counts), scaled down to about M states per bond"""
-    from qcmaquis_amd.dmrg import allowed_bonds
+def scale_to(bonds, caps, M):
+    """sector sizes of `bonds` scaled so that every bond holds about M states, each sector at most
+    its number of states (caps: the untruncated bond indices)"""
     from qcmaquis_amd.model import Index
     out = []
-    for b in allowed_bonds(phys_list, total, 1 << 40):
-        sizes = np.array(b.sizes, np.float64)
-        if sizes.sum() > M:
-            sizes = np.maximum(1.0, np.floor(sizes * M / sizes.sum()))
-        out.append(Index.from_dict({tuple(int(x) for x in q): int(m) for q, m in zip(b.charges, sizes)}))
+    for b, f in zip(bonds, caps):
+        cap = {c: s for c, s in zip(f.charges, f.sizes)}
+        sizes = {c: s for c, s in zip(b.charges, b.sizes) if c in cap}
+        for _ in range(8):
+            tot = sum(sizes.values())
+            free = [c for c in sizes if sizes[c] < cap[c]]
+            if tot >= M or not free:
+                break
+            g = 1.0 + (M - tot) / sum(sizes[c] for c in free)
+            for c in free:
+                sizes[c] = min(cap[c], max(1, int(round(sizes[c] * g))))
+        out.append(Index.from_dict(sizes))
     return out
 
 
-def batch_instance(ctx, L, M, seed):
-    """one instance of configuration 4: seeded integrals (PCG64(seed), L-rank 16), half filling,
-    a right-normalised random MPS of bond dimension about M, boundaries grown on the device to the
-    middle two-site problem (sites L/2-1, L/2); returns its (mpo, left, right, layout) and psi"""
-    from qcmaquis_amd.dmrg import merge_two, right_normalize, twosite_offsets
+def su2_workload(ctx, L=40, M=2048, seed=1003, rank=80, warm_M=128, warm_iter=2):
+    """Configuration 3: 40 orbitals, 40 electrons, SU2U1, S = 0, seeded integrals (PCG64(1003),
+    L-rank 80), bond dimension M = 2048 at the middle two-site problem (sites L/2-1, L/2).
+    Sector structure: a left-to-right half sweep at M = warm_M (FP32 Davidson, warm_iter iterations
+    per site) from a random MPS gives the sectors a DMRG state of this Hamiltonian occupies at
+    every bond; their sizes are scaled to about M states per bond (each at most its number of
+    spin-coupled states), filled with random numbers, left-normalised up to the middle problem and
+    right-normalised after it; its boundaries are grown on the device."""
+    from qcmaquis_amd import abi
+    from qcmaquis_amd.dmrg import TwoSiteDMRG, allowed_bonds_su2, left_normalize, right_normalize
     from qcmaquis_amd.engine import DeviceBoundary, DeviceMPO, DeviceMPS, lstep, rstep
     from qcmaquis_amd.model import MPSLayout
     from qcmaquis_amd.qc import QCHamiltonian, make_integrals
-    h, eri, ec = make_integrals(L, seed, 16)
-    ham = QCHamiltonian(h, eri, ec)
+    SU2 = abi.MQ_SYM_SU2U1
+    h, eri, ec = make_integrals(L, seed, rank)
+    t0 = time.perf_counter()
+    ham = QCHamiltonian(h, eri, ec, symmetry=SU2)
+    t_mpo = time.perf_counter() - t0
+    log("SU2U1 MPO built in %.1f s, bond dims max %d" % (t_mpo, max(ham.bond_dims)))
+    t0 = time.perf_counter()
+    d = TwoSiteDMRG(ham, L // 2, L // 2, M=warm_M, ctx=ctx, init_m=8, tol=1e-6, max_iter=warm_iter,
+                    precision=abi.MQ_PREC_FP32)
+    for i in range(L - 1):
+        d.step(i, True)
+    t_warm = time.perf_counter() - t0
+    log("warm-up half sweep at M=%d: %.1f s, E = %.8f" % (warm_M, t_warm, d.energies[-1]))
     phys = ham.phys()
-    bonds = scaled_bonds([phys] * L, (L // 2, L // 2), M)
+    warm = [d.lay[k].left for k in range(L)] + [d.lay[-1].right]
+    del d
+    bonds = scale_to(warm, allowed_bonds_su2([phys] * L, (L, 0), 1 << 60), M)
+    mid = L // 2 - 1
     lay = [MPSLayout(bonds[k], phys, bonds[k + 1]) for k in range(L)]
     rng = np.random.default_rng(seed)
     data = [rng.normal(size=x.n) for x in lay]
-    right_normalize(lay, data)
-    mid = L // 2 - 1
-    R = DeviceBoundary.initial(ctx, lay[-1].right, ham.mpo(L - 1).col_dim)
-    for k in range(L - 1, mid + 1, -1):
-        m = DeviceMPS(ctx, lay[k], data[k])
-        R = rstep(ctx, DeviceMPO(ctx, ham.mpo(k)), R, m, m)
-    Lb = DeviceBoundary.initial(ctx, lay[0].left, ham.mpo(0).row_dim)
+    left_normalize(lay, data, mid)
+    right_normalize(lay, data, stop=mid + 2)
+    t0 = time.perf_counter()
+    Lb = DeviceBoundary.initial(ctx, lay[0].left, 1)
     for k in range(mid):
         m = DeviceMPS(ctx, lay[k], data[k])
         Lb = lstep(ctx, DeviceMPO(ctx, ham.mpo(k)), Lb, m, m)
-    lay2, psi2 = merge_two(lay[mid], data[mid], lay[mid + 1], data[mid + 1], ham.phys(True), twosite_offsets(phys))
-    return (DeviceMPO(ctx, ham.mpo(mid, True)), Lb, R, lay2), psi2 / np.linalg.norm(psi2)
+    R = DeviceBoundary.initial(ctx, lay[-1].right, 1)
+    for k in range(L - 1, mid + 1, -1):
+        m = DeviceMPS(ctx, lay[k], data[k])
+        R = rstep(ctx, DeviceMPO(ctx, ham.mpo(k)), R, m, m)
+    ctx.sync()
+    t_grow = time.perf_counter() - t0
+    from qcmaquis_amd.model import Index  # noqa: F401
+    lay2 = MPSLayout(lay[mid].left, ham.phys(True), lay[mid + 1].right)
+    mpo = DeviceMPO(ctx, ham.mpo(mid, True))
+    li, ri = Lb.info(), R.info()
+    log("boundaries grown in %.1f s: left %d elements, right %d elements" % (
+        t_grow, li.total_elements, ri.total_elements))
+    desc = {"workload": f"configuration 3: {L} orbitals, {L} electrons, SU2U1, S=0, M={M}, seeded FCIDUMP "
+                        f"(PCG64({seed}), rank {rank}); two-site H*psi at sites ({mid},{mid + 1}) of an MPS with the "
+                        f"sector structure of a DMRG state (M={warm_M} half sweep) scaled to M={M}",
+            "symmetry": "SU2U1", "L": L, "M": M, "M_left": lay2.left.total(), "M_right": lay2.right.total(),
+            "bond_dims": [b.total() for b in bonds], "sectors_left": len(lay2.left),
+            "sectors_right": len(lay2.right), "D_left": mpo.spec.row_dim, "D_right": mpo.spec.col_dim,
+            "mpo_bond_dims_max": int(max(ham.bond_dims)), "psi_elements": lay2.n,
+            "boundary_elements": [int(li.total_elements), int(ri.total_elements)],
+            "setup_seconds": {"mpo": round(t_mpo, 2), "warm_half_sweep": round(t_warm, 2),
+                              "boundaries": round(t_grow, 2)}}
+    return dict(mpo=mpo, mpo_spec=ham.mpo(mid, True), L=Lb, R=R, layout=lay2, sym=SU2, desc=desc, keep=(ham,))
 
 
 def batch_sweep_bench(args, ctx):
@@ -151,7 +193,7 @@ def batch_sweep_bench(args, ctx):
     concurrently (BatchTwoSiteDMRG: one batch plan and one device lock-step Davidson per step).
     FP64 Davidson, 8 iterations, tol 1e-8."""
     from qcmaquis_amd import abi
-    from qcmaquis_amd.batch import BatchTwoSiteDMRG
+    from qcmaquis_amd.batch import BatchTwoSiteDMRG, scaled_bonds
     from qcmaquis_amd.dmrg import TwoSiteDMRG, right_normalize
     from qcmaquis_amd.model import MPSLayout
     from qcmaquis_amd.qc import QCHamiltonian, make_integrals
@@ -217,7 +259,7 @@ def batch_bench(args, ctx):
 
     import torch
     from qcmaquis_amd import abi
-    from qcmaquis_amd.batch import lockstep_davidson
+    from qcmaquis_amd.batch import config4_instance, lockstep_davidson
     from qcmaquis_amd.engine import HpsiPlan, HpsiPlanBatch
     rng = np.random.default_rng(1004)
     shapes = [(int(rng.integers(10, 21)), int(rng.choice([128, 256, 512]))) for _ in range(args.instances)]
@@ -228,7 +270,7 @@ def batch_bench(args, ctx):
     t_all = time.perf_counter()
     for i, (L, M) in enumerate(shapes):
         t0 = time.perf_counter()
-        item, psi0 = batch_instance(ctx, L, M, 1004 + 1 + i)
+        item, psi0 = config4_instance(ctx, L, M, 1004 + 1 + i)
         tb = time.perf_counter() - t0
         plan = HpsiPlan(ctx, *item)
         st = plan.stats()
@@ -376,55 +418,57 @@ class ProfilerRegion:
 
 
 def sweep_compare(ctx, L=24, M=512, seed=1002, n_sweeps=4):
-    """FP64 sweeps from a seeded random MPS; during the last one every two-site problem is also
-    solved with the mixed-precision Davidson (FP32 contraction, FP64 refinement, same start vector
-    and 8-iteration budget).  Reported: the FP64 energies per sweep, the largest |E_mixed - E_fp64|
-    over the site problems of the last sweep, and the last sweep's time with each Davidson (host
-    schedule, SVD and growth are the same for both)."""
+    """Three two-site DMRG runs of n_sweeps sweeps each from the same seeded random MPS, Davidson
+    8 iterations and tol 1e-8 per step: FP64 with the grouped launches, mixed precision (FP32
+    contraction, FP64 refinement of every eigenvector) and FP64 with the reference's launch
+    structure (one library dgemm per T task and per cohort, one gather per term).  Reported per
+    run: the wall time and final energy of every sweep, and for the last sweep of the mixed and
+    the reference-style runs against the FP64 run: |dE| of the final energy, the largest
+    difference in truncated weight over the sweep's steps, and the final Davidson residual."""
     from qcmaquis_amd import abi
     from qcmaquis_amd.dmrg import TwoSiteDMRG
     from qcmaquis_amd.qc import QCHamiltonian, make_integrals
     h, eri, ec = make_integrals(L, seed, 16)
     ham = QCHamiltonian(h, eri, ec)
-    d = TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8)
-    energies = []
-    for s in range(n_sweeps):
-        if s == n_sweeps - 1:
-            d.compare_precision = []
-            d.compare_plain = True
-            ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 1)
-        t0 = dict(d.timing)
-        e, dt = d.sweep()
-        energies.append(e)
+    runs = {}
+    for name, prec in (("fp64", abi.MQ_PREC_FP64), ("mixed", abi.MQ_PREC_FP32), ("fp64_plain", abi.MQ_PREC_FP64_PLAIN)):
+        ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, int(prec == abi.MQ_PREC_FP64_PLAIN))
+        d = TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8, precision=prec)
+        rec = {"sweep_seconds": [], "sweep_energies": []}
+        for s in range(n_sweeps):
+            n0 = len(d.trunc)
+            e, dt = d.sweep()
+            rec["sweep_seconds"].append(round(dt, 3))
+            rec["sweep_energies"].append(e)
+        rec["last_sweep_truncated_weights"] = d.trunc[n0:]
+        rec["final_residual_norm"] = d.last_residual
+        rec["timing_s"] = {k: round(v, 3) for k, v in d.timing.items()}
+        runs[name] = rec
+        log("sweeps %s: %s s, E %s" % (name, rec["sweep_seconds"], ["%.10f" % x for x in rec["sweep_energies"]]))
+        del d
     ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
-    cmp = d.compare_precision
-    dav64 = sum(c["seconds"] for c in cmp)
-    dav32 = sum(c["seconds_other"] for c in cmp)
-    davplain = sum(c["seconds_plain"] for c in cmp)
-    # the plain tables the comparison needs are not part of the sweep either
-    dt -= d.timing.get("plan_plain", 0.0) - t0.get("plan_plain", 0.0)
-    de = max(abs(c["e_other"] - c["e"]) for c in cmp)
-    de_tight = max(abs(c["e_tight_mixed"] - c["e_tight_fp64"]) for c in cmp)
-    worst = max(cmp, key=lambda c: abs(c["e_tight_mixed"] - c["e_tight_fp64"]))
-    dt -= sum(c["seconds_extra"] for c in cmp)  # the comparison solves are not part of the sweep
-    out = {"fp64_sweep_energies": energies, "sweep_seconds_fp64": round(dt, 3),
-           "sweep_seconds_mixed": round(dt - dav64 + dav32, 3),
-           "sweep_seconds_fp64_plain": round(dt - dav64 + davplain, 3),
-           "davidson_seconds_fp64": round(dav64, 3), "davidson_seconds_mixed": round(dav32, 3),
-           "davidson_seconds_fp64_plain": round(davplain, 3),
-           "max_abs_delta_e_plain_vs_fp64": max(abs(c["e_plain"] - c["e"]) for c in cmp),
-           "max_abs_delta_e_converged": de_tight, "max_abs_delta_e_8_iterations": de, "worst_site": worst,
-           "timing_s": {k: round(v - t0.get(k, 0.0), 3) for k, v in d.timing.items()},
-           "note": (f"{L} orbitals, M={M}: sweep {n_sweeps - 1} of {n_sweeps} FP64 sweeps from a seeded random MPS; "
-                    "each of its 46 two-site problems also solved with the mixed Davidson in the same "
-                    "8-iteration budget (sweep time) and, for the energy difference, with both Davidsons "
-                    "converged to a residual of 1e-10 |E| (at most 40 iterations); fp64_plain = the same FP64 "
-                    "Davidson with the reference's launch structure (one library dgemm per T task and per "
-                    "cohort, one gather per term)")}
-    log("sweep comparison: |dE| converged %.3e (8 iterations: %.3e), sweep %.2f s (fp64) / %.2f s (mixed) / "
-        "%.2f s (fp64 plain), davidson %.2f / %.2f / %.2f s" % (
-            de_tight, de, out["sweep_seconds_fp64"], out["sweep_seconds_mixed"], out["sweep_seconds_fp64_plain"],
-            dav64, dav32, davplain))
+    ref = runs["fp64"]
+    cmp = {}
+    for name in ("mixed", "fp64_plain"):
+        r = runs[name]
+        cmp[name] = {
+            "abs_delta_e_final": abs(r["sweep_energies"][-1] - ref["sweep_energies"][-1]),
+            "max_abs_delta_truncated_weight_last_sweep": float(np.max(np.abs(
+                np.array(r["last_sweep_truncated_weights"]) - np.array(ref["last_sweep_truncated_weights"])))),
+            "final_residual_norm": r["final_residual_norm"],
+            "last_sweep_seconds": r["sweep_seconds"][-1],
+            "last_sweep_speedup_vs_fp64_plain": round(runs["fp64_plain"]["sweep_seconds"][-1] / r["sweep_seconds"][-1], 3)}
+    out = {"runs": {k: {kk: vv for kk, vv in v.items() if kk != "last_sweep_truncated_weights"} for k, v in runs.items()},
+           "vs_fp64": cmp, "fp64_final_residual_norm": ref["final_residual_norm"],
+           "fp64_last_sweep_speedup_vs_fp64_plain": round(runs["fp64_plain"]["sweep_seconds"][-1] /
+                                                          ref["sweep_seconds"][-1], 3),
+           "note": f"{L} orbitals, M={M}, {n_sweeps} sweeps per run from the same seeded random MPS, Davidson 8 "
+                   "iterations / tol 1e-8 per two-site step; every run is swept for real (its own energies, "
+                   "truncations and boundaries)"}
+    log("sweep comparison: last sweep %.2f s (fp64) / %.2f s (mixed) / %.2f s (fp64 plain); mixed |dE| %.2e, "
+        "|d tw| %.2e" % (ref["sweep_seconds"][-1], runs["mixed"]["sweep_seconds"][-1],
+                         runs["fp64_plain"]["sweep_seconds"][-1], cmp["mixed"]["abs_delta_e_final"],
+                         cmp["mixed"]["max_abs_delta_truncated_weight_last_sweep"]))
     return out
 
 
@@ -457,7 +501,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="qc", choices=["qc", "synthetic", "batch", "batch-sweep"])
+    ap.add_argument("--workload", default="qc", choices=["qc", "qc3", "synthetic", "batch", "batch-sweep"])
+    ap.add_argument("--M", type=int, default=None, help="bond dimension of the qc / qc3 workloads")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batch workload")
     ap.add_argument("--warm-sweeps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -487,13 +532,18 @@ def main():
             torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}), flush=True)
         return
-    ctx = Context(0, abi.MQ_SYM_TWOU1)
+    ctx = Context(0, abi.MQ_SYM_SU2U1 if args.workload == "qc3" else abi.MQ_SYM_TWOU1)
     if args.workload in ("batch", "batch-sweep"):
         (batch_bench if args.workload == "batch" else batch_sweep_bench)(args, ctx)
         ctx.close()
         return
     t0 = time.perf_counter()
-    w = qc_workload(ctx, warm_sweeps=args.warm_sweeps, dump_bonds=args.dump_bonds) if args.workload == "qc" else synthetic_workload(ctx)
+    if args.workload == "qc":
+        w = qc_workload(ctx, M=args.M or 512, warm_sweeps=args.warm_sweeps, dump_bonds=args.dump_bonds)
+    elif args.workload == "qc3":
+        w = su2_workload(ctx, M=args.M or 2048)
+    else:
+        w = synthetic_workload(ctx)
     log("workload built in %.1f s: %s" % (time.perf_counter() - t0, w["desc"]))
     ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, int(not args.no_plain))  # also the reference-style launches
     plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
@@ -603,10 +653,12 @@ def main():
     value = st["flops_total"] / (ms * 1e-3) / 1e12
     gemm_ms = stage[1] + stage[3]
     gemm_tf = st["flops_gemm_exec"] / (gemm_ms * 1e-3) / 1e12
-    # S-assembly as executed: every 16-line block reads the union of its T tiles once and writes
-    # its S tiles (FP64)
-    s_bytes = 8 * (st["union_elements"] + st["s_line_elements"])
+    # the term sum's algorithmic bytes: every T element read once and every S element written once
+    # (FP64); as executed, every 16-line block reads the union of its T tiles (T elements read
+    # union_elements / t_elements times)
+    s_bytes = 8 * (st["t_elements"] + st["s_line_elements"])
     s_gbs = s_bytes / (stage[2] * 1e-3) / 1e9
+    s_exec_gbs = 8 * (st["union_elements"] + st["s_line_elements"]) / (stage[2] * 1e-3) / 1e9
     rl_gemm = {"bound": "mfma", "kernel": GEMM_FAMILY,
                "achieved": round(gemm_tf, 4), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                "frac": round(gemm_tf / PEAK_FP64_TFLOPS, 5), "traffic": None,
@@ -614,10 +666,11 @@ def main():
                    st["flops_gemm_exec"], gemm_ms)}
     rl_sasm = {"bound": "hbm", "kernel": "sblock_kernel<double>", "achieved": round(s_gbs, 2), "peak": PEAK_HBM_GBS,
                "unit": "GB/s", "frac": round(s_gbs / PEAK_HBM_GBS, 5), "traffic": None,
-               "work_per_launch": "%d bytes (%d blocks of 16 lines: union T tile reads %d el, S writes %d el; "
-                                  "%d terms, %d el in the per-term formulation)" % (
-                   s_bytes, st["n_sblocks"], st["union_elements"], st["s_line_elements"], st["n_terms"],
-                   st["gathered_elements"])}
+               "achieved_as_executed": round(s_exec_gbs, 2),
+               "work_per_launch": "%d bytes: T read once (%d el) and S written once (%d el); as executed %d blocks "
+                                  "of 16 lines read %d union T tile elements; %d terms, %d el in the per-term "
+                                  "formulation" % (s_bytes, st["t_elements"], st["s_line_elements"], st["n_sblocks"],
+                                                   st["union_elements"], st["n_terms"], st["gathered_elements"])}
     # the dominant kernel: the term sum is one launch per H*psi and the largest single share of its
     # time; the GEMM stages are spread over three tile-shape instances per stage and a reduce
     dominant, other = rl_sasm, rl_gemm

@@ -152,6 +152,8 @@ typedef struct {
     uint64_t small_lines;             /* S tiles summed term by term (MQ_OPT_SMALL_TILE_MAX) */
     uint64_t small_terms;             /* their terms */
     uint64_t small_gathered;          /* their T elements read: sum over terms of the tile elements */
+    uint64_t max_union_tiles;         /* largest union of T tiles of one 16-line block */
+    uint64_t split_reduces;           /* split-K reduce jobs of the T-GEMM and L*S GEMM stages */
 } mq_hpsi_stats_t;
 
 typedef struct {
@@ -225,6 +227,9 @@ int mq_rstep(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t right, mq_mps_t bra, mq_m
              int symmetric, int precision, mq_boundary_t* out);
 
 /* ---- H*psi (reference: create_contraction_schedule + super_hamil_mv) ----------------------- */
+/* The plan copies what it needs of the left boundary and reads the right boundary's storage in
+ * place: it keeps that storage alive (shared with the boundary), so the right boundary may be
+ * destroyed before the plan. */
 int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_boundary_t right,
                         const mq_index_desc* left_i, const mq_index_desc* phys_i,
                         const mq_index_desc* right_i, mq_hpsi_plan_t* out);
@@ -262,9 +267,10 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t plan);
 
 /* ---- Jacobi-Davidson (reference: solve_ietl_jcd, ietl_jacobi_davidson.h:35-76,
  *      jacobi_davidson::calculate_eigenvalue, ietl/jacobi.h:386-474) ------------------------ */
-/* precision MQ_PREC_FP32: the first max_iter - R iterations use FP32 H*psi (stopping at a relative
- * residual of max(tol, 1e-6)), then at most R = max(1, max_iter / 4) iterations refine the Ritz
- * vector with FP64 H*psi; orthogonalisation, Rayleigh matrix and residuals are FP64 throughout */
+/* precision MQ_PREC_FP32: at most max_iter - R iterations, R = max(1, max_iter / 4), use FP32
+ * H*psi (stopping at a relative residual of max(tol, 1e-6)), then the iterations left of max_iter
+ * (at least R) refine the Ritz vector with FP64 H*psi; orthogonalisation, Rayleigh matrix and
+ * residuals are FP64 throughout */
 int mq_davidson(mq_hpsi_plan_t plan, const double* psi0, int n_ortho, const double* const* ortho,
                 double tol, int max_iter, int precision, double* eval, double* psi_out,
                 mq_davidson_stats_t* stats);

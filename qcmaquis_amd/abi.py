@@ -104,6 +104,8 @@ class HpsiStats(C.Structure):
         ("small_lines", C.c_uint64),
         ("small_terms", C.c_uint64),
         ("small_gathered", C.c_uint64),
+        ("max_union_tiles", C.c_uint64),
+        ("split_reduces", C.c_uint64),
     ]
 
 

@@ -107,3 +107,54 @@ class BatchTwoSiteDMRG:
         for r in self.runs:
             r.sweep_number += 1
         return [r.energies[-1] for r in self.runs]
+
+
+def scaled_bonds(phys_list, total, M):
+    """bond indices of a random MPS of bond dimension about M: every sector allowed by the total
+    quantum numbers, sizes proportional to the number of states the sector can hold (min of the
+    left and right counts), scaled down to about M states per bond"""
+    from .dmrg import allowed_bonds
+    from .model import Index
+    out = []
+    for b in allowed_bonds(phys_list, total, 1 << 40):
+        sizes = np.array(b.sizes, np.float64)
+        if sizes.sum() > M:
+            sizes = np.maximum(1.0, np.floor(sizes * M / sizes.sum()))
+        out.append(Index.from_dict({tuple(int(x) for x in q): int(m) for q, m in zip(b.charges, sizes)}))
+    return out
+
+
+def config4_instance(ctx, L, M, seed, with_host=False):
+    """one instance of configuration 4 (many independent site problems): seeded integrals
+    (PCG64(seed), L-rank 16), half filling, TwoU1, a right-normalised random MPS of bond dimension
+    about M, boundaries grown on the device to the middle two-site problem (sites L/2-1, L/2);
+    returns its (mpo, left, right, layout) and a normalised start vector, and with with_host also
+    (two-site MPO spec, left boundary, right boundary) on the host"""
+    from .dmrg import merge_two, right_normalize, twosite_offsets
+    from .engine import DeviceBoundary, DeviceMPO, DeviceMPS, lstep, rstep
+    from .model import MPSLayout
+    from .qc import QCHamiltonian, make_integrals
+    h, eri, ec = make_integrals(L, seed, 16)
+    ham = QCHamiltonian(h, eri, ec)
+    phys = ham.phys()
+    bonds = scaled_bonds([phys] * L, (L // 2, L // 2), M)
+    lay = [MPSLayout(bonds[k], phys, bonds[k + 1]) for k in range(L)]
+    rng = np.random.default_rng(seed)
+    data = [rng.normal(size=x.n) for x in lay]
+    right_normalize(lay, data)
+    mid = L // 2 - 1
+    R = DeviceBoundary.initial(ctx, lay[-1].right, ham.mpo(L - 1).col_dim)
+    for k in range(L - 1, mid + 1, -1):
+        m = DeviceMPS(ctx, lay[k], data[k])
+        R = rstep(ctx, DeviceMPO(ctx, ham.mpo(k)), R, m, m)
+    Lb = DeviceBoundary.initial(ctx, lay[0].left, ham.mpo(0).row_dim)
+    for k in range(mid):
+        m = DeviceMPS(ctx, lay[k], data[k])
+        Lb = lstep(ctx, DeviceMPO(ctx, ham.mpo(k)), Lb, m, m)
+    lay2, psi2 = merge_two(lay[mid], data[mid], lay[mid + 1], data[mid + 1], ham.phys(True), twosite_offsets(phys))
+    item = (DeviceMPO(ctx, ham.mpo(mid, True)), Lb, R, lay2)
+    psi0 = psi2 / np.linalg.norm(psi2)
+    if with_host:
+        return item, psi0, (ham.mpo(mid, True).to_spec(), Lb.download(), R.download())
+    return item, psi0
+

@@ -49,6 +49,9 @@ struct mq_boundary_s {
     double* data = nullptr;
     uint64_t n = 0;
     mq_growth_stats_t grow{};  // how mq_lstep / mq_rstep built it
+    // owner of the device block `data`: H*psi plans that read the boundary in place share it, so
+    // the block outlives mq_boundary_destroy while a plan still reads it
+    std::shared_ptr<void> hold;
 };
 
 struct mq_mps_s {
@@ -226,6 +229,11 @@ void dev_free(void* p) {
     }
 }
 
+// shared owner of a device block (freed with dev_free when the last owner lets go)
+std::shared_ptr<void> own_device(void* p) {
+    return std::shared_ptr<void>(p, [](void* q) { dev_free(q); });
+}
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int d) {
@@ -376,10 +384,11 @@ struct mq_hpsi_plan_s {
     mq_ctx_t ctx;
     HpsiPlanHost host;
     const double* rdata = nullptr;
+    std::shared_ptr<void> rhold;  // the right boundary's storage, read in place by every H*psi
     GemmStage tg, cg;
     SAsmStage sa;
     int64_t n_lines = 0, n_sblocks = 0;
-    uint64_t s_line_elements = 0, gathered_elements = 0, union_elements = 0;
+    uint64_t s_line_elements = 0, gathered_elements = 0, union_elements = 0, max_union_tiles = 0;
     double *T = nullptr, *S = nullptr, *Lpanel = nullptr;
     // FP32 operands of the mixed-precision H*psi, made on first use
     float *T32 = nullptr, *S32 = nullptr, *Lpanel32 = nullptr, *R32 = nullptr, *psi32 = nullptr;
@@ -521,6 +530,7 @@ int mq_boundary_create_initial(mq_ctx_t ctx, const mq_index_desc* idx, uint32_t 
         std::unique_ptr<mq_boundary_s> guard(b);
         b->n = b->idx.layout();
         b->data = dev_alloc<double>(b->n);
+        b->hold = own_device(b->data);
         hipcheck(launch_fill(b->data, b->n, 1.0, ctx->stream), "fill");
         hipcheck(hipStreamSynchronize(ctx->stream), "sync");
         *out = guard.release();
@@ -534,6 +544,7 @@ int mq_boundary_upload(mq_ctx_t ctx, const mq_bindex_desc* d, const double* data
         std::unique_ptr<mq_boundary_s> guard(b);
         b->n = b->idx.layout();
         b->data = dev_alloc<double>(b->n);
+        b->hold = own_device(b->data);
         if (b->n) {
             require(data != nullptr, "mq_boundary_upload: null data");
             hipcheck(hipMemcpyAsync(b->data, data, b->n * sizeof(double), hipMemcpyHostToDevice, ctx->stream), "upload");
@@ -591,7 +602,7 @@ int mq_boundary_destroy(mq_boundary_t b) {
     if (!b) return MQ_OK;
     int rc = guarded(b->ctx, [&] {
         (void)hipStreamSynchronize(b->ctx->stream);
-        dev_free(b->data);
+        b->hold.reset();
     });
     delete b;
     return rc;
@@ -694,6 +705,7 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
     std::unique_ptr<mq_boundary_s> guard(nb);
     nb->n = P.out_elems;
     nb->data = dev_alloc<double>(nb->n);
+    nb->hold = own_device(nb->data);
     double* T = dev_alloc<double>(P.t_elems);
     double* S = dev_alloc<double>(P.s_elems);
     GemmStage tg, cg;
@@ -814,6 +826,7 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
     p->n_lines = int64_t(H.lines.size());
     p->n_sblocks = int64_t(SB.blocks.size());
     p->union_elements = SB.gathered;
+    for (auto const& b : SB.blocks) p->max_union_tiles = std::max<uint64_t>(p->max_union_tiles, uint64_t(b.nu));
     if (ctx->plain_tables && p->n_instances == 1) {
         auto tp = std::chrono::steady_clock::now();
         check_gemms(H.t_gemms_task, sizes);
@@ -859,6 +872,7 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
         std::unique_ptr<mq_hpsi_plan_s> guard(p);
         p->psi_n = p->host.mps.elements();
         p->rdata = right->data;
+        p->rhold = right->hold;
         finish_plan(ctx, p, {LpanelPart{left->data, left->n, p->host.lpanel_jobs}}, right->n,
                     {0, p->host.lines.size()}, t0);
         *out = guard.release();
@@ -1022,6 +1036,8 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
     s->small_lines = p->small_lines;
     s->small_terms = p->small_terms;
     s->small_gathered = p->small_gathered;
+    s->max_union_tiles = p->max_union_tiles;
+    s->split_reduces = uint64_t(p->tg.n_reduces + p->cg.n_reduces);
     return MQ_OK;
 }
 
@@ -1411,14 +1427,14 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
             } else if (precision == MQ_PREC_FP64 || max_iter < 2) {
                 jd(MQ_PREC_FP64, tol, max_iter);
             } else {
-                // mixed precision within the same iteration budget: FP32 H*psi for the first
-                // max_iter - R iterations (stopping at the FP32 floor), then the Ritz vector is
-                // refined with FP64 H*psi for at most R = max(1, max_iter / 4) iterations
+                // mixed precision within the same iteration budget: FP32 H*psi for at most
+                // max_iter - R iterations, R = max(1, max_iter / 4) (stopping at the FP32 floor),
+                // then the Ritz vector is refined with FP64 H*psi for the iterations that are left
                 const int R = std::max(1, max_iter / 4);
                 jd(MQ_PREC_FP32, std::max(tol, 1e-6), max_iter - R);
                 int it32 = it;
                 hipcheck(hipMemcpyAsync(V[0], u, n * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
-                jd(MQ_PREC_FP64, tol, R);
+                jd(MQ_PREC_FP64, tol, max_iter - it32);
                 refine = it;
                 it += it32;
             }
@@ -1598,7 +1614,7 @@ extern "C" int mq_davidson_batch(mq_hpsi_plan_t p, const double* psi0, double to
                 total = its;
                 lockstep = it_run;
                 hipcheck(hipMemcpyAsync(V[0], u, n * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
-                jd(MQ_PREC_FP64, tol, R);
+                jd(MQ_PREC_FP64, tol, max_iter - lockstep);
                 for (int i = 0; i < ns; ++i) total[i] += its[i];
                 refine = it_run;
                 lockstep += it_run;

@@ -251,10 +251,12 @@ def allowed_bonds_su2(phys_list, total, m_cap, full_right=False):
     return bonds
 
 
-def right_normalize(layouts, data):
-    """right-canonical form by QR sweeps from the right (MPS::canonize)"""
+def right_normalize(layouts, data, stop=1):
+    """right-canonical form of sites stop .. n-1 by QR sweeps from the right (MPS::canonize); the
+    remainder of each QR is multiplied into the left neighbour, and with stop == 1 site 0 is
+    normalised"""
     n = len(layouts)
-    for k in range(n - 1, 0, -1):
+    for k in range(n - 1, stop - 1, -1):
         lay, x = layouts[k], data[k]
         new_sizes, Rts = [], []
         blocks = []
@@ -282,7 +284,8 @@ def right_normalize(layouts, data):
             out = blk @ Rts[mb]
             nlp[npl.lp_off[mb]:npl.lp_off[mb + 1]] = out.T.reshape(-1)
         layouts[k - 1], data[k - 1] = npl, npl.to_right(nlp)
-    data[0] = data[0] / np.linalg.norm(data[0])
+    if stop == 1:
+        data[0] = data[0] / np.linalg.norm(data[0])
     return layouts, data
 
 
@@ -292,6 +295,30 @@ def initial_bonds(sym, phys, L, n_up, n_dn, m_cap, full_right=False):
     if sym == abi.MQ_SYM_SU2U1:
         return allowed_bonds_su2([phys] * L, (n_up + n_dn, n_up - n_dn), m_cap, full_right)
     return allowed_bonds([phys] * L, (n_up, n_dn), m_cap, full_right)
+
+
+def left_normalize(layouts, data, upto):
+    """left-canonical form of sites 0 .. upto-1 by QR sweeps from the left (MPS::canonize); the
+    remainder of each QR is multiplied into the right neighbour"""
+    for k in range(upto):
+        lay, x = layouts[k], data[k]
+        lp = lay.to_left(x)
+        new_sizes, Rs, blocks = [], [], []
+        for rb in range(len(lay.right)):
+            rows = lay.lp_rows[rb]
+            X = lp[lay.lp_off[rb]:lay.lp_off[rb + 1]].reshape(lay.right.sizes[rb], rows).T
+            Q, R = np.linalg.qr(X)
+            blocks.append(Q)
+            Rs.append(R)
+            new_sizes.append(Q.shape[1])
+        mid = Index(lay.right.charges, new_sizes)
+        nl = MPSLayout(lay.left, lay.phys, mid)
+        nlp = np.zeros(nl.n)
+        for mb in range(len(mid)):
+            nlp[nl.lp_off[mb]:nl.lp_off[mb + 1]] = blocks[mb].T.reshape(-1)
+        layouts[k], data[k] = nl, nl.to_right(nlp)
+        layouts[k + 1], data[k + 1] = _absorb_into_right_neighbour(layouts[k + 1], data[k + 1], mid, Rs)
+    return layouts, data
 
 
 class TwoSiteDMRG:
@@ -339,6 +366,7 @@ class TwoSiteDMRG:
             m = DeviceMPS(self.ctx, self.lay[k], self.data[k])
             self.right[k] = rstep(self.ctx, self.mpo1[k], self.right[k + 1], m, m)
         self.energies, self.trunc = [], []
+        self.last_residual = None  # Davidson residual norm of the last step
         self.timing = {"davidson": 0.0, "hpsi": 0.0, "plan": 0.0, "svd": 0.0, "grow": 0.0, "merge": 0.0,
                        "mpo": 0.0, "plan_tables": 0.0, "plan_blocks": 0.0, "plan_total": 0.0}
         self.hpsi_flops = 0
@@ -397,6 +425,7 @@ class TwoSiteDMRG:
             self.timing["compare"] = self.timing.get("compare", 0.0) + extra
             t3s = t3 + extra  # the comparison solves are not charged to the split
         pst = plan.stats()
+        self.last_residual = st["residual_norm"]
         self.hpsi_flops += pst["flops_total"] * st["iterations"]
         self.timing["plan_tables"] += pst["seconds_tables"]
         self.timing["plan_blocks"] += pst["seconds_blocks"]

@@ -240,3 +240,22 @@ def fci_spin_spectrum(h, eri, n, two_s, ecore=0.0, tol=1e-8):
             hi.pop(j)
     assert not hi, "unmatched higher-spin eigenvalues"
     return np.array(out)
+
+
+def ladder_hubbard(legs_len, t=1.0, t_rung=1.0, U=8.0):
+    """integrals of the fermion Hubbard model on an open two-leg ladder of legs_len rungs, in the
+    quantum-chemistry form: h_ij = -t on leg bonds, -t_rung on rung bonds, (ii|ii) = U; site of
+    (x, y) = 2 x + y (rung partners adjacent)"""
+    n = 2 * legs_len
+    h = np.zeros((n, n))
+    for x in range(legs_len):
+        a, b = 2 * x, 2 * x + 1
+        h[a, b] = h[b, a] = -t_rung
+        if x + 1 < legs_len:
+            for y in (0, 1):
+                i, j = 2 * x + y, 2 * (x + 1) + y
+                h[i, j] = h[j, i] = -t
+    eri = np.zeros((n, n, n, n))
+    for i in range(n):
+        eri[i, i, i, i] = U
+    return h, eri

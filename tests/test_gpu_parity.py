@@ -226,3 +226,28 @@ def test_errors_are_codes(ctx2u1):
     from qcmaquis_amd.engine import EngineError
     with pytest.raises(EngineError):
         HpsiPlan(ctx2u1, mpo, L, R, ch.layouts[1])
+
+
+def test_plan_outlives_its_right_boundary(ctx2u1, oracle_lib):
+    """the plan reads the right boundary in place and shares its storage: destroying the boundary
+    handle and allocating new boundaries of the same size does not change the plan's H*psi"""
+    ch = make_chain(21, n_sites=4, max_m=6, D=8)
+    mpos, L_dev, L_or, R_dev, R_or = _grow_both(ctx2u1, oracle_lib, ch)
+    k = 1
+    lay = ch.layouts[k]
+    R = rstep(ctx2u1, mpos[k + 1], R_dev[k + 2], DeviceMPS(ctx2u1, ch.layouts[k + 1], ch.data[k + 1]),
+              DeviceMPS(ctx2u1, ch.layouts[k + 1], ch.data[k + 1]))
+    plan = HpsiPlan(ctx2u1, mpos[k], L_dev[k], R, lay)
+    psi = np.random.default_rng(3).normal(size=lay.n)
+    before = plan.apply(psi)
+    R.close()
+    R_dev[k + 1].close()
+    others = [rstep(ctx2u1, mpos[k + 1], R_dev[k + 2], DeviceMPS(ctx2u1, ch.layouts[k + 1], 2.0 * ch.data[k + 1]),
+                    DeviceMPS(ctx2u1, ch.layouts[k + 1], ch.data[k + 1])) for _ in range(3)]
+    after = plan.apply(psi)
+    assert np.array_equal(before, after)
+    ref = oracle_lib.hpsi(ch.sym, ch.mpos[k], L_or[k], R_or[k + 1], lay, psi)
+    assert rel_err(after, ref) < TOL
+    plan.close()
+    for o in others:
+        o.close()
