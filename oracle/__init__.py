@@ -41,6 +41,8 @@ def lib():
         L.oracle_sixj.argtypes = [C.c_int] * 6
         L.oracle_hpsi.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, P(abi.BIndexDesc), dp, idx, idx, idx,
                                   dp, dp, C.c_int, dp]
+        L.oracle_hpsi_sample.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, P(abi.BIndexDesc), dp, idx,
+                                         idx, idx, dp, dp, C.c_int, dp, P(C.c_uint8), dp]
         for f in (L.oracle_lstep, L.oracle_rstep):
             f.restype = C.c_void_p
             f.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, idx, idx, idx, dp, idx, idx, idx, dp, C.c_int,
@@ -91,6 +93,25 @@ def hpsi(sym, mpo, L, R, layout, psi, nthreads=0, return_flops=False):
     if rc != 0:
         raise RuntimeError("oracle_hpsi failed")
     return (out, fl.value) if return_flops else out
+
+
+def hpsi_sample(sym, mpo, L, R, layout, psi, mask, nthreads=0):
+    """the H*psi of the input blocks with mask[lb] set only: returns (partial output, their flops in
+    the reference's model, seconds of the contraction loop, schedule build excluded)"""
+    psi = np.ascontiguousarray(psi, np.float64)
+    out = np.zeros_like(psi)
+    md, Ld, Rd = mpo.desc(), L.desc(), R.desc()
+    li, pi, ri = layout.left.desc(), layout.phys.desc(), layout.right.desc()
+    fl, sec = C.c_double(), C.c_double()
+    m = np.ascontiguousarray(np.asarray(mask, np.uint8))
+    Ldat = np.ascontiguousarray(L.data)
+    Rdat = np.ascontiguousarray(R.data)
+    rc = lib().oracle_hpsi_sample(sym, C.byref(md), C.byref(Ld), _dp(Ldat), C.byref(Rd), _dp(Rdat), C.byref(li),
+                                  C.byref(pi), C.byref(ri), _dp(psi), _dp(out), int(nthreads), C.byref(fl),
+                                  m.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(sec))
+    if rc != 0:
+        raise RuntimeError("oracle_hpsi_sample failed")
+    return out, fl.value, sec.value
 
 
 def _export(h, bra, ket):

@@ -637,9 +637,13 @@ double oracle_sixj(int a, int b, int c, int d, int e, int f) { return sixj(a, b,
 
 // H*psi on the CPU: schedule (create_contraction_schedule) + super_hamil_mv's OpenMP branch.
 // psi / out: right-paired blocks of left_i concatenated without padding.
-int oracle_hpsi(int sym, const mq_mpo_desc* mpod, const mq_bindex_desc* Ld, const double* Ldata, const mq_bindex_desc* Rd,
-                const double* Rdata, const mq_index_desc* li, const mq_index_desc* pi, const mq_index_desc* ri,
-                const double* psi, double* out, int nthreads, double* flops_out) {
+// lb_mask (may be NULL): only the input blocks lb_in with lb_mask[lb_in] != 0 are scheduled and
+// contracted (a bounded sample of the H*psi); flops_out then counts those blocks' flops and
+// seconds_out (may be NULL) receives the time of the contraction loop alone.
+int oracle_hpsi_sample(int sym, const mq_mpo_desc* mpod, const mq_bindex_desc* Ld, const double* Ldata,
+                       const mq_bindex_desc* Rd, const double* Rdata, const mq_index_desc* li, const mq_index_desc* pi,
+                       const mq_index_desc* ri, const double* psi, double* out, int nthreads, double* flops_out,
+                       const unsigned char* lb_mask, double* seconds_out) {
     try {
         g_sym = sym;
         Mpo mpo = make_mpo(*mpod);
@@ -660,6 +664,7 @@ int oracle_hpsi(int sym, const mq_mpo_desc* mpod, const mq_bindex_desc* Ld, cons
         BIdx const& RI = R.idx;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
         for (long lb_in = 0; lb_in < long(nl); ++lb_in) {
+            if (lb_mask && !lb_mask[lb_in]) continue;
             MPSBlock& mb = sched[lb_in];
             rshtm_t(RI, left_i, right_i, phys, pb, unsigned(lb_in), mb, right_i);
             // shtm_tasks (site_hamil_tasks.hpp:41-137)
@@ -725,16 +730,26 @@ int oracle_hpsi(int sym, const mq_mpo_desc* mpod, const mq_bindex_desc* Ld, cons
         for (size_t b = 0; b < nl; ++b) psiv[b] = psi + boff[b];
         std::fill(out, out + boff[nl], 0.0);
         std::vector<std::mutex> mtx(nl);
+        double t0 = omp_get_wtime();
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : omp_get_max_threads())
         for (long lb_in = 0; lb_in < long(nl); ++lb_in) {
+            if (lb_mask && !lb_mask[lb_in]) continue;
             auto T = sched[lb_in].create_T(RI, rview, psiv, lrs);
             for (auto const& c : sched[lb_in].data) c.contract(lview, T, out + boff[c.rb], mtx[c.rb]);
         }
+        if (seconds_out) *seconds_out = omp_get_wtime() - t0;
         return 0;
     } catch (std::exception const& e) {
         fprintf(stderr, "oracle_hpsi: %s\n", e.what());
         return -1;
     }
+}
+
+int oracle_hpsi(int sym, const mq_mpo_desc* mpod, const mq_bindex_desc* Ld, const double* Ldata, const mq_bindex_desc* Rd,
+                const double* Rdata, const mq_index_desc* li, const mq_index_desc* pi, const mq_index_desc* ri,
+                const double* psi, double* out, int nthreads, double* flops_out) {
+    return oracle_hpsi_sample(sym, mpod, Ld, Ldata, Rd, Rdata, li, pi, ri, psi, out, nthreads, flops_out, nullptr,
+                              nullptr);
 }
 
 // overlap_mpo_left_step CPU branch.  bra: left-paired data, ket: right-paired data.

@@ -84,3 +84,21 @@ def test_hermitian_compression_consistent(oracle_lib):
         l, r = c["lr"]
         j = L.cohort_index(L.ket.charges[r], L.bra.charges[l])
         assert j < len(L.cohorts) and not L.cohorts[j]["tr"]
+
+
+def test_sampled_hpsi_partitions_the_full_one(oracle_lib):
+    """oracle.hpsi_sample over complementary sets of input blocks adds up to the full H*psi, and
+    the sampled flops add up to the full count"""
+    from tests.chains import make_chain
+    ch = make_chain(9, n_sites=4, max_m=6, D=8)
+    from tests.chains import oracle_left, oracle_right
+    L = oracle_left(oracle_lib, ch, 1)[1]
+    R = oracle_right(oracle_lib, ch, 2)[2]
+    lay = ch.layouts[1]
+    psi = np.random.default_rng(4).normal(size=lay.n)
+    full, fl = oracle_lib.hpsi(ch.sym, ch.mpos[1], L, R, lay, psi, return_flops=True)
+    mask = np.arange(len(lay.left)) % 2
+    a, fa, _ = oracle_lib.hpsi_sample(ch.sym, ch.mpos[1], L, R, lay, psi, mask)
+    b, fb, _ = oracle_lib.hpsi_sample(ch.sym, ch.mpos[1], L, R, lay, psi, 1 - mask)
+    assert np.max(np.abs(a + b - full)) <= 1e-12 * np.max(np.abs(full))
+    assert abs(fa + fb - fl) <= 1e-9 * fl and fa > 0 and fb > 0
