@@ -274,6 +274,18 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t plan);
 int mq_davidson(mq_hpsi_plan_t plan, const double* psi0, int n_ortho, const double* const* ortho,
                 double tol, int max_iter, int precision, double* eval, double* psi_out,
                 mq_davidson_stats_t* stats);
+/* Jacobi-Davidson that allocates no device memory.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as mq_davidson; its search space (2 max_iter + 2
+ * + n_ortho vectors of the plan's psi length) lives in one fixed device array of
+ * MQ_NOALLOC_SCRATCH_ELEMS doubles per GPU, held until the process ends.  Larger problems, other
+ * precisions than MQ_PREC_FP64 and plans with HIP graphs enabled return MQ_ERR_INVALID.  Every call
+ * on one GPU uses that GPU's array: calls that overlap on one GPU (from several contexts or
+ * threads) race. */
+#define MQ_NOALLOC_SCRATCH_ELEMS (1u << 23)
+int mq_davidson_noalloc(mq_hpsi_plan_t plan, const double* psi0, int n_ortho, const double* const* ortho,
+                        double tol, int max_iter, int precision, double* eval, double* psi_out,
+                        mq_davidson_stats_t* stats);
 /* Lowest eigenpair of every instance of a batch plan (mq_hpsi_plan_create_batch), the iteration of
  * mq_davidson run in lock step: one batched H*psi per iteration, the vector operations as
  * segmented launches over all instances.  No counterpart in the reference (which solves one site

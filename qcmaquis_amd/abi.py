@@ -20,6 +20,7 @@ MQ_PREC_FP64 = 0
 MQ_PREC_FP32 = 1
 MQ_PREC_BF16X3 = 2
 MQ_PREC_FP64_PLAIN = 3
+MQ_NOALLOC_SCRATCH_ELEMS = 1 << 23
 MQ_OPT_PLAIN_TABLES = 1
 MQ_OPT_SMALL_TILE_MAX = 2
 MQ_SMALL_TILE_AUTO = -1
@@ -162,6 +163,7 @@ SIGNATURES = [
     ("mq_hpsi_plan_use_graphs", i32, [vp, i32]),
     ("mq_hpsi_plan_destroy", i32, [vp]),
     ("mq_davidson", i32, [vp, dp, i32, P(dp), C.c_double, i32, i32, dp, dp, P(DavidsonStats)]),
+    ("mq_davidson_noalloc", i32, [vp, dp, i32, P(dp), C.c_double, i32, i32, dp, dp, P(DavidsonStats)]),
     ("mq_davidson_batch", i32, [vp, dp, C.c_double, i32, i32, dp, dp, P(C.c_int32), P(DavidsonStats)]),
     ("mq_qc_build", i32, [i32, i32, dp, dp, C.c_double, C.c_double, P(vp)]),
     ("mq_qc_info", i32, [vp, P(i32), P(C.c_uint32), dp]),

@@ -36,6 +36,7 @@ struct mq_ctx_s {
     bool plain_tables = false;         // MQ_OPT_PLAIN_TABLES
     int32_t small_tile_max = MQ_SMALL_TILE_AUTO;  // MQ_OPT_SMALL_TILE_MAX
     rocblas_handle blas = nullptr;     // library GEMMs of MQ_PREC_FP64_PLAIN, made on first use
+    double* noalloc_scratch = nullptr; // this device's scratch array of mq_davidson_noalloc
 };
 
 struct mq_mpo_s {
@@ -445,6 +446,22 @@ int mq_index_position(const mq_index_desc* idx, int32_t q0, int32_t q1) {
     }
 }
 
+// device address of the scratch array of mq_davidson_noalloc, looked up once per device with that
+// device current (each GPU holds its own copy of the array)
+static double* noalloc_scratch_of(int device) {
+    static std::mutex mtx;
+    static std::vector<double*> addr;
+    std::lock_guard<std::mutex> lock(mtx);
+    if (int(addr.size()) <= device) addr.resize(size_t(device) + 1, nullptr);
+    if (!addr[device]) {
+        void* p = nullptr;
+        hipcheck(noalloc_scratch_address(&p), "scratch symbol address");
+        require(p != nullptr, "scratch symbol address: null");
+        addr[device] = static_cast<double*>(p);
+    }
+    return addr[device];
+}
+
 int mq_ctx_create(int device, int symmetry, mq_ctx_t* out) {
     if (!out) return MQ_ERR_INVALID;
     *out = nullptr;
@@ -462,6 +479,7 @@ int mq_ctx_create(int device, int symmetry, mq_ctx_t* out) {
     int rc = guarded(c, [&] {
         hipcheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream create");
         c->dwork = dev_alloc<double>(DOT_BLOCKS + 64);
+        c->noalloc_scratch = noalloc_scratch_of(device);
     });
     if (rc != MQ_OK) {
         delete c;
@@ -1324,10 +1342,11 @@ void small_sym_lowest(std::vector<double> a, int n, double& theta, std::vector<d
 
 }  // namespace
 
-extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, const double* const* ortho, double tol,
-                           int max_iter, int precision, double* eval, double* psi_out, mq_davidson_stats_t* stats) {
-    if (!p) return MQ_ERR_INVALID;
-    return guarded(p->ctx, [&] {
+// The Jacobi-Davidson of mq_davidson.  Its vectors come from dev_alloc, or, with an arena, are
+// carved from that device array (arena_elems doubles, every vector 256-byte aligned).
+static void davidson_core(mq_hpsi_plan_t p, const double* psi0, int n_ortho, const double* const* ortho, double tol,
+                          int max_iter, int precision, double* eval, double* psi_out, mq_davidson_stats_t* stats,
+                          double* arena, uint64_t arena_elems) {
         require(p->n_instances == 1, "mq_davidson: a batch plan holds several independent problems");
         require(psi0 && eval && psi_out && max_iter >= 1 && n_ortho >= 0, "mq_davidson: bad argument");
         require(n_ortho == 0 || ortho, "mq_davidson: null ortho vectors");
@@ -1338,7 +1357,18 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
         // the reference truncates the ortho list to at most n-1 vectors (ietl_jacobi_davidson.h:41-42)
         int no = std::min<int64_t>(n_ortho, int64_t(n) - 1);
         std::vector<double*> bufs;
-        auto alloc = [&]() { bufs.push_back(dev_alloc<double>(n)); return bufs.back(); };
+        uint64_t carved = 0;
+        const uint64_t stride = (n + 31) / 32 * 32;
+        auto alloc = [&]() {
+            if (arena) {
+                require(carved + stride <= arena_elems, "mq_davidson_noalloc: search space outside the scratch array");
+                double* v = arena + carved;
+                carved += stride;
+                return v;
+            }
+            bufs.push_back(dev_alloc<double>(n));
+            return bufs.back();
+        };
         double* sc = p->ctx->dwork + DOT_BLOCKS;  // device scalars
         double* work = p->ctx->dwork;
         double hpsi_s = 0;
@@ -1454,6 +1484,33 @@ extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, co
             throw;
         }
         for (auto b : bufs) dev_free(b);
+}
+
+extern "C" int mq_davidson(mq_hpsi_plan_t p, const double* psi0, int n_ortho, const double* const* ortho, double tol,
+                           int max_iter, int precision, double* eval, double* psi_out, mq_davidson_stats_t* stats) {
+    if (!p) return MQ_ERR_INVALID;
+    return guarded(p->ctx, [&] {
+        davidson_core(p, psi0, n_ortho, ortho, tol, max_iter, precision, eval, psi_out, stats, nullptr, 0);
+    });
+}
+
+extern "C" int mq_davidson_noalloc(mq_hpsi_plan_t p, const double* psi0, int n_ortho, const double* const* ortho,
+                                   double tol, int max_iter, int precision, double* eval, double* psi_out,
+                                   mq_davidson_stats_t* stats) {
+    if (!p) return MQ_ERR_INVALID;
+    return guarded(p->ctx, [&] {
+        require(p->ctx->noalloc_scratch != nullptr, "mq_davidson_noalloc: no scratch array on this device");
+        require(precision == MQ_PREC_FP64, "mq_davidson_noalloc: precision must be MQ_PREC_FP64");
+        require(!p->use_graphs, "mq_davidson_noalloc: the plan replays HIP graphs (their buffers are allocated)");
+        require(max_iter >= 1 && n_ortho >= 0, "mq_davidson_noalloc: bad argument");
+        const uint64_t stride = (p->psi_n + 31) / 32 * 32;
+        const uint64_t need = stride * uint64_t(2 * max_iter + 2 + std::min<int64_t>(n_ortho, int64_t(p->psi_n) - 1));
+        if (need > MQ_NOALLOC_SCRATCH_ELEMS)
+            throw Error(MQ_ERR_INVALID, "mq_davidson_noalloc: the search space needs " + std::to_string(need) +
+                                            " doubles, the scratch array holds " +
+                                            std::to_string(MQ_NOALLOC_SCRATCH_ELEMS));
+        davidson_core(p, psi0, n_ortho, ortho, tol, max_iter, precision, eval, psi_out, stats, p->ctx->noalloc_scratch,
+                      MQ_NOALLOC_SCRATCH_ELEMS);
     });
 }
 

@@ -620,6 +620,13 @@ hipError_t launch_seg_axpby(double* y, const double* x, const int64_t* off, int 
     return hipGetLastError();
 }
 
+// Search space of mq_davidson_noalloc: one copy per GPU, allocated with the code object and held
+// until the process ends.  Host code passes its device address (noalloc_scratch_address), never
+// the symbol itself.
+static __device__ __align__(256) double noalloc_scratch[MQ_NOALLOC_SCRATCH_ELEMS];
+
+hipError_t noalloc_scratch_address(void** ptr) { return hipGetSymbolAddress(ptr, HIP_SYMBOL(noalloc_scratch)); }
+
 hipError_t launch_axpby(double* y, const double* x, uint64_t n, double a, double b, hipStream_t stream) {
     hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, y, x, n, a, b);
     return hipGetLastError();

@@ -44,6 +44,9 @@ hipError_t launch_slines(const SLine* lines, int64_t n_lines, const STerm* terms
 // dst = float(src)
 hipError_t launch_convert(const double* src, float* dst, uint64_t n, hipStream_t stream);
 hipError_t launch_fill(double* p, uint64_t n, double v, hipStream_t stream);
+// device address (on the current device) of the fixed scratch array of the allocation-free entry
+// points, MQ_NOALLOC_SCRATCH_ELEMS doubles
+hipError_t noalloc_scratch_address(void** ptr);
 hipError_t launch_copy_jobs(const CopyJob* jobs, int32_t n_jobs, const double* src, double* dst, hipStream_t stream);
 
 // right-paired <-> left-paired MPS reshape segments (one per (left sector, sigma) with a right
