@@ -28,6 +28,7 @@
 #include <memory>
 #include <set>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "core.hpp"
@@ -686,6 +687,17 @@ struct SU2QC {
     // SU(2) Kronecker product of two site operators between bond spins j (left), jm (middle),
     // jp (right) at total rank k (site_operator_algorithms.h:214-335); m[in][out] over product
     // states a*4+b
+    // mod_coupling and the 6j factor of kron take spins of at most 4 (as 2S): memoised
+    mutable std::unordered_map<uint64_t, double> cpl_cache;
+    double cached(int kind, int a, int b, int c, int d, int e, int f, int g = 0, int h = 0, int i = 0) const {
+        uint64_t key = uint64_t(kind);
+        for (int v : {a, b, c, d, e, f, g, h, i}) key = key * 8 + uint64_t(v);
+        auto it = cpl_cache.find(key);
+        if (it != cpl_cache.end()) return it->second;
+        double v = kind == 0 ? mod_coupling(a, b, c, d, e, f, g, h, i) : wigner_6j(a, b, c, d, e, f);
+        cpl_cache.emplace(key, v);
+        return v;
+    }
     void kron(int oa, int ob, int j, int jm, int jp, int k, std::array<double, 256>& out) const {
         SOp A = T.ops[oa], B = T.ops[ob];
         // a rank-0 partner of a rank > 0 operator acts as the full identity on the singly
@@ -693,7 +705,7 @@ struct SU2QC {
         if (A.K > 0 && B.K == 0 && B.m[SB][SC] == 0.0) { B.m[SB][SC] = 1.0; B.m[SC][SB] = 1.0; }
         if (A.K == 0 && B.K > 0 && A.m[SB][SC] == 0.0) { A.m[SB][SC] = 1.0; A.m[SC][SB] = 1.0; }
         int k1 = A.K, k2 = B.K;
-        double f = std::sqrt((jm + 1.0) * (k + 1.0)) * wigner_6j(j, jp, k, k2, k1, jm);
+        double f = std::sqrt((jm + 1.0) * (k + 1.0)) * cached(1, j, jp, k, k2, k1, jm);
         if (((j + jp + k1 + k2) / 2) % 2) f = -f;
         out.fill(0.0);
         if (f == 0.0) return;
@@ -705,8 +717,8 @@ struct SU2QC {
                     for (int ob2 = 0; ob2 < 4; ++ob2) {
                         double vb = B.m[ib][ob2];
                         if (vb == 0.0) continue;
-                        double c = mod_coupling(abs_spin(ia), abs_spin(ib), product_spin(ia, ib), k1, k2, k,
-                                                abs_spin(oa2), abs_spin(ob2), product_spin(oa2, ob2));
+                        double c = cached(0, abs_spin(ia), abs_spin(ib), product_spin(ia, ib), k1, k2, k,
+                                          abs_spin(oa2), abs_spin(ob2), product_spin(oa2, ob2));
                         out[(ia * 4 + ib) * 16 + (oa2 * 4 + ob2)] += va * vb * c * f;
                     }
             }
