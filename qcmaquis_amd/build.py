@@ -73,6 +73,23 @@ def build_h5(verbose=False):
     return H5_LIB
 
 
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "engine_test.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "engine_test")
+
+
+def build_cpp_test(verbose=False):
+    """the C++ program that exercises include/qcmaquis_amd.hpp against the library (run by
+    tests/test_cpp_engine.py on the GPU)"""
+    tmp = CPP_TEST_BIN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", tmp, CPP_TEST_SRC,
+          "-L" + PKG, "-lqcmaquis_amd", "-Wl,-rpath,$ORIGIN/../../qcmaquis_amd"])
+    os.replace(tmp, CPP_TEST_BIN)
+    if verbose:
+        print("built", CPP_TEST_BIN)
+    return CPP_TEST_BIN
+
+
 if __name__ == "__main__":
     build_library(verbose=True)
     build_h5(verbose=True)
+    build_cpp_test(verbose=True)

@@ -334,8 +334,6 @@ class TwoSiteDMRG:
         sweep appends to (results.ResultArchive); first_sweep: number of the first sweep"""
         self.ham, self.M, self.tol, self.max_iter, self.cutoff = ham, M, tol, max_iter, cutoff
         self.precision = precision
-        self.compare_precision = None  # a list: every step also runs Davidson in the other precision
-        self.compare_plain = False  # with compare_precision: also the MQ_PREC_FP64_PLAIN Davidson
         self.verbose = verbose
         self.sym = getattr(ham, "sym", abi.MQ_SYM_TWOU1)
         self.ctx = ctx or Context(0, self.sym)
@@ -398,32 +396,6 @@ class TwoSiteDMRG:
         t2 = time.perf_counter()
         e, psi, st = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter, precision=self.precision)
         t3 = time.perf_counter()
-        t3s = t3  # start of the split (after any comparison solves)
-        if self.compare_precision is not None:
-            # the same site problem, start vector and iteration budget in the other precision
-            other = abi.MQ_PREC_FP32 if self.precision == abi.MQ_PREC_FP64 else abi.MQ_PREC_FP64
-            c0 = time.perf_counter()
-            e2, _, st2 = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter, precision=other)
-            c1 = time.perf_counter()
-            # both precisions converged tightly from the same start: the difference is the
-            # precision effect alone (the budgeted runs above also differ in convergence)
-            et64, _, s64 = plan.davidson(psi2, tol=1e-10, max_iter=40, precision=abi.MQ_PREC_FP64)
-            et32, _, s32 = plan.davidson(psi2, tol=1e-10, max_iter=40, precision=abi.MQ_PREC_FP32)
-            rec = {"site": i, "n": lay2.n, "e": e, "e_other": e2, "seconds": t3 - t2, "seconds_other": c1 - c0,
-                   "iterations": st["iterations"], "iterations_other": st2["iterations"],
-                   "e_tight_fp64": et64, "e_tight_mixed": et32, "residual_tight_fp64": s64["residual_norm"],
-                   "residual_tight_mixed": s32["residual_norm"]}
-            if self.compare_plain:
-                # the reference's launch structure (needs a plan made with MQ_OPT_PLAIN_TABLES)
-                p0 = time.perf_counter()
-                ep, _, stp = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter,
-                                           precision=abi.MQ_PREC_FP64_PLAIN)
-                rec.update(e_plain=ep, seconds_plain=time.perf_counter() - p0, iterations_plain=stp["iterations"])
-            rec["seconds_extra"] = time.perf_counter() - c0
-            self.compare_precision.append(rec)
-            extra = time.perf_counter() - c0
-            self.timing["compare"] = self.timing.get("compare", 0.0) + extra
-            t3s = t3 + extra  # the comparison solves are not charged to the split
         pst = plan.stats()
         self.last_residual = st["residual_norm"]
         self.hpsi_flops += pst["flops_total"] * st["iterations"]
@@ -434,13 +406,13 @@ class TwoSiteDMRG:
         self.timing["hpsi"] += st["hpsi_seconds"]
         plan.close()
         tw, lb, t4, t5 = self.accept(i, to_right, e, psi, lay2, red)
-        for k, v in (("merge", tm - t0), ("plan", t2 - t1), ("davidson", t3 - t2), ("svd", t4 - t3s), ("grow", t5 - t4)):
+        for k, v in (("merge", tm - t0), ("plan", t2 - t1), ("davidson", t3 - t2), ("svd", t4 - t3), ("grow", t5 - t4)):
             self.timing[k] += v
         if self.verbose:
             print(f"  site {i} {'->' if to_right else '<-'} E={e + self.ham.ecore:.12f} M={sum(lb.left.sizes)} "
                   f"tw={tw:.2e} it={st['iterations']} plan {t2 - t1:.3f} s (tables {pst['seconds_tables']:.3f}, "
                   f"blocks {pst['seconds_blocks']:.3f}, terms {pst['n_terms']}) davidson {t3 - t2:.3f} s "
-                  f"svd {t4 - t3s:.3f} s grow {t5 - t4:.3f} s", flush=True)
+                  f"svd {t4 - t3:.3f} s grow {t5 - t4:.3f} s", flush=True)
         return e + self.ham.ecore
 
     def site_problem(self, i):

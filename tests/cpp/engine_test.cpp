@@ -1,0 +1,181 @@
+// The C++ forwarding layer (include/qcmaquis_amd.hpp) on a small quantum-chemistry problem:
+//  * Engine::overlap_mpo_left_step / _right_step equal the C ABI's mq_lstep / mq_rstep (bitwise);
+//  * solve<double> on the one-site problem of site 1 returns the lowest eigenvalue of the dense
+//    effective Hamiltonian assembled column by column with SuperHamil::apply (1e-9 relative), and
+//    writes the eigenvector block by block into the caller's pointers (residual 1e-6);
+//  * a GMRES request and a wrong vector length raise qcmaquis_amd::Error.
+// Prints "ENGINE_TEST_OK" on success, exits non-zero otherwise.
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <random>
+
+#include "qcmaquis_amd.hpp"
+
+using namespace qcmaquis_amd;
+
+static uint64_t binom(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    uint64_t r = 1;
+    for (int i = 1; i <= k; ++i) r = r * uint64_t(n - k + i) / uint64_t(i);
+    return r;
+}
+
+// TwoU1 bond indices of L sites with (nu, nd) electrons: every sector reachable from both ends,
+// size min(states on the left, states on the right)
+static std::vector<Index> bonds_twou1(int L, int nu, int nd) {
+    std::vector<Index> out;
+    for (int k = 0; k <= L; ++k) {
+        std::map<Charge, uint32_t> m;
+        for (int a = 0; a <= std::min(k, nu); ++a)
+            for (int b = 0; b <= std::min(k, nd); ++b) {
+                uint64_t left = binom(k, a) * binom(k, b), right = binom(L - k, nu - a) * binom(L - k, nd - b);
+                if (left && right) m[Charge{a, b}] = uint32_t(std::min(left, right));
+            }
+        std::vector<std::pair<Charge, uint32_t>> v(m.rbegin(), m.rend());
+        out.push_back(Index(v));
+    }
+    return out;
+}
+
+// elements of a right-paired MPS tensor: sum over left sectors of ls * sum_sigma d_sigma * rs(lc+sigma)
+static uint64_t rp_elements(Index const& l, Index const& p, Index const& r) {
+    uint64_t n = 0;
+    for (size_t a = 0; a < l.size(); ++a)
+        for (size_t s = 0; s < p.size(); ++s) {
+            Charge c{l[a].first.q0 + p[s].first.q0, l[a].first.q1 + p[s].first.q1};
+            for (size_t b = 0; b < r.size(); ++b)
+                if (r[b].first == c) n += uint64_t(l[a].second) * p[s].second * r[b].second;
+        }
+    return n;
+}
+
+static void require(bool ok, char const* what) {
+    if (!ok) {
+        std::fprintf(stderr, "FAILED: %s\n", what);
+        std::exit(1);
+    }
+}
+
+int main() {
+    const int L = 4;
+    Context ctx(0, MQ_SYM_TWOU1);
+    // integrals: ramped h_pp, weak hopping, on-site and pair Coulomb terms with all symmetries
+    std::vector<double> h(L * L, 0.0), eri(L * L * L * L, 0.0);
+    for (int p = 0; p < L; ++p) {
+        h[p * L + p] = -2.0 + 0.7 * p;
+        if (p + 1 < L) h[p * L + p + 1] = h[(p + 1) * L + p] = -0.4;
+    }
+    auto E = [&](int p, int q, int r, int s) -> double& { return eri[((p * L + q) * L + r) * L + s]; };
+    for (int p = 0; p < L; ++p)
+        for (int q = 0; q < L; ++q) E(p, p, q, q) = p == q ? 1.1 : 0.35;
+    mq_qc_t qc = nullptr;
+    ctx.check(mq_qc_build(MQ_SYM_TWOU1, L, h.data(), eri.data(), 0.0, 1e-14, &qc));
+    std::vector<std::unique_ptr<MPOTensor>> mpo;
+    for (int k = 0; k < L; ++k) {
+        mq_mpo_desc d;
+        ctx.check(mq_qc_site_mpo(qc, k, 0, &d));
+        mpo.emplace_back(new MPOTensor(ctx, d));
+    }
+    mq_index_desc pd;
+    ctx.check(mq_qc_phys(qc, 0, &pd));
+    Index phys = Index::from_desc(pd);
+    std::vector<Index> bonds = bonds_twou1(L, 2, 2);
+    std::mt19937_64 rng(7);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    std::vector<MPSTensor> mps;
+    for (int k = 0; k < L; ++k) {
+        std::vector<double> x(rp_elements(bonds[k], phys, bonds[k + 1]));
+        for (auto& v : x) v = nd(rng);
+        mps.emplace_back(bonds[k], phys, bonds[k + 1], x);
+    }
+    // left step: facade vs C ABI
+    Boundary L0 = Boundary::initial(ctx, bonds[0], mpo[0]->row_dim());
+    Boundary L1 = Engine::overlap_mpo_left_step(ctx, mps[0], mps[0], L0, *mpo[0], true);
+    mq_boundary_t raw = nullptr;
+    ctx.check(mq_lstep(ctx.get(), mpo[0]->get(), L0.get(), mps[0].device(ctx), mps[0].device(ctx), 1, MQ_PREC_FP64, &raw));
+    Boundary Lraw(ctx, raw);
+    require(L1.download() == Lraw.download(), "left step: facade differs from mq_lstep");
+    // right steps from the end to bond 2
+    Boundary R = Boundary::initial(ctx, bonds[L], mpo[L - 1]->col_dim());
+    for (int k = L - 1; k >= 2; --k) R = Engine::overlap_mpo_right_step(ctx, mps[k], mps[k], R, *mpo[k]);
+    mq_boundary_t rraw = nullptr;
+    ctx.check(mq_rstep(ctx.get(), mpo[L - 1]->get(), Boundary::initial(ctx, bonds[L], 1).get(), mps[L - 1].device(ctx),
+                       mps[L - 1].device(ctx), 1, MQ_PREC_FP64, &rraw));
+    Boundary R3raw(ctx, rraw);
+    Boundary R3 = Engine::overlap_mpo_right_step(ctx, mps[L - 1], mps[L - 1], Boundary::initial(ctx, bonds[L], 1),
+                                                 *mpo[L - 1]);
+    require(R3.download() == R3raw.download(), "right step: facade differs from mq_rstep");
+    // one-site problem of site 1
+    SuperHamil H = Engine::contraction_schedule(ctx, mps[1], L1, R, *mpo[1]);
+    const uint64_t n = H.size();
+    require(n == mps[1].data().size(), "site problem size");
+    std::vector<double> dense(n * n), col, e(n);
+    for (uint64_t c = 0; c < n; ++c) {
+        std::fill(e.begin(), e.end(), 0.0);
+        e[c] = 1.0;
+        H.apply(e, col);
+        for (uint64_t r = 0; r < n; ++r) dense[r * n + c] = col[r];
+    }
+    // lowest eigenvalue of the dense matrix by cyclic Jacobi rotations
+    std::vector<double> a = dense;
+    for (int sweep = 0; sweep < 60; ++sweep)
+        for (uint64_t p = 0; p < n; ++p)
+            for (uint64_t q = p + 1; q < n; ++q) {
+                double apq = 0.5 * (a[p * n + q] + a[q * n + p]);
+                if (std::fabs(apq) < 1e-300) continue;
+                double tau = (a[q * n + q] - a[p * n + p]) / (2 * apq);
+                double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1 + tau * tau));
+                double cs = 1 / std::sqrt(1 + t * t), sn = t * cs;
+                for (uint64_t k = 0; k < n; ++k) {
+                    double x = a[k * n + p], y = a[k * n + q];
+                    a[k * n + p] = cs * x - sn * y;
+                    a[k * n + q] = sn * x + cs * y;
+                }
+                for (uint64_t k = 0; k < n; ++k) {
+                    double x = a[p * n + k], y = a[q * n + k];
+                    a[p * n + k] = cs * x - sn * y;
+                    a[q * n + k] = sn * x + cs * y;
+                }
+            }
+    double lowest = a[0];
+    for (uint64_t i = 1; i < n; ++i) lowest = std::min(lowest, a[i * n + i]);
+    // solve<double> with the blocks of the two left sectors' halves as the caller's views
+    std::vector<uint64_t> blocks{n / 2, n - n / 2};
+    DavidsonVector<double> init(blocks);
+    for (auto& v : init.data()) v = nd(rng);
+    std::vector<double> out0(blocks[0]), out1(blocks[1]);
+    std::vector<double*> out{out0.data(), out1.data()};
+    std::vector<DavidsonVector<double>> ortho;
+    double ev = solve<double>(out, init, H, ortho, 0.0, 1e-12, int(std::min<uint64_t>(n, 60)));
+    std::printf("n=%llu lowest dense %.12f solve %.12f\n", (unsigned long long)n, lowest, ev);
+    require(std::fabs(ev - lowest) <= 1e-9 * std::max(1.0, std::fabs(lowest)), "solve: eigenvalue");
+    std::vector<double> v(out0);
+    v.insert(v.end(), out1.begin(), out1.end()), H.apply(v, col);
+    double rn = 0, vn = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        rn += (col[i] - ev * v[i]) * (col[i] - ev * v[i]);
+        vn += v[i] * v[i];
+    }
+    require(std::sqrt(rn / vn) < 1e-6, "solve: residual of the written eigenvector");
+    bool threw = false;
+    try {
+        solve<double>(out, init, H, ortho, 1.0, 1e-8, 8);
+    } catch (Error const& err) {
+        threw = err.code() == MQ_ERR_UNSUPPORTED;
+    }
+    require(threw, "solve: gmres > 0 must raise");
+    threw = false;
+    try {
+        DavidsonVector<double> bad(std::vector<uint64_t>{n + 1});
+        std::vector<double> o(n + 1);
+        std::vector<double*> op{o.data()};
+        solve<double>(op, bad, H, ortho, 0.0, 1e-8, 8);
+    } catch (Error const& err) {
+        threw = err.code() == MQ_ERR_INVALID;
+    }
+    require(threw, "solve: wrong vector length must raise");
+    mq_qc_destroy(qc);
+    std::printf("ENGINE_TEST_OK\n");
+    return 0;
+}
