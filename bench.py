@@ -435,23 +435,33 @@ class ProfilerRegion:
         return False
 
 
-def sweep_compare(ctx, L=24, M=512, seed=1002, n_sweeps=4):
-    """Three two-site DMRG runs of n_sweeps sweeps each from the same seeded random MPS, Davidson
-    8 iterations and tol 1e-8 per step: FP64 with the grouped launches, mixed precision (FP32
-    contraction, FP64 refinement of every eigenvector) and FP64 with the reference's launch
-    structure (one library dgemm per T task and per cohort, one gather per term).  Reported per
-    run: the wall time and final energy of every sweep, and for the last sweep of the mixed and
-    the reference-style runs against the FP64 run: |dE| of the final energy, the largest
-    difference in truncated weight over the sweep's steps, and the final Davidson residual."""
+def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=2):
+    """Three two-site DMRG runs of n_sweeps sweeps each from one common start (a seeded random MPS
+    after warm_sweeps FP64 sweeps), Davidson 8 iterations and tol 1e-8 per step: FP64 with the
+    grouped launches, mixed precision (FP32 contraction, FP64 refinement of every eigenvector) and
+    FP64 with the reference's launch structure (one library dgemm per T task and per cohort, one
+    gather per term).  The common start keeps the comparison on precision: from a random MPS, runs
+    whose Davidson results differ in the last digits can settle in different local minima of the
+    sweep.  Reported per run: the wall time and final energy of every sweep, and for the last
+    sweep of the mixed and the reference-style runs against the FP64 run: |dE| of the final
+    energy, the largest difference in truncated weight over the sweep's steps, and the final
+    Davidson residual."""
     from qcmaquis_amd import abi
     from qcmaquis_amd.dmrg import TwoSiteDMRG
     from qcmaquis_amd.qc import QCHamiltonian, make_integrals
     h, eri, ec = make_integrals(L, seed, 16)
     ham = QCHamiltonian(h, eri, ec)
+    t0 = time.perf_counter()
+    d = TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8)
+    warm_e = [d.sweep()[0] for _ in range(warm_sweeps)]
+    start = ([l for l in d.lay], [x.copy() for x in d.data])
+    warm_s = time.perf_counter() - t0
+    del d
+    log("sweeps: common start after %d FP64 sweeps (%.1f s), E %s" % (warm_sweeps, warm_s, warm_e))
     runs = {}
     for name, prec in (("fp64", abi.MQ_PREC_FP64), ("mixed", abi.MQ_PREC_FP32), ("fp64_plain", abi.MQ_PREC_FP64_PLAIN)):
         ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, int(prec == abi.MQ_PREC_FP64_PLAIN))
-        d = TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8, precision=prec)
+        d = TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8, precision=prec, mps=start)
         rec = {"sweep_seconds": [], "sweep_energies": []}
         for s in range(n_sweeps):
             n0 = len(d.trunc)
@@ -480,13 +490,47 @@ def sweep_compare(ctx, L=24, M=512, seed=1002, n_sweeps=4):
            "vs_fp64": cmp, "fp64_final_residual_norm": ref["final_residual_norm"],
            "fp64_last_sweep_speedup_vs_fp64_plain": round(runs["fp64_plain"]["sweep_seconds"][-1] /
                                                           ref["sweep_seconds"][-1], 3),
-           "note": f"{L} orbitals, M={M}, {n_sweeps} sweeps per run from the same seeded random MPS, Davidson 8 "
-                   "iterations / tol 1e-8 per two-site step; every run is swept for real (its own energies, "
-                   "truncations and boundaries)"}
+           "common_start": {"fp64_sweeps": warm_sweeps, "energies": warm_e, "seconds": round(warm_s, 3)},
+           "note": f"{L} orbitals, M={M}, {n_sweeps} sweeps per run from one common start (seeded random MPS after "
+                   f"{warm_sweeps} FP64 sweeps), Davidson 8 iterations / tol 1e-8 per two-site step; every run is "
+                   "swept for real (its own energies, truncations and boundaries)"}
     log("sweep comparison: last sweep %.2f s (fp64) / %.2f s (mixed) / %.2f s (fp64 plain); mixed |dE| %.2e, "
         "|d tw| %.2e" % (ref["sweep_seconds"][-1], runs["mixed"]["sweep_seconds"][-1],
                          runs["fp64_plain"]["sweep_seconds"][-1], cmp["mixed"]["abs_delta_e_final"],
                          cmp["mixed"]["max_abs_delta_truncated_weight_last_sweep"]))
+    return out
+
+
+def davidson_compare(plan, n, seed=3):
+    """precision study on the benchmark site problem: Jacobi-Davidson from one start vector in
+    FP64 and in mixed precision (FP32 contraction, FP64 refinement of the eigenvector), with the
+    sweep budget (8 iterations, tol 1e-8) and converged (tol 1e-10, at most 60 iterations);
+    energies, |dE| of mixed vs FP64, residual norms and wall times"""
+    from qcmaquis_amd import abi
+    # start vector of a sweep's later steps: close to the ground state (30 FP64 iterations from a
+    # random vector), not the random vector itself
+    t0 = time.perf_counter()
+    e0, psi0, st0 = plan.davidson(np.random.default_rng(seed).normal(size=n), tol=1e-12, max_iter=30)
+    out = {"start": {"fp64_iterations": int(st0["iterations"]), "energy": e0,
+                     "residual_norm": float(st0["residual_norm"]), "seconds": round(time.perf_counter() - t0, 3)}}
+    log("davidson start vector: E=%.10f, |r| %.2e after %d FP64 iterations" % (e0, st0["residual_norm"],
+                                                                              st0["iterations"]))
+    for name, tol, it in (("budget", 1e-8, 8), ("converged", 1e-10, 60)):
+        rec = {}
+        for prec_name, prec in (("fp64", abi.MQ_PREC_FP64), ("mixed", abi.MQ_PREC_FP32)):
+            t0 = time.perf_counter()
+            e, _, st = plan.davidson(psi0, tol=tol, max_iter=it, precision=prec)
+            rec[prec_name] = {"energy": e, "iterations": int(st["iterations"]),
+                              "refine_iterations": int(st["refine_iterations"]),
+                              "residual_norm": float(st["residual_norm"]), "seconds": round(time.perf_counter() - t0, 3)}
+        rec["abs_delta_e_mixed_vs_fp64"] = abs(rec["mixed"]["energy"] - rec["fp64"]["energy"])
+        rec["tol"], rec["max_iter"] = tol, it
+        out[name] = rec
+        log("davidson %s: fp64 E=%.10f (%d it, |r| %.2e, %.2f s), mixed E=%.10f (%d it, %d FP64, |r| %.2e, %.2f s), "
+            "|dE| %.2e" % (name, rec["fp64"]["energy"], rec["fp64"]["iterations"], rec["fp64"]["residual_norm"],
+                           rec["fp64"]["seconds"], rec["mixed"]["energy"], rec["mixed"]["iterations"],
+                           rec["mixed"]["refine_iterations"], rec["mixed"]["residual_norm"], rec["mixed"]["seconds"],
+                           rec["abs_delta_e_mixed_vs_fp64"]))
     return out
 
 
@@ -534,7 +578,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="qc", choices=["qc", "qc3", "synthetic", "batch", "batch-sweep"])
+    ap.add_argument("--workload", default="qc3", choices=["qc", "qc3", "synthetic", "batch", "batch-sweep"])
     ap.add_argument("--M", type=int, default=None, help="bond dimension of the qc / qc3 workloads")
     ap.add_argument("--regen-bonds", action="store_true", help="qc3: redo the warm-up half sweep")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batch workload")
@@ -579,9 +623,7 @@ def main():
     else:
         w = synthetic_workload(ctx)
     log("workload built in %.1f s: %s" % (time.perf_counter() - t0, w["desc"]))
-    ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, int(not args.no_plain))  # also the reference-style launches
     plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
-    ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
     st = plan.stats()
     log("plan:", {k: st[k] for k in ("n_t_tasks", "n_cohorts", "n_lines", "n_terms", "flops_total", "flops_gemm",
                                       "flops_gemm_exec", "bytes_compulsory", "t_elements", "s_elements",
@@ -650,10 +692,17 @@ def main():
         graph_ms[name] = round((time.perf_counter() - t0) / args.steps * 1e3, 4)
     plan.use_graphs(False)
     graph_ok = bool(torch.equal(outg, out32))  # last graph call was FP32 on the same psi
+    # precision study of the Davidson on this site problem (the mixed path's energy against FP64)
+    dav = davidson_compare(plan, n) if args.workload == "qc3" else None
     # FP64 with the reference's launch structure: one library dgemm per T task and per output
-    # block, one gather per term (MQ_PREC_FP64_PLAIN), on the same plan and psi
+    # block, one gather per term (MQ_PREC_FP64_PLAIN), on a plan of its own (made with the per-task
+    # tables) and the same psi
     plain = None
     if not args.no_plain:
+        plan.close()
+        ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 1)
+        plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
+        ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
         outp = torch.empty_like(psi)
         plan.apply_device(psi.data_ptr(), outp.data_ptr(), abi.MQ_PREC_FP64_PLAIN)
         ctx.sync()
@@ -711,10 +760,11 @@ def main():
     # HBM bytes per launch (per H*psi for the GEMM family) from the committed rocprofv3 counter runs
     # of this default workload (FETCH_SIZE / WRITE_SIZE scaled by a device-copy calibration)
     try:
-        prof = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))
+        prof = json.load(open(os.path.join(ROOT, "profiles", "traffic.json"))).get(args.workload, {})
+        default_shape = args.workload != "qc" or args.warm_sweeps == 1
         for rl in (dominant, other):
             rec = prof.get(rl["kernel"])
-            if rec and args.workload == "qc" and args.warm_sweeps == 1:
+            if rec and default_shape and not args.M:
                 rl["traffic"] = round((rec["fetch_gb"] + rec["write_gb"]) * 1e9)
                 rl["traffic_source"] = rec["source"]
     except (OSError, ValueError, KeyError):
@@ -731,7 +781,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded integrals, random initial MPS)" if args.workload == "qc" else "synthetic",
+        "data": {"qc": "synthetic (seeded integrals, random initial MPS)",
+                 "qc3": "synthetic (seeded rank-80 integrals; MPS sectors from a warm-up half sweep at M=128, "
+                        "scaled to M=2048, seeded random elements, normalised)"}.get(args.workload, "synthetic"),
         "config": dict(w["desc"], step="one H*psi (Davidson iteration) of the site problem, inputs resident in HBM",
                        flops_per_step_reference_model=st["flops_total"], gemm_flops_executed=st["flops_gemm_exec"],
                        stage_ms={"setup": round(stage[0], 4), "t_gemm": round(stage[1], 4),
@@ -741,13 +793,18 @@ def main():
                                    "tflops": round(st["flops_total"] / (ms32 * 1e-3) / 1e12, 4),
                                    "speedup_vs_fp64": round(ms / ms32, 3), "rel_error_vs_fp64": err32,
                                    "stage_ms": [round(x, 4) for x in stage32]},
-                       fp64_plain=plain),
+                       fp64_plain=plain, davidson_precision=dav),
         "roofline": dominant,
         "roofline_other": other,
     }
     plan.close()
-    if args.workload == "qc" and not args.no_sweeps:
-        line["config"]["sweeps"] = sweep_compare(ctx)
+    if args.workload in ("qc", "qc3") and not args.no_sweeps:
+        # real sweeps run at configuration 2 (24 orbitals, M = 512): a 40-orbital M = 2048 sweep takes
+        # tens of minutes
+        ctx2 = Context(0, abi.MQ_SYM_TWOU1) if args.workload == "qc3" else ctx
+        line["config"]["sweeps_configuration_2"] = sweep_compare(ctx2)
+        if ctx2 is not ctx:
+            ctx2.close()
     if not args.no_cpu:
         try:
             line["cpu_baseline"] = cpu_baseline(w, st["flops_total"])
