@@ -588,6 +588,9 @@ def main():
     ap.add_argument("--no-sweeps", action="store_true", help="skip the FP64 / mixed sweep comparison")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the reference-style FP64 H*psi (thousands of library launches: slow under counters)")
+    ap.add_argument("--no-davidson", action="store_true", help="qc3: skip the Davidson precision study")
+    ap.add_argument("--chunk-mb", type=int, default=None,
+                    help="largest T + S of one plan chunk in MiB (0: one chunk; default: the library's rule)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy inside the profiler region (HBM counter scale)")
     args = ap.parse_args()
@@ -623,12 +626,15 @@ def main():
     else:
         w = synthetic_workload(ctx)
     log("workload built in %.1f s: %s" % (time.perf_counter() - t0, w["desc"]))
+    if args.chunk_mb is not None:
+        ctx.set_option(abi.MQ_OPT_CHUNK_MB, args.chunk_mb)
     plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
     st = plan.stats()
     log("plan:", {k: st[k] for k in ("n_t_tasks", "n_cohorts", "n_lines", "n_terms", "flops_total", "flops_gemm",
                                       "flops_gemm_exec", "bytes_compulsory", "t_elements", "s_elements",
                                       "s_line_elements", "gathered_elements", "n_sblocks", "union_elements",
-                                      "build_seconds")})
+                                      "sblock_macs", "n_chunks", "ts_buffer_elements", "build_seconds",
+                                      "seconds_tables", "seconds_blocks")})
     n = w["layout"].n
     psi = torch.randn(n, dtype=torch.float64, device="cuda")
     out = torch.empty_like(psi)
@@ -693,7 +699,7 @@ def main():
     plan.use_graphs(False)
     graph_ok = bool(torch.equal(outg, out32))  # last graph call was FP32 on the same psi
     # precision study of the Davidson on this site problem (the mixed path's energy against FP64)
-    dav = davidson_compare(plan, n) if args.workload == "qc3" else None
+    dav = davidson_compare(plan, n) if args.workload == "qc3" and not args.no_davidson else None
     # FP64 with the reference's launch structure: one library dgemm per T task and per output
     # block, one gather per term (MQ_PREC_FP64_PLAIN), on a plan of its own (made with the per-task
     # tables) and the same psi

@@ -62,6 +62,13 @@ extern "C" {
                                   tile in blocks); MQ_SMALL_TILE_AUTO (default): per site problem,
                                   64 when its mean S tile has at most 16 elements, else 0 */
 #define MQ_SMALL_TILE_AUTO (-1)
+#define MQ_OPT_CHUNK_MB 3      /* value >= 0: H*psi plans made afterwards split their output columns
+                                  by right sector into chunks of at most this many MiB of FP64 T and
+                                  S intermediates (a larger right sector is a chunk of its own),
+                                  run one after the other on buffers of the largest chunk's size;
+                                  0: one chunk; MQ_CHUNK_AUTO (default): 49152 MiB.  Plans made
+                                  with MQ_OPT_PLAIN_TABLES have one chunk. */
+#define MQ_CHUNK_AUTO (-1)
 
 /* MPS pairing (reference: MPSTensor::make_left_paired / make_right_paired, mpstensor.hpp:211-237) */
 #define MQ_RIGHT_PAIRED 0      /* blocks per left charge:  ls x (sum_sigma d_sigma * rs(lc+sigma)) */
@@ -140,11 +147,11 @@ typedef struct {
     uint64_t flops_gemm;              /* T-GEMM + L*S GEMM part of the reference model */
     uint64_t flops_gemm_exec;         /* flops of the GEMMs as executed by this plan */
     uint64_t bytes_compulsory;        /* FP64 bytes */
-    uint64_t t_elements, s_elements, l_panel_elements, r_tr_elements;
+    uint64_t t_elements, s_elements, l_panel_elements, r_tr_elements;  /* T and S: over all chunks */
     double   build_seconds;
     uint64_t s_line_elements;         /* sum over S lines of rows * cols (S elements written) */
     uint64_t gathered_elements;       /* sum over S terms of rows * cols (T elements of the term sum) */
-    uint64_t n_sblocks;               /* 16-line blocks of the matrix-core term sum */
+    uint64_t n_sblocks;               /* blocks (up to 64 lines) of the matrix-core term sum */
     uint64_t union_elements;          /* sum over blocks of union tiles * tile elements (T elements read) */
     double   seconds_tables;          /* host: schedule tables built and checked (part of build_seconds) */
     double   seconds_blocks;          /* host: term-sum block tables (part of build_seconds) */
@@ -152,8 +159,11 @@ typedef struct {
     uint64_t small_lines;             /* S tiles summed term by term (MQ_OPT_SMALL_TILE_MAX) */
     uint64_t small_terms;             /* their terms */
     uint64_t small_gathered;          /* their T elements read: sum over terms of the tile elements */
-    uint64_t max_union_tiles;         /* largest union of T tiles of one 16-line block */
+    uint64_t max_union_tiles;         /* largest union of T tiles of one term-sum block */
     uint64_t split_reduces;           /* split-K reduce jobs of the T-GEMM and L*S GEMM stages */
+    uint64_t sblock_macs;             /* matrix-core multiply-adds of the block term sum */
+    uint64_t n_chunks;                /* chunks of the plan (MQ_OPT_CHUNK_MB) */
+    uint64_t ts_buffer_elements;      /* T + S buffer elements (the largest chunk's) */
 } mq_hpsi_stats_t;
 
 typedef struct {

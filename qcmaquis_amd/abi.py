@@ -24,6 +24,8 @@ MQ_NOALLOC_SCRATCH_ELEMS = 1 << 23
 MQ_OPT_PLAIN_TABLES = 1
 MQ_OPT_SMALL_TILE_MAX = 2
 MQ_SMALL_TILE_AUTO = -1
+MQ_OPT_CHUNK_MB = 3
+MQ_CHUNK_AUTO = -1
 MQ_RIGHT_PAIRED = 0
 MQ_LEFT_PAIRED = 1
 MQ_NO_PARTNER = 0xFFFFFFFF
@@ -107,6 +109,9 @@ class HpsiStats(C.Structure):
         ("small_gathered", C.c_uint64),
         ("max_union_tiles", C.c_uint64),
         ("split_reduces", C.c_uint64),
+        ("sblock_macs", C.c_uint64),
+        ("n_chunks", C.c_uint64),
+        ("ts_buffer_elements", C.c_uint64),
     ]
 
 

@@ -35,6 +35,7 @@ struct mq_ctx_s {
     double* dwork = nullptr;  // DOT_BLOCKS partials + 64 scalars
     bool plain_tables = false;         // MQ_OPT_PLAIN_TABLES
     int32_t small_tile_max = MQ_SMALL_TILE_AUTO;  // MQ_OPT_SMALL_TILE_MAX
+    int32_t chunk_mb = MQ_CHUNK_AUTO;             // MQ_OPT_CHUNK_MB
     rocblas_handle blas = nullptr;     // library GEMMs of MQ_PREC_FP64_PLAIN, made on first use
     double* noalloc_scratch = nullptr; // this device's scratch array of mq_davidson_noalloc
 };
@@ -276,8 +277,17 @@ struct GemmStage {
     GemmReduce* reduces = nullptr;
     int64_t n_reduces = 0;
     double* ws = nullptr;
-    void upload(std::vector<GemmDesc> const& v, std::vector<GemmRow> const& r, hipStream_t s) {
-        GemmSchedule sch = build_gemm_schedule(v);
+    bool own_ws = true;
+    // sch: the stage's schedule (built here when null); ws_shared: a split-K workspace of at least
+    // sch->ws_elems elements owned by the caller (allocated here when null)
+    void upload(std::vector<GemmDesc> const& v, std::vector<GemmRow> const& r, hipStream_t s,
+                GemmSchedule const* pre = nullptr, double* ws_shared = nullptr) {
+        GemmSchedule local;
+        if (!pre) {
+            local = build_gemm_schedule(v);
+            pre = &local;
+        }
+        GemmSchedule const& sch = *pre;
         probs = dev_upload(v, s);
         rows = dev_upload(r, s);
         for (int sh = 0; sh < GEMM_N_SHAPES; ++sh) {
@@ -286,7 +296,8 @@ struct GemmStage {
         }
         reduces = dev_upload(sch.reduces, s);
         n_reduces = int64_t(sch.reduces.size());
-        ws = dev_alloc<double>(sch.ws_elems);
+        own_ws = ws_shared == nullptr;
+        ws = own_ws ? dev_alloc<double>(sch.ws_elems) : ws_shared;
     }
     void run(Bases const& b, hipStream_t s, GemmPrec prec) const {
         hipcheck(launch_gemm(prec, probs, rows, tiles, n_tiles, reduces, n_reduces, ws, b, s), "gemm launch");
@@ -297,7 +308,7 @@ struct GemmStage {
         rows = nullptr;
         for (auto& t : tiles) dev_free(t), t = nullptr;
         dev_free(reduces);
-        dev_free(ws);
+        if (own_ws) dev_free(ws);
         probs = nullptr;
         reduces = nullptr;
         ws = nullptr;
@@ -309,6 +320,7 @@ struct SAsmStage {
     SBlock* blocks = nullptr;
     SWork* work = nullptr;
     int64_t* utile = nullptr;
+    uint32_t* kstep = nullptr;
     double* ualpha = nullptr;
     float* ualpha32 = nullptr;
     int64_t* lsoff = nullptr;
@@ -324,6 +336,7 @@ struct SAsmStage {
         blocks = dev_upload(B.blocks, s);
         work = dev_upload(B.work, s);
         utile = dev_upload(B.utile, s);
+        kstep = dev_upload(B.kstep, s);
         ualpha = dev_upload(B.ualpha, s);
         lsoff = dev_upload(B.lsoff, s);
         llds = dev_upload(B.llds, s);
@@ -336,7 +349,7 @@ struct SAsmStage {
     }
     void run(const double* T, double* S, hipStream_t s) const {
         if (n_work)
-            hipcheck(launch_sblocks<double>(blocks, work, n_work, utile, ualpha, lsoff, llds, T, S, s), "S-assembly");
+            hipcheck(launch_sblocks<double>(blocks, work, n_work, utile, kstep, ualpha, lsoff, llds, T, S, s), "S-assembly");
         if (n_swork) hipcheck(launch_small_lines<double>(swork, n_swork, slines, sterms, T, S, s), "S-assembly");
     }
     void ensure32(hipStream_t s) {
@@ -348,13 +361,14 @@ struct SAsmStage {
     void run32(const float* T, float* S, hipStream_t s) {
         ensure32(s);
         if (n_work)
-            hipcheck(launch_sblocks<float>(blocks, work, n_work, utile, ualpha32, lsoff, llds, T, S, s), "S-assembly");
+            hipcheck(launch_sblocks<float>(blocks, work, n_work, utile, kstep, ualpha32, lsoff, llds, T, S, s), "S-assembly");
         if (n_swork) hipcheck(launch_small_lines<float>(swork, n_swork, slines, sterms, T, S, s), "S-assembly");
     }
     void release() {
-        dev_free(blocks); dev_free(work); dev_free(utile); dev_free(ualpha); dev_free(ualpha32); dev_free(lsoff);
-        dev_free(llds); dev_free(swork); dev_free(slines); dev_free(sterms);
-        blocks = nullptr; work = nullptr; utile = nullptr; ualpha = nullptr; ualpha32 = nullptr; lsoff = nullptr;
+        dev_free(blocks); dev_free(work); dev_free(utile); dev_free(kstep); dev_free(ualpha); dev_free(ualpha32);
+        dev_free(lsoff); dev_free(llds); dev_free(swork); dev_free(slines); dev_free(sterms);
+        blocks = nullptr; work = nullptr; utile = nullptr; kstep = nullptr; ualpha = nullptr; ualpha32 = nullptr;
+        lsoff = nullptr;
         llds = nullptr; swork = nullptr; slines = nullptr; sterms = nullptr;
         n_work = 0;
         n_swork = 0;
@@ -386,10 +400,15 @@ struct mq_hpsi_plan_s {
     HpsiPlanHost host;
     const double* rdata = nullptr;
     std::shared_ptr<void> rhold;  // the right boundary's storage, read in place by every H*psi
-    GemmStage tg, cg;
-    SAsmStage sa;
-    int64_t n_lines = 0, n_sblocks = 0;
-    uint64_t s_line_elements = 0, gathered_elements = 0, union_elements = 0, max_union_tiles = 0;
+    // the stages of every chunk of the plan, run chunk after chunk on the T and S buffers
+    struct Chunk {
+        GemmStage tg, cg;
+        SAsmStage sa;
+    };
+    std::vector<Chunk> chunks;
+    double* gemm_ws = nullptr;   // split-K workspace shared by the chunks' GEMM stages
+    int64_t n_lines = 0, n_sblocks = 0, n_reduces = 0;
+    uint64_t s_line_elements = 0, gathered_elements = 0, union_elements = 0, max_union_tiles = 0, sblock_macs = 0;
     double *T = nullptr, *S = nullptr, *Lpanel = nullptr;
     // FP32 operands of the mixed-precision H*psi, made on first use
     float *T32 = nullptr, *S32 = nullptr, *Lpanel32 = nullptr, *R32 = nullptr, *psi32 = nullptr;
@@ -516,6 +535,9 @@ int mq_ctx_set_option(mq_ctx_t ctx, int option, int value) {
             require(value == MQ_SMALL_TILE_AUTO || (value >= 0 && value <= SMALL_LINE_LANES),
                     "mq_ctx_set_option: MQ_OPT_SMALL_TILE_MAX is MQ_SMALL_TILE_AUTO or 0..256");
             ctx->small_tile_max = value;
+        } else if (option == MQ_OPT_CHUNK_MB) {
+            require(value == MQ_CHUNK_AUTO || value >= 0, "mq_ctx_set_option: MQ_OPT_CHUNK_MB is MQ_CHUNK_AUTO or >= 0");
+            ctx->chunk_mb = value;
         } else {
             throw Error(MQ_ERR_INVALID, "mq_ctx_set_option: unknown option");
         }
@@ -804,18 +826,23 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
     };
     HpsiPlanHost& H = p->host;
+    if (H.chunks.empty()) {  // batch plans: one chunk over the merged tables
+        HpsiChunk c;
+        c.t_gemm1 = H.t_gemms.size();
+        c.t_row1 = H.t_rows.size();
+        c.line1 = H.lines.size();
+        c.c_gemm1 = H.c_gemms.size();
+        c.t_elems = H.t_elems;
+        c.s_elems = H.s_elems;
+        H.chunks.push_back(c);
+        H.t_buf = H.t_elems;
+        H.s_buf = H.s_elems;
+    }
     uint64_t sizes[BASE_COUNT] = {0};
     sizes[BASE_PSI_IN] = p->psi_n;
     sizes[BASE_PSI_OUT] = p->psi_n;
-    sizes[BASE_T] = H.t_elems;
-    sizes[BASE_S] = H.s_elems;
     sizes[BASE_LPANEL] = H.lpanel_elems;
     sizes[BASE_BOUNDARY] = r_n;
-    check_gemms(H.t_gemms, sizes, H.t_rows);
-    check_gemms(H.c_gemms, sizes);
-    check_lines(H.lines, H.terms, H.t_elems, H.s_elems);
-    p->seconds_tables = seconds_since(t0);
-    auto t1 = std::chrono::steady_clock::now();
     // small-tile lines, decided per site problem (line_ranges: the first line of each instance and
     // the end)
     for (size_t i = 0; i + 1 < line_ranges.size(); ++i) {
@@ -823,30 +850,69 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
         mark_small_lines(H.lines, l0, l1, ctx->small_tile_max == MQ_SMALL_TILE_AUTO
                                               ? auto_small_tile_max(H.lines, l0, l1) : ctx->small_tile_max);
     }
-    SBlockTable SB = build_sblocks(H.lines, H.terms);
-    check_sblocks(SB, H.t_elems, H.s_elems);
-    p->seconds_blocks = seconds_since(t1);
-    p->small_lines = SB.small_lines.size();
-    p->small_terms = SB.small_terms.size();
-    p->small_gathered = SB.small_gathered;
+    // per chunk: its GEMM problems (T-GEMM rows re-based on the chunk's row table), checked against
+    // the chunk's T and S sizes, their schedules, and the term-sum block tables of its lines
+    const size_t nch = H.chunks.size();
+    std::vector<std::vector<GemmDesc>> tgs(nch), cgs(nch);
+    std::vector<std::vector<GemmRow>> trs(nch);
+    std::vector<GemmSchedule> tsch(nch), csch(nch);
+    std::vector<SBlockTable> sbs(nch);
+    double t_blocks = 0;
+    uint64_t ws_max = 0;
+    for (size_t c = 0; c < nch; ++c) {
+        HpsiChunk const& ch = H.chunks[c];
+        sizes[BASE_T] = ch.t_elems;
+        sizes[BASE_S] = ch.s_elems;
+        tgs[c].assign(H.t_gemms.begin() + int64_t(ch.t_gemm0), H.t_gemms.begin() + int64_t(ch.t_gemm1));
+        trs[c].assign(H.t_rows.begin() + int64_t(ch.t_row0), H.t_rows.begin() + int64_t(ch.t_row1));
+        for (auto& g : tgs[c])
+            if (g.flags & GEMM_ROWMAP) g.a -= int64_t(ch.t_row0);
+        cgs[c].assign(H.c_gemms.begin() + int64_t(ch.c_gemm0), H.c_gemms.begin() + int64_t(ch.c_gemm1));
+        check_gemms(tgs[c], sizes, trs[c]);
+        check_gemms(cgs[c], sizes);
+        HostTable<SLine> cl(H.lines.begin() + int64_t(ch.line0), H.lines.begin() + int64_t(ch.line1));
+        check_lines(cl, H.terms, ch.t_elems, ch.s_elems);
+        auto tb = std::chrono::steady_clock::now();
+        sbs[c] = build_sblocks(cl, H.terms);
+        check_sblocks(sbs[c], ch.t_elems, ch.s_elems);
+        t_blocks += seconds_since(tb);
+        tsch[c] = build_gemm_schedule(tgs[c]);
+        csch[c] = build_gemm_schedule(cgs[c]);
+        ws_max = std::max({ws_max, tsch[c].ws_elems, csch[c].ws_elems});
+    }
+    p->seconds_blocks = t_blocks;
+    p->seconds_tables = seconds_since(t0) - t_blocks;
     for (auto const& part : lparts)
         for (auto const& j : part.jobs)
             require(j.src >= 0 && j.dst >= 0 && uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= part.n &&
                         uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= H.lpanel_elems,
                     "L panel job outside its buffers");
     p->r_elems = r_n;
-    p->T = dev_alloc<double>(H.t_elems);
-    p->S = dev_alloc<double>(H.s_elems);
+    p->T = dev_alloc<double>(H.t_buf);
+    p->S = dev_alloc<double>(H.s_buf);
     p->Lpanel = dev_alloc<double>(H.lpanel_elems);
-    p->tg.upload(H.t_gemms, H.t_rows, ctx->stream);
-    p->cg.upload(H.c_gemms, {}, ctx->stream);
-    p->sa.upload(SB, ctx->stream);
+    p->gemm_ws = dev_alloc<double>(ws_max);
+    p->chunks.resize(nch);
+    for (size_t c = 0; c < nch; ++c) {
+        auto& pc = p->chunks[c];
+        pc.tg.upload(tgs[c], trs[c], ctx->stream, &tsch[c], p->gemm_ws);
+        pc.cg.upload(cgs[c], {}, ctx->stream, &csch[c], p->gemm_ws);
+        pc.sa.upload(sbs[c], ctx->stream);
+        p->n_sblocks += int64_t(sbs[c].blocks.size());
+        p->union_elements += sbs[c].gathered;
+        p->max_union_tiles = std::max<uint64_t>(p->max_union_tiles, sbs[c].max_union);
+        p->sblock_macs += sbs[c].mac;
+        p->small_lines += sbs[c].small_lines.size();
+        p->small_terms += sbs[c].small_terms.size();
+        p->small_gathered += sbs[c].small_gathered;
+        p->n_reduces += pc.tg.n_reduces + pc.cg.n_reduces;
+        sbs[c] = SBlockTable();
+    }
     p->n_lines = int64_t(H.lines.size());
-    p->n_sblocks = int64_t(SB.blocks.size());
-    p->union_elements = SB.gathered;
-    for (auto const& b : SB.blocks) p->max_union_tiles = std::max<uint64_t>(p->max_union_tiles, uint64_t(b.nu));
-    if (ctx->plain_tables && p->n_instances == 1) {
+    if (ctx->plain_tables && p->n_instances == 1 && nch == 1) {
         auto tp = std::chrono::steady_clock::now();
+        sizes[BASE_T] = H.t_buf;
+        sizes[BASE_S] = H.s_buf;
         check_gemms(H.t_gemms_task, sizes);
         check_gemms(H.c_gemms_cohort, sizes);
         p->plain.t = H.t_gemms_task;
@@ -885,8 +951,11 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
     return guarded(ctx, [&] {
         require(mpo && left && right && out, "mq_hpsi_plan_create: null argument");
         auto t0 = std::chrono::steady_clock::now();
+        // largest T + S of one chunk (elements); the per-task tables of MQ_PREC_FP64_PLAIN need one chunk
+        const int64_t mb = ctx->chunk_mb == MQ_CHUNK_AUTO ? 49152 : ctx->chunk_mb;
+        const uint64_t chunk_elems = ctx->plain_tables ? 0 : uint64_t(mb) * (uint64_t(1) << 20) / sizeof(double);
         auto* p = new mq_hpsi_plan_s{ctx, build_hpsi_plan(ctx->sym, mpo->mpo, left->idx, right->idx, index_from(left_i),
-                                                          index_from(phys_i), index_from(right_i))};
+                                                          index_from(phys_i), index_from(right_i), chunk_elems)};
         std::unique_ptr<mq_hpsi_plan_s> guard(p);
         p->psi_n = p->host.mps.elements();
         p->rdata = right->data;
@@ -1055,7 +1124,10 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
     s->small_terms = p->small_terms;
     s->small_gathered = p->small_gathered;
     s->max_union_tiles = p->max_union_tiles;
-    s->split_reduces = uint64_t(p->tg.n_reduces + p->cg.n_reduces);
+    s->sblock_macs = p->sblock_macs;
+    s->split_reduces = uint64_t(p->n_reduces);
+    s->n_chunks = uint64_t(p->chunks.size());
+    s->ts_buffer_elements = H.t_buf + H.s_buf;
     return MQ_OK;
 }
 
@@ -1070,14 +1142,14 @@ static void ensure_fp32(mq_hpsi_plan_t p) {
     if (p->T32) return;
     hipStream_t s = p->ctx->stream;
     HpsiPlanHost const& H = p->host;
-    p->S32 = dev_alloc<float>(H.s_elems);
+    p->S32 = dev_alloc<float>(H.s_buf);
     p->Lpanel32 = dev_alloc<float>(H.lpanel_elems);
     p->R32 = dev_alloc<float>(p->r_elems);
     p->psi32 = dev_alloc<float>(p->psi_n);
     hipcheck(launch_convert(p->Lpanel, p->Lpanel32, H.lpanel_elems, s), "convert");
     hipcheck(launch_convert(p->rdata, p->R32, p->r_elems, s), "convert");
-    p->sa.ensure32(s);
-    p->T32 = dev_alloc<float>(H.t_elems);
+    for (auto& c : p->chunks) c.sa.ensure32(s);
+    p->T32 = dev_alloc<float>(H.t_buf);
 }
 
 // H*psi on the context stream; ev (optional, 5 events) brackets the stages set-up, T-GEMMs,
@@ -1108,6 +1180,8 @@ static void library_gemms(mq_ctx_t ctx, std::vector<GemmDesc> const& probs, Base
     }
 }
 
+// ev (optional): 2 + 3 * chunks events: start, set-up done, then per chunk its T-GEMMs, term sum
+// and output GEMMs done
 static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int precision, hipEvent_t* ev = nullptr) {
     hipStream_t s = p->ctx->stream;
     auto mark = [&](int i) {
@@ -1123,12 +1197,14 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
         mark(0);
         mark(1);
-        p->tg.run(b, s, GEMM_F64);
-        mark(2);
-        p->sa.run(p->T, p->S, s);
-        mark(3);
-        p->cg.run(b, s, GEMM_F64);
-        mark(4);
+        for (size_t c = 0; c < p->chunks.size(); ++c) {
+            p->chunks[c].tg.run(b, s, GEMM_F64);
+            mark(int(2 + 3 * c));
+            p->chunks[c].sa.run(p->T, p->S, s);
+            mark(int(3 + 3 * c));
+            p->chunks[c].cg.run(b, s, GEMM_F64);
+            mark(int(4 + 3 * c));
+        }
     } else if (precision == MQ_PREC_FP32) {
         ensure_fp32(p);
         b.p[BASE_PSI_IN] = p->psi32;
@@ -1139,12 +1215,14 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         mark(0);
         hipcheck(launch_convert(in, p->psi32, p->psi_n, s), "convert");
         mark(1);
-        p->tg.run(b, s, GEMM_F32);
-        mark(2);
-        p->sa.run32(p->T32, p->S32, s);
-        mark(3);
-        p->cg.run(b, s, GEMM_F32_OUT_F64);
-        mark(4);
+        for (size_t c = 0; c < p->chunks.size(); ++c) {
+            p->chunks[c].tg.run(b, s, GEMM_F32);
+            mark(int(2 + 3 * c));
+            p->chunks[c].sa.run32(p->T32, p->S32, s);
+            mark(int(3 + 3 * c));
+            p->chunks[c].cg.run(b, s, GEMM_F32_OUT_F64);
+            mark(int(4 + 3 * c));
+        }
     } else if (precision == MQ_PREC_FP64_PLAIN) {
         require(p->plain.ok, "H*psi MQ_PREC_FP64_PLAIN: the plan was made without MQ_OPT_PLAIN_TABLES or is a batch plan");
         b.p[BASE_PSI_IN] = const_cast<double*>(in);
@@ -1226,12 +1304,25 @@ int mq_hpsi_timed(mq_hpsi_plan_t p, const double* in, double* out, int precision
     return guarded(p->ctx, [&] {
         require(in && out && in != out && stage_ms, "mq_hpsi_timed: bad argument");
         hipStream_t s = p->ctx->stream;
-        hipEvent_t ev[5] = {};
+        const size_t nch = precision == MQ_PREC_FP64_PLAIN ? 1 : p->chunks.size();
+        std::vector<hipEvent_t> ev(2 + 3 * nch, nullptr);
         try {
             for (auto& e : ev) hipcheck(hipEventCreate(&e), "event create");
-            hpsi_enqueue(p, in, out, precision, ev);
-            hipcheck(hipEventSynchronize(ev[4]), "event sync");
-            for (int i = 0; i < 4; ++i) hipcheck(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]), "elapsed");
+            hpsi_enqueue(p, in, out, precision, ev.data());
+            hipcheck(hipEventSynchronize(ev.back()), "event sync");
+            // stage times summed over the chunks
+            auto el = [&](size_t a, size_t b) {
+                float ms = 0;
+                hipcheck(hipEventElapsedTime(&ms, ev[a], ev[b]), "elapsed");
+                return ms;
+            };
+            stage_ms[0] = el(0, 1);
+            stage_ms[1] = stage_ms[2] = stage_ms[3] = 0;
+            for (size_t c = 0; c < nch; ++c) {
+                stage_ms[1] += el(c == 0 ? 1 : 1 + 3 * c, 2 + 3 * c);
+                stage_ms[2] += el(2 + 3 * c, 3 + 3 * c);
+                stage_ms[3] += el(3 + 3 * c, 4 + 3 * c);
+            }
         } catch (...) {
             (void)hipStreamSynchronize(s);
             for (auto e : ev)
@@ -1269,9 +1360,12 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
     if (!p) return MQ_OK;
     int rc = guarded(p->ctx, [&] {
         (void)hipStreamSynchronize(p->ctx->stream);
-        p->tg.release();
-        p->cg.release();
-        p->sa.release();
+        for (auto& c : p->chunks) {
+            c.tg.release();
+            c.cg.release();
+            c.sa.release();
+        }
+        dev_free(p->gemm_ws);
         dev_free(p->T);
         dev_free(p->S);
         dev_free(p->Lpanel);

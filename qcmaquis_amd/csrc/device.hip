@@ -6,8 +6,9 @@
 //    prefetched into registers.  Each output tile is owned by one workgroup or, when K is
 //    split, by a fixed set of partial tiles summed in order by gemm_reduce_kernel: no atomics.
 //  * sblock_kernel: the MPO-term sum (reference create_s / create_s_r, solver/tasks.cpp:425-494)
-//    as a block-sparse product on the matrix cores: 16 output tiles ("lines") of equal shape at a
-//    time, Y = alpha * X over the union of the T tiles they read.
+//    as a block-sparse product on the matrix cores: up to 64 output tiles ("lines") of equal shape
+//    at a time, in groups of 16, Y_g = alpha_g * X over the union of the T tiles they read, each
+//    group only over the k-steps whose tiles it reads.
 //  * copy / reshape / fill kernels for panel set-up and MPS pairing changes.
 //  * FP64 Davidson vector kernels with device-resident coefficients.
 #include "device.hpp"
@@ -214,80 +215,138 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __rest
 }
 
 // Block-form MPO-term sum on the matrix cores.  A work item is SBLOCK_ELEMS consecutive tile
-// elements of one block of 16 lines; each of the SB_WAVES waves owns 16 * SB_NC of them and computes
-// Y(16 lines x 16 elements) += alpha(16 x 4 tiles) * X(4 tiles x 16 elements) per matrix-core step
-// over the block's union of T tiles.  Lane l loads alpha of line l % 16 and union tile l / 16, and
-// element l % 16 of that tile: 16 consecutive values per tile.  The union's tile offsets and
-// coefficients are staged in LDS, SB_UT_CHUNK tiles at a time: the tile loads do not wait on a
-// global offset load, and the four waves read the coefficients from LDS, not four times from L2.
+// elements of one block of up to SBLOCK_GROUPS groups of 16 lines; each of the SB_WAVES waves owns
+// 16 * SB_NC of them.  Per k-step (4 union tiles) every group that reads one of its tiles computes
+// Y_g(16 lines x 16 elements) += alpha_g(16 x 4) * X(4 tiles x 16 elements) on the matrix cores:
+// lane l loads element l % 16 of tile l / 16 (16 consecutive values per tile, shared by the
+// groups) and alpha of line l % 16 for tile l / 16.  The tile offsets, k-step words and
+// coefficient sub-blocks are staged in LDS, SB_KCHUNK k-steps at a time; the next chunk's tables
+// are loaded into registers while the current chunk is summed, and the tiles of the next
+// SB_UNROLL k-steps are loaded while the current ones go through the matrix cores.  The k-step's
+// group mask is read into a scalar register, so the groups without a sub-block are skipped by the
+// whole wave.
 constexpr int SB_NC = 2;                                // 16-element column tiles per wave
 constexpr int SB_WAVES = SBLOCK_ELEMS / (16 * SB_NC);   // waves per workgroup
-constexpr int SB_UNROLL = 4;
-constexpr int SB_UT_CHUNK = 64;
-static_assert(SBLOCK_LINES == 16 && SB_WAVES * 16 * SB_NC == SBLOCK_ELEMS && SB_WAVES <= 16, "S block geometry");
-static_assert(SB_UT_CHUNK % (4 * SB_UNROLL) == 0, "offset chunks hold whole unrolled k-steps");
+constexpr int SB_THREADS = 64 * SB_WAVES;
+constexpr int SB_UNROLL = 2;                            // k-steps whose loads are issued together
+constexpr int SB_KCHUNK = 8;                            // k-steps staged in LDS at a time
+constexpr int SB_ST_ALPHA = SB_KCHUNK * SBLOCK_GROUPS * 64 / SB_THREADS;  // staged values per thread
+static_assert(SBLOCK_GROUP == 16 && SB_WAVES * 16 * SB_NC == SBLOCK_ELEMS && SB_WAVES <= 16, "S block geometry");
+static_assert(SB_KCHUNK % (2 * SB_UNROLL) == 0, "staged chunks hold whole double-buffered k-step groups");
+static_assert(4 * SB_KCHUNK <= SB_THREADS && SB_ST_ALPHA * SB_THREADS == SB_KCHUNK * SBLOCK_GROUPS * 64,
+              "one staging pass covers a chunk's tables");
 
 template <class T>
-__global__ __launch_bounds__(64 * SB_WAVES) void sblock_kernel(const SBlock* __restrict__ blocks, const SWork* __restrict__ work,
-                                                     const int64_t* __restrict__ utile, const T* __restrict__ ualpha,
-                                                     const int64_t* __restrict__ lsoff,
+__global__ __launch_bounds__(SB_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void sblock_kernel(const SBlock* __restrict__ blocks, const SWork* __restrict__ work,
+                                                     const int64_t* __restrict__ utile, const uint32_t* __restrict__ kstep,
+                                                     const T* __restrict__ ualpha, const int64_t* __restrict__ lsoff,
                                                      const int32_t* __restrict__ llds, const T* __restrict__ X,
                                                      T* __restrict__ S) {
     typedef typename Mma<T>::acc_t acc_t;
-    __shared__ int64_t uts[SB_UT_CHUNK];
-    __shared__ T uas[SB_UT_CHUNK * SBLOCK_LINES];
+    __shared__ int64_t uts[4 * SB_KCHUNK];
+    __shared__ uint32_t kws[SB_KCHUNK];
+    __shared__ T uas[SB_KCHUNK * SBLOCK_GROUPS * 64];
     const SWork w = work[blockIdx.x];
     const SBlock b = blocks[w.blk];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int n0 = w.n0 + wave * 16 * SB_NC;
     // a wave past the tile end loads and stores nothing but takes part in the barriers
     const bool active = n0 < b.E;
     const int li = lane & 15, lk = lane >> 4;
-    acc_t acc[SB_NC];
+    acc_t acc[SBLOCK_GROUPS][SB_NC];
 #pragma unroll
-    for (int j = 0; j < SB_NC; ++j) acc[j] = acc_t{0, 0, 0, 0};
-    const int64_t* ut = utile + b.u_off;
-    const T* ua = ualpha + b.u_off * SBLOCK_LINES;
-    for (int c0 = 0; c0 < b.nu; c0 += SB_UT_CHUNK) {
-        const int cn = b.nu - c0 < SB_UT_CHUNK ? b.nu - c0 : SB_UT_CHUNK;
-        __syncthreads();  // the previous chunk's offsets are consumed
-        for (int i = threadIdx.x; i < cn; i += blockDim.x) uts[i] = ut[c0 + i];
-        for (int i = threadIdx.x; i < cn * SBLOCK_LINES; i += blockDim.x) uas[i] = ua[int64_t(c0) * SBLOCK_LINES + i];
-        __syncthreads();
-        if (!active) continue;
-        // SB_UNROLL k-steps of loads are issued before their matrix-core steps
-        for (int ks = 0; ks < cn; ks += 4 * SB_UNROLL) {
-            T a[SB_UNROLL], x[SB_UNROLL][SB_NC];
+    for (int g = 0; g < SBLOCK_GROUPS; ++g)
 #pragma unroll
-            for (int u = 0; u < SB_UNROLL; ++u) {
-                const int k = ks + 4 * u + lk;
-                const bool kv = k < cn;
-                a[u] = kv ? uas[k * SBLOCK_LINES + li] : T(0);  // A[line li][tile k]
-                const int64_t to = kv ? uts[k] : 0;
+        for (int j = 0; j < SB_NC; ++j) acc[g][j] = acc_t{0, 0, 0, 0};
+    const int64_t* ut = utile + 4 * int64_t(b.k_off);
+    const uint32_t* kw = kstep + b.k_off;
+    const T* ua = ualpha + 64 * b.a_off;
+
+    // the tables of the chunk from k-step c0 in registers: this thread's slot offset, k-step word
+    // and coefficient values; ra0 / ra1 = the chunk's sub-block range
+    int64_t r_ut = -1;
+    uint32_t r_kw = 0;
+    T r_a[SB_ST_ALPHA];
+    int ra0 = 0, ra1 = 0;
+    auto fetch = [&](int c0) {
+        const int cn = b.nk - c0 < SB_KCHUNK ? b.nk - c0 : SB_KCHUNK;
+        ra0 = int(kw[c0] >> 8);
+        ra1 = c0 + cn < b.nk ? int(kw[c0 + cn] >> 8) : b.na;
+        if (tid < 4 * cn) r_ut = ut[4 * c0 + tid];
+        if (tid < cn) r_kw = kw[c0 + tid];
 #pragma unroll
-                for (int j = 0; j < SB_NC; ++j) {
-                    const int e = n0 + 16 * j + li;
-                    x[u][j] = (kv && e < b.E) ? X[to + e] : T(0);  // B[tile k][element e]
-                }
+        for (int q = 0; q < SB_ST_ALPHA; ++q) {
+            const int i = tid + SB_THREADS * q;
+            r_a[q] = i < 64 * (ra1 - ra0) ? ua[64 * int64_t(ra0) + i] : T(0);
+        }
+    };
+    // X of the k-steps [k, k + SB_UNROLL) of the staged chunk (cn k-steps)
+    auto load_x = [&](T (&x)[SB_UNROLL][SB_NC], int k, int cn) {
+#pragma unroll
+        for (int u = 0; u < SB_UNROLL; ++u) {
+            const int64_t to = k + u < cn ? uts[4 * (k + u) + lk] : -1;
+#pragma unroll
+            for (int j = 0; j < SB_NC; ++j) {
+                const int e = n0 + 16 * j + li;
+                x[u][j] = (to >= 0 && e < b.E) ? X[to + e] : T(0);  // B[tile lk][element e]
             }
+        }
+    };
+    auto mma = [&](T (&x)[SB_UNROLL][SB_NC], int k, int cn, int a0) {
 #pragma unroll
-            for (int u = 0; u < SB_UNROLL; ++u)
+        for (int u = 0; u < SB_UNROLL; ++u) {
+            if (k + u >= cn) break;
+            const uint32_t word = __builtin_amdgcn_readfirstlane(kws[k + u]);
+            int ai = int(word >> 8) - a0;
 #pragma unroll
-                for (int j = 0; j < SB_NC; ++j) acc[j] = Mma<T>::step(a[u], x[u][j], acc[j]);
+            for (int g = 0; g < SBLOCK_GROUPS; ++g) {
+                if (!((word >> g) & 1u)) continue;
+                const T a = uas[64 * ai + lane];  // A[line li][tile lk]
+                ++ai;
+#pragma unroll
+                for (int j = 0; j < SB_NC; ++j) acc[g][j] = Mma<T>::step(a, x[u][j], acc[g][j]);
+            }
+        }
+    };
+
+    if (b.nk > 0) fetch(0);
+    for (int c0 = 0; c0 < b.nk; c0 += SB_KCHUNK) {
+        const int cn = b.nk - c0 < SB_KCHUNK ? b.nk - c0 : SB_KCHUNK;
+        const int a0 = ra0;
+        __syncthreads();  // the previous chunk's tables are consumed
+        if (tid < 4 * cn) uts[tid] = r_ut;
+        if (tid < cn) kws[tid] = r_kw;
+#pragma unroll
+        for (int q = 0; q < SB_ST_ALPHA; ++q) {
+            const int i = tid + SB_THREADS * q;
+            if (i < 64 * (ra1 - ra0)) uas[i] = r_a[q];
+        }
+        __syncthreads();
+        if (c0 + SB_KCHUNK < b.nk) fetch(c0 + SB_KCHUNK);
+        if (!active) continue;
+        T xa[SB_UNROLL][SB_NC], xb[SB_UNROLL][SB_NC];
+        load_x(xa, 0, cn);
+        for (int ks = 0; ks < cn; ks += 2 * SB_UNROLL) {
+            load_x(xb, ks + SB_UNROLL, cn);
+            mma(xa, ks, cn, a0);
+            load_x(xa, ks + 2 * SB_UNROLL, cn);
+            mma(xb, ks + SB_UNROLL, cn, a0);
         }
     }
     if (!active) return;
-    // register r of lane l holds Y[line row(l, r)][element l % 16]
+    // register r of lane l holds Y_g[line row(l, r)][element l % 16]
 #pragma unroll
     for (int j = 0; j < SB_NC; ++j) {
         const int e = n0 + 16 * j + li;
         if (e >= b.E) continue;
         const int r_ = e % b.rows, c_ = e / b.rows;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = Mma<T>::row(lane, r);
-            if (i < b.nlines) S[lsoff[b.line0 + i] + r_ + int64_t(c_) * llds[b.line0 + i]] = acc[j][r];
-        }
+        for (int g = 0; g < SBLOCK_GROUPS; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = SBLOCK_GROUP * g + Mma<T>::row(lane, r);
+                if (i < b.nlines) S[lsoff[b.line0 + i] + r_ + int64_t(c_) * llds[b.line0 + i]] = acc[g][j][r];
+            }
     }
 }
 
@@ -538,19 +597,21 @@ hipError_t launch_slines(const SLine* lines, int64_t n_lines, const STerm* terms
 
 template <class T>
 hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_work, const int64_t* utile,
-                          const T* ualpha, const int64_t* lsoff, const int32_t* llds, const T* X, T* S,
-                          hipStream_t stream) {
+                          const uint32_t* kstep, const T* ualpha, const int64_t* lsoff, const int32_t* llds,
+                          const T* X, T* S, hipStream_t stream) {
     for (int64_t off = 0; off < n_work; off += 0x7fffffff) {
         int64_t cnt = n_work - off < 0x7fffffff ? n_work - off : 0x7fffffff;
-        hipLaunchKernelGGL(sblock_kernel<T>, dim3(unsigned(cnt)), dim3(64 * SB_WAVES), 0, stream, blocks, work + off, utile,
-                           ualpha, lsoff, llds, X, S);
+        hipLaunchKernelGGL(sblock_kernel<T>, dim3(unsigned(cnt)), dim3(SB_THREADS), 0, stream, blocks, work + off, utile,
+                           kstep, ualpha, lsoff, llds, X, S);
     }
     return hipGetLastError();
 }
-template hipError_t launch_sblocks<double>(const SBlock*, const SWork*, int64_t, const int64_t*, const double*,
-                                           const int64_t*, const int32_t*, const double*, double*, hipStream_t);
-template hipError_t launch_sblocks<float>(const SBlock*, const SWork*, int64_t, const int64_t*, const float*,
-                                          const int64_t*, const int32_t*, const float*, float*, hipStream_t);
+template hipError_t launch_sblocks<double>(const SBlock*, const SWork*, int64_t, const int64_t*, const uint32_t*,
+                                           const double*, const int64_t*, const int32_t*, const double*, double*,
+                                           hipStream_t);
+template hipError_t launch_sblocks<float>(const SBlock*, const SWork*, int64_t, const int64_t*, const uint32_t*,
+                                          const float*, const int64_t*, const int32_t*, const float*, float*,
+                                          hipStream_t);
 
 template <class T>
 hipError_t launch_small_lines(const SSmallWork* work, int64_t n_work, const SLine* lines, const STerm* terms,

@@ -30,7 +30,8 @@ hipError_t launch_gemm(GemmPrec prec, const GemmDesc* probs, const GemmRow* rows
 // of the table's lines
 template <class T>
 hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_work, const int64_t* utile,
-                          const T* ualpha, const int64_t* lsoff, const int32_t* llds, const T* X, T* S,
+                          const uint32_t* kstep, const T* ualpha, const int64_t* lsoff, const int32_t* llds,
+                          const T* X, T* S,
                           hipStream_t stream);
 // small-tile MPO-term sum (SBlockTable small_* tables uploaded; T and S of type T, coefficients
 // FP64, rounded to T): writes the S tiles of the small lines

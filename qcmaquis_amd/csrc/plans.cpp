@@ -19,6 +19,7 @@ namespace {
 struct CohortAcc {
     struct Unit {
         uint32_t offset = 0, ms = 0;
+        uint32_t s = UINT32_MAX;  // physical sector
         std::vector<double> alpha;
         std::vector<uint32_t> ti, col;
         std::vector<uint32_t> line_count, line_b;
@@ -42,6 +43,7 @@ struct CohortAcc {
         for (uint32_t i = 0; i < d; ++i) {
             units[sfold[s] + i].offset = uint32_t(i * ms + offset);
             units[sfold[s] + i].ms = ms;
+            units[sfold[s] + i].s = uint32_t(s);
             stripe += ms;
         }
     }
@@ -167,15 +169,20 @@ struct TableSink {
 
 // S lines and terms of all cohorts, written in parallel straight into the tables in cohort order:
 // the lines and terms of every cohort are counted first (units_of(c): its units, nrows_of(c): its
-// MPO rows), then emit(c, lines, terms) appends cohort c's lines (term ranges absolute) and terms
-template <class Units, class Rows, class Emit>
+// MPO rows, keep(c, unit): the units taken), then emit(c, lines, terms) appends cohort c's lines
+// (term ranges absolute) and terms of the units taken
+struct KeepAll {
+    bool operator()(size_t, CohortAcc::Unit const&) const { return true; }
+};
+template <class Units, class Rows, class Emit, class Keep = KeepAll>
 void emit_lines(size_t n_cohorts, HostTable<SLine>& lines, HostTable<STerm>& terms, Units units_of, Rows nrows_of,
-                Emit emit) {
+                Emit emit, Keep keep = Keep()) {
     std::vector<size_t> nl(n_cohorts), nt(n_cohorts);
 #pragma omp parallel for schedule(dynamic, 4)
     for (int64_t c = 0; c < int64_t(n_cohorts); ++c) {
         size_t a = 0, b = 0;
         for (auto const& u : units_of(size_t(c))) {
+            if (!keep(size_t(c), u)) continue;
             a += unit_lines(u, nrows_of(size_t(c)));
             b += u.ti.size();
         }
@@ -319,7 +326,7 @@ void validate_boundary_offsets(BoundaryIndex const& B) {
 //         site_hamil_schedule.hpp:40-93, right_tasks.hpp:39-78, site_hamil_tasks.hpp:41-137)
 // ----------------------------------------------------------------------------------------------
 HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, BoundaryIndex const& R,
-                             Index const& left_i, Index const& phys_i, Index const& right_i) {
+                             Index const& left_i, Index const& phys_i, Index const& right_i, uint64_t chunk_elems) {
     HpsiPlanHost P;
     P.mps = MPSLayout(left_i, phys_i, right_i);
     require(L.aux_dim == mpo.row_dim, "hpsi: left boundary aux dim != MPO row dim");
@@ -463,9 +470,96 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
     for (auto& r : recs)
         if (r) cohorts.push_back(std::move(*r));
 
-    // --- T buffer layout (only tasks referenced by some alpha are executed) ---
-    RightTStacks tstacks;
-    for (uint32_t lb_in = 0; lb_in < left_i.size(); ++lb_in) {
+    // --- S panels: per output block the cohorts stacked along K (the same in every chunk) ---
+    std::vector<uint64_t> ktot(NL, 0);
+    std::vector<uint64_t> koff(cohorts.size());
+    for (size_t c = 0; c < cohorts.size(); ++c) {
+        koff[c] = ktot[cohorts[c].lb_out];
+        ktot[cohorts[c].lb_out] += uint64_t(cohorts[c].nb) * left_i.sizes[cohorts[c].lb_in];
+    }
+    // column stripes of every output block: one per physical sector with a right sector, at the
+    // stripe's column offset of the psi block; stripes of one block in column order
+    struct Stripe { uint32_t s, rb; uint64_t col, width; };
+    std::vector<std::vector<Stripe>> stripes(NL);
+    for (size_t lb = 0; lb < NL; ++lb) {
+        for (size_t s = 0; s < NP; ++s) {
+            size_t rb = pos_lp[lb * NP + s];
+            if (rb == NR) continue;
+            stripes[lb].push_back(Stripe{uint32_t(s), uint32_t(rb), pb_off[s * NR + rb],
+                                         uint64_t(phys_i.sizes[s]) * right_i.sizes[rb]});
+        }
+        std::sort(stripes[lb].begin(), stripes[lb].end(), [](Stripe const& a, Stripe const& b) { return a.col < b.col; });
+    }
+    // used T tasks: their output right sector (the ket sector of their right cohort)
+    auto task_rb = [&](TTask const& t) {
+        auto lr = R.clr[t.ci];
+        size_t rb = right_i.position(R.ket.charges[lr.second]);
+        require(rb < NR, "hpsi: right boundary index does not match the MPS right index");
+        return rb;
+    };
+    // --- chunks: right sectors gathered in index order up to chunk_elems of T + S ---
+    std::vector<uint64_t> t_rb(NR, 0), s_rb(NR, 0);
+    for (uint32_t lb_in = 0; lb_in < NL; ++lb_in)
+        for (auto const& t : tlists[lb_in].tasks)
+            if (t.used) t_rb[task_rb(t)] += round_up(uint64_t(left_i.sizes[lb_in]) * R.nblk[t.ci_eff] *
+                                                         R.ket.sizes[R.clr[t.ci].second], 32);
+    for (size_t lb = 0; lb < NL; ++lb)
+        for (auto const& st : stripes[lb]) s_rb[st.rb] += ktot[lb] * st.width;
+    std::vector<uint32_t> chunk_of(NR, 0);
+    uint32_t n_chunks = 1;
+    if (chunk_elems) {
+        uint64_t acc = 0;
+        bool any = false;
+        for (size_t rb = 0; rb < NR; ++rb) {
+            const uint64_t need = t_rb[rb] + s_rb[rb];
+            if (any && acc + need > chunk_elems) {
+                ++n_chunks;
+                acc = 0;
+            }
+            chunk_of[rb] = n_chunks - 1;
+            acc += need;
+            any = true;
+        }
+    }
+    P.chunks.resize(n_chunks);
+    // chunk-local S panel of every (chunk, output block): the block's stripes of the chunk side by
+    // side; loc[lb][i] = local column of stripe i
+    std::vector<std::vector<uint64_t>> s_off(n_chunks, std::vector<uint64_t>(NL, 0));
+    std::vector<std::vector<uint64_t>> loc(NL);
+    std::vector<uint64_t> l_off(NL);
+    for (size_t lb = 0; lb < NL; ++lb) {
+        l_off[lb] = P.lpanel_elems;
+        P.lpanel_elems += round_up(uint64_t(left_i.sizes[lb]) * ktot[lb], 32);
+        loc[lb].resize(stripes[lb].size());
+    }
+    for (uint32_t ch = 0; ch < n_chunks; ++ch) {
+        uint64_t& se = P.chunks[ch].s_elems;
+        for (size_t lb = 0; lb < NL; ++lb) {
+            s_off[ch][lb] = se;
+            uint64_t w = 0;
+            for (size_t i = 0; i < stripes[lb].size(); ++i)
+                if (chunk_of[stripes[lb][i].rb] == ch) {
+                    loc[lb][i] = w;
+                    w += stripes[lb][i].width;
+                }
+            se += round_up(ktot[lb] * w, 32);
+        }
+    }
+    // local column of a unit's stripe in its chunk's S panel
+    auto unit_col = [&](uint32_t lb_out, CohortAcc::Unit const& u, uint32_t& ch) -> uint64_t {
+        const size_t rb = pos_lp[size_t(lb_out) * NP + u.s];
+        auto const& v = stripes[lb_out];
+        for (size_t i = 0; i < v.size(); ++i)
+            if (v[i].s == u.s) {
+                ch = chunk_of[rb];
+                return loc[lb_out][i] + (u.offset - v[i].col);
+            }
+        throw Error(MQ_ERR_INVALID, "hpsi: unit outside the stripes of its output block");
+    };
+
+    // --- T buffer layout per chunk (only tasks referenced by some alpha are executed) ---
+    std::vector<RightTStacks> tstacks(n_chunks);
+    for (uint32_t lb_in = 0; lb_in < NL; ++lb_in) {
         uint32_t ls_in = left_i.sizes[lb_in];
         for (auto& t : tlists[lb_in].tasks) {
             uint64_t csz = R.cohort_size(t.ci_eff);
@@ -479,35 +573,28 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
             size_t rb = right_i.position(R.bra.charges[lr.first]);
             require(rb < right_i.size() && right_i.sizes[rb] == rs_ket,
                     "hpsi: right boundary index does not match the MPS right index");
+            const uint32_t ch = chunk_of[task_rb(t)];
             t.rows = ls_in;
             t.size = uint64_t(ls_in) * R.nblk[t.ci_eff] * rs_bra;
-            t.t_off = P.t_elems;
-            P.t_elems += round_up(t.size, 32);
+            t.t_off = P.chunks[ch].t_elems;
+            P.chunks[ch].t_elems += round_up(t.size, 32);
             r_used.insert(t.ci_eff);
-            tstacks.add(t, ls_in, rs_ket, rs_bra, int64_t(P.mps.rp_off[lb_in]));
+            tstacks[ch].add(t, ls_in, rs_ket, rs_bra, int64_t(P.mps.rp_off[lb_in]));
             P.flops.flops_gemm_exec += 2ull * ls_in * csz;
         }
     }
-    tstacks.emit(R, BASE_PSI_IN, P.t_gemms, P.t_rows, &P.t_gemms_task);
+    for (uint32_t ch = 0; ch < n_chunks; ++ch) {
+        P.chunks[ch].t_gemm0 = P.t_gemms.size();
+        P.chunks[ch].t_row0 = P.t_rows.size();
+        tstacks[ch].emit(R, BASE_PSI_IN, P.t_gemms, P.t_rows, n_chunks == 1 ? &P.t_gemms_task : nullptr);
+        P.chunks[ch].t_gemm1 = P.t_gemms.size();
+        P.chunks[ch].t_row1 = P.t_rows.size();
+    }
 
-    // --- S panels / L panels per output block: cohorts of one lb_out stacked along K ---
-    std::vector<uint64_t> ktot(left_i.size(), 0);
-    std::vector<uint64_t> koff(cohorts.size());
-    for (size_t c = 0; c < cohorts.size(); ++c) {
-        koff[c] = ktot[cohorts[c].lb_out];
-        ktot[cohorts[c].lb_out] += uint64_t(cohorts[c].nb) * left_i.sizes[cohorts[c].lb_in];
-    }
-    std::vector<uint64_t> s_off(left_i.size()), l_off(left_i.size());
-    for (size_t lb = 0; lb < left_i.size(); ++lb) {
-        s_off[lb] = P.s_elems;
-        P.s_elems += round_up(ktot[lb] * P.mps.rp_cols[lb], 32);
-        l_off[lb] = P.lpanel_elems;
-        P.lpanel_elems += round_up(uint64_t(left_i.sizes[lb]) * ktot[lb], 32);
-    }
+    // --- L panels and flop counts per cohort ---
     for (size_t c = 0; c < cohorts.size(); ++c) {
         auto& rec = cohorts[c];
         uint32_t ls_in = left_i.sizes[rec.lb_in], ls_out = left_i.sizes[rec.lb_out];
-        uint64_t ldS = ktot[rec.lb_out];
         require(rec.acc.stripe == P.mps.rp_cols[rec.lb_out], "hpsi: cohort stripe != psi block width");
         // L panel copy job
         CopyJob job{};
@@ -526,7 +613,7 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
             job.nblocks = int32_t(rec.nb);
             job.trans = 1;
         }
-        if (uint64_t(rec.nb) * ls_in * ls_out) P.lpanel_jobs.push_back(job);
+        if (uint64_t(rec.nb) * ls_in * ls_out != 0) P.lpanel_jobs.push_back(job);
         l_used.insert(rec.ci_eff);
         for (auto& u : rec.acc.units) P.flops.flops_total += 2ull * ls_in * u.ms * u.alpha.size();
         uint64_t g = 2ull * ls_out * ls_in * rec.nb * rec.acc.stripe;
@@ -535,57 +622,105 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
         P.flops.flops_gemm_exec += g;
         P.flops.n_terms += rec.acc.n_terms;
     }
-    // S lines
-    emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c) -> auto const& { return cohorts[c].acc.units; },
-               [&](size_t c) { return cohorts[c].nb; }, [&](size_t c, auto& lines, auto& terms) {
-        auto const& rec = cohorts[c];
-        uint32_t ls_in = left_i.sizes[rec.lb_in];
-        uint64_t ldS = ktot[rec.lb_out];
-        for (auto const& u : rec.acc.units) {
-            uint32_t seek = 0;
-            for (size_t l = 0; l < u.line_count.size(); ++l) {
-                SLine line{};
-                line.s_off = int64_t(s_off[rec.lb_out] + koff[c] + uint64_t(u.line_b[l]) * ls_in + uint64_t(u.offset) * ldS);
-                line.lds = int32_t(ldS);
-                line.rows = int32_t(ls_in);
-                line.cols = int32_t(u.ms);
-                line.t0 = int32_t(terms.size());
-                for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
-                    TTask const& t = tlists[rec.lb_in].tasks[u.ti[k]];
-                    check_tile(t, uint64_t(u.col[k]) * ls_in, ls_in, u.ms);
-                    terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * ls_in)});
+    // --- S lines, chunk by chunk ---
+    for (uint32_t ch = 0; ch < n_chunks; ++ch) {
+        P.chunks[ch].line0 = P.lines.size();
+        emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c) -> auto const& { return cohorts[c].acc.units; },
+                   [&](size_t c) { return cohorts[c].nb; }, [&](size_t c, auto& lines, auto& terms) {
+            auto const& rec = cohorts[c];
+            uint32_t ls_in = left_i.sizes[rec.lb_in];
+            uint64_t ldS = ktot[rec.lb_out];
+            for (auto const& u : rec.acc.units) {
+                if (!u.ms) continue;
+                uint32_t uch = 0;
+                const uint64_t col = unit_col(rec.lb_out, u, uch);
+                if (uch != ch) continue;
+                const uint64_t base = s_off[ch][rec.lb_out] + koff[c] + col * ldS;
+                uint32_t seek = 0;
+                for (size_t l = 0; l < u.line_count.size(); ++l) {
+                    SLine line{};
+                    line.s_off = int64_t(base + uint64_t(u.line_b[l]) * ls_in);
+                    line.lds = int32_t(ldS);
+                    line.rows = int32_t(ls_in);
+                    line.cols = int32_t(u.ms);
+                    line.t0 = int32_t(terms.size());
+                    for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
+                        TTask const& t = tlists[rec.lb_in].tasks[u.ti[k]];
+                        check_tile(t, uint64_t(u.col[k]) * ls_in, ls_in, u.ms);
+                        terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * ls_in)});
+                    }
+                    line.t1 = int32_t(terms.size());
+                    seek += u.line_count[l];
+                    lines.push_back(line);
                 }
-                line.t1 = int32_t(terms.size());
-                seek += u.line_count[l];
-                lines.push_back(line);
+                add_empty_lines(u, rec.nb, int32_t(terms.size()), lines, [&](uint32_t b) {
+                    SLine line{};
+                    line.s_off = int64_t(base + uint64_t(b) * ls_in);
+                    line.lds = int32_t(ldS);
+                    line.rows = int32_t(ls_in);
+                    line.cols = int32_t(u.ms);
+                    return line;
+                });
             }
-            add_empty_lines(u, rec.nb, int32_t(terms.size()), lines, [&](uint32_t b) {
-                SLine line{};
-                line.s_off = int64_t(s_off[rec.lb_out] + koff[c] + uint64_t(b) * ls_in + uint64_t(u.offset) * ldS);
-                line.lds = int32_t(ldS);
-                line.rows = int32_t(ls_in);
-                line.cols = int32_t(u.ms);
-                return line;
-            });
-        }
-    });
+        }, [&](size_t c, CohortAcc::Unit const& u) {
+            // the units of the chunk's right sectors (a unit without columns has no terms)
+            if (!u.ms) return false;
+            uint32_t uch = 0;
+            unit_col(cohorts[c].lb_out, u, uch);
+            return uch == ch;
+        });
+        P.chunks[ch].line1 = P.lines.size();
+    }
     P.n_cohorts = cohorts.size();
-    for (uint32_t lb = 0; lb < left_i.size(); ++lb) {
-        uint32_t ls = left_i.sizes[lb];
-        P.c_gemms.push_back(gemm(int32_t(ls), int32_t(P.mps.rp_cols[lb]), int32_t(ktot[lb]), BASE_LPANEL,
-                                 int64_t(l_off[lb]), int32_t(ls), BASE_S, int64_t(s_off[lb]), int32_t(ktot[lb]),
-                                 BASE_PSI_OUT, int64_t(P.mps.rp_off[lb]), int32_t(ls)));
+    // L*S products: per chunk and output block, one per run of the chunk's stripes that are
+    // adjacent both in the psi block and in the chunk's S panel (one per block with one chunk)
+    for (uint32_t ch = 0; ch < n_chunks; ++ch) {
+        P.chunks[ch].c_gemm0 = P.c_gemms.size();
+        for (uint32_t lb = 0; lb < NL; ++lb) {
+            uint32_t ls = left_i.sizes[lb];
+            auto const& v = stripes[lb];
+            for (size_t i = 0; i < v.size();) {
+                if (chunk_of[v[i].rb] != ch) {
+                    ++i;
+                    continue;
+                }
+                size_t j = i + 1;
+                uint64_t w = v[i].width;
+                while (j < v.size() && chunk_of[v[j].rb] == ch && v[j].col == v[i].col + w && loc[lb][j] == loc[lb][i] + w) {
+                    w += v[j].width;
+                    ++j;
+                }
+                P.c_gemms.push_back(gemm(int32_t(ls), int32_t(w), int32_t(ktot[lb]), BASE_LPANEL, int64_t(l_off[lb]),
+                                         int32_t(ls), BASE_S, int64_t(s_off[ch][lb] + loc[lb][i] * ktot[lb]),
+                                         int32_t(ktot[lb]), BASE_PSI_OUT,
+                                         int64_t(P.mps.rp_off[lb] + v[i].col * ls), int32_t(ls)));
+                i = j;
+            }
+        }
+        P.chunks[ch].c_gemm1 = P.c_gemms.size();
     }
     // the reference's L*S products, one per cohort, each added into its output block (Cohort::contract,
     // tasks.cpp:286-344): the K range of the cohort in its block's stacked L panel and S panel
-    for (size_t c = 0; c < cohorts.size(); ++c) {
-        auto const& rec = cohorts[c];
-        uint32_t ls_out = left_i.sizes[rec.lb_out];
-        int32_t kc = int32_t(uint64_t(rec.nb) * left_i.sizes[rec.lb_in]);
-        P.c_gemms_cohort.push_back(gemm(int32_t(ls_out), int32_t(P.mps.rp_cols[rec.lb_out]), kc, BASE_LPANEL,
-                                        int64_t(l_off[rec.lb_out] + uint64_t(ls_out) * koff[c]), int32_t(ls_out),
-                                        BASE_S, int64_t(s_off[rec.lb_out] + koff[c]), int32_t(ktot[rec.lb_out]),
-                                        BASE_PSI_OUT, int64_t(P.mps.rp_off[rec.lb_out]), int32_t(ls_out)));
+    if (n_chunks == 1) {
+        // one chunk: the S panel columns are the psi block's columns
+        for (size_t lb = 0; lb < NL; ++lb)
+            for (size_t i = 0; i < stripes[lb].size(); ++i)
+                require(loc[lb][i] == stripes[lb][i].col, "hpsi: psi block columns are not its stripes side by side");
+        for (size_t c = 0; c < cohorts.size(); ++c) {
+            auto const& rec = cohorts[c];
+            uint32_t ls_out = left_i.sizes[rec.lb_out];
+            int32_t kc = int32_t(uint64_t(rec.nb) * left_i.sizes[rec.lb_in]);
+            P.c_gemms_cohort.push_back(gemm(int32_t(ls_out), int32_t(P.mps.rp_cols[rec.lb_out]), kc, BASE_LPANEL,
+                                            int64_t(l_off[rec.lb_out] + uint64_t(ls_out) * koff[c]), int32_t(ls_out),
+                                            BASE_S, int64_t(s_off[0][rec.lb_out] + koff[c]), int32_t(ktot[rec.lb_out]),
+                                            BASE_PSI_OUT, int64_t(P.mps.rp_off[rec.lb_out]), int32_t(ls_out)));
+        }
+    }
+    for (auto const& ch : P.chunks) {
+        P.t_elems += ch.t_elems;
+        P.s_elems += ch.s_elems;
+        P.t_buf = std::max(P.t_buf, ch.t_elems);
+        P.s_buf = std::max(P.s_buf, ch.s_elems);
     }
     uint64_t bytes = 0;
     for (auto ci : l_used) bytes += L.cohort_size(ci);
@@ -1256,18 +1391,26 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
         for (uint32_t s0 = 0; s0 < v->size(); s0 += SBLOCK_LINES)
             spans.push_back(Span{v, s0, std::min<uint32_t>(SBLOCK_LINES, uint32_t(v->size()) - s0)});
     const int64_t nb = int64_t(spans.size());
-    std::vector<std::vector<int64_t>> unions(spans.size());
-    std::vector<std::vector<double>> alphas(spans.size());
+    struct BlockTabs {
+        std::vector<int64_t> slots;   // 4 per k-step
+        std::vector<uint32_t> words;  // per k-step
+        std::vector<double> alpha;    // sub-blocks
+        uint32_t nu = 0;
+    };
+    std::vector<BlockTabs> tabs(spans.size());
 #pragma omp parallel
     {
         // union of one block's T tiles, found with an open-addressing table in order of first use
-        // (lines in block order, each line's terms in term order), then sorted by T offset; each
-        // coefficient is summed in the line's term order
+        // (lines in block order, each line's terms in term order) with the groups reading each tile
+        // and each line's coefficient (summed in the line's term order); then ordered by
+        // (groups, T offset) and cut into k-steps
         std::vector<int64_t> key;
         std::vector<int32_t> slot;
-        std::vector<std::pair<int64_t, int32_t>> order;
-        std::vector<double> sal;
-#pragma omp for schedule(dynamic, 64)
+        std::vector<int64_t> u;
+        std::vector<uint32_t> pat;
+        std::vector<double> al;
+        std::vector<std::pair<std::pair<uint32_t, int64_t>, int32_t>> order;
+#pragma omp for schedule(dynamic, 16)
         for (int64_t bi = 0; bi < nb; ++bi) {
             Span const& sp = spans[size_t(bi)];
             size_t nref = 0;
@@ -1279,8 +1422,9 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
             while (cap < 2 * nref) cap *= 2;
             key.assign(cap, -1);
             slot.resize(cap);
-            std::vector<int64_t>& u = unions[size_t(bi)];
-            std::vector<double>& al = alphas[size_t(bi)];
+            u.clear();
+            pat.clear();
+            al.clear();
             for (uint32_t q = 0; q < sp.n; ++q) {
                 SLine const& l = lines[(*sp.v)[sp.s0 + q]];
                 for (int32_t k = l.t0; k < l.t1; ++k) {
@@ -1291,30 +1435,55 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
                         key[h] = t;
                         slot[h] = int32_t(u.size());
                         u.push_back(t);
+                        pat.push_back(0);
                         al.resize(al.size() + SBLOCK_LINES, 0.0);
                     }
+                    pat[size_t(slot[h])] |= 1u << (q / SBLOCK_GROUP);
                     al[size_t(slot[h]) * SBLOCK_LINES + q] += terms[size_t(k)].alpha;
                 }
             }
-            // union tiles in T order (the term-sum kernel then reads T in ascending address order)
-            if (std::is_sorted(u.begin(), u.end())) continue;
             order.resize(u.size());
-            for (size_t i = 0; i < u.size(); ++i) order[i] = std::make_pair(u[i], int32_t(i));
+            for (size_t i = 0; i < u.size(); ++i) order[i] = std::make_pair(std::make_pair(pat[i], u[i]), int32_t(i));
             std::sort(order.begin(), order.end());
-            sal.resize(al.size());
-            for (size_t i = 0; i < u.size(); ++i) {
-                u[i] = order[i].first;
-                std::copy_n(al.begin() + int64_t(order[i].second) * SBLOCK_LINES, SBLOCK_LINES,
-                            sal.begin() + int64_t(i) * SBLOCK_LINES);
+            BlockTabs& bt = tabs[size_t(bi)];
+            bt.nu = uint32_t(u.size());
+            const size_t nk = (u.size() + 3) / 4;
+            bt.slots.assign(4 * nk, -1);
+            bt.words.resize(nk);
+            uint32_t na = 0;
+            for (size_t k = 0; k < nk; ++k) {
+                uint32_t m = 0;
+                for (size_t t = 4 * k; t < std::min(u.size(), 4 * k + 4); ++t) {
+                    bt.slots[t] = order[t].first.second;
+                    m |= order[t].first.first;
+                }
+                require(na < (1u << 24), "S blocks: too many coefficient sub-blocks in one block");
+                bt.words[k] = m | (na << 8);
+                for (int g = 0; g < SBLOCK_GROUPS; ++g) {
+                    if (!((m >> g) & 1)) continue;
+                    const size_t a0 = bt.alpha.size();
+                    bt.alpha.resize(a0 + 64, 0.0);
+                    for (size_t t = 4 * k; t < std::min(u.size(), 4 * k + 4); ++t) {
+                        const int32_t src = order[t].second;
+                        for (int i = 0; i < SBLOCK_GROUP; ++i)
+                            bt.alpha[a0 + (t - 4 * k) * SBLOCK_GROUP + size_t(i)] =
+                                al[size_t(src) * SBLOCK_LINES + size_t(g * SBLOCK_GROUP + i)];
+                    }
+                    ++na;
+                }
             }
-            al.swap(sal);
         }
     }
     // concatenate in block order
-    std::vector<int64_t> u_off(spans.size() + 1, 0);
-    for (size_t bi = 0; bi < spans.size(); ++bi) u_off[bi + 1] = u_off[bi] + int64_t(unions[bi].size());
-    B.utile.resize(size_t(u_off.back()));
-    B.ualpha.resize(size_t(u_off.back()) * SBLOCK_LINES);
+    std::vector<int64_t> k_off(spans.size() + 1, 0), a_off(spans.size() + 1, 0);
+    for (size_t bi = 0; bi < spans.size(); ++bi) {
+        k_off[bi + 1] = k_off[bi] + int64_t(tabs[bi].words.size());
+        a_off[bi + 1] = a_off[bi] + int64_t(tabs[bi].alpha.size() / 64);
+    }
+    require(k_off.back() < int64_t(INT32_MAX), "S blocks: more than 2^31 k-steps");
+    B.utile.resize(size_t(4 * k_off.back()));
+    B.kstep.resize(size_t(k_off.back()));
+    B.ualpha.resize(size_t(64 * a_off.back()));
     B.lsoff.assign(spans.size() * SBLOCK_LINES, 0);
     B.llds.assign(spans.size() * SBLOCK_LINES, 1);
     B.blocks.resize(spans.size());
@@ -1322,17 +1491,20 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
     for (int64_t bi = 0; bi < nb; ++bi) {
         Span const& sp = spans[size_t(bi)];
         SLine const& first = lines[(*sp.v)[sp.s0]];
-        std::copy(unions[size_t(bi)].begin(), unions[size_t(bi)].end(), B.utile.begin() + u_off[size_t(bi)]);
-        std::copy(alphas[size_t(bi)].begin(), alphas[size_t(bi)].end(),
-                  B.ualpha.begin() + u_off[size_t(bi)] * SBLOCK_LINES);
+        BlockTabs& bt = tabs[size_t(bi)];
+        std::copy(bt.slots.begin(), bt.slots.end(), B.utile.begin() + 4 * k_off[size_t(bi)]);
+        std::copy(bt.words.begin(), bt.words.end(), B.kstep.begin() + k_off[size_t(bi)]);
+        std::copy(bt.alpha.begin(), bt.alpha.end(), B.ualpha.begin() + 64 * a_off[size_t(bi)]);
         for (uint32_t q = 0; q < sp.n; ++q) {
             SLine const& l = lines[(*sp.v)[sp.s0 + q]];
             B.lsoff[size_t(bi) * SBLOCK_LINES + q] = l.s_off;
             B.llds[size_t(bi) * SBLOCK_LINES + q] = l.lds;
         }
         SBlock& blk = B.blocks[size_t(bi)];
-        blk.u_off = u_off[size_t(bi)];
-        blk.nu = int32_t(unions[size_t(bi)].size());
+        blk.a_off = a_off[size_t(bi)];
+        blk.k_off = int32_t(k_off[size_t(bi)]);
+        blk.nk = int32_t(bt.words.size());
+        blk.na = int32_t(bt.alpha.size() / 64);
         blk.rows = first.rows;
         blk.E = first.rows * first.cols;
         blk.nlines = int32_t(sp.n);
@@ -1342,29 +1514,42 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
     for (int32_t bi = 0; bi < int32_t(nb); ++bi) {
         SBlock const& blk = B.blocks[size_t(bi)];
         for (int32_t e = 0; e < blk.E; e += SBLOCK_ELEMS) B.work.push_back(SWork{bi, e});
-        B.gathered += uint64_t(blk.nu) * uint64_t(blk.E);
+        B.gathered += uint64_t(tabs[size_t(bi)].nu) * uint64_t(blk.E);
+        B.max_union = std::max<uint64_t>(B.max_union, tabs[size_t(bi)].nu);
+        B.mac += uint64_t(blk.na) * 64 * uint64_t(blk.E);
     }
-    B.mac = B.gathered * SBLOCK_LINES;
     if (any_small) build_small_lines(B, lines, terms, is_small);
     return B;
 }
 
 void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size) {
     require(B.lsoff.size() == B.blocks.size() * SBLOCK_LINES && B.llds.size() == B.lsoff.size() &&
-                B.ualpha.size() == B.utile.size() * SBLOCK_LINES, "S blocks: malformed tables");
+                B.utile.size() == 4 * B.kstep.size() && B.ualpha.size() % 64 == 0, "S blocks: malformed tables");
+    const uint64_t n_sub = B.ualpha.size() / 64;
     int bad = 0;  // 1 malformed block, 2 tile outside T, 3 line outside S, 4 malformed work item
 #pragma omp parallel for schedule(dynamic, 256) reduction(max : bad)
     for (int64_t bi = 0; bi < int64_t(B.blocks.size()); ++bi) {
         SBlock const& b = B.blocks[size_t(bi)];
-        if (!(b.nu >= 0 && b.E > 0 && b.rows > 0 && b.E % b.rows == 0 && b.nlines > 0 && b.nlines <= SBLOCK_LINES &&
-              b.u_off >= 0 && uint64_t(b.u_off) + uint64_t(b.nu) <= B.utile.size() && b.line0 >= 0 &&
+        if (!(b.nk >= 0 && b.k_off >= 0 && uint64_t(b.k_off) + uint64_t(b.nk) <= B.kstep.size() && b.na >= 0 &&
+              b.a_off >= 0 && uint64_t(b.a_off) + uint64_t(b.na) <= n_sub && b.E > 0 && b.rows > 0 &&
+              b.E % b.rows == 0 && b.nlines > 0 && b.nlines <= SBLOCK_LINES && b.line0 >= 0 &&
               uint64_t(b.line0) + SBLOCK_LINES <= B.lsoff.size())) {
             bad = std::max(bad, 1);
             continue;
         }
-        for (int32_t k = 0; k < b.nu; ++k) {
-            int64_t o = B.utile[size_t(b.u_off) + k];
-            if (!(o >= 0 && uint64_t(o) + uint64_t(b.E) <= t_size)) bad = std::max(bad, 2);
+        // k-step words: group masks within the block's groups, sub-blocks consecutive and inside
+        // the block's range
+        uint32_t next = 0;
+        const uint32_t gmask = (1u << ((b.nlines + SBLOCK_GROUP - 1) / SBLOCK_GROUP)) - 1;
+        for (int32_t k = 0; k < b.nk; ++k) {
+            const uint32_t w = B.kstep[size_t(b.k_off) + k], m = w & 0xffu;
+            if ((w >> 8) != next || (m & ~gmask)) bad = std::max(bad, 1);
+            next += uint32_t(__builtin_popcount(m));
+        }
+        if (next != uint32_t(b.na)) bad = std::max(bad, 1);
+        for (int64_t k = 4 * int64_t(b.k_off); k < 4 * (int64_t(b.k_off) + b.nk); ++k) {
+            int64_t o = B.utile[size_t(k)];
+            if (!(o == -1 || (o >= 0 && uint64_t(o) + uint64_t(b.E) <= t_size))) bad = std::max(bad, 2);
         }
         int32_t cols = b.E / b.rows;
         for (int32_t i = 0; i < b.nlines; ++i) {

@@ -93,19 +93,29 @@ struct STerm {
 };
 
 // Block form of the MPO-term sum for the matrix cores.  Lines of equal tile shape are grouped in
-// blocks of SBLOCK_LINES; a block stores the sorted union of the T tiles its lines read and a
-// dense SBLOCK_LINES x nu coefficient matrix (tile-major: alpha of line i for union
-// tile k at ualpha[16 (u_off + k) + i], zero where line i does not read tile k, summed where it
-// reads it twice), so the block's S tiles are Y = alpha * X with X the nu x E matrix of the tiles'
-// elements.
-constexpr int SBLOCK_LINES = 16;
+// blocks of up to SBLOCK_LINES, SBLOCK_GROUPS groups of 16.  A block stores the union of the T
+// tiles its lines read, ordered by the set of groups that read each tile (then by T offset) and cut
+// into k-steps of 4 tile slots (the last padded with empty slots, offset -1); utile holds the 4
+// slots of k-step k of the block at 4 (k_off + k).  For every k-step, each group that reads one
+// of its tiles has a 16 x 4 coefficient sub-block (64 values, [slot t][line i] at 16 t + i: alpha of
+// line i for that tile, zero where the line does not read it, summed where it reads it twice);
+// the sub-blocks of a block are stored in k-step order, groups ascending, from ualpha[64 a_off].
+// kstep word of k-step k: bits 0..7 = the groups with a sub-block, bits 8..31 = its first sub-block
+// relative to a_off.  Group g of the block's S tiles is then Y_g = sum over k-steps of
+// alpha_g,k * X_k with X_k the 4 x E matrix of the step's tile elements: a group does matrix-core
+// work only on the k-steps whose tiles it reads, while the tiles are read once per block.
+constexpr int SBLOCK_GROUP = 16;
+constexpr int SBLOCK_GROUPS = 4;
+constexpr int SBLOCK_LINES = SBLOCK_GROUP * SBLOCK_GROUPS;
+static_assert(SBLOCK_GROUPS <= 8, "k-step group masks are 8 bits");
 struct SBlock {
-    int64_t u_off;            // first union tile (utile / ualpha index)
-    int32_t nu, E, rows;      // union size, tile elements (rows x cols), tile rows
+    int64_t a_off;            // first coefficient sub-block (64 values each)
+    int32_t k_off, nk, na;    // first k-step (kstep index; utile from 4 k_off), k-steps, sub-blocks
+    int32_t E, rows;          // tile elements (rows x cols), tile rows
     int32_t nlines, line0;    // lines in the block, first entry of lsoff / llds
     int32_t pad;
 };
-static_assert(sizeof(SBlock) == 32, "SBlock layout");
+static_assert(sizeof(SBlock) == 40, "SBlock layout");
 struct SWork {
     int32_t blk, n0;          // block and first tile element of one workgroup
 };
@@ -120,12 +130,14 @@ struct SSmallWork {
 struct SBlockTable {
     std::vector<SBlock> blocks;
     std::vector<SWork> work;
-    HostTable<int64_t> utile;   // T element offset of each union tile
-    HostTable<double> ualpha;
+    HostTable<int64_t> utile;     // T element offset of each union tile slot (-1: empty slot)
+    HostTable<uint32_t> kstep;    // k-step words
+    HostTable<double> ualpha;     // coefficient sub-blocks
     std::vector<int64_t> lsoff;   // S offset of each line (SBLOCK_LINES entries per block)
     std::vector<int32_t> llds;    // its leading dimension
-    uint64_t gathered = 0;        // sum over blocks of nu * E (T elements read)
-    uint64_t mac = 0;             // SBLOCK_LINES * gathered (matrix-core multiply-adds)
+    uint64_t gathered = 0;        // sum over blocks of union tiles * E (T elements read)
+    uint64_t max_union = 0;       // largest union of one block (tiles)
+    uint64_t mac = 0;             // matrix-core multiply-adds: 64 * E per sub-block
     // small-tile lines: their own line and term tables (t0 / t1 index small_terms)
     std::vector<SLine> small_lines;
     HostTable<STerm> small_terms;
@@ -201,8 +213,21 @@ struct CopyJob {
     int32_t nblocks, trans; // trans: src block is cols x rows
 };
 
+// One chunk of an H*psi plan: the output columns of a set of right sectors of the two-site MPS.
+// Its T-GEMMs, S lines and L*S GEMMs touch only the T intermediates and S panels of these
+// columns, which it addresses from offset 0: the chunks run one after the other on one T and one S
+// buffer of the largest chunk's size.  Ranges index the plan's tables.
+struct HpsiChunk {
+    size_t t_gemm0 = 0, t_gemm1 = 0;   // t_gemms (and t_gemms_task when the plan has one chunk)
+    size_t t_row0 = 0, t_row1 = 0;     // t_rows
+    size_t line0 = 0, line1 = 0;       // lines
+    size_t c_gemm0 = 0, c_gemm1 = 0;   // c_gemms
+    uint64_t t_elems = 0, s_elems = 0;
+};
+
 struct HpsiPlanHost {
     MPSLayout mps;
+    std::vector<HpsiChunk> chunks;  // empty: one chunk over all tables
     std::vector<GemmDesc> t_gemms, c_gemms;
     std::vector<GemmRow> t_rows;  // row table of the stacked T-GEMMs
     std::vector<GemmDesc> t_gemms_task;  // the same T-GEMMs, one problem per task (reference layout)
@@ -210,7 +235,8 @@ struct HpsiPlanHost {
     HostTable<SLine> lines;
     HostTable<STerm> terms;
     std::vector<CopyJob> lpanel_jobs;
-    uint64_t t_elems = 0, s_elems = 0, lpanel_elems = 0;
+    uint64_t t_elems = 0, s_elems = 0, lpanel_elems = 0;  // T and S: sums over the chunks
+    uint64_t t_buf = 0, s_buf = 0;                          // T and S buffers: the largest chunk's
     uint64_t n_cohorts = 0, n_t_tasks = 0;
     uint64_t bytes_compulsory = 0;
     FlopStats flops;
@@ -228,8 +254,11 @@ struct GrowPlanHost {
     FlopStats flops;
 };
 
+// chunk_elems: largest T + S (elements) of one chunk; the right sectors are gathered in index
+// order into chunks up to that size (a sector larger than it is a chunk of its own); 0 = one chunk
 HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, BoundaryIndex const& R,
-                             Index const& left_i, Index const& phys_i, Index const& right_i);
+                             Index const& left_i, Index const& phys_i, Index const& right_i,
+                             uint64_t chunk_elems = 0);
 
 GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, MPSLayout const& bra,
                               MPSLayout const& ket, bool symmetric);

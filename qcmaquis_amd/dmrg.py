@@ -406,6 +406,12 @@ class TwoSiteDMRG:
         self.timing["hpsi"] += st["hpsi_seconds"]
         plan.close()
         tw, lb, t4, t5 = self.accept(i, to_right, e, psi, lay2, red)
+        # boundaries this sweep direction has passed are grown again on the way back: free them
+        # (the edge boundaries stay), so the device holds about one boundary per bond
+        if to_right and i + 2 < self.L:
+            self.right.pop(i + 2, None)
+        elif not to_right and i > 0:
+            self.left.pop(i, None)
         for k, v in (("merge", tm - t0), ("plan", t2 - t1), ("davidson", t3 - t2), ("svd", t4 - t3), ("grow", t5 - t4)):
             self.timing[k] += v
         if self.verbose:

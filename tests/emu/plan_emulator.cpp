@@ -78,8 +78,8 @@ void run_gemms(std::vector<GemmDesc> const& G, std::vector<GemmRow> const& rows,
 }
 
 // the MPO-term sum as the device runs it: the block table built from the lines, each work item
-// covering SBLOCK_ELEMS tile elements of one block (four waves of 16 * SBLOCK_ELEMS / 64); every
-// (line, element) of the table must be written exactly once
+// covering SBLOCK_ELEMS tile elements of one block, the groups of 16 lines summing the sub-blocks of
+// the k-steps their masks name; every (line, element) of the table must be written exactly once
 // (tiles of at most small_max elements: the small-tile work items, 2^lg lanes per line, lane e
 // summing element e over the line's terms)
 // Returns the number of small-tile lines.
@@ -109,20 +109,32 @@ uint64_t run_lines(HostTable<SLine>& lines, HostTable<STerm> const& terms, const
     for (char d : sdone) require(d, "emulated small lines: line not written");
     std::vector<std::vector<char>> done(B.blocks.size());
     for (size_t i = 0; i < B.blocks.size(); ++i) done[i].assign(size_t(B.blocks[i].E) * SBLOCK_LINES, 0);
-    const int per_wave = 16 * (SBLOCK_ELEMS / 64);
     for (SWork const& w : B.work) {
         SBlock const& b = B.blocks[size_t(w.blk)];
-        for (int wave = 0; wave < 4; ++wave)
-            for (int e = w.n0 + wave * per_wave; e < std::min(b.E, w.n0 + (wave + 1) * per_wave); ++e)
-                for (int i = 0; i < b.nlines; ++i) {
-                    double acc = 0.0;
-                    for (int k = 0; k < b.nu; ++k)
-                        acc += B.ualpha[size_t(b.u_off + k) * SBLOCK_LINES + i] * T[B.utile[size_t(b.u_off + k)] + e];
-                    char& d = done[size_t(w.blk)][size_t(e) * SBLOCK_LINES + i];
-                    require(!d, "emulated S blocks: element written twice");
-                    d = 1;
-                    S[B.lsoff[size_t(b.line0 + i)] + e % b.rows + int64_t(e / b.rows) * B.llds[size_t(b.line0 + i)]] = acc;
+        for (int e = w.n0; e < std::min(b.E, w.n0 + SBLOCK_ELEMS); ++e) {
+            // the k-steps in order, each group with a sub-block adding alpha_g(16 x 4) * X(4 tiles)
+            double acc[SBLOCK_LINES] = {0.0};
+            for (int32_t k = 0; k < b.nk; ++k) {
+                const uint32_t word = B.kstep[size_t(b.k_off + k)];
+                int64_t ai = b.a_off + int64_t(word >> 8);
+                for (int g = 0; g < SBLOCK_GROUPS; ++g) {
+                    if (!((word >> g) & 1u)) continue;
+                    for (int t = 0; t < 4; ++t) {
+                        const int64_t to = B.utile[size_t(4 * int64_t(b.k_off + k) + t)];
+                        const double x = to >= 0 ? T[to + e] : 0.0;
+                        for (int i = 0; i < SBLOCK_GROUP; ++i)
+                            acc[g * SBLOCK_GROUP + i] += B.ualpha[size_t(64 * ai + 16 * t + i)] * x;
+                    }
+                    ++ai;
                 }
+            }
+            for (int i = 0; i < b.nlines; ++i) {
+                char& d = done[size_t(w.blk)][size_t(e) * SBLOCK_LINES + i];
+                require(!d, "emulated S blocks: element written twice");
+                d = 1;
+                S[B.lsoff[size_t(b.line0 + i)] + e % b.rows + int64_t(e / b.rows) * B.llds[size_t(b.line0 + i)]] = acc[i];
+            }
+        }
     }
     for (size_t i = 0; i < B.blocks.size(); ++i)
         for (int e = 0; e < B.blocks[i].E; ++e)
@@ -172,37 +184,55 @@ extern "C" {
 
 int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const double* Ldata, const mq_bindex_desc* Rd,
              const double* Rdata, const mq_index_desc* li, const mq_index_desc* pi, const mq_index_desc* ri,
-             const double* psi, double* out, int small_max, uint64_t* n_small_lines) {
+             const double* psi, double* out, int small_max, uint64_t* n_small_lines, uint64_t chunk_elems,
+             uint64_t* n_chunks, uint64_t* ts_elems) {
     try {
         MPO mpo(*md);
         BoundaryIndex L = boundary_index_from(*Ld), R = boundary_index_from(*Rd);
         uint64_t Ln = L.layout(), Rn = R.layout();
-        HpsiPlanHost P = build_hpsi_plan(Sym(sym), mpo, L, R, Index(*li), Index(*pi), Index(*ri));
+        HpsiPlanHost P = build_hpsi_plan(Sym(sym), mpo, L, R, Index(*li), Index(*pi), Index(*ri), chunk_elems);
         uint64_t n = P.mps.elements();
-        uint64_t sizes[BASE_COUNT] = {0};
-        sizes[BASE_PSI_IN] = sizes[BASE_PSI_OUT] = n;
-        sizes[BASE_T] = P.t_elems;
-        sizes[BASE_S] = P.s_elems;
-        sizes[BASE_LPANEL] = P.lpanel_elems;
-        sizes[BASE_BOUNDARY] = Rn;
-        check_gemms(P.t_gemms, sizes, P.t_rows);
-        check_gemms(P.c_gemms, sizes);
-        check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
         for (auto const& j : P.lpanel_jobs)
             require(uint64_t(j.src) + uint64_t(j.rows) * j.cols * j.nblocks <= Ln &&
                     uint64_t(j.dst) + uint64_t(j.rows) * j.cols * j.nblocks <= P.lpanel_elems, "copy job out of range");
-        std::vector<double> T(P.t_elems + 1, std::nan("")), S(P.s_elems + 1, std::nan("")), Lp(P.lpanel_elems + 1, std::nan(""));
+        std::vector<double> Lp(P.lpanel_elems + 1, std::nan(""));
         run_copy(P.lpanel_jobs, Ldata, Lp.data());
-        double* base[BASE_COUNT] = {nullptr};
-        base[BASE_PSI_IN] = const_cast<double*>(psi);
-        base[BASE_PSI_OUT] = out;
-        base[BASE_T] = T.data();
-        base[BASE_S] = S.data();
-        base[BASE_LPANEL] = Lp.data();
-        base[BASE_BOUNDARY] = const_cast<double*>(Rdata);
-        run_gemms(P.t_gemms, P.t_rows, base);
-        *n_small_lines = run_lines(P.lines, P.terms, T.data(), S.data(), P.t_elems, P.s_elems, small_max);
-        run_gemms(P.c_gemms, {}, base);
+        // psi' starts as NaN: every element must be written by exactly the chunks' output GEMMs
+        std::fill(out, out + n, std::nan(""));
+        *n_small_lines = 0;
+        *n_chunks = P.chunks.size();
+        *ts_elems = P.t_elems + P.s_elems;
+        require(!P.chunks.empty(), "plan without chunks");
+        for (HpsiChunk const& ch : P.chunks) {
+            require(ch.t_elems <= P.t_buf && ch.s_elems <= P.s_buf, "chunk larger than the buffers");
+            uint64_t sizes[BASE_COUNT] = {0};
+            sizes[BASE_PSI_IN] = sizes[BASE_PSI_OUT] = n;
+            sizes[BASE_T] = ch.t_elems;
+            sizes[BASE_S] = ch.s_elems;
+            sizes[BASE_LPANEL] = P.lpanel_elems;
+            sizes[BASE_BOUNDARY] = Rn;
+            std::vector<GemmDesc> tg(P.t_gemms.begin() + int64_t(ch.t_gemm0), P.t_gemms.begin() + int64_t(ch.t_gemm1));
+            std::vector<GemmRow> tr(P.t_rows.begin() + int64_t(ch.t_row0), P.t_rows.begin() + int64_t(ch.t_row1));
+            for (auto& g : tg)
+                if (g.flags & GEMM_ROWMAP) g.a -= int64_t(ch.t_row0);
+            std::vector<GemmDesc> cg(P.c_gemms.begin() + int64_t(ch.c_gemm0), P.c_gemms.begin() + int64_t(ch.c_gemm1));
+            HostTable<SLine> cl(P.lines.begin() + int64_t(ch.line0), P.lines.begin() + int64_t(ch.line1));
+            check_gemms(tg, sizes, tr);
+            check_gemms(cg, sizes);
+            check_lines(cl, P.terms, ch.t_elems, ch.s_elems);
+            // fresh NaN buffers per chunk: a chunk that read another chunk's intermediates shows
+            std::vector<double> T(P.t_buf + 1, std::nan("")), S(P.s_buf + 1, std::nan(""));
+            double* base[BASE_COUNT] = {nullptr};
+            base[BASE_PSI_IN] = const_cast<double*>(psi);
+            base[BASE_PSI_OUT] = out;
+            base[BASE_T] = T.data();
+            base[BASE_S] = S.data();
+            base[BASE_LPANEL] = Lp.data();
+            base[BASE_BOUNDARY] = const_cast<double*>(Rdata);
+            run_gemms(tg, tr, base);
+            *n_small_lines += run_lines(cl, P.terms, T.data(), S.data(), ch.t_elems, ch.s_elems, small_max);
+            run_gemms(cg, {}, base);
+        }
         return 0;
     } catch (std::exception const& e) {
         fprintf(stderr, "emu_hpsi: %s\n", e.what());
