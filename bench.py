@@ -25,7 +25,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"]
+METRIC = "sweep time & H\u00b7\u03c8 TFLOP/s at M=2048: mixed+batched vs FP64 plain, |\u0394E|\u22641e-6 Eh"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 GEMM_FAMILY = "gemm_kernel<*, double, double> (T-GEMM and L*S GEMM launches, split-K reduce)"
@@ -202,6 +202,76 @@ def su2_workload(ctx, L=40, M=2048, seed=1003, rank=80, warm_M=128, warm_iter=2,
             "setup_seconds": {"mpo": round(t_mpo, 2), "warm_half_sweep": round(t_warm, 2),
                               "boundaries": round(t_grow, 2)}}
     return dict(mpo=mpo, mpo_spec=ham.mpo(mid, True), L=Lb, R=R, layout=lay2, sym=SU2, desc=desc, keep=(ham,))
+
+
+def qc3_sweep_bench(args, ctx):
+    """Real two-site sweeps at configuration 3 (40 orbitals, SU2U1, S=0, M=2048): the MPS of the
+    H*psi workload (sectors of the warm-up half sweep scaled to M=2048, seeded random elements) is
+    the common start of one full sweep per precision (FP64 grouped, mixed; with --plain-sweep also
+    FP64 with the reference's launch structure), Davidson 8 iterations / tol 1e-8 per step.  Every
+    run plans, solves, splits (truncating to M) and grows its own boundaries; prints one JSON line
+    with the sweep times, their breakdown and the final energies."""
+    from qcmaquis_amd import abi
+    from qcmaquis_amd.dmrg import TwoSiteDMRG, allowed_bonds_su2
+    from qcmaquis_amd.model import Index, MPSLayout
+    from qcmaquis_amd.qc import QCHamiltonian, make_integrals
+    import gc
+    L, M, seed, rank = 40, args.M or 2048, 1003, 80
+    h, eri, ec = make_integrals(L, seed, rank)
+    ham = QCHamiltonian(h, eri, ec, symmetry=abi.MQ_SYM_SU2U1)
+    rec = json.load(open(BONDS3))
+    warm = [Index([tuple(c) for c in b["charges"]], b["sizes"]) for b in rec["bonds"]]
+    phys = ham.phys()
+    bonds = scale_to(warm, allowed_bonds_su2([phys] * L, (L, 0), 1 << 30), M)
+    lay = [MPSLayout(bonds[k], phys, bonds[k + 1]) for k in range(L)]
+    rng = np.random.default_rng(seed)
+    data = [rng.normal(size=x.n) for x in lay]
+    t0 = time.perf_counter()
+    for k in range(L - 1):  # every two-site MPO once (cached by the Hamiltonian for all runs)
+        ham.mpo(k, True)
+    t_mpo = time.perf_counter() - t0
+    log("qc3 sweeps: %d two-site MPOs built in %.1f s" % (L - 1, t_mpo))
+    runs = {}
+    allp = {"fp64": abi.MQ_PREC_FP64, "mixed": abi.MQ_PREC_FP32, "fp64_plain": abi.MQ_PREC_FP64_PLAIN}
+    names = args.sweep_runs.split(",") + (["fp64_plain"] if args.plain_sweep else [])
+    precs = [(n, allp[n]) for n in names]
+    for name, prec in precs:
+        ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, int(prec == abi.MQ_PREC_FP64_PLAIN))
+        t0 = time.perf_counter()
+        d = TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8, precision=prec,
+                        mps=(lay, data), verbose=True)
+        t_init = time.perf_counter() - t0
+        log("qc3 sweeps %s: start state normalised and right boundaries grown in %.1f s" % (name, t_init))
+        e, dt = d.sweep()
+        r = {"sweep_seconds": round(dt, 3), "init_seconds": round(t_init, 3), "final_energy": e,
+             "lowest_energy": min(d.energies), "max_truncated_weight": float(max(d.trunc)),
+             "final_residual_norm": d.last_residual, "hpsi_flops": int(d.hpsi_flops),
+             "timing_s": {k: round(v, 3) for k, v in d.timing.items()}}
+        r["hpsi_tflops_in_davidson"] = round(d.hpsi_flops / max(d.timing["hpsi"], 1e-9) / 1e12, 3)
+        runs[name] = r
+        log("qc3 sweeps %s: %.1f s, E %.10f, timing %s" % (name, dt, e, r["timing_s"]))
+        del d
+        gc.collect()
+    ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
+    ref = runs.get("fp64")
+    main_run = "mixed" if "mixed" in runs else names[0]
+    line = {"metric": METRIC, "value": runs[main_run]["sweep_seconds"], "unit": f"s (one full sweep, {main_run})",
+            "n_gpus": 1, "higher_is_better": False, "dtype": "f64 (FP32 contraction in the mixed run)",
+            "data": "synthetic (seeded rank-80 integrals; MPS sectors from a warm-up half sweep at M=128, scaled "
+                    "to M=2048, seeded random elements)",
+            "config": {"workload": f"configuration 3 sweeps: {L} orbitals, {L} electrons, SU2U1, S=0, M={M}, "
+                                   "one full two-site sweep (78 steps) per run from one common start, Davidson 8 "
+                                   "iterations / tol 1e-8",
+                       "two_site_mpo_seconds": round(t_mpo, 3), "runs": runs}}
+    if ref and "mixed" in runs:
+        line["config"]["mixed_abs_delta_e_vs_fp64"] = abs(runs["mixed"]["final_energy"] - ref["final_energy"])
+        line["config"]["mixed_speedup_vs_fp64"] = round(ref["sweep_seconds"] / runs["mixed"]["sweep_seconds"], 3)
+    if "fp64_plain" in runs and ref and "mixed" in runs:
+        line["config"]["mixed_speedup_vs_fp64_plain"] = round(runs["fp64_plain"]["sweep_seconds"] /
+                                                              runs["mixed"]["sweep_seconds"], 3)
+        line["config"]["fp64_speedup_vs_fp64_plain"] = round(runs["fp64_plain"]["sweep_seconds"] /
+                                                             ref["sweep_seconds"], 3)
+    print(json.dumps(line), flush=True)
 
 
 def batch_sweep_bench(args, ctx):
@@ -578,7 +648,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="qc3", choices=["qc", "qc3", "synthetic", "batch", "batch-sweep"])
+    ap.add_argument("--workload", default="qc3", choices=["qc", "qc3", "qc3-sweep", "synthetic", "batch", "batch-sweep"])
+    ap.add_argument("--plain-sweep", action="store_true", help="qc3-sweep: also sweep with the reference-style FP64")
+    ap.add_argument("--sweep-runs", default="fp64,mixed", help="qc3-sweep: runs (fp64, mixed, fp64_plain)")
     ap.add_argument("--M", type=int, default=None, help="bond dimension of the qc / qc3 workloads")
     ap.add_argument("--regen-bonds", action="store_true", help="qc3: redo the warm-up half sweep")
     ap.add_argument("--instances", type=int, default=64, help="instances of the batch workload")
@@ -613,7 +685,13 @@ def main():
             torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}), flush=True)
         return
-    ctx = Context(0, abi.MQ_SYM_SU2U1 if args.workload == "qc3" else abi.MQ_SYM_TWOU1)
+    ctx = Context(0, abi.MQ_SYM_SU2U1 if args.workload in ("qc3", "qc3-sweep") else abi.MQ_SYM_TWOU1)
+    if args.workload == "qc3-sweep":
+        if args.chunk_mb is not None:
+            ctx.set_option(abi.MQ_OPT_CHUNK_MB, args.chunk_mb)
+        qc3_sweep_bench(args, ctx)
+        ctx.close()
+        return
     if args.workload in ("batch", "batch-sweep"):
         (batch_bench if args.workload == "batch" else batch_sweep_bench)(args, ctx)
         ctx.close()

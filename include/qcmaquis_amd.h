@@ -277,10 +277,12 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t plan);
 
 /* ---- Jacobi-Davidson (reference: solve_ietl_jcd, ietl_jacobi_davidson.h:35-76,
  *      jacobi_davidson::calculate_eigenvalue, ietl/jacobi.h:386-474) ------------------------ */
-/* precision MQ_PREC_FP32: at most max_iter - R iterations, R = max(1, max_iter / 4), use FP32
- * H*psi (stopping at a relative residual of max(tol, 1e-6)), then the iterations left of max_iter
- * (at least R) refine the Ritz vector with FP64 H*psi; orthogonalisation, Rayleigh matrix and
- * residuals are FP64 throughout */
+/* precision MQ_PREC_FP32: one search space is built with FP32 H*psi for up to max_iter
+ * iterations, stopping at a relative residual of max(tol, 1e-5); then the Ritz vector is refined
+ * with FP64 H*psi from a restart for the iterations left of max_iter, at least one (so the
+ * returned eigenvalue is the FP64 Rayleigh quotient of the returned vector; max_iter + 1 H*psi
+ * when the FP32 phase uses the whole budget); orthogonalisation, Rayleigh matrix and residuals
+ * are FP64 throughout */
 int mq_davidson(mq_hpsi_plan_t plan, const double* psi0, int n_ortho, const double* const* ortho,
                 double tol, int max_iter, int precision, double* eval, double* psi_out,
                 mq_davidson_stats_t* stats);

@@ -87,7 +87,7 @@ struct DevCache {
     uint64_t idle_bytes = 0;
 };
 constexpr uint64_t CACHE_MIN = uint64_t(1) << 20;
-constexpr uint64_t CACHE_CAP = uint64_t(16) << 30;
+constexpr uint64_t CACHE_CAP = uint64_t(96) << 30;
 
 DevCache& devcache() {
     static DevCache* c = new DevCache;  // never destroyed: frees may come from static destructors
@@ -1142,14 +1142,16 @@ static void ensure_fp32(mq_hpsi_plan_t p) {
     if (p->T32) return;
     hipStream_t s = p->ctx->stream;
     HpsiPlanHost const& H = p->host;
-    p->S32 = dev_alloc<float>(H.s_buf);
+    // FP32 T and S live in the FP64 buffers: the H*psi of both precisions run on one stream, one
+    // after the other
+    p->S32 = reinterpret_cast<float*>(p->S);
     p->Lpanel32 = dev_alloc<float>(H.lpanel_elems);
     p->R32 = dev_alloc<float>(p->r_elems);
     p->psi32 = dev_alloc<float>(p->psi_n);
     hipcheck(launch_convert(p->Lpanel, p->Lpanel32, H.lpanel_elems, s), "convert");
     hipcheck(launch_convert(p->rdata, p->R32, p->r_elems, s), "convert");
     for (auto& c : p->chunks) c.sa.ensure32(s);
-    p->T32 = dev_alloc<float>(H.t_buf);
+    p->T32 = reinterpret_cast<float*>(p->T);
 }
 
 // H*psi on the context stream; ev (optional, 5 events) brackets the stages set-up, T-GEMMs,
@@ -1374,8 +1376,6 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
             dev_free(g.in);
             dev_free(g.out);
         }
-        dev_free(p->T32);
-        dev_free(p->S32);
         dev_free(p->Lpanel32);
         dev_free(p->R32);
         dev_free(p->psi32);
@@ -1392,6 +1392,11 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
 
 // ---- Jacobi-Davidson ----------------------------------------------------------------------------
 namespace {
+
+// relative residual at which a mixed-precision Davidson leaves FP32 H*psi for the FP64
+// refinement: a decade above the FP32 contraction's relative error (about 2e-6 on the
+// benchmark problems)
+constexpr double MIXED_SWITCH = 1e-5;
 
 // lowest eigenpair of a small symmetric matrix by cyclic Jacobi rotations (n <= 64)
 void small_sym_lowest(std::vector<double> a, int n, double& theta, std::vector<double>& vec) {
@@ -1551,14 +1556,15 @@ static void davidson_core(mq_hpsi_plan_t p, const double* psi0, int n_ortho, con
             } else if (precision == MQ_PREC_FP64 || max_iter < 2) {
                 jd(MQ_PREC_FP64, tol, max_iter);
             } else {
-                // mixed precision within the same iteration budget: FP32 H*psi for at most
-                // max_iter - R iterations, R = max(1, max_iter / 4) (stopping at the FP32 floor),
-                // then the Ritz vector is refined with FP64 H*psi for the iterations that are left
-                const int R = std::max(1, max_iter / 4);
-                jd(MQ_PREC_FP32, std::max(tol, 1e-6), max_iter - R);
+                // mixed precision: one search space built with FP32 H*psi for up to max_iter
+                // iterations, stopping once the relative residual is within MIXED_SWITCH (a decade
+                // above the FP32 floor); then the Ritz vector is refined with FP64 H*psi from a
+                // restart for the iterations left, at least one (the FP64 Rayleigh quotient and
+                // residual of the returned vector)
+                jd(MQ_PREC_FP32, std::max(tol, MIXED_SWITCH), max_iter);
                 int it32 = it;
                 hipcheck(hipMemcpyAsync(V[0], u, n * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
-                jd(MQ_PREC_FP64, tol, max_iter - it32);
+                jd(MQ_PREC_FP64, tol, std::max(1, max_iter - it32));
                 refine = it;
                 it += it32;
             }
@@ -1758,14 +1764,13 @@ extern "C" int mq_davidson_batch(mq_hpsi_plan_t p, const double* psi0, double to
                 total = its;
                 lockstep = it_run;
             } else {
-                // FP32 H*psi first (to the FP32 floor), then FP64 refinement of every Ritz vector, as
-                // mq_davidson does for one instance
-                const int R = std::max(1, max_iter / 4);
-                jd(MQ_PREC_FP32, std::max(tol, 1e-6), max_iter - R);
+                // FP32 H*psi first (to MIXED_SWITCH), then FP64 refinement of every Ritz vector for the
+                // lock-step iterations left (at least one), as mq_davidson does for one instance
+                jd(MQ_PREC_FP32, std::max(tol, MIXED_SWITCH), max_iter);
                 total = its;
                 lockstep = it_run;
                 hipcheck(hipMemcpyAsync(V[0], u, n * sizeof(double), hipMemcpyDeviceToDevice, s), "copy");
-                jd(MQ_PREC_FP64, tol, max_iter - lockstep);
+                jd(MQ_PREC_FP64, tol, std::max(1, max_iter - lockstep));
                 for (int i = 0; i < ns; ++i) total[i] += its[i];
                 refine = it_run;
                 lockstep += it_run;
