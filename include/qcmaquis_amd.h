@@ -50,7 +50,9 @@ extern "C" {
  * (rocBLAS) dgemm per T task, one S-assembly workgroup per S tile that gathers every term, and
  * one dgemm per cohort added into its output block (reference: MPSBlock::create_T_gpu,
  * Cohort::create_s_r_gpu / compute_s_stackedv, Cohort::contract_gpu; tasks.cpp:286-344, 466-504,
- * 650-752).  Needs a plan made while the context option MQ_OPT_PLAIN_TABLES is set. */
+ * 650-752).  Needs a plan made while the context option MQ_OPT_PLAIN_TABLES is set.  A plan with
+ * several chunks (MQ_OPT_CHUNK_MB) runs them one after the other, a cohort's dgemm split by the
+ * runs of its output block's columns in the chunk. */
 #define MQ_PREC_FP64_PLAIN 3
 
 /* context options (mq_ctx_set_option) */
@@ -66,8 +68,7 @@ extern "C" {
                                   by right sector into chunks of at most this many MiB of FP64 T and
                                   S intermediates (a larger right sector is a chunk of its own),
                                   run one after the other on buffers of the largest chunk's size;
-                                  0: one chunk; MQ_CHUNK_AUTO (default): 49152 MiB.  Plans made
-                                  with MQ_OPT_PLAIN_TABLES have one chunk. */
+                                  0: one chunk; MQ_CHUNK_AUTO (default): 49152 MiB. */
 #define MQ_CHUNK_AUTO (-1)
 
 /* MPS pairing (reference: MPSTensor::make_left_paired / make_right_paired, mpstensor.hpp:211-237) */

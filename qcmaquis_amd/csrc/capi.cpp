@@ -430,6 +430,8 @@ struct mq_hpsi_plan_s {
     struct Plain {
         bool ok = false;
         std::vector<GemmDesc> t, c;
+        struct Range { size_t t0, t1, c0, c1, l0, l1; };  // per chunk: t, c and line ranges
+        std::vector<Range> ranges;
         SLine* lines = nullptr;
         STerm* terms = nullptr;
         int64_t n_lines = 0;
@@ -909,12 +911,19 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
         sbs[c] = SBlockTable();
     }
     p->n_lines = int64_t(H.lines.size());
-    if (ctx->plain_tables && p->n_instances == 1 && nch == 1) {
+    if (ctx->plain_tables && p->n_instances == 1) {
         auto tp = std::chrono::steady_clock::now();
-        sizes[BASE_T] = H.t_buf;
-        sizes[BASE_S] = H.s_buf;
-        check_gemms(H.t_gemms_task, sizes);
-        check_gemms(H.c_gemms_cohort, sizes);
+        for (HpsiChunk const& ch : H.chunks) {
+            sizes[BASE_T] = ch.t_elems;
+            sizes[BASE_S] = ch.s_elems;
+            std::vector<GemmDesc> t(H.t_gemms_task.begin() + int64_t(ch.t_task0),
+                                    H.t_gemms_task.begin() + int64_t(ch.t_task1));
+            std::vector<GemmDesc> c(H.c_gemms_cohort.begin() + int64_t(ch.c_coh0),
+                                    H.c_gemms_cohort.begin() + int64_t(ch.c_coh1));
+            check_gemms(t, sizes);
+            check_gemms(c, sizes);
+            p->plain.ranges.push_back({ch.t_task0, ch.t_task1, ch.c_coh0, ch.c_coh1, ch.line0, ch.line1});
+        }
         p->plain.t = H.t_gemms_task;
         p->plain.c = H.c_gemms_cohort;
         p->plain.lines = dev_upload(H.lines, ctx->stream);
@@ -951,9 +960,9 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
     return guarded(ctx, [&] {
         require(mpo && left && right && out, "mq_hpsi_plan_create: null argument");
         auto t0 = std::chrono::steady_clock::now();
-        // largest T + S of one chunk (elements); the per-task tables of MQ_PREC_FP64_PLAIN need one chunk
+        // largest T + S of one chunk (elements)
         const int64_t mb = ctx->chunk_mb == MQ_CHUNK_AUTO ? 49152 : ctx->chunk_mb;
-        const uint64_t chunk_elems = ctx->plain_tables ? 0 : uint64_t(mb) * (uint64_t(1) << 20) / sizeof(double);
+        const uint64_t chunk_elems = uint64_t(mb) * (uint64_t(1) << 20) / sizeof(double);
         auto* p = new mq_hpsi_plan_s{ctx, build_hpsi_plan(ctx->sym, mpo->mpo, left->idx, right->idx, index_from(left_i),
                                                           index_from(phys_i), index_from(right_i), chunk_elems)};
         std::unique_ptr<mq_hpsi_plan_s> guard(p);
@@ -1159,7 +1168,8 @@ static void ensure_fp32(mq_hpsi_plan_t p) {
 // run on the FP32 matrix cores with FP32 T and S, and psi' is written in FP64.
 // One rocBLAS dgemm (or strided-batched dgemm) per problem on the context stream: the launch
 // structure of the reference's per-task cuBLAS calls (tasks.cpp:286-344, 650-752).
-static void library_gemms(mq_ctx_t ctx, std::vector<GemmDesc> const& probs, Bases const& b, double beta = 0.0) {
+static void library_gemms(mq_ctx_t ctx, const GemmDesc* first, const GemmDesc* last, Bases const& b,
+                          double beta = 0.0) {
     if (!ctx->blas) {
         require(rocblas_create_handle(&ctx->blas) == rocblas_status_success, "rocblas_create_handle failed");
         require(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host) == rocblas_status_success,
@@ -1167,7 +1177,8 @@ static void library_gemms(mq_ctx_t ctx, std::vector<GemmDesc> const& probs, Base
     }
     require(rocblas_set_stream(ctx->blas, ctx->stream) == rocblas_status_success, "rocblas_set_stream failed");
     const double one = 1.0;
-    for (GemmDesc const& g : probs) {
+    for (const GemmDesc* gp = first; gp != last; ++gp) {
+        GemmDesc const& g = *gp;
         if (g.M <= 0 || g.N <= 0) continue;
         const double* A = static_cast<const double*>(b.p[g.base_a]) + g.a;
         const double* B = static_cast<const double*>(b.p[g.base_b]) + g.b;
@@ -1233,16 +1244,20 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         b.p[BASE_LPANEL] = p->Lpanel;
         b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
         mark(0);
-        mark(1);
-        library_gemms(p->ctx, p->plain.t, b);
-        mark(2);
-        hipcheck(launch_slines(p->plain.lines, p->plain.n_lines, p->plain.terms, p->T, p->S, s), "S lines");
-        mark(3);
         // psi' = 0, then every cohort's product is added in stream order (the reference's mutex /
         // atomic accumulation into the output block)
         hipcheck(hipMemsetAsync(out, 0, p->psi_n * sizeof(double), s), "psi' clear");
-        library_gemms(p->ctx, p->plain.c, b, 1.0);
-        mark(4);
+        mark(1);
+        for (size_t c = 0; c < p->plain.ranges.size(); ++c) {
+            auto const& r = p->plain.ranges[c];
+            library_gemms(p->ctx, p->plain.t.data() + r.t0, p->plain.t.data() + r.t1, b);
+            mark(int(2 + 3 * c));
+            hipcheck(launch_slines(p->plain.lines + r.l0, int64_t(r.l1 - r.l0), p->plain.terms, p->T, p->S, s),
+                     "S lines");
+            mark(int(3 + 3 * c));
+            library_gemms(p->ctx, p->plain.c.data() + r.c0, p->plain.c.data() + r.c1, b, 1.0);
+            mark(int(4 + 3 * c));
+        }
     } else {
         throw Error(MQ_ERR_UNSUPPORTED,
                     "H*psi precision: MQ_PREC_FP64, MQ_PREC_FP32 and MQ_PREC_FP64_PLAIN are available");
@@ -1306,7 +1321,7 @@ int mq_hpsi_timed(mq_hpsi_plan_t p, const double* in, double* out, int precision
     return guarded(p->ctx, [&] {
         require(in && out && in != out && stage_ms, "mq_hpsi_timed: bad argument");
         hipStream_t s = p->ctx->stream;
-        const size_t nch = precision == MQ_PREC_FP64_PLAIN ? 1 : p->chunks.size();
+        const size_t nch = precision == MQ_PREC_FP64_PLAIN ? p->plain.ranges.size() : p->chunks.size();
         std::vector<hipEvent_t> ev(2 + 3 * nch, nullptr);
         try {
             for (auto& e : ev) hipcheck(hipEventCreate(&e), "event create");

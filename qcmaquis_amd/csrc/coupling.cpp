@@ -2,7 +2,10 @@
 
 #include <algorithm>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <stdexcept>
+#include <tuple>
 
 #include "core.hpp"
 
@@ -110,6 +113,20 @@ Wigner9jCache::Wigner9jCache(int J, int I, int Ip) {
             set_coupling(J, two_s, Jp, t / 4, Ks[t], Aps[t], I, two_sp, Ip, 1.0,
                          &coeff_[4 * t + 48 * (Jp - jp_min_)]);
     }
+}
+
+std::shared_ptr<const Wigner9jCache> wigner9j_cache(int J, int I, int Ip) {
+    static std::mutex m;
+    static std::map<std::tuple<int, int, int>, std::shared_ptr<const Wigner9jCache>> tables;
+    const auto key = std::make_tuple(J, I, Ip);
+    {
+        std::lock_guard<std::mutex> lock(m);
+        auto it = tables.find(key);
+        if (it != tables.end()) return it->second;
+    }
+    auto t = std::make_shared<const Wigner9jCache>(J, I, Ip);  // built outside the lock
+    std::lock_guard<std::mutex> lock(m);
+    return tables.emplace(key, t).first->second;
 }
 
 void Wigner9jCache::set_scale(int A, int K, int Ap, int Jp, double scale, double out[4]) const {

@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 namespace mq {
@@ -40,5 +41,8 @@ private:
     std::vector<double> coeff_;
     static int hash(int a, int b, int c) { return 4 * a + b + ((a == 1 && b == 1) ? c : 0); }
 };
+// the table of (J, I, I'), made once per process and shared (thread-safe): the schedule builders
+// ask for the same few spin triples at every cohort of every site
+std::shared_ptr<const Wigner9jCache> wigner9j_cache(int J, int I, int Ip);
 
 }  // namespace mq

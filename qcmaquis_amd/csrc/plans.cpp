@@ -77,18 +77,45 @@ struct TTask {
     uint64_t size = 0;   // elements of the T intermediate
     uint32_t rows = 0;   // its leading dimension
 };
+// The (column, cohort) -> task lookup is an open-addressing table (linear probing, at most half
+// full): the schedule builders look up one task per operator entry, tens of millions per site
+// problem.  A list is filled by one thread and then only read.
 struct TList {
     std::vector<TTask> tasks;
-    std::unordered_map<uint64_t, uint32_t> lookup;
+    std::vector<uint64_t> keys;   // key + 1 per slot (0 = empty)
+    std::vector<uint32_t> vals;
+    static uint64_t key_of(uint64_t col, uint32_t ci) { return (col << 24) ^ ci; }
+    size_t slot(uint64_t k) const {
+        return size_t((k * 0x9E3779B97F4A7C15ull) >> 17) & (keys.size() - 1);
+    }
+    void insert(uint64_t k, uint32_t v) {
+        size_t h = slot(k);
+        while (keys[h] != 0) {
+            if (keys[h] == k + 1) return;  // the first task of a key is kept
+            h = (h + 1) & (keys.size() - 1);
+        }
+        keys[h] = k + 1;
+        vals[h] = v;
+    }
     void add(uint64_t col, uint32_t ci, uint32_t ci_eff, uint32_t lb_ket) {
-        uint64_t key = (col << 24) ^ ci;
         require(col < (uint64_t(1) << 40) && ci < (1u << 24), "T list: key overflow");
-        if (!lookup.count(key)) lookup[key] = uint32_t(tasks.size());
+        if (2 * (tasks.size() + 1) > keys.size()) {  // grow and re-insert in task order
+            std::vector<uint64_t> old_keys;
+            old_keys.swap(keys);
+            keys.assign(std::max<size_t>(16, 2 * old_keys.size()), 0);
+            vals.assign(keys.size(), 0);
+            for (size_t i = 0; i < tasks.size(); ++i) insert(key_of(tasks[i].psi_col, tasks[i].ci), uint32_t(i));
+        }
+        insert(key_of(col, ci), uint32_t(tasks.size()));
         tasks.push_back(TTask{col, ci, ci_eff, lb_ket});
     }
     uint32_t get(uint64_t col, uint32_t ci) const {
-        auto it = lookup.find((col << 24) ^ ci);
-        return it == lookup.end() ? UINT32_MAX : it->second;
+        if (keys.empty()) return UINT32_MAX;
+        const uint64_t k = key_of(col, ci);
+        for (size_t h = slot(k);; h = (h + 1) & (keys.size() - 1)) {
+            if (keys[h] == k + 1) return vals[h];
+            if (keys[h] == 0) return UINT32_MAX;
+        }
     }
 };
 
@@ -421,8 +448,8 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                 Charge rc_out = right_i.charges[rb_out];
                 uint32_t rs_out = right_i.sizes[rb_out];
                 coh.add_unit(s, phys_i.sizes[s], rs_out, pb_off[s * NR + rb_out]);
-                std::unique_ptr<Wigner9jCache> w9j;
-                if (su2) w9j.reset(new Wigner9jCache(spin_of(lc_in), spin_of(lc_out), spin_of(rc_out)));
+                std::shared_ptr<const Wigner9jCache> w9j;
+                if (su2) w9j = wigner9j_cache(spin_of(lc_in), spin_of(lc_out), spin_of(rc_out));
 
                 for (uint32_t b1 : l_rows[ci]) {
                     uint32_t left_idx = uint32_t(L.offset(ci, b1) / (int64_t(ls_in) * ls_out));
@@ -586,9 +613,11 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
     for (uint32_t ch = 0; ch < n_chunks; ++ch) {
         P.chunks[ch].t_gemm0 = P.t_gemms.size();
         P.chunks[ch].t_row0 = P.t_rows.size();
-        tstacks[ch].emit(R, BASE_PSI_IN, P.t_gemms, P.t_rows, n_chunks == 1 ? &P.t_gemms_task : nullptr);
+        P.chunks[ch].t_task0 = P.t_gemms_task.size();
+        tstacks[ch].emit(R, BASE_PSI_IN, P.t_gemms, P.t_rows, &P.t_gemms_task);
         P.chunks[ch].t_gemm1 = P.t_gemms.size();
         P.chunks[ch].t_row1 = P.t_rows.size();
+        P.chunks[ch].t_task1 = P.t_gemms_task.size();
     }
 
     // --- L panels and flop counts per cohort ---
@@ -673,11 +702,14 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
     }
     P.n_cohorts = cohorts.size();
     // L*S products: per chunk and output block, one per run of the chunk's stripes that are
-    // adjacent both in the psi block and in the chunk's S panel (one per block with one chunk)
+    // adjacent both in the psi block and in the chunk's S panel (one per block with one chunk);
+    // and the reference's products, one per cohort (Cohort::contract, tasks.cpp:286-344), each
+    // added into its output block: the K range of the cohort in its block's stacked L panel and
+    // S panel, per run of the chunk's columns when the plan has several chunks
     for (uint32_t ch = 0; ch < n_chunks; ++ch) {
-        P.chunks[ch].c_gemm0 = P.c_gemms.size();
+        // runs of the chunk's stripes of every output block: (first stripe, width)
+        std::vector<std::vector<std::pair<size_t, uint64_t>>> runs(NL);
         for (uint32_t lb = 0; lb < NL; ++lb) {
-            uint32_t ls = left_i.sizes[lb];
             auto const& v = stripes[lb];
             for (size_t i = 0; i < v.size();) {
                 if (chunk_of[v[i].rb] != ch) {
@@ -690,32 +722,39 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                     w += v[j].width;
                     ++j;
                 }
-                P.c_gemms.push_back(gemm(int32_t(ls), int32_t(w), int32_t(ktot[lb]), BASE_LPANEL, int64_t(l_off[lb]),
-                                         int32_t(ls), BASE_S, int64_t(s_off[ch][lb] + loc[lb][i] * ktot[lb]),
-                                         int32_t(ktot[lb]), BASE_PSI_OUT,
-                                         int64_t(P.mps.rp_off[lb] + v[i].col * ls), int32_t(ls)));
+                runs[lb].push_back({i, w});
                 i = j;
             }
         }
+        P.chunks[ch].c_gemm0 = P.c_gemms.size();
+        for (uint32_t lb = 0; lb < NL; ++lb) {
+            uint32_t ls = left_i.sizes[lb];
+            for (auto const& r : runs[lb])
+                P.c_gemms.push_back(gemm(int32_t(ls), int32_t(r.second), int32_t(ktot[lb]), BASE_LPANEL, int64_t(l_off[lb]),
+                                         int32_t(ls), BASE_S, int64_t(s_off[ch][lb] + loc[lb][r.first] * ktot[lb]),
+                                         int32_t(ktot[lb]), BASE_PSI_OUT,
+                                         int64_t(P.mps.rp_off[lb] + stripes[lb][r.first].col * ls), int32_t(ls)));
+        }
         P.chunks[ch].c_gemm1 = P.c_gemms.size();
+        P.chunks[ch].c_coh0 = P.c_gemms_cohort.size();
+        for (size_t c = 0; c < cohorts.size(); ++c) {
+            auto const& rec = cohorts[c];
+            const uint32_t lb = rec.lb_out, ls_out = left_i.sizes[lb];
+            const int32_t kc = int32_t(uint64_t(rec.nb) * left_i.sizes[rec.lb_in]);
+            for (auto const& r : runs[lb])
+                P.c_gemms_cohort.push_back(gemm(int32_t(ls_out), int32_t(r.second), kc, BASE_LPANEL,
+                                                int64_t(l_off[lb] + uint64_t(ls_out) * koff[c]), int32_t(ls_out), BASE_S,
+                                                int64_t(s_off[ch][lb] + koff[c] + loc[lb][r.first] * ktot[lb]),
+                                                int32_t(ktot[lb]), BASE_PSI_OUT,
+                                                int64_t(P.mps.rp_off[lb] + stripes[lb][r.first].col * ls_out),
+                                                int32_t(ls_out)));
+        }
+        P.chunks[ch].c_coh1 = P.c_gemms_cohort.size();
     }
-    // the reference's L*S products, one per cohort, each added into its output block (Cohort::contract,
-    // tasks.cpp:286-344): the K range of the cohort in its block's stacked L panel and S panel
-    if (n_chunks == 1) {
-        // one chunk: the S panel columns are the psi block's columns
+    if (n_chunks == 1)  // one chunk: the S panel columns are the psi block's columns
         for (size_t lb = 0; lb < NL; ++lb)
             for (size_t i = 0; i < stripes[lb].size(); ++i)
                 require(loc[lb][i] == stripes[lb][i].col, "hpsi: psi block columns are not its stripes side by side");
-        for (size_t c = 0; c < cohorts.size(); ++c) {
-            auto const& rec = cohorts[c];
-            uint32_t ls_out = left_i.sizes[rec.lb_out];
-            int32_t kc = int32_t(uint64_t(rec.nb) * left_i.sizes[rec.lb_in]);
-            P.c_gemms_cohort.push_back(gemm(int32_t(ls_out), int32_t(P.mps.rp_cols[rec.lb_out]), kc, BASE_LPANEL,
-                                            int64_t(l_off[rec.lb_out] + uint64_t(ls_out) * koff[c]), int32_t(ls_out),
-                                            BASE_S, int64_t(s_off[0][rec.lb_out] + koff[c]), int32_t(ktot[rec.lb_out]),
-                                            BASE_PSI_OUT, int64_t(P.mps.rp_off[rec.lb_out]), int32_t(ls_out)));
-        }
-    }
     for (auto const& ch : P.chunks) {
         P.t_elems += ch.t_elems;
         P.s_elems += ch.s_elems;
@@ -1141,8 +1180,8 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
                 if (cand.empty()) continue;
                 std::sort(cand.begin(), cand.end());
                 cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-                std::unique_ptr<Wigner9jCache> w9j;
-                if (su2) w9j.reset(new Wigner9jCache(spin_of(lc_ket), spin_of(lc_bra), spin_of(rc_bra)));
+                std::shared_ptr<const Wigner9jCache> w9j;
+                if (su2) w9j = wigner9j_cache(spin_of(lc_ket), spin_of(lc_bra), spin_of(rc_bra));
                 // the terms of this unit, enumerated from the MPO columns b2 that have a right block
                 // and then put in the reference's order: MPO row b1, then b2, term, operator block
                 // (the b1-outer loop of rshtm_tasks, right_tasks.hpp:79-167)

@@ -218,10 +218,12 @@ struct CopyJob {
 // columns, which it addresses from offset 0: the chunks run one after the other on one T and one S
 // buffer of the largest chunk's size.  Ranges index the plan's tables.
 struct HpsiChunk {
-    size_t t_gemm0 = 0, t_gemm1 = 0;   // t_gemms (and t_gemms_task when the plan has one chunk)
+    size_t t_gemm0 = 0, t_gemm1 = 0;   // t_gemms
     size_t t_row0 = 0, t_row1 = 0;     // t_rows
     size_t line0 = 0, line1 = 0;       // lines
     size_t c_gemm0 = 0, c_gemm1 = 0;   // c_gemms
+    size_t t_task0 = 0, t_task1 = 0;   // t_gemms_task
+    size_t c_coh0 = 0, c_coh1 = 0;     // c_gemms_cohort
     uint64_t t_elems = 0, s_elems = 0;
 };
 
@@ -231,7 +233,9 @@ struct HpsiPlanHost {
     std::vector<GemmDesc> t_gemms, c_gemms;
     std::vector<GemmRow> t_rows;  // row table of the stacked T-GEMMs
     std::vector<GemmDesc> t_gemms_task;  // the same T-GEMMs, one problem per task (reference layout)
-    std::vector<GemmDesc> c_gemms_cohort;  // the L*S products per cohort, to be summed into psi'
+    // the L*S products per cohort (per cohort and run of the chunk's columns when the plan has
+    // several chunks), to be summed into psi'
+    std::vector<GemmDesc> c_gemms_cohort;
     HostTable<SLine> lines;
     HostTable<STerm> terms;
     std::vector<CopyJob> lpanel_jobs;

@@ -31,7 +31,7 @@ def lib():
         dp = P(C.c_double)
         idx = P(abi.IndexDesc)
         L.emu_hpsi.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, P(abi.BIndexDesc), dp, idx, idx, idx, dp, dp,
-                               C.c_int, P(C.c_uint64), C.c_uint64, P(C.c_uint64), P(C.c_uint64)]
+                               C.c_int, P(C.c_uint64), C.c_uint64, P(C.c_uint64), P(C.c_uint64), C.c_int]
         for f in (L.emu_lstep, L.emu_rstep):
             f.restype = C.c_void_p
             f.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, idx, idx, idx, dp, idx, idx, idx, dp, C.c_int]
@@ -51,11 +51,12 @@ last_chunks = 0
 last_ts = 0
 
 
-def hpsi(sym, mpo, L, R, layout, psi, small_max=-1, chunk_elems=0):
+def hpsi(sym, mpo, L, R, layout, psi, small_max=-1, chunk_elems=0, plain=False):
     """H*psi through the product's tables; small_max: tiles of at most this many elements take the
     small-tile term sum (MQ_OPT_SMALL_TILE_MAX; -1 = the automatic rule), their count is left in
     last_small_lines; chunk_elems: largest T + S of one chunk (0 = one chunk), the number of chunks
-    is left in last_chunks and the plan's T + S elements in last_ts"""
+    is left in last_chunks and the plan's T + S elements in last_ts; plain: run the reference-style
+    tables (MQ_PREC_FP64_PLAIN) instead of the grouped ones"""
     global last_small_lines, last_chunks, last_ts
     psi = np.ascontiguousarray(psi, np.float64)
     out = np.zeros_like(psi)
@@ -65,7 +66,7 @@ def hpsi(sym, mpo, L, R, layout, psi, small_max=-1, chunk_elems=0):
     ns, nc, nts = C.c_uint64(), C.c_uint64(), C.c_uint64()
     rc = lib().emu_hpsi(sym, C.byref(md), C.byref(Ld), _dp(Ldat), C.byref(Rd), _dp(Rdat), C.byref(li), C.byref(pi),
                         C.byref(ri), _dp(psi), _dp(out), int(small_max), C.byref(ns), int(chunk_elems), C.byref(nc),
-                        C.byref(nts))
+                        C.byref(nts), int(bool(plain)))
     if rc != 0:
         raise RuntimeError("emu_hpsi failed")
     last_small_lines = ns.value

@@ -185,7 +185,7 @@ extern "C" {
 int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const double* Ldata, const mq_bindex_desc* Rd,
              const double* Rdata, const mq_index_desc* li, const mq_index_desc* pi, const mq_index_desc* ri,
              const double* psi, double* out, int small_max, uint64_t* n_small_lines, uint64_t chunk_elems,
-             uint64_t* n_chunks, uint64_t* ts_elems) {
+             uint64_t* n_chunks, uint64_t* ts_elems, int plain) {
     try {
         MPO mpo(*md);
         BoundaryIndex L = boundary_index_from(*Ld), R = boundary_index_from(*Rd);
@@ -229,9 +229,40 @@ int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const dou
             base[BASE_S] = S.data();
             base[BASE_LPANEL] = Lp.data();
             base[BASE_BOUNDARY] = const_cast<double*>(Rdata);
-            run_gemms(tg, tr, base);
-            *n_small_lines += run_lines(cl, P.terms, T.data(), S.data(), ch.t_elems, ch.s_elems, small_max);
-            run_gemms(cg, {}, base);
+            if (!plain) {
+                run_gemms(tg, tr, base);
+                *n_small_lines += run_lines(cl, P.terms, T.data(), S.data(), ch.t_elems, ch.s_elems, small_max);
+                run_gemms(cg, {}, base);
+                continue;
+            }
+            // the reference-style tables: one product per T task, one gather per line, one product
+            // per (cohort, column run) added into psi' (cleared once before the first chunk)
+            if (&ch == &P.chunks.front()) std::fill(out, out + n, 0.0);
+            std::vector<GemmDesc> tt(P.t_gemms_task.begin() + int64_t(ch.t_task0),
+                                     P.t_gemms_task.begin() + int64_t(ch.t_task1));
+            std::vector<GemmDesc> cc(P.c_gemms_cohort.begin() + int64_t(ch.c_coh0),
+                                     P.c_gemms_cohort.begin() + int64_t(ch.c_coh1));
+            check_gemms(tt, sizes);
+            check_gemms(cc, sizes);
+            run_gemms(tt, {}, base);
+            for (SLine const& l : cl)
+                for (int e = 0; e < l.rows * l.cols; ++e) {
+                    double acc = 0.0;
+                    for (int32_t k = l.t0; k < l.t1; ++k) acc += P.terms[size_t(k)].alpha * T[size_t(P.terms[size_t(k)].t_off + e)];
+                    S[size_t(l.s_off + e % l.rows + int64_t(e / l.rows) * l.lds)] = acc;
+                }
+            for (GemmDesc const& g : cc) {
+                require(g.batch == 1 && !(g.flags & (GEMM_TRANS_A | GEMM_TRANS_B | GEMM_ROWMAP)), "plain L*S problem shape");
+                const double* A = base[g.base_a] + g.a;
+                const double* B = base[g.base_b] + g.b;
+                double* Cm = base[g.base_c] + g.c;
+                for (int nn = 0; nn < g.N; ++nn)
+                    for (int mm = 0; mm < g.M; ++mm) {
+                        double acc = 0.0;
+                        for (int k = 0; k < g.K; ++k) acc += A[mm + int64_t(k) * g.lda] * B[k + int64_t(nn) * g.ldb];
+                        Cm[mm + int64_t(nn) * g.ldc] += acc;
+                    }
+            }
         }
         return 0;
     } catch (std::exception const& e) {

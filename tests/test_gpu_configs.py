@@ -3,7 +3,8 @@
 * configuration 2 (24 orbitals, 24 electrons, TwoU1, M = 512, integrals PCG64(1002) rank 16):
   after one warm sweep, the middle two-site H*psi (FP64 to 1e-12, FP32 to 1e-5 relative), the same
   with the plan's output columns in chunks (MQ_OPT_CHUNK_MB 256: more than four chunks run on
-  buffers of less than half of T + S, also as a captured graph) and the left and right growth
+  buffers of less than half of T + S, also as a captured graph and with the reference-style
+  tables) and the left and right growth
   steps of the accepted tensors (1e-12).  The plan statistics show that the
   paths this problem exercises ran: term-sum blocks with more than 64 union tiles (several union
   chunks per block) and split-K GEMM reductions (K up to ~9e4);
@@ -54,20 +55,23 @@ def test_config2_site_problem_vs_oracle(ctx2u1, oracle_lib):
     plan.close()
     # the same site problem with its output columns in chunks of at most 256 MiB of T + S
     ctx2u1.set_option(abi.MQ_OPT_CHUNK_MB, 256)
+    ctx2u1.set_option(abi.MQ_OPT_PLAIN_TABLES, 1)
     try:
         cplan = HpsiPlan(ctx2u1, mpo, Ld, Rd, lay2)
     finally:
         ctx2u1.set_option(abi.MQ_OPT_CHUNK_MB, abi.MQ_CHUNK_AUTO)
+        ctx2u1.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
     cst = cplan.stats()
     errc = rel(cplan.apply(x), ref)
     errc32 = rel(cplan.apply(x, precision=abi.MQ_PREC_FP32), ref)
+    errp = rel(cplan.apply(x, precision=abi.MQ_PREC_FP64_PLAIN), ref)
     cplan.use_graphs(True)
     errg = rel(cplan.apply(x), ref)
     print(f"  chunked plan: {cst['n_chunks']} chunks, T+S buffers {cst['ts_buffer_elements']} of "
           f"{cst['t_elements'] + cst['s_elements']} elements; H*psi FP64 rel err {errc:.2e}, FP32 {errc32:.2e}, "
-          f"graph {errg:.2e}", flush=True)
+          f"graph {errg:.2e}, reference-style {errp:.2e}", flush=True)
     assert cst["n_chunks"] > 4 and cst["ts_buffer_elements"] < (cst["t_elements"] + cst["s_elements"]) // 2
-    assert errc < 1e-12 and errc32 < 1e-5 and errg < 1e-12
+    assert errc < 1e-12 and errc32 < 1e-5 and errg < 1e-12 and errp < 1e-12
     cplan.close()
     # growth of the middle tensors, left and right
     lay, data = d.lay[mid], d.data[mid]

@@ -51,9 +51,9 @@ def test_chain_emulated(oracle_lib, sym, seed, kw):
             assert rel_err(a, b) < 1e-12, (k, small_max)
             n_small[small_max] = n_small.get(small_max, 0) + emu.last_small_lines
         # output columns in chunks by right sector: one sector per chunk (budget 1 element), and
-        # chunks of about a third of the intermediates; chunks run on fresh NaN buffers and psi'
-        # starts as NaN, so a chunk reading another chunk's intermediates or a column left
-        # unwritten shows
+        # chunks of about a third of the intermediates, with the grouped and the reference-style
+        # tables; chunks run on fresh NaN buffers and psi' starts as NaN (grouped), so a chunk
+        # reading another chunk's intermediates or a column left unwritten shows
         chunks = []
         for budget in (1, "third"):
             if budget == "third":
@@ -61,6 +61,9 @@ def test_chain_emulated(oracle_lib, sym, seed, kw):
             a = emu.hpsi(sym, ch.mpos[k], Lo[k], Ro[k + 1], ch.layouts[k], psi, -1, chunk_elems=budget)
             assert rel_err(a, b) < 1e-12, (k, budget)
             chunks.append(emu.last_chunks)
+            # the reference-style tables of the same chunks
+            a = emu.hpsi(sym, ch.mpos[k], Lo[k], Ro[k + 1], ch.layouts[k], psi, -1, chunk_elems=budget, plain=True)
+            assert rel_err(a, b) < 1e-12, (k, budget, "plain")
         assert 1 <= chunks[1] <= chunks[0] <= len(ch.layouts[k].right)
         max_chunks = max(max_chunks, chunks[0])
     assert n_small[0] == 0 and 0 < n_small[6] < n_small[256]
