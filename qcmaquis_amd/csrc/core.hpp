@@ -22,6 +22,11 @@ struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
+// the message is made into a string only when the check fails (checks sit in loops that run tens
+// of millions of times per plan)
+inline void require(bool ok, const char* what) {
+    if (__builtin_expect(!ok, 0)) throw Error(MQ_ERR_INVALID, what);
+}
 inline void require(bool ok, const std::string& what) {
     if (!ok) throw Error(MQ_ERR_INVALID, what);
 }

@@ -17,11 +17,14 @@ namespace {
 // Output-tile accumulator of one cohort: units (sigma', ss2) with their term lines, in the
 // reference's insertion order (Cohort / SUnit, solver/tasks.h:50-190, tasks.cpp:72-196).
 struct CohortAcc {
+    struct Term {
+        double alpha;
+        uint32_t ti, col;  // T task and its column block
+    };
     struct Unit {
         uint32_t offset = 0, ms = 0;
         uint32_t s = UINT32_MAX;  // physical sector
-        std::vector<double> alpha;
-        std::vector<uint32_t> ti, col;
+        std::vector<Term> terms;
         std::vector<uint32_t> line_count, line_b;
         uint32_t pending = 0;
     };
@@ -30,6 +33,7 @@ struct CohortAcc {
     uint64_t stripe = 0;
     std::vector<uint32_t> bcount;  // terms per MPO index b (growth steps)
     uint64_t n_terms = 0;
+    std::vector<uint32_t> dirty;   // units with pending terms, in order of their first term
 
     CohortAcc(Index const& phys, uint32_t mpodim) : sfold(phys.size()), bcount(mpodim, 0) {
         uint32_t acc = 0;
@@ -48,21 +52,22 @@ struct CohortAcc {
         }
     }
     void push(size_t s, uint32_t ss2, double alpha, uint32_t ti, uint32_t col) {
-        Unit& u = units[sfold[s] + ss2];
-        u.alpha.push_back(alpha);
-        u.ti.push_back(ti);
-        u.col.push_back(col);
-        ++u.pending;
+        const uint32_t ui = sfold[s] + ss2;
+        Unit& u = units[ui];
+        u.terms.push_back(Term{alpha, ti, col});
+        if (!u.pending++) dirty.push_back(ui);
         ++n_terms;
     }
+    // closes the line of MPO index b in every unit that received terms since the last line
     void add_line(uint32_t b) {
-        for (auto& u : units)
-            if (u.pending) {
-                u.line_count.push_back(u.pending);
-                u.line_b.push_back(b);
-                if (b < bcount.size()) bcount[b] += u.pending;
-                u.pending = 0;
-            }
+        for (uint32_t ui : dirty) {
+            Unit& u = units[ui];
+            u.line_count.push_back(u.pending);
+            u.line_b.push_back(b);
+            if (b < bcount.size()) bcount[b] += u.pending;
+            u.pending = 0;
+        }
+        dirty.clear();
     }
 };
 
@@ -134,7 +139,9 @@ inline void op_iterate(OpBlock const& blk, const double cpl[4], CohortAcc& coh, 
         uint32_t ti = tl.get(mps_offset + uint64_t(e.row) * m2_size, ci);
         if (ti != UINT32_MAX) {
             // cohorts of one T list may be built concurrently: every writer stores the same flag
-            __atomic_store_n(&tl.tasks[ti].used, true, __ATOMIC_RELAXED);
+            // (read first, so the threads do not keep invalidating each other's copy of the line)
+            if (!__atomic_load_n(&tl.tasks[ti].used, __ATOMIC_RELAXED))
+                __atomic_store_n(&tl.tasks[ti].used, true, __ATOMIC_RELAXED);
             coh.push(s, e.col, alpha, ti, boundary_col);
         }
     }
@@ -211,7 +218,7 @@ void emit_lines(size_t n_cohorts, HostTable<SLine>& lines, HostTable<STerm>& ter
         for (auto const& u : units_of(size_t(c))) {
             if (!keep(size_t(c), u)) continue;
             a += unit_lines(u, nrows_of(size_t(c)));
-            b += u.ti.size();
+            b += u.terms.size();
         }
         nl[size_t(c)] = a;
         nt[size_t(c)] = b;
@@ -469,7 +476,6 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                                 if (!R.has_block(ci_right, b2)) continue;
                                 Charge rc_in = right_i.charges[rb_in];
                                 uint32_t rs_in = right_i.sizes[rb_in];
-                                uint64_t in_offset = pb_off[size_t(br.s_in) * NR + rb_in];
                                 int64_t right_offset = R.offset(ci_right, b2);
                                 double cpl[4];
                                 double scale = R.conjugate_scale(ci_right, b2) * mpo.terms[t].scale *
@@ -478,7 +484,7 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                                     w9j->set_scale(A, K, Ap, spin_of(rc_in), scale, cpl);
                                 else
                                     cpl[0] = cpl[1] = cpl[2] = cpl[3] = scale;
-                                op_iterate(blk, cpl, coh, s, rs_in, tl, in_offset, ci_right,
+                                op_iterate(blk, cpl, coh, s, rs_in, tl, pb_off[size_t(br.s_in) * NR + rb_in], ci_right,
                                            uint32_t(right_offset / rs_in));
                             }
                         }
@@ -644,7 +650,7 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
         }
         if (uint64_t(rec.nb) * ls_in * ls_out != 0) P.lpanel_jobs.push_back(job);
         l_used.insert(rec.ci_eff);
-        for (auto& u : rec.acc.units) P.flops.flops_total += 2ull * ls_in * u.ms * u.alpha.size();
+        for (auto& u : rec.acc.units) P.flops.flops_total += 2ull * ls_in * u.ms * u.terms.size();
         uint64_t g = 2ull * ls_out * ls_in * rec.nb * rec.acc.stripe;
         P.flops.flops_total += g;
         P.flops.flops_gemm += g;
@@ -674,9 +680,9 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                     line.cols = int32_t(u.ms);
                     line.t0 = int32_t(terms.size());
                     for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
-                        TTask const& t = tlists[rec.lb_in].tasks[u.ti[k]];
-                        check_tile(t, uint64_t(u.col[k]) * ls_in, ls_in, u.ms);
-                        terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * ls_in)});
+                        TTask const& t = tlists[rec.lb_in].tasks[u.terms[k].ti];
+                        check_tile(t, uint64_t(u.terms[k].col) * ls_in, ls_in, u.ms);
+                        terms.push_back(STerm{u.terms[k].alpha, int64_t(t.t_off + uint64_t(u.terms[k].col) * ls_in)});
                     }
                     line.t1 = int32_t(terms.size());
                     seek += u.line_count[l];
@@ -1006,7 +1012,7 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
         require(stripe == bra.lp_rows[rec.rb_bra], "lstep: cohort stripe != bra left-paired rows");
         sbase[c] = P.s_elems;
         P.s_elems += round_up(stripe * rec.nrows * rs_ket, 32);
-        for (auto& u : rec.acc.units) P.flops.flops_total += 2ull * rs_ket * u.ms * u.alpha.size();
+        for (auto& u : rec.acc.units) P.flops.flops_total += 2ull * rs_ket * u.ms * u.terms.size();
         uint32_t ci_new = new_ci[c];
         P.c_gemms.push_back(gemm(int32_t(rs_bra), int32_t(rec.nrows * rs_ket), int32_t(stripe), BASE_BRA,
                                  int64_t(bra.lp_off[rec.rb_bra]), int32_t(stripe), BASE_S, int64_t(sbase[c]),
@@ -1033,9 +1039,9 @@ GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, M
                 line.cols = int32_t(rs_ket);
                 line.t0 = int32_t(terms.size());
                 for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
-                    TTask const& t = tlists[rec.rb_ket].tasks[u.ti[k]];
-                    check_tile(t, uint64_t(u.col[k]) * rs_ket, u.ms, rs_ket);
-                    terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * rs_ket)});
+                    TTask const& t = tlists[rec.rb_ket].tasks[u.terms[k].ti];
+                    check_tile(t, uint64_t(u.terms[k].col) * rs_ket, u.ms, rs_ket);
+                    terms.push_back(STerm{u.terms[k].alpha, int64_t(t.t_off + uint64_t(u.terms[k].col) * rs_ket)});
                 }
                 line.t1 = int32_t(terms.size());
                 seek += u.line_count[l];
@@ -1285,7 +1291,7 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
         uint64_t lds = uint64_t(rec.nrows) * ls_ket;
         sbase[c] = P.s_elems;
         P.s_elems += round_up(lds * stripe, 32);
-        for (auto& u : rec.acc.units) P.flops.flops_total += 2ull * ls_ket * u.ms * u.alpha.size();
+        for (auto& u : rec.acc.units) P.flops.flops_total += 2ull * ls_ket * u.ms * u.terms.size();
         uint32_t ci_new = new_ci[c];
         P.c_gemms.push_back(gemm(int32_t(ls_ket), int32_t(ls_bra), int32_t(stripe), BASE_S, int64_t(sbase[c]),
                                  int32_t(lds), BASE_BRA, int64_t(bra.rp_off[rec.lb_bra]), int32_t(ls_bra), BASE_NEW,
@@ -1312,9 +1318,9 @@ GrowPlanHost build_rstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& R, M
                 line.cols = int32_t(u.ms);
                 line.t0 = int32_t(terms.size());
                 for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
-                    TTask const& t = tlists[rec.lb_ket].tasks[u.ti[k]];
-                    check_tile(t, uint64_t(u.col[k]) * ls_ket, ls_ket, u.ms);
-                    terms.push_back(STerm{u.alpha[k], int64_t(t.t_off + uint64_t(u.col[k]) * ls_ket)});
+                    TTask const& t = tlists[rec.lb_ket].tasks[u.terms[k].ti];
+                    check_tile(t, uint64_t(u.terms[k].col) * ls_ket, ls_ket, u.ms);
+                    terms.push_back(STerm{u.terms[k].alpha, int64_t(t.t_off + uint64_t(u.terms[k].col) * ls_ket)});
                 }
                 line.t1 = int32_t(terms.size());
                 seek += u.line_count[l];
