@@ -505,7 +505,7 @@ class ProfilerRegion:
         return False
 
 
-def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=2):
+def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4):
     """Three two-site DMRG runs of n_sweeps sweeps each from one common start (a seeded random MPS
     after warm_sweeps FP64 sweeps), Davidson 8 iterations and tol 1e-8 per step: FP64 with the
     grouped launches, mixed precision (FP32 contraction, FP64 refinement of every eigenvector) and
@@ -784,9 +784,12 @@ def main():
     plain = None
     if not args.no_plain:
         plan.close()
+        # one chunk: one dgemm per cohort over all its columns, as the reference issues them
         ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 1)
+        ctx.set_option(abi.MQ_OPT_CHUNK_MB, 0)
         plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
         ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
+        ctx.set_option(abi.MQ_OPT_CHUNK_MB, abi.MQ_CHUNK_AUTO if args.chunk_mb is None else args.chunk_mb)
         outp = torch.empty_like(psi)
         plan.apply_device(psi.data_ptr(), outp.data_ptr(), abi.MQ_PREC_FP64_PLAIN)
         ctx.sync()

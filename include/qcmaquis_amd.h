@@ -310,6 +310,17 @@ int mq_davidson_noalloc(mq_hpsi_plan_t plan, const double* psi0, int n_ortho, co
 int mq_davidson_batch(mq_hpsi_plan_t plan, const double* psi0, double tol, int max_iter, int precision,
                       double* evals, double* psi_out, int32_t* iterations, mq_davidson_stats_t* stats);
 
+/* ---- two-site split (reference: TwoSiteTensor::split_mps_l2r / split_mps_r2l, twositetensor.hpp:
+ *      155-200, with svd_truncate, block_matrix_algorithms.h:186-244: one SVD per symmetry block) ---- */
+/* Thin SVD of n dense blocks on the device: block i is rows[i] x cols[i], column-major, the blocks
+ * concatenated in a (host); per block, with k = min(rows, cols): s gets its k singular values
+ * (descending), u its rows x k left singular vectors and vt the k x cols right ones (column-major),
+ * each concatenated over the blocks (host arrays).  One rocSOLVER dgesvd per block on the
+ * context stream; synchronous.  The caller truncates (estimate_truncation) and forms the site
+ * tensors. */
+int mq_block_svd(mq_ctx_t ctx, int n, const int32_t* rows, const int32_t* cols, const double* a, double* s,
+                 double* u, double* vt);
+
 /* ---- quantum-chemistry Hamiltonian MPO (host only; input producer of the path) -------------
  * reference: models/chem/2u1/model.hpp (create_terms), generate_mpo/tagged_mpo_maker_optim.hpp,
  * mp_tensors/ts_ops.hpp:96-185 (two-site MPO).  h1: L x L row-major h_pq; eri: L^4 row-major

@@ -45,7 +45,7 @@ def build_library(verbose=False):
         list(ex.map(lambda j: _run(j[0]), jobs))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-fopenmp", "-o", tmp] + [o for _, o in jobs] +
-         ["-L/opt/rocm/lib", "-lrocblas"])
+         ["-L/opt/rocm/lib", "-lrocblas", "-lrocsolver"])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

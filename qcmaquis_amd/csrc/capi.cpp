@@ -3,6 +3,7 @@
 // codes with the message kept on the context.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 #include <omp.h>
 
 #include <algorithm>
@@ -76,8 +77,9 @@ void hipcheck(hipError_t e, const char* what) {
 // plans of similar sizes at every site, and hipMalloc / hipFree of gigabyte buffers cost
 // milliseconds each (hipFree also waits for the device).  A freed block serves a later request
 // of at most its size and at least 4/5 of it on the same device; at most CACHE_CAP bytes are kept
-// (the largest idle blocks are released first), and all idle blocks are released before an
-// allocation that the device cannot satisfy is given up.  A block enters the cache after a device
+// (the smallest idle blocks are released first: the large plan buffers of one site are the ones
+// the next site can reuse, and small blocks are cheap to allocate again), and all idle blocks are
+// released before an allocation that the device cannot satisfy is given up.  A block enters the cache after a device
 // synchronisation (as hipFree waits for the device), so a reused block is idle; like a fresh
 // hipMalloc it holds undefined contents.
 struct DevCache {
@@ -222,12 +224,12 @@ void dev_free(void* p) {
     int cur = 0;
     (void)hipGetDevice(&cur);
     while (c.idle_bytes > CACHE_CAP) {
-        auto last = std::prev(c.idle.end());
-        if (last->second.first != cur) (void)hipSetDevice(last->second.first);
-        (void)hipFree(last->second.second);
-        if (last->second.first != cur) (void)hipSetDevice(cur);
-        c.idle_bytes -= last->first;
-        c.idle.erase(last);
+        auto first = c.idle.begin();
+        if (first->second.first != cur) (void)hipSetDevice(first->second.first);
+        (void)hipFree(first->second.second);
+        if (first->second.first != cur) (void)hipSetDevice(cur);
+        c.idle_bytes -= first->first;
+        c.idle.erase(first);
     }
 }
 
@@ -1168,14 +1170,20 @@ static void ensure_fp32(mq_hpsi_plan_t p) {
 // run on the FP32 matrix cores with FP32 T and S, and psi' is written in FP64.
 // One rocBLAS dgemm (or strided-batched dgemm) per problem on the context stream: the launch
 // structure of the reference's per-task cuBLAS calls (tasks.cpp:286-344, 650-752).
-static void library_gemms(mq_ctx_t ctx, const GemmDesc* first, const GemmDesc* last, Bases const& b,
-                          double beta = 0.0) {
+// the context's library handle (made on first use) on the context stream
+static rocblas_handle blas_handle(mq_ctx_t ctx) {
     if (!ctx->blas) {
         require(rocblas_create_handle(&ctx->blas) == rocblas_status_success, "rocblas_create_handle failed");
         require(rocblas_set_pointer_mode(ctx->blas, rocblas_pointer_mode_host) == rocblas_status_success,
                 "rocblas_set_pointer_mode failed");
     }
     require(rocblas_set_stream(ctx->blas, ctx->stream) == rocblas_status_success, "rocblas_set_stream failed");
+    return ctx->blas;
+}
+
+static void library_gemms(mq_ctx_t ctx, const GemmDesc* first, const GemmDesc* last, Bases const& b,
+                          double beta = 0.0) {
+    blas_handle(ctx);
     const double one = 1.0;
     for (const GemmDesc* gp = first; gp != last; ++gp) {
         GemmDesc const& g = *gp;
@@ -1347,6 +1355,64 @@ int mq_hpsi_timed(mq_hpsi_plan_t p, const double* in, double* out, int precision
             throw;
         }
         for (auto e : ev) (void)hipEventDestroy(e);
+    });
+}
+
+// Thin SVD of every block on the device, one rocSOLVER dgesvd per block on the context stream
+// (U and V^T of the min(rows, cols) singular vectors); inputs and outputs are host arrays.
+int mq_block_svd(mq_ctx_t ctx, int n, const int32_t* rows, const int32_t* cols, const double* a, double* s,
+                 double* u, double* vt) {
+    return guarded(ctx, [&] {
+        require(n >= 0 && (n == 0 || (rows && cols && a && s && u && vt)), "mq_block_svd: null argument");
+        if (n == 0) return;
+        std::vector<uint64_t> oa(size_t(n) + 1, 0), os(size_t(n) + 1, 0), ou(size_t(n) + 1, 0), ov(size_t(n) + 1, 0);
+        int32_t kmax = 1;
+        for (int i = 0; i < n; ++i) {
+            require(rows[i] > 0 && cols[i] > 0, "mq_block_svd: empty block");
+            const uint64_t r = uint64_t(rows[i]), c = uint64_t(cols[i]), k = std::min(r, c);
+            oa[size_t(i) + 1] = oa[size_t(i)] + r * c;
+            os[size_t(i) + 1] = os[size_t(i)] + k;
+            ou[size_t(i) + 1] = ou[size_t(i)] + r * k;
+            ov[size_t(i) + 1] = ov[size_t(i)] + k * c;
+            kmax = std::max<int32_t>(kmax, int32_t(k));
+        }
+        hipStream_t st = ctx->stream;
+        rocblas_handle h = blas_handle(ctx);
+        double *dA = nullptr, *dS = nullptr, *dU = nullptr, *dV = nullptr, *dE = nullptr;
+        rocblas_int* dinfo = nullptr;
+        auto release = [&] {
+            (void)hipStreamSynchronize(st);
+            dev_free(dA); dev_free(dS); dev_free(dU); dev_free(dV); dev_free(dE); dev_free(dinfo);
+        };
+        try {
+            dA = dev_alloc<double>(oa[size_t(n)]);
+            dS = dev_alloc<double>(os[size_t(n)]);
+            dU = dev_alloc<double>(ou[size_t(n)]);
+            dV = dev_alloc<double>(ov[size_t(n)]);
+            dE = dev_alloc<double>(uint64_t(kmax));
+            dinfo = dev_alloc<rocblas_int>(uint64_t(n));
+            upload_bytes(dA, a, oa[size_t(n)] * sizeof(double), st);
+            for (int i = 0; i < n; ++i) {
+                const rocblas_int r = rows[i], c = cols[i], k = std::min(r, c);
+                require(rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, r, c, dA + oa[size_t(i)], r,
+                                         dS + os[size_t(i)], dU + ou[size_t(i)], r, dV + ov[size_t(i)], k, dE,
+                                         rocblas_outofplace, dinfo + i) == rocblas_status_success,
+                        "mq_block_svd: rocsolver_dgesvd failed");
+            }
+            std::vector<rocblas_int> info(static_cast<size_t>(n));
+            hipcheck(hipMemcpyAsync(s, dS, os[size_t(n)] * sizeof(double), hipMemcpyDeviceToHost, st), "download");
+            hipcheck(hipMemcpyAsync(u, dU, ou[size_t(n)] * sizeof(double), hipMemcpyDeviceToHost, st), "download");
+            hipcheck(hipMemcpyAsync(vt, dV, ov[size_t(n)] * sizeof(double), hipMemcpyDeviceToHost, st), "download");
+            hipcheck(hipMemcpyAsync(info.data(), dinfo, size_t(n) * sizeof(rocblas_int), hipMemcpyDeviceToHost, st),
+                     "download");
+            hipcheck(hipStreamSynchronize(st), "block svd");
+            for (int i = 0; i < n; ++i)
+                require(info[size_t(i)] == 0, "mq_block_svd: the SVD of a block did not converge");
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
     });
 }
 

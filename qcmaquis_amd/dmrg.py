@@ -16,7 +16,7 @@ import time
 import numpy as np
 
 from . import abi
-from .engine import Context, DeviceBoundary, DeviceMPO, DeviceMPS, HpsiPlan, lstep, rstep
+from .engine import Context, DeviceBoundary, DeviceMPO, DeviceMPS, HpsiPlan, block_svd, lstep, rstep
 from .model import Index, MPSLayout, qadd, qsub
 
 
@@ -152,12 +152,13 @@ def block_svds(blocks):
     return {m: out[m] for m in blocks}
 
 
-def split_two(lay2, psi2, phys, off12, Mmax, cutoff, to_right, details=None, sym=abi.MQ_SYM_TWOU1):
+def split_two(lay2, psi2, phys, off12, Mmax, cutoff, to_right, details=None, sym=abi.MQ_SYM_TWOU1, svd_fn=None):
     """SVD split of a two-site tensor (sequentially coupled basis for SU2U1); to_right: A = U
     left-normalised, B = S V^T (split_mps_l2r), else A = U S, B = V^T right-normalised
-    (split_mps_r2l); details (a dict) receives the truncation record"""
+    (split_mps_r2l); details (a dict) receives the truncation record; svd_fn: the block SVD
+    ({m: X} -> {m: (U, s, Vt)}; default: LAPACK on the host cores, engine.block_svd on the device)"""
     mats = split_matrices(lay2, psi2, phys, off12, sym)
-    svd = block_svds({m: X for m, (X, _, _) in mats.items()})
+    svd = (svd_fn or block_svds)({m: X for m, (X, _, _) in mats.items()})
     keeps, tw = estimate_truncation({m: s[1] for m, s in svd.items()}, Mmax, cutoff)
     if details is not None:
         details.update(truncation_details({m: s[1] for m, s in svd.items()}, Mmax, cutoff))
@@ -327,11 +328,18 @@ class TwoSiteDMRG:
     the Davidson solve and back before the split (su2.TwoSiteReduction)."""
 
     def __init__(self, ham, n_up, n_dn, M, ctx=None, init_m=8, seed=42, tol=1e-8, max_iter=8, cutoff=1e-16,
-                 verbose=False, precision=abi.MQ_PREC_FP64, mps=None, resultfile=None, first_sweep=0):
+                 verbose=False, precision=abi.MQ_PREC_FP64, mps=None, resultfile=None, first_sweep=0,
+                 device_split=False, snapshots=(), snapshot_dir="."):
         """precision: arithmetic of the Davidson H*psi (MQ_PREC_FP32 = FP32 contraction with an FP64
         refinement of each eigenvector, see mq_davidson); mps: (layouts, data) to start from (e.g.
         results.load_checkpoint) instead of a random state; resultfile: HDF5 result file that every
-        sweep appends to (results.ResultArchive); first_sweep: number of the first sweep"""
+        sweep appends to (results.ResultArchive); first_sweep: number of the first sweep;
+        device_split: the block SVDs of the two-site split on the device (mq_block_svd); snapshots:
+        (2 sweep + direction, site) pairs whose site problem is saved before its solve, as
+        snapshot_dir/snapshot_<twosweep>_<site>.h5 (snapshot.save; ts_optimize.hpp:185-213)"""
+        self.device_split = device_split
+        self.snapshots = {(int(a), int(b)) for a, b in snapshots}
+        self.snapshot_dir = snapshot_dir
         self.ham, self.M, self.tol, self.max_iter, self.cutoff = ham, M, tol, max_iter, cutoff
         self.precision = precision
         self.verbose = verbose
@@ -392,6 +400,11 @@ class TwoSiteDMRG:
         mpo = self.ts_mpo(i)
         t1 = time.perf_counter()
         self.timing["mpo"] += t1 - tm
+        twosweep = 2 * self.sweep_number + (0 if to_right else 1)
+        if (twosweep, i) in self.snapshots:
+            from . import snapshot
+            snapshot.save(os.path.join(self.snapshot_dir, "snapshot_%d_%d.h5" % (twosweep, i)), self.sym, mpo.spec,
+                          self.left[i], self.right[i + 2], lay2, psi2, i, twosweep, self.ham.ecore)
         plan = HpsiPlan(self.ctx, mpo, self.left[i], self.right[i + 2], lay2)
         t2 = time.perf_counter()
         e, psi, st = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter, precision=self.precision)
@@ -438,7 +451,9 @@ class TwoSiteDMRG:
                 red = TwoSiteReduction(lay2, self.phys, self.off12)
             psi = red.unreduce(psi)
         trunc = {}
-        la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right, trunc, self.sym)
+        svd_fn = (lambda blocks: block_svd(self.ctx, blocks)) if self.device_split else None
+        la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right, trunc, self.sym,
+                                     svd_fn)
         t4 = time.perf_counter()
         self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1] = la, a, lb, b
         if to_right:

@@ -178,6 +178,29 @@ def rstep(ctx, mpo, right, bra, ket, symmetric=True, precision=abi.MQ_PREC_FP64)
     return DeviceBoundary(ctx, h)
 
 
+def block_svd(ctx, blocks):
+    """thin SVD of every block on the device (mq_block_svd, one rocSOLVER dgesvd per block; the
+    decomposition of the two-site split, svd_truncate, block_matrix_algorithms.h:186-244):
+    {key: X} -> {key: (U, s, Vt)} with U rows x k, s descending, Vt k x cols, k = min(rows, cols)"""
+    keys = list(blocks)
+    if not keys:
+        return {}
+    rows = np.array([blocks[k].shape[0] for k in keys], np.int32)
+    cols = np.array([blocks[k].shape[1] for k in keys], np.int32)
+    ks = np.minimum(rows, cols).astype(np.int64)
+    a = np.ascontiguousarray(np.concatenate([np.asarray(blocks[k], np.float64).ravel(order="F") for k in keys]))
+    s = np.empty(int(ks.sum()))
+    u = np.empty(int((rows.astype(np.int64) * ks).sum()))
+    vt = np.empty(int((ks * cols).sum()))
+    ctx.check(ctx.lib.mq_block_svd(ctx.h, len(keys), rows.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   cols.ctypes.data_as(C.POINTER(C.c_int32)), _dp(a), _dp(s), _dp(u), _dp(vt)))
+    out, o_s, o_u, o_v = {}, 0, 0, 0
+    for key, r, c, k in zip(keys, rows.tolist(), cols.tolist(), ks.tolist()):
+        out[key] = (u[o_u:o_u + r * k].reshape(k, r).T, s[o_s:o_s + k], vt[o_v:o_v + k * c].reshape(c, k).T)
+        o_s, o_u, o_v = o_s + k, o_u + r * k, o_v + k * c
+    return out
+
+
 class HpsiPlan(_Handle):
     """create_contraction_schedule + the device work lists of one site (mq_hpsi_plan_create)."""
 
