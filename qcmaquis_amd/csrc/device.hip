@@ -44,18 +44,19 @@ template <> struct Mma<float> {
 
 template <class T> __device__ inline T ld_or_zero(const T* p, bool ok) { return ok ? *p : T(0); }
 
-// One BM x BN output tile (BM = 32 FM, BN = 32 FN) per 256-thread workgroup; the four waves form a
-// 2 x 2 grid of 16 FM x 16 FN wave tiles, each FM x FN matrix-core tiles.  Operands of type TI,
+// One BM x BN output tile (BM = 32 FM, BN = 16 FN WN) per workgroup of 2 WN waves; the waves form a
+// 2 x WN grid of 16 FM x 16 FN wave tiles, each FM x FN matrix-core tiles.  Operands of type TI,
 // computed in TI, output converted to TO.  The tile reduces over [k0, k1) and writes C, or its
 // partial tile (type TI) into the workspace (split K).
-template <int FM, int FN, class TI, class TO, int KS = BK, int MINB = 1>
-__global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restrict__ probs,
-                                                         const GemmRow* __restrict__ rows,
-                                                         const GemmTile* __restrict__ tiles, Bases bases,
-                                                         TI* __restrict__ ws) {
-    constexpr int BM = 32 * FM, BN = 32 * FN;
-    constexpr int NA = BM * KS / 256, NB = BN * KS / 256;  // operand elements staged per thread and K slice
-    static_assert(KS % 4 == 0 && NA * 256 == BM * KS && NB * 256 == BN * KS, "K slice geometry");
+template <int FM, int FN, class TI, class TO, int KS = BK, int MINB = 1, int WN = 2>
+__global__ __launch_bounds__(128 * WN, MINB) void gemm_kernel(const GemmDesc* __restrict__ probs,
+                                                              const GemmRow* __restrict__ rows,
+                                                              const GemmTile* __restrict__ tiles, Bases bases,
+                                                              TI* __restrict__ ws) {
+    constexpr int THREADS = 128 * WN;
+    constexpr int BM = 32 * FM, BN = 16 * FN * WN;
+    constexpr int NA = BM * KS / THREADS, NB = BN * KS / THREADS;  // operand elements staged per thread and K slice
+    static_assert(KS % 4 == 0 && NA * THREADS == BM * KS && NB * THREADS == BN * KS, "K slice geometry");
     __shared__ TI As[KS][BM + LDS_PAD];
     __shared__ TI Bs[KS][BN + LDS_PAD];
     typedef typename Mma<TI>::acc_t acc_t;
@@ -72,12 +73,12 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
     const TI* B = static_cast<const TI*>(bases.p[g.base_b]) + g.b + int64_t(tile.batch) * g.sb;
     // stacked problem: the tile's rows are staged in LDS once (the A loads and the C stores look them
     // up there); A is not transposed, so each thread loads from one row (tid % BM) of the tile
-    // (stacked problems only get tile heights that divide 256)
+    // (stacked problems only get tile heights that divide the workgroup size)
     __shared__ GemmRow Rs[BM];
     const TI* a_row = A;
     int64_t a_rld = 0;
     if (rmap) {
-        for (int i = tid; i < BM; i += 256)
+        for (int i = tid; i < BM; i += THREADS)
             if (m0 + i < g.M) Rs[i] = rows[g.a + m0 + i];
         __syncthreads();
         if (m0 + tid % BM < g.M) {
@@ -91,13 +92,13 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
     int a_m[NA], a_k[NA], b_k[NB], b_n[NB];
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
-        const int idx = tid + 256 * i;
+        const int idx = tid + THREADS * i;
         if (!ta) { a_m[i] = idx % BM; a_k[i] = idx / BM; }
         else     { a_k[i] = idx % KS; a_m[i] = idx / KS; }
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-        const int idx = tid + 256 * i;
+        const int idx = tid + THREADS * i;
         if (!tb) { b_k[i] = idx % KS; b_n[i] = idx / KS; }
         else     { b_n[i] = idx % BN; b_k[i] = idx / BN; }
     }
@@ -184,14 +185,10 @@ __global__ __launch_bounds__(256, MINB) void gemm_kernel(const GemmDesc* __restr
 }
 
 // C tile = sum of its split partials, added in split order; blockIdx.y selects 256 elements of
-// the tile, so one output tile is reduced by up to 32 workgroups
-constexpr int REDUCE_CHUNKS = 32;  // 8192 (largest tile) / 256
-static_assert(gemm_bm(GEMM_128x64) * gemm_bn(GEMM_128x64) <= REDUCE_CHUNKS * 256 &&
-                  gemm_bm(GEMM_32x128) * gemm_bn(GEMM_32x128) <= REDUCE_CHUNKS * 256 &&
-                  gemm_bm(GEMM_64x64) * gemm_bn(GEMM_64x64) <= REDUCE_CHUNKS * 256 &&
-                  gemm_bm(GEMM_96x64) * gemm_bn(GEMM_96x64) <= REDUCE_CHUNKS * 256 &&
-                  gemm_bm(GEMM_64x128) * gemm_bn(GEMM_64x128) <= REDUCE_CHUNKS * 256,
-              "reduce grid covers every tile shape");
+// the tile, so one output tile is reduced by up to 64 workgroups
+constexpr int REDUCE_CHUNKS = 64;  // 16384 (largest tile) / 256
+constexpr bool reduce_covers(int sh) { return sh == GEMM_N_SHAPES || (gemm_bm(sh) * gemm_bn(sh) <= REDUCE_CHUNKS * 256 && reduce_covers(sh + 1)); }
+static_assert(reduce_covers(0), "reduce grid covers every tile shape");
 template <class TW, class TO>
 __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __restrict__ probs,
                                                           const GemmReduce* __restrict__ jobs, Bases bases,
@@ -554,6 +551,9 @@ void launch_gemm_t(const GemmDesc* probs, const GemmRow* rows, const GemmTile* c
             else if (sh == GEMM_64x128)
                 hipLaunchKernelGGL((gemm_kernel<2, 4, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    rows, tiles[sh] + off, bases, w);
+            else if (sh == GEMM_128x128)
+                hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO, GEMM_KS, GEMM_MINB / 2, 4>), dim3(unsigned(cnt)), dim3(512), 0,
+                                   stream, probs, rows, tiles[sh] + off, bases, w);
             else
                 hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    rows, tiles[sh] + off, bases, w);

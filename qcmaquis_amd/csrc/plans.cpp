@@ -1652,9 +1652,11 @@ GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t tar
     // along K anyway: then the wider tile halves the A re-reads at no cost in parallelism
     bool wide = false;
     auto shape_of = [&](GemmDesc const& g) {
+        const bool w = wide && g.N > 64 && !(g.flags & GEMM_ROWMAP);
         return g.M <= 32 ? GEMM_32x128
              : g.M <= 64 ? ((wide && g.N > 64) ? GEMM_64x128 : GEMM_64x64)
-             : (g.M <= 96 && !(g.flags & GEMM_ROWMAP)) ? GEMM_96x64 : GEMM_128x64;
+             : (g.M <= 96 && !(g.flags & GEMM_ROWMAP)) ? GEMM_96x64
+             : w ? GEMM_128x128 : GEMM_128x64;
     };
     auto tiles_of = [&](GemmDesc const& g) {
         int sh = shape_of(g);

@@ -149,20 +149,28 @@ constexpr int SMALL_LINE_LANES = 256;
 // work items cover SBLOCK_ELEMS consecutive tile elements of one block
 constexpr int SBLOCK_ELEMS = 256;
 
-// Execution schedule of a grouped GEMM stage.  Output tiles come in five shapes (BM x BN):
-// 32x128 for M <= 32, 64x64 for M <= 64 (64x128 when N > 64 and the stage is split along K), 96x64
-// for M <= 96 (problems without a row table), 128x64 above; so a problem with a short M edge is not padded to a taller tile and the B panel of a tall
-// problem is read once.  When the stage has too few
+// Execution schedule of a grouped GEMM stage.  Output tiles come in six shapes (BM x BN):
+// 32x128 for M <= 32, 64x64 for M <= 64, 96x64 for M <= 96 (problems without a row table), 128x64
+// above; in a stage that is split along K (fewer output tiles than the device takes) problems
+// with N > 64 take the 128-column tiles 64x128 and (M > 96, without a row table) 128x128, so the A
+// panel is read once per 128 output columns (128x128 on eight waves).  So a
+// problem with a short M edge is not padded to a taller tile and the B panel of a tall problem is
+// read once.  When the stage has too few
 // tiles to fill the device, long reductions are split along K: each split tile writes its
 // partial BM x BN tile (column-major, ld BM) into a workspace and a reduce job adds the
 // partials in split order into C, so the result does not depend on scheduling.
-enum GemmShape { GEMM_32x128 = 0, GEMM_64x64 = 1, GEMM_128x64 = 2, GEMM_96x64 = 3, GEMM_64x128 = 4, GEMM_N_SHAPES = 5 };
+enum GemmShape {
+    GEMM_32x128 = 0, GEMM_64x64 = 1, GEMM_128x64 = 2, GEMM_96x64 = 3, GEMM_64x128 = 4, GEMM_128x128 = 5,
+    GEMM_N_SHAPES = 6
+};
 constexpr int gemm_bm(int shape) {
     return shape == GEMM_32x128 ? 32
          : (shape == GEMM_64x64 || shape == GEMM_64x128) ? 64
          : shape == GEMM_96x64 ? 96 : 128;
 }
-constexpr int gemm_bn(int shape) { return (shape == GEMM_32x128 || shape == GEMM_64x128) ? 128 : 64; }
+constexpr int gemm_bn(int shape) {
+    return (shape == GEMM_32x128 || shape == GEMM_64x128 || shape == GEMM_128x128) ? 128 : 64;
+}
 struct GemmTile {
     int32_t prob, batch, tm, tn;
     int32_t k0, k1;   // reduction range of this tile
