@@ -70,6 +70,9 @@ extern "C" {
                                   run one after the other on buffers of the largest chunk's size;
                                   0: one chunk; MQ_CHUNK_AUTO (default): 49152 MiB. */
 #define MQ_CHUNK_AUTO (-1)
+#define MQ_OPT_TERM_SOURCES 4  /* value 1: H*psi plans made afterwards (single site problems) also keep,
+                                  per S term, the MPO term it comes from, for
+                                  mq_hpsi_term_gradient (default 0) */
 
 /* MPS pairing (reference: MPSTensor::make_left_paired / make_right_paired, mpstensor.hpp:211-237) */
 #define MQ_RIGHT_PAIRED 0      /* blocks per left charge:  ls x (sum_sigma d_sigma * rs(lc+sigma)) */
@@ -274,6 +277,17 @@ int mq_hpsi_timed(mq_hpsi_plan_t plan, const double* psi_in_dev, double* psi_out
 int mq_hpsi_plan_use_graphs(mq_hpsi_plan_t plan, int enable);
 /* host pointers, synchronous */
 int mq_hpsi_host(mq_hpsi_plan_t plan, const double* psi_in, double* psi_out, int precision);
+/* MPO-term adjoint of the site problem: grad[t] = dE/d(scale of term t of the plan's MPO) for
+ * E = <psi| H_eff |psi> (psi: host, the plan's layout; grad: host, n_grad = the MPO's n_terms), in
+ * FP64 on the device: T = psi * R as in H*psi, Y = L_panel^T psi where the term sum writes S, then
+ * per S term the dot product of its T tile with its line's Y tile, times d alpha / d(scale),
+ * summed per MPO term.  sum_t scale_t grad[t] = E.  Over a chain (left boundaries grown from the
+ * tensors the right ones were, every site's single-site MPO) these are the energy's derivatives
+ * with respect to every MPO-term scale, which mq_qc_integral_gradient turns into dE/dh and
+ * dE/d(pq|rs): the 1- and 2-RDMs by Hellmann-Feynman.  Needs a plan made with
+ * MQ_OPT_TERM_SOURCES.  No counterpart in the reference (its RDMs come from one expectation value
+ * per element, measurements/tagged_nrankrdm.h:413-661). */
+int mq_hpsi_term_gradient(mq_hpsi_plan_t plan, const double* psi, double* grad, int64_t n_grad);
 int mq_hpsi_plan_destroy(mq_hpsi_plan_t plan);
 
 /* ---- Jacobi-Davidson (reference: solve_ietl_jcd, ietl_jacobi_davidson.h:35-76,
@@ -329,6 +343,22 @@ int mq_block_svd(mq_ctx_t ctx, int n, const int32_t* rows, const int32_t* cols, 
 typedef struct mq_qc_s* mq_qc_t;
 int mq_qc_build(int symmetry, int L, const double* h1, const double* eri, double ecore, double cutoff,
                 mq_qc_t* out);
+/* mq_qc_build with flags.  MQ_QC_GRADIENT (SU2U1): the gradient layout -- every canonical
+ * integral's terms are kept whatever its value (cutoff unused, zero scales kept), so the chain's
+ * structure does not depend on the integral values; single-site terms are one MPO term per
+ * operator; and the handle keeps the map from integrals to MPO terms for mq_qc_integral_gradient.
+ * The energy of the chain is the one of mq_qc_build. */
+#define MQ_QC_GRADIENT 1
+int mq_qc_build_ex(int symmetry, int L, const double* h1, const double* eri, double ecore, double cutoff, int flags,
+                   mq_qc_t* out);
+/* Chain rule from MPO-term scales to integrals (the builder is linear in the integrals): given
+ * term_grads[p][t] = dE/d(scale of term t of site p's single-site MPO) for p = 0 .. L-1 (e.g. from
+ * mq_hpsi_term_gradient), writes dE/dh1[i*L+j] (j <= i) into dh (L*L) and dE/deri[((i*L+j)*L+k)*L+l]
+ * (i >= j, k >= l, (i,j) >= (k,l)) into deri (L^4): the derivatives with respect to the canonical
+ * entries that the builder reads, zero at every other position.  Needs a handle built with
+ * MQ_QC_GRADIENT.  No counterpart in the reference, which measures the RDMs element by element
+ * (measurements/tagged_nrankrdm.h:413-661); these are the same quantities contracted once. */
+int mq_qc_integral_gradient(mq_qc_t qc, const double* const* term_grads, double* dh, double* deri);
 /* L, the L+1 MPO bond dimensions, the core energy (any pointer may be NULL) */
 int mq_qc_info(mq_qc_t qc, int* L, uint32_t* bond_dims, double* ecore);
 /* descriptor of the site MPO (twosite = 0) or of the two-site MPO of sites (site, site+1); the

@@ -26,6 +26,8 @@ MQ_OPT_SMALL_TILE_MAX = 2
 MQ_SMALL_TILE_AUTO = -1
 MQ_OPT_CHUNK_MB = 3
 MQ_CHUNK_AUTO = -1
+MQ_OPT_TERM_SOURCES = 4
+MQ_QC_GRADIENT = 1
 MQ_RIGHT_PAIRED = 0
 MQ_LEFT_PAIRED = 1
 MQ_NO_PARTNER = 0xFFFFFFFF
@@ -166,12 +168,15 @@ SIGNATURES = [
     ("mq_hpsi_timed", i32, [vp, vp, vp, i32, P(C.c_float)]),
     ("mq_hpsi_host", i32, [vp, dp, dp, i32]),
     ("mq_hpsi_plan_use_graphs", i32, [vp, i32]),
+    ("mq_hpsi_term_gradient", i32, [vp, dp, dp, C.c_int64]),
     ("mq_hpsi_plan_destroy", i32, [vp]),
     ("mq_davidson", i32, [vp, dp, i32, P(dp), C.c_double, i32, i32, dp, dp, P(DavidsonStats)]),
     ("mq_davidson_noalloc", i32, [vp, dp, i32, P(dp), C.c_double, i32, i32, dp, dp, P(DavidsonStats)]),
     ("mq_davidson_batch", i32, [vp, dp, C.c_double, i32, i32, dp, dp, P(C.c_int32), P(DavidsonStats)]),
     ("mq_block_svd", i32, [vp, i32, P(C.c_int32), P(C.c_int32), dp, dp, dp, dp]),
     ("mq_qc_build", i32, [i32, i32, dp, dp, C.c_double, C.c_double, P(vp)]),
+    ("mq_qc_build_ex", i32, [i32, i32, dp, dp, C.c_double, C.c_double, i32, P(vp)]),
+    ("mq_qc_integral_gradient", i32, [vp, P(dp), dp, dp]),
     ("mq_qc_info", i32, [vp, P(i32), P(C.c_uint32), dp]),
     ("mq_qc_site_mpo", i32, [vp, i32, i32, P(MPODesc)]),
     ("mq_qc_phys", i32, [vp, i32, P(IndexDesc)]),

@@ -37,6 +37,7 @@ struct mq_ctx_s {
     bool plain_tables = false;         // MQ_OPT_PLAIN_TABLES
     int32_t small_tile_max = MQ_SMALL_TILE_AUTO;  // MQ_OPT_SMALL_TILE_MAX
     int32_t chunk_mb = MQ_CHUNK_AUTO;             // MQ_OPT_CHUNK_MB
+    bool term_sources = false;         // MQ_OPT_TERM_SOURCES
     rocblas_handle blas = nullptr;     // library GEMMs of MQ_PREC_FP64_PLAIN, made on first use
     double* noalloc_scratch = nullptr; // this device's scratch array of mq_davidson_noalloc
 };
@@ -438,6 +439,20 @@ struct mq_hpsi_plan_s {
         STerm* terms = nullptr;
         int64_t n_lines = 0;
     } plain;
+    // MPO-term adjoint (plans made with MQ_OPT_TERM_SOURCES): per chunk the GEMMs of
+    // Y = L_panel^T psi (written where the chunk's S lines sit) and the term work items
+    struct Adjoint {
+        bool ok = false;
+        uint32_t n_mpo_terms = 0;
+        std::vector<GemmStage> yg;
+        std::vector<TermGradWork*> work;
+        std::vector<int64_t> n_work;
+        SLine* lines = nullptr;
+        STerm* terms = nullptr;
+        uint32_t* src = nullptr;
+        double* beta = nullptr;
+        double* grad = nullptr;
+    } adj;
 };
 
 // ================================================================================================
@@ -542,6 +557,8 @@ int mq_ctx_set_option(mq_ctx_t ctx, int option, int value) {
         } else if (option == MQ_OPT_CHUNK_MB) {
             require(value == MQ_CHUNK_AUTO || value >= 0, "mq_ctx_set_option: MQ_OPT_CHUNK_MB is MQ_CHUNK_AUTO or >= 0");
             ctx->chunk_mb = value;
+        } else if (option == MQ_OPT_TERM_SOURCES) {
+            ctx->term_sources = value != 0;
         } else {
             throw Error(MQ_ERR_INVALID, "mq_ctx_set_option: unknown option");
         }
@@ -934,6 +951,32 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
         p->plain.ok = true;
         p->seconds_plain = seconds_since(tp);
     }
+    if (!H.term_src.empty()) {
+        require(p->n_instances == 1, "MPO-term adjoint: single site problems only");
+        auto& A = p->adj;
+        A.yg.resize(nch);
+        for (size_t c = 0; c < nch; ++c) {
+            HpsiChunk const& ch = H.chunks[c];
+            sizes[BASE_T] = ch.t_elems;
+            sizes[BASE_S] = ch.s_elems;
+            std::vector<GemmDesc> ys = adjoint_y_gemms(cgs[c]);
+            check_gemms(ys, sizes);
+            A.yg[c].upload(ys, {}, ctx->stream);
+            std::vector<TermGradWork> w = term_grad_work(H.lines, ch.line0, ch.line1);
+            A.work.push_back(dev_upload(w, ctx->stream));
+            A.n_work.push_back(int64_t(w.size()));
+        }
+        for (size_t k = 0; k < H.terms.size(); ++k)
+            require(H.term_src[k] < p->adj.n_mpo_terms, "MPO-term adjoint: term source outside the MPO");
+        A.lines = dev_upload(H.lines, ctx->stream);
+        A.terms = dev_upload(H.terms, ctx->stream);
+        A.src = dev_upload(H.term_src, ctx->stream);
+        A.beta = dev_upload(H.term_beta, ctx->stream);
+        A.grad = dev_alloc<double>(std::max<uint32_t>(A.n_mpo_terms, 1));
+        A.ok = true;
+        H.term_src.clear(); H.term_src.shrink_to_fit();
+        H.term_beta.clear(); H.term_beta.shrink_to_fit();
+    }
     std::vector<GemmDesc>().swap(H.t_gemms_task);
     std::vector<GemmDesc>().swap(H.c_gemms_cohort);
     for (auto const& part : lparts) {
@@ -966,8 +1009,10 @@ int mq_hpsi_plan_create(mq_ctx_t ctx, mq_mpo_t mpo, mq_boundary_t left, mq_bound
         const int64_t mb = ctx->chunk_mb == MQ_CHUNK_AUTO ? 49152 : ctx->chunk_mb;
         const uint64_t chunk_elems = uint64_t(mb) * (uint64_t(1) << 20) / sizeof(double);
         auto* p = new mq_hpsi_plan_s{ctx, build_hpsi_plan(ctx->sym, mpo->mpo, left->idx, right->idx, index_from(left_i),
-                                                          index_from(phys_i), index_from(right_i), chunk_elems)};
+                                                          index_from(phys_i), index_from(right_i), chunk_elems,
+                                                          ctx->term_sources)};
         std::unique_ptr<mq_hpsi_plan_s> guard(p);
+        p->adj.n_mpo_terms = uint32_t(mpo->mpo.terms.size());
         p->psi_n = p->host.mps.elements();
         p->rdata = right->data;
         p->rhold = right->hold;
@@ -1439,10 +1484,54 @@ int mq_hpsi_host(mq_hpsi_plan_t p, const double* in, double* out, int precision)
     });
 }
 
+int mq_hpsi_term_gradient(mq_hpsi_plan_t p, const double* psi, double* grad, int64_t n_grad) {
+    if (!p) return MQ_ERR_INVALID;
+    return guarded(p->ctx, [&] {
+        require(psi && grad, "mq_hpsi_term_gradient: null argument");
+        require(p->adj.ok, "mq_hpsi_term_gradient: the plan was made without MQ_OPT_TERM_SOURCES");
+        require(n_grad == int64_t(p->adj.n_mpo_terms), "mq_hpsi_term_gradient: n_grad is not the MPO's term count");
+        hipStream_t s = p->ctx->stream;
+        const uint64_t n = p->psi_n;
+        double* din = dev_alloc<double>(n);
+        try {
+            hipcheck(hipMemcpyAsync(din, psi, n * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+            hipcheck(launch_fill(p->adj.grad, p->adj.n_mpo_terms, 0.0, s), "clear");
+            Bases b{};
+            b.p[BASE_PSI_IN] = din;
+            b.p[BASE_T] = p->T;
+            b.p[BASE_S] = p->S;
+            b.p[BASE_LPANEL] = p->Lpanel;
+            b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
+            for (size_t c = 0; c < p->chunks.size(); ++c) {
+                p->chunks[c].tg.run(b, s, GEMM_F64);   // T = psi * R
+                p->adj.yg[c].run(b, s, GEMM_F64);      // Y = L_panel^T psi, in the S buffer
+                hipcheck(launch_term_grad(p->adj.work[c], p->adj.n_work[c], p->adj.lines, p->adj.terms, p->adj.src,
+                                          p->adj.beta, p->S, p->T, p->adj.grad, s),
+                         "term gradients");
+            }
+            hipcheck(hipMemcpyAsync(grad, p->adj.grad, p->adj.n_mpo_terms * sizeof(double), hipMemcpyDeviceToHost, s),
+                     "download");
+            hipcheck(hipStreamSynchronize(s), "term gradients");
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            dev_free(din);
+            throw;
+        }
+        dev_free(din);
+    });
+}
+
 int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
     if (!p) return MQ_OK;
     int rc = guarded(p->ctx, [&] {
         (void)hipStreamSynchronize(p->ctx->stream);
+        for (auto& y : p->adj.yg) y.release();
+        for (auto* w : p->adj.work) dev_free(w);
+        dev_free(p->adj.lines);
+        dev_free(p->adj.terms);
+        dev_free(p->adj.src);
+        dev_free(p->adj.beta);
+        dev_free(p->adj.grad);
         for (auto& c : p->chunks) {
             c.tg.release();
             c.cg.release();

@@ -391,6 +391,31 @@ __global__ __launch_bounds__(256) void sline_kernel(const SLine* __restrict__ li
     }
 }
 
+// MPO-term adjoint of the term sum.  E = <psi|H psi> = sum over terms k of alpha_k <Y_line(k),
+// T_k> with Y = L_panel^T psi laid out like S, so dE/d(scale of MPO term m) is the sum over the
+// terms k from m of beta_k <Y_line(k), T_k> (beta_k = d alpha_k / d scale_m).  A workgroup takes
+// a run of one line's terms; each group of 2^lg lanes forms one term's dot product over the tile
+// elements and reduces it with cross-lane exchanges inside the group.
+__global__ __launch_bounds__(TERM_GRAD_THREADS) void term_grad_kernel(
+    const TermGradWork* __restrict__ work, const SLine* __restrict__ lines, const STerm* __restrict__ terms,
+    const uint32_t* __restrict__ src, const double* __restrict__ beta, const double* __restrict__ Y,
+    const double* __restrict__ X, double* __restrict__ grad) {
+    const TermGradWork w = work[blockIdx.x];
+    const SLine l = lines[w.line];
+    const int E = l.rows * l.cols;
+    const int G = 1 << w.lg;
+    const int grp = int(threadIdx.x) >> w.lg, e0 = int(threadIdx.x) & (G - 1);
+    const int ngroups = TERM_GRAD_THREADS >> w.lg;
+    for (int k = w.t0 + grp; k < w.t1; k += ngroups) {
+        const int64_t toff = terms[k].t_off;
+        double acc = 0.0;
+        for (int e = e0; e < E; e += G) acc += Y[l.s_off + e % l.rows + int64_t(e / l.rows) * l.lds] * X[toff + e];
+        // the lanes of one group share k, so a group is active or idle as a whole
+        for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+        if (e0 == 0) atomicAdd(grad + src[k], beta[k] * acc);
+    }
+}
+
 template <class TI, class TO>
 __global__ void convert_kernel(const TI* __restrict__ src, TO* __restrict__ dst, uint64_t n) {
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
@@ -627,6 +652,17 @@ template hipError_t launch_small_lines<double>(const SSmallWork*, int64_t, const
                                                double*, hipStream_t);
 template hipError_t launch_small_lines<float>(const SSmallWork*, int64_t, const SLine*, const STerm*, const float*,
                                               float*, hipStream_t);
+
+hipError_t launch_term_grad(const TermGradWork* work, int64_t n_work, const SLine* lines, const STerm* terms,
+                            const uint32_t* src, const double* beta, const double* Y, const double* T, double* grad,
+                            hipStream_t stream) {
+    for (int64_t off = 0; off < n_work; off += 0x7fffffff) {
+        const int64_t cnt = n_work - off < 0x7fffffff ? n_work - off : 0x7fffffff;
+        hipLaunchKernelGGL(term_grad_kernel, dim3(unsigned(cnt)), dim3(TERM_GRAD_THREADS), 0, stream, work + off, lines,
+                           terms, src, beta, Y, T, grad);
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_convert(const double* src, float* dst, uint64_t n, hipStream_t stream) {
     if (!n) return hipSuccess;

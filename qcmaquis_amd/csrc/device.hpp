@@ -42,6 +42,11 @@ hipError_t launch_small_lines(const SSmallWork* work, int64_t n_work, const SLin
 // terms (the per-term formulation the block-form kernel replaces)
 hipError_t launch_slines(const SLine* lines, int64_t n_lines, const STerm* terms, const double* T, double* S,
                          hipStream_t stream);
+// MPO-term adjoint: for every term k of the work items' lines, grad[src[k]] += beta[k] * <Y tile of
+// the line, T tile of the term> (FP64 atomics; the order of the additions is not fixed)
+hipError_t launch_term_grad(const TermGradWork* work, int64_t n_work, const SLine* lines, const STerm* terms,
+                            const uint32_t* src, const double* beta, const double* Y, const double* T, double* grad,
+                            hipStream_t stream);
 // dst = float(src)
 hipError_t launch_convert(const double* src, float* dst, uint64_t n, hipStream_t stream);
 hipError_t launch_fill(double* p, uint64_t n, double v, hipStream_t stream);

@@ -27,6 +27,8 @@ struct CohortAcc {
         std::vector<Term> terms;
         std::vector<uint32_t> line_count, line_b;
         uint32_t pending = 0;
+        // with term sources: per term, the MPO term it comes from and its alpha at unit term scale
+        std::vector<std::pair<uint32_t, double>> src;
     };
     std::vector<uint32_t> sfold;
     std::vector<Unit> units;
@@ -57,6 +59,10 @@ struct CohortAcc {
         u.terms.push_back(Term{alpha, ti, col});
         if (!u.pending++) dirty.push_back(ui);
         ++n_terms;
+    }
+    void push_src(size_t s, uint32_t ss2, double alpha, uint32_t ti, uint32_t col, uint32_t mpo_term, double beta) {
+        units[sfold[s] + ss2].src.push_back({mpo_term, beta});
+        push(s, ss2, alpha, ti, col);
     }
     // closes the line of MPO index b in every unit that received terms since the last line
     void add_line(uint32_t b) {
@@ -131,18 +137,23 @@ inline int casenr(int32_t rspin, int32_t cspin) {
     return 0;
 }
 
-// iterate the sparse entries of one W block (reference: detail::op_iterate, op_iterate.hpp:37-69)
+// iterate the sparse entries of one W block (reference: detail::op_iterate, op_iterate.hpp:37-69);
+// Src: also record each term's MPO term and its alpha at unit term scale (coupling cpl1)
+template <bool Src = false>
 inline void op_iterate(OpBlock const& blk, const double cpl[4], CohortAcc& coh, size_t s, uint32_t m2_size,
-                       TList& tl, uint64_t mps_offset, uint32_t ci, uint32_t boundary_col) {
+                       TList& tl, uint64_t mps_offset, uint32_t ci, uint32_t boundary_col,
+                       uint32_t mpo_term = 0, const double* cpl1 = nullptr) {
     for (auto const& e : blk.entries) {
-        double alpha = e.coeff * cpl[casenr(e.row_spin, e.col_spin)];
+        const int cn = casenr(e.row_spin, e.col_spin);
+        double alpha = e.coeff * cpl[cn];
         uint32_t ti = tl.get(mps_offset + uint64_t(e.row) * m2_size, ci);
         if (ti != UINT32_MAX) {
             // cohorts of one T list may be built concurrently: every writer stores the same flag
             // (read first, so the threads do not keep invalidating each other's copy of the line)
             if (!__atomic_load_n(&tl.tasks[ti].used, __ATOMIC_RELAXED))
                 __atomic_store_n(&tl.tasks[ti].used, true, __ATOMIC_RELAXED);
-            coh.push(s, e.col, alpha, ti, boundary_col);
+            if (Src) coh.push_src(s, e.col, alpha, ti, boundary_col, mpo_term, e.coeff * cpl1[cn]);
+            else coh.push(s, e.col, alpha, ti, boundary_col);
         }
     }
 }
@@ -360,7 +371,8 @@ void validate_boundary_offsets(BoundaryIndex const& B) {
 //         site_hamil_schedule.hpp:40-93, right_tasks.hpp:39-78, site_hamil_tasks.hpp:41-137)
 // ----------------------------------------------------------------------------------------------
 HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, BoundaryIndex const& R,
-                             Index const& left_i, Index const& phys_i, Index const& right_i, uint64_t chunk_elems) {
+                             Index const& left_i, Index const& phys_i, Index const& right_i, uint64_t chunk_elems,
+                             bool term_sources) {
     HpsiPlanHost P;
     P.mps = MPSLayout(left_i, phys_i, right_i);
     require(L.aux_dim == mpo.row_dim, "hpsi: left boundary aux dim != MPO row dim");
@@ -484,8 +496,20 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                                     w9j->set_scale(A, K, Ap, spin_of(rc_in), scale, cpl);
                                 else
                                     cpl[0] = cpl[1] = cpl[2] = cpl[3] = scale;
-                                op_iterate(blk, cpl, coh, s, rs_in, tl, pb_off[size_t(br.s_in) * NR + rb_in], ci_right,
-                                           uint32_t(right_offset / rs_in));
+                                if (term_sources) {
+                                    // the same coupling at unit term scale: d alpha / d(term scale)
+                                    double cpl1[4];
+                                    const double scale1 = R.conjugate_scale(ci_right, b2) * L.conjugate_scale(ci, b1);
+                                    if (su2)
+                                        w9j->set_scale(A, K, Ap, spin_of(rc_in), scale1, cpl1);
+                                    else
+                                        cpl1[0] = cpl1[1] = cpl1[2] = cpl1[3] = scale1;
+                                    op_iterate<true>(blk, cpl, coh, s, rs_in, tl, pb_off[size_t(br.s_in) * NR + rb_in],
+                                                     ci_right, uint32_t(right_offset / rs_in), t, cpl1);
+                                } else {
+                                    op_iterate(blk, cpl, coh, s, rs_in, tl, pb_off[size_t(br.s_in) * NR + rb_in],
+                                               ci_right, uint32_t(right_offset / rs_in));
+                                }
                             }
                         }
                     }
@@ -658,6 +682,12 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
         P.flops.n_terms += rec.acc.n_terms;
     }
     // --- S lines, chunk by chunk ---
+    uint64_t n_all_terms = 0;
+    if (term_sources) {  // every term is emitted once, in some chunk
+        for (auto const& rec : cohorts) n_all_terms += rec.acc.n_terms;
+        P.term_src.resize(n_all_terms);
+        P.term_beta.resize(n_all_terms);
+    }
     for (uint32_t ch = 0; ch < n_chunks; ++ch) {
         P.chunks[ch].line0 = P.lines.size();
         emit_lines(cohorts.size(), P.lines, P.terms, [&](size_t c) -> auto const& { return cohorts[c].acc.units; },
@@ -682,6 +712,10 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
                     for (uint32_t k = seek; k < seek + u.line_count[l]; ++k) {
                         TTask const& t = tlists[rec.lb_in].tasks[u.terms[k].ti];
                         check_tile(t, uint64_t(u.terms[k].col) * ls_in, ls_in, u.ms);
+                        if (term_sources) {
+                            P.term_src[terms.size()] = u.src[k].first;
+                            P.term_beta[terms.size()] = u.src[k].second;
+                        }
                         terms.push_back(STerm{u.terms[k].alpha, int64_t(t.t_off + uint64_t(u.terms[k].col) * ls_in)});
                     }
                     line.t1 = int32_t(terms.size());
@@ -706,6 +740,7 @@ HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, Bo
         });
         P.chunks[ch].line1 = P.lines.size();
     }
+    require(!term_sources || P.terms.size() == n_all_terms, "hpsi: term sources do not cover the term table");
     P.n_cohorts = cohorts.size();
     // L*S products: per chunk and output block, one per run of the chunk's stripes that are
     // adjacent both in the psi block and in the chunk's S panel (one per block with one chunk);
@@ -1565,6 +1600,33 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
     }
     if (any_small) build_small_lines(B, lines, terms, is_small);
     return B;
+}
+
+std::vector<GemmDesc> adjoint_y_gemms(std::vector<GemmDesc> const& c_gemms) {
+    std::vector<GemmDesc> ys;
+    ys.reserve(c_gemms.size());
+    for (GemmDesc const& g : c_gemms) {
+        require(g.flags == 0 && g.batch == 1 && g.base_a == BASE_LPANEL && g.base_b == BASE_S && g.base_c == BASE_PSI_OUT,
+                "MPO-term adjoint: unexpected L*S product");
+        // Y (K x N, where the product reads S) = L_panel^T (K x M) * psi (M x N, where it writes psi')
+        ys.push_back(gemm(g.K, g.N, g.M, BASE_LPANEL, g.a, g.lda, BASE_PSI_IN, g.c, g.ldc, BASE_S, g.b, g.ldb,
+                          GEMM_TRANS_A));
+    }
+    return ys;
+}
+
+std::vector<TermGradWork> term_grad_work(HostTable<SLine> const& lines, size_t l0, size_t l1) {
+    std::vector<TermGradWork> w;
+    for (size_t li = l0; li < l1; ++li) {
+        SLine const& l = lines[li];
+        const int64_t E = int64_t(l.rows) * l.cols;
+        int lg = 0;
+        while (lg < 6 && (int64_t(1) << lg) < E) ++lg;
+        require(li < size_t(INT32_MAX), "MPO-term adjoint: line index overflow");
+        for (int32_t t = l.t0; t < l.t1; t += TERM_GRAD_RUN)
+            w.push_back(TermGradWork{int32_t(li), t, l.t1 - t > TERM_GRAD_RUN ? t + TERM_GRAD_RUN : l.t1, lg});
+    }
+    return w;
 }
 
 void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size) {

@@ -144,6 +144,18 @@ struct SBlockTable {
     std::vector<SSmallWork> small_work;
     uint64_t small_gathered = 0;  // sum over small lines of terms * tile elements
 };
+// Work item of the MPO-term adjoint: the terms [t0, t1) of one S line, 2^lg lanes per term (one
+// dot product of the line's tile of Y = L_panel^T psi with the term's T tile each)
+struct TermGradWork {
+    int32_t line, t0, t1, lg;
+};
+constexpr int TERM_GRAD_THREADS = 256;
+constexpr int32_t TERM_GRAD_RUN = 256;  // terms per work item at most
+// the GEMMs Y = L_panel^T psi of the L*S products psi' = L_panel S (Y written where S is read)
+std::vector<GemmDesc> adjoint_y_gemms(std::vector<GemmDesc> const& c_gemms);
+// work items of the lines [l0, l1): runs of at most TERM_GRAD_RUN terms, 2^lg >= tile elements
+// lanes per term (at most 64)
+std::vector<TermGradWork> term_grad_work(HostTable<SLine> const& lines, size_t l0, size_t l1);
 // lanes of a workgroup of the small-tile term sum; tiles up to this many elements can go there
 constexpr int SMALL_LINE_LANES = 256;
 // work items cover SBLOCK_ELEMS consecutive tile elements of one block
@@ -246,6 +258,9 @@ struct HpsiPlanHost {
     std::vector<GemmDesc> c_gemms_cohort;
     HostTable<SLine> lines;
     HostTable<STerm> terms;
+    // with term sources: per S term, the MPO term it comes from and d alpha / d(that term's scale)
+    HostTable<uint32_t> term_src;
+    HostTable<double> term_beta;
     std::vector<CopyJob> lpanel_jobs;
     uint64_t t_elems = 0, s_elems = 0, lpanel_elems = 0;  // T and S: sums over the chunks
     uint64_t t_buf = 0, s_buf = 0;                          // T and S buffers: the largest chunk's
@@ -267,10 +282,11 @@ struct GrowPlanHost {
 };
 
 // chunk_elems: largest T + S (elements) of one chunk; the right sectors are gathered in index
-// order into chunks up to that size (a sector larger than it is a chunk of its own); 0 = one chunk
+// order into chunks up to that size (a sector larger than it is a chunk of its own); 0 = one chunk.
+// term_sources: also fill term_src / term_beta (the MPO-term adjoint)
 HpsiPlanHost build_hpsi_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, BoundaryIndex const& R,
                              Index const& left_i, Index const& phys_i, Index const& right_i,
-                             uint64_t chunk_elems = 0);
+                             uint64_t chunk_elems = 0, bool term_sources = false);
 
 GrowPlanHost build_lstep_plan(Sym sym, MPO const& mpo, BoundaryIndex const& L, MPSLayout const& bra,
                               MPSLayout const& ket, bool symmetric);

@@ -581,15 +581,24 @@ struct mq_qc_s {
 extern "C" {
 
 int mq_qc_build(int symmetry, int L, const double* h1, const double* eri, double ecore, double cutoff, mq_qc_t* out) {
-    if (!out || L < 2 || !h1 || !eri) return MQ_ERR_INVALID;
+    return mq_qc_build_ex(symmetry, L, h1, eri, ecore, cutoff, 0, out);
+}
+
+int mq_qc_build_ex(int symmetry, int L, const double* h1, const double* eri, double ecore, double cutoff, int flags,
+                   mq_qc_t* out) {
+    if (!out || L < 2 || !h1 || !eri || (flags & ~MQ_QC_GRADIENT)) return MQ_ERR_INVALID;
     if (symmetry != MQ_SYM_TWOU1 && symmetry != MQ_SYM_SU2U1) return MQ_ERR_UNSUPPORTED;
+    if ((flags & MQ_QC_GRADIENT) && symmetry != MQ_SYM_SU2U1) {
+        fprintf(stderr, "mq_qc_build_ex: the gradient layout is built for SU2U1 chains only\n");
+        return MQ_ERR_UNSUPPORTED;
+    }
     try {
         std::unique_ptr<mq_qc_s> q(new mq_qc_s());
         q->sym = symmetry;
         q->b.ecore = ecore;
         if (symmetry == MQ_SYM_SU2U1) {
             q->b.L = L;
-            q->su2 = mq::su2qc_build(L, h1, eri, cutoff);
+            q->su2 = mq::su2qc_build(L, h1, eri, cutoff, (flags & MQ_QC_GRADIENT) != 0);
             for (int k = 0; k < 2; ++k) mq::su2qc_phys(k, q->phys_charges[k], q->phys_sizes[k]);
         } else {
             q->b.L = L;
@@ -651,6 +660,24 @@ int mq_qc_phys(mq_qc_t q, int twosite, mq_index_desc* out) {
     out->charges = q->phys_charges[k].data();
     out->sizes = q->phys_sizes[k].data();
     return MQ_OK;
+}
+
+int mq_qc_integral_gradient(mq_qc_t q, const double* const* term_grads, double* dh, double* deri) {
+    if (!q || !term_grads || !dh || !deri) return MQ_ERR_INVALID;
+    if (q->sym != MQ_SYM_SU2U1) return MQ_ERR_UNSUPPORTED;
+    try {
+        const int L = q->length();
+        std::vector<const double*> e(term_grads, term_grads + L);
+        for (const double* x : e) mq::require(x != nullptr, "mq_qc_integral_gradient: null term-gradient array");
+        mq::su2qc_integral_gradient(*q->su2, e, dh, deri);
+        return MQ_OK;
+    } catch (mq::Error const& e) {
+        fprintf(stderr, "mq_qc_integral_gradient: %s\n", e.what());
+        return e.code;
+    } catch (std::exception const& e) {
+        fprintf(stderr, "mq_qc_integral_gradient: %s\n", e.what());
+        return MQ_ERR_INVALID;
+    }
 }
 
 int mq_qc_destroy(mq_qc_t q) {

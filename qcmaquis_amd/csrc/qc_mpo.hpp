@@ -56,7 +56,13 @@ struct SU2QC;
 struct SU2QCDeleter {
     void operator()(SU2QC* p) const;
 };
-std::unique_ptr<SU2QC, SU2QCDeleter> su2qc_build(int L, const double* h1, const double* eri, double cutoff);
+// gradient_layout: keep every canonical integral's terms (zeros included), single-site terms one
+// per operator, and the map from integrals to MPO terms (su2qc_integral_gradient)
+std::unique_ptr<SU2QC, SU2QCDeleter> su2qc_build(int L, const double* h1, const double* eri, double cutoff,
+                                                 bool gradient_layout = false);
+// dE/dh and dE/d(pq|rs) at the canonical integral positions (zero elsewhere) from dE/d(term scale)
+// of every site MPO (term_grads[p]: one value per term of site p's MPO)
+void su2qc_integral_gradient(SU2QC const& q, std::vector<const double*> const& term_grads, double* dh, double* deri);
 int su2qc_length(SU2QC const& q);
 std::vector<uint32_t> const& su2qc_bond_dims(SU2QC const& q);
 FlatMPO const& su2qc_site(SU2QC& q, int p);

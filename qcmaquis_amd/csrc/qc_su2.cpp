@@ -187,8 +187,12 @@ struct TermSet {
     std::map<QTerm, double> strings;           // 2-, 3- and 4-site strings
     std::vector<std::array<double, 16>> site;  // K = 0 single-site terms, m[in*4+out]
     std::vector<char> has_site;
+    // gradient layout: the single-site terms kept per operator (site_ops[p][op] = coefficient)
+    // instead of summed into one matrix
+    bool by_op = false;
+    std::vector<std::map<int, double>> site_ops;
 
-    explicit TermSet(int L_) : L(L_), site(L_), has_site(L_, 0) {
+    explicit TermSet(int L_, bool by_op_ = false) : L(L_), site(L_), has_site(L_, 0), by_op(by_op_), site_ops(L_) {
         for (auto& s : site) s.fill(0.0);
     }
 
@@ -205,9 +209,13 @@ struct TermSet {
         strings[t] += c;
     }
     void add_site(OpTable const& T, int p, int op, double c) {
+        has_site[p] = 1;
+        if (by_op) {
+            site_ops[p][op] += c;
+            return;
+        }
         for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j) site[p][i * 4 + j] += c * T.ops[op].m[i][j];
-        has_site[p] = 1;
     }
 
     // TermMakerSU2 shapes (term_maker.h:95-225)
@@ -442,6 +450,14 @@ struct SU2QC {
     std::vector<std::vector<int>> site_ops;  // per site: FlatMPO op index -> table op
     std::vector<std::unique_ptr<FlatMPO>> two;
     std::vector<uint32_t> bond_dims;
+    // gradient layout (build with grad = true): every canonical integral's terms are kept, zero
+    // coefficients included, so the chain's structure does not depend on the integral values;
+    // single-site terms are one MPO term per operator; and each string remembers the MPO term its
+    // coefficient sits on (the term scale is the sum of the coefficients of the strings on it)
+    bool grad = false;
+    std::map<QTerm, std::tuple<int, Key, Key, int>> detach_at;  // string -> (site, keys, op)
+    std::map<QTerm, std::pair<int, uint32_t>> string_term;      // string -> (site, term)
+    std::vector<std::map<int, uint32_t>> site_term;             // site -> op -> term
 
     // ---- key properties ----
     int key_spin(Key const& k) const {
@@ -491,7 +507,10 @@ struct SU2QC {
         auto& cell = prempo[p][{a, b}];
         if (cell.terms.empty()) cell.terms.push_back({op, 1.0});
     }
-    void detach(int p, Key const& a, Key const& b, int op, double c) { prempo[p][{a, b}].add(op, c); }
+    void detach(QTerm const& t, int p, Key const& a, Key const& b, int op, double c) {
+        prempo[p][{a, b}].add(op, c);
+        if (grad) detach_at[t] = std::make_tuple(p, a, b, op);
+    }
 
     void add_string(QTerm const& t, double c) {
         auto nferm = [&](int from) {
@@ -503,7 +522,7 @@ struct SU2QC {
         const int* O = t.op;
         if (t.n == 2) {
             Key k = key_of(3, 1, P + 1, O + 1);
-            detach(P[0], TL, k, O[0], c);
+            detach(t, P[0], TL, k, O[0], c);
             insert_fill(P[0] + 1, P[1], k, fill_for(k, nferm(1)));
             attach(P[1], k, TR, O[1]);
         } else if (t.n == 3) {
@@ -511,7 +530,7 @@ struct SU2QC {
             attach(P[0], TL, k1, O[0]);
             insert_fill(P[0] + 1, P[1], k1, fill_for(k1, nferm(1)));
             Key k2 = key_of(3, 1, P + 2, O + 2);
-            detach(P[1], k1, k2, O[1], c);
+            detach(t, P[1], k1, k2, O[1], c);
             insert_fill(P[1] + 1, P[2], k2, fill_for(k2, nferm(2)));
             attach(P[2], k2, TR, O[2]);
         } else if (P[1] + P[2] < L - 1) {
@@ -522,7 +541,7 @@ struct SU2QC {
             attach(P[1], k1, k2, O[1]);
             insert_fill(P[1] + 1, P[2], k2, fill_for(k2, nferm(2)));
             Key k3 = key_of(3, 1, P + 3, O + 3);
-            detach(P[2], k2, k3, O[2], c);
+            detach(t, P[2], k2, k3, O[2], c);
             insert_fill(P[2] + 1, P[3], k3, fill_for(k3, nferm(3)));
             attach(P[3], k3, TR, O[3]);
         } else {
@@ -530,7 +549,7 @@ struct SU2QC {
             attach(P[0], TL, k1, O[0]);
             insert_fill(P[0] + 1, P[1], k1, fill_for(k1, nferm(1)));
             Key k2 = key_of(3, 2, P + 2, O + 2);
-            detach(P[1], k1, k2, O[1], c);
+            detach(t, P[1], k1, k2, O[1], c);
             insert_fill(P[1] + 1, P[2], k2, fill_for(k2, nferm(2)));
             Key k3 = key_of(3, 1, P + 3, O + 3);
             attach(P[2], k2, k3, O[2]);
@@ -539,32 +558,45 @@ struct SU2QC {
         }
     }
 
-    void build(const double* h1, const double* eri, double cutoff) {
-        TermSet S(L);
+    // the canonical integrals (i >= j, k >= l, (i, j) >= (k, l); one-electron ones with k = l = -1)
+    // in the builder's order
+    template <class F>
+    void for_each_integral(F f) const {
         for (int i = 0; i < L; ++i)
-            for (int j = 0; j <= i; ++j) {
-                double v = h1[size_t(i) * L + j];
-                if (std::fabs(v) > cutoff) S.integral(T, i, j, -1, -1, v);
-            }
+            for (int j = 0; j <= i; ++j) f(i, j, -1, -1);
         for (int i = 0; i < L; ++i)
             for (int j = 0; j <= i; ++j)
                 for (int k = 0; k <= i; ++k)
                     for (int l = 0; l <= k; ++l) {
                         if (k == i && l > j) continue;
-                        double v = eri[((size_t(i) * L + j) * L + k) * L + l];
-                        if (std::fabs(v) > cutoff) S.integral(T, i, j, k, l, v);
+                        f(i, j, k, l);
                     }
+    }
+    static size_t integral_pos(int L, int i, int j, int k, int l) {
+        return k < 0 ? size_t(i) * L + j : ((size_t(i) * L + j) * L + k) * L + l;
+    }
+
+    void build(const double* h1, const double* eri, double cutoff) {
+        TermSet S(L, grad);
+        for_each_integral([&](int i, int j, int k, int l) {
+            double v = k < 0 ? h1[integral_pos(L, i, j, k, l)] : eri[integral_pos(L, i, j, k, l)];
+            if (grad || std::fabs(v) > cutoff) S.integral(T, i, j, k, l, v);
+        });
         prempo.assign(L, {});
         int rightmost_left = 0, leftmost_right = L - 1;
         for (auto const& kv : S.strings) {
-            if (kv.second == 0.0) continue;
+            if (kv.second == 0.0 && !grad) continue;
             QTerm const& t = kv.first;
             rightmost_left = std::max(rightmost_left, t.pos[0]);
             leftmost_right = std::min(leftmost_right, t.pos[t.n - 1]);
             add_string(t, kv.second);
         }
         for (int p = 0; p < L; ++p)
-            if (S.has_site[p]) {
+            if (S.has_site[p] && grad) {
+                rightmost_left = std::max(rightmost_left, p);
+                leftmost_right = std::min(leftmost_right, p);
+                for (auto const& kv : S.site_ops[p]) prempo[p][{TL, TR}].add(kv.first, kv.second);
+            } else if (S.has_site[p]) {
                 rightmost_left = std::max(rightmost_left, p);
                 leftmost_right = std::min(leftmost_right, p);
                 SOp s;
@@ -602,6 +634,7 @@ struct SU2QC {
     }
 
     void assemble() {
+        std::map<std::tuple<int, Key, Key, int>, uint32_t> flat_of;  // grad: (site, keys, op) -> term
         site_mpo.assign(L, FlatMPO());
         site_ops.assign(L, {});
         bond_dims.assign(L + 1, 1);
@@ -626,21 +659,23 @@ struct SU2QC {
             FlatMPO& F = site_mpo[p];
             F.row_dim = bond_dims[p];
             F.col_dim = rdim;
-            std::map<std::pair<uint32_t, uint32_t>, Cell const*> cells;
+            std::map<std::pair<uint32_t, uint32_t>, std::pair<Key, Key> const*> cells;
             for (auto const& kv : prempo[p]) {
                 auto li = left.find(kv.first.first);
                 require(li != left.end(), "su2 qc: left key not found");
-                cells[{li->second, right[kv.first.second]}] = &kv.second;
+                cells[{li->second, right[kv.first.second]}] = &kv.first;
             }
             std::map<int, uint32_t> op_index;
             for (auto const& c : cells)
-                for (auto const& t : c.second->terms) {
-                    if (t.second == 0.0) continue;
+                for (auto const& t : prempo[p].at(*c.second).terms) {
+                    if (t.second == 0.0 && !grad) continue;
                     auto it = op_index.find(t.first);
                     if (it == op_index.end()) {
                         it = op_index.emplace(t.first, append_op(F, t.first)).first;
                         site_ops[p].push_back(t.first);
                     }
+                    if (grad) flat_of[std::make_tuple(p, c.second->first, c.second->second, t.first)] =
+                        uint32_t(F.term_op.size());
                     F.term_b.push_back(c.first.first);
                     F.term_b.push_back(c.first.second);
                     F.term_op.push_back(it->second);
@@ -681,6 +716,44 @@ struct SU2QC {
         }
         prempo.clear();
         two.resize(L > 0 ? L - 1 : 0);
+        if (!grad) return;
+        for (auto const& kv : detach_at) {
+            auto it = flat_of.find(kv.second);
+            require(it != flat_of.end(), "su2 qc: a string's coefficient term is missing");
+            string_term[kv.first] = {std::get<0>(kv.second), it->second};
+        }
+        detach_at.clear();
+        site_term.assign(L, {});
+        for (auto const& kv : flat_of)
+            if (std::get<1>(kv.first) == TL && std::get<2>(kv.first) == TR)
+                site_term[std::get<0>(kv.first)][std::get<3>(kv.first)] = kv.second;
+    }
+
+    // dE/d(integral) of every canonical integral from dE/d(term scale) of every site MPO term:
+    // the builder is linear in the integrals, so the derivative of a term scale with respect to one
+    // integral is what that integral alone (at 1) adds to it
+    void integral_gradient(std::vector<const double*> const& e, double* dh, double* deri) const {
+        require(grad, "su2 qc: integral gradients need the gradient layout");
+        require(e.size() == size_t(L), "su2 qc: one term-gradient array per site expected");
+        std::fill(dh, dh + size_t(L) * L, 0.0);
+        std::fill(deri, deri + size_t(L) * L * L * L, 0.0);
+        for_each_integral([&](int i, int j, int k, int l) {
+            TermSet S(L, true);
+            S.integral(T, i, j, k, l, 1.0);
+            double g = 0.0;
+            for (auto const& kv : S.strings) {
+                auto it = string_term.find(kv.first);
+                require(it != string_term.end(), "su2 qc: an integral's string is not in the chain");
+                g += kv.second * e[size_t(it->second.first)][it->second.second];
+            }
+            for (int p = 0; p < L; ++p)
+                for (auto const& kv : S.site_ops[p]) {
+                    auto it = site_term[size_t(p)].find(kv.first);
+                    require(it != site_term[size_t(p)].end(), "su2 qc: a single-site term is not in the chain");
+                    g += kv.second * e[size_t(p)][it->second];
+                }
+            (k < 0 ? dh : deri)[integral_pos(L, i, j, k, l)] = g;
+        });
     }
 
     // ---- two-site MPO ----
@@ -831,11 +904,16 @@ struct SU2QC {
 
 void SU2QCDeleter::operator()(SU2QC* p) const { delete p; }
 
-std::unique_ptr<SU2QC, SU2QCDeleter> su2qc_build(int L, const double* h1, const double* eri, double cutoff) {
+std::unique_ptr<SU2QC, SU2QCDeleter> su2qc_build(int L, const double* h1, const double* eri, double cutoff,
+                                                 bool gradient_layout) {
     std::unique_ptr<SU2QC, SU2QCDeleter> q(new SU2QC());
     q->L = L;
+    q->grad = gradient_layout;
     q->build(h1, eri, cutoff);
     return q;
+}
+void su2qc_integral_gradient(SU2QC const& q, std::vector<const double*> const& term_grads, double* dh, double* deri) {
+    q.integral_gradient(term_grads, dh, deri);
 }
 int su2qc_length(SU2QC const& q) { return q.L; }
 std::vector<uint32_t> const& su2qc_bond_dims(SU2QC const& q) { return q.bond_dims; }

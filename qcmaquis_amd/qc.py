@@ -119,16 +119,20 @@ class QCHamiltonian:
     """Quantum-chemistry MPO chain (mq_qc_build): TwoU1 (charges (N_up, N_down)) or SU2U1
     (charges (N, 2S), spin-adapted operators)."""
 
-    def __init__(self, h, eri, ecore=0.0, cutoff=1e-20, symmetry=abi.MQ_SYM_TWOU1):
+    def __init__(self, h, eri, ecore=0.0, cutoff=1e-20, symmetry=abi.MQ_SYM_TWOU1, gradient_layout=False):
+        """gradient_layout (SU2U1): the chain of mq_qc_build_ex(MQ_QC_GRADIENT), whose structure
+        does not depend on the integral values and which maps MPO-term gradients back to the
+        integrals (integral_gradient)"""
         self.lib = abi.lib()
         self.sym = int(symmetry)
         self.L = h.shape[0]
         self.h = np.ascontiguousarray(h, np.float64)
         self.eri = np.ascontiguousarray(eri, np.float64)
+        self.gradient_layout = bool(gradient_layout)
         hd = C.c_void_p()
         dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
-        rc = self.lib.mq_qc_build(self.sym, self.L, dp(self.h), dp(self.eri), float(ecore), float(cutoff),
-                                  C.byref(hd))
+        rc = self.lib.mq_qc_build_ex(self.sym, self.L, dp(self.h), dp(self.eri), float(ecore), float(cutoff),
+                                     abi.MQ_QC_GRADIENT if gradient_layout else 0, C.byref(hd))
         if rc != 0:
             raise RuntimeError(f"mq_qc_build failed ({rc})")
         self.h_qc = hd
@@ -152,6 +156,31 @@ class QCHamiltonian:
                 raise RuntimeError("mq_qc_site_mpo failed")
             self._cache[key] = RawMPO(self, d)
         return self._cache[key]
+
+    def term_scales(self, site):
+        """the term scales of the single-site MPO of one site (a copy)"""
+        d = self.mpo(site).desc()
+        return np.ctypeslib.as_array(d.term_scale, shape=(d.n_terms,)).copy() if d.n_terms else np.zeros(0)
+
+    def integral_gradient(self, term_grads):
+        """mq_qc_integral_gradient: (dE/dh, dE/deri) at the canonical integral positions (h[i, j],
+        j <= i; eri[i, j, k, l], i >= j, k >= l, (i, j) >= (k, l)), zero elsewhere, from
+        term_grads[p] = dE/d(term scales of site p's single-site MPO)"""
+        if not self.gradient_layout:
+            raise ValueError("integral gradients need a chain built with gradient_layout=True")
+        L = self.L
+        arrs = [np.ascontiguousarray(g, np.float64) for g in term_grads]
+        for p, a in enumerate(arrs):
+            if a.size != self.mpo(p).desc().n_terms:
+                raise ValueError("site %d: %d term gradients for %d terms" % (p, a.size, self.mpo(p).desc().n_terms))
+        ptrs = (C.POINTER(C.c_double) * L)(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
+        dh = np.zeros((L, L))
+        deri = np.zeros((L, L, L, L))
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        rc = self.lib.mq_qc_integral_gradient(self.h_qc, ptrs, dp(dh), dp(deri))
+        if rc != 0:
+            raise RuntimeError(f"mq_qc_integral_gradient failed ({rc})")
+        return dh, deri
 
     def __del__(self):
         try:

@@ -32,6 +32,8 @@ def lib():
         idx = P(abi.IndexDesc)
         L.emu_hpsi.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, P(abi.BIndexDesc), dp, idx, idx, idx, dp, dp,
                                C.c_int, P(C.c_uint64), C.c_uint64, P(C.c_uint64), P(C.c_uint64), C.c_int]
+        L.emu_term_grad.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, P(abi.BIndexDesc), dp, idx, idx,
+                                    idx, dp, dp, C.c_uint64]
         for f in (L.emu_lstep, L.emu_rstep):
             f.restype = C.c_void_p
             f.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, idx, idx, idx, dp, idx, idx, idx, dp, C.c_int]
@@ -73,6 +75,20 @@ def hpsi(sym, mpo, L, R, layout, psi, small_max=-1, chunk_elems=0, plain=False):
     last_chunks = nc.value
     last_ts = nts.value
     return out
+
+
+def term_grad(sym, mpo, L, R, layout, psi, chunk_elems=0):
+    """dE/d(scale of every MPO term) of E = <psi|H_eff|psi> through the product's adjoint tables"""
+    psi = np.ascontiguousarray(psi, np.float64)
+    md, Ld, Rd = mpo.desc(), L.desc(), R.desc()
+    grad = np.zeros(md.n_terms)
+    li, pi, ri = layout.left.desc(), layout.phys.desc(), layout.right.desc()
+    Ldat, Rdat = np.ascontiguousarray(L.data), np.ascontiguousarray(R.data)
+    rc = lib().emu_term_grad(sym, C.byref(md), C.byref(Ld), _dp(Ldat), C.byref(Rd), _dp(Rdat), C.byref(li),
+                             C.byref(pi), C.byref(ri), _dp(psi), _dp(grad), int(chunk_elems))
+    if rc != 0:
+        raise RuntimeError("emu_term_grad failed")
+    return grad
 
 
 def _grow(fn, sym, mpo, B, bra_layout, bra_rp, ket_layout, ket_rp, symmetric, bra_idx, ket_idx):

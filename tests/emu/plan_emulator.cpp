@@ -271,6 +271,78 @@ int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const dou
     }
 }
 
+// the MPO-term adjoint through the product's tables (term sources, Y products, term work items),
+// with the device kernel's arithmetic: grad[src] += beta * <Y tile, T tile> per term
+int emu_term_grad(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const double* Ldata,
+                  const mq_bindex_desc* Rd, const double* Rdata, const mq_index_desc* li, const mq_index_desc* pi,
+                  const mq_index_desc* ri, const double* psi, double* grad, uint64_t chunk_elems) {
+    try {
+        MPO mpo(*md);
+        BoundaryIndex L = boundary_index_from(*Ld), R = boundary_index_from(*Rd);
+        uint64_t Rn = R.layout();
+        L.layout();
+        HpsiPlanHost P = build_hpsi_plan(Sym(sym), mpo, L, R, Index(*li), Index(*pi), Index(*ri), chunk_elems, true);
+        require(P.term_src.size() == P.terms.size() && P.term_beta.size() == P.terms.size(), "term sources missing");
+        uint64_t n = P.mps.elements();
+        std::vector<double> Lp(P.lpanel_elems + 1, std::nan(""));
+        run_copy(P.lpanel_jobs, Ldata, Lp.data());
+        std::fill(grad, grad + mpo.terms.size(), 0.0);
+        for (HpsiChunk const& ch : P.chunks) {
+            uint64_t sizes[BASE_COUNT] = {0};
+            sizes[BASE_PSI_IN] = sizes[BASE_PSI_OUT] = n;
+            sizes[BASE_T] = ch.t_elems;
+            sizes[BASE_S] = ch.s_elems;
+            sizes[BASE_LPANEL] = P.lpanel_elems;
+            sizes[BASE_BOUNDARY] = Rn;
+            std::vector<GemmDesc> tg(P.t_gemms.begin() + int64_t(ch.t_gemm0), P.t_gemms.begin() + int64_t(ch.t_gemm1));
+            std::vector<GemmRow> tr(P.t_rows.begin() + int64_t(ch.t_row0), P.t_rows.begin() + int64_t(ch.t_row1));
+            for (auto& g : tg)
+                if (g.flags & GEMM_ROWMAP) g.a -= int64_t(ch.t_row0);
+            std::vector<GemmDesc> cg(P.c_gemms.begin() + int64_t(ch.c_gemm0), P.c_gemms.begin() + int64_t(ch.c_gemm1));
+            std::vector<GemmDesc> yg = adjoint_y_gemms(cg);
+            HostTable<SLine> cl(P.lines.begin() + int64_t(ch.line0), P.lines.begin() + int64_t(ch.line1));
+            check_gemms(tg, sizes, tr);
+            check_gemms(yg, sizes);
+            check_lines(cl, P.terms, ch.t_elems, ch.s_elems);
+            std::vector<double> T(P.t_buf + 1, std::nan("")), Y(P.s_buf + 1, std::nan(""));
+            double* base[BASE_COUNT] = {nullptr};
+            base[BASE_PSI_IN] = const_cast<double*>(psi);
+            base[BASE_T] = T.data();
+            base[BASE_S] = Y.data();
+            base[BASE_LPANEL] = Lp.data();
+            base[BASE_BOUNDARY] = const_cast<double*>(Rdata);
+            run_gemms(tg, tr, base);
+            run_gemms(yg, {}, base);
+            std::vector<char> covered(P.terms.size(), 0);
+            for (TermGradWork const& w : term_grad_work(P.lines, ch.line0, ch.line1)) {
+                SLine const& l = P.lines[size_t(w.line)];
+                require(w.lg >= 0 && w.lg <= 6 && (int64_t(1) << w.lg) >= std::min<int64_t>(64, int64_t(l.rows) * l.cols),
+                        "term work: lanes per term");
+                require(w.t0 >= l.t0 && w.t1 <= l.t1 && w.t1 - w.t0 <= TERM_GRAD_RUN, "term work: range");
+                for (int32_t k = w.t0; k < w.t1; ++k) {
+                    require(!covered[size_t(k)], "term work: a term in two items");
+                    covered[size_t(k)] = 1;
+                    double acc = 0.0;
+                    for (int e = 0; e < l.rows * l.cols; ++e) {
+                        double y = Y[size_t(l.s_off + e % l.rows + int64_t(e / l.rows) * l.lds)];
+                        require(!std::isnan(y), "term work: Y element not written");
+                        acc += y * T[size_t(P.terms[size_t(k)].t_off + e)];
+                    }
+                    require(P.term_src[size_t(k)] < mpo.terms.size(), "term source out of range");
+                    grad[P.term_src[size_t(k)]] += P.term_beta[size_t(k)] * acc;
+                }
+            }
+            for (size_t li2 = ch.line0; li2 < ch.line1; ++li2)
+                for (int32_t k = P.lines[li2].t0; k < P.lines[li2].t1; ++k)
+                    require(covered[size_t(k)], "term work: a term in no item");
+        }
+        return 0;
+    } catch (std::exception const& e) {
+        fprintf(stderr, "emu_term_grad: %s\n", e.what());
+        return -1;
+    }
+}
+
 static EmuBoundary* grow(bool left, int sym, const mq_mpo_desc* md, const mq_bindex_desc* Bd, const double* Bdata,
                          const mq_index_desc* bl, const mq_index_desc* bp, const mq_index_desc* br, const double* bra_rp,
                          const mq_index_desc* kl, const mq_index_desc* kp, const mq_index_desc* kr, const double* ket_rp,
