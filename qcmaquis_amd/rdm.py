@@ -25,6 +25,7 @@ import itertools
 
 import numpy as np
 
+from . import abi
 from .engine import DeviceBoundary, DeviceMPO, DeviceMPS, lstep
 from .qc import QCHamiltonian
 
@@ -66,18 +67,20 @@ def eri_images(p, q, r, s):
     return sorted(out)
 
 
-def h_gradient(ctx, layouts, data, entries):
+def h_gradient(ctx, layouts, data, entries, symmetry=abi.MQ_SYM_TWOU1):
     """dE/dh_pq for the (p, q) entries (h symmetric: h_pq and h_qp are one parameter)"""
     L = len(layouts)
     zero = np.zeros((L, L, L, L))
-    return np.array([expval(ctx, layouts, data, QCHamiltonian(unit_h(L, p, q), zero)) for p, q in entries])
+    return np.array([expval(ctx, layouts, data, QCHamiltonian(unit_h(L, p, q), zero, symmetry=symmetry))
+                     for p, q in entries])
 
 
-def eri_gradient(ctx, layouts, data, entries):
+def eri_gradient(ctx, layouts, data, entries, symmetry=abi.MQ_SYM_TWOU1):
     """dE/d(pq|rs) for the (p, q, r, s) entries (all eight permutational images one parameter)"""
     L = len(layouts)
     zero = np.zeros((L, L))
-    return np.array([expval(ctx, layouts, data, QCHamiltonian(zero, unit_eri(L, *e))) for e in entries])
+    return np.array([expval(ctx, layouts, data, QCHamiltonian(zero, unit_eri(L, *e), symmetry=symmetry))
+                     for e in entries])
 
 
 def eri_orbits(L):
