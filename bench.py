@@ -274,6 +274,98 @@ def qc3_sweep_bench(args, ctx):
     print(json.dumps(line), flush=True)
 
 
+def adjoint_bench(args, ctx):
+    """Configuration 5: 16 orbitals, 16 electrons, SU2U1, S=0, M=512, converged two-site DMRG
+    ground state; the integral gradients dE/dh and dE/d(pq|rs) (the 1- and 2-RDMs) from stored
+    boundaries (qcmaquis_amd.adjoint), with every right boundary kept and with the kept ones capped
+    at 25 % of their bytes; checked against central finite differences of <psi|H(h, eri)|psi> on 32
+    random integral entries (step 1e-5), the derivative-MPO expectation values of a few entries
+    (one chain growth each, the per-element measurement the adjoint replaces), the RDM traces and
+    Euler's identity (sum theta dE/dtheta = E - E_core)."""
+    from qcmaquis_amd import abi
+    from qcmaquis_amd.adjoint import energy_gradient, rdms
+    from qcmaquis_amd.dmrg import TwoSiteDMRG, right_normalize
+    from qcmaquis_amd.qc import QCHamiltonian, make_integrals
+    from qcmaquis_amd.rdm import eri_gradient, expval, h_gradient
+    SU2 = abi.MQ_SYM_SU2U1
+    L, N, M, seed = 16, 16, args.M or 512, 1005
+    h, eri, ec = make_integrals(L, seed, 16)
+    t0 = time.perf_counter()
+    ham = QCHamiltonian(h, eri, ec, symmetry=SU2, gradient_layout=True)
+    t_mpo = time.perf_counter() - t0
+    d = TwoSiteDMRG(ham, N // 2, N // 2, M=M, ctx=ctx, tol=1e-10, max_iter=12)
+    energies, prev, t_dmrg = [], None, time.perf_counter()
+    for s in range(12):
+        e, dt = d.sweep()
+        energies.append(e)
+        log("adjoint workload: sweep %d E %.12f (%.2f s)" % (s, e, dt))
+        if prev is not None and abs(e - prev) < 1e-9:
+            break
+        prev = e
+    t_dmrg = time.perf_counter() - t_dmrg
+    runs = {}
+    for name, cap in (("all_kept", None), ("capped_25pct", 0.25)):
+        E, dh, deri, st = energy_gradient(ctx, ham, d.lay, d.data, mem_fraction=cap)
+        runs[name] = {"seconds": round(st["seconds"], 3), "energy": E, "right_steps": st["right_steps"],
+                      "regrown": st["regrown"], "all_right_MiB": round(st["all_right_bytes"] / 2 ** 20, 1),
+                      "peak_kept_MiB": round(st["peak_kept_bytes"] / 2 ** 20, 1),
+                      "peak_boundary_MiB": round(st["peak_boundary_bytes"] / 2 ** 20, 1),
+                      "site_energy_spread": float(np.ptp(st["site_energies"]))}
+        runs[name]["_g"] = (dh, deri)
+        log("adjoint workload: %s %.3f s, E %.12f, %s" % (name, st["seconds"], E,
+                                                        {k: v for k, v in runs[name].items() if k != "_g"}))
+    dh, deri = runs["capped_25pct"].pop("_g")
+    dh0, deri0 = runs["all_kept"].pop("_g")
+    scale = max(np.max(np.abs(dh0)), np.max(np.abs(deri0)))
+    cap_diff = max(np.max(np.abs(dh - dh0)), np.max(np.abs(deri - deri0))) / scale
+    gamma, gam = rdms(dh, deri)
+    euler = float(np.sum(dh * h) + np.sum(deri * eri)) - (runs["capped_25pct"]["energy"] - ec)
+    # finite differences of the fixed normalised state's energy
+    lay = list(d.lay)
+    dat = [np.array(x) for x in d.data]
+    right_normalize(lay, dat)
+    hs = [(i, j) for i in range(L) for j in range(i + 1)]
+    es = [(i, j, k, l) for i in range(L) for j in range(i + 1) for k in range(i + 1) for l in range(k + 1)
+          if not (k == i and l > j)]
+    rng = np.random.default_rng(5)
+    picks = [("h", hs[i]) for i in rng.choice(len(hs), 12, replace=False)] + \
+            [("eri", es[i]) for i in rng.choice(len(es), 20, replace=False)]
+    step, worst, t_fd = 1e-5, 0.0, time.perf_counter()
+    for kind, idx in picks:
+        vals = []
+        for sgn in (1.0, -1.0):
+            hp, ep = h.copy(), eri.copy()
+            (hp if kind == "h" else ep)[idx] += sgn * step
+            vals.append(expval(ctx, lay, dat, QCHamiltonian(hp, ep, symmetry=SU2)))
+        worst = max(worst, abs((vals[0] - vals[1]) / (2 * step) - (dh if kind == "h" else deri)[idx]))
+    t_fd = time.perf_counter() - t_fd
+    # the per-element measurement: one derivative-MPO chain growth per entry
+    t1 = time.perf_counter()
+    sample_h = hs[:2]
+    sample_e = es[len(es) // 2:len(es) // 2 + 2]
+    ref_h = h_gradient(ctx, lay, dat, sample_h, SU2)
+    ref_e = eri_gradient(ctx, lay, dat, sample_e, SU2)
+    t_elem = (time.perf_counter() - t1) / 4
+    elem_diff = max(max(abs(a - dh[x]) for a, x in zip(ref_h, sample_h)),
+                    max(abs(a - deri[x]) for a, x in zip(ref_e, sample_e)))
+    n_params = len(hs) + len(es)
+    line = {"metric": "integral gradients (1- and 2-RDM) from stored boundaries, configuration 5",
+            "value": runs["capped_25pct"]["seconds"], "unit": "s (all dE/dh and dE/d(pq|rs), boundaries capped at 25%)",
+            "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic (seeded rank-16 integrals)",
+            "config": {"workload": f"configuration 5: {L} orbitals, {N} electrons, SU2U1, S=0, M={M}, converged "
+                                   "two-site DMRG ground state",
+                       "mpo_seconds": round(t_mpo, 3), "dmrg_seconds": round(t_dmrg, 3), "sweep_energies": energies,
+                       "runs": runs, "capped_vs_all_kept_rel_diff": cap_diff,
+                       "fd_check": {"entries": len(picks), "step": step, "max_abs_error": worst,
+                                    "seconds": round(t_fd, 3)},
+                       "per_element_measurement": {"entries": 4, "max_abs_diff": elem_diff,
+                                                   "seconds_per_entry": round(t_elem, 4), "parameters": n_params,
+                                                   "extrapolated_seconds_all": round(t_elem * n_params, 1)},
+                       "euler_residual": euler, "trace_gamma": float(np.trace(gamma)),
+                       "trace_Gamma_pprr": float(np.einsum("pprr->", gam))}}
+    print(json.dumps(line), flush=True)
+
+
 def batch_sweep_bench(args, ctx):
     """Configuration 4 as sweeps: `--instances` runs (sizes and integrals as in batch_bench), each
     a two-site DMRG from a right-normalised random MPS of bond dimension about M; one sweep of
@@ -648,7 +740,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="qc3", choices=["qc", "qc3", "qc3-sweep", "synthetic", "batch", "batch-sweep"])
+    ap.add_argument("--workload", default="qc3",
+                    choices=["qc", "qc3", "qc3-sweep", "synthetic", "batch", "batch-sweep", "adjoint"])
     ap.add_argument("--plain-sweep", action="store_true", help="qc3-sweep: also sweep with the reference-style FP64")
     ap.add_argument("--sweep-runs", default="fp64,mixed", help="qc3-sweep: runs (fp64, mixed, fp64_plain)")
     ap.add_argument("--M", type=int, default=None, help="bond dimension of the qc / qc3 workloads")
@@ -685,7 +778,11 @@ def main():
             torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}), flush=True)
         return
-    ctx = Context(0, abi.MQ_SYM_SU2U1 if args.workload in ("qc3", "qc3-sweep") else abi.MQ_SYM_TWOU1)
+    ctx = Context(0, abi.MQ_SYM_SU2U1 if args.workload in ("qc3", "qc3-sweep", "adjoint") else abi.MQ_SYM_TWOU1)
+    if args.workload == "adjoint":
+        adjoint_bench(args, ctx)
+        ctx.close()
+        return
     if args.workload == "qc3-sweep":
         if args.chunk_mb is not None:
             ctx.set_option(abi.MQ_OPT_CHUNK_MB, args.chunk_mb)

@@ -77,17 +77,28 @@ void hipcheck(hipError_t e, const char* what) {
 // Device blocks of 1 MiB and more are kept for reuse when freed: a sweep creates and destroys
 // plans of similar sizes at every site, and hipMalloc / hipFree of gigabyte buffers cost
 // milliseconds each (hipFree also waits for the device).  A freed block serves a later request
-// of at most its size and at least 4/5 of it on the same device; at most CACHE_CAP bytes are kept
-// (the smallest idle blocks are released first: the large plan buffers of one site are the ones
-// the next site can reuse, and small blocks are cheap to allocate again), and all idle blocks are
-// released before an allocation that the device cannot satisfy is given up.  A block enters the cache after a device
+// of at most its size and at least 4/5 of it on the same device; at most CACHE_CAP bytes are kept,
+// the blocks idle longest released first (a sweep's next sites reuse what its last sites freed,
+// large plan buffers and small tables alike), and all idle blocks are released before an
+// allocation that the device cannot satisfy is given up.  A block enters the cache after a device
 // synchronisation (as hipFree waits for the device), so a reused block is idle; like a fresh
 // hipMalloc it holds undefined contents.
 struct DevCache {
+    struct Idle {
+        int dev;
+        void* p;
+        uint64_t seq;  // when it became idle
+    };
     std::mutex m;
-    std::unordered_map<void*, std::pair<int, uint64_t>> live;    // block -> (device, bytes)
-    std::multimap<uint64_t, std::pair<int, void*>> idle;         // bytes -> (device, block)
-    uint64_t idle_bytes = 0;
+    std::unordered_map<void*, std::pair<int, uint64_t>> live;   // block -> (device, bytes)
+    std::multimap<uint64_t, Idle> idle;                         // bytes -> idle block
+    std::map<uint64_t, std::multimap<uint64_t, Idle>::iterator> age;  // seq -> its idle entry
+    uint64_t idle_bytes = 0, next_seq = 0;
+    void drop(std::multimap<uint64_t, Idle>::iterator it) {  // forget an idle entry (no hipFree)
+        idle_bytes -= it->first;
+        age.erase(it->second.seq);
+        idle.erase(it);
+    }
 };
 constexpr uint64_t CACHE_MIN = uint64_t(1) << 20;
 constexpr uint64_t CACHE_CAP = uint64_t(96) << 30;
@@ -97,20 +108,23 @@ DevCache& devcache() {
     return *c;
 }
 
-// releases the idle blocks (of one device, or all with dev < 0); caller holds the lock
-void cache_release_locked(DevCache& c, int dev) {
+void free_on(int dev, void* p) {
     int cur = 0;
     (void)hipGetDevice(&cur);
+    if (dev != cur) (void)hipSetDevice(dev);
+    (void)hipFree(p);
+    if (dev != cur) (void)hipSetDevice(cur);
+}
+
+// releases the idle blocks (of one device, or all with dev < 0); caller holds the lock
+void cache_release_locked(DevCache& c, int dev) {
     for (auto it = c.idle.begin(); it != c.idle.end();) {
-        if (dev >= 0 && it->second.first != dev) {
-            ++it;
-            continue;
+        auto next = std::next(it);
+        if (dev < 0 || it->second.dev == dev) {
+            free_on(it->second.dev, it->second.p);
+            c.drop(it);
         }
-        if (it->second.first != cur) (void)hipSetDevice(it->second.first);
-        (void)hipFree(it->second.second);
-        if (it->second.first != cur) (void)hipSetDevice(cur);
-        c.idle_bytes -= it->first;
-        it = c.idle.erase(it);
+        it = next;
     }
 }
 
@@ -124,11 +138,10 @@ T* dev_alloc(uint64_t n) {
     std::lock_guard<std::mutex> lock(c.m);
     if (bytes >= CACHE_MIN) {
         for (auto it = c.idle.lower_bound(bytes); it != c.idle.end() && it->first <= bytes + bytes / 4; ++it)
-            if (it->second.first == dev) {
-                void* p = it->second.second;
+            if (it->second.dev == dev) {
+                void* p = it->second.p;
                 c.live[p] = {dev, it->first};
-                c.idle_bytes -= it->first;
-                c.idle.erase(it);
+                c.drop(it);
                 return static_cast<T*>(p);
             }
     }
@@ -219,18 +232,15 @@ void dev_free(void* p) {
     }
     // work on any stream may still read the block (hipFree would wait for the device too)
     (void)hipDeviceSynchronize();
-    c.idle.emplace(it->second.second, std::make_pair(it->second.first, p));
+    const uint64_t seq = c.next_seq++;
+    auto ins = c.idle.emplace(it->second.second, DevCache::Idle{it->second.first, p, seq});
+    c.age.emplace(seq, ins);
     c.idle_bytes += it->second.second;
     c.live.erase(it);
-    int cur = 0;
-    (void)hipGetDevice(&cur);
     while (c.idle_bytes > CACHE_CAP) {
-        auto first = c.idle.begin();
-        if (first->second.first != cur) (void)hipSetDevice(first->second.first);
-        (void)hipFree(first->second.second);
-        if (first->second.first != cur) (void)hipSetDevice(cur);
-        c.idle_bytes -= first->first;
-        c.idle.erase(first);
+        auto oldest = c.age.begin()->second;
+        free_on(oldest->second.dev, oldest->second.p);
+        c.drop(oldest);
     }
 }
 
