@@ -756,6 +756,13 @@ def main():
     ap.add_argument("--no-davidson", action="store_true", help="qc3: skip the Davidson precision study")
     ap.add_argument("--chunk-mb", type=int, default=None,
                     help="largest T + S of one plan chunk in MiB (0: one chunk; default: the library's rule)")
+    ap.add_argument("--sblock-panel", type=int, default=None,
+                    help="term-sum work order: blocks per panel (0: block by block; default: the library's rule)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="qc / qc3: schedule the benchmark plan's GEMM stages by the default tile rule instead of "
+                         "the autotuner's timed pick (MQ_OPT_AUTOTUNE)")
+    ap.add_argument("--sblock-xcd", type=int, default=None,
+                    help="1: each XCD takes a contiguous range of the term-sum work items")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy inside the profiler region (HBM counter scale)")
     args = ap.parse_args()
@@ -779,6 +786,11 @@ def main():
         print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": b.numel() * 8}), flush=True)
         return
     ctx = Context(0, abi.MQ_SYM_SU2U1 if args.workload in ("qc3", "qc3-sweep", "adjoint") else abi.MQ_SYM_TWOU1)
+    if args.sblock_panel is not None:
+        ctx.set_option(abi.MQ_OPT_SBLOCK_PANEL, args.sblock_panel)
+    if args.sblock_xcd is not None:
+        ctx.set_option(abi.MQ_OPT_SBLOCK_XCD, args.sblock_xcd)
+
     if args.workload == "adjoint":
         adjoint_bench(args, ctx)
         ctx.close()
@@ -803,7 +815,12 @@ def main():
     log("workload built in %.1f s: %s" % (time.perf_counter() - t0, w["desc"]))
     if args.chunk_mb is not None:
         ctx.set_option(abi.MQ_OPT_CHUNK_MB, args.chunk_mb)
+    # the benchmark plan's GEMM stages take the autotuner's timed pick (plan creation, untimed);
+    # the sweeps below keep the default rule (their stage keys rarely repeat)
+    autotune = args.workload in ("qc", "qc3") and not args.no_autotune
+    ctx.set_option(abi.MQ_OPT_AUTOTUNE, int(autotune))
     plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
+    ctx.set_option(abi.MQ_OPT_AUTOTUNE, 0)
     st = plan.stats()
     log("plan:", {k: st[k] for k in ("n_t_tasks", "n_cohorts", "n_lines", "n_terms", "flops_total", "flops_gemm",
                                       "flops_gemm_exec", "bytes_compulsory", "t_elements", "s_elements",
@@ -981,6 +998,11 @@ def main():
         "roofline": dominant,
         "roofline_other": other,
     }
+    if autotune:
+        line["config"]["autotune"] = {"picks_t_ls": [st["gemm_policy_t"], st["gemm_policy_c"]],
+                                      "seconds_tune": round(st["seconds_tune"], 3),
+                                      "table": [{"key": k, "kind": kd, "pick": pk, "best_ms": round(b, 4),
+                                                 "default_ms": round(dft, 4)} for k, kd, pk, b, dft in ctx.tuned()]}
     plan.close()
     if args.workload in ("qc", "qc3") and not args.no_sweeps:
         # real sweeps run at configuration 2 (24 orbitals, M = 512): a 40-orbital M = 2048 sweep takes

@@ -70,6 +70,18 @@ extern "C" {
                                   run one after the other on buffers of the largest chunk's size;
                                   0: one chunk; MQ_CHUNK_AUTO (default): 49152 MiB. */
 #define MQ_CHUNK_AUTO (-1)
+#define MQ_OPT_SBLOCK_PANEL 5  /* value >= 0: plans made afterwards order the term sum's work items
+                                  element range by element range over panels of this many
+                                  neighbouring blocks (which share most of their T tiles); 0: block
+                                  by block; MQ_SBLOCK_PANEL_AUTO (default): 0 */
+#define MQ_SBLOCK_PANEL_AUTO (-1)
+#define MQ_OPT_SBLOCK_XCD 6    /* value 1: plans made afterwards give each XCD a contiguous range of
+                                  the term sum's work items instead of every eighth (default 0) */
+#define MQ_OPT_AUTOTUNE 7      /* value 1: H*psi plans made afterwards choose the schedule of each
+                                  GEMM stage among the candidate tile rules by timing them, once per
+                                  key of the stage's (m, n, k) histogram; later stages with a known
+                                  key take the recorded pick without timing (mq_ctx_tuned lists
+                                  them).  Default 0: the default rule. */
 #define MQ_OPT_TERM_SOURCES 4  /* value 1: H*psi plans made afterwards (single site problems) also keep,
                                   per S term, the MPO term it comes from, for
                                   mq_hpsi_term_gradient (default 0) */
@@ -168,6 +180,9 @@ typedef struct {
     uint64_t sblock_macs;             /* matrix-core multiply-adds of the block term sum */
     uint64_t n_chunks;                /* chunks of the plan (MQ_OPT_CHUNK_MB) */
     uint64_t ts_buffer_elements;      /* T + S buffer elements (the largest chunk's) */
+    int32_t  gemm_policy_t, gemm_policy_c;  /* GEMM schedule candidates of the T and L*S stages
+                                               (chunk 0; 0 = the default rule, MQ_OPT_AUTOTUNE) */
+    double   seconds_tune;            /* device time spent timing schedule candidates (MQ_OPT_AUTOTUNE) */
 } mq_hpsi_stats_t;
 
 typedef struct {
@@ -196,6 +211,11 @@ int         mq_ctx_create(int device, int symmetry, mq_ctx_t* out);
 int         mq_ctx_destroy(mq_ctx_t ctx);
 const char* mq_last_error(mq_ctx_t ctx);
 int         mq_ctx_sync(mq_ctx_t ctx);
+/* the autotuner's table (MQ_OPT_AUTOTUNE): for up to n recorded stage keys (key, stage kind
+ * 0 = T-GEMMs / 1 = L*S, chosen candidate, its time and the default rule's time in ms); returns the
+ * number of recorded keys.  No counterpart in the reference (its GEMMs are library calls). */
+int         mq_ctx_tuned(mq_ctx_t ctx, int n, uint64_t* keys, int32_t* kinds, int32_t* picks, float* best_ms,
+                         float* default_ms);
 void*       mq_ctx_stream(mq_ctx_t ctx);      /* hipStream_t of the context */
 int         mq_ctx_set_option(mq_ctx_t ctx, int option, int value);
 int         mq_version(void);

@@ -27,6 +27,10 @@ MQ_SMALL_TILE_AUTO = -1
 MQ_OPT_CHUNK_MB = 3
 MQ_CHUNK_AUTO = -1
 MQ_OPT_TERM_SOURCES = 4
+MQ_OPT_SBLOCK_PANEL = 5
+MQ_SBLOCK_PANEL_AUTO = -1
+MQ_OPT_SBLOCK_XCD = 6
+MQ_OPT_AUTOTUNE = 7
 MQ_QC_GRADIENT = 1
 MQ_RIGHT_PAIRED = 0
 MQ_LEFT_PAIRED = 1
@@ -114,6 +118,9 @@ class HpsiStats(C.Structure):
         ("sblock_macs", C.c_uint64),
         ("n_chunks", C.c_uint64),
         ("ts_buffer_elements", C.c_uint64),
+        ("gemm_policy_t", C.c_int32),
+        ("gemm_policy_c", C.c_int32),
+        ("seconds_tune", C.c_double),
     ]
 
 
@@ -169,6 +176,7 @@ SIGNATURES = [
     ("mq_hpsi_host", i32, [vp, dp, dp, i32]),
     ("mq_hpsi_plan_use_graphs", i32, [vp, i32]),
     ("mq_hpsi_term_gradient", i32, [vp, dp, dp, C.c_int64]),
+    ("mq_ctx_tuned", i32, [vp, i32, P(C.c_uint64), P(i32), P(i32), P(C.c_float), P(C.c_float)]),
     ("mq_hpsi_plan_destroy", i32, [vp]),
     ("mq_davidson", i32, [vp, dp, i32, P(dp), C.c_double, i32, i32, dp, dp, P(DavidsonStats)]),
     ("mq_davidson_noalloc", i32, [vp, dp, i32, P(dp), C.c_double, i32, i32, dp, dp, P(DavidsonStats)]),

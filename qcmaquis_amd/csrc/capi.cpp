@@ -38,6 +38,14 @@ struct mq_ctx_s {
     int32_t small_tile_max = MQ_SMALL_TILE_AUTO;  // MQ_OPT_SMALL_TILE_MAX
     int32_t chunk_mb = MQ_CHUNK_AUTO;             // MQ_OPT_CHUNK_MB
     bool term_sources = false;         // MQ_OPT_TERM_SOURCES
+    int32_t sblock_panel = MQ_SBLOCK_PANEL_AUTO;  // MQ_OPT_SBLOCK_PANEL
+    bool sblock_xcd = false;                      // MQ_OPT_SBLOCK_XCD
+    bool autotune = false;                        // MQ_OPT_AUTOTUNE
+    struct Tuned {
+        int32_t pick;
+        float best_ms, default_ms;
+    };
+    std::map<std::pair<uint64_t, int>, Tuned> tuned;  // (stage key, kind) -> pick
     rocblas_handle blas = nullptr;     // library GEMMs of MQ_PREC_FP64_PLAIN, made on first use
     double* noalloc_scratch = nullptr; // this device's scratch array of mq_davidson_noalloc
 };
@@ -345,7 +353,9 @@ struct SAsmStage {
     SLine* slines = nullptr;
     STerm* sterms = nullptr;
     int64_t n_swork = 0;
-    void upload(SBlockTable const& B, hipStream_t s) {
+    bool xcd_contiguous = false;  // the work list is panel-ordered: each XCD takes a range of it
+    void upload(SBlockTable const& B, hipStream_t s, bool xcd = false) {
+        xcd_contiguous = xcd;
         blocks = dev_upload(B.blocks, s);
         work = dev_upload(B.work, s);
         utile = dev_upload(B.utile, s);
@@ -362,7 +372,8 @@ struct SAsmStage {
     }
     void run(const double* T, double* S, hipStream_t s) const {
         if (n_work)
-            hipcheck(launch_sblocks<double>(blocks, work, n_work, utile, kstep, ualpha, lsoff, llds, T, S, s), "S-assembly");
+            hipcheck(launch_sblocks<double>(blocks, work, n_work, utile, kstep, ualpha, lsoff, llds, T, S, xcd_contiguous, s),
+                     "S-assembly");
         if (n_swork) hipcheck(launch_small_lines<double>(swork, n_swork, slines, sterms, T, S, s), "S-assembly");
     }
     void ensure32(hipStream_t s) {
@@ -374,7 +385,9 @@ struct SAsmStage {
     void run32(const float* T, float* S, hipStream_t s) {
         ensure32(s);
         if (n_work)
-            hipcheck(launch_sblocks<float>(blocks, work, n_work, utile, kstep, ualpha32, lsoff, llds, T, S, s), "S-assembly");
+            hipcheck(launch_sblocks<float>(blocks, work, n_work, utile, kstep, ualpha32, lsoff, llds, T, S, xcd_contiguous,
+                                           s),
+                     "S-assembly");
         if (n_swork) hipcheck(launch_small_lines<float>(swork, n_swork, slines, sterms, T, S, s), "S-assembly");
     }
     void release() {
@@ -387,6 +400,9 @@ struct SAsmStage {
         n_swork = 0;
     }
 };
+
+// blocks per panel of the term-sum work order (MQ_OPT_SBLOCK_PANEL)
+int32_t sblock_panel(mq_ctx_t ctx) { return ctx->sblock_panel == MQ_SBLOCK_PANEL_AUTO ? 0 : ctx->sblock_panel; }
 
 Index index_from(const mq_index_desc* d) {
     require(d != nullptr, "null index descriptor");
@@ -420,6 +436,9 @@ struct mq_hpsi_plan_s {
     };
     std::vector<Chunk> chunks;
     double* gemm_ws = nullptr;   // split-K workspace shared by the chunks' GEMM stages
+    uint64_t gemm_ws_elems = 0;
+    int32_t policy_t = 0, policy_c = 0;  // schedule candidates of chunk 0's stages
+    double seconds_tune = 0;
     int64_t n_lines = 0, n_sblocks = 0, n_reduces = 0;
     uint64_t s_line_elements = 0, gathered_elements = 0, union_elements = 0, max_union_tiles = 0, sblock_macs = 0;
     double *T = nullptr, *S = nullptr, *Lpanel = nullptr;
@@ -569,10 +588,35 @@ int mq_ctx_set_option(mq_ctx_t ctx, int option, int value) {
             ctx->chunk_mb = value;
         } else if (option == MQ_OPT_TERM_SOURCES) {
             ctx->term_sources = value != 0;
+        } else if (option == MQ_OPT_SBLOCK_PANEL) {
+            require(value == MQ_SBLOCK_PANEL_AUTO || value >= 0,
+                    "mq_ctx_set_option: MQ_OPT_SBLOCK_PANEL is MQ_SBLOCK_PANEL_AUTO or >= 0");
+            ctx->sblock_panel = value;
+        } else if (option == MQ_OPT_SBLOCK_XCD) {
+            ctx->sblock_xcd = value != 0;
+        } else if (option == MQ_OPT_AUTOTUNE) {
+            ctx->autotune = value != 0;
         } else {
             throw Error(MQ_ERR_INVALID, "mq_ctx_set_option: unknown option");
         }
     });
+}
+
+int mq_ctx_tuned(mq_ctx_t ctx, int n, uint64_t* keys, int32_t* kinds, int32_t* picks, float* best_ms,
+                 float* default_ms) {
+    if (!ctx || n < 0) return MQ_ERR_INVALID;
+    int i = 0;
+    for (auto const& kv : ctx->tuned) {
+        if (i < n) {
+            if (keys) keys[i] = kv.first.first;
+            if (kinds) kinds[i] = kv.first.second;
+            if (picks) picks[i] = kv.second.pick;
+            if (best_ms) best_ms[i] = kv.second.best_ms;
+            if (default_ms) default_ms[i] = kv.second.default_ms;
+        }
+        ++i;
+    }
+    return i;
 }
 
 int mq_ctx_sync(mq_ctx_t ctx) {
@@ -760,7 +804,7 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
     check_gemms(P.c_gemms, sizes);
     check_lines(P.lines, P.terms, P.t_elems, P.s_elems);
     st.seconds_tables = seconds_since(t);
-    SBlockTable SB = build_sblocks(P.lines, P.terms);
+    SBlockTable SB = build_sblocks(P.lines, P.terms, sblock_panel(ctx));
     check_sblocks(SB, P.t_elems, P.s_elems);
     st.seconds_blocks = seconds_since(t);
     st.n_lines = P.lines.size();
@@ -784,7 +828,7 @@ static void run_growth(mq_ctx_t ctx, GrowPlanHost& P, double const* bnd, double 
     try {
         tg.upload(P.t_gemms, P.t_rows, ctx->stream);
         cg.upload(P.c_gemms, {}, ctx->stream);
-        sa.upload(SB, ctx->stream);
+        sa.upload(SB, ctx->stream, ctx->sblock_xcd);
         st.seconds_upload = seconds_since(t);
         Bases b{};
         b.p[BASE_T] = T;
@@ -849,6 +893,96 @@ struct LpanelPart {
     std::vector<CopyJob> jobs;
 };
 
+// GEMM tile autotuner: each GEMM stage of the plan whose histogram key the context has not seen
+// is timed under every schedule candidate (gemm_policy; one warm-up launch, then the faster of
+// two timed launches, HIP events on the context stream); the fastest is recorded for the key and
+// a stage with a known key takes the recorded candidate.  The stage is re-uploaded with its
+// schedule unless that is the default one it already has.
+static void autotune_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<std::vector<GemmDesc>> const& tgs,
+                          std::vector<std::vector<GemmRow>> const& trs, std::vector<std::vector<GemmDesc>> const& cgs) {
+    hipStream_t s = ctx->stream;
+    double* psi_in = dev_alloc<double>(p->psi_n);
+    double* psi_out = dev_alloc<double>(p->psi_n);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto t_start = std::chrono::steady_clock::now();
+    try {
+        hipcheck(launch_fill(psi_in, p->psi_n, 1.0 / std::sqrt(double(std::max<uint64_t>(p->psi_n, 1))), s), "tune");
+        hipcheck(hipEventCreate(&e0), "event");
+        hipcheck(hipEventCreate(&e1), "event");
+        Bases b{};
+        b.p[BASE_PSI_IN] = psi_in;
+        b.p[BASE_PSI_OUT] = psi_out;
+        b.p[BASE_T] = p->T;
+        b.p[BASE_S] = p->S;
+        b.p[BASE_LPANEL] = p->Lpanel;
+        b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
+        auto time_stage = [&](GemmStage const& st) {
+            float best = 1e30f;
+            st.run(b, s, GEMM_F64);
+            for (int r = 0; r < 2; ++r) {
+                hipcheck(hipEventRecord(e0, s), "event");
+                st.run(b, s, GEMM_F64);
+                hipcheck(hipEventRecord(e1, s), "event");
+                hipcheck(hipEventSynchronize(e1), "event");
+                float ms = 0;
+                hipcheck(hipEventElapsedTime(&ms, e0, e1), "event");
+                best = std::min(best, ms);
+            }
+            return best;
+        };
+        for (size_t c = 0; c < p->chunks.size(); ++c)
+            for (int kind = 0; kind < 2; ++kind) {
+                std::vector<GemmDesc> const& descs = kind == 0 ? tgs[c] : cgs[c];
+                std::vector<GemmRow> const& rows = kind == 0 ? trs[c] : std::vector<GemmRow>();
+                GemmStage& stage = kind == 0 ? p->chunks[c].tg : p->chunks[c].cg;
+                if (descs.empty()) continue;
+                const auto key = std::make_pair(gemm_stage_key(descs), kind);
+                auto it = ctx->tuned.find(key);
+                if (it == ctx->tuned.end()) {
+                    mq_ctx_s::Tuned t{0, 1e30f, 0.0f};
+                    for (int cand = 0; cand < GEMM_N_POLICIES; ++cand) {
+                        GemmSchedule sch = build_gemm_schedule(descs, gemm_policy(cand));
+                        GemmStage tmp;
+                        tmp.upload(descs, rows, s, &sch);
+                        float ms = 0;
+                        try {
+                            ms = time_stage(tmp);
+                        } catch (...) {
+                            (void)hipStreamSynchronize(s);
+                            tmp.release();
+                            throw;
+                        }
+                        tmp.release();
+                        if (cand == 0) t.default_ms = ms;
+                        if (ms < t.best_ms) t.best_ms = ms, t.pick = cand;
+                    }
+                    it = ctx->tuned.emplace(key, t).first;
+                }
+                const int pick = it->second.pick;
+                if (c == 0) (kind == 0 ? p->policy_t : p->policy_c) = pick;
+                if (pick == 0) continue;
+                GemmSchedule sch = build_gemm_schedule(descs, gemm_policy(pick));
+                (void)hipStreamSynchronize(s);
+                stage.release();
+                stage.upload(descs, rows, s, &sch, sch.ws_elems <= p->gemm_ws_elems ? p->gemm_ws : nullptr);
+            }
+        p->n_reduces = 0;
+        for (auto const& pc : p->chunks) p->n_reduces += pc.tg.n_reduces + pc.cg.n_reduces;
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        dev_free(psi_in);
+        dev_free(psi_out);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dev_free(psi_in);
+    dev_free(psi_out);
+    p->seconds_tune = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+}
+
 // device side of an H*psi plan whose host tables are in p->host (p->psi_n, p->rdata set): checks,
 // term-sum blocks, buffers, uploads, L panels; frees the host term tables
 static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart> const& lparts, uint64_t r_n,
@@ -904,7 +1038,7 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
         HostTable<SLine> cl(H.lines.begin() + int64_t(ch.line0), H.lines.begin() + int64_t(ch.line1));
         check_lines(cl, H.terms, ch.t_elems, ch.s_elems);
         auto tb = std::chrono::steady_clock::now();
-        sbs[c] = build_sblocks(cl, H.terms);
+        sbs[c] = build_sblocks(cl, H.terms, sblock_panel(ctx));
         check_sblocks(sbs[c], ch.t_elems, ch.s_elems);
         t_blocks += seconds_since(tb);
         tsch[c] = build_gemm_schedule(tgs[c]);
@@ -923,12 +1057,13 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
     p->S = dev_alloc<double>(H.s_buf);
     p->Lpanel = dev_alloc<double>(H.lpanel_elems);
     p->gemm_ws = dev_alloc<double>(ws_max);
+    p->gemm_ws_elems = ws_max;
     p->chunks.resize(nch);
     for (size_t c = 0; c < nch; ++c) {
         auto& pc = p->chunks[c];
         pc.tg.upload(tgs[c], trs[c], ctx->stream, &tsch[c], p->gemm_ws);
         pc.cg.upload(cgs[c], {}, ctx->stream, &csch[c], p->gemm_ws);
-        pc.sa.upload(sbs[c], ctx->stream);
+        pc.sa.upload(sbs[c], ctx->stream, ctx->sblock_xcd);
         p->n_sblocks += int64_t(sbs[c].blocks.size());
         p->union_elements += sbs[c].gathered;
         p->max_union_tiles = std::max<uint64_t>(p->max_union_tiles, sbs[c].max_union);
@@ -997,6 +1132,7 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
         hipcheck(e, "L panel copy");
         hipcheck(e2, "L panel copy");
     }
+    if (ctx->autotune) autotune_plan(ctx, p, tgs, trs, cgs);
     // the host tables are on the device now; keep only what the statistics need
     for (SLine const& l : H.lines) {
         uint64_t e = uint64_t(l.rows) * uint64_t(l.cols);
@@ -1194,6 +1330,9 @@ int mq_hpsi_plan_stats(mq_hpsi_plan_t p, mq_hpsi_stats_t* s) {
     s->split_reduces = uint64_t(p->n_reduces);
     s->n_chunks = uint64_t(p->chunks.size());
     s->ts_buffer_elements = H.t_buf + H.s_buf;
+    s->gemm_policy_t = p->policy_t;
+    s->gemm_policy_c = p->policy_c;
+    s->seconds_tune = p->seconds_tune;
     return MQ_OK;
 }
 
@@ -1668,6 +1807,20 @@ static void davidson_core(mq_hpsi_plan_t p, const double* psi0, int n_ortho, con
                 hipcheck(hipStreamSynchronize(s), "dot");
                 return h;
             };
+            // out[i] = x(i) . y(i) for i < count: the products into device scalars (one batch of at
+            // most DOT_BATCH at a time), then one download and one wait per batch
+            constexpr int SC_BATCH = 8, DOT_BATCH = 48;
+            static_assert(SC_BATCH + DOT_BATCH <= 64, "device scalars of the context");
+            auto dots_host = [&](int count, auto x_of, auto y_of, double* out) {
+                for (int c0 = 0; c0 < count; c0 += DOT_BATCH) {
+                    const int cn = std::min(DOT_BATCH, count - c0);
+                    for (int i = 0; i < cn; ++i)
+                        hipcheck(launch_dot(x_of(c0 + i), y_of(c0 + i), n, work, sc + SC_BATCH + i, s), "dot");
+                    hipcheck(hipMemcpyAsync(out + c0, sc + SC_BATCH, cn * sizeof(double), hipMemcpyDeviceToHost, s),
+                             "dot");
+                    hipcheck(hipStreamSynchronize(s), "dot");
+                }
+            };
             // y -= (x.y)/(x.x) x   with coefficients kept on the device
             auto subtract_along = [&](double* y, const double* x, bool normalized) {
                 hipcheck(launch_dot(x, y, n, work, sc + 1, s), "dot");
@@ -1702,7 +1855,11 @@ static void davidson_core(mq_hpsi_plan_t p, const double* psi0, int n_ortho, con
                 hpsi_run(p, t, VA[it], prec);
                 hipcheck(hipStreamSynchronize(s), "hpsi");
                 hpsi_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
-                for (int i = 0; i <= it; ++i) M[size_t(i) * max_iter + it] = dot_host(V[i], VA[it]);
+                {
+                    std::vector<double> row(size_t(it) + 1);
+                    dots_host(it + 1, [&](int i) { return V[i]; }, [&](int) { return VA[it]; }, row.data());
+                    for (int i = 0; i <= it; ++i) M[size_t(i) * max_iter + it] = row[size_t(i)];
+                }
                 int dim = it + 1;
                 std::vector<double> Ms(size_t(dim) * dim), svec;
                 for (int i = 0; i < dim; ++i)
@@ -1723,7 +1880,10 @@ static void davidson_core(mq_hpsi_plan_t p, const double* psi0, int n_ortho, con
                 bool conv = rnorm <= tol_run * std::fabs(theta) || rnorm < tol_run;  // iteration.h:57
                 if (conv || it >= iters) break;
                 // correction t = -r + (r.u)/(u.u) u  (jcd_gmres_solver with no GMRES steps, jacobi.h:152-156)
-                double dru = dot_host(r, u), duu = dot_host(u, u);
+                double du[2];
+                dots_host(2, [&](int i) { return i == 0 ? (const double*)r : (const double*)u; },
+                          [&](int) { return (const double*)u; }, du);
+                const double dru = du[0], duu = du[1];
                 hipcheck(launch_axpby(V[it], r, n, 0.0, -1.0, s), "axpby");
                 hipcheck(launch_axpby(V[it], u, n, 1.0, dru / duu, s), "axpby");
             }

@@ -223,6 +223,7 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __rest
 // group mask is read into a scalar register, so the groups without a sub-block are skipped by the
 // whole wave.
 constexpr int SB_NC = 2;                                // 16-element column tiles per wave
+constexpr int SB_XCDS = 8;                              // XCDs of an MI355X
 constexpr int SB_WAVES = SBLOCK_ELEMS / (16 * SB_NC);   // waves per workgroup
 constexpr int SB_THREADS = 64 * SB_WAVES;
 constexpr int SB_UNROLL = 2;                            // k-steps whose loads are issued together
@@ -238,12 +239,22 @@ __global__ __launch_bounds__(SB_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
                                                      const int64_t* __restrict__ utile, const uint32_t* __restrict__ kstep,
                                                      const T* __restrict__ ualpha, const int64_t* __restrict__ lsoff,
                                                      const int32_t* __restrict__ llds, const T* __restrict__ X,
-                                                     T* __restrict__ S) {
+                                                     T* __restrict__ S, int xcd_contiguous) {
     typedef typename Mma<T>::acc_t acc_t;
     __shared__ int64_t uts[4 * SB_KCHUNK];
     __shared__ uint32_t kws[SB_KCHUNK];
     __shared__ T uas[SB_KCHUNK * SBLOCK_GROUPS * 64];
-    const SWork w = work[blockIdx.x];
+    // Workgroups are dispatched to the XCDs in turn (workgroup i on XCD i mod 8); with
+    // xcd_contiguous each XCD takes a contiguous range of the work list instead, so consecutive
+    // work items (neighbouring blocks' slices of shared T tiles) meet in one XCD's L2.  The map is
+    // a bijection of [0, gridDim.x).
+    int64_t wi = blockIdx.x;
+    if (xcd_contiguous) {
+        const int64_t n = gridDim.x, q = n / SB_XCDS, r = n % SB_XCDS;
+        const int64_t x = blockIdx.x % SB_XCDS, k = blockIdx.x / SB_XCDS;
+        wi = x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+    }
+    const SWork w = work[wi];
     const SBlock b = blocks[w.blk];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int n0 = w.n0 + wave * 16 * SB_NC;
@@ -623,19 +634,19 @@ hipError_t launch_slines(const SLine* lines, int64_t n_lines, const STerm* terms
 template <class T>
 hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_work, const int64_t* utile,
                           const uint32_t* kstep, const T* ualpha, const int64_t* lsoff, const int32_t* llds,
-                          const T* X, T* S, hipStream_t stream) {
+                          const T* X, T* S, bool xcd_contiguous, hipStream_t stream) {
     for (int64_t off = 0; off < n_work; off += 0x7fffffff) {
         int64_t cnt = n_work - off < 0x7fffffff ? n_work - off : 0x7fffffff;
         hipLaunchKernelGGL(sblock_kernel<T>, dim3(unsigned(cnt)), dim3(SB_THREADS), 0, stream, blocks, work + off, utile,
-                           kstep, ualpha, lsoff, llds, X, S);
+                           kstep, ualpha, lsoff, llds, X, S, int(xcd_contiguous));
     }
     return hipGetLastError();
 }
 template hipError_t launch_sblocks<double>(const SBlock*, const SWork*, int64_t, const int64_t*, const uint32_t*,
-                                           const double*, const int64_t*, const int32_t*, const double*, double*,
+                                           const double*, const int64_t*, const int32_t*, const double*, double*, bool,
                                            hipStream_t);
 template hipError_t launch_sblocks<float>(const SBlock*, const SWork*, int64_t, const int64_t*, const uint32_t*,
-                                          const float*, const int64_t*, const int32_t*, const float*, float*,
+                                          const float*, const int64_t*, const int32_t*, const float*, float*, bool,
                                           hipStream_t);
 
 template <class T>

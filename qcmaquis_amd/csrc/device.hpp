@@ -27,12 +27,11 @@ hipError_t launch_gemm(GemmPrec prec, const GemmDesc* probs, const GemmRow* rows
                        const int64_t* n_tiles, const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases,
                        hipStream_t stream);
 // block-form MPO-term sum (SBlockTable uploaded; alpha, T and S of type T): writes every S tile
-// of the table's lines
+// of the table's lines; xcd_contiguous: each XCD takes a contiguous range of the work list
 template <class T>
 hipError_t launch_sblocks(const SBlock* blocks, const SWork* work, int64_t n_work, const int64_t* utile,
                           const uint32_t* kstep, const T* ualpha, const int64_t* lsoff, const int32_t* llds,
-                          const T* X, T* S,
-                          hipStream_t stream);
+                          const T* X, T* S, bool xcd_contiguous, hipStream_t stream);
 // small-tile MPO-term sum (SBlockTable small_* tables uploaded; T and S of type T, coefficients
 // FP64, rounded to T): writes the S tiles of the small lines
 template <class T>

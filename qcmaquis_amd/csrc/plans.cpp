@@ -1431,7 +1431,8 @@ void mark_small_lines(HostTable<SLine>& lines, size_t l0, size_t l1, int32_t lim
     }
 }
 
-SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const& terms) {
+SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const& terms, int32_t panel) {
+    require(panel >= 0, "S blocks: negative panel");
     SBlockTable B;
     auto is_small = [&](SLine const& l) {
         return (l.flags & SLINE_SMALL) && l.rows > 0 && l.cols > 0;
@@ -1591,9 +1592,21 @@ SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const&
         blk.line0 = int32_t(bi * SBLOCK_LINES);
         blk.pad = 0;
     }
+    // work items: block by block, or (panel > 0) per panel of up to `panel` consecutive blocks of
+    // one tile shape, element range by element range, so that neighbouring blocks -- which share
+    // most of their union tiles -- read the same slice of those tiles in consecutive items
+    for (int32_t p0 = 0; p0 < int32_t(nb);) {
+        int32_t p1 = p0 + 1;
+        if (panel > 0)
+            while (p1 < int32_t(nb) && p1 - p0 < panel && B.blocks[size_t(p1)].E == B.blocks[size_t(p0)].E &&
+                   B.blocks[size_t(p1)].rows == B.blocks[size_t(p0)].rows)
+                ++p1;
+        for (int32_t e = 0; e < B.blocks[size_t(p0)].E; e += SBLOCK_ELEMS)
+            for (int32_t bi = p0; bi < p1; ++bi) B.work.push_back(SWork{bi, e});
+        p0 = p1;
+    }
     for (int32_t bi = 0; bi < int32_t(nb); ++bi) {
         SBlock const& blk = B.blocks[size_t(bi)];
-        for (int32_t e = 0; e < blk.E; e += SBLOCK_ELEMS) B.work.push_back(SWork{bi, e});
         B.gathered += uint64_t(tabs[size_t(bi)].nu) * uint64_t(blk.E);
         B.max_union = std::max<uint64_t>(B.max_union, tabs[size_t(bi)].nu);
         B.mac += uint64_t(blk.na) * 64 * uint64_t(blk.E);
@@ -1666,9 +1679,24 @@ void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size) {
                 bad = std::max(bad, 3);
         }
     }
-    for (SWork const& w : B.work)
-        if (!(w.blk >= 0 && size_t(w.blk) < B.blocks.size() && w.n0 >= 0 && w.n0 < B.blocks[size_t(w.blk)].E))
-            bad = std::max(bad, 4);
+    // every element range of every block in exactly one work item (in any order)
+    if (bad == 0) {
+        std::vector<uint64_t> first(B.blocks.size() + 1, 0);
+        for (size_t bi = 0; bi < B.blocks.size(); ++bi)
+            first[bi + 1] = first[bi] + uint64_t((B.blocks[bi].E + SBLOCK_ELEMS - 1) / SBLOCK_ELEMS);
+        std::vector<char> seen(first.back(), 0);
+        if (B.work.size() != first.back()) bad = std::max(bad, 4);
+        for (SWork const& w : B.work) {
+            if (!(w.blk >= 0 && size_t(w.blk) < B.blocks.size() && w.n0 >= 0 && w.n0 < B.blocks[size_t(w.blk)].E &&
+                  w.n0 % SBLOCK_ELEMS == 0)) {
+                bad = std::max(bad, 4);
+                continue;
+            }
+            char& c = seen[first[size_t(w.blk)] + uint64_t(w.n0 / SBLOCK_ELEMS)];
+            if (c) bad = std::max(bad, 4);
+            c = 1;
+        }
+    }
     // small-tile tables: 5 malformed line, 6 term tile outside T, 7 malformed work item
     const int64_t nsl = int64_t(B.small_lines.size());
 #pragma omp parallel for schedule(dynamic, 1024) reduction(max : bad)
@@ -1705,19 +1733,69 @@ void check_sblocks(SBlockTable const& B, uint64_t t_size, uint64_t s_size) {
 }
 
 // ---- grouped GEMM schedule ---------------------------------------------------------------------
-GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t target_items, int32_t min_chunk) {
+GemmPolicy gemm_policy(int i) {
+    require(i >= 0 && i < GEMM_N_POLICIES, "gemm policy: index out of range");
+    GemmPolicy p;
+    switch (i) {
+        case 1: p.wide = 1; break;              // 64- and 128-column tiles only by M
+        case 2: p.wide = 2; break;              // 128-column tiles whenever N > 64
+        case 3: p.tiles96 = 0; break;           // 128-row tiles from 65 rows
+        case 4: p.target_items = 8192; break;   // split along K more readily
+        case 5: p.target_items = 2048; break;   // split along K less
+        case 6: p.min_chunk = 512; break;       // longer K ranges per split tile
+        default: break;
+    }
+    return p;
+}
+
+uint64_t gemm_stage_key(std::vector<GemmDesc> const& probs) {
+    double w = 0, lm = 0, ln = 0, lk = 0, f32 = 0, f64 = 0, f96 = 0, tiles = 0;
+    bool rowmap = false;
+    for (auto const& g : probs) {
+        if (g.M <= 0 || g.N <= 0 || g.K <= 0) continue;
+        const double f = 2.0 * g.M * g.N * g.K * g.batch;
+        w += f;
+        lm += f * std::log2(double(g.M));
+        ln += f * std::log2(double(g.N));
+        lk += f * std::log2(double(g.K));
+        if (g.M <= 32) f32 += f;
+        else if (g.M <= 64) f64 += f;
+        else if (g.M <= 96) f96 += f;
+        tiles += std::ceil(g.M / 128.0) * std::ceil(g.N / 64.0) * g.batch;
+        rowmap |= (g.flags & GEMM_ROWMAP) != 0;
+    }
+    if (w == 0) return 0;
+    auto q = [](double x, double scale, uint64_t bits) {
+        return std::min<uint64_t>((uint64_t(1) << bits) - 1, uint64_t(std::max(0.0, std::floor(x * scale + 0.5))));
+    };
+    uint64_t k = 0;
+    k = (k << 7) | q(lm / w, 4, 7);
+    k = (k << 7) | q(ln / w, 4, 7);
+    k = (k << 7) | q(lk / w, 4, 7);
+    k = (k << 4) | q(f32 / w, 8, 4);
+    k = (k << 4) | q(f64 / w, 8, 4);
+    k = (k << 4) | q(f96 / w, 8, 4);
+    k = (k << 6) | q(std::log2(std::max(1.0, tiles)), 1, 6);
+    k = (k << 1) | uint64_t(rowmap);
+    return k + 1;
+}
+
+GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, GemmPolicy const& pol) {
+    const int64_t target_items = pol.target_items;
+    const int32_t min_chunk = pol.min_chunk;
     require(min_chunk >= 16 && min_chunk % 16 == 0, "gemm schedule: chunk must be a multiple of 16");
+    require(pol.wide >= 0 && pol.wide <= 2, "gemm schedule: wide mode 0, 1 or 2");
     GemmSchedule S;
     // 96-row tiles (for 64 < M <= 96) only for problems with a plain A operand: a stacked problem's
     // threads each load one row, which needs a tile height that divides 256.  64x128 tiles (for
     // 32 < M <= 64, N > 64) only when the stage has too few tiles to fill the device and is split
     // along K anyway: then the wider tile halves the A re-reads at no cost in parallelism
-    bool wide = false;
+    bool wide = pol.wide == 2;
     auto shape_of = [&](GemmDesc const& g) {
         const bool w = wide && g.N > 64 && !(g.flags & GEMM_ROWMAP);
         return g.M <= 32 ? GEMM_32x128
              : g.M <= 64 ? ((wide && g.N > 64) ? GEMM_64x128 : GEMM_64x64)
-             : (g.M <= 96 && !(g.flags & GEMM_ROWMAP)) ? GEMM_96x64
+             : (pol.tiles96 && g.M <= 96 && !(g.flags & GEMM_ROWMAP)) ? GEMM_96x64
              : w ? GEMM_128x128 : GEMM_128x64;
     };
     auto tiles_of = [&](GemmDesc const& g) {
@@ -1736,7 +1814,7 @@ GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t tar
     int64_t n_tiles = 0;
     double work = 0;
     count(n_tiles, work);
-    if (n_tiles < target_items && target_items > 0) {
+    if (n_tiles < target_items && target_items > 0 && pol.wide == 0) {
         wide = true;
         count(n_tiles, work);
     }

@@ -205,15 +205,40 @@ struct GemmSchedule {
         return n;
     }
 };
-// target_items: number of workgroups a stage should provide before K is split;
-// min_chunk: shortest K range of a split tile (a multiple of the K step 16)
-GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t target_items = 4096,
-                                 int32_t min_chunk = 256);
+// The choices of a grouped GEMM schedule that the tile autotuner varies.  wide: 0 = 128-column
+// tiles (64x128, 128x128) for N > 64 when the stage is split along K, 1 = never, 2 = always;
+// tiles96: 96-row tiles for 64 < M <= 96 (problems without a row table), else 128 rows;
+// target_items: workgroups a stage should provide before K is split; min_chunk: shortest K range
+// of a split tile (a multiple of the K step 16)
+struct GemmPolicy {
+    int32_t wide = 0;
+    int32_t tiles96 = 1;
+    int64_t target_items = 4096;
+    int32_t min_chunk = 256;
+};
+constexpr int GEMM_N_POLICIES = 7;
+// candidate i of the autotuner (0: the default rule)
+GemmPolicy gemm_policy(int i);
+GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, GemmPolicy const& pol);
+inline GemmSchedule build_gemm_schedule(std::vector<GemmDesc> const& probs, int64_t target_items = 4096,
+                                        int32_t min_chunk = 256) {
+    GemmPolicy p;
+    p.target_items = target_items;
+    p.min_chunk = min_chunk;
+    return build_gemm_schedule(probs, p);
+}
+// Key of a stage's (m, n, k) histogram for the autotuner: the flop-weighted means of log2 M,
+// log2 N and log2 K (in quarters), the share of flops in problems of at most 32, 64 and 96 rows
+// (in eighths), log2 of the output tiles at the default rule, and whether rows come from a row
+// table; stages of neighbouring site problems usually share it
+uint64_t gemm_stage_key(std::vector<GemmDesc> const& probs);
 
 
 // block table of a list of S lines (lines without terms become zero tiles)
 // lines marked SLINE_SMALL go to the small-tile tables, the others into 16-line blocks
-SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const& terms);
+// panel > 0: work items ordered element range by element range over panels of up to that many
+// consecutive blocks of one tile shape (0: block by block)
+SBlockTable build_sblocks(HostTable<SLine> const& lines, HostTable<STerm> const& terms, int32_t panel = 0);
 // small-tile limit of the lines [l0, l1) of one site problem under the automatic rule: 64 when
 // their mean tile has at most 16 elements (then the 16-line blocks mostly hold tiles a few
 // elements wide, whose dense coefficient blocks cost more than the terms), else 0

@@ -49,6 +49,18 @@ class Context:
         """mq_ctx_set_option (abi.MQ_OPT_PLAIN_TABLES: plans made afterwards can run MQ_PREC_FP64_PLAIN)"""
         self.check(self.lib.mq_ctx_set_option(self.h, int(option), int(value)))
 
+    def tuned(self):
+        """the GEMM autotuner's table (MQ_OPT_AUTOTUNE): [(key, kind (0 = T-GEMMs, 1 = L*S), pick,
+        best ms, default ms)]"""
+        n = self.lib.mq_ctx_tuned(self.h, 0, None, None, None, None, None)
+        if n <= 0:
+            return []
+        keys = (C.c_uint64 * n)()
+        kinds, picks = (C.c_int32 * n)(), (C.c_int32 * n)()
+        best, dflt = (C.c_float * n)(), (C.c_float * n)()
+        self.lib.mq_ctx_tuned(self.h, n, keys, kinds, picks, best, dflt)
+        return [(int(keys[i]), int(kinds[i]), int(picks[i]), float(best[i]), float(dflt[i])) for i in range(n)]
+
     @property
     def stream(self):
         return self.lib.mq_ctx_stream(self.h)

@@ -32,6 +32,8 @@ def lib():
         idx = P(abi.IndexDesc)
         L.emu_hpsi.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, P(abi.BIndexDesc), dp, idx, idx, idx, dp, dp,
                                C.c_int, P(C.c_uint64), C.c_uint64, P(C.c_uint64), P(C.c_uint64), C.c_int]
+        L.emu_set_sblock_panel.argtypes = [C.c_int]
+        L.emu_set_gemm_policy.argtypes = [C.c_int]
         L.emu_term_grad.argtypes = [C.c_int, P(abi.MPODesc), P(abi.BIndexDesc), dp, P(abi.BIndexDesc), dp, idx, idx,
                                     idx, dp, dp, C.c_uint64]
         for f in (L.emu_lstep, L.emu_rstep):

@@ -19,8 +19,13 @@ namespace {
 // executes the stage as the device does: the tiles of the GEMM schedule, each over its K range,
 // writing C or a workspace partial, then the split-K reduce jobs.  The schedule is built with a
 // K chunk of 16 so that every reduction longer than one K slice goes through the split path.
+// GEMM schedule: < 0 every reduction longer than one K slice split (below), else the autotuner's
+// candidate of that index
+int g_policy = -1;
+
 void run_gemms(std::vector<GemmDesc> const& G, std::vector<GemmRow> const& rows, double* const* base) {
-    GemmSchedule sch = build_gemm_schedule(G, int64_t(1) << 40, 16);
+    GemmSchedule sch = g_policy < 0 ? build_gemm_schedule(G, int64_t(1) << 40, 16)
+                                    : build_gemm_schedule(G, gemm_policy(g_policy));
     std::vector<double> ws(sch.ws_elems + 1, std::nan(""));
     std::vector<char> seen(sch.ws_elems + 1, 0);
     for (int sh = 0; sh < GEMM_N_SHAPES; ++sh) {
@@ -83,11 +88,14 @@ void run_gemms(std::vector<GemmDesc> const& G, std::vector<GemmRow> const& rows,
 // (tiles of at most small_max elements: the small-tile work items, 2^lg lanes per line, lane e
 // summing element e over the line's terms)
 // Returns the number of small-tile lines.
+// panel of the term-sum work order (MQ_OPT_SBLOCK_PANEL; the product's automatic value by default)
+int32_t g_panel = 16;
+
 uint64_t run_lines(HostTable<SLine>& lines, HostTable<STerm> const& terms, const double* T, double* S,
                    uint64_t t_size, uint64_t s_size, int32_t small_max = 0) {
     mark_small_lines(lines, 0, lines.size(),
                      small_max == MQ_SMALL_TILE_AUTO ? auto_small_tile_max(lines, 0, lines.size()) : small_max);
-    SBlockTable B = build_sblocks(lines, terms);
+    SBlockTable B = build_sblocks(lines, terms, g_panel);
     check_sblocks(B, t_size, s_size);
     std::vector<char> sdone(B.small_lines.size(), 0);
     uint64_t small_covered = 0;
@@ -181,6 +189,9 @@ struct EmuBoundary {
 }  // namespace
 
 extern "C" {
+
+void emu_set_sblock_panel(int panel) { g_panel = panel; }
+void emu_set_gemm_policy(int policy) { g_policy = policy; }
 
 int emu_hpsi(int sym, const mq_mpo_desc* md, const mq_bindex_desc* Ld, const double* Ldata, const mq_bindex_desc* Rd,
              const double* Rdata, const mq_index_desc* li, const mq_index_desc* pi, const mq_index_desc* ri,

@@ -50,6 +50,23 @@ def test_chain_emulated(oracle_lib, sym, seed, kw):
             a = emu.hpsi(sym, ch.mpos[k], Lo[k], Ro[k + 1], ch.layouts[k], psi, small_max)
             assert rel_err(a, b) < 1e-12, (k, small_max)
             n_small[small_max] = n_small.get(small_max, 0) + emu.last_small_lines
+        # term-sum work items block by block, and element range by element range over panels of
+        # two and of very many blocks (the block table's checks require every range exactly once)
+        for panel in (0, 2, 1 << 20):
+            emu.lib().emu_set_sblock_panel(panel)
+            try:
+                a = emu.hpsi(sym, ch.mpos[k], Lo[k], Ro[k + 1], ch.layouts[k], psi, 0)
+            finally:
+                emu.lib().emu_set_sblock_panel(16)
+            assert rel_err(a, b) < 1e-12, (k, panel)
+        # every schedule candidate of the GEMM autotuner
+        for policy in range(7):
+            emu.lib().emu_set_gemm_policy(policy)
+            try:
+                a = emu.hpsi(sym, ch.mpos[k], Lo[k], Ro[k + 1], ch.layouts[k], psi, -1)
+            finally:
+                emu.lib().emu_set_gemm_policy(-1)
+            assert rel_err(a, b) < 1e-12, (k, "policy", policy)
         # output columns in chunks by right sector: one sector per chunk (budget 1 element), and
         # chunks of about a third of the intermediates, with the grouped and the reference-style
         # tables; chunks run on fresh NaN buffers and psi' starts as NaN (grouped), so a chunk
