@@ -933,7 +933,8 @@ static void autotune_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<std::vect
         for (size_t c = 0; c < p->chunks.size(); ++c)
             for (int kind = 0; kind < 2; ++kind) {
                 std::vector<GemmDesc> const& descs = kind == 0 ? tgs[c] : cgs[c];
-                std::vector<GemmRow> const& rows = kind == 0 ? trs[c] : std::vector<GemmRow>();
+                static const std::vector<GemmRow> no_rows;
+                std::vector<GemmRow> const& rows = kind == 0 ? trs[c] : no_rows;
                 GemmStage& stage = kind == 0 ? p->chunks[c].tg : p->chunks[c].cg;
                 if (descs.empty()) continue;
                 const auto key = std::make_pair(gemm_stage_key(descs), kind);
