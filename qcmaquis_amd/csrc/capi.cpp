@@ -1097,8 +1097,9 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
         p->plain.ok = true;
         p->seconds_plain = seconds_since(tp);
     }
-    if (!H.term_src.empty()) {
-        require(p->n_instances == 1, "MPO-term adjoint: single site problems only");
+    if (ctx->term_sources && p->n_instances == 1) {  // batch plans keep no term sources
+        require(H.term_src.size() == H.terms.size() && H.term_beta.size() == H.terms.size(),
+                "MPO-term adjoint: the plan's term sources do not cover its terms");
         auto& A = p->adj;
         A.yg.resize(nch);
         for (size_t c = 0; c < nch; ++c) {
