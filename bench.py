@@ -597,17 +597,17 @@ class ProfilerRegion:
         return False
 
 
-def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4):
-    """Three two-site DMRG runs of n_sweeps sweeps each from one common start (a seeded random MPS
-    after warm_sweeps FP64 sweeps), Davidson 8 iterations and tol 1e-8 per step: FP64 with the
-    grouped launches, mixed precision (FP32 contraction, FP64 refinement of every eigenvector) and
-    FP64 with the reference's launch structure (one library dgemm per T task and per cohort, one
-    gather per term).  The common start keeps the comparison on precision: from a random MPS, runs
-    whose Davidson results differ in the last digits can settle in different local minima of the
-    sweep.  Reported per run: the wall time and final energy of every sweep, and for the last
-    sweep of the mixed and the reference-style runs against the FP64 run: |dE| of the final
-    energy, the largest difference in truncated weight over the sweep's steps, and the final
-    Davidson residual."""
+def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4, extra_sweeps=4):
+    """Three two-site DMRG runs from one common start (a seeded random MPS after warm_sweeps FP64
+    sweeps), Davidson 8 iterations and tol 1e-8 per step: FP64 with the grouped launches and mixed
+    precision (FP32 contraction, FP64 refinement of every eigenvector), n_sweeps + extra_sweeps
+    sweeps each, and FP64 with the reference's launch structure (one library dgemm per T task and
+    per cohort, one gather per term), n_sweeps sweeps.  The common start keeps the comparison on
+    precision: from a random MPS, runs whose Davidson results differ in the last digits can settle
+    in different local minima of the sweep.  Reported per run: the wall time and final energy of
+    every sweep; against the FP64 run, at sweep n_sweeps (all three runs) and at the last sweep
+    (FP64 and mixed): |dE| of the final energy, the largest difference in truncated weight over
+    the sweep's steps, and the final Davidson residual; speed-ups at sweep n_sweeps."""
     from qcmaquis_amd import abi
     from qcmaquis_amd.dmrg import TwoSiteDMRG
     from qcmaquis_amd.qc import QCHamiltonian, make_integrals
@@ -621,16 +621,19 @@ def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4):
     del d
     log("sweeps: common start after %d FP64 sweeps (%.1f s), E %s" % (warm_sweeps, warm_s, warm_e))
     runs = {}
-    for name, prec in (("fp64", abi.MQ_PREC_FP64), ("mixed", abi.MQ_PREC_FP32), ("fp64_plain", abi.MQ_PREC_FP64_PLAIN)):
+    for name, prec, count in (("fp64", abi.MQ_PREC_FP64, n_sweeps + extra_sweeps),
+                              ("mixed", abi.MQ_PREC_FP32, n_sweeps + extra_sweeps),
+                              ("fp64_plain", abi.MQ_PREC_FP64_PLAIN, n_sweeps)):
         ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, int(prec == abi.MQ_PREC_FP64_PLAIN))
         d = TwoSiteDMRG(ham, L // 2, L // 2, M=M, ctx=ctx, tol=1e-8, max_iter=8, precision=prec, mps=start)
-        rec = {"sweep_seconds": [], "sweep_energies": []}
-        for s in range(n_sweeps):
+        rec = {"sweep_seconds": [], "sweep_energies": [], "truncated_weights": [], "residuals": []}
+        for s in range(count):
             n0 = len(d.trunc)
             e, dt = d.sweep()
             rec["sweep_seconds"].append(round(dt, 3))
             rec["sweep_energies"].append(e)
-        rec["last_sweep_truncated_weights"] = d.trunc[n0:]
+            rec["truncated_weights"].append(d.trunc[n0:])
+            rec["residuals"].append(d.last_residual)
         rec["final_residual_norm"] = d.last_residual
         rec["timing_s"] = {k: round(v, 3) for k, v in d.timing.items()}
         runs[name] = rec
@@ -638,28 +641,37 @@ def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4):
         del d
     ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
     ref = runs["fp64"]
+
+    def versus(r, k):
+        return {"abs_delta_e": abs(r["sweep_energies"][k] - ref["sweep_energies"][k]),
+                "max_abs_delta_truncated_weight": float(np.max(np.abs(np.array(r["truncated_weights"][k]) -
+                                                                      np.array(ref["truncated_weights"][k])))),
+                "final_residual_norm": r["residuals"][k]}
+    k4 = n_sweeps - 1
     cmp = {}
     for name in ("mixed", "fp64_plain"):
         r = runs[name]
-        cmp[name] = {
-            "abs_delta_e_final": abs(r["sweep_energies"][-1] - ref["sweep_energies"][-1]),
-            "max_abs_delta_truncated_weight_last_sweep": float(np.max(np.abs(
-                np.array(r["last_sweep_truncated_weights"]) - np.array(ref["last_sweep_truncated_weights"])))),
-            "final_residual_norm": r["final_residual_norm"],
-            "last_sweep_seconds": r["sweep_seconds"][-1],
-            "last_sweep_speedup_vs_fp64_plain": round(runs["fp64_plain"]["sweep_seconds"][-1] / r["sweep_seconds"][-1], 3)}
-    out = {"runs": {k: {kk: vv for kk, vv in v.items() if kk != "last_sweep_truncated_weights"} for k, v in runs.items()},
+        cmp[name] = {f"at_sweep_{n_sweeps}": versus(r, k4),
+                     "sweep_seconds": r["sweep_seconds"][k4],
+                     "speedup_vs_fp64_plain": round(runs["fp64_plain"]["sweep_seconds"][k4] / r["sweep_seconds"][k4], 3)}
+    last = n_sweeps + extra_sweeps - 1
+    cmp["mixed"][f"at_sweep_{last + 1}"] = versus(runs["mixed"], last)
+    fp64_change = abs(ref["sweep_energies"][last] - ref["sweep_energies"][last - 1])
+    out = {"runs": {k: {kk: vv for kk, vv in v.items() if kk not in ("truncated_weights", "residuals")}
+                    for k, v in runs.items()},
            "vs_fp64": cmp, "fp64_final_residual_norm": ref["final_residual_norm"],
-           "fp64_last_sweep_speedup_vs_fp64_plain": round(runs["fp64_plain"]["sweep_seconds"][-1] /
-                                                          ref["sweep_seconds"][-1], 3),
+           "fp64_energy_change_in_last_sweep": fp64_change,
+           "fp64_speedup_vs_fp64_plain": round(runs["fp64_plain"]["sweep_seconds"][k4] / ref["sweep_seconds"][k4], 3),
            "common_start": {"fp64_sweeps": warm_sweeps, "energies": warm_e, "seconds": round(warm_s, 3)},
-           "note": f"{L} orbitals, M={M}, {n_sweeps} sweeps per run from one common start (seeded random MPS after "
-                   f"{warm_sweeps} FP64 sweeps), Davidson 8 iterations / tol 1e-8 per two-site step; every run is "
-                   "swept for real (its own energies, truncations and boundaries)"}
-    log("sweep comparison: last sweep %.2f s (fp64) / %.2f s (mixed) / %.2f s (fp64 plain); mixed |dE| %.2e, "
-        "|d tw| %.2e" % (ref["sweep_seconds"][-1], runs["mixed"]["sweep_seconds"][-1],
-                         runs["fp64_plain"]["sweep_seconds"][-1], cmp["mixed"]["abs_delta_e_final"],
-                         cmp["mixed"]["max_abs_delta_truncated_weight_last_sweep"]))
+           "note": f"{L} orbitals, M={M}; from one common start (seeded random MPS after {warm_sweeps} FP64 sweeps) "
+                   f"{n_sweeps + extra_sweeps} sweeps of FP64 and mixed and {n_sweeps} of the reference-style FP64, "
+                   "Davidson 8 iterations / tol 1e-8 per two-site step; every run is swept for real (its own "
+                   "energies, truncations and boundaries)"}
+    log("sweep comparison: sweep %d %.2f s (fp64) / %.2f s (mixed) / %.2f s (fp64 plain); mixed |dE| %.2e after %d "
+        "sweeps, %.2e after %d (FP64 moved %.2e in its last sweep)"
+        % (n_sweeps, ref["sweep_seconds"][k4], runs["mixed"]["sweep_seconds"][k4], runs["fp64_plain"]["sweep_seconds"][k4],
+           cmp["mixed"][f"at_sweep_{n_sweeps}"]["abs_delta_e"], n_sweeps,
+           cmp["mixed"][f"at_sweep_{last + 1}"]["abs_delta_e"], last + 1, fp64_change))
     return out
 
 

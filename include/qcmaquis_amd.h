@@ -82,6 +82,11 @@ extern "C" {
                                   key of the stage's (m, n, k) histogram; later stages with a known
                                   key take the recorded pick without timing (mq_ctx_tuned lists
                                   them).  Default 0: the default rule. */
+#define MQ_OPT_JCD_GMRES 8     /* value 0..64: GMRES steps of mq_davidson's correction equation (the
+                                  reference's ietl_jcd_gmres; jcd_gmres_solver, ietl/jacobi.h:126-164,
+                                  ietl_gmres, ietl/gmres.h:89-145); 0 (default): the plain correction
+                                  t = -r + (r.u / u.u) u.  mq_davidson_noalloc refuses values > 0; the
+                                  batch Davidson uses the plain correction. */
 #define MQ_OPT_TERM_SOURCES 4  /* value 1: H*psi plans made afterwards (single site problems) also keep,
                                   per S term, the MPO term it comes from, for
                                   mq_hpsi_term_gradient (default 0) */

@@ -266,13 +266,14 @@ private:
 // Jacobi-Davidson of the site problem H from `initial`, orthogonal to `ortho`; the eigenvector is
 // written block by block into the caller's `out` pointers (one per block of `initial`, as the
 // reference writes into ret.data(), solver.cpp:22-25); returns the eigenvalue.  gmres: the
-// reference's GMRES correction steps; this path takes the gmres = 0 correction (its default).
+// reference's GMRES steps of the correction equation (ietl_jcd_gmres, truncated to an integer as
+// the reference's size_t parameter; 0 = the plain correction), set on the context for this call.
 template <class T>
 T solve(std::vector<T*>& out, DavidsonVector<T>& initial, SuperHamil const& H,
         std::vector<DavidsonVector<T>> const& ortho, double gmres, double tol, int maxiter,
         int precision = MQ_PREC_FP64, mq_davidson_stats_t* stats = nullptr) {
     static_assert(sizeof(T) == sizeof(double), "solve: the ABI value type is FP64");
-    if (gmres != 0.0) throw Error(MQ_ERR_UNSUPPORTED, "solve: GMRES correction steps are not available");
+    if (!(gmres >= 0.0 && gmres <= 64.0)) throw Error(MQ_ERR_INVALID, "solve: gmres steps must be 0..64");
     if (out.size() != initial.n_blocks()) throw Error(MQ_ERR_INVALID, "solve: one output pointer per block");
     if (initial.data().size() != H.size()) throw Error(MQ_ERR_INVALID, "solve: vector length != site problem");
     std::vector<const double*> op;
@@ -280,8 +281,11 @@ T solve(std::vector<T*>& out, DavidsonVector<T>& initial, SuperHamil const& H,
     std::vector<double> psi(initial.data().size());
     double e = 0;
     mq_davidson_stats_t st{};
-    H.context().check(mq_davidson(H.get(), initial.data().data(), int(op.size()), op.empty() ? nullptr : op.data(),
-                                  tol, maxiter, precision, &e, psi.data(), &st));
+    H.context().check(mq_ctx_set_option(H.context().get(), MQ_OPT_JCD_GMRES, int(gmres)));
+    const int rc = mq_davidson(H.get(), initial.data().data(), int(op.size()), op.empty() ? nullptr : op.data(), tol,
+                               maxiter, precision, &e, psi.data(), &st);
+    (void)mq_ctx_set_option(H.context().get(), MQ_OPT_JCD_GMRES, 0);
+    H.context().check(rc);
     for (size_t b = 0; b < out.size(); ++b)
         std::copy(psi.begin() + (initial[b] - initial[0]), psi.begin() + (initial[b] - initial[0]) + initial.block_size(b),
                   out[b]);

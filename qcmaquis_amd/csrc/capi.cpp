@@ -41,6 +41,7 @@ struct mq_ctx_s {
     int32_t sblock_panel = MQ_SBLOCK_PANEL_AUTO;  // MQ_OPT_SBLOCK_PANEL
     bool sblock_xcd = false;                      // MQ_OPT_SBLOCK_XCD
     bool autotune = false;                        // MQ_OPT_AUTOTUNE
+    int32_t jcd_gmres = 0;                        // MQ_OPT_JCD_GMRES
     struct Tuned {
         int32_t pick;
         float best_ms, default_ms;
@@ -596,6 +597,9 @@ int mq_ctx_set_option(mq_ctx_t ctx, int option, int value) {
             ctx->sblock_xcd = value != 0;
         } else if (option == MQ_OPT_AUTOTUNE) {
             ctx->autotune = value != 0;
+        } else if (option == MQ_OPT_JCD_GMRES) {
+            require(value >= 0 && value <= 64, "mq_ctx_set_option: MQ_OPT_JCD_GMRES is 0..64");
+            ctx->jcd_gmres = value;
         } else {
             throw Error(MQ_ERR_INVALID, "mq_ctx_set_option: unknown option");
         }
@@ -1792,11 +1796,19 @@ static void davidson_core(mq_hpsi_plan_t p, const double* psi0, int n_ortho, con
         double* sc = p->ctx->dwork + DOT_BLOCKS;  // device scalars
         double* work = p->ctx->dwork;
         double hpsi_s = 0;
+        // GMRES steps of the correction equation (ietl_jcd_gmres; 0 = the plain correction)
+        const int gm = p->ctx->jcd_gmres;
+        require(gm == 0 || !arena, "mq_davidson_noalloc: the GMRES correction (MQ_OPT_JCD_GMRES) needs allocations");
         try {
             std::vector<double*> V(max_iter), VA(max_iter), O(no);
             for (int i = 0; i < max_iter; ++i) { V[i] = alloc(); VA[i] = alloc(); }
             double* u = alloc();
             double* r = alloc();
+            // GMRES basis (gm + 1 vectors), its work vector and the projected operand
+            std::vector<double*> G(gm > 0 ? gm + 1 : 0);
+            for (auto& g : G) g = alloc();
+            double* gw = gm > 0 ? alloc() : nullptr;
+            double* gt = gm > 0 ? alloc() : nullptr;
             for (int k = 0; k < no; ++k) {
                 O[k] = alloc();
                 hipcheck(hipMemcpyAsync(O[k], ortho[k], n * sizeof(double), hipMemcpyHostToDevice, s), "upload");
@@ -1838,6 +1850,72 @@ static void davidson_core(mq_hpsi_plan_t p, const double* psi0, int n_ortho, con
             std::vector<double> M(size_t(max_iter) * max_iter, 0.0);
             double theta = 0, rnorm = 0;
             int it = 0;
+            // y = (1 - uu*)(H - theta)(1 - uu*) x  (jcd_solver_operator, jacobi.h:263-281)
+            auto jcd_op = [&](const double* x, double* y, int prec, double theta_) {
+                hipcheck(launch_axpby(gt, x, n, 0.0, 1.0, s), "axpby");
+                hipcheck(launch_dot(u, x, n, work, sc + 3, s), "dot");
+                hipcheck(launch_axpy_dev(gt, u, n, sc + 3, nullptr, -1.0, s), "axpy");
+                auto h0 = std::chrono::steady_clock::now();
+                hpsi_run(p, gt, y, prec);
+                hipcheck(hipStreamSynchronize(s), "hpsi");
+                hpsi_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - h0).count();
+                hipcheck(launch_axpby(y, gt, n, 1.0, -theta_, s), "axpby");
+                hipcheck(launch_dot(u, y, n, work, sc + 4, s), "dot");
+                hipcheck(launch_axpy_dev(y, u, n, sc + 4, nullptr, -1.0, s), "axpy");
+            };
+            // x <- GMRES(op, b = -r, x0 = x) with at most gm - 1 Arnoldi steps and absolute
+            // tolerance abs_tol on the rotated residual (ietl_gmres, gmres.h:89-145, restated with its
+            // rule that the step which meets the tolerance is left out of the update)
+            auto gmres = [&](double* x, int prec, double theta_, double abs_tol) {
+                const int m = gm;
+                std::vector<double> sv(size_t(m) + 1, 0.0), cs(size_t(m) + 1, 0.0), sn(size_t(m) + 1, 0.0);
+                std::vector<double> H(size_t(m + 1) * size_t(m + 1), 0.0);
+                auto Hat = [&](int i, int j) -> double& { return H[size_t(i) * size_t(m + 1) + size_t(j)]; };
+                jcd_op(x, G[0], prec, theta_);
+                hipcheck(launch_axpby(G[0], r, n, -1.0, -1.0, s), "axpby");  // -r - op(x0)
+                sv[0] = std::sqrt(dot_host(G[0], G[0]));
+                if (std::fabs(sv[0]) < abs_tol) return;
+                hipcheck(launch_axpby(G[0], G[0], n, 0.0, 1.0 / sv[0], s), "axpby");
+                int i = 0;
+                auto rot = [](double& dx, double& dy, double c, double sn_) {
+                    const double r0 = c * dx + sn_ * dy, r1 = -sn_ * dx + c * dy;
+                    dx = r0;
+                    dy = r1;
+                };
+                for (; i < m - 1; ++i) {
+                    jcd_op(G[i], gw, prec, theta_);
+                    for (int k = 0; k <= i; ++k) {
+                        Hat(k, i) = dot_host(gw, G[k]);
+                        hipcheck(launch_axpby(gw, G[k], n, 1.0, -Hat(k, i), s), "axpby");
+                    }
+                    Hat(i + 1, i) = std::sqrt(dot_host(gw, gw));
+                    hipcheck(launch_axpby(G[i + 1], gw, n, 0.0, 1.0 / Hat(i + 1, i), s), "axpby");
+                    for (int k = 0; k < i; ++k) rot(Hat(k, i), Hat(k + 1, i), cs[size_t(k)], sn[size_t(k)]);
+                    const double dx = Hat(i, i), dy = Hat(i + 1, i);
+                    if (dy == 0.0) {
+                        cs[size_t(i)] = 1.0;
+                        sn[size_t(i)] = 0.0;
+                    } else if (std::fabs(dy) > std::fabs(dx)) {
+                        const double q = dx / dy;
+                        sn[size_t(i)] = 1.0 / std::sqrt(1.0 + q * q);
+                        cs[size_t(i)] = q * sn[size_t(i)];
+                    } else {
+                        const double q = dy / dx;
+                        cs[size_t(i)] = 1.0 / std::sqrt(1.0 + q * q);
+                        sn[size_t(i)] = q * cs[size_t(i)];
+                    }
+                    rot(Hat(i, i), Hat(i + 1, i), cs[size_t(i)], sn[size_t(i)]);
+                    rot(sv[size_t(i)], sv[size_t(i) + 1], cs[size_t(i)], sn[size_t(i)]);
+                    if (std::fabs(sv[size_t(i) + 1]) < abs_tol) break;
+                }
+                // back substitution over the first i columns, then x += sum_k y_k G[k]
+                std::vector<double> y(sv.begin(), sv.begin() + i);
+                for (int a = i - 1; a >= 0; --a) {
+                    y[size_t(a)] /= Hat(a, a);
+                    for (int b = a - 1; b >= 0; --b) y[size_t(b)] -= Hat(b, a) * y[size_t(a)];
+                }
+                for (int k = 0; k < i; ++k) hipcheck(launch_axpby(x, G[k], n, 1.0, y[size_t(k)], s), "axpby");
+            };
             // one Jacobi-Davidson run from V[0] with H*psi in precision `prec`; leaves the Ritz
             // vector in u
             auto jd = [&](int prec, double tol_run, int iters) {
@@ -1888,6 +1966,9 @@ static void davidson_core(mq_hpsi_plan_t p, const double* psi0, int n_ortho, con
                 const double dru = du[0], duu = du[1];
                 hipcheck(launch_axpby(V[it], r, n, 0.0, -1.0, s), "axpby");
                 hipcheck(launch_axpby(V[it], u, n, 1.0, dru / duu, s), "axpby");
+                // with GMRES steps, that is the start of an approximate solution of the correction
+                // equation to the tolerance 2^-(it+1) (jcd_gmres_solver, jacobi.h:126-164)
+                if (gm > 0) gmres(V[it], prec, theta, 1.0 / std::pow(2.0, double(it + 1)));
             }
             };
             int refine = 0;

@@ -329,14 +329,16 @@ class TwoSiteDMRG:
 
     def __init__(self, ham, n_up, n_dn, M, ctx=None, init_m=8, seed=42, tol=1e-8, max_iter=8, cutoff=1e-16,
                  verbose=False, precision=abi.MQ_PREC_FP64, mps=None, resultfile=None, first_sweep=0,
-                 device_split=False, snapshots=(), snapshot_dir="."):
+                 device_split=False, snapshots=(), snapshot_dir=".", gmres=0):
         """precision: arithmetic of the Davidson H*psi (MQ_PREC_FP32 = FP32 contraction with an FP64
         refinement of each eigenvector, see mq_davidson); mps: (layouts, data) to start from (e.g.
         results.load_checkpoint) instead of a random state; resultfile: HDF5 result file that every
         sweep appends to (results.ResultArchive); first_sweep: number of the first sweep;
         device_split: the block SVDs of the two-site split on the device (mq_block_svd); snapshots:
         (2 sweep + direction, site) pairs whose site problem is saved before its solve, as
-        snapshot_dir/snapshot_<twosweep>_<site>.h5 (snapshot.save; ts_optimize.hpp:185-213)"""
+        snapshot_dir/snapshot_<twosweep>_<site>.h5 (snapshot.save; ts_optimize.hpp:185-213); gmres:
+        GMRES steps of the Davidson correction equation (ietl_jcd_gmres, default 0)"""
+        self.gmres = int(gmres)
         self.device_split = device_split
         self.snapshots = {(int(a), int(b)) for a, b in snapshots}
         self.snapshot_dir = snapshot_dir
@@ -407,7 +409,8 @@ class TwoSiteDMRG:
                           self.left[i], self.right[i + 2], lay2, psi2, i, twosweep, self.ham.ecore)
         plan = HpsiPlan(self.ctx, mpo, self.left[i], self.right[i + 2], lay2)
         t2 = time.perf_counter()
-        e, psi, st = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter, precision=self.precision)
+        e, psi, st = plan.davidson(psi2, tol=self.tol, max_iter=self.max_iter, precision=self.precision,
+                                   gmres=self.gmres or None)
         t3 = time.perf_counter()
         pst = plan.stats()
         self.last_residual = st["residual_norm"]

@@ -249,8 +249,16 @@ class HpsiPlan(_Handle):
         """Stream-ordered H*psi on device pointers (context stream)."""
         self.ctx.check(self.ctx.lib.mq_hpsi(self.h, C.c_void_p(in_ptr), C.c_void_p(out_ptr), int(precision)))
 
-    def davidson(self, psi0, ortho=(), tol=1e-8, max_iter=8, precision=abi.MQ_PREC_FP64):
-        """Jacobi-Davidson lowest eigenpair (solve<double>, solver/solver.cpp:13-28)."""
+    def davidson(self, psi0, ortho=(), tol=1e-8, max_iter=8, precision=abi.MQ_PREC_FP64, gmres=None):
+        """Jacobi-Davidson lowest eigenpair (solve<double>, solver/solver.cpp:13-28); gmres: GMRES
+        steps of the correction equation for this call (ietl_jcd_gmres; None: the context's
+        MQ_OPT_JCD_GMRES)"""
+        if gmres is not None:
+            self.ctx.set_option(abi.MQ_OPT_JCD_GMRES, int(gmres))
+            try:
+                return self.davidson(psi0, ortho, tol, max_iter, precision)
+            finally:
+                self.ctx.set_option(abi.MQ_OPT_JCD_GMRES, 0)
         psi0 = np.ascontiguousarray(psi0, np.float64)
         out = np.zeros_like(psi0)
         ev = C.c_double()

@@ -55,6 +55,45 @@ def test_davidson_vs_dense(ctx2u1, oracle_lib):
     assert st2["refine_iterations"] >= 1
 
 
+def test_davidson_gmres_matches_restatement(ctx2u1, oracle_lib):
+    """mq_davidson with and without the GMRES correction (MQ_OPT_JCD_GMRES) against the oracle's
+    restatement of the reference's Jacobi-Davidson on the dense H_eff: the same Ritz value after
+    every budget, the same iteration counts, the lowest eigenvalue at convergence"""
+    from oracle.jcd import jcd
+    print("davidson with GMRES correction vs the restatement", flush=True)
+    h, eri, _ = make_integrals(6, 4, rank=6)
+    ham = QCHamiltonian(h, eri)
+    d = TwoSiteDMRG(ham, 3, 3, M=12, ctx=ctx2u1, init_m=6, seed=2)
+    i = 2
+    for k in range(i):  # the left boundaries up to the site problem
+        d.step(k, True)
+    lay2, psi2 = merge_two(d.lay[i], d.data[i], d.lay[i + 1], d.data[i + 1], d.phys2, d.off12)
+    plan = HpsiPlan(ctx2u1, d.ts_mpo(i), d.left[i], d.right[i + 2], lay2)
+    n = lay2.n
+    H = np.zeros((n, n))
+    for c in range(n):
+        e = np.zeros(n)
+        e[c] = 1.0
+        H[:, c] = plan.apply(e)
+    H = 0.5 * (H + H.T)
+    w = np.linalg.eigvalsh(H)
+    try:
+        for steps in (0, 3, 6):
+            ctx2u1.set_option(abi.MQ_OPT_JCD_GMRES, steps)
+            for budget, tol in ((4, 1e-12), (40, 1e-10)):
+                ev, vec, st = plan.davidson(psi2, tol=tol, max_iter=min(n, budget))
+                th, u, it, _ = jcd(H, psi2, tol=tol, max_iter=min(n, budget), gmres_steps=steps)
+                print(f"  gmres {steps} budget {budget}: device {ev:.12f} ({st['iterations']} it), restated "
+                      f"{th:.12f} ({it} it), lowest {w[0]:.12f}", flush=True)
+                assert st["iterations"] == it
+                assert abs(ev - th) < 1e-9 * max(1.0, abs(th))
+                if budget == 40:
+                    assert abs(ev - w[0]) < 1e-8 * max(1.0, abs(w[0]))
+    finally:
+        ctx2u1.set_option(abi.MQ_OPT_JCD_GMRES, 0)
+    plan.close()
+
+
 @pytest.mark.parametrize("M,tol", [(400, 1e-9), (16, None)])
 def test_sweeps_small6(ctx2u1, M, tol):
     g = GOLD["small6"]
