@@ -360,6 +360,19 @@ int mq_davidson_batch(mq_hpsi_plan_t plan, const double* psi0, double tol, int m
 int mq_block_svd(mq_ctx_t ctx, int n, const int32_t* rows, const int32_t* cols, const double* a, double* s,
                  double* u, double* vt);
 
+/* Noise term of the density-matrix split (reference: alpha_dm_direct / alpha_dm_direct_right,
+ * move_boundary.hpp:38-145, with Cohort::lbtm / rbtm, tasks.cpp:346-380).  Grow the boundary with
+ * an identity bra and symmetric = 0 (no block then aliases its partner's): its blocks are the S
+ * tiles of that growth step (a right step completes partner cohorts in any case: give it the MPO
+ * without its Hermitian-partner tables).  side 0 (a left boundary): out_k = alpha * sum over the cohorts whose
+ * row (bra) charge is charges[k] of X X^T, X = the cohort's blocks side by side; side 1 (a right
+ * boundary, whose columns are the bra's): out_k = alpha * sum over the blocks X_b of the cohorts
+ * whose column charge is charges[k] of X_b^T X_b.  The n charges (n x 2) and sizes must match
+ * entries of the grouped index; out (host) receives the n blocks sizes[k] x sizes[k],
+ * column-major, concatenated.  rocBLAS dgemm on the context stream; synchronous. */
+int mq_boundary_gram(mq_boundary_t b, int side, double alpha, int n, const int32_t* charges, const uint32_t* sizes,
+                     double* out);
+
 /* ---- quantum-chemistry Hamiltonian MPO (host only; input producer of the path) -------------
  * reference: models/chem/2u1/model.hpp (create_terms), generate_mpo/tagged_mpo_maker_optim.hpp,
  * mp_tensors/ts_ops.hpp:96-185 (two-site MPO).  h1: L x L row-major h_pq; eri: L^4 row-major

@@ -183,6 +183,7 @@ SIGNATURES = [
     ("mq_davidson_noalloc", i32, [vp, dp, i32, P(dp), C.c_double, i32, i32, dp, dp, P(DavidsonStats)]),
     ("mq_davidson_batch", i32, [vp, dp, C.c_double, i32, i32, dp, dp, P(C.c_int32), P(DavidsonStats)]),
     ("mq_block_svd", i32, [vp, i32, P(C.c_int32), P(C.c_int32), dp, dp, dp, dp]),
+    ("mq_boundary_gram", i32, [vp, i32, C.c_double, i32, P(C.c_int32), P(C.c_uint32), dp]),
     ("mq_qc_build", i32, [i32, i32, dp, dp, C.c_double, C.c_double, P(vp)]),
     ("mq_qc_build_ex", i32, [i32, i32, dp, dp, C.c_double, C.c_double, i32, P(vp)]),
     ("mq_qc_integral_gradient", i32, [vp, P(dp), dp, dp]),

@@ -1558,6 +1558,78 @@ int mq_hpsi_timed(mq_hpsi_plan_t p, const double* in, double* out, int precision
     });
 }
 
+// Noise term of the density-matrix split: the Gram matrices of a boundary's blocks, grouped by the
+// charge of their rows (side 0: X X^T per cohort, its blocks side by side) or of their columns
+// (side 1: X_b^T X_b per block), accumulated in one device buffer by rocBLAS dgemm (beta = 1,
+// one stream, so the accumulations are ordered).
+int mq_boundary_gram(mq_boundary_t b, int side, double alpha, int n, const int32_t* charges, const uint32_t* sizes,
+                     double* out) {
+    if (!b) return MQ_ERR_INVALID;
+    mq_ctx_t ctx = b->ctx;
+    return guarded(ctx, [&] {
+        require((side == 0 || side == 1) && n >= 0 && (n == 0 || (charges && sizes && out)),
+                "mq_boundary_gram: bad argument");
+        BoundaryIndex const& B = b->idx;
+        Index const& grouped = side == 0 ? B.bra : B.ket;
+        std::vector<uint64_t> off(size_t(n) + 1, 0);
+        for (int k = 0; k < n; ++k) off[size_t(k) + 1] = off[size_t(k)] + uint64_t(sizes[k]) * sizes[k];
+        std::vector<int> slot(grouped.size(), -1);  // output block of every sector of the grouped index
+        for (int k = 0; k < n; ++k) {
+            const size_t pos = grouped.position(Charge(charges[2 * k], charges[2 * k + 1]));
+            require(pos < grouped.size() && grouped.sizes[pos] == sizes[k] && slot[pos] < 0,
+                    "mq_boundary_gram: charge or size not in the boundary's index");
+            slot[pos] = k;
+        }
+        for (uint32_t ci = 0; ci < B.n_cohorts(); ++ci)
+            for (uint32_t x = 0; x < B.aux_dim; ++x)
+                require(B.offsets[ci][x] < 0 || (!B.tr[ci] && B.transposes[ci][x] == 0 && B.conj_scales[ci][x] == 1.0),
+                        "mq_boundary_gram: a block aliases its partner's (grow with symmetric = 0 and no partners)");
+        double* d = dev_alloc<double>(std::max<uint64_t>(off.back(), 1));
+        try {
+            hipStream_t s = ctx->stream;
+            hipcheck(launch_fill(d, off.back(), 0.0, s), "clear");
+            blas_handle(ctx);
+            const double one = 1.0;
+            for (uint32_t ci = 0; ci < B.n_cohorts(); ++ci) {
+                const int k = slot[side == 0 ? B.clr[ci].first : B.clr[ci].second];
+                const int ls = int(B.lsz[ci]), rs = int(B.rsz[ci]);
+                if (k < 0 || B.tr[ci] || B.nblk[ci] == 0 || ls == 0 || rs == 0) continue;
+                const double* X = b->data + B.data_off[ci];
+                double* o = d + off[size_t(k)];
+                rocblas_status st = rocblas_status_success;
+                // blocks stored back to back (offsets 0, ls*rs, ...): one GEMM over the cohort
+                bool packed = side == 0;
+                for (uint32_t x = 0, next = 0; x < B.aux_dim && packed; ++x)
+                    if (B.offsets[ci][x] >= 0) packed = B.offsets[ci][x] == int64_t(next++) * ls * rs;
+                if (packed) {
+                    const int64_t K = int64_t(B.nblk[ci]) * rs;
+                    require(K < INT32_MAX, "mq_boundary_gram: cohort too wide");
+                    st = rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose, ls, ls, int(K),
+                                       &alpha, X, ls, X, ls, &one, o, ls);
+                } else {
+                    for (uint32_t x = 0; x < B.aux_dim && st == rocblas_status_success; ++x) {
+                        if (B.offsets[ci][x] < 0) continue;
+                        const double* Xb = X + B.offsets[ci][x];
+                        st = side == 0 ? rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_transpose,
+                                                       ls, ls, rs, &alpha, Xb, ls, Xb, ls, &one, o, ls)
+                                       : rocblas_dgemm(ctx->blas, rocblas_operation_transpose, rocblas_operation_none,
+                                                       rs, rs, ls, &alpha, Xb, ls, Xb, ls, &one, o, rs);
+                    }
+                }
+                require(st == rocblas_status_success, "rocblas dgemm failed");
+            }
+            if (off.back())
+                hipcheck(hipMemcpyAsync(out, d, off.back() * sizeof(double), hipMemcpyDeviceToHost, s), "download");
+            hipcheck(hipStreamSynchronize(s), "gram");
+        } catch (...) {
+            (void)hipStreamSynchronize(ctx->stream);
+            dev_free(d);
+            throw;
+        }
+        dev_free(d);
+    });
+}
+
 // Thin SVD of every block on the device, one rocSOLVER dgesvd per block on the context stream
 // (U and V^T of the min(rows, cols) singular vectors); inputs and outputs are host arrays.
 int mq_block_svd(mq_ctx_t ctx, int n, const int32_t* rows, const int32_t* cols, const double* a, double* s,

@@ -10,13 +10,14 @@ Per two-site step (reference ts_optimize::sweep, ts_optimize.hpp:114-362):
 The SVD split runs on the host (numpy/LAPACK) in this version; boundaries and the contraction live
 in HBM.  MPS site tensors are kept as right-paired vectors with their MPSLayout.
 """
+import ctypes as C
 import os
 import time
 
 import numpy as np
 
 from . import abi
-from .engine import Context, DeviceBoundary, DeviceMPO, DeviceMPS, HpsiPlan, block_svd, lstep, rstep
+from .engine import Context, DeviceBoundary, DeviceMPO, DeviceMPS, HpsiPlan, block_svd, boundary_gram, lstep, rstep
 from .model import Index, MPSLayout, qadd, qsub
 
 
@@ -192,6 +193,137 @@ def split_two(lay2, psi2, phys, off12, Mmax, cutoff, to_right, details=None, sym
     return la, la.to_right(a_lp), lb, b_rp, tw
 
 
+def heev_truncation(evals, Mmax, cutoff):
+    """keeps per block and the truncation record of heev_truncate (block_matrix_algorithms.h:
+    298-354): the positive eigenvalues of the density matrix are replaced by their square roots
+    (the others are kept as they are), estimate_truncation runs on those, and the record's
+    truncated weight is the truncated fraction (sum of the discarded roots / sum of all)"""
+    roots = {m: np.where(w > 0.0, np.sqrt(np.maximum(w, 0.0)), w) for m, w in evals.items()}
+    keeps, _ = estimate_truncation(roots, Mmax, cutoff)
+    rec = truncation_details(roots, Mmax, cutoff)
+    rec["TruncatedWeight"] = rec["TruncatedFraction"]
+    return keeps, rec
+
+
+class _Unpaired:
+    """an MPO tensor with its Hermitian-partner tables removed: growth steps then compute every
+    MPO index and alias no block to its partner's transpose (the noise term needs all S tiles of
+    a bra that differs from the ket; alpha_dm_direct_right schedules with skip = false)"""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.row_dim, self.col_dim = spec.row_dim, spec.col_dim
+
+    def desc(self):
+        d = self.spec.desc()
+        out = type(d).from_buffer_copy(d)
+        out.left_herm = out.right_herm = C.POINTER(C.c_uint32)()
+        out.left_phase = out.right_phase = C.POINTER(C.c_int32)()
+        out._keep = d
+        return out
+
+
+def predict_split(ctx, lay2, psi2, phys, off12, Mmax, cutoff, alpha, to_right, boundary, mpo, details=None,
+                  sym=abi.MQ_SYM_TWOU1):
+    """density-matrix split of a two-site tensor with the noise term (Engine::predict_split_l2r /
+    predict_split_r2l, boundary_ops/prediction.hpp:164-235; twosite_truncation = heev).
+
+    l2r (boundary = left boundary of site i, mpo = the one-site MPO of site i): X_m = psi2 with rows
+    (s1, l) and columns (s2, r) per middle charge m; the density matrix is X_m X_m^T + alpha
+    sum_b S_b S_b^T, S_b the S tiles of the left growth of the tensor with left index l, physical
+    index s1 and the fused right index (s2, r) (alpha_dm, prediction.hpp:40-53 with
+    alpha_dm_direct).  Growing that tensor with an identity bra gives S_b as the new boundary's
+    blocks; mq_boundary_gram sums their Gram matrices on the device.  heev_truncate keeps its
+    leading eigenvectors U: site i = U (left-normalised), site i+1 = U^T X.
+    r2l (boundary = right boundary of site i+2, mpo = the one-site MPO of site i+1): the density
+    matrix is X_m^T X_m + alpha sum_b S_b^T S_b from the right growth of the tensor with the fused
+    left index (s1, l); site i+1 = U^T (right-normalised), site i = X U.
+    alpha = 0 is the SVD split (the reference returns split_mps_l2r / split_mps_r2l).
+    Returns (layout i, right-paired data i, layout i+1, data i+1, truncated weight)."""
+    if alpha == 0.0:
+        return split_two(lay2, psi2, phys, off12, Mmax, cutoff, to_right, details, sym)
+    mats = split_matrices(lay2, psi2, phys, off12, sym)
+    keys = list(mats)
+    rows_n = {m: mats[m][0].shape[0] for m in keys}
+    cols_n = {m: mats[m][0].shape[1] for m in keys}
+    left, right = lay2.left, lay2.right
+    if to_right:
+        fused = Index(keys, [cols_n[m] for m in keys])  # (s2, r) fused
+        ident = Index(keys, [rows_n[m] for m in keys])
+        lk, lb_ = MPSLayout(left, phys, fused), MPSLayout(left, phys, ident)
+        ket_lp, bra_lp = np.zeros(lk.n), np.zeros(lb_.n)
+        for m in keys:
+            X, rows, _ = mats[m]
+            kb, bb = lk.right.position(m), lb_.right.position(m)
+            kblk = ket_lp[lk.lp_off[kb]:lk.lp_off[kb + 1]].reshape(cols_n[m], lk.lp_rows[kb]).T
+            bblk = bra_lp[lb_.lp_off[bb]:lb_.lp_off[bb + 1]].reshape(rows_n[m], lb_.lp_rows[bb]).T
+            for s1, lbi, r0 in rows:
+                ls = left.sizes[lbi]
+                o = lk.lpb[(s1, left.charges[lbi])]
+                kblk[o:o + ls, :] = X[r0:r0 + ls, :]
+                bblk[o:o + ls, r0:r0 + ls] = np.eye(ls)
+        bra, ket = DeviceMPS(ctx, lb_, lb_.to_right(bra_lp)), DeviceMPS(ctx, lk, lk.to_right(ket_lp))
+        grown = lstep(ctx, mpo, boundary, bra, ket, symmetric=False)
+        noise = boundary_gram(grown, 0, alpha, [(m, rows_n[m]) for m in keys])
+        dms = {m: mats[m][0] @ mats[m][0].T + noise[m] for m in keys}
+    else:
+        fused = Index(keys, [rows_n[m] for m in keys])  # (s1, l) fused
+        ident = Index(keys, [cols_n[m] for m in keys])
+        lk, lb_ = MPSLayout(fused, phys, right), MPSLayout(ident, phys, right)
+        ket_rp, bra_rp = np.zeros(lk.n), np.zeros(lb_.n)
+        for m in keys:
+            X, _, cols = mats[m]
+            kb, bb = lk.left.position(m), lb_.left.position(m)
+            kblk = lk.rp_block(ket_rp, kb)
+            bblk = lb_.rp_block(bra_rp, bb)
+            for s2, rbi, c0 in cols:
+                rs = right.sizes[rbi]
+                o = lk.rpb[(s2, right.charges[rbi])]
+                kblk[:, o:o + rs] = X[:, c0:c0 + rs]
+                bblk[c0:c0 + rs, o:o + rs] = np.eye(rs)
+        bra, ket = DeviceMPS(ctx, lb_, bra_rp), DeviceMPS(ctx, lk, ket_rp)
+        # the right step completes partner cohorts in any case: grow with the unpaired MPO
+        unpaired = DeviceMPO(ctx, _Unpaired(mpo.spec))
+        grown = rstep(ctx, unpaired, boundary, bra, ket, symmetric=False)
+        unpaired.close()
+        noise = boundary_gram(grown, 1, alpha, [(m, cols_n[m]) for m in keys])
+        dms = {m: mats[m][0].T @ mats[m][0] + noise[m] for m in keys}
+    grown.close()
+    bra.close()
+    ket.close()
+    eig = {}
+    for m, dm in dms.items():
+        w, V = np.linalg.eigh(0.5 * (dm + dm.T))
+        eig[m] = (w[::-1], V[:, ::-1])
+    keeps, rec = heev_truncation({m: w for m, (w, _) in eig.items()}, Mmax, cutoff)
+    if details is not None:
+        details.update(rec)
+    mid = Index([m for m in keys if keeps[m] > 0], [keeps[m] for m in keys if keeps[m] > 0])
+    la = MPSLayout(left, phys, mid)
+    lb = MPSLayout(mid, phys, right)
+    a_lp = np.zeros(la.n)
+    b_rp = np.zeros(lb.n)
+    for m in keys:
+        k = keeps[m]
+        if k == 0:
+            continue
+        X, rows, cols = mats[m]
+        U = eig[m][1][:, :k]
+        Am, Bm = (U, U.T @ X) if to_right else (X @ U, U.T)
+        mb = mid.position(m)
+        a_blk = a_lp[la.lp_off[mb]:la.lp_off[mb + 1]].reshape(k, la.lp_rows[mb]).T
+        for s1, lbi, r0 in rows:
+            ls = left.sizes[lbi]
+            o = la.lpb[(s1, left.charges[lbi])]
+            a_blk[o:o + ls, :] = Am[r0:r0 + ls, :]
+        b_blk = b_rp[lb.rp_off[mb]:lb.rp_off[mb + 1]].reshape(lb.rp_cols[mb], k).T
+        for s2, rbi, c0 in cols:
+            rs = right.sizes[rbi]
+            o = lb.rpb[(s2, right.charges[rbi])]
+            b_blk[:, o:o + rs] = Bm[:, c0:c0 + rs]
+    return la, la.to_right(a_lp), lb, b_rp, rec["TruncatedWeight"]
+
+
 # ---- MPS initialisation ------------------------------------------------------------------------
 def allowed_bonds(phys_list, total, m_cap, full_right=False):
     """bond indices with per-sector sizes min(#states from the left, #states to the right, m_cap)
@@ -329,7 +461,8 @@ class TwoSiteDMRG:
 
     def __init__(self, ham, n_up, n_dn, M, ctx=None, init_m=8, seed=42, tol=1e-8, max_iter=8, cutoff=1e-16,
                  verbose=False, precision=abi.MQ_PREC_FP64, mps=None, resultfile=None, first_sweep=0,
-                 device_split=False, snapshots=(), snapshot_dir=".", gmres=0):
+                 device_split=False, snapshots=(), snapshot_dir=".", gmres=0, twosite_truncation="svd",
+                 alpha_initial=1e-2, alpha_main=1e-4, alpha_final=1e-8, ngrowsweeps=0, nmainsweeps=0):
         """precision: arithmetic of the Davidson H*psi (MQ_PREC_FP32 = FP32 contraction with an FP64
         refinement of each eigenvector, see mq_davidson); mps: (layouts, data) to start from (e.g.
         results.load_checkpoint) instead of a random state; resultfile: HDF5 result file that every
@@ -337,7 +470,14 @@ class TwoSiteDMRG:
         device_split: the block SVDs of the two-site split on the device (mq_block_svd); snapshots:
         (2 sweep + direction, site) pairs whose site problem is saved before its solve, as
         snapshot_dir/snapshot_<twosweep>_<site>.h5 (snapshot.save; ts_optimize.hpp:185-213); gmres:
-        GMRES steps of the Davidson correction equation (ietl_jcd_gmres, default 0)"""
+        GMRES steps of the Davidson correction equation (ietl_jcd_gmres, default 0);
+        twosite_truncation: "svd" (split_two) or "heev" (predict_split: the density matrix with the
+        noise term alpha, alpha_initial in the first ngrowsweeps sweeps, alpha_main in the next
+        nmainsweeps, alpha_final after; ts_optimize.hpp:275-296, defaults of DmrgParameters.h)"""
+        if twosite_truncation not in ("svd", "heev"):
+            raise ValueError("twosite_truncation must be 'svd' or 'heev'")
+        self.twosite_truncation = twosite_truncation
+        self.alphas = (float(alpha_initial), float(alpha_main), float(alpha_final), int(ngrowsweeps), int(nmainsweeps))
         self.gmres = int(gmres)
         self.device_split = device_split
         self.snapshots = {(int(a), int(b)) for a, b in snapshots}
@@ -454,9 +594,17 @@ class TwoSiteDMRG:
                 red = TwoSiteReduction(lay2, self.phys, self.off12)
             psi = red.unreduce(psi)
         trunc = {}
-        svd_fn = (lambda blocks: block_svd(self.ctx, blocks)) if self.device_split else None
-        la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right, trunc, self.sym,
-                                     svd_fn)
+        if self.twosite_truncation == "heev":
+            a0, a1, a2, ngs, nms = self.alphas
+            sw = self.sweep_number
+            alpha = a0 if sw < ngs else (a1 if sw < ngs + nms else a2)
+            boundary, mpo = (self.left[i], self.mpo1[i]) if to_right else (self.right[i + 2], self.mpo1[i + 1])
+            la, a, lb, b, tw = predict_split(self.ctx, lay2, psi, self.phys, self.off12, self.M, self.cutoff, alpha,
+                                             to_right, boundary, mpo, trunc, self.sym)
+        else:
+            svd_fn = (lambda blocks: block_svd(self.ctx, blocks)) if self.device_split else None
+            la, a, lb, b, tw = split_two(lay2, psi, self.phys, self.off12, self.M, self.cutoff, to_right, trunc,
+                                         self.sym, svd_fn)
         t4 = time.perf_counter()
         self.lay[i], self.data[i], self.lay[i + 1], self.data[i + 1] = la, a, lb, b
         if to_right:

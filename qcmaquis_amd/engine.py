@@ -213,6 +213,28 @@ def block_svd(ctx, blocks):
     return out
 
 
+def boundary_gram(boundary, side, alpha, blocks):
+    """noise term of the density-matrix split (mq_boundary_gram; alpha_dm_direct and
+    alpha_dm_direct_right, move_boundary.hpp:38-145): for a boundary grown with an identity bra and
+    symmetric = False, {charge: alpha * sum X X^T} over the cohorts of each row charge (side 0) or
+    {charge: alpha * sum X_b^T X_b} over the blocks of each column charge (side 1); blocks: list of
+    (charge, size) of the grouped index"""
+    if not blocks:
+        return {}
+    charges = np.ascontiguousarray(np.array([c for c, _ in blocks], np.int32).reshape(-1, 2))
+    sizes = np.array([n for _, n in blocks], np.uint32)
+    out = np.empty(int((sizes.astype(np.int64) ** 2).sum()))
+    ctx = boundary.ctx
+    ctx.check(ctx.lib.mq_boundary_gram(boundary.h, int(side), float(alpha), len(blocks),
+                                       charges.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       sizes.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(out)))
+    res, o = {}, 0
+    for (c, n) in blocks:
+        res[c] = out[o:o + n * n].reshape(n, n).T
+        o += n * n
+    return res
+
+
 class HpsiPlan(_Handle):
     """create_contraction_schedule + the device work lists of one site (mq_hpsi_plan_create)."""
 

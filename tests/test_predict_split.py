@@ -1,0 +1,224 @@
+"""Density-matrix split with the noise term (dmrg.predict_split; the reference's predict_split_l2r /
+predict_split_r2l with alpha_dm_direct, twosite_truncation = heev) on the CPU: the device calls
+(the identity-bra growth step and mq_boundary_gram) are replaced by the oracle's growth steps and a
+numpy sum of the grown blocks' Gram matrices, so the host logic is checked without a GPU against:
+
+* a known answer of the heev_truncate record (square roots of the positive eigenvalues only, the
+  truncated fraction reported as the truncated weight);
+* the dense definition on a TwoU1 problem: the density matrix X X^T + alpha sum_b S_b S_b^T (or
+  X^T X + alpha sum_b S_b^T S_b) with S_b contracted from dense L, W and psi by one einsum; the
+  split's A B equals the projection of X onto the kept eigenvectors of that matrix, its record the
+  one of the matrix's eigenvalues, and A (l2r) or B (r2l) is orthonormal;
+* a special case on SU2U1: with an identity boundary and an identity MPO the noise is alpha X X^T,
+  so the split keeps the SVD split's states and its truncated fraction;
+* alpha = 0 is the SVD split.
+"""
+import numpy as np
+import pytest
+
+from qcmaquis_amd import abi, dmrg
+from qcmaquis_amd.dmrg import heev_truncation, merge_two, predict_split, split_matrices, split_two, twosite_offsets
+from qcmaquis_amd.model import BoundarySpec, MPOSpec, MPSLayout, SiteOp
+from qcmaquis_amd.synthetic import fuse_index
+from oracle import dense
+from tests.chains import make_chain, oracle_left, oracle_right
+
+
+class Wrap:
+    def __init__(self, spec):
+        self.spec = spec
+
+    def close(self):
+        pass
+
+
+class FakeMPS(Wrap):
+    def __init__(self, ctx, layout, data):
+        self.layout, self.data = layout, np.array(data)
+
+
+def spec_gram(b, side, alpha, blocks):
+    """alpha * sum X X^T per row charge (side 0) or X_b^T X_b per column charge (side 1)"""
+    spec = b.spec
+    out = {c: np.zeros((n, n)) for c, n in blocks}
+    store = spec.cohort_offsets()
+    for ci, coh in enumerate(spec.cohorts):
+        if coh.get("tr", 0):
+            assert all(o < 0 for o in coh["offsets"])
+            continue
+        lb, rb = coh["lr"]
+        ls, rs = spec.bra.sizes[lb], spec.ket.sizes[rb]
+        c = spec.bra.charges[lb] if side == 0 else spec.ket.charges[rb]
+        if c not in out:
+            continue
+        for o in coh["offsets"]:
+            if o < 0:
+                continue
+            X = spec.data[store[ci] + o: store[ci] + o + ls * rs].reshape(rs, ls).T
+            out[c] += alpha * (X @ X.T if side == 0 else X.T @ X)
+    return out
+
+
+@pytest.fixture
+def cpu_split(monkeypatch, oracle_lib):
+    def grow(fn, sym):
+        def step(ctx, mpo, b, bra, ket, symmetric=True):
+            assert not symmetric
+            return Wrap(fn(sym, mpo.spec, b.spec, bra.layout, bra.data, ket.layout, ket.data, symmetric=False,
+                           nthreads=1))
+        return step
+
+    def setup(sym):
+        monkeypatch.setattr(dmrg, "DeviceMPS", FakeMPS)
+        monkeypatch.setattr(dmrg, "DeviceMPO", lambda ctx, spec: Wrap(spec))
+        monkeypatch.setattr(dmrg, "lstep", grow(oracle_lib.lstep, sym))
+        monkeypatch.setattr(dmrg, "rstep", grow(oracle_lib.rstep, sym))
+        monkeypatch.setattr(dmrg, "boundary_gram", spec_gram)
+    return setup
+
+
+def test_heev_truncation_known_answer():
+    evals = {(2, 0): np.array([4.0, 1.0, -0.04]), (1, 1): np.array([0.25, 0.0])}
+    keeps, rec = heev_truncation(evals, 1, 0.0)
+    # roots 2, 1, 0.5, 0 and the negative eigenvalue as it is; the cut is the value at Mmax = 1
+    assert keeps == {(2, 0): 2, (1, 1): 0}
+    assert rec["BondDimension"] == 2
+    assert rec["TruncatedFraction"] == pytest.approx(0.46 / 3.46, rel=1e-14)
+    assert rec["TruncatedWeight"] == rec["TruncatedFraction"]
+    assert rec["SmallestEV"] == pytest.approx(0.5, rel=1e-14)
+    keeps, rec = heev_truncation(evals, 10, 0.3)  # relative cutoff 0.3 of the largest root (2)
+    assert keeps == {(2, 0): 2, (1, 1): 0} and rec["SmallestEV"] == pytest.approx(0.3)
+
+
+def _problem(sym, seed, max_m):
+    ch = make_chain(seed, sym, n_sites=4, total=(2, 2) if sym == abi.MQ_SYM_TWOU1 else (4, 0), max_m=max_m, D=5)
+    phys = ch.phys[0]
+    phys2 = fuse_index(phys, phys)
+    off12 = twosite_offsets(phys)
+    lay2, psi2 = merge_two(ch.layouts[1], ch.data[1], ch.layouts[2], ch.data[2], phys2, off12)
+    psi2 = psi2 / np.linalg.norm(psi2)
+    return ch, phys, off12, lay2, psi2
+
+
+def _expected_dense(ch, oracle_lib, phys, off12, lay2, psi2, alpha, to_right):
+    """density matrix of every middle charge from dense L (or R), W and psi"""
+    ol, op_, orr = dense.offsets(lay2.left), dense.offsets(phys), dense.offsets(lay2.right)
+    op2 = dense.offsets(lay2.phys)
+    psi3 = dense.dense_mps(lay2, psi2)
+    d = op_[-1]
+    psi4 = np.zeros((ol[-1], d, d, orr[-1]))
+    for (s1, s2), (c, o) in off12.items():
+        psi4[:, op_[phys.position(s1)], op_[phys.position(s2)], :] = psi3[:, op2[lay2.phys.position(c)] + o, :]
+    if to_right:
+        L = oracle_left(oracle_lib, ch, 1)[1]
+        W = dense.dense_mpo(ch.mpos[1], phys)
+        S = np.einsum("xij,xyst,jsur->yitur", dense.dense_boundary(L), W, psi4, optimize=True)
+        S = S.reshape(S.shape[0], ol[-1] * d, -1)
+        DM = alpha * np.einsum("yak,ybk->ab", S, S, optimize=True)
+        P = psi4.reshape(ol[-1] * d, -1)
+        DM += P @ P.T
+    else:
+        R = oracle_right(oracle_lib, ch, 3)[3]
+        W = dense.dense_mpo(ch.mpos[2], phys)
+        S = np.einsum("aqsk,xyst,ykl->xaqtl", psi4, W, dense.dense_boundary(R), optimize=True)
+        S = S.reshape(S.shape[0], ol[-1] * d, d * orr[-1])
+        DM = alpha * np.einsum("xka,xkb->ab", S, S, optimize=True)
+        P = psi4.reshape(ol[-1] * d, d * orr[-1])
+        DM += P.T @ P
+    mats = split_matrices(lay2, psi2, phys, off12, ch.sym)
+    out = {}
+    for m, (X, rows, cols) in mats.items():
+        if to_right:
+            idx = np.concatenate([(np.arange(lay2.left.sizes[lb]) + ol[lb]) * d + op_[phys.position(s1)]
+                                  for s1, lb, _ in rows])
+        else:
+            idx = np.concatenate([op_[phys.position(s2)] * orr[-1] + orr[rb] + np.arange(lay2.right.sizes[rb])
+                                  for s2, rb, _ in cols])
+        out[m] = DM[np.ix_(idx, idx)]
+    return mats, out
+
+
+@pytest.mark.parametrize("to_right", [True, False])
+def test_noise_split_matches_dense_definition(cpu_split, oracle_lib, to_right):
+    sym = abi.MQ_SYM_TWOU1
+    cpu_split(sym)
+    ch, phys, off12, lay2, psi2 = _problem(sym, 31, 4)
+    alpha, Mmax = 0.3, 5
+    mats, dms = _expected_dense(ch, oracle_lib, phys, off12, lay2, psi2, alpha, to_right)
+    L = oracle_left(oracle_lib, ch, 1)[1]
+    R = oracle_right(oracle_lib, ch, 3)[3]
+    boundary, mpo = (Wrap(L), Wrap(ch.mpos[1])) if to_right else (Wrap(R), Wrap(ch.mpos[2]))
+    rec = {}
+    la, a, lb, b, tw = predict_split(None, lay2, psi2, phys, off12, Mmax, 0.0, alpha, to_right, boundary, mpo, rec,
+                                     sym)
+    eig = {m: np.linalg.eigh(dm) for m, dm in dms.items()}
+    keeps, want = heev_truncation({m: w[::-1] for m, (w, _) in eig.items()}, Mmax, 0.0)
+    # eigenvalues at the rounding level (~1e-16 of the largest) enter the record as their square
+    # roots (~1e-8), so the truncated fraction agrees to that level only
+    assert rec["BondDimension"] == want["BondDimension"] and tw == rec["TruncatedWeight"]
+    for k in ("TruncatedFraction", "TruncatedWeight", "SmallestEV"):
+        assert abs(rec[k] - want[k]) < 1e-7, k
+    assert sum(keeps.values()) < sum(X.shape[0 if to_right else 1] for X, _, _ in mats.values())  # it truncates
+    assert dict(zip(la.right.charges, la.right.sizes)) == {m: k for m, k in keeps.items() if k}
+    # the split reassembles to X projected on the kept eigenvectors
+    _, got = merge_two(la, a, lb, b, lay2.phys, off12)
+    got_m = split_matrices(lay2, got, phys, off12, sym)
+    for m, (X, _, _) in mats.items():
+        U = eig[m][1][:, ::-1][:, :keeps[m]]
+        want_m = U @ (U.T @ X) if to_right else (X @ U) @ U.T
+        assert np.max(np.abs(got_m[m][0] - want_m)) < 1e-12
+    # A left-normalised (l2r) or B right-normalised (r2l)
+    if to_right:
+        for mb in range(len(la.right)):
+            blk = la.to_left(a)[la.lp_off[mb]:la.lp_off[mb + 1]].reshape(la.right.sizes[mb], -1).T
+            assert np.allclose(blk.T @ blk, np.eye(blk.shape[1]), atol=1e-12)
+    else:
+        for mb in range(len(lb.left)):
+            blk = lb.rp_block(b, mb)
+            assert np.allclose(blk @ blk.T, np.eye(blk.shape[0]), atol=1e-12)
+
+
+def _identity_boundary(index):
+    cohorts, data = [], []
+    for i, s in enumerate(index.sizes):
+        cohorts.append(dict(lr=(i, i), offsets=[0]))
+        data.append(np.eye(s).ravel(order="F"))
+    return BoundarySpec(index, index, 1, cohorts, np.concatenate(data))
+
+
+def _identity_mpo(phys, sym):
+    ents = lambda c: [(0, 0, 1.0) + ((abs(c[1]), abs(c[1])) if sym == abi.MQ_SYM_SU2U1 else ())]
+    op = SiteOp([(c, c, ents(c)) for c in phys.charges], spin=0)
+    return MPOSpec(1, 1, [op], {(0, 0): [(0, 1.0)]}, left_spin=[0], right_spin=[0], left_herm=[0], left_phase=[1],
+                   right_herm=[0], right_phase=[1])
+
+
+@pytest.mark.parametrize("sym", [abi.MQ_SYM_SU2U1, abi.MQ_SYM_TWOU1])
+@pytest.mark.parametrize("to_right", [True, False])
+def test_identity_mpo_noise_keeps_the_svd_split(cpu_split, sym, to_right):
+    cpu_split(sym)
+    ch, phys, off12, lay2, psi2 = _problem(sym, 8, 5)
+    alpha, Mmax = 0.5, 6
+    b = Wrap(_identity_boundary(lay2.left if to_right else lay2.right))
+    rec, rec_svd = {}, {}
+    la, a, lb, bb, _ = predict_split(None, lay2, psi2, phys, off12, Mmax, 0.0, alpha, to_right, b,
+                                     Wrap(_identity_mpo(phys, sym)), rec, sym)
+    la2, a2, lb2, b2, _ = split_two(lay2, psi2, phys, off12, Mmax, 0.0, to_right, rec_svd, sym)
+    assert la.right == la2.right and rec["BondDimension"] == rec_svd["BondDimension"]
+    # zero eigenvalues of the density matrix come out at the rounding level and enter the record as
+    # their square roots (~1e-8)
+    assert abs(rec["TruncatedFraction"] - rec_svd["TruncatedFraction"]) < 1e-7
+    assert abs(rec["SmallestEV"] - rec_svd["SmallestEV"]) < 1e-7
+    _, got = merge_two(la, a, lb, bb, lay2.phys, off12)
+    _, ref = merge_two(la2, a2, lb2, b2, lay2.phys, off12)
+    assert np.max(np.abs(got - ref)) < 1e-12
+
+
+def test_alpha_zero_is_the_svd_split(cpu_split):
+    sym = abi.MQ_SYM_TWOU1
+    cpu_split(sym)
+    ch, phys, off12, lay2, psi2 = _problem(sym, 31, 4)
+    r1, r2 = {}, {}
+    got = predict_split(None, lay2, psi2, phys, off12, 5, 0.0, 0.0, True, None, None, r1, sym)
+    ref = split_two(lay2, psi2, phys, off12, 5, 0.0, True, r2, sym)
+    assert r1 == r2 and all(np.array_equal(x, y) for x, y in zip((got[1], got[3]), (ref[1], ref[3])))
