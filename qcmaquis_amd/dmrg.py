@@ -680,22 +680,106 @@ def _absorb_into_left_neighbour(lay, data, mid, Ls):
     return nl, nl.to_right(nlp)
 
 
-class SingleSiteDMRG:
-    """Single-site DMRG (ss_optimize.hpp without the subspace-expansion noise): each step solves the
-    one-site problem H_eff(L_i, W_i, R_{i+1}) with the device Jacobi-Davidson, then moves the
-    orthogonality centre with an SVD of the site tensor (normalize_left / normalize_right,
-    mpstensor.hpp:242) and multiplies the remainder into the neighbour; the left or right boundary
-    of the moved bond is grown on the device.  Bond dimensions do not grow, so the bonds the state
-    starts from (allowed sectors capped at M) are the variational space."""
+def grow_sweep_step(ctx, lay, psi, Mmax, cutoff, alpha, to_right, boundary, mpo, details=None):
+    """single-site subspace expansion (Engine::grow_l2r_sweep / grow_r2l_sweep, engine.hpp:82-118):
+    l2r: the density matrix of the solved site tensor psi, left-paired (rows (s, l), columns the
+    right bond), X_m X_m^T + alpha sum_b S_b S_b^T with S_b the S tiles of the left growth of psi
+    (predict_new_state_l2r_sweep, prediction.hpp:70-92, alpha_dm); heev_truncate keeps its leading
+    eigenvectors U, which span up to all (s, l) rows of a sector, so the bond can grow; the site
+    becomes U and the right neighbour takes U^T psi (predict_lanczos_l2r_sweep,
+    prediction.hpp:94-108).  r2l: X_m^T X_m + alpha sum_b S_b^T S_b per left sector of the
+    right-paired psi from the right growth, the site becomes U^T and the left neighbour takes
+    psi U (prediction.hpp:110-148).  The S tiles come from a growth step with an identity bra and
+    mq_boundary_gram, as in predict_split.  Returns (new layout, right-paired data, middle index,
+    carry per middle sector, truncated weight)."""
+    if to_right:
+        lp = lay.to_left(psi)
+        keys = [(rb, c) for rb, c in enumerate(lay.right.charges) if lay.lp_rows[rb] and lay.right.sizes[rb]]
+        Xs = {c: lp[lay.lp_off[rb]:lay.lp_off[rb + 1]].reshape(lay.right.sizes[rb], lay.lp_rows[rb]).T
+              for rb, c in keys}
+        dims = {c: lay.lp_rows[rb] for rb, c in keys}
+    else:
+        keys = [(lb, c) for lb, c in enumerate(lay.left.charges) if lay.rp_cols[lb] and lay.left.sizes[lb]]
+        Xs = {c: rp_block(lay, psi, lb) for lb, c in keys}
+        dims = {c: lay.rp_cols[lb] for lb, c in keys}
+    charges = [c for _, c in keys]
+    if alpha != 0.0 and charges:
+        ident = Index(charges, [dims[c] for c in charges])
+        if to_right:
+            bl = MPSLayout(lay.left, lay.phys, ident)
+            b_lp = np.zeros(bl.n)
+            for mb, c in enumerate(bl.right.charges):
+                b_lp[bl.lp_off[mb]:bl.lp_off[mb + 1]] = np.eye(dims[c]).reshape(-1)
+            bra, ket = DeviceMPS(ctx, bl, bl.to_right(b_lp)), DeviceMPS(ctx, lay, psi)
+            grown = lstep(ctx, mpo, boundary, bra, ket, symmetric=False)
+        else:
+            bl = MPSLayout(ident, lay.phys, lay.right)
+            b_rp = np.zeros(bl.n)
+            for mb, c in enumerate(bl.left.charges):
+                b_rp[bl.rp_off[mb]:bl.rp_off[mb + 1]] = np.eye(dims[c]).reshape(-1)
+            bra, ket = DeviceMPS(ctx, bl, b_rp), DeviceMPS(ctx, lay, psi)
+            unpaired = DeviceMPO(ctx, _Unpaired(mpo.spec))
+            grown = rstep(ctx, unpaired, boundary, bra, ket, symmetric=False)
+            unpaired.close()
+        noise = boundary_gram(grown, 0 if to_right else 1, alpha, [(c, dims[c]) for c in charges])
+        grown.close()
+        bra.close()
+        ket.close()
+    else:
+        noise = {c: 0.0 for c in charges}
+    eig = {}
+    for c in charges:
+        X = Xs[c]
+        dm = (X @ X.T if to_right else X.T @ X) + noise[c]
+        w, V = np.linalg.eigh(0.5 * (dm + dm.T))
+        eig[c] = (w[::-1], V[:, ::-1])
+    keeps, rec = heev_truncation({c: w for c, (w, _) in eig.items()}, Mmax, cutoff)
+    if details is not None:
+        details.update(rec)
+    mid = Index([c for c in charges if keeps[c] > 0], [keeps[c] for c in charges if keeps[c] > 0])
+    carry = []
+    if to_right:
+        na = MPSLayout(lay.left, lay.phys, mid)
+        a_lp = np.zeros(na.n)
+        for mb, c in enumerate(mid.charges):
+            U = eig[c][1][:, :keeps[c]]
+            a_lp[na.lp_off[mb]:na.lp_off[mb + 1]] = U.T.reshape(-1)
+            carry.append(U.T @ Xs[c])
+        return na, na.to_right(a_lp), mid, carry, rec["TruncatedWeight"]
+    nb = MPSLayout(mid, lay.phys, lay.right)
+    b_rp = np.zeros(nb.n)
+    for mb, c in enumerate(mid.charges):
+        U = eig[c][1][:, :keeps[c]]
+        b_rp[nb.rp_off[mb]:nb.rp_off[mb + 1]] = U.reshape(-1)  # block U^T (k x cols), column-major
+        carry.append(Xs[c] @ U)
+    return nb, b_rp, mid, carry, rec["TruncatedWeight"]
 
-    def __init__(self, ham, n_up, n_dn, M, ctx=None, seed=42, tol=1e-8, max_iter=8, precision=abi.MQ_PREC_FP64):
+
+class SingleSiteDMRG:
+    """Single-site DMRG (ss_optimize.hpp): each step solves the one-site problem
+    H_eff(L_i, W_i, R_{i+1}) with the device Jacobi-Davidson, then moves the orthogonality centre
+    and grows the left or right boundary of the moved bond on the device.  The move is the
+    reference's subspace expansion (grow_sweep_step: the density matrix with the noise term alpha,
+    truncated to M states, so bonds grow from init_m up to M; alpha_initial in the first ngrowsweeps
+    sweeps, alpha_main in the next nmainsweeps, alpha_final after, ss_optimize.hpp:147-190) or, with
+    expansion=False, an SVD of the site tensor without truncation (normalize_left /
+    normalize_right, mpstensor.hpp:242), for which the bonds the state starts from are the
+    variational space."""
+
+    def __init__(self, ham, n_up, n_dn, M, ctx=None, seed=42, tol=1e-8, max_iter=8, precision=abi.MQ_PREC_FP64,
+                 expansion=False, init_m=None, cutoff=1e-16, alpha_initial=1e-2, alpha_main=1e-4, alpha_final=1e-8,
+                 ngrowsweeps=0, nmainsweeps=0):
+        self.M, self.cutoff, self.expansion = M, cutoff, bool(expansion)
+        self.alphas = (float(alpha_initial), float(alpha_main), float(alpha_final), int(ngrowsweeps), int(nmainsweeps))
+        self.sweep_number = 0
+        self.trunc = []
         self.ham, self.tol, self.max_iter, self.precision = ham, tol, max_iter, precision
         self.sym = getattr(ham, "sym", abi.MQ_SYM_TWOU1)
         self.ctx = ctx or Context(0, self.sym)
         self.L = ham.L
         self.phys = ham.phys()
         rng = np.random.default_rng(seed)
-        bonds = initial_bonds(self.sym, self.phys, self.L, n_up, n_dn, M)
+        bonds = initial_bonds(self.sym, self.phys, self.L, n_up, n_dn, M if init_m is None else init_m)
         self.lay = [MPSLayout(bonds[k], self.phys, bonds[k + 1]) for k in range(self.L)]
         self.data = [rng.normal(size=l.n) for l in self.lay]
         right_normalize(self.lay, self.data)
@@ -712,7 +796,25 @@ class SingleSiteDMRG:
         e, psi, _ = plan.davidson(self.data[i], tol=self.tol, max_iter=self.max_iter, precision=self.precision)
         plan.close()
         lay = self.lay[i]
-        if to_right:
+        if self.expansion:
+            a0, a1, a2, ngs, nms = self.alphas
+            sw = self.sweep_number
+            alpha = a0 if sw < ngs else (a1 if sw < ngs + nms else a2)
+            bnd = self.left[i] if to_right else self.right[i + 1]
+            nl, nd, mid, carry, tw = grow_sweep_step(self.ctx, lay, psi, self.M, self.cutoff, alpha, to_right, bnd,
+                                                     self.mpo1[i])
+            self.trunc.append(tw)
+            self.lay[i], self.data[i] = nl, nd
+            m = DeviceMPS(self.ctx, nl, nd)
+            if to_right:
+                self.lay[i + 1], self.data[i + 1] = _absorb_into_right_neighbour(self.lay[i + 1], self.data[i + 1], mid,
+                                                                                 carry)
+                self.left[i + 1] = lstep(self.ctx, self.mpo1[i], self.left[i], m, m)
+            else:
+                self.lay[i - 1], self.data[i - 1] = _absorb_into_left_neighbour(self.lay[i - 1], self.data[i - 1], mid,
+                                                                                carry)
+                self.right[i] = rstep(self.ctx, self.mpo1[i], self.right[i + 1], m, m)
+        elif to_right:
             lp = lay.to_left(psi)
             us, carry, sizes = {}, [], []
             for rb in range(len(lay.right)):
@@ -758,4 +860,5 @@ class SingleSiteDMRG:
             self.step(i, True)
         for i in range(self.L - 1, 0, -1):
             self.step(i, False)
+        self.sweep_number += 1
         return self.energies[-1], time.perf_counter() - t0

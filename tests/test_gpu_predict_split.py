@@ -7,7 +7,10 @@ growth step and mq_boundary_gram; the reference's predict_split_l2r / predict_sp
   device split equals the split with the oracle's growth steps and a host Gram sum, both
   directions (same kept states, record to 1e-7, A B to 1e-10 relative);
 * SU2U1 sweeps with twosite_truncation = heev (a growth sweep at alpha 1e-2, then alpha 1e-8) and
-  untruncated bond dimension reach the determinant FCI energy, as the SVD sweeps do.
+  untruncated bond dimension reach the determinant FCI energy, as the SVD sweeps do;
+* the single-site subspace expansion (dmrg.grow_sweep_step) on the device against the dense
+  density matrices (TwoU1, both directions, truncating and growing), and single-site sweeps with
+  the expansion from 4 states per sector reach the determinant FCI energy (SU2U1).
 """
 import json
 import os
@@ -20,7 +23,8 @@ from qcmaquis_amd.dmrg import TwoSiteDMRG, heev_truncation, merge_two, predict_s
 from qcmaquis_amd.engine import DeviceBoundary, DeviceMPO
 from qcmaquis_amd.qc import QCHamiltonian, make_integrals
 from tests.chains import oracle_left, oracle_right
-from tests.test_predict_split import Wrap, FakeMPS, _expected_dense, _problem, spec_gram
+from tests.test_predict_split import (FakeMPS, Wrap, _expansion_expected, _expected_dense, _one_site, _problem,
+                                     check_expansion, spec_gram)
 
 pytestmark = pytest.mark.gpu
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fci_golden.json")))
@@ -105,3 +109,31 @@ def test_su2_noise_sweeps_equal_fci(ctxsu2):
         tw = d.iteration_results["TruncatedWeight"]
         print(f"  sweep {s}: E={e:.12f} ({dt:.2f} s) max truncated fraction {max(tw):.2e}", flush=True)
     assert abs(e - g["e0"]) < 1e-9, (e, g["e0"])
+
+
+@pytest.mark.parametrize("Mmax", [4, 40])
+@pytest.mark.parametrize("to_right", [True, False])
+def test_device_expansion_matches_dense_definition(ctx2u1, oracle_lib, to_right, Mmax):
+    print(f"single-site expansion on the device, TwoU1, to_right={to_right}, Mmax={Mmax}", flush=True)
+    ch, lay, psi = _one_site(abi.MQ_SYM_TWOU1, 12, 4)
+    Xs, dms, spec = _expansion_expected(ch, oracle_lib, lay, psi, 0.2, to_right)
+    b, mpo = DeviceBoundary.upload(ctx2u1, spec), DeviceMPO(ctx2u1, ch.mpos[1])
+    rec = {}
+    out = dmrg.grow_sweep_step(ctx2u1, lay, psi, Mmax, 1e-6, 0.2, to_right, b, mpo, rec)
+    check_expansion(Xs, dms, Mmax, 1e-6, to_right, out, rec)
+    b.close()
+    mpo.close()
+
+
+def test_su2_single_site_expansion_sweeps_equal_fci(ctxsu2):
+    from qcmaquis_amd.dmrg import SingleSiteDMRG
+    g = GOLD["small6"]
+    print("su2 single-site sweeps with the subspace expansion, small6, from 4 states per sector", flush=True)
+    h, eri, ec = make_integrals(g["L"], g["seed"], g["rank"])
+    ham = QCHamiltonian(h, eri, ec, symmetry=SU2)
+    d = SingleSiteDMRG(ham, 3, 3, M=400, ctx=ctxsu2, tol=1e-10, max_iter=20, expansion=True, init_m=4,
+                       alpha_initial=1e-2, ngrowsweeps=2)
+    for s in range(8):
+        e, dt = d.sweep()
+        print(f"  sweep {s}: E={e:.12f} ({dt:.2f} s) bonds {[sum(l.right.sizes) for l in d.lay]}", flush=True)
+    assert abs(e - g["e0"]) < 1e-8, (e, g["e0"])

@@ -222,3 +222,136 @@ def test_alpha_zero_is_the_svd_split(cpu_split):
     got = predict_split(None, lay2, psi2, phys, off12, 5, 0.0, 0.0, True, None, None, r1, sym)
     ref = split_two(lay2, psi2, phys, off12, 5, 0.0, True, r2, sym)
     assert r1 == r2 and all(np.array_equal(x, y) for x, y in zip((got[1], got[3]), (ref[1], ref[3])))
+
+
+# ---- single-site subspace expansion (grow_sweep_step) -------------------------------------------
+def _one_site(sym, seed, max_m):
+    ch = make_chain(seed, sym, n_sites=4, total=(2, 2) if sym == abi.MQ_SYM_TWOU1 else (4, 0), max_m=max_m, D=5)
+    lay, psi = ch.layouts[1], ch.data[1] / np.linalg.norm(ch.data[1])
+    return ch, lay, psi
+
+
+def _expansion_blocks(lay, psi, to_right):
+    """X per charge: left-paired blocks per right sector (l2r) or right-paired per left sector"""
+    if to_right:
+        lp = lay.to_left(psi)
+        return {c: lp[lay.lp_off[rb]:lay.lp_off[rb + 1]].reshape(lay.right.sizes[rb], lay.lp_rows[rb]).T
+                for rb, c in enumerate(lay.right.charges) if lay.lp_rows[rb]}
+    return {c: lay.rp_block(psi, lb) for lb, c in enumerate(lay.left.charges) if lay.rp_cols[lb]}
+
+
+def _reassembled(to_right, nl, nd, mid, carry):
+    """the site tensor times the carry, per middle charge: U (U^T X) or (X U) U^T"""
+    out = {}
+    if to_right:
+        lp = nl.to_left(nd)
+        for mb, c in enumerate(mid.charges):
+            U = lp[nl.lp_off[mb]:nl.lp_off[mb + 1]].reshape(mid.sizes[mb], nl.lp_rows[mb]).T
+            out[c] = U @ carry[mb]
+    else:
+        for mb, c in enumerate(mid.charges):
+            out[c] = carry[mb] @ nl.rp_block(nd, mb)
+    return out
+
+
+def _expansion_expected(ch, oracle_lib, lay, psi, alpha, to_right):
+    """(X per charge, dense density matrix per charge, boundary spec) of the expansion at site 1"""
+    ol, op_, orr = dense.offsets(lay.left), dense.offsets(lay.phys), dense.offsets(lay.right)
+    d = op_[-1]
+    psi3 = dense.dense_mps(lay, psi)
+    W = dense.dense_mpo(ch.mpos[1], lay.phys)
+    if to_right:
+        bnd = oracle_left(oracle_lib, ch, 1)[1]
+        S = np.einsum("xij,xyst,jsr->yitr", dense.dense_boundary(bnd), W, psi3, optimize=True)
+        S = S.reshape(S.shape[0], ol[-1] * d, orr[-1])
+        P = psi3.reshape(ol[-1] * d, orr[-1])
+        DM = alpha * np.einsum("yak,ybk->ab", S, S, optimize=True) + P @ P.T
+    else:
+        bnd = oracle_right(oracle_lib, ch, 2)[2]
+        S = np.einsum("jsk,xyst,ykl->xjtl", psi3, W, dense.dense_boundary(bnd), optimize=True)
+        S = S.reshape(S.shape[0], ol[-1], d * orr[-1])
+        P = psi3.reshape(ol[-1], d * orr[-1])
+        DM = alpha * np.einsum("xka,xkb->ab", S, S, optimize=True) + P.T @ P
+    Xs = _expansion_blocks(lay, psi, to_right)
+    dms = {}
+    for c in Xs:
+        if to_right:
+            rb = lay.right.position(c)
+            idx = np.zeros(lay.lp_rows[rb], int)
+            for s, sc in enumerate(lay.phys.charges):
+                lb = lay.left.position((c[0] - sc[0], c[1] - sc[1]))
+                if lb < len(lay.left):
+                    o = lay.lpb[(sc, lay.left.charges[lb])]
+                    idx[o:o + lay.left.sizes[lb]] = (ol[lb] + np.arange(lay.left.sizes[lb])) * d + op_[s]
+        else:
+            lb = lay.left.position(c)
+            idx = np.zeros(lay.rp_cols[lb], int)
+            for s, sc in enumerate(lay.phys.charges):
+                rb = lay.right.position((c[0] + sc[0], c[1] + sc[1]))
+                if rb < len(lay.right):
+                    o = lay.rpb[(sc, lay.right.charges[rb])]
+                    idx[o:o + lay.right.sizes[rb]] = op_[s] * orr[-1] + orr[rb] + np.arange(lay.right.sizes[rb])
+        dms[c] = DM[np.ix_(idx, idx)]
+    return Xs, dms, bnd
+
+
+def check_expansion(Xs, dms, Mmax, cutoff, to_right, out, rec):
+    """the expansion step's result against the dense density matrices; returns (keeps, middle index)"""
+    nl, nd, mid, carry, tw = out
+    eig = {c: np.linalg.eigh(m) for c, m in dms.items()}
+    keeps, want = heev_truncation({c: w[::-1] for c, (w, _) in eig.items()}, Mmax, cutoff)
+    assert rec["BondDimension"] == want["BondDimension"] and abs(tw - want["TruncatedWeight"]) < 1e-7
+    assert dict(zip(mid.charges, mid.sizes)) == {c: k for c, k in keeps.items() if k}
+    got = _reassembled(to_right, nl, nd, mid, carry)
+    for c, k in keeps.items():
+        if not k:
+            continue
+        U = eig[c][1][:, ::-1][:, :k]
+        X = Xs[c]
+        assert np.max(np.abs(got[c] - (U @ (U.T @ X) if to_right else (X @ U) @ U.T))) < 1e-12
+    return keeps, mid
+
+
+@pytest.mark.parametrize("Mmax", [4, 40])
+@pytest.mark.parametrize("to_right", [True, False])
+def test_expansion_matches_dense_definition(cpu_split, oracle_lib, to_right, Mmax):
+    """Mmax 4 truncates; Mmax 40 keeps every state of positive weight, and the noise term gives
+    some sectors more states than the bond had (the expansion).  A relative cutoff of 1e-6 keeps
+    the count of kept states away from eigenvalues at the rounding level, whose sign (and so
+    whether they are kept) is not fixed."""
+    sym = abi.MQ_SYM_TWOU1
+    cpu_split(sym)
+    ch, lay, psi = _one_site(sym, 12, 4)
+    alpha = 0.2
+    Xs, dms, bnd = _expansion_expected(ch, oracle_lib, lay, psi, alpha, to_right)
+    rec = {}
+    out = dmrg.grow_sweep_step(None, lay, psi, Mmax, 1e-6, alpha, to_right, Wrap(bnd), Wrap(ch.mpos[1]), rec)
+    keeps, mid = check_expansion(Xs, dms, Mmax, 1e-6, to_right, out, rec)
+    bond = lay.right if to_right else lay.left
+    if Mmax == 4:
+        assert sum(mid.sizes) < bond.total()
+    else:
+        assert any(k > bond.size_of(c) for c, k in keeps.items())
+
+
+@pytest.mark.parametrize("sym", [abi.MQ_SYM_SU2U1, abi.MQ_SYM_TWOU1])
+@pytest.mark.parametrize("to_right", [True, False])
+def test_expansion_with_identity_mpo_keeps_the_tensor(cpu_split, sym, to_right):
+    """noise alpha X X^T: no truncation (Mmax above every sector), so the site tensor times the
+    carry is psi itself, and the site tensor is orthonormal"""
+    cpu_split(sym)
+    ch, lay, psi = _one_site(sym, 5, 4)
+    b = Wrap(_identity_boundary(lay.left if to_right else lay.right))
+    nl, nd, mid, carry, _ = dmrg.grow_sweep_step(None, lay, psi, 10 ** 6, 0.0, 0.7, to_right, b,
+                                                 Wrap(_identity_mpo(lay.phys, sym)))
+    Xs = _expansion_blocks(lay, psi, to_right)
+    got = _reassembled(to_right, nl, nd, mid, carry)
+    for c, X in Xs.items():
+        assert np.max(np.abs(got[c] - X)) < 1e-12
+    for mb, c in enumerate(mid.charges):
+        if to_right:
+            U = nl.to_left(nd)[nl.lp_off[mb]:nl.lp_off[mb + 1]].reshape(mid.sizes[mb], -1).T
+            assert np.allclose(U.T @ U, np.eye(U.shape[1]), atol=1e-12)
+        else:
+            B = nl.rp_block(nd, mb)
+            assert np.allclose(B @ B.T, np.eye(B.shape[0]), atol=1e-12)
