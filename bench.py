@@ -984,7 +984,7 @@ def main():
                "unit": "GB/s", "frac": round(s_gbs / PEAK_HBM_GBS, 5), "traffic": None,
                "achieved_as_executed": round(s_exec_gbs, 2),
                "work_per_launch": "%d bytes: T read once (%d el) and S written once (%d el); as executed %d blocks "
-                                  "of 16 lines read %d union T tile elements; %d terms, %d el in the per-term "
+                                  "of up to 64 lines read %d union T tile elements; %d terms, %d el in the per-term "
                                   "formulation" % (s_bytes, st["t_elements"], st["s_line_elements"], st["n_sblocks"],
                                                    st["union_elements"], st["n_terms"], st["gathered_elements"])}
     # the dominant kernel: the term sum is one launch per H*psi and the largest single share of its
