@@ -793,6 +793,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="qc / qc3: schedule the benchmark plan's GEMM stages by the default tile rule instead of "
                          "the autotuner's timed pick (MQ_OPT_AUTOTUNE)")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run the benchmark plan's chunks one after the other instead of overlapping each chunk's "
+                         "output GEMMs with the next chunk's T-GEMMs and term sum (MQ_OPT_OVERLAP)")
     ap.add_argument("--sblock-xcd", type=int, default=None,
                     help="1: each XCD takes a contiguous range of the term-sum work items")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -851,8 +854,10 @@ def main():
     # the sweeps below keep the default rule (their stage keys rarely repeat)
     autotune = args.workload in ("qc", "qc3") and not args.no_autotune
     ctx.set_option(abi.MQ_OPT_AUTOTUNE, int(autotune))
+    ctx.set_option(abi.MQ_OPT_OVERLAP, int(not args.no_overlap))
     plan = HpsiPlan(ctx, w["mpo"], w["L"], w["R"], w["layout"])
     ctx.set_option(abi.MQ_OPT_AUTOTUNE, 0)
+    ctx.set_option(abi.MQ_OPT_OVERLAP, 1)
     st = plan.stats()
     log("plan:", {k: st[k] for k in ("n_t_tasks", "n_cohorts", "n_lines", "n_terms", "flops_total", "flops_gemm",
                                       "flops_gemm_exec", "bytes_compulsory", "t_elements", "s_elements",
@@ -1030,6 +1035,7 @@ def main():
         "roofline": dominant,
         "roofline_other": other,
     }
+    line["config"]["chunks_overlapped"] = bool(st["n_chunks"] > 1 and not args.no_overlap)
     if autotune:
         line["config"]["autotune"] = {"picks_t_ls": [st["gemm_policy_t"], st["gemm_policy_c"]],
                                       "seconds_tune": round(st["seconds_tune"], 3),

@@ -87,6 +87,12 @@ extern "C" {
                                   ietl_gmres, ietl/gmres.h:89-145); 0 (default): the plain correction
                                   t = -r + (r.u / u.u) u.  mq_davidson_noalloc refuses values > 0; the
                                   batch Davidson uses the plain correction. */
+#define MQ_OPT_OVERLAP 9       /* value 1 (default): H*psi plans made afterwards that run in two or more
+                                  chunks overlap chunk c's output GEMMs (on a second stream of the
+                                  context) with chunk c + 1's T-GEMMs and term sum, with a second S
+                                  buffer (a plan without room for it runs the chunks in turn);
+                                  0: one chunk after the other.  Timed H*psi (mq_hpsi_timed) runs the
+                                  chunks in turn. */
 #define MQ_OPT_TERM_SOURCES 4  /* value 1: H*psi plans made afterwards (single site problems) also keep,
                                   per S term, the MPO term it comes from, for
                                   mq_hpsi_term_gradient (default 0) */

@@ -32,6 +32,7 @@ MQ_SBLOCK_PANEL_AUTO = -1
 MQ_OPT_SBLOCK_XCD = 6
 MQ_OPT_AUTOTUNE = 7
 MQ_OPT_JCD_GMRES = 8
+MQ_OPT_OVERLAP = 9
 MQ_QC_GRADIENT = 1
 MQ_RIGHT_PAIRED = 0
 MQ_LEFT_PAIRED = 1

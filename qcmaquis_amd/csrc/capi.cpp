@@ -42,6 +42,8 @@ struct mq_ctx_s {
     bool sblock_xcd = false;                      // MQ_OPT_SBLOCK_XCD
     bool autotune = false;                        // MQ_OPT_AUTOTUNE
     int32_t jcd_gmres = 0;                        // MQ_OPT_JCD_GMRES
+    bool overlap = true;                          // MQ_OPT_OVERLAP
+    hipStream_t stream2 = nullptr;     // second stream of overlapped H*psi chunks, made on first use
     struct Tuned {
         int32_t pick;
         float best_ms, default_ms;
@@ -438,13 +440,22 @@ struct mq_hpsi_plan_s {
     std::vector<Chunk> chunks;
     double* gemm_ws = nullptr;   // split-K workspace shared by the chunks' GEMM stages
     uint64_t gemm_ws_elems = 0;
+    // overlapped chunks (MQ_OPT_OVERLAP, plans of two or more chunks): chunk c's output GEMMs run on
+    // the context's second stream from the S buffer of parity c % 2 (S, S2) and their own split-K
+    // workspace, while chunk c + 1's T-GEMMs and term sum run on the context stream; per chunk one
+    // event marks its term sum and one its output GEMMs
+    double* S2 = nullptr;
+    double* gemm_ws2 = nullptr;
+    uint64_t gemm_ws2_elems = 0;
+    std::vector<hipEvent_t> ev_sa, ev_cg;
+    hipEvent_t ev_fork = nullptr;
     int32_t policy_t = 0, policy_c = 0;  // schedule candidates of chunk 0's stages
     double seconds_tune = 0;
     int64_t n_lines = 0, n_sblocks = 0, n_reduces = 0;
     uint64_t s_line_elements = 0, gathered_elements = 0, union_elements = 0, max_union_tiles = 0, sblock_macs = 0;
     double *T = nullptr, *S = nullptr, *Lpanel = nullptr;
     // FP32 operands of the mixed-precision H*psi, made on first use
-    float *T32 = nullptr, *S32 = nullptr, *Lpanel32 = nullptr, *R32 = nullptr, *psi32 = nullptr;
+    float *T32 = nullptr, *S32 = nullptr, *S32b = nullptr, *Lpanel32 = nullptr, *R32 = nullptr, *psi32 = nullptr;
     // captured H*psi per precision (FP64, FP32) on plan-owned psi buffers
     struct Graph {
         hipGraphExec_t exec = nullptr;
@@ -568,6 +579,10 @@ int mq_ctx_destroy(mq_ctx_t ctx) {
             cache_release_locked(c, ctx->device);
         }
         if (ctx->blas) (void)rocblas_destroy_handle(ctx->blas);
+        if (ctx->stream2) {
+            (void)hipStreamSynchronize(ctx->stream2);
+            (void)hipStreamDestroy(ctx->stream2);
+        }
         (void)hipStreamDestroy(ctx->stream);
     });
     delete ctx;
@@ -597,6 +612,9 @@ int mq_ctx_set_option(mq_ctx_t ctx, int option, int value) {
             ctx->sblock_xcd = value != 0;
         } else if (option == MQ_OPT_AUTOTUNE) {
             ctx->autotune = value != 0;
+        } else if (option == MQ_OPT_OVERLAP) {
+            require(value == 0 || value == 1, "mq_ctx_set_option: MQ_OPT_OVERLAP is 0 or 1");
+            ctx->overlap = value == 1;
         } else if (option == MQ_OPT_JCD_GMRES) {
             require(value >= 0 && value <= 64, "mq_ctx_set_option: MQ_OPT_JCD_GMRES is 0..64");
             ctx->jcd_gmres = value;
@@ -969,7 +987,9 @@ static void autotune_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<std::vect
                 GemmSchedule sch = build_gemm_schedule(descs, gemm_policy(pick));
                 (void)hipStreamSynchronize(s);
                 stage.release();
-                stage.upload(descs, rows, s, &sch, sch.ws_elems <= p->gemm_ws_elems ? p->gemm_ws : nullptr);
+                double* ws = kind == 1 && p->S2 ? p->gemm_ws2 : p->gemm_ws;
+                const uint64_t ws_n = kind == 1 && p->S2 ? p->gemm_ws2_elems : p->gemm_ws_elems;
+                stage.upload(descs, rows, s, &sch, sch.ws_elems <= ws_n ? ws : nullptr);
             }
         p->n_reduces = 0;
         for (auto const& pc : p->chunks) p->n_reduces += pc.tg.n_reduces + pc.cg.n_reduces;
@@ -1028,7 +1048,7 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
     std::vector<GemmSchedule> tsch(nch), csch(nch);
     std::vector<SBlockTable> sbs(nch);
     double t_blocks = 0;
-    uint64_t ws_max = 0;
+    uint64_t ws_max = 0, ws_c_max = 0;
     for (size_t c = 0; c < nch; ++c) {
         HpsiChunk const& ch = H.chunks[c];
         sizes[BASE_T] = ch.t_elems;
@@ -1049,6 +1069,7 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
         tsch[c] = build_gemm_schedule(tgs[c]);
         csch[c] = build_gemm_schedule(cgs[c]);
         ws_max = std::max({ws_max, tsch[c].ws_elems, csch[c].ws_elems});
+        ws_c_max = std::max(ws_c_max, csch[c].ws_elems);
     }
     p->seconds_blocks = t_blocks;
     p->seconds_tables = seconds_since(t0) - t_blocks;
@@ -1063,11 +1084,35 @@ static void finish_plan(mq_ctx_t ctx, mq_hpsi_plan_s* p, std::vector<LpanelPart>
     p->Lpanel = dev_alloc<double>(H.lpanel_elems);
     p->gemm_ws = dev_alloc<double>(ws_max);
     p->gemm_ws_elems = ws_max;
+    if (ctx->overlap && nch > 1) {
+        // the second S buffer, workspace and stream; a device without room for them runs the
+        // chunks one after the other
+        try {
+            p->S2 = dev_alloc<double>(H.s_buf);
+            p->gemm_ws2 = dev_alloc<double>(std::max<uint64_t>(ws_c_max, 1));
+            p->gemm_ws2_elems = ws_c_max;
+        } catch (Error const&) {
+            dev_free(p->S2);
+            dev_free(p->gemm_ws2);
+            p->S2 = p->gemm_ws2 = nullptr;
+            p->gemm_ws2_elems = 0;
+        }
+        if (p->S2) {
+            if (!ctx->stream2) hipcheck(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking), "stream create");
+            p->ev_sa.assign(nch, nullptr);
+            p->ev_cg.assign(nch, nullptr);
+            hipcheck(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming), "event");
+            for (size_t c = 0; c < nch; ++c) {
+                hipcheck(hipEventCreateWithFlags(&p->ev_sa[c], hipEventDisableTiming), "event");
+                hipcheck(hipEventCreateWithFlags(&p->ev_cg[c], hipEventDisableTiming), "event");
+            }
+        }
+    }
     p->chunks.resize(nch);
     for (size_t c = 0; c < nch; ++c) {
         auto& pc = p->chunks[c];
         pc.tg.upload(tgs[c], trs[c], ctx->stream, &tsch[c], p->gemm_ws);
-        pc.cg.upload(cgs[c], {}, ctx->stream, &csch[c], p->gemm_ws);
+        pc.cg.upload(cgs[c], {}, ctx->stream, &csch[c], p->S2 ? p->gemm_ws2 : p->gemm_ws);
         pc.sa.upload(sbs[c], ctx->stream, ctx->sblock_xcd);
         p->n_sblocks += int64_t(sbs[c].blocks.size());
         p->union_elements += sbs[c].gathered;
@@ -1356,6 +1401,7 @@ static void ensure_fp32(mq_hpsi_plan_t p) {
     // FP32 T and S live in the FP64 buffers: the H*psi of both precisions run on one stream, one
     // after the other
     p->S32 = reinterpret_cast<float*>(p->S);
+    p->S32b = reinterpret_cast<float*>(p->S2);
     p->Lpanel32 = dev_alloc<float>(H.lpanel_elems);
     p->R32 = dev_alloc<float>(p->r_elems);
     p->psi32 = dev_alloc<float>(p->psi_n);
@@ -1401,6 +1447,38 @@ static void library_gemms(mq_ctx_t ctx, const GemmDesc* first, const GemmDesc* l
     }
 }
 
+// The chunks of an H*psi with their output GEMMs on the second stream: chunk c's term sum writes
+// the S buffer of parity c % 2 once chunk c - 2's output GEMMs (on the second stream) have read
+// it; chunk c's output GEMMs start when its term sum is done.  The T buffer is written and read on
+// the context stream only.  Both streams start after the work already on the context stream, and
+// the context stream waits for the last output GEMMs, so callers (and graph capture) see one
+// stream.
+static void hpsi_overlapped(mq_hpsi_plan_t p, Bases b, bool fp32) {
+    hipStream_t s = p->ctx->stream, s2 = p->ctx->stream2;
+    const size_t n = p->chunks.size();
+    hipcheck(hipEventRecord(p->ev_fork, s), "event");
+    hipcheck(hipStreamWaitEvent(s2, p->ev_fork, 0), "wait");
+    for (size_t c = 0; c < n; ++c) {
+        auto& ch = p->chunks[c];
+        const bool odd = c % 2 == 1;
+        if (c >= 2) hipcheck(hipStreamWaitEvent(s, p->ev_cg[c - 2], 0), "wait");
+        if (fp32) {
+            b.p[BASE_S] = odd ? static_cast<void*>(p->S32b) : static_cast<void*>(p->S32);
+            ch.tg.run(b, s, GEMM_F32);
+            ch.sa.run32(p->T32, odd ? p->S32b : p->S32, s);
+        } else {
+            b.p[BASE_S] = odd ? p->S2 : p->S;
+            ch.tg.run(b, s, GEMM_F64);
+            ch.sa.run(p->T, odd ? p->S2 : p->S, s);
+        }
+        hipcheck(hipEventRecord(p->ev_sa[c], s), "event");
+        hipcheck(hipStreamWaitEvent(s2, p->ev_sa[c], 0), "wait");
+        ch.cg.run(b, s2, fp32 ? GEMM_F32_OUT_F64 : GEMM_F64);
+        hipcheck(hipEventRecord(p->ev_cg[c], s2), "event");
+    }
+    hipcheck(hipStreamWaitEvent(s, p->ev_cg[n - 1], 0), "wait");
+}
+
 // ev (optional): 2 + 3 * chunks events: start, set-up done, then per chunk its T-GEMMs, term sum
 // and output GEMMs done
 static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int precision, hipEvent_t* ev = nullptr) {
@@ -1418,6 +1496,10 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         b.p[BASE_BOUNDARY] = const_cast<double*>(p->rdata);
         mark(0);
         mark(1);
+        if (!ev && p->S2) {
+            hpsi_overlapped(p, b, false);
+            return;
+        }
         for (size_t c = 0; c < p->chunks.size(); ++c) {
             p->chunks[c].tg.run(b, s, GEMM_F64);
             mark(int(2 + 3 * c));
@@ -1436,6 +1518,10 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         mark(0);
         hipcheck(launch_convert(in, p->psi32, p->psi_n, s), "convert");
         mark(1);
+        if (!ev && p->S2) {
+            hpsi_overlapped(p, b, true);
+            return;
+        }
         for (size_t c = 0; c < p->chunks.size(); ++c) {
             p->chunks[c].tg.run(b, s, GEMM_F32);
             mark(int(2 + 3 * c));
@@ -1765,6 +1851,13 @@ int mq_hpsi_plan_destroy(mq_hpsi_plan_t p) {
             c.sa.release();
         }
         dev_free(p->gemm_ws);
+        dev_free(p->gemm_ws2);
+        dev_free(p->S2);
+        if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+        for (auto e : p->ev_sa)
+            if (e) (void)hipEventDestroy(e);
+        for (auto e : p->ev_cg)
+            if (e) (void)hipEventDestroy(e);
         dev_free(p->T);
         dev_free(p->S);
         dev_free(p->Lpanel);

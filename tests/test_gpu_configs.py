@@ -3,8 +3,9 @@
 * configuration 2 (24 orbitals, 24 electrons, TwoU1, M = 512, integrals PCG64(1002) rank 16):
   after one warm sweep, the middle two-site H*psi (FP64 to 1e-12, FP32 to 1e-5 relative), the same
   with the plan's output columns in chunks (MQ_OPT_CHUNK_MB 256: more than four chunks run on
-  buffers of less than half of T + S, also as a captured graph and with the reference-style
-  tables) and the left and right growth
+  buffers of less than half of T + S, also as a captured graph replayed several times and with the
+  reference-style tables; the chunks overlapped on two streams equal the chunks in turn bit for
+  bit) and the left and right growth
   steps of the accepted tensors (1e-12).  The plan statistics show that the
   paths this problem exercises ran: term-sum blocks with more than 64 union tiles (several union
   chunks per block) and split-K GEMM reductions (K up to ~9e4);
@@ -62,11 +63,27 @@ def test_config2_site_problem_vs_oracle(ctx2u1, oracle_lib):
         ctx2u1.set_option(abi.MQ_OPT_CHUNK_MB, abi.MQ_CHUNK_AUTO)
         ctx2u1.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
     cst = cplan.stats()
-    errc = rel(cplan.apply(x), ref)
-    errc32 = rel(cplan.apply(x, precision=abi.MQ_PREC_FP32), ref)
+    yc = cplan.apply(x)
+    yc32 = cplan.apply(x, precision=abi.MQ_PREC_FP32)
+    errc, errc32 = rel(yc, ref), rel(yc32, ref)
     errp = rel(cplan.apply(x, precision=abi.MQ_PREC_FP64_PLAIN), ref)
+    # the chunks overlap on two streams (MQ_OPT_OVERLAP, default): the same result as one chunk after
+    # the other, bit for bit
+    ctx2u1.set_option(abi.MQ_OPT_CHUNK_MB, 256)
+    ctx2u1.set_option(abi.MQ_OPT_OVERLAP, 0)
+    try:
+        splan = HpsiPlan(ctx2u1, mpo, Ld, Rd, lay2)
+    finally:
+        ctx2u1.set_option(abi.MQ_OPT_CHUNK_MB, abi.MQ_CHUNK_AUTO)
+        ctx2u1.set_option(abi.MQ_OPT_OVERLAP, 1)
+    assert np.array_equal(splan.apply(x), yc) and np.array_equal(splan.apply(x, precision=abi.MQ_PREC_FP32), yc32)
+    splan.close()
     cplan.use_graphs(True)
     errg = rel(cplan.apply(x), ref)
+    # graph replays, with eager work of the other precision between them, equal the eager result
+    for _ in range(3):
+        assert np.array_equal(cplan.apply(x), yc)
+        assert np.array_equal(cplan.apply(x, precision=abi.MQ_PREC_FP32), yc32)
     print(f"  chunked plan: {cst['n_chunks']} chunks, T+S buffers {cst['ts_buffer_elements']} of "
           f"{cst['t_elements'] + cst['s_elements']} elements; H*psi FP64 rel err {errc:.2e}, FP32 {errc32:.2e}, "
           f"graph {errg:.2e}, reference-style {errp:.2e}", flush=True)
