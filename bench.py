@@ -597,7 +597,7 @@ class ProfilerRegion:
         return False
 
 
-def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4, extra_sweeps=4):
+def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4, extra_sweeps=4, cpu_sweep=False):
     """Three two-site DMRG runs from one common start (a seeded random MPS after warm_sweeps FP64
     sweeps), Davidson 8 iterations and tol 1e-8 per step: FP64 with the grouped launches and mixed
     precision (FP32 contraction, FP64 refinement of every eigenvector), n_sweeps + extra_sweeps
@@ -607,7 +607,10 @@ def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4, extra_
     in different local minima of the sweep.  Reported per run: the wall time and final energy of
     every sweep; against the FP64 run, at sweep n_sweeps (all three runs) and at the last sweep
     (FP64 and mixed): |dE| of the final energy, the largest difference in truncated weight over
-    the sweep's steps, and the final Davidson residual; speed-ups at sweep n_sweeps."""
+    the sweep's steps, and the final Davidson residual; speed-ups at sweep n_sweeps.  cpu_sweep: the
+    CPU time of the FP64 run's last sweep, estimated as the H*psi flops of its Davidson iterations
+    (the reference's flop model) over the CPU baseline's rate on the middle site problem after that
+    sweep (the oracle on the host cores, cpu_baseline)."""
     from qcmaquis_amd import abi
     from qcmaquis_amd.dmrg import TwoSiteDMRG
     from qcmaquis_amd.qc import QCHamiltonian, make_integrals
@@ -629,7 +632,9 @@ def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4, extra_
         rec = {"sweep_seconds": [], "sweep_energies": [], "truncated_weights": [], "residuals": []}
         for s in range(count):
             n0 = len(d.trunc)
+            f0 = d.hpsi_flops
             e, dt = d.sweep()
+            rec.setdefault("hpsi_flops", []).append(d.hpsi_flops - f0)
             rec["sweep_seconds"].append(round(dt, 3))
             rec["sweep_energies"].append(e)
             rec["truncated_weights"].append(d.trunc[n0:])
@@ -637,6 +642,12 @@ def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4, extra_
         rec["final_residual_norm"] = d.last_residual
         rec["timing_s"] = {k: round(v, 3) for k, v in d.timing.items()}
         runs[name] = rec
+        if name == "fp64" and cpu_sweep:
+            try:
+                cpu_est = cpu_sweep_estimate(ctx, d, ham, rec["hpsi_flops"][-1], rec["sweep_seconds"][-1])
+            except Exception as e:  # the GPU measurement stands on its own
+                log("cpu sweep estimate failed:", repr(e))
+                cpu_est = None
         log("sweeps %s: %s s, E %s" % (name, rec["sweep_seconds"], ["%.10f" % x for x in rec["sweep_energies"]]))
         del d
     ctx.set_option(abi.MQ_OPT_PLAIN_TABLES, 0)
@@ -663,6 +674,7 @@ def sweep_compare(ctx, L=24, M=512, seed=1002, warm_sweeps=2, n_sweeps=4, extra_
            "fp64_energy_change_in_last_sweep": fp64_change,
            "fp64_speedup_vs_fp64_plain": round(runs["fp64_plain"]["sweep_seconds"][k4] / ref["sweep_seconds"][k4], 3),
            "common_start": {"fp64_sweeps": warm_sweeps, "energies": warm_e, "seconds": round(warm_s, 3)},
+           "cpu_fp64_sweep": cpu_est if cpu_sweep else None,
            "note": f"{L} orbitals, M={M}; from one common start (seeded random MPS after {warm_sweeps} FP64 sweeps) "
                    f"{n_sweeps + extra_sweeps} sweeps of FP64 and mixed and {n_sweeps} of the reference-style FP64, "
                    "Davidson 8 iterations / tol 1e-8 per two-site step; every run is swept for real (its own "
@@ -705,6 +717,33 @@ def davidson_compare(plan, n, seed=3):
                            rec["fp64"]["seconds"], rec["mixed"]["energy"], rec["mixed"]["iterations"],
                            rec["mixed"]["refine_iterations"], rec["mixed"]["residual_norm"], rec["mixed"]["seconds"],
                            rec["abs_delta_e_mixed_vs_fp64"]))
+    return out
+
+
+def cpu_sweep_estimate(ctx, d, ham, sweep_flops, gpu_sweep_seconds):
+    """CPU time of one configuration-2 FP64 sweep: the sweep's H*psi flops over the CPU baseline's
+    rate on the middle two-site problem of the swept state (H*psi only: schedules, growth and the
+    split are not counted, so the estimate is low)"""
+    from qcmaquis_amd import abi
+    from qcmaquis_amd.engine import HpsiPlan
+    mid = d.L // 2 - 1
+    for k in range(mid):
+        d.step(k, True)
+    mpo, Ld, Rd, lay2, _ = d.site_problem(mid)
+    plan = HpsiPlan(ctx, mpo, Ld, Rd, lay2)
+    flops = plan.stats()["flops_total"]
+    plan.close()
+    w = {"L": Ld, "R": Rd, "layout": lay2, "sym": getattr(ham, "sym", abi.MQ_SYM_TWOU1),
+         "mpo_spec": ham.mpo(mid, True).to_spec()}
+    cpu = cpu_baseline(w, flops, budget_s=16.0)
+    sec = sweep_flops / (cpu["value"] * 1e12)
+    out = {"seconds_per_sweep_estimated": round(sec, 2), "sweep_hpsi_flops": int(sweep_flops),
+           "rate_tflops": cpu["value"], "cores": cpu["cores"], "sample": cpu["sample"],
+           "gpu_fp64_sweep_seconds": gpu_sweep_seconds, "gpu_speedup": round(sec / gpu_sweep_seconds, 1),
+           "note": "the last FP64 sweep's Davidson H*psi flops (reference model) at the oracle's rate on the "
+                   "middle two-site problem (%d flops per H*psi), H*psi only" % flops}
+    log("cpu sweep estimate (configuration 2, H*psi only): %.1f s per sweep at %.4f TFLOP/s (GPU sweep %.2f s)"
+        % (sec, cpu["value"], gpu_sweep_seconds))
     return out
 
 
@@ -1046,7 +1085,7 @@ def main():
         # real sweeps run at configuration 2 (24 orbitals, M = 512): a 40-orbital M = 2048 sweep takes
         # tens of minutes
         ctx2 = Context(0, abi.MQ_SYM_TWOU1) if args.workload == "qc3" else ctx
-        line["config"]["sweeps_configuration_2"] = sweep_compare(ctx2)
+        line["config"]["sweeps_configuration_2"] = sweep_compare(ctx2, cpu_sweep=not args.no_cpu)
         if ctx2 is not ctx:
             ctx2.close()
     if not args.no_cpu:
