@@ -735,7 +735,7 @@ def cpu_sweep_estimate(ctx, d, ham, sweep_flops, gpu_sweep_seconds):
     plan.close()
     w = {"L": Ld, "R": Rd, "layout": lay2, "sym": getattr(ham, "sym", abi.MQ_SYM_TWOU1),
          "mpo_spec": ham.mpo(mid, True).to_spec()}
-    cpu = cpu_baseline(w, flops, budget_s=16.0)
+    cpu = cpu_baseline(w, flops, budget_s=16.0, what="the middle two-site problem of configuration 2")
     sec = sweep_flops / (cpu["value"] * 1e12)
     out = {"seconds_per_sweep_estimated": round(sec, 2), "sweep_hpsi_flops": int(sweep_flops),
            "rate_tflops": cpu["value"], "cores": cpu["cores"], "sample": cpu["sample"],
@@ -747,7 +747,7 @@ def cpu_sweep_estimate(ctx, d, ham, sweep_flops, gpu_sweep_seconds):
     return out
 
 
-def cpu_baseline(w, flops, budget_s=30.0, rate_guess_tflops=0.06):
+def cpu_baseline(w, flops, budget_s=30.0, rate_guess_tflops=0.06, what="the benchmark site problem"):
     """the oracle (restated reference CPU path) on the same site problem, all host cores, in two
     threadings: OpenMP over the MPS blocks with one BLAS thread each, and one block at a time with
     every core in BLAS; the better one is the baseline.  When the whole H*psi would take more than
@@ -782,7 +782,7 @@ def cpu_baseline(w, flops, budget_s=30.0, rate_guess_tflops=0.06):
         best = max(variants, key=lambda k: variants[k]["tflops"])
         v = variants[best]
         return {"value": v["tflops"], "unit": "TFLOP/s", "cores": cores, "kind": "port",
-                "sample": f"{v['hpsi']} full H*psi of the benchmark site problem ({v['seconds_per_hpsi']:.3f} s each, "
+                "sample": f"{v['hpsi']} full H*psi of {what} ({v['seconds_per_hpsi']:.3f} s each, "
                           f"schedule build excluded; {best}: {v['openmp_threads']} OpenMP threads over MPS blocks, "
                           f"{v['blas_threads']} {blas_name} threads)",
                 "seconds_per_hpsi": v["seconds_per_hpsi"], "threadings": variants}
