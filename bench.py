@@ -951,6 +951,25 @@ def main():
     stage32 /= nrep
     ms32 = wall32 / args.steps * 1e3
     err32 = float((out32 - ref).norm() / ref.norm())
+    # split-BF16 GEMM products (MQ_PREC_BF16X3): the FP32 path with hi hi + hi lo + lo hi products on
+    # the BF16 matrix cores (eager launches)
+    out3 = torch.empty_like(psi)
+    for _ in range(args.warmup):
+        plan.apply_device(psi.data_ptr(), out3.data_ptr(), abi.MQ_PREC_BF16X3)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.apply_device(psi.data_ptr(), out3.data_ptr(), abi.MQ_PREC_BF16X3)
+    ctx.sync()
+    ms3 = (time.perf_counter() - t0) / args.steps * 1e3
+    stage3 = np.zeros(4)
+    for _ in range(nrep):
+        ctx.check(ctx.lib.mq_hpsi_timed(plan.h, C.c_void_p(psi.data_ptr()), C.c_void_p(out3.data_ptr()),
+                                        abi.MQ_PREC_BF16X3, sms))
+        stage3 += np.array(list(sms))
+    stage3 /= nrep
+    err3 = float((out3 - ref).norm() / ref.norm())
+    log("BF16x3 H*psi: %.3f ms, stages %s, relative error vs FP64 %.2e" % (ms3, np.round(stage3, 4).tolist(), err3))
     # the same H*psi as captured HIP graphs (psi copied into and out of plan-owned buffers)
     plan.use_graphs(True)
     graph_ms = {}
@@ -1070,6 +1089,12 @@ def main():
                                    "tflops": round(st["flops_total"] / (ms32 * 1e-3) / 1e12, 4),
                                    "speedup_vs_fp64": round(ms / ms32, 3), "rel_error_vs_fp64": err32,
                                    "stage_ms": [round(x, 4) for x in stage32]},
+                       bf16x3={"ms_per_step": round(ms3, 4),
+                               "tflops": round(st["flops_total"] / (ms3 * 1e-3) / 1e12, 4),
+                               "speedup_vs_fp64": round(ms / ms3, 3), "rel_error_vs_fp64": err3,
+                               "stage_ms": [round(x, 4) for x in stage3],
+                               "note": "FP32 operands and term sum; GEMM products split into BF16 parts "
+                                       "(hi hi + hi lo + lo hi) on the BF16 matrix cores, FP32 accumulation"},
                        fp64_plain=plain, davidson_precision=dav),
         "roofline": dominant,
         "roofline_other": other,

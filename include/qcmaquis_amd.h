@@ -44,8 +44,11 @@ extern "C" {
 /* device arithmetic of the sector GEMMs */
 #define MQ_PREC_FP64   0
 #define MQ_PREC_FP32   1
-#define MQ_PREC_BF16X3 2         /* reserved (split-BF16 products): not built; H*psi returns
-                                    MQ_ERR_UNSUPPORTED */
+#define MQ_PREC_BF16X3 2         /* H*psi (mq_hpsi, mq_hpsi_timed) as MQ_PREC_FP32, with the GEMM
+                                    stages' products split into BF16 parts on the BF16 matrix cores
+                                    (x = hi + lo, |x - hi - lo| <= 2^-16 |x|; hi hi + hi lo + lo hi,
+                                    FP32 accumulation); eager launches only (no graph); not a
+                                    Davidson precision */
 /* FP64 with the reference's launch structure, for comparison with the grouped path: one library
  * (rocBLAS) dgemm per T task, one S-assembly workgroup per S tile that gathers every term, and
  * one dgemm per cohort added into its output block (reference: MPSBlock::create_T_gpu,

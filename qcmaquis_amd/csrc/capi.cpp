@@ -1453,7 +1453,7 @@ static void library_gemms(mq_ctx_t ctx, const GemmDesc* first, const GemmDesc* l
 // the context stream only.  Both streams start after the work already on the context stream, and
 // the context stream waits for the last output GEMMs, so callers (and graph capture) see one
 // stream.
-static void hpsi_overlapped(mq_hpsi_plan_t p, Bases b, bool fp32) {
+static void hpsi_overlapped(mq_hpsi_plan_t p, Bases b, bool fp32, bool bf3) {
     hipStream_t s = p->ctx->stream, s2 = p->ctx->stream2;
     const size_t n = p->chunks.size();
     hipcheck(hipEventRecord(p->ev_fork, s), "event");
@@ -1464,7 +1464,7 @@ static void hpsi_overlapped(mq_hpsi_plan_t p, Bases b, bool fp32) {
         if (c >= 2) hipcheck(hipStreamWaitEvent(s, p->ev_cg[c - 2], 0), "wait");
         if (fp32) {
             b.p[BASE_S] = odd ? static_cast<void*>(p->S32b) : static_cast<void*>(p->S32);
-            ch.tg.run(b, s, GEMM_F32);
+            ch.tg.run(b, s, bf3 ? GEMM_BF3 : GEMM_F32);
             ch.sa.run32(p->T32, odd ? p->S32b : p->S32, s);
         } else {
             b.p[BASE_S] = odd ? p->S2 : p->S;
@@ -1473,7 +1473,7 @@ static void hpsi_overlapped(mq_hpsi_plan_t p, Bases b, bool fp32) {
         }
         hipcheck(hipEventRecord(p->ev_sa[c], s), "event");
         hipcheck(hipStreamWaitEvent(s2, p->ev_sa[c], 0), "wait");
-        ch.cg.run(b, s2, fp32 ? GEMM_F32_OUT_F64 : GEMM_F64);
+        ch.cg.run(b, s2, fp32 ? (bf3 ? GEMM_BF3_OUT_F64 : GEMM_F32_OUT_F64) : GEMM_F64);
         hipcheck(hipEventRecord(p->ev_cg[c], s2), "event");
     }
     hipcheck(hipStreamWaitEvent(s, p->ev_cg[n - 1], 0), "wait");
@@ -1497,7 +1497,7 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         mark(0);
         mark(1);
         if (!ev && p->S2) {
-            hpsi_overlapped(p, b, false);
+            hpsi_overlapped(p, b, false, false);
             return;
         }
         for (size_t c = 0; c < p->chunks.size(); ++c) {
@@ -1508,7 +1508,9 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
             p->chunks[c].cg.run(b, s, GEMM_F64);
             mark(int(4 + 3 * c));
         }
-    } else if (precision == MQ_PREC_FP32) {
+    } else if (precision == MQ_PREC_FP32 || precision == MQ_PREC_BF16X3) {
+        // BF16X3: the FP32 operands and term sum, with split-BF16 products in the GEMM stages
+        const bool bf3 = precision == MQ_PREC_BF16X3;
         ensure_fp32(p);
         b.p[BASE_PSI_IN] = p->psi32;
         b.p[BASE_T] = p->T32;
@@ -1519,15 +1521,15 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         hipcheck(launch_convert(in, p->psi32, p->psi_n, s), "convert");
         mark(1);
         if (!ev && p->S2) {
-            hpsi_overlapped(p, b, true);
+            hpsi_overlapped(p, b, true, bf3);
             return;
         }
         for (size_t c = 0; c < p->chunks.size(); ++c) {
-            p->chunks[c].tg.run(b, s, GEMM_F32);
+            p->chunks[c].tg.run(b, s, bf3 ? GEMM_BF3 : GEMM_F32);
             mark(int(2 + 3 * c));
             p->chunks[c].sa.run32(p->T32, p->S32, s);
             mark(int(3 + 3 * c));
-            p->chunks[c].cg.run(b, s, GEMM_F32_OUT_F64);
+            p->chunks[c].cg.run(b, s, bf3 ? GEMM_BF3_OUT_F64 : GEMM_F32_OUT_F64);
             mark(int(4 + 3 * c));
         }
     } else if (precision == MQ_PREC_FP64_PLAIN) {
@@ -1554,7 +1556,7 @@ static void hpsi_enqueue(mq_hpsi_plan_t p, const double* in, double* out, int pr
         }
     } else {
         throw Error(MQ_ERR_UNSUPPORTED,
-                    "H*psi precision: MQ_PREC_FP64, MQ_PREC_FP32 and MQ_PREC_FP64_PLAIN are available");
+                    "H*psi precision: MQ_PREC_FP64, MQ_PREC_FP32, MQ_PREC_BF16X3 and MQ_PREC_FP64_PLAIN are available");
     }
 }
 
@@ -1592,7 +1594,7 @@ static void hpsi_graph_enqueue(mq_hpsi_plan_t p, const double* in, double* out, 
 }
 
 static void hpsi_run(mq_hpsi_plan_t p, const double* in, double* out, int precision) {
-    if (p->use_graphs && precision != MQ_PREC_FP64_PLAIN) hpsi_graph_enqueue(p, in, out, precision);
+    if (p->use_graphs && (precision == MQ_PREC_FP64 || precision == MQ_PREC_FP32)) hpsi_graph_enqueue(p, in, out, precision);
     else hpsi_enqueue(p, in, out, precision);
 }
 

@@ -13,6 +13,8 @@
 //  * FP64 Davidson vector kernels with device-resident coefficients.
 #include "device.hpp"
 
+#include <type_traits>
+
 namespace mq {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
@@ -44,11 +46,25 @@ template <> struct Mma<float> {
 
 template <class T> __device__ inline T ld_or_zero(const T* p, bool ok) { return ok ? *p : T(0); }
 
+// Split-BF16 products (precision MQ_PREC_BF16X3): an FP32 operand x = hi + lo with hi its leading
+// 16 bits (a BF16 value, cut) and lo the rest rounded to nearest BF16, so |x - hi - lo| <= 2^-16 |x|;
+// a product is hi_a hi_b + hi_a lo_b + lo_a hi_b on the BF16 matrix cores with FP32 accumulation
+// (lo_a lo_b, at most 2^-14 of it, is left out).  The 16x16x16 step: lane l supplies four
+// consecutive k of A[l % 16][4 (l / 16) + q] and B[4 (l / 16) + q][l % 16]; the accumulator
+// layout is the one of the FP32 16x16x4 step.
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+__device__ inline void split_bf16(float x, short& hi, short& lo) {
+    const uint32_t u = __float_as_uint(x);
+    hi = short(u >> 16);
+    const uint32_t r = __float_as_uint(x - __uint_as_float(u & 0xffff0000u));
+    lo = short((r + 0x7fffu + ((r >> 16) & 1u)) >> 16);
+}
+
 // One BM x BN output tile (BM = 32 FM, BN = 16 FN WN) per workgroup of 2 WN waves; the waves form a
 // 2 x WN grid of 16 FM x 16 FN wave tiles, each FM x FN matrix-core tiles.  Operands of type TI,
 // computed in TI, output converted to TO.  The tile reduces over [k0, k1) and writes C, or its
 // partial tile (type TI) into the workspace (split K).
-template <int FM, int FN, class TI, class TO, int KS = BK, int MINB = 1, int WN = 2>
+template <int FM, int FN, class TI, class TO, int KS = BK, int MINB = 1, int WN = 2, bool BF3 = false>
 __global__ __launch_bounds__(128 * WN, MINB) void gemm_kernel(const GemmDesc* __restrict__ probs,
                                                               const GemmRow* __restrict__ rows,
                                                               const GemmTile* __restrict__ tiles, Bases bases,
@@ -57,6 +73,7 @@ __global__ __launch_bounds__(128 * WN, MINB) void gemm_kernel(const GemmDesc* __
     constexpr int BM = 32 * FM, BN = 16 * FN * WN;
     constexpr int NA = BM * KS / THREADS, NB = BN * KS / THREADS;  // operand elements staged per thread and K slice
     static_assert(KS % 4 == 0 && NA * THREADS == BM * KS && NB * THREADS == BN * KS, "K slice geometry");
+    static_assert(!BF3 || (std::is_same<TI, float>::value && KS % 16 == 0), "split-BF16 products: FP32 operands, K slices of 16");
     __shared__ TI As[KS][BM + LDS_PAD];
     __shared__ TI Bs[KS][BN + LDS_PAD];
     typedef typename Mma<TI>::acc_t acc_t;
@@ -140,6 +157,40 @@ __global__ __launch_bounds__(128 * WN, MINB) void gemm_kernel(const GemmDesc* __
         for (int i = 0; i < NB; ++i) Bs[b_k[i]][b_n[i]] = rb[i];
         __syncthreads();
         if (k0 + KS < k_end) load(k0 + KS);
+        if constexpr (BF3) {
+#pragma unroll
+            for (int kk = 0; kk < KS; kk += 16) {
+                bf16x4_t ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        short h, l;
+                        split_bf16(As[kk + 4 * fk + q][wm + 16 * i + fr], h, l);
+                        ah[i][q] = h;
+                        al[i][q] = l;
+                    }
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        short h, l;
+                        split_bf16(Bs[kk + 4 * fk + q][wn + 16 * j + fr], h, l);
+                        bh[j][q] = h;
+                        bl[j][q] = l;
+                    }
+                // operands swapped as below: the instruction accumulates C^T = B^T A^T
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bh[j], ah[i], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bh[j], al[i], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bl[j], ah[i], acc[i][j], 0, 0, 0);
+                    }
+            }
+            continue;
+        }
 #pragma unroll
         for (int kk = 0; kk < KS; kk += 4) {
             TI af[FM], bf[FN];
@@ -568,30 +619,31 @@ namespace {
 // shape on the H*psi stages of a 24-orbital, M = 512 site problem (short and long reductions alike)
 constexpr int GEMM_KS = 8, GEMM_MINB = 4;
 
-template <class TI, class TO>
+template <class TI, class TO, bool BF3 = false>
 void launch_gemm_t(const GemmDesc* probs, const GemmRow* rows, const GemmTile* const* tiles, const int64_t* n_tiles,
                    const GemmReduce* reduces, int64_t n_reduces, void* ws, Bases bases, hipStream_t stream) {
     TI* w = static_cast<TI*>(ws);
+    constexpr int KSL = BF3 ? 16 : GEMM_KS;  // split-BF16 steps take 16 k at a time
     for (int sh = 0; sh < GEMM_N_SHAPES; ++sh) {
         for (int64_t off = 0; off < n_tiles[sh]; off += 0x7fffffff) {
             int64_t cnt = n_tiles[sh] - off < 0x7fffffff ? n_tiles[sh] - off : 0x7fffffff;
             if (sh == GEMM_32x128)
-                hipLaunchKernelGGL((gemm_kernel<1, 4, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                hipLaunchKernelGGL((gemm_kernel<1, 4, TI, TO, KSL, GEMM_MINB, 2, BF3>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    rows, tiles[sh] + off, bases, w);
             else if (sh == GEMM_64x64)
-                hipLaunchKernelGGL((gemm_kernel<2, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                hipLaunchKernelGGL((gemm_kernel<2, 2, TI, TO, KSL, GEMM_MINB, 2, BF3>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    rows, tiles[sh] + off, bases, w);
             else if (sh == GEMM_96x64)
-                hipLaunchKernelGGL((gemm_kernel<3, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                hipLaunchKernelGGL((gemm_kernel<3, 2, TI, TO, KSL, GEMM_MINB, 2, BF3>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    rows, tiles[sh] + off, bases, w);
             else if (sh == GEMM_64x128)
-                hipLaunchKernelGGL((gemm_kernel<2, 4, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                hipLaunchKernelGGL((gemm_kernel<2, 4, TI, TO, KSL, GEMM_MINB, 2, BF3>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    rows, tiles[sh] + off, bases, w);
             else if (sh == GEMM_128x128)
-                hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO, GEMM_KS, GEMM_MINB / 2, 4>), dim3(unsigned(cnt)), dim3(512), 0,
+                hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO, KSL, GEMM_MINB / 2, 4, BF3>), dim3(unsigned(cnt)), dim3(512), 0,
                                    stream, probs, rows, tiles[sh] + off, bases, w);
             else
-                hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO, GEMM_KS, GEMM_MINB>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
+                hipLaunchKernelGGL((gemm_kernel<4, 2, TI, TO, KSL, GEMM_MINB, 2, BF3>), dim3(unsigned(cnt)), dim3(256), 0, stream, probs,
                                    rows, tiles[sh] + off, bases, w);
         }
     }
@@ -616,6 +668,12 @@ hipError_t launch_gemm(GemmPrec prec, const GemmDesc* probs, const GemmRow* rows
             break;
         case GEMM_F32_OUT_F64:
             launch_gemm_t<float, double>(probs, rows, tiles, n_tiles, reduces, n_reduces, ws, bases, stream);
+            break;
+        case GEMM_BF3:
+            launch_gemm_t<float, float, true>(probs, rows, tiles, n_tiles, reduces, n_reduces, ws, bases, stream);
+            break;
+        case GEMM_BF3_OUT_F64:
+            launch_gemm_t<float, double, true>(probs, rows, tiles, n_tiles, reduces, n_reduces, ws, bases, stream);
             break;
         default: return hipErrorInvalidValue;
     }

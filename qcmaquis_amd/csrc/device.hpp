@@ -18,7 +18,8 @@ struct Bases {
 
 // arithmetic of a grouped GEMM launch: operands, compute and workspace in FP64 or FP32; the
 // output in the same type, or FP64 from FP32 operands
-enum GemmPrec { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_F32_OUT_F64 = 2 };
+// GEMM_BF3*: FP32 operands, split-BF16 products (hi hi + hi lo + lo hi) on the BF16 matrix cores
+enum GemmPrec { GEMM_F64 = 0, GEMM_F32 = 1, GEMM_F32_OUT_F64 = 2, GEMM_BF3 = 3, GEMM_BF3_OUT_F64 = 4 };
 
 // grouped GEMM stage: tiles of each shape (GemmShape) then the split-K reduce jobs; ws is the
 // workspace of the split partials (GemmSchedule::ws_elems elements of the compute type); rows is

@@ -1,9 +1,9 @@
 """Parity with the CPU oracle on the benchmark configurations' own site problems.
 
 * configuration 2 (24 orbitals, 24 electrons, TwoU1, M = 512, integrals PCG64(1002) rank 16):
-  after one warm sweep, the middle two-site H*psi (FP64 to 1e-12, FP32 to 1e-5 relative), the same
-  with the plan's output columns in chunks (MQ_OPT_CHUNK_MB 256: more than four chunks run on
-  buffers of less than half of T + S, also as a captured graph replayed several times and with the
+  after one warm sweep, the middle two-site H*psi (FP64 to 1e-12, FP32 to 1e-5, BF16x3 split-BF16
+  GEMM products to 5e-4 relative), the same with the plan's output columns in chunks
+  (MQ_OPT_CHUNK_MB 256: more than four chunks run on buffers of less than half of T + S, also as a captured graph replayed several times and with the
   reference-style tables; the chunks overlapped on two streams equal the chunks in turn bit for
   bit) and the left and right growth
   steps of the accepted tensors (1e-12).  The plan statistics show that the
@@ -51,8 +51,11 @@ def test_config2_site_problem_vs_oracle(ctx2u1, oracle_lib):
     ref = oracle_lib.hpsi(abi.MQ_SYM_TWOU1, spec, Ls, Rs, lay2, x)
     err = rel(plan.apply(x), ref)
     err32 = rel(plan.apply(x, precision=abi.MQ_PREC_FP32), ref)
-    print(f"  H*psi FP64 rel err {err:.2e}, FP32 {err32:.2e}", flush=True)
-    assert err < 1e-12 and err32 < 1e-5
+    # split-BF16 GEMM products keep 16 bits of each operand (|x - hi - lo| <= 2^-16 |x|, lo lo left
+    # out): about 1e-4 of the largest element
+    err3 = rel(plan.apply(x, precision=abi.MQ_PREC_BF16X3), ref)
+    print(f"  H*psi FP64 rel err {err:.2e}, FP32 {err32:.2e}, BF16x3 {err3:.2e}", flush=True)
+    assert err < 1e-12 and err32 < 1e-5 and err3 < 5e-4
     plan.close()
     # the same site problem with its output columns in chunks of at most 256 MiB of T + S
     ctx2u1.set_option(abi.MQ_OPT_CHUNK_MB, 256)
@@ -66,6 +69,8 @@ def test_config2_site_problem_vs_oracle(ctx2u1, oracle_lib):
     yc = cplan.apply(x)
     yc32 = cplan.apply(x, precision=abi.MQ_PREC_FP32)
     errc, errc32 = rel(yc, ref), rel(yc32, ref)
+    errc3 = rel(cplan.apply(x, precision=abi.MQ_PREC_BF16X3), ref)  # overlapped chunks, split-BF16 GEMMs
+    assert errc3 < 5e-4
     errp = rel(cplan.apply(x, precision=abi.MQ_PREC_FP64_PLAIN), ref)
     # the chunks overlap on two streams (MQ_OPT_OVERLAP, default): the same result as one chunk after
     # the other, bit for bit
