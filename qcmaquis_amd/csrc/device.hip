@@ -273,12 +273,24 @@ __global__ __launch_bounds__(256) void gemm_reduce_kernel(const GemmDesc* __rest
 // SB_UNROLL k-steps are loaded while the current ones go through the matrix cores.  The k-step's
 // group mask is read into a scalar register, so the groups without a sub-block are skipped by the
 // whole wave.
-constexpr int SB_NC = 2;                                // 16-element column tiles per wave
+#ifndef MQ_SB_NC
+#define MQ_SB_NC 2
+#endif
+#ifndef MQ_SB_UNROLL
+#define MQ_SB_UNROLL 2
+#endif
+#ifndef MQ_SB_KCHUNK
+#define MQ_SB_KCHUNK 8
+#endif
+#ifndef MQ_SB_WAVES_PER_EU
+#define MQ_SB_WAVES_PER_EU 4
+#endif
+constexpr int SB_NC = MQ_SB_NC;                         // 16-element column tiles per wave
 constexpr int SB_XCDS = 8;                              // XCDs of an MI355X
 constexpr int SB_WAVES = SBLOCK_ELEMS / (16 * SB_NC);   // waves per workgroup
 constexpr int SB_THREADS = 64 * SB_WAVES;
-constexpr int SB_UNROLL = 2;                            // k-steps whose loads are issued together
-constexpr int SB_KCHUNK = 8;                            // k-steps staged in LDS at a time
+constexpr int SB_UNROLL = MQ_SB_UNROLL;                 // k-steps whose loads are issued together
+constexpr int SB_KCHUNK = MQ_SB_KCHUNK;                 // k-steps staged in LDS at a time
 constexpr int SB_ST_ALPHA = SB_KCHUNK * SBLOCK_GROUPS * 64 / SB_THREADS;  // staged values per thread
 static_assert(SBLOCK_GROUP == 16 && SB_WAVES * 16 * SB_NC == SBLOCK_ELEMS && SB_WAVES <= 16, "S block geometry");
 static_assert(SB_KCHUNK % (2 * SB_UNROLL) == 0, "staged chunks hold whole double-buffered k-step groups");
@@ -286,7 +298,7 @@ static_assert(4 * SB_KCHUNK <= SB_THREADS && SB_ST_ALPHA * SB_THREADS == SB_KCHU
               "one staging pass covers a chunk's tables");
 
 template <class T>
-__global__ __launch_bounds__(SB_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void sblock_kernel(const SBlock* __restrict__ blocks, const SWork* __restrict__ work,
+__global__ __launch_bounds__(SB_THREADS) __attribute__((amdgpu_waves_per_eu(MQ_SB_WAVES_PER_EU))) void sblock_kernel(const SBlock* __restrict__ blocks, const SWork* __restrict__ work,
                                                      const int64_t* __restrict__ utile, const uint32_t* __restrict__ kstep,
                                                      const T* __restrict__ ualpha, const int64_t* __restrict__ lsoff,
                                                      const int32_t* __restrict__ llds, const T* __restrict__ X,
