@@ -53,12 +53,14 @@ def test_batch_hpsi_matches_instances(ctx2u1):
     psi = rng.normal(size=batch.n)
     got = batch.split(batch.apply(psi))
     got32 = batch.split(batch.apply(psi, abi.MQ_PREC_FP32))
-    for p, x, g, g32 in zip(plans, batch.split(psi), got, got32):
+    got3 = batch.split(batch.apply(psi, abi.MQ_PREC_BF16X3))  # split-BF16 GEMM products
+    for p, x, g, g32, g3 in zip(plans, batch.split(psi), got, got32, got3):
         ref = p.apply(x)
         err = np.linalg.norm(g - ref) / np.linalg.norm(ref)
         err32 = np.linalg.norm(g32 - ref) / np.linalg.norm(ref)
-        print(f"  n={x.size}: fp64 {err:.2e} fp32 {err32:.2e}", flush=True)
-        assert err < 1e-12 and err32 < 1e-5
+        err3 = np.linalg.norm(g3 - ref) / np.linalg.norm(ref)
+        print(f"  n={x.size}: fp64 {err:.2e} fp32 {err32:.2e} bf16x3 {err3:.2e}", flush=True)
+        assert err < 1e-12 and err32 < 1e-5 and err3 < 5e-4
     bs = batch.stats()
     for f in ("psi_elements", "n_t_tasks", "n_cohorts", "n_lines", "n_terms", "flops_total", "flops_gemm",
               "bytes_compulsory", "t_elements", "s_elements", "l_panel_elements"):

@@ -88,7 +88,9 @@ def test_twou1_large_sectors(ctx2u1, oracle_lib):
 def test_fp32_hpsi(ctx2u1, ctxsu2, oracle_lib, sym):
     """mixed precision: FP32 operands and intermediates on the FP32 matrix cores, psi' in FP64;
     max-norm relative error against the FP64 oracle below 1e-5 (FP32 rounding of psi, L, R, T
-    and S with K-long FP32 sums)"""
+    and S with K-long FP32 sums); the split-BF16 products (MQ_PREC_BF16X3: each FP32 operand as
+    hi + lo BF16 parts, |x - hi - lo| <= 2^-16 |x|, lo lo left out, so about 3 * 2^-16 per
+    product) below 5e-4, over sector dims 1..90 (ragged K, every tile shape)"""
     from qcmaquis_amd.synthetic import TWOU1_SITE, fuse_index
     ctx = ctx2u1 if sym == abi.MQ_SYM_TWOU1 else ctxsu2
     print(f"fp32 hpsi sym={sym}", flush=True)
@@ -106,8 +108,9 @@ def test_fp32_hpsi(ctx2u1, ctxsu2, oracle_lib, sym):
         ref = oracle_lib.hpsi(ch.sym, ch.mpos[k], L_or[k], R_or[k + 1], lay, psi)
         got = plan.apply(psi, precision=abi.MQ_PREC_FP32)
         err = rel_err(got, ref)
-        print(f"fp32 hpsi site {k}: n={lay.n} err={err:.2e}", flush=True)
-        assert err < 1e-5
+        err3 = rel_err(plan.apply(psi, precision=abi.MQ_PREC_BF16X3), ref)
+        print(f"fp32 hpsi site {k}: n={lay.n} err={err:.2e} bf16x3 {err3:.2e}", flush=True)
+        assert err < 1e-5 and err3 < 5e-4
         # FP64 after FP32 on the same plan is unchanged
         assert rel_err(plan.apply(psi), ref) < TOL
         plan.close()
